@@ -1,0 +1,298 @@
+/*
+ * Test infrastructure (oracle): radix-2 NTTs and the Reed-Solomon row encoder.
+ * Restates crates/tessera-pcs/src/ntt.rs (twiddle tables, Cooley-Tukey / Gentleman-Sande
+ * kernels, depth-first to 2^12-entry leaves, natural-order transforms) and
+ * crates/tessera-pcs/src/encoding.rs (RowEncoder: CT with the inverse table, spread, masks,
+ * GS with the forward table). Rows are split over pthreads for the CPU baseline.
+ */
+#include <pthread.h>
+#include <stdlib.h>
+#include <string.h>
+#include "oracle.h"
+#include "internal.h"
+
+#define LEAF (1u << 12) /* ntt.rs:59 */
+
+static inline uint64_t fadd(uint64_t a, uint64_t b) { return orc_fp_add(a, b); }
+static inline uint64_t fsub(uint64_t a, uint64_t b) { return orc_fp_sub(a, b); }
+static inline uint64_t fmul(uint64_t a, uint64_t b) { return orc_fp_mul(a, b); }
+
+/* reverse_bits over `bits` bits (ntt.rs:79-84). */
+static uint64_t rev(uint64_t i, uint32_t bits) {
+    uint64_t r = 0;
+    for (uint32_t b = 0; b < bits; b++) r |= ((i >> b) & 1) << (bits - 1 - b);
+    return r;
+}
+
+/* bit_reversed_powers: out[t] = scale * base^{rev(t)}, t < 2^bits (ntt.rs:86-108). */
+static void bit_reversed_powers(uint64_t base, uint32_t bits, uint64_t scale, uint64_t *out) {
+    uint64_t squares[33];
+    uint64_t sq = base;
+    for (uint32_t i = 0; i < bits; i++) {
+        squares[i] = sq;
+        sq = fmul(sq, sq);
+    }
+    out[0] = scale;
+    uint64_t len = 1;
+    for (int e = (int)bits - 1; e >= 0; e--) {
+        for (uint64_t t = 0; t < len; t++) out[len + t] = fmul(out[t], squares[e]);
+        len *= 2;
+    }
+}
+
+/* twiddles (ntt.rs:110-125): Z[t] = w^{rev(t)}, w = w_n or w_n^{-1}, t < n/2. */
+void orc_twiddles(uint32_t log_n, int inverse, uint64_t *out) {
+    if (log_n == 0) return;
+    uint64_t root = orc_fp_root(log_n);
+    uint64_t base = inverse ? orc_fp_pow(root, (1ULL << log_n) - 1) : root;
+    bit_reversed_powers(base, log_n - 1, 1, out);
+}
+
+/* inverse_of_power_of_two (ntt.rs:127-131). */
+static uint64_t inv_pow2(uint32_t log_n) { return orc_fp_pow(ORC_P / 2 + 1, log_n); }
+
+/* The shift of a kernel: NULL for the subgroup, else s^{2^e} for e < 32 (ntt.rs:133-185). */
+static inline uint64_t twiddle(const uint64_t *shift_pows, uint64_t z, uint64_t half) {
+    if (!shift_pows) return z;
+    return fmul(z, shift_pows[__builtin_ctzll(half)]);
+}
+
+static void ct_bfly(uint64_t *lo, uint64_t *hi, uint64_t n, uint64_t c, int unit) {
+    for (uint64_t i = 0; i < n; i++) {
+        uint64_t t = unit ? hi[i] : fmul(hi[i], c);
+        hi[i] = fsub(lo[i], t);
+        lo[i] = fadd(lo[i], t);
+    }
+}
+static void gs_bfly(uint64_t *lo, uint64_t *hi, uint64_t n, uint64_t c, int unit) {
+    for (uint64_t i = 0; i < n; i++) {
+        uint64_t a = lo[i], b = hi[i];
+        lo[i] = fadd(a, b);
+        hi[i] = unit ? fsub(a, b) : fmul(fsub(a, b), c);
+    }
+}
+
+/* ct_leaf (ntt.rs:257-281). */
+static void ct_leaf(uint64_t *v, uint64_t len, const uint64_t *tw, const uint64_t *sp,
+                    uint64_t group) {
+    uint64_t half = len / 2, first = group, count = 1;
+    while (half > 0) {
+        for (uint64_t g = 0; g < count; g++) {
+            uint64_t *chunk = v + g * 2 * half;
+            int unit = (!sp && first + g == 0);
+            ct_bfly(chunk, chunk + half, half, twiddle(sp, tw[first + g], half), unit);
+        }
+        half /= 2;
+        first *= 2;
+        count *= 2;
+    }
+}
+
+/* ct_group (ntt.rs:232-255): depth-first until a group fits the 2^12-entry leaf. */
+static void ct_group(uint64_t *v, uint64_t len, const uint64_t *tw, const uint64_t *sp,
+                     uint64_t group) {
+    if (len <= LEAF) {
+        ct_leaf(v, len, tw, sp, group);
+        return;
+    }
+    uint64_t half = len / 2;
+    ct_bfly(v, v + half, half, twiddle(sp, tw[group], half), !sp && group == 0);
+    ct_group(v, half, tw, sp, 2 * group);
+    ct_group(v + half, half, tw, sp, 2 * group + 1);
+}
+
+/* gs_leaf (ntt.rs:318-340). */
+static void gs_leaf(uint64_t *v, uint64_t len, const uint64_t *tw, const uint64_t *sp,
+                    uint64_t group) {
+    uint64_t half = 1, count = len / 2, first = group * count;
+    while (count > 0) {
+        for (uint64_t g = 0; g < count; g++) {
+            uint64_t *chunk = v + g * 2 * half;
+            int unit = (!sp && first + g == 0);
+            gs_bfly(chunk, chunk + half, half, twiddle(sp, tw[first + g], half), unit);
+        }
+        half *= 2;
+        first /= 2;
+        count /= 2;
+    }
+}
+
+/* gs_group (ntt.rs:296-316). */
+static void gs_group(uint64_t *v, uint64_t len, const uint64_t *tw, const uint64_t *sp,
+                     uint64_t group) {
+    if (len <= LEAF) {
+        gs_leaf(v, len, tw, sp, group);
+        return;
+    }
+    uint64_t half = len / 2;
+    gs_group(v, half, tw, sp, 2 * group);
+    gs_group(v + half, half, tw, sp, 2 * group + 1);
+    gs_bfly(v, v + half, half, twiddle(sp, tw[group], half), !sp && group == 0);
+}
+
+/* bit_reverse_scaled (ntt.rs:343-393), written as a plain permutation. */
+static void bit_reverse_scaled(uint64_t *v, uint32_t bits, const uint64_t *scale) {
+    uint64_t n = 1ULL << bits;
+    for (uint64_t i = 0; i < n; i++) {
+        uint64_t j = rev(i, bits);
+        if (i < j) {
+            uint64_t x = v[i], y = v[j];
+            v[i] = scale ? fmul(y, *scale) : y;
+            v[j] = scale ? fmul(x, *scale) : x;
+        } else if (i == j && scale) {
+            v[i] = fmul(v[i], *scale);
+        }
+    }
+}
+
+static void shift_powers(uint64_t s, uint64_t out[32]) {
+    out[0] = s;
+    for (int e = 1; e < 32; e++) out[e] = fmul(out[e - 1], out[e - 1]);
+}
+
+/* NttPlan::ntt / intt / coset_ntt / coset_intt (ntt.rs:461-516). */
+void orc_ntt(uint64_t *v, uint32_t log_n) {
+    uint64_t *tw = malloc(sizeof(uint64_t) * ((1ULL << log_n) / 2 + 1));
+    orc_twiddles(log_n, 0, tw);
+    ct_group(v, 1ULL << log_n, tw, NULL, 0);
+    bit_reverse_scaled(v, log_n, NULL);
+    free(tw);
+}
+void orc_intt(uint64_t *v, uint32_t log_n) {
+    uint64_t *tw = malloc(sizeof(uint64_t) * ((1ULL << log_n) / 2 + 1));
+    uint64_t s = inv_pow2(log_n);
+    orc_twiddles(log_n, 1, tw);
+    bit_reverse_scaled(v, log_n, &s);
+    gs_group(v, 1ULL << log_n, tw, NULL, 0);
+    free(tw);
+}
+void orc_coset_ntt(uint64_t *v, uint32_t log_n, uint64_t shift) {
+    uint64_t *tw = malloc(sizeof(uint64_t) * ((1ULL << log_n) / 2 + 1));
+    uint64_t sp[32];
+    orc_twiddles(log_n, 0, tw);
+    shift_powers(shift, sp);
+    ct_group(v, 1ULL << log_n, tw, sp, 0);
+    bit_reverse_scaled(v, log_n, NULL);
+    free(tw);
+}
+void orc_coset_intt(uint64_t *v, uint32_t log_n, uint64_t shift) {
+    uint64_t *tw = malloc(sizeof(uint64_t) * ((1ULL << log_n) / 2 + 1));
+    uint64_t sp[32], s = inv_pow2(log_n);
+    orc_twiddles(log_n, 1, tw);
+    shift_powers(orc_fp_inv(shift), sp);
+    bit_reverse_scaled(v, log_n, &s);
+    gs_group(v, 1ULL << log_n, tw, sp, 0);
+    free(tw);
+}
+
+/* ---- RowEncoder (encoding.rs:126-340) ---- */
+typedef struct {
+    uint32_t log_k, log_r;
+    uint64_t P;
+    uint64_t *inverse; /* w_k^{-rev(t)}, t < k/2 */
+    uint64_t *forward; /* w_n^{rev(t)},  t < n/2 */
+    uint64_t *spread;  /* 7^{rev_k(t)}/k, t < k */
+    uint64_t shift_pow_k;
+    const uint64_t *rows, *masks;
+    uint64_t *codewords;
+} encoder_t;
+
+/* spread (encoding.rs:323-340): entry t times scale[t] to position s*t, zeros after it. */
+static void spread(uint64_t *cw, uint64_t n, const uint64_t *scale, uint64_t k) {
+    uint64_t stride = n / k;
+    for (uint64_t m = k; m >= 2; m /= 2) {
+        for (uint64_t t = m / 2; t < m; t++) {
+            uint64_t y = cw[t];
+            cw[stride * t] = fmul(y, scale[t]);
+            for (uint64_t z = 1; z < stride; z++) cw[stride * t + z] = 0;
+        }
+    }
+    cw[0] = fmul(cw[0], scale[0]);
+    for (uint64_t z = 1; z < stride; z++) cw[z] = 0;
+}
+
+/* encode (encoding.rs:292-320). */
+static void encode_one(const encoder_t *e, const uint64_t *row, const uint64_t *mask,
+                       uint64_t *cw) {
+    uint64_t k = 1ULL << e->log_k, n = k << e->log_r, stride = 1ULL << e->log_r;
+    memcpy(cw, row, k * 8);
+    ct_group(cw, k, e->inverse, NULL, 0);
+    spread(cw, n, e->spread, k);
+    uint64_t power = 1;
+    for (uint64_t i = 0; i < e->P; i++) {
+        uint64_t scaled = fmul(mask[i], power);
+        uint64_t low = stride * rev(i, e->log_k);
+        cw[low] = fsub(cw[low], scaled);
+        cw[low + stride / 2] = fmul(scaled, e->shift_pow_k);
+        power = fmul(power, 7);
+    }
+    gs_group(cw, n, e->forward, NULL, 0);
+}
+
+static void encode_range(void *arg, uint64_t lo, uint64_t hi) {
+    const encoder_t *e = arg;
+    uint64_t k = 1ULL << e->log_k, n = k << e->log_r;
+    for (uint64_t r = lo; r < hi; r++)
+        encode_one(e, e->rows + r * k, e->masks ? e->masks + r * e->P : NULL, e->codewords + r * n);
+}
+
+int orc_encode_rows(uint32_t log_k, uint32_t log_r, uint64_t P, const uint64_t *rows,
+                    const uint64_t *masks, uint64_t count, uint64_t *codewords, int threads) {
+    if (log_r == 0 || log_k + log_r > 32 || P > (1ULL << log_k)) return -1;
+    encoder_t e;
+    uint64_t k = 1ULL << log_k, n = k << log_r;
+    e.log_k = log_k;
+    e.log_r = log_r;
+    e.P = P;
+    e.inverse = malloc(8 * (k / 2 + 1));
+    e.forward = malloc(8 * (n / 2 + 1));
+    e.spread = malloc(8 * k);
+    orc_twiddles(log_k, 1, e.inverse);
+    orc_twiddles(log_k + log_r, 0, e.forward);
+    bit_reversed_powers(7, log_k, inv_pow2(log_k), e.spread);
+    e.shift_pow_k = orc_fp_pow(7, k);
+    e.rows = rows;
+    e.masks = masks;
+    e.codewords = codewords;
+    orc_parallel_for(count, threads, encode_range, &e);
+    free(e.inverse);
+    free(e.forward);
+    free(e.spread);
+    return 0;
+}
+
+/* ---- a small static-split thread pool ---- */
+typedef struct {
+    void (*fn)(void *, uint64_t, uint64_t);
+    void *arg;
+    uint64_t lo, hi;
+} job_t;
+static void *job_run(void *p) {
+    job_t *j = p;
+    j->fn(j->arg, j->lo, j->hi);
+    return NULL;
+}
+void orc_parallel_for(uint64_t count, int threads, void (*fn)(void *, uint64_t, uint64_t),
+                      void *arg) {
+    if (threads <= 1 || count <= 1) {
+        fn(arg, 0, count);
+        return;
+    }
+    if ((uint64_t)threads > count) threads = (int)count;
+    pthread_t tid[256];
+    job_t jobs[256];
+    if (threads > 256) threads = 256;
+    for (int t = 0; t < threads; t++) {
+        jobs[t].fn = fn;
+        jobs[t].arg = arg;
+        jobs[t].lo = count * t / threads;
+        jobs[t].hi = count * (t + 1) / threads;
+        if (t > 0 && pthread_create(&tid[t], NULL, job_run, &jobs[t]) != 0) {
+            job_run(&jobs[t]); /* spawn failure: run it on this thread */
+            tid[t] = 0;
+        }
+    }
+    job_run(&jobs[0]);
+    for (int t = 1; t < threads; t++)
+        if (tid[t]) pthread_join(tid[t], NULL);
+}

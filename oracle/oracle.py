@@ -1,0 +1,330 @@
+"""Test infrastructure: ctypes bindings of the CPU oracle (liboracle.so).
+
+The oracle is a plain-C restatement of the reference's CPU algorithms (see oracle.h). Only
+tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg import this module, as the
+checker or the timed CPU baseline; the product (tessera_amd/) never does.
+"""
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "liboracle.so")
+P = 0xFFFFFFFF00000001
+
+_lib = None
+
+
+def build():
+    subprocess.run(["make", "-s", "-C", HERE], check=True)
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            build()
+        _lib = ctypes.CDLL(LIB_PATH)
+        _declare(_lib)
+    return _lib
+
+
+u64 = ctypes.c_uint64
+u32 = ctypes.c_uint32
+vp = ctypes.c_void_p
+cp = ctypes.c_char_p
+i32 = ctypes.c_int
+i64 = ctypes.c_int64
+sz = ctypes.c_size_t
+
+
+class Transcript(ctypes.Structure):
+    _fields_ = [("state", ctypes.c_uint8 * 32), ("counter", u64)]
+
+    def state_hex(self):
+        return bytes(self.state).hex()
+
+
+class Fp2(ctypes.Structure):
+    _fields_ = [("c0", u64), ("c1", u64)]
+
+
+def _declare(L):
+    sig = {
+        "orc_fp_add": (u64, [u64, u64]),
+        "orc_fp_sub": (u64, [u64, u64]),
+        "orc_fp_mul": (u64, [u64, u64]),
+        "orc_fp_pow": (u64, [u64, u64]),
+        "orc_fp_inv": (u64, [u64]),
+        "orc_fp_root": (u64, [u32]),
+        "orc_fp2_mul_v": (None, [vp, vp, vp]),
+        "orc_fp2_inv_v": (None, [vp, vp]),
+        "orc_blake3": (None, [i32, vp, sz, vp, sz, vp, sz]),
+        "orc_domain_count": (i32, []),
+        "orc_domain_context": (cp, [i32]),
+        "orc_domain_key": (None, [i32, vp]),
+        "orc_hash": (None, [i32, vp, sz, vp]),
+        "orc_twiddles": (None, [u32, i32, vp]),
+        "orc_ntt": (None, [vp, u32]),
+        "orc_intt": (None, [vp, u32]),
+        "orc_coset_ntt": (None, [vp, u32, u64]),
+        "orc_coset_intt": (None, [vp, u32, u64]),
+        "orc_encode_rows": (i32, [u32, u32, u64, vp, vp, u64, vp, i32]),
+        "orc_column_init": (None, [vp, u64, vp]),
+        "orc_column_absorb": (None, [vp, u64, vp, u64, i32]),
+        "orc_merkle_node": (None, [vp, vp, vp]),
+        "orc_merkle_tree": (i32, [vp, u64, vp, i32]),
+        "orc_merkle_open": (i64, [vp, u64, vp, u64, vp, u64]),
+        "orc_merkle_verify": (i32, [vp, u64, vp, vp, u64, vp, u64]),
+        "orc_commit_unmasked": (i32, [u32, u32, u64, vp, vp, u64, vp, vp, i32]),
+        "orc_transcript_init": (None, [vp, vp, sz]),
+        "orc_transcript_absorb": (None, [vp, cp, vp, sz]),
+        "orc_transcript_squeeze": (None, [vp, cp, vp]),
+        "orc_transcript_challenge_fp": (u64, [vp, cp, vp]),
+        "orc_transcript_challenge_fp2": (Fp2, [vp, cp, vp]),
+        "orc_transcript_challenge_indices": (None, [vp, cp, u32, u32, vp]),
+        "orc_transcript_grind": (u64, [vp, cp, ctypes.c_uint8]),
+        "orc_transcript_check_grind": (i32, [vp, cp, ctypes.c_uint8, u64]),
+        "orc_transcript_absorb_fp2s": (None, [vp, cp, vp, sz]),
+    }
+    for name, (res, args) in sig.items():
+        fn = getattr(L, name, None)
+        if fn is None:
+            continue
+        fn.restype = res
+        fn.argtypes = args
+
+
+def _buf(a):
+    return a.ctypes.data_as(vp)
+
+
+def _bytes_ptr(b):
+    return ctypes.cast(ctypes.c_char_p(b), vp) if b else None
+
+
+# ---------------- field ----------------
+
+def fp_add(a, b):
+    return lib().orc_fp_add(a, b)
+
+
+def fp_sub(a, b):
+    return lib().orc_fp_sub(a, b)
+
+
+def fp_mul(a, b):
+    return lib().orc_fp_mul(a, b)
+
+
+def fp_inv(a):
+    return lib().orc_fp_inv(a)
+
+
+def fp_pow(a, e):
+    return lib().orc_fp_pow(a, e)
+
+
+def fp_root(log_n):
+    return lib().orc_fp_root(log_n)
+
+
+def fp2_mul(a, b):
+    x = np.array(a, dtype=np.uint64)
+    y = np.array(b, dtype=np.uint64)
+    o = np.zeros(2, dtype=np.uint64)
+    lib().orc_fp2_mul_v(_buf(x), _buf(y), _buf(o))
+    return (int(o[0]), int(o[1]))
+
+
+def fp2_inv(a):
+    x = np.array(a, dtype=np.uint64)
+    o = np.zeros(2, dtype=np.uint64)
+    lib().orc_fp2_inv_v(_buf(x), _buf(o))
+    return (int(o[0]), int(o[1]))
+
+
+# ---------------- hashing ----------------
+
+def blake3(data: bytes, mode=0, key: bytes = b"", out_len=32):
+    out = ctypes.create_string_buffer(out_len)
+    kb = ctypes.create_string_buffer(key, len(key)) if key else None
+    db = ctypes.create_string_buffer(data, len(data)) if data else None
+    lib().orc_blake3(mode, ctypes.cast(kb, vp) if kb else None, len(key),
+                     ctypes.cast(db, vp) if db else None, len(data), ctypes.cast(out, vp), out_len)
+    return out.raw
+
+
+def domain_contexts():
+    L = lib()
+    return [L.orc_domain_context(i).decode() for i in range(L.orc_domain_count())]
+
+
+def domain_key(d):
+    out = ctypes.create_string_buffer(32)
+    lib().orc_domain_key(d, out)
+    return out.raw
+
+
+def hash_domain(d, data: bytes):
+    out = ctypes.create_string_buffer(32)
+    db = ctypes.create_string_buffer(data, len(data)) if data else None
+    lib().orc_hash(d, ctypes.cast(db, vp) if db else None, len(data), out)
+    return out.raw
+
+
+DOMAIN_INDEX = {name: i for i, name in enumerate([
+    "envelope", "commitment-id", "column-init", "column-step", "merkle-node", "column-merge",
+    "weights-root", "transcript-init", "transcript-absorb", "transcript-squeeze",
+    "transcript-ratchet", "pow", "coin-commit", "coin-toss", "table", "audit-sample",
+    "audit-select", "claim-list", "statement"])}
+
+
+# ---------------- NTT / encoding ----------------
+
+def ntt(values, log_n, inverse=False, shift=None):
+    v = np.ascontiguousarray(np.array(values, dtype=np.uint64))
+    L = lib()
+    if shift is None:
+        (L.orc_intt if inverse else L.orc_ntt)(_buf(v), log_n)
+    else:
+        (L.orc_coset_intt if inverse else L.orc_coset_ntt)(_buf(v), log_n, shift)
+    return v
+
+
+def twiddles(log_n, inverse=False):
+    out = np.zeros(max(1, (1 << log_n) // 2), dtype=np.uint64)
+    lib().orc_twiddles(log_n, int(inverse), _buf(out))
+    return out[: (1 << log_n) // 2]
+
+
+def encode_rows(rows, log_k, log_r, masks=None, mask_coeffs=0, threads=1):
+    rows = np.ascontiguousarray(rows, dtype=np.uint64).reshape(-1)
+    k = 1 << log_k
+    count = rows.size // k
+    out = np.zeros(count * (k << log_r), dtype=np.uint64)
+    mb = None
+    if mask_coeffs:
+        masks = np.ascontiguousarray(masks, dtype=np.uint64).reshape(-1)
+        mb = _buf(masks)
+    rc = lib().orc_encode_rows(log_k, log_r, mask_coeffs, _buf(rows), mb, count, _buf(out), threads)
+    if rc:
+        raise ValueError("bad encoder shape")
+    return out.reshape(count, k << log_r)
+
+
+# ---------------- chains / Merkle / commitment ----------------
+
+def column_chains(commitment_id: bytes, rows, columns, block, threads=1):
+    rows = np.ascontiguousarray(rows, dtype=np.uint64).reshape(-1)
+    m = rows.size // columns if columns else 0
+    chains = ctypes.create_string_buffer(32 * columns)
+    L = lib()
+    L.orc_column_init(commitment_id, columns, chains)
+    for r in range(0, m, block):
+        c = min(block, m - r)
+        blk = np.ascontiguousarray(rows[r * columns:(r + c) * columns])
+        L.orc_column_absorb(chains, columns, _buf(blk), c, threads)
+    raw = chains.raw
+    return [raw[32 * j:32 * j + 32] for j in range(columns)]
+
+
+def merkle_tree(leaves, threads=1):
+    n = len(leaves)
+    nodes = ctypes.create_string_buffer(32 * (2 * n - 1))
+    if lib().orc_merkle_tree(b"".join(leaves), n, nodes, threads):
+        raise ValueError("leaf count must be a power of two")
+    return nodes.raw
+
+
+def merkle_open(nodes: bytes, n, indices):
+    idx = np.array(indices, dtype=np.uint64)
+    cap = 64 * max(1, len(indices)) + 64
+    sib = ctypes.create_string_buffer(32 * cap)
+    got = lib().orc_merkle_open(nodes, n, _buf(idx), len(indices), sib, cap)
+    if got < 0:
+        raise ValueError("bad query")
+    raw = sib.raw
+    return [raw[32 * i:32 * i + 32] for i in range(got)]
+
+
+def merkle_verify(root: bytes, n, pairs, siblings):
+    idx = np.array([i for i, _ in pairs], dtype=np.uint64)
+    digs = b"".join(d for _, d in pairs)
+    return bool(lib().orc_merkle_verify(root, n, _buf(idx), digs, len(pairs),
+                                        b"".join(siblings) if siblings else None, len(siblings)))
+
+
+def commit_unmasked(rows, log_k, log_r, block, commitment_id: bytes, threads=1):
+    """(codewords, nodes) of ReferenceCommitment::commit_unmasked."""
+    rows = np.ascontiguousarray(rows, dtype=np.uint64).reshape(-1)
+    k = 1 << log_k
+    n = k << log_r
+    count = rows.size // k
+    cw = np.zeros(count * n, dtype=np.uint64)
+    nodes = ctypes.create_string_buffer(32 * (2 * n - 1))
+    rc = lib().orc_commit_unmasked(log_k, log_r, block, commitment_id, _buf(rows), count,
+                                   _buf(cw), nodes, threads)
+    if rc:
+        raise ValueError("bad commitment shape")
+    return cw.reshape(count, n), nodes.raw
+
+
+def answer(codewords, nodes, n, queries):
+    """ReferenceCommitment::answer: columns of sorted distinct queries + multiproof."""
+    distinct = sorted(set(int(q) for q in queries))
+    cols = [codewords[:, j].copy() for j in distinct]
+    return distinct, cols, merkle_open(nodes, n, distinct)
+
+
+# ---------------- transcript ----------------
+
+class OracleTranscript:
+    def __init__(self, context_tce: bytes):
+        self.t = Transcript()
+        lib().orc_transcript_init(ctypes.byref(self.t), context_tce, len(context_tce))
+
+    @property
+    def state(self):
+        return bytes(self.t.state)
+
+    @property
+    def counter(self):
+        return int(self.t.counter)
+
+    def absorb(self, label: str, tce: bytes):
+        lib().orc_transcript_absorb(ctypes.byref(self.t), label.encode(), tce or None, len(tce))
+
+    def absorb_fp2s(self, label: str, values):
+        v = np.ascontiguousarray(np.array(values, dtype=np.uint64).reshape(-1))
+        lib().orc_transcript_absorb_fp2s(ctypes.byref(self.t), label.encode(),
+                                         _buf(v) if v.size else None, v.size // 2)
+
+    def squeeze(self, label: str) -> bytes:
+        out = ctypes.create_string_buffer(32)
+        lib().orc_transcript_squeeze(ctypes.byref(self.t), label.encode(), out)
+        return out.raw
+
+    def challenge_fp(self, label: str):
+        rej = ctypes.c_int(0)
+        v = lib().orc_transcript_challenge_fp(ctypes.byref(self.t), label.encode(), ctypes.byref(rej))
+        return int(v), rej.value
+
+    def challenge_fp2(self, label: str):
+        rej = ctypes.c_int(0)
+        v = lib().orc_transcript_challenge_fp2(ctypes.byref(self.t), label.encode(), ctypes.byref(rej))
+        return (int(v.c0), int(v.c1)), rej.value
+
+    def challenge_indices(self, label: str, log_n: int, count: int):
+        out = np.zeros(max(count, 1), dtype=np.uint64)
+        lib().orc_transcript_challenge_indices(ctypes.byref(self.t), label.encode(), log_n, count,
+                                               _buf(out))
+        return [int(x) for x in out[:count]]
+
+    def grind(self, label: str, bits: int):
+        return int(lib().orc_transcript_grind(ctypes.byref(self.t), label.encode(), bits))
+
+    def check_grind(self, label: str, bits: int, nonce: int):
+        return bool(lib().orc_transcript_check_grind(ctypes.byref(self.t), label.encode(), bits, nonce))

@@ -1,0 +1,160 @@
+/*
+ * tessera_mi355x — the C ABI of the MI355X (gfx950) prover back end for Tessera's
+ * data-parallel core: Goldilocks Reed-Solomon row encoding (NTT/LDE), BLAKE3 column chains and
+ * Merkle commitment, and the sumcheck / LogUp evaluate-and-fold rounds.
+ *
+ * Plain C: pointers, explicit sizes and status codes; no HIP or torch types. Field elements are
+ * canonical Goldilocks u64 (little-endian, in [0, p), p = 2^64 - 2^32 + 1); an Fp2 element is
+ * two u64 (c0, c1) with X^2 = 7; a digest is 32 bytes. Unless a function says otherwise, element
+ * pointers may be host memory or device memory of the context's GPU; the library detects which.
+ *
+ * Every entry point checks its shapes before it launches anything or moves a transcript, and
+ * reports failure through a status code, with the reason in tsr_last_error() (thread-local).
+ * TSR_DECLINED means the caller should compute the result on the CPU instead; nothing was
+ * changed. A context is used by one thread at a time; separate contexts may run concurrently.
+ *
+ * Each declaration cites the reference interface it replaces (AbdelStark/tessera, file:line).
+ */
+#ifndef TESSERA_MI355X_H
+#define TESSERA_MI355X_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum {
+    TSR_OK = 0,
+    TSR_DECLINED = 1,  /* caller falls back to the CPU path; no side effects */
+    TSR_BAD_SHAPE = 2, /* a length or parameter check failed; no side effects */
+    TSR_DEVICE = 3,    /* a HIP runtime error */
+    TSR_OOM = 4,       /* device memory exhausted or above the context's budget */
+    TSR_BAD_ARG = 5    /* null pointer or unusable argument */
+} tsr_status;
+
+typedef struct tsr_ctx tsr_ctx;
+typedef struct tsr_commitment tsr_commitment;
+typedef struct tsr_chains tsr_chains;
+typedef struct tsr_transcript tsr_transcript;
+
+/* The reason for the last non-OK status on this thread (empty when none). */
+const char *tsr_last_error(void);
+/* The library's version string. */
+const char *tsr_version(void);
+
+/* ---- context: one GPU, one stream (replaces Device::new, crates/tessera-gpu/src/device.rs:125-146) */
+tsr_status tsr_ctx_create(int device, tsr_ctx **out);
+void tsr_ctx_destroy(tsr_ctx *ctx);
+/* Wait for all work of this context (Device::synchronize, device.rs:220). */
+tsr_status tsr_ctx_synchronize(tsr_ctx *ctx);
+/* Cap the context's device memory (bytes, 0 = no cap); larger requests return TSR_OOM. */
+tsr_status tsr_ctx_set_budget(tsr_ctx *ctx, uint64_t bytes);
+/* Device memory currently held by the context (bytes). */
+uint64_t tsr_ctx_memory(const tsr_ctx *ctx);
+/* Device buffers owned by the context, for callers that keep inputs resident in HBM
+ * (Device::upload / zeroed / download, device.rs:149-247). */
+tsr_status tsr_alloc(tsr_ctx *ctx, uint64_t bytes, void **out);
+tsr_status tsr_free(tsr_ctx *ctx, void *ptr);
+tsr_status tsr_copy(tsr_ctx *ctx, void *dst, const void *src, uint64_t bytes);
+/* Fill dst[i] = SplitMix64 stream value i, words >= p rejected, exactly uniform (the generator of
+ * crates/tessera-gpu/tests/commit.rs:19-30 with rejection). Device-side bench input generator. */
+tsr_status tsr_fill_uniform(tsr_ctx *ctx, uint64_t *dst, uint64_t count, uint64_t seed);
+
+/* ---- row encoding (RowEncoder, crates/tessera-pcs/src/encoding.rs:126-290;
+ *      Device::encoder / encode_rows, crates/tessera-gpu/src/commit.rs:178,277) ---- */
+/* Encode `count` rows of k = 2^log_k values (row-major) into codewords of n = 2^(log_k+r)
+ * symbols: symbol j = g(7 * w_n^j), g = f + (X^k - 1) m, with `mask_coeffs` = P mask
+ * coefficients per row in `masks` (count x P, m_0 first; NULL when P = 0).
+ * Errors (in this order): r = 0 or log_k + r > 32 (BAD_SHAPE), P > k (BAD_SHAPE). */
+tsr_status tsr_encode_rows(tsr_ctx *ctx, uint32_t log_k, uint32_t log2_inv_rate,
+                           uint64_t mask_coeffs, const uint64_t *rows, const uint64_t *masks,
+                           uint64_t count, uint64_t *codewords);
+
+/* ---- column chains (ColumnChains, crates/tessera-hash/src/columns.rs:72-148;
+ *      Device::column_chains / absorb_block / download_chains, commit.rs:418,460,598) ---- */
+/* d_j[0] = H(column-init, commitment_id || u64LE(j)) for `columns` columns, blocks of `block` rows. */
+tsr_status tsr_chains_create(tsr_ctx *ctx, const uint8_t commitment_id[32], uint64_t columns,
+                             uint64_t block, tsr_chains **out);
+/* Absorb `count` rows (row-major, `columns` symbols each): 1 <= count <= block; a block shorter
+ * than `block` must be the last (ColumnError::{BlockSize, AfterFinalBlock}). */
+tsr_status tsr_chains_absorb(tsr_chains *chains, const uint64_t *rows, uint64_t count);
+/* The chains' current digests, 32 bytes per column (ColumnChains::finish). */
+tsr_status tsr_chains_read(tsr_chains *chains, uint8_t *out);
+void tsr_chains_free(tsr_chains *chains);
+
+/* ---- Merkle (MerkleTree::new / root, crates/tessera-hash/src/merkle.rs:93-119) ---- */
+/* root of the binary BLAKE3 tree over n = 2^m leaves (32 bytes each). */
+tsr_status tsr_merkle_root(tsr_ctx *ctx, const uint8_t *leaves, uint64_t n, uint8_t root[32]);
+
+/* ---- the unmasked reference commitment (ReferenceCommitment::commit_unmasked / root /
+ *      leaves / answer, crates/tessera-pcs/src/commitment.rs:343-598) ---- */
+typedef struct {
+    uint32_t log2_k;         /* k = 2^log2_k values per row */
+    uint32_t log2_inv_rate;  /* r >= 1: n = 2^r k symbols per codeword */
+    uint64_t mask_coeffs;    /* P: must be 0 for commit_unmasked */
+    uint64_t row_block;      /* B >= 1 rows per chain step */
+    uint8_t commitment_id[32];
+} tsr_commitment_params;
+
+/* Encode `count` rows, chain each column over them in blocks of B (leaf = last digest; the
+ * leaves of a commitment of no row are d_j[0]), build the Merkle tree. Codewords, leaves and the
+ * tree stay resident on the device for answer(). Errors: r = 0 or log_k + r > 32, P != 0,
+ * B = 0 (BAD_SHAPE), before any work. */
+tsr_status tsr_commit_unmasked(tsr_ctx *ctx, const tsr_commitment_params *params,
+                               const uint64_t *rows, uint64_t count, tsr_commitment **out);
+tsr_status tsr_commitment_root(const tsr_commitment *c, uint8_t root[32]);
+/* n leaves, 32 bytes each, column order. */
+tsr_status tsr_commitment_leaves(const tsr_commitment *c, uint8_t *out);
+/* The device address of the codewords (count x n u64, row-major) and their shape. */
+tsr_status tsr_commitment_codewords(const tsr_commitment *c, const uint64_t **dev_codewords,
+                                    uint64_t *count, uint64_t *n);
+/* answer(queries): the sorted distinct indices, the column of each (one symbol per row) and the
+ * canonical multiproof. Output capacities are checked first: `distinct_cap` indices (and
+ * distinct_cap x count symbols), `siblings_cap` digests; the sizes needed are always written.
+ * Errors: q = 0 (NoQueries, BAD_SHAPE), a query >= n (QueryRange, BAD_SHAPE), capacity short
+ * (BAD_SHAPE). */
+tsr_status tsr_commitment_answer(const tsr_commitment *c, const uint64_t *queries, uint64_t q,
+                                 uint64_t *distinct_out, uint64_t distinct_cap,
+                                 uint64_t *columns_out, uint8_t *siblings_out,
+                                 uint64_t siblings_cap, uint64_t *n_distinct,
+                                 uint64_t *n_siblings);
+void tsr_commitment_free(tsr_commitment *c);
+
+/* ---- hashing and the Fiat-Shamir transcript (crates/tessera-hash/src/domain.rs:177-181,
+ *      crates/tessera-hash/src/transcript.rs:40-290) ---- */
+/* H(domain, bytes) = BLAKE3.derive_key("tessera/v1/<name>", bytes); `domain` indexes the
+ * reference's 19 domains in declaration order (domain.rs:41-81), e.g. 2 = column-init.
+ * The hash and the transcript run on the host; `ctx` may be NULL for them. */
+tsr_status tsr_hash(tsr_ctx *ctx, int domain, const uint8_t *data, uint64_t len, uint8_t out[32]);
+/* Transcript::new(context): state = H(transcript-init, TCE(SessionContext)). */
+tsr_status tsr_transcript_create(tsr_ctx *ctx, const uint8_t *context_tce, uint64_t len,
+                                 tsr_transcript **out);
+void tsr_transcript_free(tsr_transcript *t);
+tsr_status tsr_transcript_state(const tsr_transcript *t, uint8_t state[32], uint64_t *counter);
+/* absorb(label, value): `tce` is the value's TCE encoding. Labels are printable ASCII. */
+tsr_status tsr_transcript_absorb(tsr_transcript *t, const char *label, const uint8_t *tce,
+                                 uint64_t len);
+tsr_status tsr_transcript_squeeze(tsr_transcript *t, const char *label, uint8_t out[32]);
+tsr_status tsr_transcript_challenge_fp(tsr_transcript *t, const char *label, uint64_t *out);
+tsr_status tsr_transcript_challenge_fp2(tsr_transcript *t, const char *label, uint64_t out[2]);
+tsr_status tsr_transcript_challenge_indices(tsr_transcript *t, const char *label,
+                                            uint32_t log_n, uint32_t count, uint64_t *out);
+tsr_status tsr_transcript_grind(tsr_transcript *t, const char *label, uint8_t bits,
+                                uint64_t *nonce);
+tsr_status tsr_transcript_check_grind(tsr_transcript *t, const char *label, uint8_t bits,
+                                      uint64_t nonce, int *valid);
+
+/* ---- per-kernel timing with HIP events on the context's stream (the reference's CUDA-event
+ *      harness, docs/perf/v0.1/cc-h100.md:30-40) ---- */
+/* enable = 1 starts recording an event pair around every launch; 0 stops; 2 clears. */
+tsr_status tsr_ctx_profile(tsr_ctx *ctx, int enable);
+/* Total milliseconds and launch count of the named kernel since the last clear. */
+tsr_status tsr_ctx_kernel_time(tsr_ctx *ctx, const char *kernel, double *total_ms,
+                               uint64_t *launches);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -328,3 +328,13 @@ class OracleTranscript:
 
     def check_grind(self, label: str, bits: int, nonce: int):
         return bool(lib().orc_transcript_check_grind(ctypes.byref(self.t), label.encode(), bits, nonce))
+
+
+def column_chain_at(commitment_id: bytes, j: int, symbols, block):
+    """The chain of column j alone (columns.rs:72-104): init with u64LE(j), then one step per block."""
+    d = hash_domain(DOMAIN_INDEX["column-init"], bytes(commitment_id) + int(j).to_bytes(8, "little"))
+    syms = [int(x) for x in symbols]
+    for r in range(0, len(syms), block):
+        part = b"".join(s.to_bytes(8, "little") for s in syms[r:r + block])
+        d = hash_domain(DOMAIN_INDEX["column-step"], d + part)
+    return d

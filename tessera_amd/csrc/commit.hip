@@ -1,0 +1,233 @@
+// Column chains, Merkle trees and the unmasked commitment on gfx950.
+//
+// Semantics (bit for bit): crates/tessera-hash/src/columns.rs:72-148 (d_j[0] = H(column-init,
+// id || u64LE(j)); d_j <- H(column-step, d_j || u64LE(symbols of the block's rows))),
+// crates/tessera-hash/src/merkle.rs:10-225 (H(merkle-node, l || r), levels leaves-first,
+// canonical multiproofs) and crates/tessera-pcs/src/commitment.rs:343-598 (commit_unmasked,
+// answer). One lane per column: a wave reads 64 consecutive symbols of a row (512 B, coalesced)
+// per load, and every chain step is a single-chunk BLAKE3 kept in registers.
+#include <hip/hip_runtime.h>
+#include <algorithm>
+#include <stdint.h>
+#include <string.h>
+#include <vector>
+#include "blake3.h"
+#include "commit.h"
+#include "field.h"
+
+namespace tsr {
+
+struct Key8 {
+    uint32_t w[8];
+};
+
+static Key8 key_of(const Ctx *ctx, int d) {
+    Key8 k;
+    memcpy(k.w, ctx->keys[d], 32);
+    return k;
+}
+
+// d <- H(column-step, d || u64LE(col[0]) || u64LE(col[stride]) || ...), count <= 124 symbols.
+__device__ __forceinline__ void chain_step(uint32_t d[8], const uint64_t *__restrict__ col,
+                                           uint64_t stride, uint32_t count, const Key8 &key) {
+    const uint32_t total = 32 + 8 * count;  // bytes
+    const uint32_t nblocks = (total + 63) / 64;
+    uint32_t cv[8];
+#pragma unroll
+    for (int i = 0; i < 8; i++) cv[i] = key.w[i];
+    for (uint32_t blk = 0; blk < nblocks; blk++) {
+        uint32_t m[16];
+        if (blk == 0) {
+#pragma unroll
+            for (int i = 0; i < 8; i++) m[i] = d[i];
+#pragma unroll
+            for (int i = 0; i < 4; i++) {
+                uint64_t v = (uint32_t)i < count ? col[(uint64_t)i * stride] : 0;
+                m[8 + 2 * i] = (uint32_t)v;
+                m[9 + 2 * i] = (uint32_t)(v >> 32);
+            }
+        } else {
+            const uint32_t s0 = 8 * blk - 4;
+#pragma unroll
+            for (int i = 0; i < 8; i++) {
+                uint32_t s = s0 + i;
+                uint64_t v = s < count ? col[(uint64_t)s * stride] : 0;
+                m[2 * i] = (uint32_t)v;
+                m[2 * i + 1] = (uint32_t)(v >> 32);
+            }
+        }
+        uint32_t flags = b3::DERIVE_KEY_MATERIAL;
+        if (blk == 0) flags |= b3::CHUNK_START;
+        uint32_t len = 64;
+        if (blk == nblocks - 1) {
+            flags |= b3::CHUNK_END | b3::ROOT;
+            len = total - 64 * blk;
+        }
+        b3::compress(cv, m, 0, len, flags, cv);
+    }
+#pragma unroll
+    for (int i = 0; i < 8; i++) d[i] = cv[i];
+}
+
+// Every column's whole chain over `rows` rows (blocks of `block`, the last may be short), the
+// last digest written as the column's leaf.
+__global__ void __launch_bounds__(256)
+k_chain_leaves(const uint64_t *__restrict__ cw, uint64_t n, uint64_t rows, uint32_t block,
+               Key8 init_key, Key8 step_key, Key8 id, uint32_t *__restrict__ leaves) {
+    uint64_t j = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= n) return;
+    uint32_t d[8];
+    b3::hash_id_index(init_key.w, id.w, j, d);
+    for (uint64_t r0 = 0; r0 < rows; r0 += block) {
+        uint32_t count = (uint32_t)(rows - r0 < block ? rows - r0 : block);
+        chain_step(d, cw + r0 * n + j, n, count, step_key);
+    }
+    uint4 *o = (uint4 *)(leaves + 8 * j);
+    o[0] = make_uint4(d[0], d[1], d[2], d[3]);
+    o[1] = make_uint4(d[4], d[5], d[6], d[7]);
+}
+
+__global__ void __launch_bounds__(256)
+k_chain_init(uint64_t n, Key8 init_key, Key8 id, uint32_t *__restrict__ chains) {
+    uint64_t j = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= n) return;
+    uint32_t d[8];
+    b3::hash_id_index(init_key.w, id.w, j, d);
+#pragma unroll
+    for (int i = 0; i < 8; i++) chains[8 * j + i] = d[i];
+}
+
+__global__ void __launch_bounds__(256)
+k_chain_absorb(const uint64_t *__restrict__ rows, uint64_t n, uint32_t count, Key8 step_key,
+               uint32_t *__restrict__ chains) {
+    uint64_t j = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= n) return;
+    uint32_t d[8];
+#pragma unroll
+    for (int i = 0; i < 8; i++) d[i] = chains[8 * j + i];
+    chain_step(d, rows + j, n, count, step_key);
+#pragma unroll
+    for (int i = 0; i < 8; i++) chains[8 * j + i] = d[i];
+}
+
+// One Merkle level: out[i] = H(merkle-node, in[2i] || in[2i+1]).
+__global__ void __launch_bounds__(256)
+k_merkle_level(const uint32_t *__restrict__ in, uint32_t *__restrict__ out, uint64_t count,
+               Key8 key) {
+    uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= count) return;
+    const uint4 *l = (const uint4 *)(in + 16 * i);
+    uint4 a = l[0], b = l[1], c = l[2], e = l[3];
+    uint32_t L[8] = {a.x, a.y, a.z, a.w, b.x, b.y, b.z, b.w};
+    uint32_t R[8] = {c.x, c.y, c.z, c.w, e.x, e.y, e.z, e.w};
+    uint32_t d[8];
+    b3::hash64(key.w, L, R, d);
+    uint4 *o = (uint4 *)(out + 8 * i);
+    o[0] = make_uint4(d[0], d[1], d[2], d[3]);
+    o[1] = make_uint4(d[4], d[5], d[6], d[7]);
+}
+
+__global__ void k_gather_columns(const uint64_t *__restrict__ cw, uint64_t n, uint64_t rows,
+                                 const uint64_t *__restrict__ idx, uint64_t q,
+                                 uint64_t *__restrict__ out) {
+    uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= q * rows) return;
+    uint64_t c = t / rows, r = t % rows;
+    out[t] = cw[r * n + idx[c]];
+}
+
+__global__ void k_gather_digests(const uint32_t *__restrict__ nodes,
+                                 const uint64_t *__restrict__ idx, uint64_t q,
+                                 uint32_t *__restrict__ out) {
+    uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= q * 8) return;
+    out[t] = nodes[8 * idx[t / 8] + t % 8];
+}
+
+static unsigned blocks_for(uint64_t n, unsigned per) { return (unsigned)((n + per - 1) / per); }
+
+Status merkle_build(Ctx *ctx, uint32_t *nodes, uint64_t n, hipStream_t stream) {
+    Key8 key = key_of(ctx, DOM_MERKLE_NODE);
+    uint64_t start = 0, width = n;
+    KernelTimer timer(ctx, "k_merkle_level", stream);
+    while (width > 1) {
+        uint64_t count = width / 2;
+        hipLaunchKernelGGL(k_merkle_level, dim3(blocks_for(count, 256)), dim3(256), 0, stream,
+                           nodes + 8 * start, nodes + 8 * (start + width), count, key);
+        start += width;
+        width /= 2;
+    }
+    return check_launch("merkle levels");
+}
+
+Status chain_leaves(Ctx *ctx, const uint64_t *cw, uint64_t n, uint64_t rows, uint32_t block,
+                    const uint8_t id[32], uint32_t *leaves, hipStream_t stream) {
+    Key8 idk;
+    memcpy(idk.w, id, 32);
+    KernelTimer timer(ctx, "k_chain_leaves", stream);
+    hipLaunchKernelGGL(k_chain_leaves, dim3(blocks_for(n, 256)), dim3(256), 0, stream, cw, n,
+                       rows, block, key_of(ctx, DOM_COLUMN_INIT), key_of(ctx, DOM_COLUMN_STEP),
+                       idk, leaves);
+    return check_launch("column chains");
+}
+
+Status chains_init(Ctx *ctx, const uint8_t id[32], uint64_t n, uint32_t *chains,
+                   hipStream_t stream) {
+    Key8 idk;
+    memcpy(idk.w, id, 32);
+    hipLaunchKernelGGL(k_chain_init, dim3(blocks_for(n, 256)), dim3(256), 0, stream, n,
+                       key_of(ctx, DOM_COLUMN_INIT), idk, chains);
+    return check_launch("column init");
+}
+
+Status chains_absorb(Ctx *ctx, const uint64_t *rows, uint64_t n, uint32_t count,
+                     uint32_t *chains, hipStream_t stream) {
+    hipLaunchKernelGGL(k_chain_absorb, dim3(blocks_for(n, 256)), dim3(256), 0, stream, rows, n,
+                       count, key_of(ctx, DOM_COLUMN_STEP), chains);
+    return check_launch("column absorb");
+}
+
+Status gather_columns(const uint64_t *cw, uint64_t n, uint64_t rows, const uint64_t *idx,
+                      uint64_t q, uint64_t *out, hipStream_t stream) {
+    if (q * rows == 0) return Status::ok();
+    hipLaunchKernelGGL(k_gather_columns, dim3(blocks_for(q * rows, 256)), dim3(256), 0, stream,
+                       cw, n, rows, idx, q, out);
+    return check_launch("gather columns");
+}
+
+Status gather_digests(const uint32_t *nodes, const uint64_t *idx, uint64_t q, uint32_t *out,
+                      hipStream_t stream) {
+    if (q == 0) return Status::ok();
+    hipLaunchKernelGGL(k_gather_digests, dim3(blocks_for(q * 8, 256)), dim3(256), 0, stream,
+                       nodes, idx, q, out);
+    return check_launch("gather digests");
+}
+
+// Canonical multiproof sibling positions (merkle.rs:126-169), as indices into the node array.
+std::vector<uint64_t> multiproof_positions(uint64_t n, std::vector<uint64_t> known) {
+    std::vector<uint64_t> out;
+    std::sort(known.begin(), known.end());
+    known.erase(std::unique(known.begin(), known.end()), known.end());
+    if (known.size() == n) return out;
+    uint64_t start = 0, width = n;
+    while (width > 1) {
+        std::vector<uint64_t> parents;
+        size_t i = 0;
+        while (i < known.size()) {
+            uint64_t node = known[i];
+            if ((node & 1) == 0 && i + 1 < known.size() && known[i + 1] == node + 1) {
+                i += 2;
+            } else {
+                out.push_back(start + (node ^ 1));
+                i += 1;
+            }
+            parents.push_back(node / 2);
+        }
+        known.swap(parents);
+        start += width;
+        width /= 2;
+    }
+    return out;
+}
+
+}  // namespace tsr

@@ -1,0 +1,105 @@
+// Goldilocks field arithmetic for gfx950 kernels and the host side of the library.
+//
+// p = 2^64 - 2^32 + 1, so 2^64 = 2^32 - 1 (mod p) and 2^96 = -1 (mod p). Elements are u64;
+// every function here returns the canonical representative in [0, p). Corrections are
+// branch-free selects (v_cndmask on the device), following the reference's branch-free field
+// (crates/tessera-field/src/goldilocks.rs:34-82). Fp2 = Fp[X]/(X^2 - 7) (ext2.rs:104-247).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#define TSR_HD __host__ __device__ __forceinline__
+
+namespace gl {
+
+constexpr uint64_t P = 0xFFFFFFFF00000001ull;
+constexpr uint64_t EPS = 0xFFFFFFFFull;  // 2^64 mod p
+
+TSR_HD uint64_t canon(uint64_t x) { return x >= P ? x - P : x; }
+
+TSR_HD uint64_t add(uint64_t a, uint64_t b) {
+    uint64_t s = a + b;
+    s += (s < a) ? EPS : 0;  // a wrap adds 2^64 = EPS
+    return canon(s);
+}
+
+TSR_HD uint64_t sub(uint64_t a, uint64_t b) {
+    uint64_t d = a - b;
+    d -= (a < b) ? EPS : 0;  // a borrow subtracts 2^64 = EPS
+    return d;
+}
+
+TSR_HD uint64_t neg(uint64_t a) { return sub(0, a); }
+
+TSR_HD uint64_t mulhi(uint64_t a, uint64_t b) {
+#ifdef __HIP_DEVICE_COMPILE__
+    return __umul64hi(a, b);
+#else
+    return (uint64_t)(((unsigned __int128)a * b) >> 64);
+#endif
+}
+
+// x = lo + 2^64 hi_lo + 2^96 hi_hi = lo - hi_hi + hi_lo (2^32 - 1) (mod p).
+TSR_HD uint64_t reduce128(uint64_t lo, uint64_t hi) {
+    uint32_t hi_hi = (uint32_t)(hi >> 32);
+    uint32_t hi_lo = (uint32_t)hi;
+    uint64_t t0 = lo - hi_hi;
+    t0 -= (lo < hi_hi) ? EPS : 0;
+    uint64_t t1 = ((uint64_t)hi_lo << 32) - hi_lo;
+    uint64_t t2 = t0 + t1;
+    t2 += (t2 < t1) ? EPS : 0;
+    return canon(t2);
+}
+
+TSR_HD uint64_t mul(uint64_t a, uint64_t b) { return reduce128(a * b, mulhi(a, b)); }
+
+TSR_HD uint64_t sqr(uint64_t a) { return mul(a, a); }
+
+inline __host__ __device__ uint64_t pow(uint64_t a, uint64_t e) {
+    uint64_t acc = 1;
+    while (e) {
+        if (e & 1) acc = mul(acc, a);
+        a = mul(a, a);
+        e >>= 1;
+    }
+    return acc;
+}
+
+inline __host__ __device__ uint64_t inv(uint64_t a) { return pow(a, P - 2); }
+
+// A primitive 2^log_n-th root of unity: 7^((p-1)/2^32) squared 32 - log_n times.
+inline __host__ __device__ uint64_t root(uint32_t log_n) {
+    uint64_t r = pow(7, (P - 1) >> 32);
+    for (uint32_t i = log_n; i < 32; i++) r = mul(r, r);
+    return r;
+}
+
+struct fp2 {
+    uint64_t c0, c1;
+};
+
+TSR_HD fp2 f2(uint64_t c0, uint64_t c1) { return fp2{c0, c1}; }
+TSR_HD fp2 add(fp2 a, fp2 b) { return fp2{add(a.c0, b.c0), add(a.c1, b.c1)}; }
+TSR_HD fp2 sub(fp2 a, fp2 b) { return fp2{sub(a.c0, b.c0), sub(a.c1, b.c1)}; }
+TSR_HD fp2 neg(fp2 a) { return fp2{neg(a.c0), neg(a.c1)}; }
+TSR_HD fp2 mul_base(fp2 a, uint64_t b) { return fp2{mul(a.c0, b), mul(a.c1, b)}; }
+TSR_HD uint64_t mul7(uint64_t a) {
+    // a*7 < 2^67: lo + hi*(2^64 mod p) with hi < 8.
+    uint64_t lo = a * 7, hi = mulhi(a, 7);
+    uint64_t t = lo + hi * EPS;
+    t += (t < lo) ? EPS : 0;
+    return canon(t);
+}
+TSR_HD fp2 mul(fp2 a, fp2 b) {
+    uint64_t v0 = mul(a.c0, b.c0), v1 = mul(a.c1, b.c1);
+    uint64_t cross = mul(add(a.c0, a.c1), add(b.c0, b.c1));
+    return fp2{add(v0, mul7(v1)), sub(sub(cross, v0), v1)};
+}
+TSR_HD bool eq(fp2 a, fp2 b) { return a.c0 == b.c0 && a.c1 == b.c1; }
+inline __host__ __device__ fp2 inv(fp2 a) {
+    uint64_t norm = sub(mul(a.c0, a.c0), mul7(mul(a.c1, a.c1)));
+    uint64_t ni = inv(norm);
+    return fp2{mul(a.c0, ni), neg(mul(a.c1, ni))};
+}
+
+}  // namespace gl

@@ -1,0 +1,29 @@
+// The row encoder's plan and tables (see ntt.hip).
+#pragma once
+#include <utility>
+#include <vector>
+#include "common.h"
+
+namespace tsr {
+
+struct Encoder {
+    Ctx *ctx = nullptr;
+    uint32_t log_k = 0, log_r = 0, tk = 0;
+    uint64_t P = 0;
+    uint64_t shift_pow_k = 0;
+    // (lo, nb) stage-bit ranges of the strided passes, in launch order.
+    std::vector<std::pair<uint32_t, uint32_t>> ct_passes, gs_passes;
+    DevBuf<uint64_t> zk_inv, zn_fwd, spread_tab, pow7_tab;
+
+    Status init(Ctx *ctx, uint32_t log_k, uint32_t log_r, uint64_t P);
+    // u64 elements of scratch encode() needs for `count` rows.
+    uint64_t scratch_elems(uint64_t count) const;
+    // All pointers are device pointers; launches on `stream`, does not synchronize.
+    Status encode(const uint64_t *rows, const uint64_t *masks, uint64_t count,
+                  uint64_t *codewords, uint64_t *scratch, hipStream_t stream) const;
+};
+
+// The context's cached encoder for (log_k, r, P): tables are built and uploaded once.
+Status get_encoder(Ctx *ctx, uint32_t log_k, uint32_t log_r, uint64_t P, Encoder **out);
+
+}  // namespace tsr

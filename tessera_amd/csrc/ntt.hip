@@ -1,0 +1,295 @@
+// Reed-Solomon row encoding on gfx950: the NTT kernels.
+//
+// What is computed is fixed by the reference's RowEncoder (crates/tessera-pcs/src/encoding.rs:
+// 292-320): codeword symbol j = g(7 * w_n^j), g = f + (X^k - 1) m. The kernels follow the same
+// algebra — Cooley-Tukey with the inverse twiddles of size k (natural -> bit-reversed, giving
+// k f_{rev(t)}), a spread by 7^{rev_k(t)}/k to stride 2^r with the mask terms, then
+// Gentleman-Sande with the forward twiddles of size n (bit-reversed -> natural) — but organised
+// for the GPU:
+//
+//   * k_tile_pass: a run of up to 9 radix-2 stages over index bits [lo, lo+nb) of every row,
+//     on LDS tiles of 2^(nb+logc) elements: 2^nb strided points x 2^logc consecutive columns,
+//     so each wave reads and writes 128-byte runs (logc = 4) from HBM.
+//   * k_encode_mid: the fused middle — the low CT stages of the size-k inverse transform on a
+//     contiguous tile of 2^tk values, the spread and mask, and the low GS stages of the size-n
+//     forward transform on the 2^(tk+r) symbols that tile expands to. One HBM read of the row
+//     tile, one write of the codeword tile.
+//
+// Twiddles come from tables Z[t] = w^{rev(t)} (t < size/2; ntt.rs:110-125), read through the
+// L2/MALL; the group of a butterfly at stage bit b is g = p >> (b + 1) for the low index p.
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include "field.h"
+#include "ntt.h"
+
+namespace tsr {
+
+constexpr int TILE_THREADS = 512;
+constexpr int TILE_MAX_LOG = 13;  // 8192 u64 = 64 KiB of LDS
+
+// Stages over bits [lo, lo+nb) of every row of length 2^L. Element e of the tile is
+// (mid = e >> logc, col = e & (C-1)); global index p = hi*2^(lo+nb) + mid*2^lo + lblk*C + col.
+template <bool GS>
+__global__ void __launch_bounds__(TILE_THREADS)
+k_tile_pass(const uint64_t *__restrict__ in, uint64_t *__restrict__ out, uint64_t in_stride,
+            uint64_t out_stride, uint32_t L, uint32_t lo, uint32_t nb, uint32_t logc,
+            const uint64_t *__restrict__ tw) {
+    extern __shared__ uint64_t lds[];
+    const uint32_t C = 1u << logc;
+    const uint32_t tiles_per_hi = 1u << (lo - logc);
+    const uint64_t hi = blockIdx.x / tiles_per_hi;
+    const uint64_t lblk = blockIdx.x % tiles_per_hi;
+    const uint64_t row = blockIdx.y;
+    const uint64_t base = (hi << (lo + nb)) + (lblk << logc);
+    const uint32_t tile = 1u << (nb + logc);
+    const uint64_t *src = in + row * in_stride;
+    uint64_t *dst = out + row * out_stride;
+
+    for (uint32_t e = threadIdx.x; e < tile; e += TILE_THREADS) {
+        uint64_t mid = e >> logc, col = e & (C - 1);
+        lds[e] = src[base + (mid << lo) + col];
+    }
+    __syncthreads();
+
+    const uint32_t bfly = tile >> 1;
+    for (uint32_t s = 0; s < nb; s++) {
+        const uint32_t sb = GS ? s : (nb - 1 - s);  // stage bit relative to lo
+        const uint32_t b = lo + sb;
+        for (uint32_t q = threadIdx.x; q < bfly; q += TILE_THREADS) {
+            uint32_t col = q & (C - 1), mq = q >> logc;
+            uint32_t mid = ((mq >> sb) << (sb + 1)) | (mq & ((1u << sb) - 1));
+            uint32_t e0 = (mid << logc) | col, e1 = e0 + (1u << (sb + logc));
+            uint64_t p = base + ((uint64_t)mid << lo) + col;
+            uint64_t w = tw[p >> (b + 1)];
+            uint64_t u = lds[e0], v = lds[e1];
+            if (GS) {
+                lds[e0] = gl::add(u, v);
+                lds[e1] = gl::mul(gl::sub(u, v), w);
+            } else {
+                uint64_t t = gl::mul(v, w);
+                lds[e0] = gl::add(u, t);
+                lds[e1] = gl::sub(u, t);
+            }
+        }
+        __syncthreads();
+    }
+
+    for (uint32_t e = threadIdx.x; e < tile; e += TILE_THREADS) {
+        uint64_t mid = e >> logc, col = e & (C - 1);
+        dst[base + (mid << lo) + col] = lds[e];
+    }
+}
+
+__device__ __forceinline__ uint32_t rev_bits(uint64_t x, uint32_t bits) {
+    return bits == 0 ? 0u : (uint32_t)(__builtin_bitreverse64(x) >> (64 - bits));
+}
+
+// The fused middle of the encoding, one tile of 2^tk row values -> 2^(tk+r) codeword symbols.
+__global__ void __launch_bounds__(TILE_THREADS)
+k_encode_mid(const uint64_t *__restrict__ in, uint64_t in_stride, uint64_t *__restrict__ cw,
+             uint64_t cw_stride, uint32_t log_k, uint32_t log_r, uint32_t tk,
+             const uint64_t *__restrict__ zk_inv, const uint64_t *__restrict__ zn,
+             const uint64_t *__restrict__ spread, const uint64_t *__restrict__ masks,
+             uint64_t P, const uint64_t *__restrict__ pow7, uint64_t shift_pow_k) {
+    extern __shared__ uint64_t lds[];
+    const uint32_t Tk = 1u << tk, Tn = Tk << log_r, s = 1u << log_r;
+    const uint64_t c = blockIdx.x, row = blockIdx.y;
+    const uint64_t kbase = c << tk, nbase = c << (tk + log_r);
+    const uint64_t *src = in + row * in_stride;
+
+    for (uint32_t e = threadIdx.x; e < Tk; e += TILE_THREADS) lds[e] = src[kbase + e];
+    __syncthreads();
+
+    // Cooley-Tukey stages b = tk-1 .. 0 of the size-k inverse transform.
+    for (int b = (int)tk - 1; b >= 0; b--) {
+        for (uint32_t q = threadIdx.x; q < (Tk >> 1); q += TILE_THREADS) {
+            uint32_t e0 = ((q >> b) << (b + 1)) | (q & ((1u << b) - 1)), e1 = e0 + (1u << b);
+            uint64_t w = zk_inv[(kbase + e0) >> (b + 1)];
+            uint64_t u = lds[e0], v = lds[e1];
+            uint64_t t = gl::mul(v, w);
+            lds[e0] = gl::add(u, t);
+            lds[e1] = gl::sub(u, t);
+        }
+        __syncthreads();
+    }
+
+    // Spread (encoding.rs:323-340) and masks (encoding.rs:303-312): entry t times
+    // 7^{rev_k(t)}/k to position s*t, mask terms at s*t and s*t + s/2, zero elsewhere.
+    // Registers hold the k values while the n tile is written over them.
+    constexpr int PER = (8192 >> 1) / TILE_THREADS;  // at most Tk/threads values per thread
+    uint64_t held[PER];
+#pragma unroll
+    for (int i = 0; i < PER; i++) {
+        uint32_t e = threadIdx.x + i * TILE_THREADS;
+        held[i] = e < Tk ? lds[e] : 0;
+    }
+    __syncthreads();
+    for (uint32_t e = threadIdx.x; e < Tn; e += TILE_THREADS) lds[e] = 0;
+    __syncthreads();
+#pragma unroll
+    for (int i = 0; i < PER; i++) {
+        uint32_t e = threadIdx.x + i * TILE_THREADS;
+        if (e < Tk) {
+            uint64_t t = kbase + e;
+            uint64_t v = gl::mul(held[i], spread[t]);
+            uint32_t mi = rev_bits(t, log_k);
+            if (mi < P) {
+                uint64_t scaled = gl::mul(masks[row * P + mi], pow7[mi]);
+                v = gl::sub(v, scaled);
+                lds[e * s + (s >> 1)] = gl::mul(scaled, shift_pow_k);
+            }
+            lds[e * s] = v;
+        }
+    }
+    __syncthreads();
+
+    // Gentleman-Sande stages b = 0 .. tk+r-1 of the size-n forward transform.
+    const uint32_t tn = tk + log_r;
+    for (uint32_t b = 0; b < tn; b++) {
+        for (uint32_t q = threadIdx.x; q < (Tn >> 1); q += TILE_THREADS) {
+            uint32_t e0 = ((q >> b) << (b + 1)) | (q & ((1u << b) - 1)), e1 = e0 + (1u << b);
+            uint64_t w = zn[(nbase + e0) >> (b + 1)];
+            uint64_t u = lds[e0], v = lds[e1];
+            lds[e0] = gl::add(u, v);
+            lds[e1] = gl::mul(gl::sub(u, v), w);
+        }
+        __syncthreads();
+    }
+
+    uint64_t *dst = cw + row * cw_stride;
+    for (uint32_t e = threadIdx.x; e < Tn; e += TILE_THREADS) dst[nbase + e] = lds[e];
+}
+
+// ---------------------------------------------------------------------------------------------
+
+static void bit_reversed_powers(uint64_t base, uint32_t bits, uint64_t scale,
+                                std::vector<uint64_t> &out) {
+    std::vector<uint64_t> squares(bits);
+    uint64_t sq = base;
+    for (uint32_t i = 0; i < bits; i++) {
+        squares[i] = sq;
+        sq = gl::mul(sq, sq);
+    }
+    out.assign(1, scale);
+    out.reserve((size_t)1 << bits);
+    for (int e = (int)bits - 1; e >= 0; e--) {
+        size_t len = out.size();
+        for (size_t t = 0; t < len; t++) out.push_back(gl::mul(out[t], squares[e]));
+    }
+}
+
+// Z[t] = w^{rev(t)} for t < n/2 (ntt.rs:110-125).
+static std::vector<uint64_t> twiddle_table(uint32_t log_n, bool inverse) {
+    std::vector<uint64_t> out;
+    if (log_n == 0) return std::vector<uint64_t>(1, 1);
+    uint64_t root = gl::root(log_n);
+    uint64_t base = inverse ? gl::pow(root, (1ull << log_n) - 1) : root;
+    bit_reversed_powers(base, log_n - 1, 1, out);
+    return out;
+}
+
+static std::vector<std::pair<uint32_t, uint32_t>> split_bits(uint32_t lo, uint32_t hi,
+                                                             uint32_t max_nb) {
+    std::vector<std::pair<uint32_t, uint32_t>> passes;
+    uint32_t bits = hi > lo ? hi - lo : 0;
+    if (!bits) return passes;
+    uint32_t count = (bits + max_nb - 1) / max_nb;
+    uint32_t at = lo;
+    for (uint32_t i = 0; i < count; i++) {
+        uint32_t nb = bits / count + (i < bits % count ? 1 : 0);
+        passes.push_back({at, nb});
+        at += nb;
+    }
+    return passes;
+}
+
+Status Encoder::init(Ctx *ctx, uint32_t log_k, uint32_t log_r, uint64_t P) {
+    this->ctx = ctx;
+    this->log_k = log_k;
+    this->log_r = log_r;
+    this->P = P;
+    const uint32_t log_n = log_k + log_r;
+    const uint64_t k = 1ull << log_k;
+    tk = log_k < (uint32_t)(TILE_MAX_LOG - log_r) ? log_k : (uint32_t)(TILE_MAX_LOG - log_r);
+    // CT passes over k-bits [tk, log_k), run from the top bits down.
+    auto ct = split_bits(tk, log_k, TILE_MAX_LOG - 4);
+    ct_passes.assign(ct.rbegin(), ct.rend());
+    gs_passes = split_bits(tk + log_r, log_n, TILE_MAX_LOG - 4);
+
+    std::vector<uint64_t> zki = twiddle_table(log_k, true);
+    std::vector<uint64_t> zn = twiddle_table(log_n, false);
+    std::vector<uint64_t> spread;
+    bit_reversed_powers(7, log_k, gl::pow(gl::P / 2 + 1, log_k), spread);
+    std::vector<uint64_t> pow7(P ? P : 1, 1);
+    for (uint64_t i = 1; i < P; i++) pow7[i] = gl::mul(pow7[i - 1], 7);
+    shift_pow_k = gl::pow(7, k);
+
+    Status st;
+    if ((st = zk_inv.upload(ctx, zki))) return st;
+    if ((st = zn_fwd.upload(ctx, zn))) return st;
+    if ((st = spread_tab.upload(ctx, spread))) return st;
+    if ((st = pow7_tab.upload(ctx, pow7))) return st;
+    return Status::ok();
+}
+
+uint64_t Encoder::scratch_elems(uint64_t count) const {
+    return ct_passes.empty() ? 0 : count << log_k;
+}
+
+Status Encoder::encode(const uint64_t *rows, const uint64_t *masks, uint64_t count,
+                       uint64_t *codewords, uint64_t *scratch, hipStream_t stream) const {
+    const uint64_t k = 1ull << log_k, n = k << log_r;
+    for (uint64_t r0 = 0; r0 < count; r0 += 65535) {
+        const uint64_t rc = count - r0 < 65535 ? count - r0 : 65535;
+        const uint64_t *src = rows + r0 * k;
+        uint64_t src_stride = k;
+        for (auto &ps : ct_passes) {
+            uint32_t lo = ps.first, nb = ps.second, logc = lo < 4 ? lo : 4;
+            dim3 grid((unsigned)(k >> (nb + logc)), (unsigned)rc);
+            size_t shm = sizeof(uint64_t) << (nb + logc);
+            KernelTimer timer(ctx, "k_tile_pass<ct>", stream);
+            hipLaunchKernelGGL(k_tile_pass<false>, grid, dim3(TILE_THREADS), shm, stream, src,
+                               scratch + r0 * k, src_stride, k, log_k, lo, nb, logc,
+                               zk_inv.ptr);
+            src = scratch + r0 * k;
+        }
+        {
+            dim3 grid((unsigned)(k >> tk), (unsigned)rc);
+            size_t shm = sizeof(uint64_t) << (tk + log_r);
+            KernelTimer timer(ctx, "k_encode_mid", stream);
+            hipLaunchKernelGGL(k_encode_mid, grid, dim3(TILE_THREADS), shm, stream, src,
+                               src_stride, codewords + r0 * n, n, log_k, log_r, tk,
+                               zk_inv.ptr, zn_fwd.ptr, spread_tab.ptr,
+                               masks ? masks + r0 * P : nullptr, P, pow7_tab.ptr,
+                               shift_pow_k);
+        }
+        for (auto &ps : gs_passes) {
+            uint32_t lo = ps.first, nb = ps.second, logc = lo < 4 ? lo : 4;
+            uint32_t log_n = log_k + log_r;
+            dim3 grid((unsigned)(n >> (nb + logc)), (unsigned)rc);
+            size_t shm = sizeof(uint64_t) << (nb + logc);
+            KernelTimer timer(ctx, "k_tile_pass<gs>", stream);
+            hipLaunchKernelGGL(k_tile_pass<true>, grid, dim3(TILE_THREADS), shm, stream,
+                               codewords + r0 * n, codewords + r0 * n, n, n, log_n, lo, nb,
+                               logc, zn_fwd.ptr);
+        }
+    }
+    return check_launch("encode_rows");
+}
+
+Status get_encoder(Ctx *ctx, uint32_t log_k, uint32_t log_r, uint64_t P, Encoder **out) {
+    auto key = std::make_tuple(log_k, log_r, P);
+    auto it = ctx->encoders.find(key);
+    if (it != ctx->encoders.end()) {
+        *out = it->second.get();
+        return Status::ok();
+    }
+    std::shared_ptr<Encoder> e = std::make_shared<Encoder>();
+    Status st = e->init(ctx, log_k, log_r, P);
+    if (st) return st;
+    *out = e.get();
+    ctx->encoders.emplace(key, std::move(e));
+    return Status::ok();
+}
+
+}  // namespace tsr

@@ -1,0 +1,195 @@
+"""The commitment side of Tessera's prover on the MI355X back end, with the reference's API shape.
+
+Mirrors crates/tessera-pcs (RowEncoder, CommitmentParams, ReferenceCommitment, ColumnAnswer) and
+crates/tessera-hash (ColumnChains, MerkleTree::root) over the C ABI. Field elements are numpy
+uint64 arrays of canonical Goldilocks values; digests are 32-byte `bytes`. Errors are raised as
+the reference's PcsError/ColumnError variants would be (tessera_amd.lib.BadShape with the
+variant's name in the message), before any device work.
+"""
+import ctypes
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from .device import Device, DeviceBuffer, ptr_of
+from .lib import CommitmentParams as _CParams
+from .lib import check, lib, u64, vp
+
+
+class RowEncoder:
+    """RowEncoder::new(message_len, log2_inv_rate, mask_coeffs) (encoding.rs:126-170)."""
+
+    def __init__(self, device: Device, message_len: int, log2_inv_rate: int, mask_coeffs: int = 0):
+        if message_len <= 0 or message_len & (message_len - 1):
+            from .lib import BadShape
+            raise BadShape(2, f"NotPowerOfTwo: {message_len}")
+        self.device = device
+        self.log_k = message_len.bit_length() - 1
+        self.log_r = log2_inv_rate
+        self.mask_coeffs = mask_coeffs
+
+    def message_len(self):
+        return 1 << self.log_k
+
+    def codeword_len(self):
+        return 1 << (self.log_k + self.log_r)
+
+    def encode_rows(self, rows, masks=None, out=None):
+        """encode_rows (encoding.rs:257-290): rows (count x k) -> codewords (count x n)."""
+        k, n = self.message_len(), self.codeword_len()
+        if isinstance(rows, DeviceBuffer):
+            count = rows.count // k
+        else:
+            rows = np.ascontiguousarray(rows, dtype=np.uint64).reshape(-1)
+            count = rows.size // k
+        if masks is not None and not isinstance(masks, DeviceBuffer):
+            masks = np.ascontiguousarray(masks, dtype=np.uint64).reshape(-1)
+        if out is None:
+            out = np.zeros(count * n, dtype=np.uint64)
+        check(lib().tsr_encode_rows(self.device.handle, self.log_k, self.log_r, self.mask_coeffs,
+                                    ptr_of(rows), ptr_of(masks), count, ptr_of(out)))
+        return out.reshape(count, n) if isinstance(out, np.ndarray) else out
+
+    def encode_row(self, row, masks=None):
+        return self.encode_rows(row, masks)[0]
+
+
+class ColumnChains:
+    """ColumnChains::new / absorb_block / finish (crates/tessera-hash/src/columns.rs:72-148)."""
+
+    def __init__(self, device: Device, commitment_id: bytes, columns: int, block: int):
+        h = vp()
+        check(lib().tsr_chains_create(device.handle, bytes(commitment_id), columns, block,
+                                      ctypes.byref(h)))
+        self.handle = h.value
+        self.columns = columns
+
+    def absorb_block(self, rows):
+        if isinstance(rows, DeviceBuffer):
+            count = rows.count // self.columns
+        else:
+            rows = np.ascontiguousarray(rows, dtype=np.uint64).reshape(-1)
+            count = rows.size // self.columns if self.columns else 0
+            if rows.size % self.columns:
+                from .lib import BadShape
+                raise BadShape(2, "PartialRow: symbols do not fill whole rows")
+        check(lib().tsr_chains_absorb(self.handle, ptr_of(rows), count))
+
+    def finish(self):
+        out = ctypes.create_string_buffer(32 * self.columns)
+        check(lib().tsr_chains_read(self.handle, out))
+        raw = out.raw
+        lib().tsr_chains_free(self.handle)
+        self.handle = None
+        return [raw[32 * j:32 * j + 32] for j in range(self.columns)]
+
+
+def merkle_root(device: Device, leaves):
+    """MerkleTree::new(leaves).root() (crates/tessera-hash/src/merkle.rs:93-119)."""
+    data = b"".join(leaves)
+    out = ctypes.create_string_buffer(32)
+    check(lib().tsr_merkle_root(device.handle, data, len(leaves), out))
+    return out.raw
+
+
+@dataclass
+class CommitmentParamsPy:
+    """CommitmentParams (crates/tessera-pcs/src/commitment.rs:79-103)."""
+    log2_k: int
+    log2_inv_rate: int
+    mask_coeffs: int
+    row_block: int
+    commitment_id: bytes = field(default=bytes(32))
+
+    def c(self):
+        p = _CParams()
+        p.log2_k = self.log2_k
+        p.log2_inv_rate = self.log2_inv_rate
+        p.mask_coeffs = self.mask_coeffs
+        p.row_block = self.row_block
+        p.commitment_id[:] = list(self.commitment_id)
+        return p
+
+
+@dataclass
+class ColumnAnswer:
+    """ColumnAnswer (commitment.rs:578-598): columns of the distinct queries + multiproof."""
+    indices: list
+    columns: list
+    siblings: list
+
+
+class ReferenceCommitment:
+    """ReferenceCommitment::commit_unmasked / root / leaves / answer (commitment.rs:343-576),
+    with the codewords, leaves and tree resident in HBM."""
+
+    def __init__(self, device, handle, params, count):
+        self.device = device
+        self.handle = handle
+        self.params = params
+        self.count = count
+        self.n = 1 << (params.log2_k + params.log2_inv_rate)
+
+    @classmethod
+    def commit_unmasked(cls, device: Device, params: CommitmentParamsPy, rows):
+        k = 1 << params.log2_k
+        if isinstance(rows, DeviceBuffer):
+            count = rows.count // k
+        else:
+            rows = np.ascontiguousarray(rows, dtype=np.uint64).reshape(-1)
+            if rows.size % k:
+                from .lib import BadShape
+                raise BadShape(2, "MatrixLength: rows do not hold N*k values")
+            count = rows.size // k
+        h = vp()
+        cp = params.c()
+        check(lib().tsr_commit_unmasked(device.handle, ctypes.byref(cp), ptr_of(rows), count,
+                                        ctypes.byref(h)))
+        return cls(device, h.value, params, count)
+
+    def root(self) -> bytes:
+        out = ctypes.create_string_buffer(32)
+        check(lib().tsr_commitment_root(self.handle, out))
+        return out.raw
+
+    def leaves(self):
+        out = ctypes.create_string_buffer(32 * self.n)
+        check(lib().tsr_commitment_leaves(self.handle, out))
+        raw = out.raw
+        return [raw[32 * j:32 * j + 32] for j in range(self.n)]
+
+    def codewords_device_ptr(self):
+        p = vp()
+        cnt = u64()
+        n = u64()
+        check(lib().tsr_commitment_codewords(self.handle, ctypes.byref(p), ctypes.byref(cnt),
+                                             ctypes.byref(n)))
+        return p.value, cnt.value, n.value
+
+    def answer(self, queries) -> ColumnAnswer:
+        q = np.ascontiguousarray(np.array(queries, dtype=np.uint64))
+        cap_d = max(1, len(set(int(x) for x in q)))
+        cap_s = 64 * cap_d + 64
+        distinct = np.zeros(cap_d, dtype=np.uint64)
+        cols = np.zeros(cap_d * max(self.count, 1), dtype=np.uint64)
+        sib = ctypes.create_string_buffer(32 * cap_s)
+        nd, ns = u64(), u64()
+        check(lib().tsr_commitment_answer(self.handle, q.ctypes.data if q.size else None, q.size,
+                                          distinct.ctypes.data, cap_d, cols.ctypes.data, sib, cap_s,
+                                          ctypes.byref(nd), ctypes.byref(ns)))
+        d = nd.value
+        raw = sib.raw
+        columns = [cols[i * self.count:(i + 1) * self.count].copy() for i in range(d)]
+        return ColumnAnswer([int(x) for x in distinct[:d]], columns,
+                            [raw[32 * i:32 * i + 32] for i in range(ns.value)])
+
+    def free(self):
+        if self.handle:
+            lib().tsr_commitment_free(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass

@@ -25,9 +25,12 @@ void orc_hasher_init(orc_hasher *x, int domain);
 void orc_hasher_update(orc_hasher *x, const void *in, size_t len);
 void orc_hasher_finalize(const orc_hasher *x, uint8_t out[32]);
 
-/* Run fn(arg, i) for i in [0, count) over `threads` pthreads (static contiguous split). */
+/* Run fn(arg, lo, hi) over [0, count) split into `threads` contiguous ranges on pthreads. */
 void orc_parallel_for(uint64_t count, int threads, void (*fn)(void *, uint64_t, uint64_t),
                       void *arg);
+/* The same, fn also given its range's index t in [0, threads). */
+void orc_parallel_for_t(uint64_t count, int threads,
+                        void (*fn)(void *, int, uint64_t, uint64_t), void *arg);
 
 static inline void orc_put_u64(uint8_t *p, uint64_t v) {
     for (int i = 0; i < 8; i++) p[i] = (uint8_t)(v >> (8 * i));

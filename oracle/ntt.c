@@ -263,36 +263,54 @@ int orc_encode_rows(uint32_t log_k, uint32_t log_r, uint64_t P, const uint64_t *
 
 /* ---- a small static-split thread pool ---- */
 typedef struct {
-    void (*fn)(void *, uint64_t, uint64_t);
+    void (*fn)(void *, int, uint64_t, uint64_t);
     void *arg;
+    int t;
     uint64_t lo, hi;
 } job_t;
 static void *job_run(void *p) {
     job_t *j = p;
-    j->fn(j->arg, j->lo, j->hi);
+    j->fn(j->arg, j->t, j->lo, j->hi);
     return NULL;
 }
-void orc_parallel_for(uint64_t count, int threads, void (*fn)(void *, uint64_t, uint64_t),
-                      void *arg) {
+void orc_parallel_for_t(uint64_t count, int threads,
+                        void (*fn)(void *, int, uint64_t, uint64_t), void *arg) {
     if (threads <= 1 || count <= 1) {
-        fn(arg, 0, count);
+        fn(arg, 0, 0, count);
         return;
     }
     if ((uint64_t)threads > count) threads = (int)count;
-    pthread_t tid[256];
-    job_t jobs[256];
     if (threads > 256) threads = 256;
+    pthread_t tid[256];
+    int spawned[256] = {0};
+    job_t jobs[256];
     for (int t = 0; t < threads; t++) {
         jobs[t].fn = fn;
         jobs[t].arg = arg;
+        jobs[t].t = t;
         jobs[t].lo = count * t / threads;
         jobs[t].hi = count * (t + 1) / threads;
-        if (t > 0 && pthread_create(&tid[t], NULL, job_run, &jobs[t]) != 0) {
-            job_run(&jobs[t]); /* spawn failure: run it on this thread */
-            tid[t] = 0;
-        }
+    }
+    for (int t = 1; t < threads; t++) {
+        if (pthread_create(&tid[t], NULL, job_run, &jobs[t]) == 0) spawned[t] = 1;
+        else job_run(&jobs[t]); /* spawn failure: run it on this thread */
     }
     job_run(&jobs[0]);
     for (int t = 1; t < threads; t++)
-        if (tid[t]) pthread_join(tid[t], NULL);
+        if (spawned[t]) pthread_join(tid[t], NULL);
+}
+
+typedef struct {
+    void (*fn)(void *, uint64_t, uint64_t);
+    void *arg;
+} plain_t;
+static void plain_run(void *p, int t, uint64_t lo, uint64_t hi) {
+    (void)t;
+    plain_t *x = p;
+    x->fn(x->arg, lo, hi);
+}
+void orc_parallel_for(uint64_t count, int threads, void (*fn)(void *, uint64_t, uint64_t),
+                      void *arg) {
+    plain_t x = {fn, arg};
+    orc_parallel_for_t(count, threads, plain_run, &x);
 }

@@ -338,3 +338,96 @@ def column_chain_at(commitment_id: bytes, j: int, symbols, block):
         part = b"".join(s.to_bytes(8, "little") for s in syms[r:r + block])
         d = hash_domain(DOMAIN_INDEX["column-step"], d + part)
     return d
+
+
+# ---------------- sumcheck ----------------
+
+class _Table(ctypes.Structure):
+    _fields_ = [("data", vp), ("ext", i32), ("is_public", i32)]
+
+
+class _Instance(ctypes.Structure):
+    _fields_ = [("vars", u32), ("factors", u32), ("tables", ctypes.POINTER(_Table))]
+
+
+def _sumcheck_decl():
+    L = lib()
+    L.orc_prove_batch.restype = i64
+    L.orc_prove_batch.argtypes = [vp, cp, u32, ctypes.POINTER(_Instance), u32, vp, vp, vp, vp, vp, i32]
+    L.orc_eq_table.restype = None
+    L.orc_eq_table.argtypes = [vp, u32, vp]
+    L.orc_mle_eval.restype = None
+    L.orc_mle_eval.argtypes = [vp, i32, u32, vp, vp]
+    L.orc_round_eval.restype = None
+    L.orc_round_eval.argtypes = [vp, u32, vp, vp, vp]
+    return L
+
+
+def prove_batch(transcript, prefix, degree, instances, threads=1):
+    """prove_batch of one stage. instances: list of lists of (values, ext, public) where values is
+    a numpy uint64 array (2^n entries, or 2^n x 2 for ext). Returns dict(gamma, claim, rounds,
+    challenges, finals)."""
+    L = _sumcheck_decl()
+    keep = []
+    insts = (_Instance * len(instances))()
+    n_max = 0
+    n_final = 0
+    for i, tabs in enumerate(instances):
+        arr = (_Table * len(tabs))()
+        vars_ = None
+        for k, (vals, ext, public) in enumerate(tabs):
+            v = np.ascontiguousarray(np.asarray(vals, dtype=np.uint64).reshape(-1))
+            keep.append(v)
+            n = v.size // (2 if ext else 1)
+            vars_ = n.bit_length() - 1
+            arr[k].data = v.ctypes.data
+            arr[k].ext = int(bool(ext))
+            arr[k].is_public = int(bool(public))
+            n_final += 0 if public else 1
+        keep.append(arr)
+        insts[i].vars = vars_
+        insts[i].factors = len(tabs)
+        insts[i].tables = arr
+        n_max = max(n_max, vars_)
+    gamma = np.zeros(2, dtype=np.uint64)
+    claim = np.zeros(2, dtype=np.uint64)
+    rounds = np.zeros(max(1, n_max * degree * 2), dtype=np.uint64)
+    chals = np.zeros(max(1, n_max * 2), dtype=np.uint64)
+    finals = np.zeros(max(1, n_final * 2), dtype=np.uint64)
+    nf = L.orc_prove_batch(ctypes.byref(transcript.t), prefix.encode(), degree, insts, len(instances),
+                           gamma.ctypes.data, claim.ctypes.data, rounds.ctypes.data, chals.ctypes.data,
+                           finals.ctypes.data, threads)
+    if nf < 0:
+        raise ValueError("bad batch")
+    return {"gamma": tuple(int(x) for x in gamma), "claim": tuple(int(x) for x in claim),
+            "rounds": rounds[:n_max * degree * 2].reshape(n_max, degree, 2),
+            "challenges": chals[:n_max * 2].reshape(n_max, 2),
+            "finals": finals[:nf * 2].reshape(nf, 2)}
+
+
+def eq_table(point):
+    L = _sumcheck_decl()
+    p = np.ascontiguousarray(np.asarray(point, dtype=np.uint64).reshape(-1))
+    n = p.size // 2
+    out = np.zeros((1 << n) * 2, dtype=np.uint64)
+    L.orc_eq_table(p.ctypes.data if p.size else None, n, out.ctypes.data)
+    return out.reshape(-1, 2)
+
+
+def mle_eval(table, ext, point):
+    L = _sumcheck_decl()
+    t = np.ascontiguousarray(np.asarray(table, dtype=np.uint64).reshape(-1))
+    p = np.ascontiguousarray(np.asarray(point, dtype=np.uint64).reshape(-1))
+    out = np.zeros(2, dtype=np.uint64)
+    L.orc_mle_eval(t.ctypes.data, int(ext), p.size // 2, p.ctypes.data if p.size else None, out.ctypes.data)
+    return (int(out[0]), int(out[1]))
+
+
+def round_eval(evals, degree, claim, r):
+    L = _sumcheck_decl()
+    e = np.ascontiguousarray(np.asarray(evals, dtype=np.uint64).reshape(-1))
+    c = np.array(claim, dtype=np.uint64)
+    rr = np.array(r, dtype=np.uint64)
+    out = np.zeros(2, dtype=np.uint64)
+    L.orc_round_eval(e.ctypes.data, degree, c.ctypes.data, rr.ctypes.data, out.ctypes.data)
+    return (int(out[0]), int(out[1]))

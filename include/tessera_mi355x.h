@@ -146,6 +146,31 @@ tsr_status tsr_transcript_grind(tsr_transcript *t, const char *label, uint8_t bi
 tsr_status tsr_transcript_check_grind(tsr_transcript *t, const char *label, uint8_t bits,
                                       uint64_t nonce, int *valid);
 
+/* ---- the batched product sumcheck (prove_batch, crates/tessera-sumcheck/src/prover.rs:1122-1145;
+ *      Table / Instance::new, prover.rs:78-305; SumcheckProof, proof.rs:16-24) ---- */
+typedef struct {
+    const uint64_t *data; /* 2^vars entries: u64 each (Fp, Table::base) or 2 u64 (Fp2, Table::ext) */
+    uint32_t ext;         /* 1: Fp2 entries */
+    uint32_t is_public;   /* 1: a public factor, left out of the final evaluations (Table::public) */
+} tsr_table;
+typedef struct {
+    uint32_t vars;            /* n_i: every table of the instance holds 2^n_i entries */
+    uint32_t factors;         /* the instance's relation is the product of its tables */
+    const tsr_table *tables;
+} tsr_instance;
+/* Proves one batched product sumcheck of `degree` under the labels of `prefix` (e.g. "stage/0")
+ * on transcript `t`: gamma from <prefix>/gamma, instance i weighted gamma^i and front-padded to
+ * n_max = max n_i, round j's message p_j(0), p_j(2), ..., p_j(d) absorbed under
+ * <prefix>/round/{j}, r_j squeezed under <prefix>/r/{j}, the final evaluations (public factors
+ * left out) absorbed under <prefix>/evals. Outputs: gamma (1 Fp2), rounds (n_max x degree Fp2),
+ * challenges (n_max Fp2), finals (n_finals Fp2; capacity finals_cap). The rounds and their
+ * transcript run on the device. Errors, before the transcript moves: degree outside 1..=10,
+ * no instance, an instance with no table or more tables than the degree (BAD_SHAPE). */
+tsr_status tsr_sumcheck_prove(tsr_ctx *ctx, tsr_transcript *t, const char *prefix,
+                              uint32_t degree, const tsr_instance *instances, uint32_t count,
+                              uint64_t *gamma_out, uint64_t *rounds_out, uint64_t *challenges_out,
+                              uint64_t *finals_out, uint64_t finals_cap, uint64_t *n_finals);
+
 /* ---- per-kernel timing with HIP events on the context's stream (the reference's CUDA-event
  *      harness, docs/perf/v0.1/cc-h100.md:30-40) ---- */
 /* enable = 1 starts recording an event pair around every launch; 0 stops; 2 clears. */

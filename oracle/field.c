@@ -90,6 +90,9 @@ orc_fp2 orc_fp2_inv(orc_fp2 a) {
     orc_fp2 r = {orc_fp_mul(a.c0, ni), orc_fp_sub(0, orc_fp_mul(a.c1, ni))};
     return r;
 }
+void orc_fp_mul_vec(const uint64_t *a, const uint64_t *b, uint64_t *out, uint64_t n) {
+    for (uint64_t i = 0; i < n; i++) out[i] = orc_fp_mul(a[i], b[i]);
+}
 void orc_fp2_mul_v(const uint64_t a[2], const uint64_t b[2], uint64_t out[2]) {
     orc_fp2 x = {a[0], a[1]}, y = {b[0], b[1]};
     orc_fp2 r = orc_fp2_mul(x, y);

@@ -53,6 +53,7 @@ orc_fp2 orc_fp2_mul(orc_fp2 a, orc_fp2 b);
 orc_fp2 orc_fp2_mul_base(orc_fp2 a, uint64_t b);
 orc_fp2 orc_fp2_inv(orc_fp2 a);
 void orc_fp2_mul_v(const uint64_t a[2], const uint64_t b[2], uint64_t out[2]);
+void orc_fp_mul_vec(const uint64_t *a, const uint64_t *b, uint64_t *out, uint64_t n);
 void orc_fp2_inv_v(const uint64_t a[2], uint64_t out[2]);
 
 /* ---- BLAKE3 (spec; crate blake3 1.8.7 as called by crates/tessera-hash/src/domain.rs) ---- */
@@ -155,21 +156,25 @@ void orc_round_eval(const uint64_t *evals /* d Fp2 */, uint32_t degree, const ui
 
 /* ---- LogUp (crates/tessera-logup/src/{instance,prover}.rs, tessera-sumcheck trees.rs) ---- */
 typedef struct {
-    uint64_t rows;           /* T */
-    const uint64_t *keys;    /* T keys */
-    const uint64_t *values;  /* T values or NULL (has_values = 0) */
-    uint64_t lookups;        /* N >= 1 */
-    const uint64_t *indices; /* N row indices */
+    uint64_t rows;                 /* T >= 1 */
+    const uint64_t *keys;          /* T keys */
+    const uint64_t *values;        /* T values, or NULL for a table without values */
+    uint64_t lookups;              /* N >= 1 */
+    const uint64_t *lookup_keys;   /* N keys */
+    const uint64_t *lookup_values; /* N values, NULL iff values is NULL */
+    const uint64_t *lookup_rows;   /* N row indices (orc_multiplicities only) */
+    const uint64_t *multiplicities;/* 2^{h_t} entries (Fp) */
 } orc_logup_instance;
 /* multiplicities over 2^{h_t} table leaves (instance.rs:225-253). */
 void orc_multiplicities(const orc_logup_instance *inst, uint64_t *out /* 2^h_t */);
-/* One LogUp batch, lookup and table tree paired per instance (TreeId::paired). Writes alpha,
- * beta, roots (4 Fp2 per instance), values (Fp2 per instance with values) and the final
- * transcript; level messages into `levels` (caller-sized; see logup.c). Returns 0 or -1. */
+/* prove_logup (prover.rs:361-469) with fraction trees only: schedule lists 2*count tree ids
+ * (2k = lookup tree k, 2k+1 = table tree k). Outputs alpha, beta, roots (4 Fp2 per instance),
+ * values (one Fp2 per instance with values), leaf claims (4 Fp2 per instance: lookup num/den,
+ * table num/den) and a per-level log (see logup.c). Returns 0. */
 int orc_prove_logup(orc_transcript *t, const orc_logup_instance *inst, uint32_t count,
-                    uint64_t *alpha, uint64_t *beta, uint64_t *roots, uint64_t *values,
-                    uint64_t *level_log, uint64_t level_log_cap, uint64_t *level_log_len,
-                    int threads);
+                    const uint32_t *schedule, uint64_t *alpha, uint64_t *beta, uint64_t *roots,
+                    uint64_t *values, uint64_t *n_values, uint64_t *claims, uint64_t *log,
+                    uint64_t cap, uint64_t *log_len);
 
 #ifdef __cplusplus
 }

@@ -431,3 +431,101 @@ def round_eval(evals, degree, claim, r):
     out = np.zeros(2, dtype=np.uint64)
     L.orc_round_eval(e.ctypes.data, degree, c.ctypes.data, rr.ctypes.data, out.ctypes.data)
     return (int(out[0]), int(out[1]))
+
+
+def fp_mul_vec(a, b):
+    """Entrywise a[i] * b[i] mod p."""
+    L = lib()
+    L.orc_fp_mul_vec.restype = None
+    L.orc_fp_mul_vec.argtypes = [vp, vp, vp, u64]
+    a = np.ascontiguousarray(a, dtype=np.uint64)
+    b = np.ascontiguousarray(b, dtype=np.uint64)
+    out = np.empty_like(a)
+    L.orc_fp_mul_vec(a.ctypes.data, b.ctypes.data, out.ctypes.data, a.size)
+    return out
+
+
+# ---------------- LogUp ----------------
+
+class _LogUpInstance(ctypes.Structure):
+    _fields_ = [("rows", u64), ("keys", vp), ("values", vp), ("lookups", u64), ("lookup_keys", vp),
+                ("lookup_values", vp), ("lookup_rows", vp), ("multiplicities", vp)]
+
+
+def multiplicities(rows, lookup_rows):
+    """Exact counts over 2^ceil(log2 T) leaves, row 0 counting the lookup tree's padding."""
+    L = lib()
+    L.orc_multiplicities.restype = None
+    L.orc_multiplicities.argtypes = [ctypes.POINTER(_LogUpInstance), vp]
+    lr = np.ascontiguousarray(lookup_rows, dtype=np.uint64)
+    inst = _LogUpInstance(rows, None, None, lr.size, None, None, lr.ctypes.data, None)
+    h = (rows - 1).bit_length() if rows > 1 else 0
+    out = np.zeros(1 << h, dtype=np.uint64)
+    L.orc_multiplicities(ctypes.byref(inst), out.ctypes.data)
+    return out
+
+
+def prove_logup(transcript, instances, schedule):
+    """instances: list of dicts with keys, values (or None), lookup_keys, lookup_values (or None),
+    multiplicities. schedule: tree ids (2k lookup, 2k+1 table). Returns a dict with alpha, beta,
+    roots, values, claims and the per-level records."""
+    L = lib()
+    L.orc_prove_logup.restype = i32
+    L.orc_prove_logup.argtypes = [vp, ctypes.POINTER(_LogUpInstance), u32, vp, vp, vp, vp, vp, vp,
+                                  vp, vp, u64, vp]
+    keep = []
+    arr = (_LogUpInstance * max(1, len(instances)))()
+
+    def buf(x):
+        if x is None:
+            return None
+        a = np.ascontiguousarray(x, dtype=np.uint64)
+        keep.append(a)
+        return a.ctypes.data
+    for k, ins in enumerate(instances):
+        arr[k].rows = len(ins["keys"])
+        arr[k].keys = buf(ins["keys"])
+        arr[k].values = buf(ins["values"])
+        arr[k].lookups = len(ins["lookup_keys"])
+        arr[k].lookup_keys = buf(ins["lookup_keys"])
+        arr[k].lookup_values = buf(ins["lookup_values"])
+        arr[k].lookup_rows = None
+        arr[k].multiplicities = buf(ins["multiplicities"])
+    n = len(instances)
+    sched = np.ascontiguousarray(schedule, dtype=np.uint32)
+    alpha = np.zeros(2, np.uint64)
+    beta = np.zeros(2, np.uint64)
+    roots = np.zeros(max(1, 8 * n), np.uint64)
+    values = np.zeros(max(1, 2 * n), np.uint64)
+    claims = np.zeros(max(1, 8 * n), np.uint64)
+    nv = ctypes.c_uint64(0)
+    cap = 1 << 22
+    log = np.zeros(cap, np.uint64)
+    ll = ctypes.c_uint64(0)
+    L.orc_prove_logup(ctypes.byref(transcript.t), arr, n, sched.ctypes.data if sched.size else None,
+                      alpha.ctypes.data, beta.ctypes.data, roots.ctypes.data, values.ctypes.data,
+                      ctypes.byref(nv), claims.ctypes.data, log.ctypes.data, cap, ctypes.byref(ll))
+    assert ll.value <= cap
+    return {"alpha": tuple(int(x) for x in alpha), "beta": tuple(int(x) for x in beta),
+            "roots": roots[:8 * n].reshape(-1, 2), "values": values[:2 * nv.value].reshape(-1, 2),
+            "claims": claims[:8 * n].reshape(-1, 2), "levels": parse_level_log(log[:ll.value])}
+
+
+def parse_level_log(log):
+    out, at = [], 0
+    log = [int(x) for x in log]
+
+    def f2(i):
+        return (log[i], log[i + 1])
+    while at < len(log):
+        n = log[at]; at += 1
+        lv = {"gamma": f2(at), "lambda": f2(at + 2)}; at += 4
+        lv["rounds"] = [[f2(at + 6 * j + 2 * z) for z in range(3)] for j in range(n)]; at += 6 * n
+        lv["challenges"] = [f2(at + 2 * j) for j in range(n)]; at += 2 * n
+        nf = log[at]; at += 1
+        lv["final_evals"] = [f2(at + 2 * i) for i in range(nf)]; at += 2 * nf
+        lv["rho"] = f2(at); at += 2
+        lv["state"] = b"".join(w.to_bytes(8, "little") for w in log[at:at + 4]).hex(); at += 4
+        lv["counter"] = log[at]; at += 1
+        out.append(lv)
+    return out

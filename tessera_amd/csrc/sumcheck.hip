@@ -1,0 +1,553 @@
+// The batched product sumcheck on gfx950, with the round loop and its transcript on the device.
+//
+// Semantics: crates/tessera-sumcheck/src/prover.rs:1122-1410 (prove_batch / prove_checked,
+// State::{new, add_round, final_evals}, padding_scale): instance i on n_i variables, weight
+// gamma^i, front-padded to n_max; round j's message is p_j(0), p_j(2), ..., p_j(d) of the
+// batched polynomial, absorbed as TCE Vec<Fp2> under <prefix>/round/{j}; r_j is squeezed under
+// <prefix>/r/{j}; a padding round adds 2^{pad - j - 1} gamma^i S_i at every node; binding is
+// t[k] = t[2k] + r (t[2k+1] - t[2k]) (tessera-poly multilinear.rs:70-121).
+//
+// Execution: the host squeezes gamma, then enqueues the whole loop on the context's stream:
+//   first-round sums of every instance (k_sc_round, no bind), then for each round j a one-block
+//   k_sc_transcript (reduces the per-block partial sums, forms the message, absorbs it, squeezes
+//   r_j with the device transcript) followed by k_sc_round for every instance in its own rounds
+//   (binds all its tables with r_j and, fused in the same pass, sums the next round's node
+//   products). One synchronization at the end; the host then absorbs the final evaluations.
+#include <hip/hip_runtime.h>
+#include <algorithm>
+#include <memory>
+#include <stdint.h>
+#include <string.h>
+#include <string>
+#include <vector>
+#include "common.h"
+#include "devtranscript.h"
+#include "field.h"
+#include "sumcheck.h"
+#include "transcript.h"
+
+
+namespace tsr {
+
+using gl::fp2;
+
+constexpr int SC_MAXF = 10;
+constexpr int SC_MAXD = 10;
+constexpr int SC_THREADS = 256;
+constexpr int SC_MAX_BLOCKS = 1024;
+
+struct ScTables {
+    const void *src[SC_MAXF];
+    fp2 *dst[SC_MAXF];
+};
+
+__device__ __forceinline__ fp2 ld_entry(const void *p, bool base, uint64_t x) {
+    if (base) return fp2{((const uint64_t *)p)[x], 0};
+    return ((const fp2 *)p)[x];
+}
+
+// Node products of one pair: slots [p(0), p(2), ..., p(D), p(1)] (D+1 of them).
+template <int D>
+__device__ __forceinline__ void pair_products(const fp2 *a, const fp2 *b, uint32_t F,
+                                              fp2 prod[D + 1]) {
+    for (uint32_t k = 0; k < F; k++) {
+        fp2 diff = gl::sub(b[k], a[k]);
+        fp2 line = gl::add(b[k], diff);  // z = 2
+        if (k == 0) {
+            prod[0] = a[k];
+            prod[D] = b[k];
+#pragma unroll
+            for (int z = 2; z <= D; z++) {
+                prod[z - 1] = line;
+                line = gl::add(line, diff);
+            }
+        } else {
+            prod[0] = gl::mul(prod[0], a[k]);
+            prod[D] = gl::mul(prod[D], b[k]);
+#pragma unroll
+            for (int z = 2; z <= D; z++) {
+                prod[z - 1] = gl::mul(prod[z - 1], line);
+                line = gl::add(line, diff);
+            }
+        }
+    }
+}
+
+__device__ __forceinline__ uint64_t shfl_xor_u64(uint64_t v, int mask) {
+    uint32_t lo = (uint32_t)v, hi = (uint32_t)(v >> 32);
+    lo = __shfl_xor(lo, mask, 64);
+    hi = __shfl_xor(hi, mask, 64);
+    return ((uint64_t)hi << 32) | lo;
+}
+
+// One own round of one instance: bind every table with r (when out_len > 0) and/or sum the node
+// products over `pairs` pairs (when pairs > 0), writing one partial per block.
+//   sums only:   pairs of the unbound tables src (entries 2q, 2q+1)
+//   bind + sums: dst[2q], dst[2q+1] from src[4q..4q+3]; the pair (dst[2q], dst[2q+1]) summed
+//   bind only:   dst[q] from src[2q], src[2q+1] for q < out_len
+template <int D>
+__global__ void __launch_bounds__(SC_THREADS)
+k_sc_round(ScTables t, uint32_t F, uint32_t base_mask, uint64_t out_len, uint64_t pairs,
+           const fp2 *__restrict__ r_ptr, fp2 *__restrict__ partials) {
+    const bool bind = out_len > 0;
+    const fp2 r = bind ? *r_ptr : fp2{0, 0};
+    const uint64_t items = pairs > 0 ? pairs : out_len;
+    const uint64_t stride = (uint64_t)gridDim.x * SC_THREADS;
+    fp2 acc[D + 1];
+#pragma unroll
+    for (int z = 0; z <= D; z++) acc[z] = fp2{0, 0};
+
+    for (uint64_t q = (uint64_t)blockIdx.x * SC_THREADS + threadIdx.x; q < items; q += stride) {
+        fp2 a[SC_MAXF], b[SC_MAXF];
+        for (uint32_t k = 0; k < F; k++) {
+            const bool base = (base_mask >> k) & 1;
+            if (!bind) {
+                a[k] = ld_entry(t.src[k], base, 2 * q);
+                b[k] = ld_entry(t.src[k], base, 2 * q + 1);
+            } else if (pairs > 0) {
+                fp2 e0 = ld_entry(t.src[k], base, 4 * q), e1 = ld_entry(t.src[k], base, 4 * q + 1);
+                fp2 e2 = ld_entry(t.src[k], base, 4 * q + 2), e3 = ld_entry(t.src[k], base, 4 * q + 3);
+                a[k] = gl::add(e0, gl::mul(r, gl::sub(e1, e0)));
+                b[k] = gl::add(e2, gl::mul(r, gl::sub(e3, e2)));
+                t.dst[k][2 * q] = a[k];
+                t.dst[k][2 * q + 1] = b[k];
+            } else {
+                fp2 e0 = ld_entry(t.src[k], base, 2 * q), e1 = ld_entry(t.src[k], base, 2 * q + 1);
+                t.dst[k][q] = gl::add(e0, gl::mul(r, gl::sub(e1, e0)));
+            }
+        }
+        if (pairs > 0) {
+            fp2 prod[D + 1];
+            pair_products<D>(a, b, F, prod);
+#pragma unroll
+            for (int z = 0; z <= D; z++) acc[z] = gl::add(acc[z], prod[z]);
+        }
+    }
+    if (pairs == 0) return;
+
+    // wave reduction, then across the block's waves through LDS
+    __shared__ fp2 red[SC_THREADS / 64][D + 1];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+#pragma unroll
+    for (int z = 0; z <= D; z++) {
+        fp2 v = acc[z];
+        for (int m = 32; m >= 1; m >>= 1) {
+            fp2 o{shfl_xor_u64(v.c0, m), shfl_xor_u64(v.c1, m)};
+            v = gl::add(v, o);
+        }
+        if (lane == 0) red[wave][z] = v;
+    }
+    __syncthreads();
+    if (threadIdx.x <= D) {
+        fp2 s = red[0][threadIdx.x];
+        for (int w = 1; w < SC_THREADS / 64; w++) s = gl::add(s, red[w][threadIdx.x]);
+        partials[(uint64_t)blockIdx.x * (D + 1) + threadIdx.x] = s;
+    }
+}
+
+// Per-instance facts the transcript kernel needs.
+struct ScInst {
+    uint32_t vars, pad;
+    fp2 weight;    // gamma^i
+    fp2 wsum;      // gamma^i S_i (set by the host for vars = 0, by round 0 otherwise)
+    fp2 *partials; // (D+1) per block of its last k_sc_round
+};
+
+struct ScCtl {
+    DevTranscript t;
+    fp2 r;  // the current round's challenge
+};
+
+// Round j: reduce the partial sums, form the message, absorb, squeeze r_j.
+// nblk[i]: the number of blocks whose partials instance i left for this round.
+template <int D>
+__global__ void __launch_bounds__(SC_THREADS)
+k_sc_transcript(uint32_t j, ScInst *__restrict__ inst, uint32_t count,
+                const uint32_t *__restrict__ nblk, ScCtl *__restrict__ ctl, DevKeys keys,
+                const uint8_t *__restrict__ labels, const uint32_t *__restrict__ label_lens,
+                fp2 *__restrict__ rounds_out, fp2 *__restrict__ chals_out) {
+    extern __shared__ fp2 sums[];  // count x (D+1)
+    for (uint32_t x = threadIdx.x; x < count * (D + 1); x += SC_THREADS) {
+        uint32_t i = x / (D + 1), z = x % (D + 1);
+        fp2 s{0, 0};
+        if (j >= inst[i].pad || (j == 0 && inst[i].vars > 0)) {
+            const fp2 *p = inst[i].partials;
+            for (uint32_t b = 0; b < nblk[i]; b++) s = gl::add(s, p[(uint64_t)b * (D + 1) + z]);
+        }
+        sums[x] = s;
+    }
+    __syncthreads();
+    if (threadIdx.x != 0) return;
+    fp2 msg[D];
+    for (int z = 0; z < D; z++) msg[z] = fp2{0, 0};
+    for (uint32_t i = 0; i < count; i++) {
+        ScInst &in = inst[i];
+        const fp2 *s = sums + (uint64_t)i * (D + 1);
+        if (j == 0 && in.pad > 0 && in.vars > 0)
+            in.wsum = gl::mul(in.weight, gl::add(s[0], s[D]));  // S_i = p(0) + p(1)
+        if (j < in.pad) {
+            uint64_t scale = 1ull << (in.pad - j - 1);
+            fp2 v = gl::mul_base(in.wsum, scale);
+            for (int z = 0; z < D; z++) msg[z] = gl::add(msg[z], v);
+        } else {
+            for (int z = 0; z < D; z++) msg[z] = gl::add(msg[z], gl::mul(in.weight, s[z]));
+        }
+    }
+    DevTranscript t = ctl->t;
+    dev_absorb_fp2s(t, keys, labels + (2 * j) * LABEL_SLOT, label_lens[2 * j], msg, D);
+    fp2 r = dev_challenge_fp2(t, keys, labels + (2 * j + 1) * LABEL_SLOT, label_lens[2 * j + 1]);
+    ctl->t = t;
+    ctl->r = r;
+    for (int z = 0; z < D; z++) rounds_out[(uint64_t)j * D + z] = msg[z];
+    chals_out[j] = r;
+}
+
+// Gathers each final evaluation: entry 0 of the buffer at ptrs[e] (base or Fp2).
+__global__ void k_sc_gather(const void *const *__restrict__ ptrs, const uint32_t *__restrict__ base,
+                            uint32_t n, fp2 *__restrict__ out) {
+    uint32_t e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e < n) out[e] = ld_entry(ptrs[e], base[e] != 0, 0);
+}
+
+// ---------------------------------------------------------------------------------------------
+
+template <int D>
+static void launch_round(dim3 grid, hipStream_t s, const ScTables &t, uint32_t F,
+                         uint32_t base_mask, uint64_t out_len, uint64_t pairs, const fp2 *r,
+                         fp2 *partials) {
+    hipLaunchKernelGGL(k_sc_round<D>, grid, dim3(SC_THREADS), 0, s, t, F, base_mask, out_len,
+                       pairs, r, partials);
+}
+
+static void launch_round_d(uint32_t d, dim3 grid, hipStream_t s, const ScTables &t, uint32_t F,
+                           uint32_t base_mask, uint64_t out_len, uint64_t pairs, const fp2 *r,
+                           fp2 *partials) {
+    switch (d) {
+#define CASE(D) \
+    case D: launch_round<D>(grid, s, t, F, base_mask, out_len, pairs, r, partials); break;
+        CASE(1) CASE(2) CASE(3) CASE(4) CASE(5) CASE(6) CASE(7) CASE(8) CASE(9) CASE(10)
+#undef CASE
+    }
+}
+
+static void launch_transcript_d(uint32_t d, hipStream_t s, uint32_t j, ScInst *inst,
+                                uint32_t count, const uint32_t *nblk, ScCtl *ctl,
+                                const DevKeys &keys, const uint8_t *labels,
+                                const uint32_t *lens, fp2 *rounds, fp2 *chals) {
+    size_t shm = sizeof(fp2) * count * (d + 1);
+    switch (d) {
+#define CASE(D)                                                                               \
+    case D:                                                                                   \
+        hipLaunchKernelGGL(k_sc_transcript<D>, dim3(1), dim3(SC_THREADS), shm, s, j, inst,     \
+                           count, nblk, ctl, keys, labels, lens, rounds, chals);              \
+        break;
+        CASE(1) CASE(2) CASE(3) CASE(4) CASE(5) CASE(6) CASE(7) CASE(8) CASE(9) CASE(10)
+#undef CASE
+    }
+}
+
+static uint32_t blocks_for_pairs(uint64_t items) {
+    uint64_t b = (items + SC_THREADS - 1) / SC_THREADS;
+    return (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(b, SC_MAX_BLOCKS));
+}
+
+DevKeys dev_keys(const Ctx *ctx) {
+    DevKeys k;
+    memcpy(k.absorb, ctx->keys[DOM_TRANSCRIPT_ABSORB], 32);
+    memcpy(k.squeeze, ctx->keys[DOM_TRANSCRIPT_SQUEEZE], 32);
+    memcpy(k.ratchet, ctx->keys[DOM_TRANSCRIPT_RATCHET], 32);
+    return k;
+}
+
+void load_state_words(const uint8_t st[32], uint32_t w[8]) {
+    for (int i = 0; i < 8; i++)
+        w[i] = (uint32_t)st[4 * i] | ((uint32_t)st[4 * i + 1] << 8) |
+               ((uint32_t)st[4 * i + 2] << 16) | ((uint32_t)st[4 * i + 3] << 24);
+}
+
+void store_state_words(const uint32_t w[8], uint8_t st[32]) {
+    for (int i = 0; i < 8; i++)
+        for (int j = 0; j < 4; j++) st[4 * i + j] = (uint8_t)(w[i] >> (8 * j));
+}
+
+Status prove_batch(Ctx *ctx, Transcript &tr, const char *prefix, uint32_t degree,
+                   const tsr_instance *instances, uint32_t count, SumcheckOut &out) {
+    hipStream_t s = ctx->stream;
+    const uint32_t D = degree;
+    std::vector<uint32_t> vars(count), pads(count);
+    uint32_t n_max = 0;
+    for (uint32_t i = 0; i < count; i++) {
+        uint64_t len = 0;
+        // lengths were checked by the caller: equal powers of two within an instance
+        (void)len;
+        vars[i] = instances[i].vars;
+        n_max = std::max(n_max, vars[i]);
+    }
+    for (uint32_t i = 0; i < count; i++) pads[i] = n_max - vars[i];
+
+    // gamma and the weights, on the host transcript (prover.rs:1147-1150)
+    std::string p(prefix);
+    gl::fp2 gamma = tr.challenge_fp2((p + "/gamma").c_str(), nullptr);
+    out.gamma = gamma;
+    std::vector<ScInst> hinst(count);
+    gl::fp2 w{1, 0};
+    for (uint32_t i = 0; i < count; i++) {
+        hinst[i].vars = vars[i];
+        hinst[i].pad = pads[i];
+        hinst[i].weight = w;
+        hinst[i].wsum = fp2{0, 0};
+        w = gl::mul(w, gamma);
+    }
+
+    // device copies of host tables; working buffers
+    struct Tab {
+        const void *src = nullptr;  // device
+        bool base = false;
+        bool is_public = false;
+        DevBuf<uint64_t> upload;
+        DevBuf<fp2> a, b;
+    };
+    std::vector<std::vector<std::unique_ptr<Tab>>> tabs(count);
+    std::vector<DevBuf<fp2>> partials(count);
+    for (uint32_t i = 0; i < count; i++) {
+        const uint64_t len = 1ull << vars[i];
+        for (uint32_t k = 0; k < instances[i].factors; k++) {
+            const tsr_table &tt = instances[i].tables[k];
+            std::unique_ptr<Tab> tab(new Tab());
+            tab->base = !tt.ext;
+            tab->is_public = tt.is_public != 0;
+            const uint64_t words = len * (tt.ext ? 2 : 1);
+            if (is_device_ptr(tt.data)) {
+                tab->src = tt.data;
+            } else {
+                Status st = tab->upload.alloc(ctx, words);
+                if (st) return st;
+                TSR_HIP(hipMemcpyAsync(tab->upload.ptr, tt.data, 8 * words,
+                                       hipMemcpyHostToDevice, s));
+                tab->src = tab->upload.ptr;
+            }
+            if (vars[i] >= 1) {
+                Status st = tab->a.alloc(ctx, len / 2);
+                if (st) return st;
+                if (vars[i] >= 2 && (st = tab->b.alloc(ctx, len / 4))) return st;
+            }
+            tabs[i].push_back(std::move(tab));
+        }
+        Status st = partials[i].alloc(ctx, (uint64_t)SC_MAX_BLOCKS * (D + 1));
+        if (st) return st;
+        hinst[i].partials = partials[i].ptr;
+    }
+
+    // S_i of instances on no variable: the product of their single entries (host side)
+    for (uint32_t i = 0; i < count; i++) {
+        if (vars[i] != 0 || pads[i] == 0) continue;
+        fp2 prod{1, 0};
+        for (auto &tab : tabs[i]) {
+            fp2 v{0, 0};
+            TSR_HIP(hipMemcpyAsync(&v, tab->src, tab->base ? 8 : 16, hipMemcpyDeviceToHost, s));
+            TSR_HIP(hipStreamSynchronize(s));
+            prod = gl::mul(prod, v);
+        }
+        hinst[i].wsum = gl::mul(hinst[i].weight, prod);
+    }
+
+    // labels of every round, the control block, per-round block counts
+    std::vector<uint8_t> labels((size_t)2 * std::max(n_max, 1u) * LABEL_SLOT, 0);
+    std::vector<uint32_t> lens(2 * std::max(n_max, 1u), 0);
+    for (uint32_t j = 0; j < n_max; j++) {
+        std::string a = p + "/round/" + std::to_string(j), b = p + "/r/" + std::to_string(j);
+        if (a.size() > LABEL_SLOT || b.size() > LABEL_SLOT)
+            return Status::fail(TSR_DECLINED, "label longer than 64 bytes");
+        memcpy(&labels[(2 * j) * LABEL_SLOT], a.data(), a.size());
+        memcpy(&labels[(2 * j + 1) * LABEL_SLOT], b.data(), b.size());
+        lens[2 * j] = (uint32_t)a.size();
+        lens[2 * j + 1] = (uint32_t)b.size();
+    }
+    // nblk[j][i]: blocks of instance i's partials that round j's message reads
+    std::vector<uint32_t> nblk((size_t)std::max(n_max, 1u) * count, 0);
+    for (uint32_t i = 0; i < count; i++) {
+        for (uint32_t j = 0; j < n_max; j++) {
+            if (vars[i] == 0) continue;
+            if (j < pads[i] && j != 0) continue;
+            // own round t = j - pad (or the first sums at j = 0 for a padded instance)
+            uint32_t t = j < pads[i] ? 0 : j - pads[i];
+            uint64_t pairs = 1ull << (vars[i] - t - 1);
+            nblk[(size_t)j * count + i] = blocks_for_pairs(pairs);
+        }
+    }
+    ScCtl hctl;
+    load_state_words(tr.state, hctl.t.state);
+    hctl.t.counter = tr.counter;
+    hctl.r = fp2{0, 0};
+
+    DevBuf<uint8_t> d_labels;
+    DevBuf<uint32_t> d_lens, d_nblk;
+    DevBuf<ScInst> d_inst;
+    DevBuf<ScCtl> d_ctl;
+    DevBuf<fp2> d_rounds, d_chals;
+    Status st;
+    if ((st = d_labels.alloc(ctx, labels.size())) || (st = d_lens.alloc(ctx, lens.size())) ||
+        (st = d_nblk.alloc(ctx, nblk.size())) || (st = d_inst.alloc(ctx, count)) ||
+        (st = d_ctl.alloc(ctx, 1)) || (st = d_rounds.alloc(ctx, (uint64_t)std::max(n_max, 1u) * D)) ||
+        (st = d_chals.alloc(ctx, std::max(n_max, 1u))))
+        return st;
+    TSR_HIP(hipMemcpyAsync(d_labels.ptr, labels.data(), labels.size(), hipMemcpyHostToDevice, s));
+    TSR_HIP(hipMemcpyAsync(d_lens.ptr, lens.data(), 4 * lens.size(), hipMemcpyHostToDevice, s));
+    TSR_HIP(hipMemcpyAsync(d_nblk.ptr, nblk.data(), 4 * nblk.size(), hipMemcpyHostToDevice, s));
+    TSR_HIP(hipMemcpyAsync(d_inst.ptr, hinst.data(), sizeof(ScInst) * count, hipMemcpyHostToDevice, s));
+    TSR_HIP(hipMemcpyAsync(d_ctl.ptr, &hctl, sizeof(ScCtl), hipMemcpyHostToDevice, s));
+    const DevKeys keys = dev_keys(ctx);
+
+    auto tables_of = [&](uint32_t i, uint32_t t, ScTables &st_, uint32_t &base_mask) {
+        // own round t binds from (t == 0: original, odd t: a, even t: b) into (even t: a, odd: b)
+        base_mask = 0;
+        for (uint32_t k = 0; k < tabs[i].size(); k++) {
+            Tab &tab = *tabs[i][k];
+            if (t == 0) {
+                st_.src[k] = tab.src;
+                if (tab.base) base_mask |= 1u << k;
+            } else {
+                st_.src[k] = (t % 2 == 1) ? (const void *)tab.a.ptr : (const void *)tab.b.ptr;
+            }
+            st_.dst[k] = (t % 2 == 0) ? tab.a.ptr : tab.b.ptr;
+        }
+    };
+
+    {
+        KernelTimer timer(ctx, "k_sc_round", s);
+        // first own round's sums of every instance on at least one variable
+        for (uint32_t i = 0; i < count; i++) {
+            if (vars[i] == 0) continue;
+            ScTables t{};
+            uint32_t bm;
+            tables_of(i, 0, t, bm);
+            uint64_t pairs = 1ull << (vars[i] - 1);
+            launch_round_d(D, dim3(blocks_for_pairs(pairs)), s, t, (uint32_t)tabs[i].size(), bm,
+                           0, pairs, nullptr, partials[i].ptr);
+        }
+    }
+    for (uint32_t j = 0; j < n_max; j++) {
+        {
+            KernelTimer timer(ctx, "k_sc_transcript", s);
+            launch_transcript_d(D, s, j, d_inst.ptr, count, d_nblk.ptr + (size_t)j * count,
+                                d_ctl.ptr, keys, d_labels.ptr, d_lens.ptr, d_rounds.ptr,
+                                d_chals.ptr);
+        }
+        KernelTimer timer(ctx, "k_sc_round", s);
+        for (uint32_t i = 0; i < count; i++) {
+            if (j < pads[i]) continue;
+            uint32_t t = j - pads[i];
+            ScTables tt{};
+            uint32_t bm;
+            tables_of(i, t, tt, bm);
+            uint64_t out_len = 1ull << (vars[i] - t - 1);
+            uint64_t pairs = out_len / 2;  // next own round's pairs (0 after the last)
+            launch_round_d(D, dim3(blocks_for_pairs(pairs ? pairs : out_len)), s, tt,
+                           (uint32_t)tabs[i].size(), bm, out_len, pairs, &d_ctl.ptr->r,
+                           partials[i].ptr);
+        }
+    }
+    if ((st = check_launch("sumcheck rounds"))) return st;
+
+    // final evaluations: each non-public table's single remaining entry
+    std::vector<const void *> fptr;
+    std::vector<uint32_t> fbase;
+    for (uint32_t i = 0; i < count; i++) {
+        for (auto &tab : tabs[i]) {
+            if (tab->is_public) continue;
+            if (vars[i] == 0) {
+                fptr.push_back(tab->src);
+                fbase.push_back(tab->base ? 1 : 0);
+            } else {
+                // the last own round t = vars-1 wrote into a (t even) or b (t odd)
+                fptr.push_back(((vars[i] - 1) % 2 == 0) ? (const void *)tab->a.ptr
+                                                         : (const void *)tab->b.ptr);
+                fbase.push_back(0);
+            }
+        }
+    }
+    out.finals.resize(fptr.size());
+    if (!fptr.empty()) {
+        DevBuf<uint64_t> d_ptrs;
+        DevBuf<uint32_t> d_base;
+        DevBuf<fp2> d_fin;
+        if ((st = d_ptrs.alloc(ctx, fptr.size())) || (st = d_base.alloc(ctx, fbase.size())) ||
+            (st = d_fin.alloc(ctx, fptr.size())))
+            return st;
+        TSR_HIP(hipMemcpyAsync(d_ptrs.ptr, fptr.data(), 8 * fptr.size(), hipMemcpyHostToDevice, s));
+        TSR_HIP(hipMemcpyAsync(d_base.ptr, fbase.data(), 4 * fbase.size(), hipMemcpyHostToDevice, s));
+        hipLaunchKernelGGL(k_sc_gather, dim3((unsigned)((fptr.size() + 63) / 64)), dim3(64), 0, s,
+                           (const void *const *)d_ptrs.ptr, d_base.ptr, (uint32_t)fptr.size(),
+                           d_fin.ptr);
+        if ((st = check_launch("sumcheck finals"))) return st;
+        TSR_HIP(hipMemcpyAsync(out.finals.data(), d_fin.ptr, 16 * fptr.size(),
+                               hipMemcpyDeviceToHost, s));
+    }
+    out.rounds.resize((size_t)n_max * D);
+    out.challenges.resize(n_max);
+    if (n_max) {
+        TSR_HIP(hipMemcpyAsync(out.rounds.data(), d_rounds.ptr, 16 * (size_t)n_max * D,
+                               hipMemcpyDeviceToHost, s));
+        TSR_HIP(hipMemcpyAsync(out.challenges.data(), d_chals.ptr, 16 * (size_t)n_max,
+                               hipMemcpyDeviceToHost, s));
+    }
+    TSR_HIP(hipMemcpyAsync(&hctl, d_ctl.ptr, sizeof(ScCtl), hipMemcpyDeviceToHost, s));
+    TSR_HIP(hipStreamSynchronize(s));
+    store_state_words(hctl.t.state, tr.state);
+    tr.counter = hctl.t.counter;
+    tr.absorb_fp2s((p + "/evals").c_str(), (const uint64_t *)out.finals.data(), out.finals.size());
+    return Status::ok();
+}
+
+}  // namespace tsr
+
+using namespace tsr;
+
+extern "C" tsr_status tsr_sumcheck_prove(tsr_ctx *ctx_, tsr_transcript *t, const char *prefix,
+                                         uint32_t degree, const tsr_instance *instances,
+                                         uint32_t count, uint64_t *gamma_out,
+                                         uint64_t *rounds_out, uint64_t *challenges_out,
+                                         uint64_t *finals_out, uint64_t finals_cap,
+                                         uint64_t *n_finals) {
+    Ctx *ctx = (Ctx *)ctx_;
+    if (!ctx || !t || !prefix) return Status::fail(TSR_BAD_ARG, "null argument").code;
+    // SumcheckError order (prover.rs:1196-1230): degree, no instances, then per instance.
+    if (degree < 1 || degree > SC_MAXD)
+        return Status::fail(TSR_BAD_SHAPE, "Degree: " + std::to_string(degree) +
+                                               " is outside 1..=10").code;
+    if (count == 0 || !instances) return Status::fail(TSR_BAD_SHAPE, "NoInstances").code;
+    if (!label_ok(prefix)) return Status::fail(TSR_BAD_ARG, "a label is printable ASCII").code;
+    uint64_t n_final = 0;
+    for (uint32_t i = 0; i < count; i++) {
+        const tsr_instance &in = instances[i];
+        if (in.factors == 0 || !in.tables)
+            return Status::fail(TSR_BAD_SHAPE, "NoTables: instance " + std::to_string(i)).code;
+        if (in.factors > degree)
+            return Status::fail(TSR_BAD_SHAPE, "TooManyTables: instance " + std::to_string(i) +
+                                                   " has " + std::to_string(in.factors) +
+                                                   " tables at degree " + std::to_string(degree)).code;
+        if (in.vars > 40)
+            return Status::fail(TSR_DECLINED, "tables above 2^40 entries").code;
+        for (uint32_t k = 0; k < in.factors; k++) {
+            if (!in.tables[k].data) return Status::fail(TSR_BAD_ARG, "null table").code;
+            if (!in.tables[k].is_public) n_final++;
+        }
+    }
+    if (count > 256) return Status::fail(TSR_DECLINED, "more than 256 instances in one batch").code;
+    if (n_finals) *n_finals = n_final;
+    if (n_final > finals_cap || !finals_out)
+        return Status::fail(TSR_BAD_SHAPE, "final evaluation buffer too small").code;
+    hipError_t e = hipSetDevice(ctx->device);
+    if (e != hipSuccess) return hip_fail(e, "hipSetDevice").code;
+    SumcheckOut out;
+    Status st = prove_batch(ctx, t->t, prefix, degree, instances, count, out);
+    if (st) return st.code;
+    if (gamma_out) {
+        gamma_out[0] = out.gamma.c0;
+        gamma_out[1] = out.gamma.c1;
+    }
+    if (rounds_out) memcpy(rounds_out, out.rounds.data(), 16 * out.rounds.size());
+    if (challenges_out) memcpy(challenges_out, out.challenges.data(), 16 * out.challenges.size());
+    memcpy(finals_out, out.finals.data(), 16 * out.finals.size());
+    return TSR_OK;
+}

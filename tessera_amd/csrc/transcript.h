@@ -51,3 +51,8 @@ struct Transcript {
 };
 
 }  // namespace tsr
+
+// The C ABI's opaque transcript handle.
+struct tsr_transcript {
+    tsr::Transcript t;
+};

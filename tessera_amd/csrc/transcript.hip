@@ -277,9 +277,6 @@ bool Transcript::check_grind(const char *label, uint8_t bits, uint64_t nonce) {
 
 using namespace tsr;
 
-struct tsr_transcript {
-    Transcript t;
-};
 
 #define T_CHECK(cond, status, msg)                          \
     do {                                                    \

@@ -81,6 +81,8 @@ _SIGS = {
     "tsr_transcript_grind": (i32, [vp, ctypes.c_char_p, ctypes.c_uint8, ctypes.POINTER(u64)]),
     "tsr_transcript_check_grind": (i32, [vp, ctypes.c_char_p, ctypes.c_uint8, u64,
                                          ctypes.POINTER(i32)]),
+    "tsr_sumcheck_prove": (i32, [vp, vp, ctypes.c_char_p, u32, vp, u32, vp, vp, vp, vp, u64,
+                                 ctypes.POINTER(u64)]),
     "tsr_ctx_profile": (i32, [vp, i32]),
     "tsr_ctx_kernel_time": (i32, [vp, ctypes.c_char_p, ctypes.POINTER(ctypes.c_double),
                                   ctypes.POINTER(u64)]),

@@ -1,0 +1,98 @@
+"""The batched product sumcheck (crates/tessera-sumcheck: Table, Instance, prove_batch) on the
+MI355X back end. Tables are numpy uint64 arrays (Fp: shape (2^n,), Fp2: shape (2^n, 2)) or
+DeviceBuffers; the round loop and its transcript run on the device."""
+import ctypes
+from dataclasses import dataclass
+
+import numpy as np
+
+from .device import DeviceBuffer
+from .lib import check, lib, u32, u64, vp
+
+
+class _CTable(ctypes.Structure):
+    _fields_ = [("data", vp), ("ext", u32), ("is_public", u32)]
+
+
+class _CInstance(ctypes.Structure):
+    _fields_ = [("vars", u32), ("factors", u32), ("tables", ctypes.POINTER(_CTable))]
+
+
+@dataclass
+class Table:
+    """Table::base / Table::ext, optionally .public() (prover.rs:78-230)."""
+    values: object
+    ext: bool = False
+    is_public: bool = False
+    vars: int = None
+
+    @staticmethod
+    def base(values):
+        return Table(values, ext=False)
+
+    @staticmethod
+    def ext_(values):
+        return Table(values, ext=True)
+
+    def public(self):
+        return Table(self.values, self.ext, True, self.vars)
+
+    def n_vars(self):
+        if self.vars is not None:
+            return self.vars
+        if isinstance(self.values, DeviceBuffer):
+            n = self.values.count // (2 if self.ext else 1)
+        else:
+            n = np.asarray(self.values).size // (2 if self.ext else 1)
+        return n.bit_length() - 1
+
+
+@dataclass
+class ProvenBatch:
+    gamma: tuple
+    rounds: np.ndarray      # n_max x degree x 2
+    challenges: np.ndarray  # n_max x 2
+    final_evals: np.ndarray  # n_finals x 2
+
+
+def prove_batch(device, transcript, prefix: str, degree: int, instances) -> ProvenBatch:
+    """prove_batch(transcript, prefix, degree, instances) (prover.rs:1122-1145); `instances` is a
+    list of lists of Table (each an Instance::new product)."""
+    _SIG = lib().tsr_sumcheck_prove
+    _SIG.restype = ctypes.c_int
+    _SIG.argtypes = [vp, vp, ctypes.c_char_p, u32, ctypes.POINTER(_CInstance), u32, vp, vp, vp, vp,
+                     u64, ctypes.POINTER(u64)]
+    keep = []
+    cinst = (_CInstance * max(1, len(instances)))()
+    n_max, n_final = 0, 0
+    for i, tabs in enumerate(instances):
+        arr = (_CTable * max(1, len(tabs)))()
+        vars_ = tabs[0].n_vars() if tabs else 0
+        for k, t in enumerate(tabs):
+            if isinstance(t.values, DeviceBuffer):
+                ptr = t.values.ptr
+            else:
+                v = np.ascontiguousarray(np.asarray(t.values, dtype=np.uint64).reshape(-1))
+                keep.append(v)
+                ptr = v.ctypes.data
+            arr[k].data = ptr
+            arr[k].ext = int(t.ext)
+            arr[k].is_public = int(t.is_public)
+            n_final += 0 if t.is_public else 1
+        keep.append(arr)
+        cinst[i].vars = vars_
+        cinst[i].factors = len(tabs)
+        cinst[i].tables = arr
+        n_max = max(n_max, vars_)
+    gamma = np.zeros(2, dtype=np.uint64)
+    rounds = np.zeros(max(1, n_max * degree) * 2, dtype=np.uint64)
+    chals = np.zeros(max(1, n_max) * 2, dtype=np.uint64)
+    finals = np.zeros(max(1, n_final) * 2, dtype=np.uint64)
+    nf = u64()
+    check(_SIG(device.handle, transcript.handle, prefix.encode(), degree, cinst, len(instances),
+               gamma.ctypes.data, rounds.ctypes.data, chals.ctypes.data, finals.ctypes.data,
+               max(1, n_final), ctypes.byref(nf)))
+    return ProvenBatch((int(gamma[0]), int(gamma[1])),
+                       rounds[:n_max * degree * 2].reshape(n_max, degree, 2),
+                       chals[:n_max * 2].reshape(n_max, 2),
+                       finals[:nf.value * 2].reshape(nf.value, 2))

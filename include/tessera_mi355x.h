@@ -171,6 +171,39 @@ tsr_status tsr_sumcheck_prove(tsr_ctx *ctx, tsr_transcript *t, const char *prefi
                               uint64_t *gamma_out, uint64_t *rounds_out, uint64_t *challenges_out,
                               uint64_t *finals_out, uint64_t finals_cap, uint64_t *n_finals);
 
+/* ---- LogUp-GKR (prove_logup, crates/tessera-logup/src/prover.rs:361-469; LogUpInstance and
+ *      multiplicities, crates/tessera-logup/src/instance.rs:19-102,225-253) ---- */
+typedef struct {
+    uint64_t rows;                  /* T >= 1 table rows */
+    const uint64_t *keys;           /* T keys (Fp) */
+    const uint64_t *values;         /* T values, or NULL for a table without values */
+    uint64_t lookups;               /* N >= 1 lookups */
+    const uint64_t *lookup_keys;    /* N keys */
+    const uint64_t *lookup_values;  /* N values; NULL exactly when `values` is NULL */
+    const uint64_t *multiplicities; /* 2^ceil(log2 T) entries (Fp) */
+} tsr_logup_instance;
+/* Proves the LogUp part of a leaf stage with fraction trees only: alpha (logup/alpha), beta
+ * (logup/beta), each instance's lookup tree (1, alpha - (key + beta value)) and table tree
+ * (-m_j, alpha - t_j), both padded with row 0, their roots absorbed under logup/roots, the
+ * levels tree/{l} in `schedule` order (2 count tree ids, 2k = lookup tree k, 2k+1 = table tree
+ * k; NULL = TreeId::paired), then the values under logup/values. Outputs: alpha, beta, roots
+ * (4 Fp2 per instance: n_a, d_a, n_t, d_t), values (one Fp2 per instance with values), claims
+ * (4 Fp2 per instance: the lookup and table trees' leaf numerator and denominator), and, when
+ * `log` is given, per level: n_rounds, gamma, lambda, rounds (n x 3 Fp2), challenges, n_evals,
+ * evals, rho, transcript state (4 u64), counter. Errors before the transcript moves: NoRows,
+ * NoLookups, ValuesMismatch, a missing multiplicity column, a schedule that does not name each
+ * tree once (BAD_SHAPE). */
+tsr_status tsr_logup_prove(tsr_ctx *ctx, tsr_transcript *t, const tsr_logup_instance *instances,
+                           uint32_t count, const uint32_t *schedule, uint64_t alpha[2],
+                           uint64_t beta[2], uint64_t *roots, uint64_t *values,
+                           uint64_t *n_values, uint64_t *claims, uint64_t *log,
+                           uint64_t log_cap, uint64_t *log_len);
+/* multiplicities(table, lookups): out[j] = lookups of row j over 2^ceil(log2 rows) entries, row 0
+ * also counting the lookup tree's 2^ceil(log2 N) - N padding leaves. LookupRow (BAD_SHAPE) when a
+ * lookup names no row. */
+tsr_status tsr_logup_multiplicities(tsr_ctx *ctx, uint64_t rows, const uint64_t *lookup_rows,
+                                    uint64_t lookups, uint64_t *out);
+
 /* ---- per-kernel timing with HIP events on the context's stream (the reference's CUDA-event
  *      harness, docs/perf/v0.1/cc-h100.md:30-40) ---- */
 /* enable = 1 starts recording an event pair around every launch; 0 stops; 2 clears. */

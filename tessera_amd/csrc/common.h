@@ -52,6 +52,21 @@ struct DevBuf {
     DevBuf() = default;
     DevBuf(const DevBuf &) = delete;
     DevBuf &operator=(const DevBuf &) = delete;
+    DevBuf(DevBuf &&o) noexcept : ctx(o.ctx), ptr(o.ptr), len(o.len) {
+        o.ptr = nullptr;
+        o.len = 0;
+    }
+    DevBuf &operator=(DevBuf &&o) noexcept {
+        if (this != &o) {
+            release();
+            ctx = o.ctx;
+            ptr = o.ptr;
+            len = o.len;
+            o.ptr = nullptr;
+            o.len = 0;
+        }
+        return *this;
+    }
     ~DevBuf() { release(); }
     void release() {
         if (ptr) dev_free(ctx, ptr, len * sizeof(T));
@@ -84,7 +99,7 @@ struct Ctx {
     uint64_t in_use = 0;  // live allocations
     uint64_t pooled = 0;  // freed blocks kept for reuse
     std::multimap<uint64_t, void *> pool;
-    std::map<std::tuple<uint32_t, uint32_t, uint64_t>, std::shared_ptr<Encoder>> encoders;
+    std::map<std::tuple<uint32_t, uint32_t, uint64_t, uint32_t>, std::shared_ptr<Encoder>> encoders;
     uint32_t keys[19][8];  // BLAKE3 context keys of the reference's 19 domains
     bool profile = false;
     struct Timed {

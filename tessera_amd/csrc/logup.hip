@@ -1,0 +1,497 @@
+// LogUp-GKR on gfx950: fraction trees built on the device, proven level by level with the
+// device-resident round loop of sumcheck.hip.
+//
+// Semantics: crates/tessera-logup/src/prover.rs:201-469 (prove_logup: alpha, beta, leaves
+// (1, alpha - (key + beta value)) padded with row 0, table leaves (-m_j, alpha - t_j), roots
+// absorbed under logup/roots, the levels of prove_trees, values under logup/values),
+// crates/tessera-logup/src/instance.rs:225-253 (multiplicities) and
+// crates/tessera-sumcheck/src/trees.rs:199-345,761-930 (FractionTree: node x of level l is
+// (nL dR + nR dL, dL dR) of entries 2x, 2x+1 of level l+1; per level: gamma, lambda, the
+// rounds of eq(r_i, x)(N0 D1 + N1 D0 + lambda D0 D1), the evals, rho under tree/l/line, next
+// claims at rho || r'_i).
+#include <hip/hip_runtime.h>
+#include <algorithm>
+#include <memory>
+#include <stdint.h>
+#include <string.h>
+#include <string>
+#include <vector>
+#include "common.h"
+#include "field.h"
+#include "sumcheck.h"
+#include "transcript.h"
+
+namespace tsr {
+
+using gl::fp2;
+
+// alpha - (key + beta value), or alpha - key without values (compress, prover.rs:201-207)
+__device__ __forceinline__ fp2 leaf_den(fp2 alpha, fp2 beta, uint64_t key, const uint64_t *vals,
+                                        uint64_t x) {
+    fp2 t = fp2{key, 0};
+    if (vals) t = gl::add(gl::mul_base(beta, vals[x]), t);
+    return gl::sub(alpha, t);
+}
+
+// Leaves of a tree of 2^h entries over `count` pairs, padded with `pad`. When mult is given
+// (a table tree) the numerators -m_x are written (base) too.
+__global__ void k_leaves(const uint64_t *__restrict__ keys, const uint64_t *__restrict__ vals,
+                         uint64_t count, uint64_t leaves, fp2 alpha, fp2 beta, fp2 pad,
+                         const uint64_t *__restrict__ mult, uint64_t *__restrict__ nout,
+                         fp2 *__restrict__ dout) {
+    uint64_t x = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (x >= leaves) return;
+    dout[x] = x < count ? leaf_den(alpha, beta, keys[x], vals, x) : pad;
+    if (mult) nout[x] = gl::neg(mult[x]);
+}
+
+// Level above: node x from children 2x, 2x+1. Numerators below: unit (nb == nullptr and
+// nbase == nullptr), base (nbase) or Fp2 (nb).
+__global__ void k_level_above(const fp2 *__restrict__ nb, const uint64_t *__restrict__ nbase,
+                              const fp2 *__restrict__ db, uint64_t count, fp2 *__restrict__ n,
+                              fp2 *__restrict__ d) {
+    uint64_t x = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (x >= count) return;
+    fp2 dl = db[2 * x], dr = db[2 * x + 1];
+    fp2 nl, nr;
+    if (nb) {
+        nl = nb[2 * x];
+        nr = nb[2 * x + 1];
+    } else if (nbase) {
+        nl = fp2{nbase[2 * x], 0};
+        nr = fp2{nbase[2 * x + 1], 0};
+    } else {
+        nl = fp2{1, 0};
+        nr = fp2{1, 0};
+    }
+    n[x] = gl::add(gl::mul(nl, dr), gl::mul(nr, dl));
+    d[x] = gl::mul(dl, dr);
+}
+
+// eq(r, x) = prod_i (x_i ? r_i : 1 - r_i), x_1 the lowest bit (tessera-poly eq.rs:23).
+__global__ void k_eq_table(const fp2 *__restrict__ point, uint32_t n, fp2 *__restrict__ out) {
+    uint64_t x = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (x >= (1ull << n)) return;
+    fp2 acc{1, 0};
+    for (uint32_t i = 0; i < n; i++) {
+        fp2 r = point[i];
+        acc = gl::mul(acc, ((x >> i) & 1) ? r : gl::sub(fp2{1, 0}, r));
+    }
+    out[x] = acc;
+}
+
+// Partial sums of eq[x] * col[x] (col padded with `pad` from `count` on), one per block.
+__global__ void __launch_bounds__(256)
+k_eq_dot(const fp2 *__restrict__ eq, const uint64_t *__restrict__ col, uint64_t count,
+         uint64_t pad, uint64_t len, fp2 *__restrict__ partials) {
+    fp2 acc{0, 0};
+    for (uint64_t x = (uint64_t)blockIdx.x * 256 + threadIdx.x; x < len; x += (uint64_t)gridDim.x * 256)
+        acc = gl::add(acc, gl::mul_base(eq[x], x < count ? col[x] : pad));
+    __shared__ fp2 red[256];
+    red[threadIdx.x] = acc;
+    __syncthreads();
+    for (int s = 128; s > 0; s >>= 1) {
+        if ((int)threadIdx.x < s) red[threadIdx.x] = gl::add(red[threadIdx.x], red[threadIdx.x + s]);
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) partials[blockIdx.x] = red[0];
+}
+
+__global__ void k_sum_partials(const fp2 *__restrict__ partials, uint32_t n, fp2 *__restrict__ out) {
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    fp2 s{0, 0};
+    for (uint32_t i = 0; i < n; i++) s = gl::add(s, partials[i]);
+    *out = s;
+}
+
+// multiplicities (instance.rs:225-253): exact counts over 2^{h_t} rows.
+__global__ void k_count_rows(const uint64_t *__restrict__ rows, uint64_t n, uint64_t limit,
+                             unsigned long long *__restrict__ counts, unsigned *__restrict__ bad) {
+    uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    uint64_t r = rows[i];
+    if (r < limit) atomicAdd(&counts[r], 1ull);
+    else atomicOr(bad, 1u);
+}
+
+static unsigned nblocks(uint64_t n, unsigned per) {
+    return (unsigned)std::max<uint64_t>(1, (n + per - 1) / per);
+}
+
+static uint32_t ceil_log2(uint64_t x) {
+    uint32_t h = 0;
+    while ((1ull << h) < x) h++;
+    return h;
+}
+
+// One fraction tree on the device: level t (0..h) holds N[t], D[t] of 2^t entries; the leaf
+// level's numerators are unit (lookup tree) or base (table tree, nbase).
+struct DevTree {
+    uint32_t h = 0, join = 0;
+    bool unit_leaves = true;
+    std::vector<DevBuf<fp2>> N, D;
+    DevBuf<uint64_t> nbase;
+    // current claim
+    std::vector<fp2> point;
+    fp2 num{0, 0}, den{0, 0};
+};
+
+static Status build_tree(Ctx *ctx, DevTree &t, uint32_t h, const uint64_t *keys,
+                         const uint64_t *vals, uint64_t count, const uint64_t *mult, fp2 alpha,
+                         fp2 beta, fp2 pad) {
+    hipStream_t s = ctx->stream;
+    t.h = h;
+    t.unit_leaves = mult == nullptr;
+    t.N = std::vector<DevBuf<fp2>>(h + 1);
+    t.D = std::vector<DevBuf<fp2>>(h + 1);
+    const uint64_t leaves = 1ull << h;
+    Status st;
+    if ((st = t.D[h].alloc(ctx, leaves))) return st;
+    if (mult && (st = t.nbase.alloc(ctx, leaves))) return st;
+    {
+        KernelTimer timer(ctx, "k_leaves", s);
+        hipLaunchKernelGGL(k_leaves, dim3(nblocks(leaves, 256)), dim3(256), 0, s, keys, vals,
+                           count, leaves, alpha, beta, pad, mult, t.nbase.ptr, t.D[h].ptr);
+    }
+    KernelTimer timer(ctx, "k_level_above", s);
+    for (int l = (int)h - 1; l >= 0; l--) {
+        const uint64_t m = 1ull << l;
+        if ((st = t.N[l].alloc(ctx, m)) || (st = t.D[l].alloc(ctx, m))) return st;
+        const fp2 *nb = (l + 1 == (int)h) ? nullptr : t.N[l + 1].ptr;
+        const uint64_t *nbb = (l + 1 == (int)h && mult) ? t.nbase.ptr : nullptr;
+        hipLaunchKernelGGL(k_level_above, dim3(nblocks(m, 256)), dim3(256), 0, s, nb, nbb,
+                           t.D[l + 1].ptr, m, t.N[l].ptr, t.D[l].ptr);
+    }
+    return check_launch("fraction tree");
+}
+
+static Status read_root(Ctx *ctx, DevTree &t) {
+    hipStream_t s = ctx->stream;
+    TSR_HIP(hipMemcpyAsync(&t.den, t.D[0].ptr, 16, hipMemcpyDeviceToHost, s));
+    if (t.h == 0) {
+        if (t.unit_leaves) {
+            t.num = fp2{1, 0};
+        } else {
+            uint64_t v = 0;
+            TSR_HIP(hipMemcpyAsync(&v, t.nbase.ptr, 8, hipMemcpyDeviceToHost, s));
+            TSR_HIP(hipStreamSynchronize(s));
+            t.num = fp2{v, 0};
+        }
+    } else {
+        TSR_HIP(hipMemcpyAsync(&t.num, t.N[0].ptr, 16, hipMemcpyDeviceToHost, s));
+    }
+    TSR_HIP(hipStreamSynchronize(s));
+    return Status::ok();
+}
+
+struct Log {
+    uint64_t *buf;
+    uint64_t cap, len = 0;
+    void put(uint64_t v) {
+        if (buf && len < cap) buf[len] = v;
+        len++;
+    }
+    void f2(const fp2 &v) {
+        put(v.c0);
+        put(v.c1);
+    }
+};
+
+Status prove_logup(Ctx *ctx, Transcript &tr, const tsr_logup_instance *inst, uint32_t count,
+                   const std::vector<uint32_t> &schedule, fp2 &alpha, fp2 &beta,
+                   std::vector<fp2> &roots, std::vector<fp2> &values, std::vector<fp2> &claims,
+                   Log &log) {
+    hipStream_t s = ctx->stream;
+    alpha = tr.challenge_fp2("logup/alpha", nullptr);
+    beta = tr.challenge_fp2("logup/beta", nullptr);
+
+    // device copies of the columns
+    std::vector<DevBuf<uint64_t>> up;
+    up.reserve(8 * count + 1);
+    auto dev = [&](const uint64_t *p, uint64_t n, const uint64_t **out) -> Status {
+        if (!p) {
+            *out = nullptr;
+            return Status::ok();
+        }
+        if (is_device_ptr(p)) {
+            *out = p;
+            return Status::ok();
+        }
+        up.emplace_back();
+        Status st = up.back().alloc(ctx, n);
+        if (st) return st;
+        TSR_HIP(hipMemcpyAsync(up.back().ptr, p, 8 * n, hipMemcpyHostToDevice, s));
+        *out = up.back().ptr;
+        return Status::ok();
+    };
+
+    std::vector<DevTree> trees(2 * count);
+    std::vector<const uint64_t *> lvals(count), tvals(count);
+    roots.clear();
+    Status st;
+    for (uint32_t k = 0; k < count; k++) {
+        const tsr_logup_instance &in = inst[k];
+        const uint32_t ha = ceil_log2(in.lookups), ht = ceil_log2(in.rows);
+        const uint64_t *keys, *vals, *lk, *lv, *m;
+        if ((st = dev(in.keys, in.rows, &keys)) || (st = dev(in.values, in.rows, &vals)) ||
+            (st = dev(in.lookup_keys, in.lookups, &lk)) ||
+            (st = dev(in.lookup_values, in.lookups, &lv)) ||
+            (st = dev(in.multiplicities, 1ull << ht, &m)))
+            return st;
+        lvals[k] = lv;
+        tvals[k] = vals;
+        // alpha - t_0, row 0's denominator (prover.rs:247-256), from the host's copy of row 0
+        uint64_t key0 = 0, val0 = 0;
+        TSR_HIP(hipMemcpyAsync(&key0, keys, 8, hipMemcpyDeviceToHost, s));
+        if (vals) TSR_HIP(hipMemcpyAsync(&val0, vals, 8, hipMemcpyDeviceToHost, s));
+        TSR_HIP(hipStreamSynchronize(s));
+        fp2 t0 = fp2{key0, 0};
+        if (vals) t0 = gl::add(gl::mul_base(beta, val0), t0);
+        const fp2 pad = gl::sub(alpha, t0);
+        if ((st = build_tree(ctx, trees[2 * k], ha, lk, lv, in.lookups, nullptr, alpha, beta, pad)) ||
+            (st = build_tree(ctx, trees[2 * k + 1], ht, keys, vals, in.rows, m, alpha, beta, pad)))
+            return st;
+        if ((st = read_root(ctx, trees[2 * k])) || (st = read_root(ctx, trees[2 * k + 1]))) return st;
+        roots.push_back(trees[2 * k].num);
+        roots.push_back(trees[2 * k].den);
+        roots.push_back(trees[2 * k + 1].num);
+        roots.push_back(trees[2 * k + 1].den);
+    }
+    tr.absorb_fp2s("logup/roots", (const uint64_t *)roots.data(), roots.size());
+
+    uint32_t tallest = 0;
+    for (uint32_t id : schedule) tallest = std::max(tallest, trees[id].h);
+    for (uint32_t id : schedule) trees[id].join = tallest - trees[id].h;
+
+    DevBuf<fp2> d_point;
+    if ((st = d_point.alloc(ctx, 64))) return st;
+    for (uint32_t level = 0; level < tallest; level++) {
+        const std::string prefix = "tree/" + std::to_string(level);
+        fp2 gamma = tr.challenge_fp2((prefix + "/gamma").c_str(), nullptr);
+        fp2 lambda = tr.challenge_fp2((prefix + "/lambda").c_str(), nullptr);
+        std::vector<uint32_t> act;
+        for (uint32_t id : schedule)
+            if (trees[id].join <= level) act.push_back(id);
+        std::vector<GenInst> insts(act.size());
+        std::vector<DevBuf<fp2>> eqs(act.size());
+        for (size_t i = 0; i < act.size(); i++) {
+            DevTree &t = trees[act[i]];
+            const uint32_t tl = level - t.join, below = tl + 1, n = (uint32_t)t.point.size();
+            if ((st = eqs[i].alloc(ctx, 1ull << n))) return st;
+            if (n) {
+                TSR_HIP(hipMemcpyAsync(d_point.ptr, t.point.data(), 16 * n, hipMemcpyHostToDevice, s));
+            }
+            {
+                KernelTimer timer(ctx, "k_eq_table", s);
+                hipLaunchKernelGGL(k_eq_table, dim3(nblocks(1ull << n, 256)), dim3(256), 0, s,
+                                   d_point.ptr, n, eqs[i].ptr);
+            }
+            GenInst &g = insts[i];
+            g.vars = n;
+            g.rel = REL_FRACTION;
+            g.claim_given = true;
+            g.claim = gl::add(t.num, gl::mul(lambda, t.den));
+            GenTable eq;
+            eq.src = eqs[i].ptr;
+            eq.is_public = true;
+            GenTable n0, n1, d0, d1;
+            const bool leaf = below == t.h;
+            if (leaf && t.unit_leaves) {
+                n0.src = n1.src = nullptr;
+            } else if (leaf) {
+                n0.src = n1.src = t.nbase.ptr;
+                n0.base = n1.base = true;
+            } else {
+                n0.src = n1.src = t.N[below].ptr;
+            }
+            d0.src = d1.src = t.D[below].ptr;
+            n0.shift = n1.shift = d0.shift = d1.shift = 1;
+            n1.off = d1.off = 1;
+            g.tabs = {eq, n0, n1, d0, d1};
+            // an awaited copy of the point into d_point must finish before the next overwrite
+            TSR_HIP(hipStreamSynchronize(s));
+        }
+        SumcheckOut out;
+        if ((st = run_batch(ctx, tr, prefix, 3, insts, gamma, lambda, out))) return st;
+        fp2 rho = tr.challenge_fp2((prefix + "/line").c_str(), nullptr);
+        const uint32_t n_max = (uint32_t)out.challenges.size();
+        log.put(n_max);
+        log.f2(gamma);
+        log.f2(lambda);
+        for (auto &v : out.rounds) log.f2(v);
+        for (auto &v : out.challenges) log.f2(v);
+        log.put(out.finals.size());
+        for (auto &v : out.finals) log.f2(v);
+        log.f2(rho);
+        for (int b = 0; b < 4; b++) {
+            uint64_t w = 0;
+            for (int j = 0; j < 8; j++) w |= (uint64_t)tr.state[8 * b + j] << (8 * j);
+            log.put(w);
+        }
+        log.put(tr.counter);
+        for (size_t i = 0; i < act.size(); i++) {
+            DevTree &t = trees[act[i]];
+            const uint32_t n = (uint32_t)t.point.size();
+            std::vector<fp2> np;
+            np.push_back(rho);
+            for (uint32_t k = 0; k < n; k++) np.push_back(out.challenges[n_max - n + k]);
+            t.point = np;
+            const fp2 *f = &out.finals[4 * i];
+            t.num = gl::add(f[0], gl::mul(rho, gl::sub(f[1], f[0])));
+            t.den = gl::add(f[2], gl::mul(rho, gl::sub(f[3], f[2])));
+            // the level below this one is no longer needed (trees.rs drop_level)
+            const uint32_t below = level - t.join + 1;
+            if (below < t.h) {
+                t.N[below].release();
+                t.D[below].release();
+            }
+        }
+    }
+
+    // values: the padded lookup value column at rho_a (prover.rs:301-318)
+    values.clear();
+    claims.clear();
+    DevBuf<fp2> d_part, d_val;
+    if ((st = d_part.alloc(ctx, 1024)) || (st = d_val.alloc(ctx, 1))) return st;
+    for (uint32_t k = 0; k < count; k++) {
+        DevTree &la = trees[2 * k], &ta = trees[2 * k + 1];
+        if (inst[k].values) {
+            const uint32_t n = la.h;
+            DevBuf<fp2> eq;
+            if ((st = eq.alloc(ctx, 1ull << n))) return st;
+            if (n) TSR_HIP(hipMemcpyAsync(d_point.ptr, la.point.data(), 16 * n, hipMemcpyHostToDevice, s));
+            hipLaunchKernelGGL(k_eq_table, dim3(nblocks(1ull << n, 256)), dim3(256), 0, s,
+                               d_point.ptr, n, eq.ptr);
+            uint64_t pad = 0;  // row 0's value fills the padding slots
+            TSR_HIP(hipMemcpyAsync(&pad, tvals[k], 8, hipMemcpyDeviceToHost, s));
+            TSR_HIP(hipStreamSynchronize(s));
+            const uint64_t len = 1ull << n;
+            const unsigned nb = std::min<unsigned>(1024, nblocks(len, 256));
+            hipLaunchKernelGGL(k_eq_dot, dim3(nb), dim3(256), 0, s, eq.ptr, lvals[k],
+                               inst[k].lookups, pad, len, d_part.ptr);
+            hipLaunchKernelGGL(k_sum_partials, dim3(1), dim3(64), 0, s, d_part.ptr, nb, d_val.ptr);
+            if ((st = check_launch("values"))) return st;
+            fp2 v{0, 0};
+            TSR_HIP(hipMemcpyAsync(&v, d_val.ptr, 16, hipMemcpyDeviceToHost, s));
+            TSR_HIP(hipStreamSynchronize(s));
+            values.push_back(v);
+        }
+        claims.push_back(la.num);
+        claims.push_back(la.den);
+        claims.push_back(ta.num);
+        claims.push_back(ta.den);
+    }
+    tr.absorb_fp2s("logup/values", (const uint64_t *)values.data(), values.size());
+    return Status::ok();
+}
+
+}  // namespace tsr
+
+using namespace tsr;
+
+extern "C" {
+
+tsr_status tsr_logup_prove(tsr_ctx *ctx_, tsr_transcript *t, const tsr_logup_instance *inst,
+                           uint32_t count, const uint32_t *schedule, uint64_t alpha_out[2],
+                           uint64_t beta_out[2], uint64_t *roots_out, uint64_t *values_out,
+                           uint64_t *n_values, uint64_t *claims_out, uint64_t *log,
+                           uint64_t log_cap, uint64_t *log_len) {
+    Ctx *ctx = (Ctx *)ctx_;
+    if (!ctx || !t || (count && !inst)) return Status::fail(TSR_BAD_ARG, "null argument").code;
+    // LogUpError order (instance.rs:140-166): per instance the columns, then the schedule.
+    for (uint32_t k = 0; k < count; k++) {
+        const tsr_logup_instance &in = inst[k];
+        if (in.rows == 0 || !in.keys)
+            return Status::fail(TSR_BAD_SHAPE, "NoRows: instance " + std::to_string(k)).code;
+        if (in.lookups == 0 || !in.lookup_keys)
+            return Status::fail(TSR_BAD_SHAPE, "NoLookups: instance " + std::to_string(k)).code;
+        if ((in.values == nullptr) != (in.lookup_values == nullptr))
+            return Status::fail(TSR_BAD_SHAPE, "ValuesMismatch: instance " + std::to_string(k)).code;
+        if (!in.multiplicities)
+            return Status::fail(TSR_BAD_SHAPE, "Multiplicities: instance " + std::to_string(k)).code;
+        if (ceil_log2(in.lookups) > 32 || ceil_log2(in.rows) > 32)
+            return Status::fail(TSR_DECLINED, "a tree above 2^32 leaves").code;
+    }
+    std::vector<uint32_t> sched(2 * count);
+    if (schedule) {
+        std::vector<int> seen(2 * count, 0);
+        for (uint32_t i = 0; i < 2 * count; i++) {
+            if (schedule[i] >= 2 * count || seen[schedule[i]]++)
+                return Status::fail(TSR_BAD_SHAPE, "ScheduleTree: position " + std::to_string(i)).code;
+            sched[i] = schedule[i];
+        }
+    } else {
+        for (uint32_t i = 0; i < 2 * count; i++) sched[i] = i;  // TreeId::paired
+    }
+    hipError_t e = hipSetDevice(ctx->device);
+    if (e != hipSuccess) return hip_fail(e, "hipSetDevice").code;
+    fp2 alpha, beta;
+    std::vector<fp2> roots, values, claims;
+    Log lg{log, log ? log_cap : 0};
+    Status st = prove_logup(ctx, t->t, inst, count, sched, alpha, beta, roots, values, claims, lg);
+    if (st) return st.code;
+    if (alpha_out) {
+        alpha_out[0] = alpha.c0;
+        alpha_out[1] = alpha.c1;
+    }
+    if (beta_out) {
+        beta_out[0] = beta.c0;
+        beta_out[1] = beta.c1;
+    }
+    if (roots_out) memcpy(roots_out, roots.data(), 16 * roots.size());
+    if (values_out) memcpy(values_out, values.data(), 16 * values.size());
+    if (n_values) *n_values = values.size();
+    if (claims_out) memcpy(claims_out, claims.data(), 16 * claims.size());
+    if (log_len) *log_len = lg.len;
+    if (log && lg.len > log_cap) return Status::fail(TSR_BAD_SHAPE, "log buffer too small").code;
+    return TSR_OK;
+}
+
+tsr_status tsr_logup_multiplicities(tsr_ctx *ctx_, uint64_t rows, const uint64_t *lookup_rows,
+                                    uint64_t lookups, uint64_t *out) {
+    Ctx *ctx = (Ctx *)ctx_;
+    if (!ctx || !out || (lookups && !lookup_rows))
+        return Status::fail(TSR_BAD_ARG, "null argument").code;
+    if (rows == 0) return Status::fail(TSR_BAD_SHAPE, "NoRows").code;
+    hipError_t e = hipSetDevice(ctx->device);
+    if (e != hipSuccess) return hip_fail(e, "hipSetDevice").code;
+    hipStream_t s = ctx->stream;
+    const uint64_t leaves = 1ull << ceil_log2(rows);
+    DevBuf<uint64_t> counts, in;
+    Status st;
+    if ((st = counts.alloc(ctx, leaves))) return st.code;
+    const uint64_t *d = lookup_rows;
+    if (lookups && !is_device_ptr(lookup_rows)) {
+        if ((st = in.alloc(ctx, lookups))) return st.code;
+        e = hipMemcpyAsync(in.ptr, lookup_rows, 8 * lookups, hipMemcpyHostToDevice, s);
+        if (e != hipSuccess) return hip_fail(e, "copy").code;
+        d = in.ptr;
+    }
+    DevBuf<uint32_t> bad;
+    if ((st = bad.alloc(ctx, 1))) return st.code;
+    hipMemsetAsync(counts.ptr, 0, 8 * leaves, s);
+    hipMemsetAsync(bad.ptr, 0, 4, s);
+    if (lookups)
+        hipLaunchKernelGGL(k_count_rows, dim3(nblocks(lookups, 256)), dim3(256), 0, s, d, lookups,
+                           rows, (unsigned long long *)counts.ptr, bad.ptr);
+    if ((st = check_launch("multiplicities"))) return st.code;
+    uint32_t hbad = 0;
+    e = hipMemcpyAsync(&hbad, bad.ptr, 4, hipMemcpyDeviceToHost, s);
+    if (e != hipSuccess) return hip_fail(e, "copy").code;
+    std::vector<uint64_t> h(leaves);
+    e = hipMemcpyAsync(h.data(), counts.ptr, 8 * leaves, hipMemcpyDeviceToHost, s);
+    if (e != hipSuccess) return hip_fail(e, "copy").code;
+    e = hipStreamSynchronize(s);
+    if (e != hipSuccess) return hip_fail(e, "sync").code;
+    if (hbad) return Status::fail(TSR_BAD_SHAPE, "LookupRow: a lookup names no row of the table").code;
+    if (lookups) h[0] += (1ull << ceil_log2(lookups)) - lookups;
+    if (is_device_ptr(out)) {
+        e = hipMemcpy(out, h.data(), 8 * leaves, hipMemcpyHostToDevice);
+        if (e != hipSuccess) return hip_fail(e, "copy").code;
+    } else {
+        memcpy(out, h.data(), 8 * leaves);
+    }
+    return TSR_OK;
+}
+
+}  // extern "C"

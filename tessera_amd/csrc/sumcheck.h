@@ -1,11 +1,15 @@
-// The batched product sumcheck (see sumcheck.hip).
+// Batched sumchecks (see sumcheck.hip): the product relation of prove_batch and the
+// fraction-tree level relation, run by one device-resident round loop.
 #pragma once
+#include <string>
 #include <vector>
 #include "common.h"
 #include "field.h"
 #include "transcript.h"
 
 namespace tsr {
+
+enum { REL_PRODUCT = 0, REL_FRACTION = 1 };
 
 struct SumcheckOut {
     gl::fp2 gamma;
@@ -14,7 +18,25 @@ struct SumcheckOut {
     std::vector<gl::fp2> finals;
 };
 
-Status prove_batch(Ctx *ctx, Transcript &tr, const char *prefix, uint32_t degree,
-                   const tsr_instance *instances, uint32_t count, SumcheckOut &out);
+// One table of a batch instance, on the device. Entry x of the original table is
+// src[(x << shift) + off]; src == nullptr is the constant 1 (no storage).
+struct GenTable {
+    const void *src = nullptr;
+    bool base = false;
+    bool is_public = false;
+    uint8_t shift = 0, off = 0;
+};
+
+struct GenInst {
+    uint32_t vars = 0;
+    int rel = REL_PRODUCT;  // REL_FRACTION: tables eq, N0, N1, D0, D1 (degree 3)
+    std::vector<GenTable> tabs;
+    bool claim_given = false;  // padding rounds use `claim` (a tree level's) instead of S_i
+    gl::fp2 claim{0, 0};
+};
+
+// The rounds and final evaluations of a batch whose gamma (and lambda) the caller squeezed.
+Status run_batch(Ctx *ctx, Transcript &tr, const std::string &prefix, uint32_t degree,
+                 std::vector<GenInst> &insts, gl::fp2 gamma, gl::fp2 lambda, SumcheckOut &out);
 
 }  // namespace tsr

@@ -1,18 +1,24 @@
-// The batched product sumcheck on gfx950, with the round loop and its transcript on the device.
+// Batched sumchecks on gfx950, with the round loop and its transcript on the device.
 //
 // Semantics: crates/tessera-sumcheck/src/prover.rs:1122-1410 (prove_batch / prove_checked,
-// State::{new, add_round, final_evals}, padding_scale): instance i on n_i variables, weight
-// gamma^i, front-padded to n_max; round j's message is p_j(0), p_j(2), ..., p_j(d) of the
-// batched polynomial, absorbed as TCE Vec<Fp2> under <prefix>/round/{j}; r_j is squeezed under
-// <prefix>/r/{j}; a padding round adds 2^{pad - j - 1} gamma^i S_i at every node; binding is
-// t[k] = t[2k] + r (t[2k+1] - t[2k]) (tessera-poly multilinear.rs:70-121).
+// State::{new, add_round, final_evals}, padding_scale) and level.rs:535-590 (prove_level):
+// instance i on n_i variables, weight gamma^i, front-padded to n_max; round j's message is
+// p_j(0), p_j(2), ..., p_j(d) of the batched polynomial, absorbed as TCE Vec<Fp2> under
+// <prefix>/round/{j}; r_j is squeezed under <prefix>/r/{j}; a padding round adds
+// 2^{pad - j - 1} gamma^i S_i at every node (S_i from the tables for prove_batch, the claim for a
+// tree level: equal for honest tables); binding is t[k] = t[2k] + r (t[2k+1] - t[2k])
+// (tessera-poly multilinear.rs:70-121).
 //
-// Execution: the host squeezes gamma, then enqueues the whole loop on the context's stream:
-//   first-round sums of every instance (k_sc_round, no bind), then for each round j a one-block
-//   k_sc_transcript (reduces the per-block partial sums, forms the message, absorbs it, squeezes
-//   r_j with the device transcript) followed by k_sc_round for every instance in its own rounds
-//   (binds all its tables with r_j and, fused in the same pass, sums the next round's node
-//   products). One synchronization at the end; the host then absorbs the final evaluations.
+// Two relations: a product of the instance's tables (prove_batch's Instance::new), and a
+// fraction-tree level eq (N0 D1 + N1 D0 + lambda D0 D1) over [eq, N0, N1, D0, D1].
+//
+// Execution: the caller squeezes gamma (and lambda) on the host transcript, then run_batch
+// enqueues the whole loop on the context's stream: the first-round sums of every instance
+// (k_sc_round, no bind), then for each round j a one-block k_sc_transcript (reduces the
+// per-block partial sums, forms the message, absorbs it, squeezes r_j with the device
+// transcript) followed by k_sc_round for every instance in its own rounds (binds all its tables
+// with r_j and, fused in the same pass, sums the next round's node products). One
+// synchronization at the end; the host then absorbs the final evaluations.
 #include <hip/hip_runtime.h>
 #include <algorithm>
 #include <memory>
@@ -26,7 +32,6 @@
 #include "sumcheck.h"
 #include "transcript.h"
 
-
 namespace tsr {
 
 using gl::fp2;
@@ -37,38 +42,60 @@ constexpr int SC_THREADS = 256;
 constexpr int SC_MAX_BLOCKS = 1024;
 
 struct ScTables {
-    const void *src[SC_MAXF];
+    const void *src[SC_MAXF];  // nullptr: the constant 1 (a fraction tree's unit numerators)
     fp2 *dst[SC_MAXF];
+    uint8_t shift[SC_MAXF], off[SC_MAXF];  // first round: entry x is src[(x << shift) + off]
 };
 
 __device__ __forceinline__ fp2 ld_entry(const void *p, bool base, uint64_t x) {
+    if (!p) return fp2{1, 0};
     if (base) return fp2{((const uint64_t *)p)[x], 0};
     return ((const fp2 *)p)[x];
 }
 
-// Node products of one pair: slots [p(0), p(2), ..., p(D), p(1)] (D+1 of them).
-template <int D>
-__device__ __forceinline__ void pair_products(const fp2 *a, const fp2 *b, uint32_t F,
-                                              fp2 prod[D + 1]) {
-    for (uint32_t k = 0; k < F; k++) {
-        fp2 diff = gl::sub(b[k], a[k]);
-        fp2 line = gl::add(b[k], diff);  // z = 2
-        if (k == 0) {
-            prod[0] = a[k];
-            prod[D] = b[k];
+// Node values of one pair, slots [p(0), p(2), ..., p(D), p(1)] (D+1 of them).
+template <int D, int REL>
+__device__ __forceinline__ void pair_values(const fp2 *a, const fp2 *b, uint32_t F,
+                                            const fp2 &lambda, fp2 out[D + 1]) {
+    if (REL == REL_PRODUCT) {
+        for (uint32_t k = 0; k < F; k++) {
+            fp2 diff = gl::sub(b[k], a[k]);
+            fp2 line = gl::add(b[k], diff);  // z = 2
+            if (k == 0) {
+                out[0] = a[k];
+                out[D] = b[k];
 #pragma unroll
-            for (int z = 2; z <= D; z++) {
-                prod[z - 1] = line;
-                line = gl::add(line, diff);
-            }
-        } else {
-            prod[0] = gl::mul(prod[0], a[k]);
-            prod[D] = gl::mul(prod[D], b[k]);
+                for (int z = 2; z <= D; z++) {
+                    out[z - 1] = line;
+                    line = gl::add(line, diff);
+                }
+            } else {
+                out[0] = gl::mul(out[0], a[k]);
+                out[D] = gl::mul(out[D], b[k]);
 #pragma unroll
-            for (int z = 2; z <= D; z++) {
-                prod[z - 1] = gl::mul(prod[z - 1], line);
-                line = gl::add(line, diff);
+                for (int z = 2; z <= D; z++) {
+                    out[z - 1] = gl::mul(out[z - 1], line);
+                    line = gl::add(line, diff);
+                }
             }
+        }
+    } else {
+        // tables eq, N0, N1, D0, D1 at the nodes z = 0, 1, 2, ..., D
+#pragma unroll
+        for (int s = 0; s <= D; s++) {
+            const int z = s == D ? 1 : (s == 0 ? 0 : s + 1);
+            fp2 v[5];
+#pragma unroll
+            for (int k = 0; k < 5; k++) {
+                fp2 diff = gl::sub(b[k], a[k]);
+                fp2 at = a[k];
+                for (int m = 0; m < z; m++) at = gl::add(at, diff);
+                v[k] = at;
+            }
+            fp2 d01 = gl::mul(v[3], v[4]);
+            fp2 inner = gl::add(gl::add(gl::mul(v[1], v[4]), gl::mul(v[2], v[3])),
+                                gl::mul(lambda, d01));
+            out[s] = gl::mul(v[0], inner);
         }
     }
 }
@@ -81,14 +108,15 @@ __device__ __forceinline__ uint64_t shfl_xor_u64(uint64_t v, int mask) {
 }
 
 // One own round of one instance: bind every table with r (when out_len > 0) and/or sum the node
-// products over `pairs` pairs (when pairs > 0), writing one partial per block.
-//   sums only:   pairs of the unbound tables src (entries 2q, 2q+1)
+// values over `pairs` pairs (when pairs > 0), writing one partial per block.
+//   sums only:   pairs of the unbound tables (entries 2q, 2q+1)
 //   bind + sums: dst[2q], dst[2q+1] from src[4q..4q+3]; the pair (dst[2q], dst[2q+1]) summed
 //   bind only:   dst[q] from src[2q], src[2q+1] for q < out_len
-template <int D>
+// `first`: the sources are the instance's original tables (base entries, strides, offsets).
+template <int D, int REL>
 __global__ void __launch_bounds__(SC_THREADS)
-k_sc_round(ScTables t, uint32_t F, uint32_t base_mask, uint64_t out_len, uint64_t pairs,
-           const fp2 *__restrict__ r_ptr, fp2 *__restrict__ partials) {
+k_sc_round(ScTables t, uint32_t F, uint32_t base_mask, bool first, uint64_t out_len,
+           uint64_t pairs, const fp2 *__restrict__ r_ptr, fp2 lambda, fp2 *__restrict__ partials) {
     const bool bind = out_len > 0;
     const fp2 r = bind ? *r_ptr : fp2{0, 0};
     const uint64_t items = pairs > 0 ? pairs : out_len;
@@ -100,27 +128,30 @@ k_sc_round(ScTables t, uint32_t F, uint32_t base_mask, uint64_t out_len, uint64_
     for (uint64_t q = (uint64_t)blockIdx.x * SC_THREADS + threadIdx.x; q < items; q += stride) {
         fp2 a[SC_MAXF], b[SC_MAXF];
         for (uint32_t k = 0; k < F; k++) {
-            const bool base = (base_mask >> k) & 1;
+            const bool base = first && ((base_mask >> k) & 1);
+            const uint32_t sh = first ? t.shift[k] : 0, of = first ? t.off[k] : 0;
+            auto at = [&](uint64_t x) { return ld_entry(t.src[k], base, (x << sh) + of); };
             if (!bind) {
-                a[k] = ld_entry(t.src[k], base, 2 * q);
-                b[k] = ld_entry(t.src[k], base, 2 * q + 1);
+                a[k] = at(2 * q);
+                b[k] = at(2 * q + 1);
             } else if (pairs > 0) {
-                fp2 e0 = ld_entry(t.src[k], base, 4 * q), e1 = ld_entry(t.src[k], base, 4 * q + 1);
-                fp2 e2 = ld_entry(t.src[k], base, 4 * q + 2), e3 = ld_entry(t.src[k], base, 4 * q + 3);
+                fp2 e0 = at(4 * q), e1 = at(4 * q + 1), e2 = at(4 * q + 2), e3 = at(4 * q + 3);
                 a[k] = gl::add(e0, gl::mul(r, gl::sub(e1, e0)));
                 b[k] = gl::add(e2, gl::mul(r, gl::sub(e3, e2)));
-                t.dst[k][2 * q] = a[k];
-                t.dst[k][2 * q + 1] = b[k];
+                if (t.dst[k]) {
+                    t.dst[k][2 * q] = a[k];
+                    t.dst[k][2 * q + 1] = b[k];
+                }
             } else {
-                fp2 e0 = ld_entry(t.src[k], base, 2 * q), e1 = ld_entry(t.src[k], base, 2 * q + 1);
-                t.dst[k][q] = gl::add(e0, gl::mul(r, gl::sub(e1, e0)));
+                fp2 e0 = at(2 * q), e1 = at(2 * q + 1);
+                if (t.dst[k]) t.dst[k][q] = gl::add(e0, gl::mul(r, gl::sub(e1, e0)));
             }
         }
         if (pairs > 0) {
-            fp2 prod[D + 1];
-            pair_products<D>(a, b, F, prod);
+            fp2 val[D + 1];
+            pair_values<D, REL>(a, b, F, lambda, val);
 #pragma unroll
-            for (int z = 0; z <= D; z++) acc[z] = gl::add(acc[z], prod[z]);
+            for (int z = 0; z <= D; z++) acc[z] = gl::add(acc[z], val[z]);
         }
     }
     if (pairs == 0) return;
@@ -148,8 +179,10 @@ k_sc_round(ScTables t, uint32_t F, uint32_t base_mask, uint64_t out_len, uint64_
 // Per-instance facts the transcript kernel needs.
 struct ScInst {
     uint32_t vars, pad;
+    uint32_t wsum_given;  // 1: wsum set by the host (claims, or instances on no variable)
+    uint32_t pad_;
     fp2 weight;    // gamma^i
-    fp2 wsum;      // gamma^i S_i (set by the host for vars = 0, by round 0 otherwise)
+    fp2 wsum;      // gamma^i S_i
     fp2 *partials; // (D+1) per block of its last k_sc_round
 };
 
@@ -170,10 +203,8 @@ k_sc_transcript(uint32_t j, ScInst *__restrict__ inst, uint32_t count,
     for (uint32_t x = threadIdx.x; x < count * (D + 1); x += SC_THREADS) {
         uint32_t i = x / (D + 1), z = x % (D + 1);
         fp2 s{0, 0};
-        if (j >= inst[i].pad || (j == 0 && inst[i].vars > 0)) {
-            const fp2 *p = inst[i].partials;
-            for (uint32_t b = 0; b < nblk[i]; b++) s = gl::add(s, p[(uint64_t)b * (D + 1) + z]);
-        }
+        const fp2 *p = inst[i].partials;
+        for (uint32_t b = 0; b < nblk[i]; b++) s = gl::add(s, p[(uint64_t)b * (D + 1) + z]);
         sums[x] = s;
     }
     __syncthreads();
@@ -183,7 +214,7 @@ k_sc_transcript(uint32_t j, ScInst *__restrict__ inst, uint32_t count,
     for (uint32_t i = 0; i < count; i++) {
         ScInst &in = inst[i];
         const fp2 *s = sums + (uint64_t)i * (D + 1);
-        if (j == 0 && in.pad > 0 && in.vars > 0)
+        if (j == 0 && in.pad > 0 && !in.wsum_given)
             in.wsum = gl::mul(in.weight, gl::add(s[0], s[D]));  // S_i = p(0) + p(1)
         if (j < in.pad) {
             uint64_t scale = 1ull << (in.pad - j - 1);
@@ -202,29 +233,37 @@ k_sc_transcript(uint32_t j, ScInst *__restrict__ inst, uint32_t count,
     chals_out[j] = r;
 }
 
-// Gathers each final evaluation: entry 0 of the buffer at ptrs[e] (base or Fp2).
+// Gathers each final evaluation: entry off[e] of the buffer at ptrs[e] (base or Fp2).
 __global__ void k_sc_gather(const void *const *__restrict__ ptrs, const uint32_t *__restrict__ base,
-                            uint32_t n, fp2 *__restrict__ out) {
+                            const uint32_t *__restrict__ off, uint32_t n, fp2 *__restrict__ out) {
     uint32_t e = blockIdx.x * blockDim.x + threadIdx.x;
-    if (e < n) out[e] = ld_entry(ptrs[e], base[e] != 0, 0);
+    if (e < n) out[e] = ld_entry(ptrs[e], base[e] != 0, off[e]);
 }
 
 // ---------------------------------------------------------------------------------------------
 
-template <int D>
+template <int D, int REL>
 static void launch_round(dim3 grid, hipStream_t s, const ScTables &t, uint32_t F,
-                         uint32_t base_mask, uint64_t out_len, uint64_t pairs, const fp2 *r,
-                         fp2 *partials) {
-    hipLaunchKernelGGL(k_sc_round<D>, grid, dim3(SC_THREADS), 0, s, t, F, base_mask, out_len,
-                       pairs, r, partials);
+                         uint32_t base_mask, bool first, uint64_t out_len, uint64_t pairs,
+                         const fp2 *r, fp2 lambda, fp2 *partials) {
+    hipLaunchKernelGGL((k_sc_round<D, REL>), grid, dim3(SC_THREADS), 0, s, t, F, base_mask,
+                       first, out_len, pairs, r, lambda, partials);
 }
 
-static void launch_round_d(uint32_t d, dim3 grid, hipStream_t s, const ScTables &t, uint32_t F,
-                           uint32_t base_mask, uint64_t out_len, uint64_t pairs, const fp2 *r,
-                           fp2 *partials) {
+static void launch_round_d(uint32_t d, int rel, dim3 grid, hipStream_t s, const ScTables &t,
+                           uint32_t F, uint32_t base_mask, bool first, uint64_t out_len,
+                           uint64_t pairs, const fp2 *r, fp2 lambda, fp2 *partials) {
+    if (rel == REL_FRACTION) {
+        launch_round<3, REL_FRACTION>(grid, s, t, F, base_mask, first, out_len, pairs, r, lambda,
+                                      partials);
+        return;
+    }
     switch (d) {
-#define CASE(D) \
-    case D: launch_round<D>(grid, s, t, F, base_mask, out_len, pairs, r, partials); break;
+#define CASE(D)                                                                           \
+    case D:                                                                               \
+        launch_round<D, REL_PRODUCT>(grid, s, t, F, base_mask, first, out_len, pairs, r,  \
+                                     lambda, partials);                                   \
+        break;
         CASE(1) CASE(2) CASE(3) CASE(4) CASE(5) CASE(6) CASE(7) CASE(8) CASE(9) CASE(10)
 #undef CASE
     }
@@ -270,90 +309,65 @@ void store_state_words(const uint32_t w[8], uint8_t st[32]) {
         for (int j = 0; j < 4; j++) st[4 * i + j] = (uint8_t)(w[i] >> (8 * j));
 }
 
-Status prove_batch(Ctx *ctx, Transcript &tr, const char *prefix, uint32_t degree,
-                   const tsr_instance *instances, uint32_t count, SumcheckOut &out) {
+Status run_batch(Ctx *ctx, Transcript &tr, const std::string &p, uint32_t D,
+                 std::vector<GenInst> &insts, fp2 gamma, fp2 lambda, SumcheckOut &out) {
     hipStream_t s = ctx->stream;
-    const uint32_t D = degree;
-    std::vector<uint32_t> vars(count), pads(count);
+    const uint32_t count = (uint32_t)insts.size();
     uint32_t n_max = 0;
-    for (uint32_t i = 0; i < count; i++) {
-        uint64_t len = 0;
-        // lengths were checked by the caller: equal powers of two within an instance
-        (void)len;
-        vars[i] = instances[i].vars;
-        n_max = std::max(n_max, vars[i]);
-    }
-    for (uint32_t i = 0; i < count; i++) pads[i] = n_max - vars[i];
-
-    // gamma and the weights, on the host transcript (prover.rs:1147-1150)
-    std::string p(prefix);
-    gl::fp2 gamma = tr.challenge_fp2((p + "/gamma").c_str(), nullptr);
+    for (auto &in : insts) n_max = std::max(n_max, in.vars);
     out.gamma = gamma;
+
     std::vector<ScInst> hinst(count);
-    gl::fp2 w{1, 0};
+    fp2 w{1, 0};
     for (uint32_t i = 0; i < count; i++) {
-        hinst[i].vars = vars[i];
-        hinst[i].pad = pads[i];
+        hinst[i].vars = insts[i].vars;
+        hinst[i].pad = n_max - insts[i].vars;
         hinst[i].weight = w;
-        hinst[i].wsum = fp2{0, 0};
+        hinst[i].wsum_given = insts[i].claim_given ? 1 : 0;
+        hinst[i].wsum = insts[i].claim_given ? gl::mul(w, insts[i].claim) : fp2{0, 0};
         w = gl::mul(w, gamma);
     }
 
-    // device copies of host tables; working buffers
-    struct Tab {
-        const void *src = nullptr;  // device
-        bool base = false;
-        bool is_public = false;
-        DevBuf<uint64_t> upload;
-        DevBuf<fp2> a, b;
-    };
-    std::vector<std::vector<std::unique_ptr<Tab>>> tabs(count);
+    // working buffers: per table a (2^{n-1}) and b (2^{n-2}) Fp2
+    std::vector<std::vector<DevBuf<fp2>>> bufa(count), bufb(count);
     std::vector<DevBuf<fp2>> partials(count);
+    Status st;
     for (uint32_t i = 0; i < count; i++) {
-        const uint64_t len = 1ull << vars[i];
-        for (uint32_t k = 0; k < instances[i].factors; k++) {
-            const tsr_table &tt = instances[i].tables[k];
-            std::unique_ptr<Tab> tab(new Tab());
-            tab->base = !tt.ext;
-            tab->is_public = tt.is_public != 0;
-            const uint64_t words = len * (tt.ext ? 2 : 1);
-            if (is_device_ptr(tt.data)) {
-                tab->src = tt.data;
-            } else {
-                Status st = tab->upload.alloc(ctx, words);
-                if (st) return st;
-                TSR_HIP(hipMemcpyAsync(tab->upload.ptr, tt.data, 8 * words,
-                                       hipMemcpyHostToDevice, s));
-                tab->src = tab->upload.ptr;
-            }
-            if (vars[i] >= 1) {
-                Status st = tab->a.alloc(ctx, len / 2);
-                if (st) return st;
-                if (vars[i] >= 2 && (st = tab->b.alloc(ctx, len / 4))) return st;
-            }
-            tabs[i].push_back(std::move(tab));
+        const uint32_t v = insts[i].vars;
+        const uint64_t len = 1ull << v;
+        bufa[i] = std::vector<DevBuf<fp2>>(insts[i].tabs.size());
+        bufb[i] = std::vector<DevBuf<fp2>>(insts[i].tabs.size());
+        for (size_t k = 0; k < insts[i].tabs.size(); k++) {
+            if (!insts[i].tabs[k].src || v == 0) continue;
+            if ((st = bufa[i][k].alloc(ctx, len / 2))) return st;
+            if (v >= 2 && (st = bufb[i][k].alloc(ctx, len / 4))) return st;
         }
-        Status st = partials[i].alloc(ctx, (uint64_t)SC_MAX_BLOCKS * (D + 1));
-        if (st) return st;
+        if ((st = partials[i].alloc(ctx, (uint64_t)SC_MAX_BLOCKS * (D + 1)))) return st;
         hinst[i].partials = partials[i].ptr;
     }
 
-    // S_i of instances on no variable: the product of their single entries (host side)
+    // S_i of instances on no variable (no claim given): the relation at the single entries
     for (uint32_t i = 0; i < count; i++) {
-        if (vars[i] != 0 || pads[i] == 0) continue;
+        if (insts[i].vars != 0 || hinst[i].pad == 0 || insts[i].claim_given) continue;
         fp2 prod{1, 0};
-        for (auto &tab : tabs[i]) {
-            fp2 v{0, 0};
-            TSR_HIP(hipMemcpyAsync(&v, tab->src, tab->base ? 8 : 16, hipMemcpyDeviceToHost, s));
-            TSR_HIP(hipStreamSynchronize(s));
+        for (auto &tab : insts[i].tabs) {
+            fp2 v{1, 0};
+            if (tab.src) {
+                v = fp2{0, 0};
+                TSR_HIP(hipMemcpyAsync(&v, (const uint8_t *)tab.src + (tab.base ? 8 : 16) * tab.off,
+                                       tab.base ? 8 : 16, hipMemcpyDeviceToHost, s));
+                TSR_HIP(hipStreamSynchronize(s));
+            }
             prod = gl::mul(prod, v);
         }
         hinst[i].wsum = gl::mul(hinst[i].weight, prod);
+        hinst[i].wsum_given = 1;
     }
 
     // labels of every round, the control block, per-round block counts
-    std::vector<uint8_t> labels((size_t)2 * std::max(n_max, 1u) * LABEL_SLOT, 0);
-    std::vector<uint32_t> lens(2 * std::max(n_max, 1u), 0);
+    const uint32_t nr = std::max(n_max, 1u);
+    std::vector<uint8_t> labels((size_t)2 * nr * LABEL_SLOT, 0);
+    std::vector<uint32_t> lens(2 * nr, 0);
     for (uint32_t j = 0; j < n_max; j++) {
         std::string a = p + "/round/" + std::to_string(j), b = p + "/r/" + std::to_string(j);
         if (a.size() > LABEL_SLOT || b.size() > LABEL_SLOT)
@@ -363,16 +377,16 @@ Status prove_batch(Ctx *ctx, Transcript &tr, const char *prefix, uint32_t degree
         lens[2 * j] = (uint32_t)a.size();
         lens[2 * j + 1] = (uint32_t)b.size();
     }
-    // nblk[j][i]: blocks of instance i's partials that round j's message reads
-    std::vector<uint32_t> nblk((size_t)std::max(n_max, 1u) * count, 0);
+    // nblk[j][i]: blocks of instance i's partials that round j's message reads (round 0 also
+    // reads a padded instance's first sums for S_i)
+    std::vector<uint32_t> nblk((size_t)nr * count, 0);
     for (uint32_t i = 0; i < count; i++) {
+        const uint32_t v = insts[i].vars, pad = hinst[i].pad;
+        if (v == 0) continue;
         for (uint32_t j = 0; j < n_max; j++) {
-            if (vars[i] == 0) continue;
-            if (j < pads[i] && j != 0) continue;
-            // own round t = j - pad (or the first sums at j = 0 for a padded instance)
-            uint32_t t = j < pads[i] ? 0 : j - pads[i];
-            uint64_t pairs = 1ull << (vars[i] - t - 1);
-            nblk[(size_t)j * count + i] = blocks_for_pairs(pairs);
+            if (j < pad && j != 0) continue;
+            uint32_t t = j < pad ? 0 : j - pad;
+            nblk[(size_t)j * count + i] = blocks_for_pairs(1ull << (v - t - 1));
         }
     }
     ScCtl hctl;
@@ -385,11 +399,10 @@ Status prove_batch(Ctx *ctx, Transcript &tr, const char *prefix, uint32_t degree
     DevBuf<ScInst> d_inst;
     DevBuf<ScCtl> d_ctl;
     DevBuf<fp2> d_rounds, d_chals;
-    Status st;
     if ((st = d_labels.alloc(ctx, labels.size())) || (st = d_lens.alloc(ctx, lens.size())) ||
         (st = d_nblk.alloc(ctx, nblk.size())) || (st = d_inst.alloc(ctx, count)) ||
-        (st = d_ctl.alloc(ctx, 1)) || (st = d_rounds.alloc(ctx, (uint64_t)std::max(n_max, 1u) * D)) ||
-        (st = d_chals.alloc(ctx, std::max(n_max, 1u))))
+        (st = d_ctl.alloc(ctx, 1)) || (st = d_rounds.alloc(ctx, (uint64_t)nr * D)) ||
+        (st = d_chals.alloc(ctx, nr)))
         return st;
     TSR_HIP(hipMemcpyAsync(d_labels.ptr, labels.data(), labels.size(), hipMemcpyHostToDevice, s));
     TSR_HIP(hipMemcpyAsync(d_lens.ptr, lens.data(), 4 * lens.size(), hipMemcpyHostToDevice, s));
@@ -398,32 +411,39 @@ Status prove_batch(Ctx *ctx, Transcript &tr, const char *prefix, uint32_t degree
     TSR_HIP(hipMemcpyAsync(d_ctl.ptr, &hctl, sizeof(ScCtl), hipMemcpyHostToDevice, s));
     const DevKeys keys = dev_keys(ctx);
 
-    auto tables_of = [&](uint32_t i, uint32_t t, ScTables &st_, uint32_t &base_mask) {
-        // own round t binds from (t == 0: original, odd t: a, even t: b) into (even t: a, odd: b)
+    // own round t binds from (t == 0: original, odd t: a, even t: b) into (even t: a, odd: b)
+    auto tables_of = [&](uint32_t i, uint32_t t, ScTables &tt, uint32_t &base_mask) {
         base_mask = 0;
-        for (uint32_t k = 0; k < tabs[i].size(); k++) {
-            Tab &tab = *tabs[i][k];
+        for (uint32_t k = 0; k < insts[i].tabs.size(); k++) {
+            const GenTable &tab = insts[i].tabs[k];
+            tt.shift[k] = tab.shift;
+            tt.off[k] = tab.off;
+            if (!tab.src) {
+                tt.src[k] = nullptr;
+                tt.dst[k] = nullptr;
+                continue;
+            }
             if (t == 0) {
-                st_.src[k] = tab.src;
+                tt.src[k] = tab.src;
                 if (tab.base) base_mask |= 1u << k;
             } else {
-                st_.src[k] = (t % 2 == 1) ? (const void *)tab.a.ptr : (const void *)tab.b.ptr;
+                tt.src[k] = (t % 2 == 1) ? (const void *)bufa[i][k].ptr : (const void *)bufb[i][k].ptr;
             }
-            st_.dst[k] = (t % 2 == 0) ? tab.a.ptr : tab.b.ptr;
+            tt.dst[k] = (t % 2 == 0) ? bufa[i][k].ptr : bufb[i][k].ptr;
         }
     };
 
     {
         KernelTimer timer(ctx, "k_sc_round", s);
-        // first own round's sums of every instance on at least one variable
         for (uint32_t i = 0; i < count; i++) {
-            if (vars[i] == 0) continue;
+            if (insts[i].vars == 0) continue;
             ScTables t{};
             uint32_t bm;
             tables_of(i, 0, t, bm);
-            uint64_t pairs = 1ull << (vars[i] - 1);
-            launch_round_d(D, dim3(blocks_for_pairs(pairs)), s, t, (uint32_t)tabs[i].size(), bm,
-                           0, pairs, nullptr, partials[i].ptr);
+            uint64_t pairs = 1ull << (insts[i].vars - 1);
+            launch_round_d(D, insts[i].rel, dim3(blocks_for_pairs(pairs)), s, t,
+                           (uint32_t)insts[i].tabs.size(), bm, true, 0, pairs, nullptr, lambda,
+                           partials[i].ptr);
         }
     }
     for (uint32_t j = 0; j < n_max; j++) {
@@ -435,50 +455,58 @@ Status prove_batch(Ctx *ctx, Transcript &tr, const char *prefix, uint32_t degree
         }
         KernelTimer timer(ctx, "k_sc_round", s);
         for (uint32_t i = 0; i < count; i++) {
-            if (j < pads[i]) continue;
-            uint32_t t = j - pads[i];
+            if (j < hinst[i].pad) continue;
+            uint32_t t = j - hinst[i].pad;
             ScTables tt{};
             uint32_t bm;
             tables_of(i, t, tt, bm);
-            uint64_t out_len = 1ull << (vars[i] - t - 1);
+            uint64_t out_len = 1ull << (insts[i].vars - t - 1);
             uint64_t pairs = out_len / 2;  // next own round's pairs (0 after the last)
-            launch_round_d(D, dim3(blocks_for_pairs(pairs ? pairs : out_len)), s, tt,
-                           (uint32_t)tabs[i].size(), bm, out_len, pairs, &d_ctl.ptr->r,
-                           partials[i].ptr);
+            launch_round_d(D, insts[i].rel, dim3(blocks_for_pairs(pairs ? pairs : out_len)), s,
+                           tt, (uint32_t)insts[i].tabs.size(), bm, t == 0, out_len, pairs,
+                           &d_ctl.ptr->r, lambda, partials[i].ptr);
         }
     }
     if ((st = check_launch("sumcheck rounds"))) return st;
 
-    // final evaluations: each non-public table's single remaining entry
+    // final evaluations: each non-public table's single remaining entry (1 for a constant table)
     std::vector<const void *> fptr;
-    std::vector<uint32_t> fbase;
+    std::vector<uint32_t> fbase, foff;
     for (uint32_t i = 0; i < count; i++) {
-        for (auto &tab : tabs[i]) {
-            if (tab->is_public) continue;
-            if (vars[i] == 0) {
-                fptr.push_back(tab->src);
-                fbase.push_back(tab->base ? 1 : 0);
+        for (size_t k = 0; k < insts[i].tabs.size(); k++) {
+            const GenTable &tab = insts[i].tabs[k];
+            if (tab.is_public) continue;
+            if (!tab.src) {
+                fptr.push_back(nullptr);
+                fbase.push_back(0);
+                foff.push_back(0);
+            } else if (insts[i].vars == 0) {
+                fptr.push_back(tab.src);
+                fbase.push_back(tab.base ? 1 : 0);
+                foff.push_back(tab.off);
             } else {
                 // the last own round t = vars-1 wrote into a (t even) or b (t odd)
-                fptr.push_back(((vars[i] - 1) % 2 == 0) ? (const void *)tab->a.ptr
-                                                         : (const void *)tab->b.ptr);
+                fptr.push_back(((insts[i].vars - 1) % 2 == 0) ? (const void *)bufa[i][k].ptr
+                                                               : (const void *)bufb[i][k].ptr);
                 fbase.push_back(0);
+                foff.push_back(0);
             }
         }
     }
     out.finals.resize(fptr.size());
+    DevBuf<uint64_t> d_ptrs;
+    DevBuf<uint32_t> d_base, d_off;
+    DevBuf<fp2> d_fin;
     if (!fptr.empty()) {
-        DevBuf<uint64_t> d_ptrs;
-        DevBuf<uint32_t> d_base;
-        DevBuf<fp2> d_fin;
         if ((st = d_ptrs.alloc(ctx, fptr.size())) || (st = d_base.alloc(ctx, fbase.size())) ||
-            (st = d_fin.alloc(ctx, fptr.size())))
+            (st = d_off.alloc(ctx, foff.size())) || (st = d_fin.alloc(ctx, fptr.size())))
             return st;
         TSR_HIP(hipMemcpyAsync(d_ptrs.ptr, fptr.data(), 8 * fptr.size(), hipMemcpyHostToDevice, s));
         TSR_HIP(hipMemcpyAsync(d_base.ptr, fbase.data(), 4 * fbase.size(), hipMemcpyHostToDevice, s));
+        TSR_HIP(hipMemcpyAsync(d_off.ptr, foff.data(), 4 * foff.size(), hipMemcpyHostToDevice, s));
         hipLaunchKernelGGL(k_sc_gather, dim3((unsigned)((fptr.size() + 63) / 64)), dim3(64), 0, s,
-                           (const void *const *)d_ptrs.ptr, d_base.ptr, (uint32_t)fptr.size(),
-                           d_fin.ptr);
+                           (const void *const *)d_ptrs.ptr, d_base.ptr, d_off.ptr,
+                           (uint32_t)fptr.size(), d_fin.ptr);
         if ((st = check_launch("sumcheck finals"))) return st;
         TSR_HIP(hipMemcpyAsync(out.finals.data(), d_fin.ptr, 16 * fptr.size(),
                                hipMemcpyDeviceToHost, s));
@@ -497,6 +525,41 @@ Status prove_batch(Ctx *ctx, Transcript &tr, const char *prefix, uint32_t degree
     tr.counter = hctl.t.counter;
     tr.absorb_fp2s((p + "/evals").c_str(), (const uint64_t *)out.finals.data(), out.finals.size());
     return Status::ok();
+}
+
+Status prove_batch(Ctx *ctx, Transcript &tr, const char *prefix, uint32_t degree,
+                   const tsr_instance *instances, uint32_t count, SumcheckOut &out,
+                   std::vector<DevBuf<uint64_t>> &uploads) {
+    hipStream_t s = ctx->stream;
+    std::string p(prefix);
+    // gamma on the host transcript (prover.rs:1147-1150)
+    fp2 gamma = tr.challenge_fp2((p + "/gamma").c_str(), nullptr);
+    std::vector<GenInst> insts(count);
+    uploads.reserve(64);
+    for (uint32_t i = 0; i < count; i++) {
+        insts[i].vars = instances[i].vars;
+        insts[i].rel = REL_PRODUCT;
+        const uint64_t len = 1ull << instances[i].vars;
+        for (uint32_t k = 0; k < instances[i].factors; k++) {
+            const tsr_table &tt = instances[i].tables[k];
+            GenTable tab;
+            tab.base = !tt.ext;
+            tab.is_public = tt.is_public != 0;
+            const uint64_t words = len * (tt.ext ? 2 : 1);
+            if (is_device_ptr(tt.data)) {
+                tab.src = tt.data;
+            } else {
+                uploads.emplace_back();
+                Status st = uploads.back().alloc(ctx, words);
+                if (st) return st;
+                TSR_HIP(hipMemcpyAsync(uploads.back().ptr, tt.data, 8 * words,
+                                       hipMemcpyHostToDevice, s));
+                tab.src = uploads.back().ptr;
+            }
+            insts[i].tabs.push_back(tab);
+        }
+    }
+    return run_batch(ctx, tr, p, degree, insts, gamma, fp2{0, 0}, out);
 }
 
 }  // namespace tsr
@@ -540,7 +603,8 @@ extern "C" tsr_status tsr_sumcheck_prove(tsr_ctx *ctx_, tsr_transcript *t, const
     hipError_t e = hipSetDevice(ctx->device);
     if (e != hipSuccess) return hip_fail(e, "hipSetDevice").code;
     SumcheckOut out;
-    Status st = prove_batch(ctx, t->t, prefix, degree, instances, count, out);
+    std::vector<DevBuf<uint64_t>> uploads;
+    Status st = prove_batch(ctx, t->t, prefix, degree, instances, count, out, uploads);
     if (st) return st.code;
     if (gamma_out) {
         gamma_out[0] = out.gamma.c0;

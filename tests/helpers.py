@@ -69,3 +69,32 @@ def context_tce(protocol="tessera-protocol/1", mode="NonInteractive", params="te
 def context_from_json(c):
     return context_tce(c["protocol"], c["mode"], c["params"], bytes.fromhex(c["statement_digest"]),
                        bytes.fromhex(c["session_id"]))
+
+
+def splitmix_raw(count, seed):
+    """The raw SplitMix64 stream (no rejection)."""
+    gamma = np.uint64(0x9E3779B97F4A7C15)
+    with np.errstate(over="ignore"):
+        z = np.uint64(seed) + np.arange(1, count + 1, dtype=np.uint64) * gamma
+        z = (z ^ (z >> np.uint64(30))) * np.uint64(0xBF58476D1CE4E5B9)
+        z = (z ^ (z >> np.uint64(27))) * np.uint64(0x94D049BB133111EB)
+        return z ^ (z >> np.uint64(31))
+
+
+def zipf_rows(count, rows, s, seed):
+    """Row indices drawn Zipf(s) over 0..rows-1 (rank k+1 has weight (k+1)^-s) by inverse CDF on
+    u = (splitmix >> 11) * 2^-53."""
+    w = np.arange(1, rows + 1, dtype=np.float64) ** (-s)
+    cdf = np.cumsum(w)
+    cdf /= cdf[-1]
+    u = (splitmix_raw(count, seed) >> np.uint64(11)).astype(np.float64) * 2.0 ** -53
+    return np.minimum(np.searchsorted(cdf, u, side="right"), rows - 1).astype(np.uint64)
+
+
+def logup_config3(lookups=1 << 22, rows=1 << 16, seed=0x1064):
+    """The LogUp workload of the sumcheck/lookup config: table keys 0..rows-1 with SplitMix64
+    values, lookups Zipf(1.1) over the rows, exact multiplicities."""
+    keys = np.arange(rows, dtype=np.uint64)
+    values = splitmix_uniform(rows, seed)
+    idx = zipf_rows(lookups, rows, 1.1, seed + 1)
+    return keys, values, idx

@@ -1,0 +1,118 @@
+"""Parity of the gfx950 LogUp (device-built fraction trees, level sumchecks with the device
+transcript) with the reference's logup batches and with the oracle: alpha, beta, roots, every
+level's messages, challenges, evaluations, rho and transcript state, values, leaf claims; the
+multiplicity counts; and the 2^22-lookup / 2^16-row Zipf(1.1) workload whole against the oracle."""
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+from tests.helpers import P, context_from_json, fp2, h, load, logup_config3
+from tests.test_oracle_logup import honest_batches, logup_inputs
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def dev():
+    import tessera_amd as T
+    d = T.Device(0)
+    yield d
+    d.close()
+
+
+def to_gpu_instances(insts):
+    from tessera_amd.logup import LogUpInstance
+    return [LogUpInstance(i["keys"], i["values"], i["lookup_keys"], i["lookup_values"],
+                          i["multiplicities"]) for i in insts]
+
+
+def as_tuples(a):
+    return [tuple(int(x) for x in v) for v in np.asarray(a).reshape(-1, 2)]
+
+
+def test_reference_logup_batches(dev):
+    from tessera_amd.logup import prove_logup
+    from tessera_amd.transcript import Transcript
+    checked = 0
+    for b in honest_batches():
+        insts, sched = logup_inputs(b)
+        # the oracle decides which batches an honest prover reproduces (forged roots aside)
+        ot = O.OracleTranscript(context_from_json(b["context"]))
+        want = O.prove_logup(ot, insts, sched)
+        if as_tuples(want["roots"]) != [fp2(x) for x in b["roots"]]:
+            continue
+        t = Transcript(context_from_json(b["context"]))
+        got = prove_logup(dev, t, to_gpu_instances(insts), sched)
+        assert got.alpha == fp2(b["alpha"]) and got.beta == fp2(b["beta"])
+        assert as_tuples(got.roots) == [fp2(x) for x in b["roots"]]
+        assert len(got.levels) == len(b["levels"])
+        for g, w in zip(got.levels, b["levels"]):
+            assert g["gamma"] == fp2(w["gamma"]) and g["lambda"] == fp2(w["lambda"])
+            assert g["rounds"] == [[fp2(v) for v in m] for m in w["rounds"]], (b["name"], w["prefix"])
+            assert g["challenges"] == [fp2(c) for c in w["challenges"]]
+            assert g["final_evals"] == [fp2(f) for f in w["final_evals"]]
+            assert g["rho"] == fp2(w["rho"]) and g["state"] == w["state"]
+        assert as_tuples(got.values) == [fp2(v) for v in b["values"]]
+        assert t.state.hex() == b["state"] and t.counter == b["counter"]
+        checked += 1
+    assert checked >= 9
+
+
+def compare_with_oracle(dev, insts, sched, ctx=b"logup"):
+    from tessera_amd.logup import prove_logup
+    from tessera_amd.transcript import Transcript
+    t = Transcript(ctx)
+    got = prove_logup(dev, t, to_gpu_instances(insts), sched)
+    ot = O.OracleTranscript(ctx)
+    want = O.prove_logup(ot, insts, sched)
+    assert got.alpha == want["alpha"] and got.beta == want["beta"]
+    assert as_tuples(got.roots) == as_tuples(want["roots"])
+    assert got.levels == want["levels"]
+    assert as_tuples(got.values) == as_tuples(want["values"])
+    assert as_tuples(got.claims) == as_tuples(want["claims"])
+    assert t.state == ot.state and t.counter == ot.counter
+    return got
+
+
+def random_instance(rng, T, N, values=True):
+    keys = rng.integers(0, P, T, dtype=np.uint64)
+    vals = rng.integers(0, P, T, dtype=np.uint64) if values else None
+    rows = rng.integers(0, T, N)
+    return {"keys": keys, "values": vals, "lookup_keys": keys[rows],
+            "lookup_values": vals[rows] if values else None,
+            "multiplicities": O.multiplicities(T, rows)}
+
+
+@pytest.mark.parametrize("shape", [[(7, 6)], [(1, 1)], [(300, 1000), (17, 5), (2, 2)],
+                                   [(1 << 10, 1 << 13)]])
+def test_random_batches(dev, shape):
+    rng = np.random.default_rng(len(shape) * 100 + shape[0][1])
+    insts = [random_instance(rng, T, N, values=(i % 2 == 0)) for i, (T, N) in enumerate(shape)]
+    sched = list(range(2 * len(insts)))
+    sched = sched[::-1] if len(insts) > 1 else sched
+    compare_with_oracle(dev, insts, sched)
+
+
+def test_multiplicities(dev):
+    from tessera_amd.logup import multiplicities
+    rng = np.random.default_rng(2)
+    rows = rng.integers(0, 1000, 5000).astype(np.uint64)
+    assert np.array_equal(multiplicities(dev, 1000, rows), O.multiplicities(1000, rows))
+    import tessera_amd as T
+    with pytest.raises(T.BadShape, match="LookupRow"):
+        multiplicities(dev, 10, np.array([3, 10], dtype=np.uint64))
+
+
+def test_config3_logup_against_oracle(dev):
+    """2^22 lookups Zipf(1.1) into 2^16 rows with values: the whole proof against the oracle."""
+    from tessera_amd.logup import multiplicities
+    keys, vals, idx = logup_config3()
+    m = multiplicities(dev, keys.size, idx)
+    assert np.array_equal(m, O.multiplicities(keys.size, idx))
+    inst = {"keys": keys, "values": vals, "lookup_keys": keys[idx], "lookup_values": vals[idx],
+            "multiplicities": m}
+    got = compare_with_oracle(dev, [inst], [0, 1], ctx=b"config-3-logup")
+    # the verifier's root check: n_a d_t + n_t d_a = 0
+    na, da, nt, dt = as_tuples(got.roots)
+    s1, s2 = O.fp2_mul(na, dt), O.fp2_mul(nt, da)
+    assert ((s1[0] + s2[0]) % P, (s1[1] + s2[1]) % P) == (0, 0)

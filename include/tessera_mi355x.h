@@ -122,6 +122,34 @@ tsr_status tsr_commitment_answer(const tsr_commitment *c, const uint64_t *querie
                                  uint64_t *n_siblings);
 void tsr_commitment_free(tsr_commitment *c);
 
+/* ---- batched commitments: many unmasked matrices in one planned launch sequence. No single
+ *      reference function: each matrix's root is ReferenceCommitment::commit_unmasked's
+ *      (crates/tessera-pcs/src/commitment.rs:343-446); the batch replaces calling it per matrix. */
+typedef struct tsr_batch tsr_batch;
+typedef struct {
+    uint32_t log2_k, log2_inv_rate; /* k = 2^log2_k, n = 2^(log2_k + r) */
+    uint64_t row_block;             /* B, 1..124 */
+    uint8_t commitment_id[32];
+    const uint64_t *rows;           /* count x k values, host or device */
+    uint64_t count;                 /* rows of the matrix (may be 0) */
+} tsr_commit_desc;
+enum {
+    TSR_BATCH_AUTOTUNE = 1, /* time the tile-size candidates of each shape group on its rows */
+    TSR_BATCH_GRAPH = 2     /* capture the launch sequence once into a hipGraph and replay it */
+};
+/* Plans a batch: device buffers for every matrix's codewords and tree, the shape groups, the
+ * block tables. Host rows are uploaded once; device rows are used in place on every run. */
+tsr_status tsr_commit_batch_create(tsr_ctx *ctx, const tsr_commit_desc *descs, uint32_t count,
+                                   uint32_t flags, tsr_batch **out);
+/* Encodes, chains and Merkle-commits every matrix (synchronously). */
+tsr_status tsr_commit_batch_run(tsr_batch *b);
+/* The count roots, 32 bytes each, in descriptor order, of the last run. */
+tsr_status tsr_commit_batch_roots(const tsr_batch *b, uint8_t *roots);
+/* The autotuner's choice per shape group: (log2_k, r, tile_log) triples, `cap` groups at most. */
+tsr_status tsr_commit_batch_tuning(const tsr_batch *b, uint32_t *tiles, uint32_t cap,
+                                   uint32_t *n_groups);
+void tsr_commit_batch_free(tsr_batch *b);
+
 /* ---- hashing and the Fiat-Shamir transcript (crates/tessera-hash/src/domain.rs:177-181,
  *      crates/tessera-hash/src/transcript.rs:40-290) ---- */
 /* H(domain, bytes) = BLAKE3.derive_key("tessera/v1/<name>", bytes); `domain` indexes the

@@ -19,4 +19,23 @@ Status gather_digests(const uint32_t *nodes, const uint64_t *idx, uint64_t q, ui
                       hipStream_t stream);
 std::vector<uint64_t> multiproof_positions(uint64_t n, std::vector<uint64_t> known);
 
+// One matrix of a batch, as the multi-matrix kernels see it.
+struct MatDesc {
+    const uint64_t *cw;  // rows x n codewords
+    uint64_t n, rows;
+    uint32_t block;
+    uint32_t id[8];
+    uint32_t *nodes;     // (2n-1) x 8 words, leaves first
+};
+// A block of 256 consecutive columns (chains) or parents (a Merkle level) of matrix `mat`.
+struct BlockRef {
+    uint32_t mat, pad_;
+    uint64_t first;
+};
+void launch_chain_leaves_multi(Ctx *ctx, const MatDesc *mats, const BlockRef *refs,
+                               uint32_t nblocks, hipStream_t s);
+void launch_merkle_level_multi(Ctx *ctx, const MatDesc *mats, const BlockRef *refs,
+                               uint32_t nblocks, uint32_t level, hipStream_t s);
+void launch_gather_roots(const MatDesc *mats, uint32_t count, uint32_t *roots, hipStream_t s);
+
 }  // namespace tsr

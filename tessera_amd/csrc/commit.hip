@@ -144,6 +144,82 @@ __global__ void k_gather_digests(const uint32_t *__restrict__ nodes,
     out[t] = nodes[8 * idx[t / 8] + t % 8];
 }
 
+// ---- many matrices per launch (batched commitments) ----
+
+// Every column chain of every matrix in one launch: block b covers columns
+// refs[b].first .. +255 of matrix refs[b].mat.
+__global__ void __launch_bounds__(256)
+k_chain_leaves_multi(const MatDesc *__restrict__ mats, const BlockRef *__restrict__ refs,
+                     Key8 init_key, Key8 step_key) {
+    const BlockRef ref = refs[blockIdx.x];
+    const MatDesc &m = mats[ref.mat];
+    const uint64_t j = ref.first + threadIdx.x;
+    if (j >= m.n) return;
+    Key8 id;
+#pragma unroll
+    for (int i = 0; i < 8; i++) id.w[i] = m.id[i];
+    uint32_t d[8];
+    b3::hash_id_index(init_key.w, id.w, j, d);
+    for (uint64_t r0 = 0; r0 < m.rows; r0 += m.block) {
+        uint32_t count = (uint32_t)(m.rows - r0 < m.block ? m.rows - r0 : m.block);
+        chain_step(d, m.cw + r0 * m.n + j, m.n, count, step_key);
+    }
+    uint4 *o = (uint4 *)(m.nodes + 8 * j);
+    o[0] = make_uint4(d[0], d[1], d[2], d[3]);
+    o[1] = make_uint4(d[4], d[5], d[6], d[7]);
+}
+
+// One Merkle level of every matrix that still has one: block b computes parents
+// refs[b].first .. +255 of level `level` of matrix refs[b].mat.
+__global__ void __launch_bounds__(256)
+k_merkle_level_multi(const MatDesc *__restrict__ mats, const BlockRef *__restrict__ refs,
+                     uint32_t level, Key8 key) {
+    const BlockRef ref = refs[blockIdx.x];
+    const MatDesc &m = mats[ref.mat];
+    const uint64_t width_in = m.n >> (level - 1), count = m.n >> level;
+    const uint64_t i = ref.first + threadIdx.x;
+    if (i >= count) return;
+    // level l starts at node sum_{t<l} n >> t = 2n - 2 (n >> (l-1)) ... = 2n - 2 width_in
+    const uint64_t start_in = 2 * m.n - 2 * width_in, start_out = start_in + width_in;
+    const uint4 *l = (const uint4 *)(m.nodes + 8 * (start_in + 2 * i));
+    uint4 a = l[0], b = l[1], c = l[2], e = l[3];
+    uint32_t L[8] = {a.x, a.y, a.z, a.w, b.x, b.y, b.z, b.w};
+    uint32_t R[8] = {c.x, c.y, c.z, c.w, e.x, e.y, e.z, e.w};
+    uint32_t d[8];
+    b3::hash64(key.w, L, R, d);
+    uint4 *o = (uint4 *)(m.nodes + 8 * (start_out + i));
+    o[0] = make_uint4(d[0], d[1], d[2], d[3]);
+    o[1] = make_uint4(d[4], d[5], d[6], d[7]);
+}
+
+// Every matrix's root (node 2n - 2) into roots[mat].
+__global__ void k_gather_roots(const MatDesc *__restrict__ mats, uint32_t count,
+                               uint32_t *__restrict__ roots) {
+    uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= count * 8) return;
+    const MatDesc &m = mats[t / 8];
+    roots[t] = m.nodes[8 * (2 * m.n - 2) + t % 8];
+}
+
+void launch_chain_leaves_multi(Ctx *ctx, const MatDesc *mats, const BlockRef *refs,
+                               uint32_t nblocks, hipStream_t s) {
+    if (!nblocks) return;
+    hipLaunchKernelGGL(k_chain_leaves_multi, dim3(nblocks), dim3(256), 0, s, mats, refs,
+                       key_of(ctx, DOM_COLUMN_INIT), key_of(ctx, DOM_COLUMN_STEP));
+}
+
+void launch_merkle_level_multi(Ctx *ctx, const MatDesc *mats, const BlockRef *refs,
+                               uint32_t nblocks, uint32_t level, hipStream_t s) {
+    if (!nblocks) return;
+    hipLaunchKernelGGL(k_merkle_level_multi, dim3(nblocks), dim3(256), 0, s, mats, refs, level,
+                       key_of(ctx, DOM_MERKLE_NODE));
+}
+
+void launch_gather_roots(const MatDesc *mats, uint32_t count, uint32_t *roots, hipStream_t s) {
+    hipLaunchKernelGGL(k_gather_roots, dim3((count * 8 + 255) / 256), dim3(256), 0, s, mats,
+                       count, roots);
+}
+
 static unsigned blocks_for(uint64_t n, unsigned per) { return (unsigned)((n + per - 1) / per); }
 
 Status merkle_build(Ctx *ctx, uint32_t *nodes, uint64_t n, hipStream_t stream) {

@@ -33,7 +33,8 @@ template <bool GS>
 __global__ void __launch_bounds__(TILE_THREADS)
 k_tile_pass(const uint64_t *__restrict__ in, uint64_t *__restrict__ out, uint64_t in_stride,
             uint64_t out_stride, uint32_t L, uint32_t lo, uint32_t nb, uint32_t logc,
-            const uint64_t *__restrict__ tw) {
+            const uint64_t *__restrict__ tw, const uint64_t *const *in_rows,
+            uint64_t *const *out_rows) {
     extern __shared__ uint64_t lds[];
     const uint32_t C = 1u << logc;
     const uint32_t tiles_per_hi = 1u << (lo - logc);
@@ -42,8 +43,9 @@ k_tile_pass(const uint64_t *__restrict__ in, uint64_t *__restrict__ out, uint64_
     const uint64_t row = blockIdx.y;
     const uint64_t base = (hi << (lo + nb)) + (lblk << logc);
     const uint32_t tile = 1u << (nb + logc);
-    const uint64_t *src = in + row * in_stride;
-    uint64_t *dst = out + row * out_stride;
+    // rows by stride, or (batched matrices) by a per-row pointer table
+    const uint64_t *src = in_rows ? in_rows[row] : in + row * in_stride;
+    uint64_t *dst = out_rows ? out_rows[row] : out + row * out_stride;
 
     for (uint32_t e = threadIdx.x; e < tile; e += TILE_THREADS) {
         uint64_t mid = e >> logc, col = e & (C - 1);
@@ -90,12 +92,13 @@ k_encode_mid(const uint64_t *__restrict__ in, uint64_t in_stride, uint64_t *__re
              uint64_t cw_stride, uint32_t log_k, uint32_t log_r, uint32_t tk,
              const uint64_t *__restrict__ zk_inv, const uint64_t *__restrict__ zn,
              const uint64_t *__restrict__ spread, const uint64_t *__restrict__ masks,
-             uint64_t P, const uint64_t *__restrict__ pow7, uint64_t shift_pow_k) {
+             uint64_t P, const uint64_t *__restrict__ pow7, uint64_t shift_pow_k,
+             const uint64_t *const *in_rows, uint64_t *const *cw_rows) {
     extern __shared__ uint64_t lds[];
     const uint32_t Tk = 1u << tk, Tn = Tk << log_r, s = 1u << log_r;
     const uint64_t c = blockIdx.x, row = blockIdx.y;
     const uint64_t kbase = c << tk, nbase = c << (tk + log_r);
-    const uint64_t *src = in + row * in_stride;
+    const uint64_t *src = in_rows ? in_rows[row] : in + row * in_stride;
 
     for (uint32_t e = threadIdx.x; e < Tk; e += TILE_THREADS) lds[e] = src[kbase + e];
     __syncthreads();
@@ -156,7 +159,7 @@ k_encode_mid(const uint64_t *__restrict__ in, uint64_t in_stride, uint64_t *__re
         __syncthreads();
     }
 
-    uint64_t *dst = cw + row * cw_stride;
+    uint64_t *dst = cw_rows ? cw_rows[row] : cw + row * cw_stride;
     for (uint32_t e = threadIdx.x; e < Tn; e += TILE_THREADS) dst[nbase + e] = lds[e];
 }
 
@@ -203,18 +206,21 @@ static std::vector<std::pair<uint32_t, uint32_t>> split_bits(uint32_t lo, uint32
     return passes;
 }
 
-Status Encoder::init(Ctx *ctx, uint32_t log_k, uint32_t log_r, uint64_t P) {
+Status Encoder::init(Ctx *ctx, uint32_t log_k, uint32_t log_r, uint64_t P, uint32_t tile_log) {
     this->ctx = ctx;
+    if (tile_log < log_r + 1) tile_log = log_r + 1;
+    if (tile_log > TILE_MAX_LOG) tile_log = TILE_MAX_LOG;
+    this->tile_log = tile_log;
     this->log_k = log_k;
     this->log_r = log_r;
     this->P = P;
     const uint32_t log_n = log_k + log_r;
     const uint64_t k = 1ull << log_k;
-    tk = log_k < (uint32_t)(TILE_MAX_LOG - log_r) ? log_k : (uint32_t)(TILE_MAX_LOG - log_r);
+    tk = log_k < tile_log - log_r ? log_k : tile_log - log_r;
     // CT passes over k-bits [tk, log_k), run from the top bits down.
-    auto ct = split_bits(tk, log_k, TILE_MAX_LOG - 4);
+    auto ct = split_bits(tk, log_k, tile_log - 4);
     ct_passes.assign(ct.rbegin(), ct.rend());
-    gs_passes = split_bits(tk + log_r, log_n, TILE_MAX_LOG - 4);
+    gs_passes = split_bits(tk + log_r, log_n, tile_log - 4);
 
     std::vector<uint64_t> zki = twiddle_table(log_k, true);
     std::vector<uint64_t> zn = twiddle_table(log_n, false);
@@ -241,51 +247,71 @@ Status Encoder::encode(const uint64_t *rows, const uint64_t *masks, uint64_t cou
     const uint64_t k = 1ull << log_k, n = k << log_r;
     for (uint64_t r0 = 0; r0 < count; r0 += 65535) {
         const uint64_t rc = count - r0 < 65535 ? count - r0 : 65535;
-        const uint64_t *src = rows + r0 * k;
-        uint64_t src_stride = k;
-        for (auto &ps : ct_passes) {
-            uint32_t lo = ps.first, nb = ps.second, logc = lo < 4 ? lo : 4;
-            dim3 grid((unsigned)(k >> (nb + logc)), (unsigned)rc);
-            size_t shm = sizeof(uint64_t) << (nb + logc);
-            KernelTimer timer(ctx, "k_tile_pass<ct>", stream);
-            hipLaunchKernelGGL(k_tile_pass<false>, grid, dim3(TILE_THREADS), shm, stream, src,
-                               scratch + r0 * k, src_stride, k, log_k, lo, nb, logc,
-                               zk_inv.ptr);
-            src = scratch + r0 * k;
-        }
-        {
-            dim3 grid((unsigned)(k >> tk), (unsigned)rc);
-            size_t shm = sizeof(uint64_t) << (tk + log_r);
-            KernelTimer timer(ctx, "k_encode_mid", stream);
-            hipLaunchKernelGGL(k_encode_mid, grid, dim3(TILE_THREADS), shm, stream, src,
-                               src_stride, codewords + r0 * n, n, log_k, log_r, tk,
-                               zk_inv.ptr, zn_fwd.ptr, spread_tab.ptr,
-                               masks ? masks + r0 * P : nullptr, P, pow7_tab.ptr,
-                               shift_pow_k);
-        }
-        for (auto &ps : gs_passes) {
-            uint32_t lo = ps.first, nb = ps.second, logc = lo < 4 ? lo : 4;
-            uint32_t log_n = log_k + log_r;
-            dim3 grid((unsigned)(n >> (nb + logc)), (unsigned)rc);
-            size_t shm = sizeof(uint64_t) << (nb + logc);
-            KernelTimer timer(ctx, "k_tile_pass<gs>", stream);
-            hipLaunchKernelGGL(k_tile_pass<true>, grid, dim3(TILE_THREADS), shm, stream,
-                               codewords + r0 * n, codewords + r0 * n, n, n, log_n, lo, nb,
-                               logc, zn_fwd.ptr);
-        }
+        launch(rows + r0 * k, k, nullptr, scratch + r0 * k, nullptr, codewords + r0 * n, nullptr,
+               masks ? masks + r0 * P : nullptr, rc, stream);
     }
     return check_launch("encode_rows");
 }
 
-Status get_encoder(Ctx *ctx, uint32_t log_k, uint32_t log_r, uint64_t P, Encoder **out) {
-    auto key = std::make_tuple(log_k, log_r, P);
+Status Encoder::encode_rowmap(const uint64_t *const *src_rows, uint64_t *const *tmp_rows,
+                              uint64_t *const *cw_rows, uint64_t count, hipStream_t stream) const {
+    for (uint64_t r0 = 0; r0 < count; r0 += 65535) {
+        const uint64_t rc = count - r0 < 65535 ? count - r0 : 65535;
+        launch(nullptr, 0, src_rows + r0, nullptr, tmp_rows + r0, nullptr, cw_rows + r0, nullptr,
+               rc, stream);
+    }
+    return check_launch("encode_rows (batched)");
+}
+
+// One group of up to 65535 rows: the strided CT passes (rows -> scratch, then in place), the
+// fused middle (-> codewords), the strided GS passes (in place). Rows are addressed by stride
+// (src, tmp, cw) or by per-row pointer tables (src_rows, tmp_rows, cw_rows).
+void Encoder::launch(const uint64_t *src, uint64_t src_stride, const uint64_t *const *src_rows,
+                     uint64_t *tmp, uint64_t *const *tmp_rows, uint64_t *cw,
+                     uint64_t *const *cw_rows, const uint64_t *masks, uint64_t rc,
+                     hipStream_t stream) const {
+    const uint64_t k = 1ull << log_k, n = k << log_r;
+    for (auto &ps : ct_passes) {
+        uint32_t lo = ps.first, nb = ps.second, logc = lo < 4 ? lo : 4;
+        dim3 grid((unsigned)(k >> (nb + logc)), (unsigned)rc);
+        size_t shm = sizeof(uint64_t) << (nb + logc);
+        KernelTimer timer(ctx, "k_tile_pass<ct>", stream);
+        hipLaunchKernelGGL(k_tile_pass<false>, grid, dim3(TILE_THREADS), shm, stream, src, tmp,
+                           src_stride, k, log_k, lo, nb, logc, zk_inv.ptr, src_rows, tmp_rows);
+        src = tmp;
+        src_stride = k;
+        src_rows = (const uint64_t *const *)tmp_rows;
+    }
+    {
+        dim3 grid((unsigned)(k >> tk), (unsigned)rc);
+        size_t shm = sizeof(uint64_t) << (tk + log_r);
+        KernelTimer timer(ctx, "k_encode_mid", stream);
+        hipLaunchKernelGGL(k_encode_mid, grid, dim3(TILE_THREADS), shm, stream, src, src_stride,
+                           cw, n, log_k, log_r, tk, zk_inv.ptr, zn_fwd.ptr, spread_tab.ptr, masks,
+                           P, pow7_tab.ptr, shift_pow_k, src_rows, cw_rows);
+    }
+    for (auto &ps : gs_passes) {
+        uint32_t lo = ps.first, nb = ps.second, logc = lo < 4 ? lo : 4;
+        uint32_t log_n = log_k + log_r;
+        dim3 grid((unsigned)(n >> (nb + logc)), (unsigned)rc);
+        size_t shm = sizeof(uint64_t) << (nb + logc);
+        KernelTimer timer(ctx, "k_tile_pass<gs>", stream);
+        hipLaunchKernelGGL(k_tile_pass<true>, grid, dim3(TILE_THREADS), shm, stream, cw, cw, n, n,
+                           log_n, lo, nb, logc, zn_fwd.ptr, (const uint64_t *const *)cw_rows,
+                           cw_rows);
+    }
+}
+
+Status get_encoder(Ctx *ctx, uint32_t log_k, uint32_t log_r, uint64_t P, Encoder **out,
+                   uint32_t tile_log) {
+    auto key = std::make_tuple(log_k, log_r, P, tile_log);
     auto it = ctx->encoders.find(key);
     if (it != ctx->encoders.end()) {
         *out = it->second.get();
         return Status::ok();
     }
     std::shared_ptr<Encoder> e = std::make_shared<Encoder>();
-    Status st = e->init(ctx, log_k, log_r, P);
+    Status st = e->init(ctx, log_k, log_r, P, tile_log);
     if (st) return st;
     *out = e.get();
     ctx->encoders.emplace(key, std::move(e));

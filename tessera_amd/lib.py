@@ -86,6 +86,11 @@ _SIGS = {
     "tsr_logup_prove": (i32, [vp, vp, vp, u32, vp, vp, vp, vp, vp, ctypes.POINTER(u64), vp, vp,
                               u64, ctypes.POINTER(u64)]),
     "tsr_logup_multiplicities": (i32, [vp, u64, vp, u64, vp]),
+    "tsr_commit_batch_create": (i32, [vp, vp, u32, u32, ctypes.POINTER(vp)]),
+    "tsr_commit_batch_run": (i32, [vp]),
+    "tsr_commit_batch_roots": (i32, [vp, vp]),
+    "tsr_commit_batch_tuning": (i32, [vp, vp, u32, ctypes.POINTER(u32)]),
+    "tsr_commit_batch_free": (None, [vp]),
     "tsr_ctx_profile": (i32, [vp, i32]),
     "tsr_ctx_kernel_time": (i32, [vp, ctypes.c_char_p, ctypes.POINTER(ctypes.c_double),
                                   ctypes.POINTER(u64)]),

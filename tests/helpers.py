@@ -98,3 +98,15 @@ def logup_config3(lookups=1 << 22, rows=1 << 16, seed=0x1064):
     values = splitmix_uniform(rows, seed)
     idx = zipf_rows(lookups, rows, 1.1, seed + 1)
     return keys, values, idx
+
+
+def batched_shapes(count=64, seed=0xBA7C, log_k=(10, 20), cols=(16, 512)):
+    """The batched-commit workload: `count` matrices, log2 k uniform in [log_k], N rows uniform in
+    [cols] (the configs' 'columns'), from a SplitMix64 stream."""
+    r = splitmix_raw(2 * count, seed)
+    out = []
+    for i in range(count):
+        lk = log_k[0] + int(r[2 * i] % np.uint64(log_k[1] - log_k[0] + 1))
+        n = cols[0] + int(r[2 * i + 1] % np.uint64(cols[1] - cols[0] + 1))
+        out.append((lk, n))
+    return out

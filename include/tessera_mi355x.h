@@ -150,6 +150,54 @@ tsr_status tsr_commit_batch_tuning(const tsr_batch *b, uint32_t *tiles, uint32_t
                                    uint32_t *n_groups);
 void tsr_commit_batch_free(tsr_batch *b);
 
+/* ---- served proving with continuous batching. No reference counterpart: the reference proves
+ *      one statement per call and has no serving loop. Each request: a synthetic int8
+ *      transformer (4 layers, d_model 256, 4 heads, d_ff 1024, vocab 4096) decodes greedily and
+ *      writes its witness; the witness is committed (rows of 2^13, rate 1/4, B = 32, unmasked);
+ *      a degree-3 sumcheck Sum eq(r,x) A(x) B(x) over two witness tables follows on the request's
+ *      own transcript. Workers run admitted requests concurrently, one HIP stream each. ---- */
+#define TSR_MAX_TOKENS 512
+typedef struct tsr_server tsr_server;
+typedef struct {
+    uint32_t slots;       /* worker slots: requests in flight at once (1..64) */
+    uint64_t hbm_budget;  /* admission budget in bytes for the requests in flight (0 = none) */
+    uint64_t model_seed;  /* weights: SplitMix64 words mod 255, minus 127 */
+} tsr_server_config;
+typedef struct {
+    uint64_t id;           /* also the proof's session id */
+    uint32_t n_prompt;     /* >= 1 prompt tokens, each < 4096 */
+    const uint32_t *prompt;
+    uint32_t n_output;     /* tokens generated greedily; n_prompt + n_output <= 512 */
+} tsr_request;
+typedef struct {
+    uint64_t id;
+    tsr_status status;
+    uint32_t slot;
+    uint8_t root[32];               /* the witness commitment */
+    uint8_t transcript_state[32];   /* after the sumcheck's final evaluations */
+    uint64_t transcript_counter;
+    uint64_t final_a[2];            /* A~(point), the first final evaluation */
+    uint64_t witness_rows, sumcheck_vars;
+    uint32_t n_tokens;
+    uint32_t tokens[TSR_MAX_TOKENS]; /* the prompt, then the generated tokens */
+    double t_submit, t_admit, t_done; /* seconds on the server's clock */
+} tsr_result;
+tsr_status tsr_server_create(int device, const tsr_server_config *cfg, tsr_server **out);
+/* Queues a request (copied); returns at once. */
+tsr_status tsr_server_submit(tsr_server *srv, const tsr_request *req);
+/* Takes one finished result, waiting up to timeout_ms (DECLINED when none arrived). */
+tsr_status tsr_server_wait(tsr_server *srv, tsr_result *out, uint32_t timeout_ms);
+/* The most requests ever in flight together, and the footprint in flight now. */
+tsr_status tsr_server_stats(tsr_server *srv, uint32_t *max_active, uint64_t *in_flight_bytes);
+/* The plain per-request call: proves one request synchronously on the calling thread, on the
+ * server's own context (concurrent calls take turns). */
+tsr_status tsr_server_prove(tsr_server *srv, const tsr_request *req, tsr_result *out);
+/* The admission footprint of a request of `tokens` tokens, in bytes. */
+uint64_t tsr_server_footprint(uint32_t tokens);
+/* Seconds since the server started (the clock of the results' timestamps). */
+double tsr_server_clock(tsr_server *srv);
+void tsr_server_destroy(tsr_server *srv);
+
 /* ---- hashing and the Fiat-Shamir transcript (crates/tessera-hash/src/domain.rs:177-181,
  *      crates/tessera-hash/src/transcript.rs:40-290) ---- */
 /* H(domain, bytes) = BLAKE3.derive_key("tessera/v1/<name>", bytes); `domain` indexes the

@@ -529,3 +529,57 @@ def parse_level_log(log):
         lv["counter"] = log[at]; at += 1
         out.append(lv)
     return out
+
+
+# ---------------- the synthetic model and a served request's proof ----------------
+
+def model_weights(seed):
+    L = lib()
+    L.orc_model_bytes.restype = u64
+    L.orc_model_weights.restype = None
+    L.orc_model_weights.argtypes = [u64, vp]
+    w = np.zeros(L.orc_model_bytes(), dtype=np.int8)
+    L.orc_model_weights(seed, w.ctypes.data)
+    return w
+
+
+def model_decode(weights, prompt, total):
+    L = lib()
+    L.orc_model_witness_stride.restype = u64
+    L.orc_model_decode.restype = None
+    L.orc_model_decode.argtypes = [vp, vp, u32, u32, vp, vp]
+    stride = L.orc_model_witness_stride()
+    p = np.ascontiguousarray(prompt, dtype=np.uint32)
+    toks = np.zeros(total, dtype=np.uint32)
+    wit = np.zeros(total * stride, dtype=np.uint64)
+    L.orc_model_decode(weights.ctypes.data, p.ctypes.data, p.size, total, toks.ctypes.data,
+                       wit.ctypes.data)
+    return toks, wit, stride
+
+
+def served_proof(weights, request_id, prompt, n_output, threads=8):
+    """The proof a served request carries: (tokens, witness root, transcript state, counter)."""
+    import struct
+    total = len(prompt) + n_output
+    toks, wit, stride = model_decode(weights, prompt, total)
+    k = 1 << 13
+    rows = -(-wit.size // k)
+    padded = np.zeros(rows * k, dtype=np.uint64)
+    padded[:wit.size] = wit
+    _, nodes = commit_unmasked(padded, 13, 2, 32, bytes([0x5E] * 32), threads=threads)
+    root = nodes[-32:]
+    ctx = (struct.pack("<Q", 18) + b"tessera-protocol/1" + struct.pack("<H", 1) + struct.pack("<Q", 12)
+           + b"test-small/1" + bytes(32) + struct.pack("<Q", request_id) + bytes(8))
+    t = OracleTranscript(ctx)
+    t.absorb("serve/root", root)
+    n = (total * 256 - 1).bit_length()
+    r = [t.challenge_fp2(f"serve/r/{i}")[0] for i in range(n)]
+    eq = eq_table(np.array(r, dtype=np.uint64).reshape(-1))
+    A = np.zeros(1 << n, dtype=np.uint64)
+    B = np.zeros(1 << n, dtype=np.uint64)
+    w2 = wit.reshape(total, stride)
+    A[:total * 256] = w2[:, 0:256].reshape(-1)
+    B[:total * 256] = w2[:, 5 * 256:6 * 256].reshape(-1)
+    out = prove_batch(t, "stage/0", 3, [[(eq, True, True), (A, False, False), (B, False, False)]],
+                      threads=threads)
+    return toks, root, t.state, t.counter, out

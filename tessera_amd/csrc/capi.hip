@@ -56,7 +56,18 @@ void pool_trim(Ctx *ctx) {
     ctx->pooled = 0;
 }
 
+// Allocation size classes: powers of two and their midpoints (at most a third wasted), so that
+// buffers of many nearby sizes (served requests of different lengths) reuse pooled blocks.
+static uint64_t size_class(uint64_t bytes) {
+    if (bytes <= 4096) return 4096;
+    uint64_t p = 1ull << (63 - __builtin_clzll(bytes));
+    if (bytes == p) return p;
+    if (bytes <= p + p / 2) return p + p / 2;
+    return 2 * p;
+}
+
 Status dev_alloc(Ctx *ctx, uint64_t bytes, void **out) {
+    bytes = size_class(bytes);
     if (ctx->budget && ctx->in_use + bytes > ctx->budget)
         return Status::fail(TSR_OOM, "device memory budget exceeded: " +
                                          std::to_string(ctx->in_use + bytes) + " > " +
@@ -91,6 +102,7 @@ void dev_free(Ctx *ctx, void *p, uint64_t bytes) {
         (void)hipFree(p);
         return;
     }
+    bytes = size_class(bytes);
     ctx->in_use -= bytes;
     ctx->pool.emplace(bytes, p);
     ctx->pooled += bytes;

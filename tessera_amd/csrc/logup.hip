@@ -495,3 +495,11 @@ tsr_status tsr_logup_multiplicities(tsr_ctx *ctx_, uint64_t rows, const uint64_t
 }
 
 }  // extern "C"
+
+namespace tsr {
+Status eq_table(Ctx *ctx, const gl::fp2 *point, uint32_t n, gl::fp2 *out, hipStream_t s) {
+    KernelTimer timer(ctx, "k_eq_table", s);
+    hipLaunchKernelGGL(k_eq_table, dim3(nblocks(1ull << n, 256)), dim3(256), 0, s, point, n, out);
+    return check_launch("eq table");
+}
+}  // namespace tsr

@@ -39,4 +39,12 @@ struct GenInst {
 Status run_batch(Ctx *ctx, Transcript &tr, const std::string &prefix, uint32_t degree,
                  std::vector<GenInst> &insts, gl::fp2 gamma, gl::fp2 lambda, SumcheckOut &out);
 
+// prove_batch of product instances (host tables are uploaded into `uploads`).
+Status prove_batch(Ctx *ctx, Transcript &tr, const char *prefix, uint32_t degree,
+                   const tsr_instance *instances, uint32_t count, SumcheckOut &out,
+                   std::vector<DevBuf<uint64_t>> &uploads);
+
+// eq(r, .) over n variables from a device point of n Fp2 (tessera-poly eq.rs:23).
+Status eq_table(Ctx *ctx, const gl::fp2 *point, uint32_t n, gl::fp2 *out, hipStream_t s);
+
 }  // namespace tsr

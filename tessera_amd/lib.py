@@ -91,6 +91,14 @@ _SIGS = {
     "tsr_commit_batch_roots": (i32, [vp, vp]),
     "tsr_commit_batch_tuning": (i32, [vp, vp, u32, ctypes.POINTER(u32)]),
     "tsr_commit_batch_free": (None, [vp]),
+    "tsr_server_create": (i32, [i32, vp, ctypes.POINTER(vp)]),
+    "tsr_server_submit": (i32, [vp, vp]),
+    "tsr_server_wait": (i32, [vp, vp, u32]),
+    "tsr_server_stats": (i32, [vp, ctypes.POINTER(u32), ctypes.POINTER(u64)]),
+    "tsr_server_prove": (i32, [vp, vp, vp]),
+    "tsr_server_footprint": (u64, [u32]),
+    "tsr_server_clock": (ctypes.c_double, [vp]),
+    "tsr_server_destroy": (None, [vp]),
     "tsr_ctx_profile": (i32, [vp, i32]),
     "tsr_ctx_kernel_time": (i32, [vp, ctypes.c_char_p, ctypes.POINTER(ctypes.c_double),
                                   ctypes.POINTER(u64)]),

@@ -155,11 +155,15 @@ void tsr_commit_batch_free(tsr_batch *b);
  *      transformer (4 layers, d_model 256, 4 heads, d_ff 1024, vocab 4096) decodes greedily and
  *      writes its witness; the witness is committed (rows of 2^13, rate 1/4, B = 32, unmasked);
  *      a degree-3 sumcheck Sum eq(r,x) A(x) B(x) over two witness tables follows on the request's
- *      own transcript. Workers run admitted requests concurrently, one HIP stream each. ---- */
+ *      own transcript. Admitted requests decode together (one launch advances each by up to
+ *      `chunk` positions; requests join and leave between launches) and are proved on prover
+ *      streams while decoding goes on. ---- */
 #define TSR_MAX_TOKENS 512
 typedef struct tsr_server tsr_server;
 typedef struct {
-    uint32_t slots;       /* worker slots: requests in flight at once (1..64) */
+    uint32_t slots;       /* requests in flight at once, decoding or proving (1..1024) */
+    uint32_t provers;     /* prover threads, one stream each (0 = 4) */
+    uint32_t chunk;       /* positions a decoding request advances per decode launch (0 = 32) */
     uint64_t hbm_budget;  /* admission budget in bytes for the requests in flight (0 = none) */
     uint64_t model_seed;  /* weights: SplitMix64 words mod 255, minus 127 */
 } tsr_server_config;

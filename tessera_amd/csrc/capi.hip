@@ -50,10 +50,15 @@ Status check_launch(const char *what) {
     return Status::ok();
 }
 
-void pool_trim(Ctx *ctx) {
+static void pool_trim_locked(Ctx *ctx) {
     for (auto &kv : ctx->pool) (void)hipFree(kv.second);
     ctx->pool.clear();
     ctx->pooled = 0;
+}
+
+void pool_trim(Ctx *ctx) {
+    std::lock_guard<std::mutex> lk(ctx->pool_mu);
+    pool_trim_locked(ctx);
 }
 
 // Allocation size classes: powers of two and their midpoints (at most a third wasted), so that
@@ -67,6 +72,7 @@ static uint64_t size_class(uint64_t bytes) {
 }
 
 Status dev_alloc(Ctx *ctx, uint64_t bytes, void **out) {
+    std::lock_guard<std::mutex> lk(ctx->pool_mu);
     bytes = size_class(bytes);
     if (ctx->budget && ctx->in_use + bytes > ctx->budget)
         return Status::fail(TSR_OOM, "device memory budget exceeded: " +
@@ -80,12 +86,12 @@ Status dev_alloc(Ctx *ctx, uint64_t bytes, void **out) {
         ctx->in_use += bytes;
         return Status::ok();
     }
-    if (ctx->budget && ctx->in_use + ctx->pooled + bytes > ctx->budget) pool_trim(ctx);
+    if (ctx->budget && ctx->in_use + ctx->pooled + bytes > ctx->budget) pool_trim_locked(ctx);
     hipError_t e = hipMalloc(out, bytes);
     if (e == hipErrorOutOfMemory && ctx->pooled) {
         (void)hipGetLastError();
         (void)hipStreamSynchronize(ctx->stream);
-        pool_trim(ctx);
+        pool_trim_locked(ctx);
         e = hipMalloc(out, bytes);
     }
     if (e != hipSuccess) {
@@ -102,6 +108,7 @@ void dev_free(Ctx *ctx, void *p, uint64_t bytes) {
         (void)hipFree(p);
         return;
     }
+    std::lock_guard<std::mutex> lk(ctx->pool_mu);
     bytes = size_class(bytes);
     ctx->in_use -= bytes;
     ctx->pool.emplace(bytes, p);

@@ -11,17 +11,27 @@
 //     x += (W2 h) >> 8
 //   f = rq(x); logits = Wout f; next = argmax (lowest index on ties)
 // with rq(x) = clamp(x >> 2, -127, 127) and rq8(y) = clamp(y >> 8, -127, 127) (arithmetic shifts).
-// Every dot product of int8 vectors is v_dot4_i32_i8 on packed bytes.
+// Integer sums are exact in any order, so the products are split over the inputs ("split-K")
+// and the partial sums added in LDS. Bounds: |q . k| <= 64 * 127^2, so s_u <= 16129 and
+// |sum_u s_u v_u| <= 512 * 16129 * 127 < 2^31 (int32 is exact).
 //
 // The witness of a token is written as Goldilocks elements (Goldilocks::from_i64,
 // crates/tessera-field/src/goldilocks.rs:374: x >= 0 -> x, x < 0 -> p - |x|), WIT_STRIDE values:
 // per layer a, q, k, v, att, m (256 each), h (1024), x (256); then f (256) and the next token.
+//
+// Decoding: one workgroup of 1024 threads per request; a launch advances every request of a
+// task table by its own range of positions, so a serving loop can let requests join and leave
+// between launches (iteration-level batching) without changing any request's arithmetic.
+// Weight reads are packed (model.h) so each wavefront load is 256 contiguous bytes; the 7.3 MB
+// of weights are shared by all requests and stay in the L2 / last-level cache.
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 #include "field.h"
 #include "model.h"
 
 namespace tsr {
+
+constexpr int MT = 1024;  // threads of a decode workgroup
 
 __device__ __forceinline__ int32_t clamp127(int32_t v) { return v < -127 ? -127 : (v > 127 ? 127 : v); }
 
@@ -38,142 +48,220 @@ __device__ __forceinline__ int8_t weight_at(uint64_t seed, uint64_t i) {
     return (int8_t)((int)(z % 255) - 127);
 }
 
+// Offset, within its matrix, of byte r of a packed [in / 4][out] matrix: the logical o * in + i.
+__device__ __forceinline__ uint64_t unpack(uint64_t r, uint64_t in, uint64_t out, uint64_t *o_out) {
+    const uint64_t w = r >> 2, i = 4 * (w / out) + (r & 3), o = w % out;
+    *o_out = o;
+    return i;
+}
+
+// The generator index of device byte b (model.h: device layout -> logical layout).
+__device__ uint64_t logical_index(uint64_t b) {
+    uint64_t o, i;
+    if (b < MODEL_OFF_LAYER) return b;
+    if (b >= MODEL_OFF_OUT) {
+        i = unpack(b - MODEL_OFF_OUT, D_MODEL, VOCAB, &o);
+        return MODEL_OFF_OUT + o * D_MODEL + i;
+    }
+    const uint64_t l = (b - MODEL_OFF_LAYER) / LAYER_BYTES, r = (b - MODEL_OFF_LAYER) % LAYER_BYTES;
+    const uint64_t base = MODEL_OFF_LAYER + l * LAYER_BYTES;
+    if (r < DOFF_O) {
+        i = unpack(r, D_MODEL, 3 * D_MODEL, &o);
+        return base + LOFF_Q + (o / D_MODEL) * D_MODEL * D_MODEL + (o % D_MODEL) * D_MODEL + i;
+    }
+    if (r < DOFF_W1) {
+        i = unpack(r - DOFF_O, D_MODEL, D_MODEL, &o);
+        return base + LOFF_O + o * D_MODEL + i;
+    }
+    if (r < DOFF_W2) {
+        i = unpack(r - DOFF_W1, D_MODEL, D_FF, &o);
+        return base + LOFF_W1 + o * D_MODEL + i;
+    }
+    i = unpack(r - DOFF_W2, D_FF, D_MODEL, &o);
+    return base + LOFF_W2 + o * D_FF + i;
+}
+
 __global__ void k_model_weights(int8_t *__restrict__ w, uint64_t n, uint64_t seed) {
-    uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) w[i] = weight_at(seed, i);
+    uint64_t b = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (b < n) w[b] = weight_at(seed, logical_index(b));
 }
 
-// dot of a 256-byte int8 weight row with a 256-byte int8 vector in LDS (64 packed words).
-__device__ __forceinline__ int32_t dot256(const int8_t *__restrict__ row, const int32_t *packed) {
-    const int32_t *r = (const int32_t *)row;
-    int32_t acc = 0;
+// part[s * O + o] = sum over the s-th 1/S of the input words of w(o, .) . act, for a packed
+// [I4][O] matrix and an activation of I4 packed words in LDS. O * S is a multiple of MT.
+template <int O, int I4, int S>
+__device__ __forceinline__ void matvec_partials(const int32_t *__restrict__ Wp, const int32_t *act,
+                                                int32_t *part, int tid) {
+    constexpr int WPS = I4 / S;
+    static_assert((O * S) % MT == 0 && I4 % S == 0, "matvec split");
+#pragma unroll 1
+    for (int idx = tid; idx < O * S; idx += MT) {
+        const int o = idx % O, s = idx / O;
+        const int32_t *col = Wp + (uint64_t)(s * WPS) * O + o;
+        const int32_t *a = act + s * WPS;
+        int32_t acc = 0;
 #pragma unroll 16
-    for (int i = 0; i < 64; i++) acc = __builtin_amdgcn_sdot4(r[i], packed[i], acc, false);
-    return acc;
+        for (int w = 0; w < WPS; w++) acc = __builtin_amdgcn_sdot4(col[(uint64_t)w * O], a[w], acc, false);
+        part[idx] = acc;
+    }
 }
 
-// dot of a 1024-byte row with a 1024-byte vector (256 packed words).
-__device__ __forceinline__ int32_t dot1024(const int8_t *__restrict__ row, const int32_t *packed) {
-    const int32_t *r = (const int32_t *)row;
-    int32_t acc = 0;
-#pragma unroll 16
-    for (int i = 0; i < 256; i++) acc = __builtin_amdgcn_sdot4(r[i], packed[i], acc, false);
-    return acc;
-}
-
-constexpr int MT = 256;  // threads of a decode workgroup
-
-// One request's whole decode: prompt tokens prompt[0..n_prompt), generation up to `total`
-// tokens; tokens[t] and the witness of every position written out. The KV cache (layers x total
-// x 2 x 256 bytes) lives at kv.
 __global__ void __launch_bounds__(MT)
-k_decode(const int8_t *__restrict__ W, const uint32_t *__restrict__ prompt, uint32_t n_prompt,
-         uint32_t total, uint32_t *__restrict__ tokens, int8_t *__restrict__ kv,
-         uint64_t *__restrict__ wit) {
-    const int8_t *embed = W + MODEL_OFF_EMBED;
-    const int8_t *wout = W + MODEL_OFF_OUT;
+k_decode(const int8_t *__restrict__ W, const DecodeTask *__restrict__ tasks) {
+    const DecodeTask task = tasks[blockIdx.x];
+    const int tid = threadIdx.x;
+    const uint32_t tcap = (task.total + 3) & ~3u;
+    const int32_t *embed = (const int32_t *)(W + MODEL_OFF_EMBED);
+    const int32_t *wout = (const int32_t *)(W + MODEL_OFF_OUT);
+    int32_t *kt = task.kv;                                     // [L][64][tcap]
+    int32_t *vp = task.kv + (uint64_t)N_LAYERS * 64 * tcap;    // [L][tcap / 4][256]
+
     __shared__ int32_t x[D_MODEL];
-    __shared__ int32_t a_pk[D_MODEL / 4], att_pk[D_MODEL / 4], h_pk[D_FF / 4];
-    __shared__ int8_t a8[D_MODEL], att8[D_MODEL], h8[D_FF];
-    __shared__ int8_t q8[D_MODEL];
+    __shared__ int32_t a_pk[D_MODEL / 4], q_pk[D_MODEL / 4], k_pk[D_MODEL / 4], att_pk[D_MODEL / 4];
+    __shared__ int32_t h_pk[D_FF / 4];
+    __shared__ int32_t part[4096], partn[MT];
     __shared__ int32_t scores[N_HEADS][MAX_TOKENS];
     __shared__ int32_t red_v[MT];
     __shared__ uint32_t red_i[MT];
     __shared__ uint32_t cur;
-    const int tid = threadIdx.x;
-    if (tid == 0) cur = prompt[0];
+    int8_t *a8 = (int8_t *)a_pk, *q8 = (int8_t *)q_pk, *k8 = (int8_t *)k_pk, *att8 = (int8_t *)att_pk,
+           *h8 = (int8_t *)h_pk;
+
+    if (tid == 0)
+        cur = task.start < task.n_prompt ? task.prompt[task.start]
+                                         : (uint32_t)task.wit[(uint64_t)(task.start - 1) * WIT_STRIDE + WIT_NEXT];
     __syncthreads();
-    for (uint32_t t = 0; t < total; t++) {
+    for (uint32_t t = task.start; t < task.end; t++) {
         const uint32_t tok = cur;
-        if (tid == 0) tokens[t] = tok;
-        uint64_t *wt = wit + (uint64_t)t * WIT_STRIDE;
-        x[tid] = embed[(uint64_t)tok * D_MODEL + tid];
+        uint64_t *wt = task.wit + (uint64_t)t * WIT_STRIDE;
+        if (tid == 0) task.tokens[t] = tok;
+        if (tid < D_MODEL / 4) {
+            const int32_t e = embed[(uint64_t)tok * (D_MODEL / 4) + tid];
+            for (int b = 0; b < 4; b++) x[4 * tid + b] = (int8_t)(e >> (8 * b));
+        }
         __syncthreads();
         for (int l = 0; l < N_LAYERS; l++) {
-            const int8_t *L = W + MODEL_OFF_LAYER + (uint64_t)l * LAYER_BYTES;
+            const int8_t *Lw = W + MODEL_OFF_LAYER + (uint64_t)l * LAYER_BYTES;
             uint64_t *wl = wt + (uint64_t)l * WIT_LAYER;
-            int8_t *kvl = kv + ((uint64_t)l * total) * 2 * D_MODEL;
+            int32_t *ktl = kt + (uint64_t)l * 64 * tcap;
+            int32_t *vpl = vp + (uint64_t)l * (tcap / 4) * D_MODEL;
             // a = rq(x)
-            int32_t av = clamp127(x[tid] >> 2);
-            a8[tid] = (int8_t)av;
-            wl[0 * D_MODEL + tid] = from_i64(av);
-            __syncthreads();
-            if (tid < D_MODEL / 4) a_pk[tid] = ((const int32_t *)a8)[tid];
-            __syncthreads();
-            int32_t qv = clamp127(dot256(L + LOFF_Q + (uint64_t)tid * D_MODEL, a_pk) >> 8);
-            int32_t kvv = clamp127(dot256(L + LOFF_K + (uint64_t)tid * D_MODEL, a_pk) >> 8);
-            int32_t vv = clamp127(dot256(L + LOFF_V + (uint64_t)tid * D_MODEL, a_pk) >> 8);
-            q8[tid] = (int8_t)qv;
-            kvl[(uint64_t)t * 2 * D_MODEL + tid] = (int8_t)kvv;
-            kvl[(uint64_t)t * 2 * D_MODEL + D_MODEL + tid] = (int8_t)vv;
-            wl[1 * D_MODEL + tid] = from_i64(qv);
-            wl[2 * D_MODEL + tid] = from_i64(kvv);
-            wl[3 * D_MODEL + tid] = from_i64(vv);
-            __threadfence_block();
-            __syncthreads();
-            // scores s_u = relu((q . k_u) >> 6) for u <= t, per head
-            for (uint32_t e = tid; e < N_HEADS * (t + 1); e += MT) {
-                const uint32_t hd = e / (t + 1), u = e % (t + 1);
-                const int8_t *kr = kvl + (uint64_t)u * 2 * D_MODEL + hd * D_HEAD;
-                const int32_t *kp = (const int32_t *)kr;
-                const int32_t *qp = (const int32_t *)(q8 + hd * D_HEAD);
-                int32_t acc = 0;
-#pragma unroll
-                for (int i = 0; i < D_HEAD / 4; i++) acc = __builtin_amdgcn_sdot4(qp[i], kp[i], acc, false);
-                int32_t s = acc >> 6;
-                scores[hd][u] = s > 0 ? s : 0;
+            if (tid < D_MODEL) {
+                const int32_t av = clamp127(x[tid] >> 2);
+                a8[tid] = (int8_t)av;
+                wl[tid] = from_i64(av);
             }
             __syncthreads();
-            {
-                // element tid of the attention output: head tid / 64
-                const uint32_t hd = tid / D_HEAD;
-                int64_t num = 0, norm = 1;
-                for (uint32_t u = 0; u <= t; u++) {
-                    const int64_t s = scores[hd][u];
-                    num += s * (int64_t)kvl[(uint64_t)u * 2 * D_MODEL + D_MODEL + tid];
-                    norm += s;
+            // q, k, v
+            matvec_partials<3 * D_MODEL, D_MODEL / 4, 4>((const int32_t *)(Lw + DOFF_QKV), a_pk, part, tid);
+            __syncthreads();
+            if (tid < 3 * D_MODEL) {
+                const int32_t y = part[tid] + part[768 + tid] + part[1536 + tid] + part[2304 + tid];
+                const int32_t v = clamp127(y >> 8);
+                wl[D_MODEL + tid] = from_i64(v);  // q, k, v are consecutive in the witness
+                if (tid < D_MODEL) {
+                    q8[tid] = (int8_t)v;
+                } else if (tid < 2 * D_MODEL) {
+                    k8[tid - D_MODEL] = (int8_t)v;
+                } else {
+                    const int i = tid - 2 * D_MODEL;
+                    ((int8_t *)vpl)[((uint64_t)(t / 4) * D_MODEL + i) * 4 + (t & 3)] = (int8_t)v;
                 }
-                int32_t o = clamp127((int32_t)(num / norm));
+            }
+            __syncthreads();
+            if (tid < 64) ktl[(uint64_t)tid * tcap + t] = k_pk[tid];
+            __threadfence_block();
+            __syncthreads();
+            // scores s_u = relu((q . k_u) >> 6), u <= t, per head; zeros up to a multiple of 4
+            const uint32_t n = t + 1, n4 = (n + 3) & ~3u;
+            for (uint32_t e = tid; e < N_HEADS * n; e += MT) {
+                const uint32_t hd = e / n, u = e % n;
+                const int32_t *kc = ktl + (uint64_t)(hd * 16) * tcap + u;
+                int32_t acc = 0;
+#pragma unroll
+                for (int w = 0; w < 16; w++) acc = __builtin_amdgcn_sdot4(q_pk[hd * 16 + w], kc[(uint64_t)w * tcap], acc, false);
+                const int32_t s = acc >> 6;
+                scores[hd][u] = s > 0 ? s : 0;
+            }
+            if (tid < 16) {
+                const uint32_t u = n + (tid & 3);
+                if (u < n4) scores[tid >> 2][u] = 0;
+            }
+            __syncthreads();
+            // o_i = (sum_u s_u v_u(i)) / (1 + sum_u s_u): element i = tid % 256, quarter s = tid / 256
+            {
+                const int i = tid & (D_MODEL - 1), s = tid >> 8, hd = i / D_HEAD;
+                const int32_t *vc = vpl + i;
+                int32_t num = 0, nsum = 0;
+                for (uint32_t wd = s; wd < n4 / 4; wd += 4) {
+                    const int32_t word = vc[(uint64_t)wd * D_MODEL];
+#pragma unroll
+                    for (int b = 0; b < 4; b++) {
+                        const int32_t sc = scores[hd][4 * wd + b];
+                        num += sc * (int32_t)(int8_t)(word >> (8 * b));
+                        nsum += sc;
+                    }
+                }
+                part[tid] = num;
+                partn[tid] = nsum;
+            }
+            __syncthreads();
+            if (tid < D_MODEL) {
+                const int32_t num = part[tid] + part[256 + tid] + part[512 + tid] + part[768 + tid];
+                const int32_t norm = 1 + partn[tid] + partn[256 + tid] + partn[512 + tid] + partn[768 + tid];
+                const int32_t o = clamp127(num / norm);
                 att8[tid] = (int8_t)o;
                 wl[4 * D_MODEL + tid] = from_i64(o);
             }
             __syncthreads();
-            if (tid < D_MODEL / 4) att_pk[tid] = ((const int32_t *)att8)[tid];
+            // x += Wo att; m = rq(x)
+            matvec_partials<D_MODEL, D_MODEL / 4, 4>((const int32_t *)(Lw + DOFF_O), att_pk, part, tid);
             __syncthreads();
-            x[tid] += dot256(L + LOFF_O + (uint64_t)tid * D_MODEL, att_pk);
-            __syncthreads();
-            int32_t mv = clamp127(x[tid] >> 2);
-            a8[tid] = (int8_t)mv;
-            wl[5 * D_MODEL + tid] = from_i64(mv);
-            __syncthreads();
-            if (tid < D_MODEL / 4) a_pk[tid] = ((const int32_t *)a8)[tid];
-            __syncthreads();
-            for (int j = tid; j < D_FF; j += MT) {
-                int32_t hv = dot256(L + LOFF_W1 + (uint64_t)j * D_MODEL, a_pk) >> 8;
-                hv = hv < 0 ? 0 : (hv > 127 ? 127 : hv);
-                h8[j] = (int8_t)hv;
-                wl[6 * D_MODEL + j] = from_i64(hv);
+            if (tid < D_MODEL) {
+                x[tid] += part[tid] + part[256 + tid] + part[512 + tid] + part[768 + tid];
+                const int32_t mv = clamp127(x[tid] >> 2);
+                a8[tid] = (int8_t)mv;
+                wl[5 * D_MODEL + tid] = from_i64(mv);
             }
             __syncthreads();
-            for (int j = tid; j < D_FF / 4; j += MT) h_pk[j] = ((const int32_t *)h8)[j];
+            // h = clamp(relu((W1 m) >> 8), 0, 127)
+            matvec_partials<D_FF, D_MODEL / 4, 4>((const int32_t *)(Lw + DOFF_W1), a_pk, part, tid);
             __syncthreads();
-            x[tid] += dot1024(L + LOFF_W2 + (uint64_t)tid * D_FF, h_pk) >> 8;
-            wl[6 * D_MODEL + D_FF + tid] = from_i64(x[tid]);
+            {
+                int32_t hv = (part[tid] + part[1024 + tid] + part[2048 + tid] + part[3072 + tid]) >> 8;
+                hv = hv < 0 ? 0 : (hv > 127 ? 127 : hv);
+                h8[tid] = (int8_t)hv;
+                wl[6 * D_MODEL + tid] = from_i64(hv);
+            }
+            __syncthreads();
+            // x += (W2 h) >> 8
+            matvec_partials<D_MODEL, D_FF / 4, 16>((const int32_t *)(Lw + DOFF_W2), h_pk, part, tid);
+            __syncthreads();
+            if (tid < D_MODEL) {
+                int32_t y = 0;
+#pragma unroll
+                for (int s = 0; s < 16; s++) y += part[s * 256 + tid];
+                x[tid] += y >> 8;
+                wl[6 * D_MODEL + D_FF + tid] = from_i64(x[tid]);
+            }
             __syncthreads();
         }
-        // logits and the greedy choice
-        int32_t fv = clamp127(x[tid] >> 2);
-        a8[tid] = (int8_t)fv;
-        wt[N_LAYERS * WIT_LAYER + tid] = from_i64(fv);
-        __syncthreads();
-        if (tid < D_MODEL / 4) a_pk[tid] = ((const int32_t *)a8)[tid];
+        // f = rq(x); logits and the greedy choice
+        if (tid < D_MODEL) {
+            const int32_t fv = clamp127(x[tid] >> 2);
+            a8[tid] = (int8_t)fv;
+            wt[N_LAYERS * WIT_LAYER + tid] = from_i64(fv);
+        }
         __syncthreads();
         int32_t best = INT32_MIN;
         uint32_t best_i = 0;
-        for (int j = tid; j < VOCAB; j += MT) {
-            int32_t lg = dot256(wout + (uint64_t)j * D_MODEL, a_pk);
-            if (lg > best) {  // j increases: the first maximum is kept
-                best = lg;
+#pragma unroll 1
+        for (int j = tid; j < VOCAB; j += MT) {  // j increases: the first maximum is kept
+            const int32_t *col = wout + j;
+            int32_t acc = 0;
+#pragma unroll 16
+            for (int w = 0; w < D_MODEL / 4; w++) acc = __builtin_amdgcn_sdot4(col[(uint64_t)w * VOCAB], a_pk[w], acc, false);
+            if (acc > best) {
+                best = acc;
                 best_i = j;
             }
         }
@@ -182,8 +270,8 @@ k_decode(const int8_t *__restrict__ W, const uint32_t *__restrict__ prompt, uint
         __syncthreads();
         for (int s = MT / 2; s > 0; s >>= 1) {
             if (tid < s) {
-                int32_t ov = red_v[tid + s];
-                uint32_t oi = red_i[tid + s];
+                const int32_t ov = red_v[tid + s];
+                const uint32_t oi = red_i[tid + s];
                 if (ov > red_v[tid] || (ov == red_v[tid] && oi < red_i[tid])) {
                     red_v[tid] = ov;
                     red_i[tid] = oi;
@@ -193,8 +281,8 @@ k_decode(const int8_t *__restrict__ W, const uint32_t *__restrict__ prompt, uint
         }
         if (tid == 0) {
             const uint32_t next = red_i[0];
-            wt[N_LAYERS * WIT_LAYER + D_MODEL] = next;
-            cur = (t + 1 < n_prompt) ? prompt[t + 1] : next;
+            wt[WIT_NEXT] = next;
+            cur = (t + 1 < task.n_prompt) ? task.prompt[t + 1] : next;
         }
         __syncthreads();
     }
@@ -206,10 +294,11 @@ Status model_weights(Ctx *ctx, int8_t *w, uint64_t seed, hipStream_t s) {
     return check_launch("model weights");
 }
 
-Status model_decode(Ctx *ctx, const int8_t *w, const uint32_t *prompt, uint32_t n_prompt,
-                    uint32_t total, uint32_t *tokens, int8_t *kv, uint64_t *wit, hipStream_t s) {
+Status model_decode(Ctx *ctx, const int8_t *w, const DecodeTask *tasks, uint32_t count,
+                    hipStream_t s) {
+    if (count == 0) return Status::ok();
     KernelTimer timer(ctx, "k_decode", s);
-    hipLaunchKernelGGL(k_decode, dim3(1), dim3(MT), 0, s, w, prompt, n_prompt, total, tokens, kv, wit);
+    hipLaunchKernelGGL(k_decode, dim3(count), dim3(MT), 0, s, w, tasks);
     return check_launch("decode");
 }
 

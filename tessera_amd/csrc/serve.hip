@@ -1,9 +1,10 @@
-// Served proving with continuous batching: requests of different lengths are admitted as worker
-// slots and device-memory budget free up, and the work of admitted requests overlaps on the GPU.
+// Served proving with continuous batching: requests of different lengths are admitted as slots
+// and device-memory budget free up, decode together in shared launches, and are proved on
+// overlapping streams.
 //
 // No reference counterpart (the reference proves one statement per call and has no serving
-// loop). Each request's pipeline only uses the library's own entry points, so its result is
-// the same alone and in any mix:
+// loop). Each request's pipeline only uses the library's own kernels and entry points, so its
+// result is the same alone and in any mix:
 //   1. decode: the synthetic integer transformer (model.hip) generates the tokens greedily and
 //      writes the witness, Goldilocks elements, WIT_STRIDE per token;
 //   2. commit: the witness as rows of k = 2^13 values (zero-padded), rate 1/4, B = 32, unmasked
@@ -13,9 +14,15 @@
 //      squeeze n challenges "serve/r/{i}";
 //   4. sumcheck: stage/0, degree 3, Sum_x eq(r, x) A(x) B(x) over layer 0's `a` and `m`
 //      activations of every token (zero-padded to 2^n entries), on the device.
-// Workers: `slots` threads, each with its own context (HIP stream), share the model weights.
-// Admission is strict FIFO: the oldest waiting request starts when a slot is free and its
-// device-memory footprint fits beside the footprints of the requests in flight.
+//
+// Engine: one decoder thread owns a context (stream). Each iteration it admits waiting requests
+// (strict FIFO: the oldest starts when fewer than `slots` requests are in flight and its
+// footprint fits the budget beside those in flight), then launches one decode over every
+// decoding request, each advancing by up to `chunk` positions (one workgroup per request), and
+// waits for it. A request whose decode is complete goes to the prover threads (`provers`, each
+// with its own context and stream), which commit and prove it while decoding continues. A
+// request's buffers come from the decoder's pool when it is admitted and return to it when its
+// proof is done; the KV cache returns as soon as its decode ends.
 #include <hip/hip_runtime.h>
 #include <algorithm>
 #include <chrono>
@@ -61,68 +68,72 @@ static uint32_t ceil_log2(uint64_t x) {
     return h;
 }
 
+static uint64_t witness_rows(uint32_t total) {
+    const uint64_t k = 1ull << SERVE_LOG_K;
+    return ((uint64_t)total * WIT_STRIDE + k - 1) / k;
+}
+
 // The device memory one request of `total` tokens needs (its admission footprint).
 uint64_t request_footprint(uint32_t total) {
-    const uint64_t k = 1ull << SERVE_LOG_K, wit = (uint64_t)total * WIT_STRIDE;
-    const uint64_t rows = (wit + k - 1) / k, n = 4 * k;
+    const uint64_t k = 1ull << SERVE_LOG_K, rows = witness_rows(total), n = 4 * k;
     const uint64_t table = 1ull << ceil_log2((uint64_t)total * D_MODEL);
     return 8 * rows * k           // witness rows
            + 8 * rows * n         // codewords
            + 8 * rows * k         // encode scratch
            + 32 * 2 * n           // tree
-           + (uint64_t)total * N_LAYERS * 2 * D_MODEL  // KV cache
+           + 4 * kv_words(total)  // KV cache
            + 3 * 16 * table * 2;  // sumcheck tables and working buffers
 }
 
-struct Job {
-    tsr_request req;
-    std::vector<uint32_t> prompt;
-    double t_submit = 0;
-};
+// A request check: 1..512 tokens, the prompt at least one, every prompt token in the vocabulary
+// (the decoder reads its embedding row without a further check).
+static Status check_request(const tsr_request *req) {
+    if (!req || !req->prompt) return Status::fail(TSR_BAD_ARG, "null request or prompt");
+    const uint64_t total = (uint64_t)req->n_prompt + req->n_output;
+    if (req->n_prompt == 0 || total > MAX_TOKENS)
+        return Status::fail(TSR_BAD_SHAPE, "a request holds 1 to 512 tokens, the prompt at least one");
+    for (uint32_t i = 0; i < req->n_prompt; i++)
+        if (req->prompt[i] >= (uint32_t)VOCAB)
+            return Status::fail(TSR_BAD_SHAPE, "a prompt token outside the vocabulary");
+    return Status::ok();
+}
 
-}  // namespace tsr
-
-struct tsr_server {
-    std::chrono::steady_clock::time_point t0;
-    int device = 0;
-    uint32_t slots = 1;
-    uint64_t budget = 0, in_flight_bytes = 0;
-    tsr_ctx *owner = nullptr;  // holds the weights
-    int8_t *weights = nullptr;
-    std::vector<tsr_ctx *> ctxs;
-    std::vector<std::thread> workers;
-    std::mutex mu;
-    std::mutex prove_mu;  // tsr_server_prove calls share the owner context: one at a time
-    std::condition_variable cv_work, cv_done;
-    std::deque<tsr::Job> queue;
-    std::deque<tsr_result> done;
-    bool stop = false;
-    uint32_t active = 0, max_active = 0;
-};
-
-namespace tsr {
-
-// The whole pipeline of one request on one context; fills `out` (not the timestamps).
-static Status prove_request(Ctx *ctx, const int8_t *weights, const tsr_request &req,
-                            tsr_result &out) {
-    hipStream_t s = ctx->stream;
-    const uint32_t total = req.n_prompt + req.n_output;
-    const uint64_t k = 1ull << SERVE_LOG_K, wit_len = (uint64_t)total * WIT_STRIDE;
-    const uint64_t rows = (wit_len + k - 1) / k;
-    Status st;
-    DevBuf<uint32_t> d_prompt, d_tokens;
-    DevBuf<int8_t> kv;
+// One request's device buffers, from the pool of the context that admitted it.
+struct ReqBufs {
+    DevBuf<uint32_t> prompt, tokens;
+    DevBuf<int32_t> kv;
     DevBuf<uint64_t> wit;
-    if ((st = d_prompt.alloc(ctx, req.n_prompt)) || (st = d_tokens.alloc(ctx, total)) ||
-        (st = kv.alloc(ctx, (uint64_t)N_LAYERS * total * 2 * D_MODEL)) ||
-        (st = wit.alloc(ctx, rows * k)))
-        return st;
-    TSR_HIP(hipMemcpyAsync(d_prompt.ptr, req.prompt, 4ull * req.n_prompt, hipMemcpyHostToDevice, s));
-    hipLaunchKernelGGL(k_fill_zero, dim3((unsigned)((rows * k - wit_len + 255) / 256 + 1)), dim3(256),
-                       0, s, wit.ptr + wit_len, rows * k - wit_len);
-    if ((st = model_decode(ctx, weights, d_prompt.ptr, req.n_prompt, total, d_tokens.ptr, kv.ptr,
-                           wit.ptr, s)))
-        return st;
+    uint32_t total = 0;
+    uint64_t rows = 0;
+
+    // Allocates, uploads the prompt and zeroes the witness padding, on `ctx`'s stream.
+    Status init(Ctx *ctx, const tsr_request &req) {
+        total = req.n_prompt + req.n_output;
+        rows = witness_rows(total);
+        const uint64_t k = 1ull << SERVE_LOG_K, wit_len = (uint64_t)total * WIT_STRIDE;
+        Status st;
+        if ((st = prompt.alloc(ctx, req.n_prompt)) || (st = tokens.alloc(ctx, total)) ||
+            (st = kv.alloc(ctx, kv_words(total))) || (st = wit.alloc(ctx, rows * k)))
+            return st;
+        TSR_HIP(hipMemcpyAsync(prompt.ptr, req.prompt, 4ull * req.n_prompt, hipMemcpyHostToDevice,
+                               ctx->stream));
+        const uint64_t pad = rows * k - wit_len;
+        hipLaunchKernelGGL(k_fill_zero, dim3((unsigned)(pad / 256 + 1)), dim3(256), 0, ctx->stream,
+                           wit.ptr + wit_len, pad);
+        return check_launch("serve buffers");
+    }
+
+    DecodeTask task(const tsr_request &req, uint32_t start, uint32_t end) const {
+        return DecodeTask{prompt.ptr, tokens.ptr, kv.ptr, wit.ptr, req.n_prompt, total, start, end};
+    }
+};
+
+// Steps 2-4 of a decoded request on `ctx` (the witness and tokens may belong to another
+// context's pool: they are only read here). Fills `out` except the timestamps.
+static Status prove_decoded(Ctx *ctx, const tsr_request &req, const ReqBufs &b, tsr_result &out) {
+    hipStream_t s = ctx->stream;
+    const uint32_t total = b.total;
+    Status st;
 
     // commit the witness rows
     tsr_commitment_params params{};
@@ -132,7 +143,7 @@ static Status prove_request(Ctx *ctx, const int8_t *weights, const tsr_request &
     params.row_block = 32;
     memset(params.commitment_id, 0x5e, 32);
     tsr_commitment *c = nullptr;
-    tsr_status cs = tsr_commit_unmasked((tsr_ctx *)ctx, &params, wit.ptr, rows, &c);
+    tsr_status cs = tsr_commit_unmasked((tsr_ctx *)ctx, &params, b.wit.ptr, b.rows, &c);
     if (cs != TSR_OK) return Status::fail(cs, std::string("serve commit: ") + tsr_last_error());
     tsr_commitment_root(c, out.root);
     tsr_commitment_free(c);
@@ -166,9 +177,9 @@ static Status prove_request(Ctx *ctx, const int8_t *weights, const tsr_request &
         return st;
     if (n) TSR_HIP(hipMemcpyAsync(d_r.ptr, r.data(), 16 * n, hipMemcpyHostToDevice, s));
     if ((st = eq_table(ctx, d_r.ptr, n, eq.ptr, s))) return st;
-    hipLaunchKernelGGL(k_gather_base, dim3((unsigned)((len + 255) / 256)), dim3(256), 0, s, wit.ptr,
+    hipLaunchKernelGGL(k_gather_base, dim3((unsigned)((len + 255) / 256)), dim3(256), 0, s, b.wit.ptr,
                        total, WIT_A0, len, A.ptr);
-    hipLaunchKernelGGL(k_gather_base, dim3((unsigned)((len + 255) / 256)), dim3(256), 0, s, wit.ptr,
+    hipLaunchKernelGGL(k_gather_base, dim3((unsigned)((len + 255) / 256)), dim3(256), 0, s, b.wit.ptr,
                        total, WIT_M0, len, B.ptr);
     if ((st = check_launch("serve tables"))) return st;
     tsr_table tabs[3] = {{(const uint64_t *)eq.ptr, 1, 1}, {A.ptr, 0, 0}, {B.ptr, 0, 0}};
@@ -179,11 +190,11 @@ static Status prove_request(Ctx *ctx, const int8_t *weights, const tsr_request &
 
     memcpy(out.transcript_state, tr.state, 32);
     out.transcript_counter = tr.counter;
-    out.witness_rows = rows;
+    out.witness_rows = b.rows;
     out.sumcheck_vars = n;
     out.n_tokens = total;
     std::vector<uint32_t> toks(total);
-    TSR_HIP(hipMemcpyAsync(toks.data(), d_tokens.ptr, 4ull * total, hipMemcpyDeviceToHost, s));
+    TSR_HIP(hipMemcpyAsync(toks.data(), b.tokens.ptr, 4ull * total, hipMemcpyDeviceToHost, s));
     TSR_HIP(hipStreamSynchronize(s));
     for (uint32_t i = 0; i < total && i < TSR_MAX_TOKENS; i++) out.tokens[i] = toks[i];
     out.final_a[0] = so.finals.size() > 0 ? so.finals[0].c0 : 0;
@@ -191,62 +202,155 @@ static Status prove_request(Ctx *ctx, const int8_t *weights, const tsr_request &
     return Status::ok();
 }
 
-// A request the pipeline can run: 1..512 tokens, the prompt at least one, every prompt token in
-// the vocabulary (the decoder reads its embedding row without a further check).
-static Status check_request(const tsr_request *req) {
-    if (!req || !req->prompt) return Status::fail(TSR_BAD_ARG, "null request or prompt");
-    const uint64_t total = (uint64_t)req->n_prompt + req->n_output;
-    if (req->n_prompt == 0 || total > MAX_TOKENS)
-        return Status::fail(TSR_BAD_SHAPE, "a request holds 1 to 512 tokens, the prompt at least one");
-    for (uint32_t i = 0; i < req->n_prompt; i++)
-        if (req->prompt[i] >= (uint32_t)VOCAB)
-            return Status::fail(TSR_BAD_SHAPE, "a prompt token outside the vocabulary");
-    return Status::ok();
+struct Job {
+    tsr_request req;
+    std::vector<uint32_t> prompt;
+    double t_submit = 0, t_admit = 0;
+    uint64_t foot = 0;
+    uint32_t pos = 0;  // decoded positions
+    ReqBufs bufs;
+    Status status;
+};
+
+}  // namespace tsr
+
+struct tsr_server {
+    std::chrono::steady_clock::time_point t0;
+    int device = 0;
+    uint32_t slots = 1, provers = 1, chunk = 32;
+    uint64_t budget = 0, in_flight_bytes = 0;
+    tsr_ctx *owner = nullptr;  // holds the weights; runs tsr_server_prove
+    tsr_ctx *dec = nullptr;    // the decoder's context: its pool holds the requests' buffers
+    int8_t *weights = nullptr;
+    std::vector<tsr_ctx *> prover_ctxs;
+    std::thread decoder;
+    std::vector<std::thread> workers;
+    std::mutex mu;
+    std::mutex prove_mu;  // tsr_server_prove calls share the owner context: one at a time
+    std::condition_variable cv_work, cv_prove, cv_done;
+    std::deque<std::unique_ptr<tsr::Job>> queue;    // waiting for admission
+    std::deque<std::unique_ptr<tsr::Job>> decoded;  // waiting for a prover
+    std::deque<tsr_result> done;
+    bool stop = false, decoder_done = false;
+    uint32_t in_flight = 0, max_active = 0;
+};
+
+namespace tsr {
+
+// Ends a job: its result goes to the done queue and its budget and slot are released.
+static void finish(tsr_server *srv, Job &job, const tsr_result *filled) {
+    tsr_result res{};
+    if (filled) res = *filled;
+    res.id = job.req.id;
+    res.status = job.status.code;
+    res.t_submit = job.t_submit;
+    res.t_admit = job.t_admit;
+    res.t_done = now_s(srv->t0);
+    job.bufs = ReqBufs();  // buffers back to the decoder's pool
+    {
+        std::lock_guard<std::mutex> lk(srv->mu);
+        srv->in_flight_bytes -= job.foot;
+        srv->in_flight--;
+        srv->done.push_back(res);
+    }
+    srv->cv_done.notify_all();
+    srv->cv_work.notify_all();
 }
 
-static void worker(tsr_server *srv, uint32_t slot) {
-    Ctx *ctx = (Ctx *)srv->ctxs[slot];
+static void decoder_loop(tsr_server *srv) {
+    Ctx *ctx = (Ctx *)srv->dec;
     (void)hipSetDevice(srv->device);
+    std::vector<std::unique_ptr<Job>> active;
+    std::vector<DecodeTask> host_tasks;
+    DevBuf<DecodeTask> dev_tasks;
+    const Status tasks_st = dev_tasks.alloc(ctx, srv->slots);
     for (;;) {
-        Job job;
-        uint64_t foot = 0;
-        double t_admit = 0;
+        std::vector<std::unique_ptr<Job>> admitted;
         {
             std::unique_lock<std::mutex> lk(srv->mu);
-            srv->cv_work.wait(lk, [&] {
-                if (srv->stop) return true;
-                if (srv->queue.empty()) return false;
-                const Job &front = srv->queue.front();
-                uint64_t f = request_footprint(front.req.n_prompt + front.req.n_output);
-                return srv->budget == 0 || srv->in_flight_bytes + f <= srv->budget ||
-                       srv->active == 0;
-            });
-            if (srv->stop && srv->queue.empty()) return;
-            job = std::move(srv->queue.front());
-            srv->queue.pop_front();
-            foot = request_footprint(job.req.n_prompt + job.req.n_output);
-            srv->in_flight_bytes += foot;
-            srv->active++;
-            srv->max_active = std::max(srv->max_active, srv->active);
-            t_admit = now_s(srv->t0);
+            auto admissible = [&] {
+                if (srv->queue.empty() || srv->in_flight >= srv->slots) return false;
+                const uint64_t f = srv->queue.front()->foot;
+                return srv->budget == 0 || srv->in_flight_bytes + f <= srv->budget || srv->in_flight == 0;
+            };
+            if (active.empty())
+                srv->cv_work.wait(lk, [&] { return admissible() || (srv->stop && srv->queue.empty()); });
+            while (admissible()) {
+                std::unique_ptr<Job> job = std::move(srv->queue.front());
+                srv->queue.pop_front();
+                srv->in_flight_bytes += job->foot;
+                srv->in_flight++;
+                srv->max_active = std::max(srv->max_active, srv->in_flight);
+                job->t_admit = now_s(srv->t0);
+                admitted.push_back(std::move(job));
+            }
+            if (active.empty() && admitted.empty() && srv->stop && srv->queue.empty()) break;
+        }
+        for (auto &job : admitted) {
+            job->req.prompt = job->prompt.data();
+            job->status = tasks_st ? tasks_st : job->bufs.init(ctx, job->req);
+            if (job->status) {
+                finish(srv, *job, nullptr);
+                continue;
+            }
+            active.push_back(std::move(job));
+        }
+        if (active.empty()) continue;
+        host_tasks.clear();
+        for (auto &job : active) {
+            const uint32_t end = std::min(job->pos + srv->chunk, job->bufs.total);
+            host_tasks.push_back(job->bufs.task(job->req, job->pos, end));
+        }
+        Status ls;
+        hipError_t e = hipMemcpyAsync(dev_tasks.ptr, host_tasks.data(), sizeof(DecodeTask) * host_tasks.size(),
+                                      hipMemcpyHostToDevice, ctx->stream);
+        if (e != hipSuccess) ls = hip_fail(e, "decode tasks");
+        if (!ls) ls = model_decode(ctx, srv->weights, dev_tasks.ptr, (uint32_t)host_tasks.size(), ctx->stream);
+        if (!ls && (e = hipStreamSynchronize(ctx->stream)) != hipSuccess) ls = hip_fail(e, "decode");
+        std::vector<std::unique_ptr<Job>> still;
+        for (size_t i = 0; i < active.size(); i++) {
+            std::unique_ptr<Job> &job = active[i];
+            job->pos = host_tasks[i].end;
+            if (ls) {
+                job->status = ls;
+                finish(srv, *job, nullptr);
+            } else if (job->pos == job->bufs.total) {
+                job->bufs.kv = DevBuf<int32_t>();  // the KV cache is done with
+                {
+                    std::lock_guard<std::mutex> lk(srv->mu);
+                    srv->decoded.push_back(std::move(job));
+                }
+                srv->cv_prove.notify_one();
+            } else {
+                still.push_back(std::move(job));
+            }
+        }
+        active.swap(still);
+    }
+    {
+        std::lock_guard<std::mutex> lk(srv->mu);
+        srv->decoder_done = true;
+    }
+    srv->cv_prove.notify_all();
+}
+
+static void prover_loop(tsr_server *srv, uint32_t slot) {
+    Ctx *ctx = (Ctx *)srv->prover_ctxs[slot];
+    (void)hipSetDevice(srv->device);
+    for (;;) {
+        std::unique_ptr<Job> job;
+        {
+            std::unique_lock<std::mutex> lk(srv->mu);
+            srv->cv_prove.wait(lk, [&] { return !srv->decoded.empty() || srv->decoder_done; });
+            if (srv->decoded.empty()) return;
+            job = std::move(srv->decoded.front());
+            srv->decoded.pop_front();
         }
         tsr_result res{};
-        res.id = job.req.id;
-        res.t_submit = job.t_submit;
-        res.t_admit = t_admit;
         res.slot = slot;
-        job.req.prompt = job.prompt.data();
-        Status st = prove_request(ctx, srv->weights, job.req, res);
-        res.status = st.code;
-        res.t_done = now_s(srv->t0);
-        {
-            std::lock_guard<std::mutex> lk(srv->mu);
-            srv->in_flight_bytes -= foot;
-            srv->active--;
-            srv->done.push_back(res);
-        }
-        srv->cv_done.notify_all();
-        srv->cv_work.notify_all();
+        job->status = prove_decoded(ctx, job->req, job->bufs, res);
+        (void)hipStreamSynchronize(ctx->stream);  // (after an error) nothing still reads the buffers
+        finish(srv, *job, &res);
     }
 }
 
@@ -258,13 +362,16 @@ extern "C" {
 
 tsr_status tsr_server_create(int device, const tsr_server_config *cfg, tsr_server **out) {
     if (!cfg || !out) return Status::fail(TSR_BAD_ARG, "null argument").code;
-    if (cfg->slots == 0 || cfg->slots > 64)
-        return Status::fail(TSR_BAD_SHAPE, "slots must be 1..64").code;
-    struct Cleanup {  // releases what a failed create made (no workers run yet)
+    if (cfg->slots == 0 || cfg->slots > 1024)
+        return Status::fail(TSR_BAD_SHAPE, "slots must be 1..1024").code;
+    if (cfg->provers > 64) return Status::fail(TSR_BAD_SHAPE, "provers must be 0..64").code;
+    if (cfg->chunk > MAX_TOKENS) return Status::fail(TSR_BAD_SHAPE, "chunk must be 0..512").code;
+    struct Cleanup {  // releases what a failed create made (no threads run yet)
         tsr_server *s;
         ~Cleanup() {
             if (!s) return;
-            for (auto *c : s->ctxs) tsr_ctx_destroy(c);
+            for (auto *c : s->prover_ctxs) tsr_ctx_destroy(c);
+            if (s->dec) tsr_ctx_destroy(s->dec);
             if (s->weights) tsr_free(s->owner, s->weights);
             if (s->owner) tsr_ctx_destroy(s->owner);
         }
@@ -274,6 +381,8 @@ tsr_status tsr_server_create(int device, const tsr_server_config *cfg, tsr_serve
     srv->t0 = std::chrono::steady_clock::now();
     srv->device = device;
     srv->slots = cfg->slots;
+    srv->provers = cfg->provers ? cfg->provers : 4;
+    srv->chunk = cfg->chunk ? cfg->chunk : 32;
     srv->budget = cfg->hbm_budget;
     tsr_status s = tsr_ctx_create(device, &srv->owner);
     if (s) return s;
@@ -284,12 +393,14 @@ tsr_status tsr_server_create(int device, const tsr_server_config *cfg, tsr_serve
     Status st = model_weights(oc, srv->weights, cfg->model_seed, oc->stream);
     if (st) return st.code;
     if ((s = tsr_ctx_synchronize(srv->owner))) return s;
-    for (uint32_t i = 0; i < cfg->slots; i++) {
+    if ((s = tsr_ctx_create(device, &srv->dec))) return s;
+    for (uint32_t i = 0; i < srv->provers; i++) {
         tsr_ctx *c = nullptr;
         if ((s = tsr_ctx_create(device, &c))) return s;
-        srv->ctxs.push_back(c);
+        srv->prover_ctxs.push_back(c);
     }
-    for (uint32_t i = 0; i < cfg->slots; i++) srv->workers.emplace_back(worker, srv.get(), i);
+    srv->decoder = std::thread(decoder_loop, srv.get());
+    for (uint32_t i = 0; i < srv->provers; i++) srv->workers.emplace_back(prover_loop, srv.get(), i);
     guard.s = nullptr;
     *out = srv.release();
     return TSR_OK;
@@ -298,12 +409,13 @@ tsr_status tsr_server_create(int device, const tsr_server_config *cfg, tsr_serve
 tsr_status tsr_server_submit(tsr_server *srv, const tsr_request *req) {
     if (!srv) return Status::fail(TSR_BAD_ARG, "null server").code;
     if (Status st = check_request(req)) return st.code;
-    Job job;
-    job.req = *req;
-    job.prompt.assign(req->prompt, req->prompt + req->n_prompt);
+    std::unique_ptr<Job> job(new Job());
+    job->req = *req;
+    job->prompt.assign(req->prompt, req->prompt + req->n_prompt);
+    job->foot = request_footprint(req->n_prompt + req->n_output);
     {
         std::lock_guard<std::mutex> lk(srv->mu);
-        job.t_submit = now_s(srv->t0);
+        job->t_submit = now_s(srv->t0);
         srv->queue.push_back(std::move(job));
     }
     srv->cv_work.notify_all();
@@ -338,7 +450,20 @@ tsr_status tsr_server_prove(tsr_server *srv, const tsr_request *req, tsr_result 
     hipError_t e = hipSetDevice(srv->device);
     if (e != hipSuccess) return hip_fail(e, "hipSetDevice").code;
     out->t_submit = out->t_admit = now_s(srv->t0);
-    Status st = prove_request(ctx, srv->weights, *req, *out);
+    ReqBufs b;
+    Status st = b.init(ctx, *req);
+    DevBuf<DecodeTask> task;
+    if (!st) st = task.alloc(ctx, 1);
+    if (!st) {
+        const DecodeTask t = b.task(*req, 0, b.total);
+        e = hipMemcpyAsync(task.ptr, &t, sizeof t, hipMemcpyHostToDevice, ctx->stream);
+        if (e != hipSuccess) st = hip_fail(e, "decode task");
+    }
+    if (!st) st = model_decode(ctx, srv->weights, task.ptr, 1, ctx->stream);
+    if (!st) {
+        b.kv = DevBuf<int32_t>();
+        st = prove_decoded(ctx, *req, b, *out);
+    }
     out->t_done = now_s(srv->t0);
     out->status = st.code;
     return st.code;
@@ -355,8 +480,10 @@ void tsr_server_destroy(tsr_server *srv) {
         srv->stop = true;
     }
     srv->cv_work.notify_all();
+    if (srv->decoder.joinable()) srv->decoder.join();
     for (auto &w : srv->workers) w.join();
-    for (auto *c : srv->ctxs) tsr_ctx_destroy(c);
+    for (auto *c : srv->prover_ctxs) tsr_ctx_destroy(c);
+    tsr_ctx_destroy(srv->dec);
     if (srv->weights) tsr_free(srv->owner, srv->weights);
     tsr_ctx_destroy(srv->owner);
     delete srv;

@@ -28,7 +28,8 @@ class Result(ctypes.Structure):
 
 
 class ServerConfig(ctypes.Structure):
-    _fields_ = [("slots", u32), ("hbm_budget", u64), ("model_seed", u64)]
+    _fields_ = [("slots", u32), ("provers", u32), ("chunk", u32), ("hbm_budget", u64),
+                ("model_seed", u64)]
 
 
 def _declare():
@@ -57,13 +58,12 @@ def footprint(tokens):
 
 
 class Server:
-    def __init__(self, device=0, slots=4, hbm_budget=0, model_seed=0x7E55E7A):
+    def __init__(self, device=0, slots=64, hbm_budget=0, model_seed=0x7E55E7A, provers=4, chunk=32):
         L = _declare()
-        cfg = ServerConfig(slots, hbm_budget, model_seed)
+        cfg = ServerConfig(slots, provers, chunk, hbm_budget, model_seed)
         h = vp()
         check(L.tsr_server_create(device, ctypes.byref(cfg), ctypes.byref(h)))
         self.handle = h.value
-        self._keep = {}
 
     @staticmethod
     def _request(rid, prompt, n_output):

@@ -1,24 +1,35 @@
 #!/usr/bin/env python3
-"""Benchmark: Goldilocks elements committed per second on one MI355X.
+"""Benchmark: Goldilocks elements committed per second on one MI355X, and served proofs/s with
+p50/p99 latency against plain per-request GPU calls.
 
-Workload (one step): the large single commitment — 256 rows of k = 2^20 uniform Goldilocks
-elements (2 GiB, resident in HBM before timing), Reed-Solomon encoded at rate 1/4 to n = 2^22
-symbols per row (8 GiB of codewords), BLAKE3 column chains over all 256 rows in blocks of B = 32
-(2^22 leaves), the Merkle root, then the root absorbed under `bench/root`, 64 column queries
+Headline workload (one step): the large single commitment — 256 rows of k = 2^20 uniform
+Goldilocks elements (2 GiB, resident in HBM before timing), Reed-Solomon encoded at rate 1/4 to
+n = 2^22 symbols per row (8 GiB of codewords), BLAKE3 column chains over all 256 rows in blocks of
+B = 32 (2^22 leaves), the Merkle root, then the root absorbed under `bench/root`, 64 column queries
 drawn with challenge_indices("bench/queries", 22, 64) and answered (columns + multiproof).
 value = 256 * 2^20 input elements per step / seconds per step.
 
-Also reported: per-kernel times from HIP events recorded on the library's stream over the timed
-steps (roofline of the dominant kernel), and the CPU baseline (the oracle's C restatement of the
-reference's CPU algorithms, all host threads) on a bounded sample of the same workload.
+Also on the same JSON line:
+  roofline       the dominant kernel of the headline step (HIP events on the library's stream)
+  cpu_baseline   the oracle's C restatement of the reference's CPU algorithms, all host threads,
+                 on a bounded sample of the headline workload
+  plain          256 x 2^14 commit + 14-variable Sum a*b sumcheck (the small plain drop-in)
+  sumcheck       24-variable degree-3 Sum eq(r,x) a b; LogUp of 2^22 Zipf(1.1) lookups into 2^16 rows
+  batched        64 matrices (log2 k ~ U[10,20], N ~ U[16,512]) committed by one planned,
+                 autotuned, graph-captured batch vs 64 sequential plain calls; roots compared
+  served         Poisson arrivals, prompt+output lengths U[16,512], continuous batching vs the
+                 plain per-request call at the same arrival times; results compared
 
-python bench.py [--steps K] [--warmup W] [--gpus 1] [--no-cpu-baseline]
+python bench.py [--steps K] [--warmup W] [--gpus 1] [--no-cpu-baseline] [--only headline]
 """
 import argparse
+import ctypes
 import json
 import os
 import sys
 import time
+
+import numpy as np
 
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
@@ -26,6 +37,11 @@ sys.path.insert(0, ROOT)
 LOG_K, LOG_R, ROWS, BLOCK, QUERIES = 20, 2, 256, 32, 64
 SEED = 0x7E55E7A
 HBM_PEAK_GBS = 8000.0
+SERVE_RATE_HZ, SERVE_REQUESTS = 400.0, 256
+
+
+def log(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
 def kernel_bytes(name, rows, log_k, log_r):
@@ -40,14 +56,23 @@ def kernel_bytes(name, rows, log_k, log_r):
     }.get(name)
 
 
-def run_gpu(steps, warmup):
+def timed(fn, reps):
+    """Median seconds of `reps` calls of fn (each call ends synchronized)."""
+    ts = []
+    for _ in range(reps):
+        t0 = time.perf_counter()
+        fn()
+        ts.append(time.perf_counter() - t0)
+    return float(np.median(ts))
+
+
+def run_headline(steps, warmup):
     import tessera_amd as T
     from tessera_amd.lib import lib
     from tessera_amd.transcript import Transcript, session_context
-    import ctypes
 
     dev = T.Device(0)
-    k, n = 1 << LOG_K, 1 << (LOG_K + LOG_R)
+    k = 1 << LOG_K
     src = dev.buffer(ROWS * k).fill_uniform(SEED)
     params = T.CommitmentParamsPy(LOG_K, LOG_R, 0, BLOCK, bytes([7] * 32))
     ctx_tce = session_context()
@@ -88,6 +113,169 @@ def run_gpu(steps, warmup):
     return (t1 - t0) / steps, kernels, root.hex(), len(ans.siblings)
 
 
+def run_plain(reps=5):
+    """256 rows x k = 2^14, rate 1/4, B = 32, the root absorbed under bench/root, then stage/0
+    Sum a*b over two uniform tables of 2^14 (degree 2), from a fresh transcript each proof."""
+    import tessera_amd as T
+    from tessera_amd.sumcheck import Table, prove_batch
+    from tessera_amd.transcript import Transcript, session_context
+    dev = T.Device(0)
+    rows = dev.buffer(256 << 14).fill_uniform(SEED + 1)
+    a = dev.buffer(1 << 14).fill_uniform(SEED + 2)
+    b = dev.buffer(1 << 14).fill_uniform(SEED + 3)
+    params = T.CommitmentParamsPy(14, 2, 0, 32, bytes([7] * 32))
+    ctx_tce = session_context()
+    out = {}
+
+    def proof():
+        c = T.ReferenceCommitment.commit_unmasked(dev, params, rows)
+        t = Transcript(ctx_tce)
+        t.absorb("bench/root", c.root())
+        c.free()
+        prove_batch(dev, t, "stage/0", 2, [[Table(a), Table(b)]])
+        out["state"] = t.state.hex()
+
+    proof()
+    sec = timed(proof, reps)
+    dev.close()
+    return {"workload": "256 x 2^14 commit (rate 1/4, B 32) + 14-variable Sum a*b sumcheck",
+            "ms_per_proof": round(1e3 * sec, 3), "elems_per_s": round((256 << 14) / sec, 1),
+            "transcript_state": out["state"]}
+
+
+def run_sumcheck(reps=3):
+    """(i) stage/0 Sum eq(r,x) a(x) b(x), n = 24, degree 3, eq public (built on the device from
+    r = 24 challenge_fp2("bench/r/{i}")), a and b uniform and resident; (ii) prove_logup of 2^22
+    lookups Zipf(1.1) into a 2^16-row table with values, multiplicities counted on the device."""
+    import tessera_amd as T
+    from tessera_amd.logup import LogUpInstance, multiplicities, prove_logup
+    from tessera_amd.sumcheck import Table, eq_table, prove_batch
+    from tessera_amd.transcript import Transcript, session_context
+    from tests.helpers import logup_config3
+    dev = T.Device(0)
+    n = 24
+    a = dev.buffer(1 << n).fill_uniform(SEED + 4)
+    b = dev.buffer(1 << n).fill_uniform(SEED + 5)
+    eq = dev.buffer(2 << n)
+    ctx_tce = session_context()
+
+    def prod3():
+        t = Transcript(ctx_tce)
+        r = [t.challenge_fp2(f"bench/r/{i}") for i in range(n)]
+        eq_table(dev, r, out=eq)
+        prove_batch(dev, t, "stage/0", 3, [[Table(eq, ext=True).public(), Table(a), Table(b)]])
+
+    prod3()
+    s_prod = timed(prod3, reps)
+    a.free(); b.free(); eq.free()
+
+    keys, vals, idx = logup_config3()
+    d_keys, d_vals = dev.upload(keys), dev.upload(vals)
+    d_lk, d_lv = dev.upload(keys[idx]), dev.upload(vals[idx])
+    d_idx = dev.upload(idx.astype(np.uint64))
+
+    def lookups():
+        m = multiplicities(dev, keys.size, d_idx)
+        t = Transcript(ctx_tce)
+        prove_logup(dev, t, [LogUpInstance(d_keys, d_vals, d_lk, d_lv, m)])
+
+    lookups()
+    s_logup = timed(lookups, reps)
+    dev.close()
+    return {"eq_a_b_n24_ms": round(1e3 * s_prod, 3),
+            "eq_a_b_ns_per_index": round(1e9 * s_prod / (1 << n), 4),
+            "logup_2p22_into_2p16_ms": round(1e3 * s_logup, 3),
+            "logup_ns_per_lookup": round(1e9 * s_logup / (1 << 22), 4)}
+
+
+def run_batched(reps=3):
+    """64 matrices (the batched-commit workload, seed 0xBA7C), uniform entries generated on the
+    device, rate 1/4, B = 32: one CommitBatch (plan, autotune, graph capture at creation; each
+    run replays the graph) vs 64 sequential commit_unmasked calls; every root compared."""
+    import tessera_amd as T
+    from tessera_amd.batch import CommitBatch
+    from tests.helpers import batched_shapes
+    dev = T.Device(0)
+    shapes = batched_shapes()
+    bufs = [dev.buffer(n << lk).fill_uniform(SEED + 100 + i) for i, (lk, n) in enumerate(shapes)]
+    total = sum(n << lk for lk, n in shapes)
+    cid = bytes([7] * 32)
+    mats = [(buf, lk, 2, 32, cid) for buf, (lk, n) in zip(bufs, shapes)]
+    t0 = time.perf_counter()
+    batch = CommitBatch(dev, mats, autotune=True, graph=True)
+    t_create = time.perf_counter() - t0
+    batch.run()
+    dev.synchronize()
+    s_batch = timed(lambda: (batch.run(), dev.synchronize()), reps)
+    roots_b = batch.roots()
+    tuning = batch.tuning()
+    batch.free()
+
+    def sequential():
+        out = []
+        for buf, lk, _, _, _ in mats:
+            c = T.ReferenceCommitment.commit_unmasked(dev, T.CommitmentParamsPy(lk, 2, 0, 32, cid), buf)
+            out.append(c.root())
+            c.free()
+        return out
+
+    roots_s = sequential()
+    s_seq = timed(sequential, reps)
+    dev.close()
+    return {"matrices": len(shapes), "elements": total,
+            "batched_ms": round(1e3 * s_batch, 3), "sequential_ms": round(1e3 * s_seq, 3),
+            "batched_elems_per_s": round(total / s_batch, 1),
+            "sequential_elems_per_s": round(total / s_seq, 1),
+            "speedup": round(s_seq / s_batch, 3), "roots_equal": roots_b == roots_s,
+            "create_ms": round(1e3 * t_create, 1),
+            "tiles": sorted({t[2] for t in tuning}) if tuning else []}
+
+
+def run_served(cpu_sample):
+    """SERVE_REQUESTS requests, Poisson arrivals at SERVE_RATE_HZ, prompt+output lengths
+    U[16,512] (prompt a quarter): the server (continuous batching, 64 slots, 4 prover streams,
+    chunks of 32 positions) vs plain per-request calls on the same arrivals (each starts at the
+    later of its arrival and the previous call's end). Latency = completion - arrival."""
+    from tessera_amd.serve import Server, report, run_plain as plain_calls, run_served as served_calls, workload
+    reqs = workload(SERVE_REQUESTS, seed=0x5E7E, rate_hz=SERVE_RATE_HZ)
+    with Server(slots=64, provers=4, chunk=32, model_seed=SEED) as srv:
+        warm = workload(8, seed=1, rate_hz=1e6)
+        served_calls(srv, warm)
+        res_s, wall_s, t0 = served_calls(srv, reqs)
+        served = report(res_s, reqs, wall_s, t0)
+        served["max_in_flight"] = srv.stats()[0]
+        res_p, lat_p, wall_p = plain_calls(srv, reqs)
+    p50, p99 = np.percentile(lat_p, 50), np.percentile(lat_p, 99)
+    service = [r.t_done - r.t_admit for r in res_p.values()]
+    plain = {"proofs_per_s": len(res_p) / wall_p, "p50_ms": 1e3 * p50, "p99_ms": 1e3 * p99,
+             "service_p50_ms": 1e3 * float(np.median(service)), "wall_s": wall_p}
+    equal = all(res_s[i].status == 0 and res_s[i].key() == res_p[i].key() for i in res_p)
+    out = {"rate_hz": SERVE_RATE_HZ, "requests": len(reqs),
+           "served": {k: round(v, 3) if isinstance(v, float) else v for k, v in served.items()},
+           "plain": {k: round(v, 3) for k, v in plain.items()},
+           "speedup_proofs_per_s": round(served["proofs_per_s"] / plain["proofs_per_s"], 3),
+           "results_equal": bool(equal)}
+    if cpu_sample:
+        out["cpu"] = cpu_served_sample(reqs[:3])
+    return out
+
+
+def cpu_served_sample(reqs):
+    """The oracle (C: decode, commit, sumcheck; all host threads) on the first requests of the
+    same workload: proofs/s for one request at a time."""
+    from oracle import oracle as O
+    threads = os.cpu_count() or 1
+    w = O.model_weights(SEED)
+    t0 = time.perf_counter()
+    tokens = 0
+    for rid, prompt, n_out, _ in reqs:
+        O.served_proof(w, rid, prompt, n_out, threads=threads)
+        tokens += len(prompt) + n_out
+    sec = time.perf_counter() - t0
+    return {"proofs_per_s": round(len(reqs) / sec, 4), "cores": threads, "kind": "port",
+            "sample": f"{len(reqs)} requests, {tokens} tokens, {sec:.1f} s"}
+
+
 def traffic_from_profiles(kernel):
     """HBM bytes per launch measured by rocprofv3 --pmc (FETCH_SIZE + WRITE_SIZE, scaled by a
     copy kernel's reading), when a committed summary holds it; else None."""
@@ -103,7 +291,6 @@ def cpu_baseline():
     """The oracle (C, all host threads) on a bounded sample of the same workload, scaled per unit:
     encode 8 of the 256 rows (per-row cost is uniform), chain 2^16 of the 2^22 columns over all 256
     rows (per-column cost is uniform), and a Merkle tree over 2^16 leaves (per-node cost uniform)."""
-    import numpy as np
     from oracle import oracle as O
     from tests.helpers import splitmix_uniform
 
@@ -141,13 +328,16 @@ def main():
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--only", choices=["headline"], default=None,
+                    help="time the headline commit only")
     a = ap.parse_args()
     if a.gpus != 1:
         print("bench.py measures one GPU: this build runs on a single MI355X; "
               "multi-GPU sharding is described in DESIGN.md, not run", file=sys.stderr)
         sys.exit(2)
 
-    sec, kernels, root, n_sib = run_gpu(a.steps, a.warmup)
+    log("headline: 256 x 2^20 commit + 64 openings")
+    sec, kernels, root, n_sib = run_headline(a.steps, a.warmup)
     elems = ROWS * (1 << LOG_K)
     value = elems / sec
     dom = max(kernels, key=lambda x: kernels[x]["total_ms"]) if kernels else None
@@ -159,7 +349,20 @@ def main():
                     "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
                     "traffic": traffic_from_profiles(dom), "algorithmic_bytes": b,
                     "avg_ms": round(kernels[dom]["avg_ms"], 4)}
-    cpu = None if a.no_cpu_baseline else cpu_baseline()
+    extra = {}
+    if a.only is None:
+        log("plain: 256 x 2^14 commit + Sum a*b")
+        extra["plain"] = run_plain()
+        log("sumcheck: n=24 eq*a*b, LogUp 2^22 lookups")
+        extra["sumcheck"] = run_sumcheck()
+        log("batched: 64 matrices")
+        extra["batched"] = run_batched()
+        log("served: Poisson arrivals")
+        extra["served"] = run_served(cpu_sample=not a.no_cpu_baseline)
+    cpu = None
+    if not a.no_cpu_baseline:
+        log("cpu baseline")
+        cpu = cpu_baseline()
     line = {
         "metric": "Goldilocks elems committed/s",
         "value": round(value, 1),
@@ -182,6 +385,7 @@ def main():
         "roofline": roofline,
         "cpu_baseline": cpu,
     }
+    line.update(extra)
     print(json.dumps(line), flush=True)
 
 

@@ -284,6 +284,11 @@ tsr_status tsr_logup_prove(tsr_ctx *ctx, tsr_transcript *t, const tsr_logup_inst
 tsr_status tsr_logup_multiplicities(tsr_ctx *ctx, uint64_t rows, const uint64_t *lookup_rows,
                                     uint64_t lookups, uint64_t *out);
 
+/* eq(r, .) over n variables (crates/tessera-poly/src/eq.rs:23: entry x = prod_i (r_i x_i +
+ * (1 - r_i)(1 - x_i)), variables LSB-first): `point` n Fp2 on the host, `out` 2^n Fp2 (device or
+ * host). BAD_SHAPE for n > 30. */
+tsr_status tsr_eq_table(tsr_ctx *ctx, const uint64_t *point, uint32_t n, uint64_t *out);
+
 /* ---- per-kernel timing with HIP events on the context's stream (the reference's CUDA-event
  *      harness, docs/perf/v0.1/cc-h100.md:30-40) ---- */
 /* enable = 1 starts recording an event pair around every launch; 0 stops; 2 clears. */

@@ -503,3 +503,35 @@ Status eq_table(Ctx *ctx, const gl::fp2 *point, uint32_t n, gl::fp2 *out, hipStr
     return check_launch("eq table");
 }
 }  // namespace tsr
+
+extern "C" tsr_status tsr_eq_table(tsr_ctx *ctx_, const uint64_t *point, uint32_t n, uint64_t *out) {
+    using namespace tsr;
+    Ctx *ctx = (Ctx *)ctx_;
+    if (!ctx || !out || (n && !point)) return Status::fail(TSR_BAD_ARG, "null argument").code;
+    if (n > 30) return Status::fail(TSR_BAD_SHAPE, "eq table of more than 30 variables").code;
+    hipError_t e = hipSetDevice(ctx->device);
+    if (e != hipSuccess) return hip_fail(e, "hipSetDevice").code;
+    hipStream_t s = ctx->stream;
+    const uint64_t len = 1ull << n;
+    DevBuf<gl::fp2> d_point, tmp;
+    Status st;
+    if ((st = d_point.alloc(ctx, n ? n : 1))) return st.code;
+    if (n) {
+        e = hipMemcpyAsync(d_point.ptr, point, 16ull * n, hipMemcpyHostToDevice, s);
+        if (e != hipSuccess) return hip_fail(e, "copy").code;
+    }
+    gl::fp2 *dst = (gl::fp2 *)out;
+    const bool dev_out = is_device_ptr(out);
+    if (!dev_out) {
+        if ((st = tmp.alloc(ctx, len))) return st.code;
+        dst = tmp.ptr;
+    }
+    if ((st = eq_table(ctx, d_point.ptr, n, dst, s))) return st.code;
+    if (!dev_out) {
+        e = hipMemcpyAsync(out, tmp.ptr, 16 * len, hipMemcpyDeviceToHost, s);
+        if (e != hipSuccess) return hip_fail(e, "copy").code;
+    }
+    e = hipStreamSynchronize(s);
+    if (e != hipSuccess) return hip_fail(e, "sync").code;
+    return TSR_OK;
+}

@@ -96,3 +96,14 @@ def prove_batch(device, transcript, prefix: str, degree: int, instances) -> Prov
                        rounds[:n_max * degree * 2].reshape(n_max, degree, 2),
                        chals[:n_max * 2].reshape(n_max, 2),
                        finals[:nf.value * 2].reshape(nf.value, 2))
+
+
+def eq_table(device, point, out=None):
+    """eq(r, .) (tessera-poly eq.rs:23) for a point of n Fp2 (array (n, 2)); into `out` (a
+    DeviceBuffer of 2^n * 2 words) or returned as a numpy (2^n, 2) array."""
+    pt = np.ascontiguousarray(np.asarray(point, dtype=np.uint64).reshape(-1, 2))
+    n = pt.shape[0]
+    dst = out if out is not None else np.zeros((1 << n, 2), dtype=np.uint64)
+    ptr = dst.ptr if isinstance(dst, DeviceBuffer) else dst.ctypes.data
+    check(lib().tsr_eq_table(device.handle, pt.ctypes.data if n else None, n, ptr))
+    return dst

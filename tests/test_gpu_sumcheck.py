@@ -144,3 +144,16 @@ def test_config3_eq_a_b_invariants(dev):
         t2 = O.fp2_mul(one_r, one_x)
         e = O.fp2_mul(e, ((rx[0] + t2[0]) % P, (rx[1] + t2[1]) % P))
     assert claim == O.fp2_mul(O.fp2_mul(e, ea), eb)
+
+
+@pytest.mark.parametrize("n", [0, 1, 5, 12])
+def test_eq_table_matches_oracle(dev, n):
+    from tessera_amd.sumcheck import eq_table
+    rng = np.random.default_rng(n)
+    r = rng.integers(0, P, (n, 2), dtype=np.uint64)
+    got = eq_table(dev, r)
+    want = O.eq_table(r.reshape(-1)).reshape(-1, 2)
+    assert np.array_equal(got, want)
+    d = dev.buffer(2 << n)
+    eq_table(dev, r, out=d)
+    assert np.array_equal(d.download().reshape(-1, 2), want)

@@ -276,6 +276,24 @@ def cpu_served_sample(reqs):
             "sample": f"{len(reqs)} requests, {tokens} tokens, {sec:.1f} s"}
 
 
+def copy_probe(nbytes=2 << 30, reps=4):
+    """Device-to-device copies of a known size: the reference reading of FETCH_SIZE and WRITE_SIZE
+    (each copy fetches and writes `nbytes`)."""
+    import tessera_amd as T
+    from tessera_amd.lib import check, lib
+    dev = T.Device(0)
+    a = dev.buffer(nbytes // 8).fill_uniform(SEED)
+    b = dev.buffer(nbytes // 8)
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        check(lib().tsr_copy(dev.handle, b.ptr, a.ptr, nbytes))
+    dev.synchronize()
+    sec = (time.perf_counter() - t0) / reps
+    print(json.dumps({"copy_bytes": nbytes, "copies": reps, "ms_per_copy": round(1e3 * sec, 3),
+                      "GBps_read_plus_write": round(2 * nbytes / sec / 1e9, 1)}), flush=True)
+    dev.close()
+
+
 def traffic_from_profiles(kernel):
     """HBM bytes per launch measured by rocprofv3 --pmc (FETCH_SIZE + WRITE_SIZE, scaled by a
     copy kernel's reading), when a committed summary holds it; else None."""
@@ -289,14 +307,14 @@ def traffic_from_profiles(kernel):
 
 def cpu_baseline():
     """The oracle (C, all host threads) on a bounded sample of the same workload, scaled per unit:
-    encode 8 of the 256 rows (per-row cost is uniform), chain 2^16 of the 2^22 columns over all 256
-    rows (per-column cost is uniform), and a Merkle tree over 2^16 leaves (per-node cost uniform)."""
+    encode 64 of the 256 rows (per-row cost is uniform), chain 2^18 of the 2^22 columns over all 256
+    rows (per-column cost is uniform), and a Merkle tree over 2^18 leaves (per-node cost uniform)."""
     from oracle import oracle as O
     from tests.helpers import splitmix_uniform
 
     threads = os.cpu_count() or 1
     k, n = 1 << LOG_K, 1 << (LOG_K + LOG_R)
-    enc_rows, cols = 8, 1 << 16
+    enc_rows, cols = 64, 1 << 18
     rows = splitmix_uniform(enc_rows * k, SEED)
     t0 = time.perf_counter()
     cw = O.encode_rows(rows, LOG_K, LOG_R, threads=threads)
@@ -316,8 +334,8 @@ def cpu_baseline():
         "cores": threads,
         "kind": "port",
         "sample": (f"oracle C restatement, {threads} threads: encode {enc_rows}/256 rows "
-                   f"({t_enc:.2f} s scaled), chains over 256 rows for 2^16/2^22 columns "
-                   f"({t_chain:.2f} s scaled), Merkle over 2^16 leaves ({t_merkle:.2f} s scaled); "
+                   f"({t_enc:.2f} s scaled), chains over 256 rows for 2^18/2^22 columns "
+                   f"({t_chain:.2f} s scaled), Merkle over 2^18 leaves ({t_merkle:.2f} s scaled); "
                    f"est. {total:.1f} s per commit"),
     }
 
@@ -328,14 +346,18 @@ def main():
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--only", choices=["headline"], default=None,
-                    help="time the headline commit only")
+    ap.add_argument("--only", choices=["headline", "copy"], default=None,
+                    help="time the headline commit only; or run the device copy that calibrates "
+                         "the HBM byte counters (4 copies of 2 GiB)")
     a = ap.parse_args()
     if a.gpus != 1:
         print("bench.py measures one GPU: this build runs on a single MI355X; "
               "multi-GPU sharding is described in DESIGN.md, not run", file=sys.stderr)
         sys.exit(2)
 
+    if a.only == "copy":
+        copy_probe()
+        return
     log("headline: 256 x 2^20 commit + 64 openings")
     sec, kernels, root, n_sib = run_headline(a.steps, a.warmup)
     elems = ROWS * (1 << LOG_K)

@@ -27,6 +27,77 @@ namespace tsr {
 constexpr int TILE_THREADS = 512;
 constexpr int TILE_MAX_LOG = 13;  // 8192 u64 = 64 KiB of LDS
 
+// ---- stage rounds on an LDS tile ------------------------------------------------------------
+// A tile holds 2^(nb+logc) elements, element e = (mid << logc) | col with mid < 2^nb the stage
+// index and col < 2^logc a column. The stage at mid bit sb pairs mid and mid + 2^sb (bit sb of
+// mid clear) with the twiddle tw[g], g = (hi << (nb - sb - 1)) | (mid >> (sb + 1)): the group
+// p >> (b + 1) of the global low index p of the pair, b the stage's global bit, hi the tile's
+// index above its nb bits. A round runs R consecutive stage bits s0..s0+R-1 in registers: each
+// thread loads the 2^R elements mid = base | (j << s0) (bits s0..s0+R-1 of base clear), applies
+// the R stages (GS: ascending bits, CT: descending), and stores them; every butterfly is the
+// radix-2 butterfly of the stage it belongs to, so rounds change the order of work only.
+template <bool GS, int R>
+__device__ __forceinline__ void stage_round(uint64_t *lds, uint32_t nb, uint32_t logc, uint32_t s0,
+                                            uint64_t hi, const uint64_t *__restrict__ tw,
+                                            uint32_t tid, uint32_t nthreads) {
+    constexpr int M = 1 << R;
+    const uint32_t groups = 1u << (nb - R + logc);
+    const uint32_t cmask = (1u << logc) - 1;
+    for (uint32_t q = tid; q < groups; q += nthreads) {
+        const uint32_t col = q & cmask, m = q >> logc;
+        const uint32_t base = ((m >> s0) << (s0 + R)) | (m & ((1u << s0) - 1));
+        uint64_t x[M];
+#pragma unroll
+        for (int j = 0; j < M; j++) x[j] = lds[((base | ((uint32_t)j << s0)) << logc) | col];
+#pragma unroll
+        for (int tt = 0; tt < R; tt++) {
+            const int t = GS ? tt : R - 1 - tt;
+            const uint32_t sb = s0 + t;
+            const uint64_t gbase = (hi << (nb - sb - 1)) | (base >> (sb + 1));
+            uint64_t w[M / 2 > 0 ? M / 2 : 1];
+#pragma unroll
+            for (int u = 0; u < (M >> (t + 1)); u++) w[u] = tw[gbase | (uint64_t)u];
+#pragma unroll
+            for (int j = 0; j < M; j++) {
+                if (j & (1 << t)) continue;
+                const uint64_t wj = w[j >> (t + 1)];
+                const uint64_t u = x[j], v = x[j + (1 << t)];
+                if (GS) {
+                    x[j] = gl::add(u, v);
+                    x[j + (1 << t)] = gl::mul(gl::sub(u, v), wj);
+                } else {
+                    const uint64_t tv = gl::mul(v, wj);
+                    x[j] = gl::add(u, tv);
+                    x[j + (1 << t)] = gl::sub(u, tv);
+                }
+            }
+        }
+#pragma unroll
+        for (int j = 0; j < M; j++) lds[((base | ((uint32_t)j << s0)) << logc) | col] = x[j];
+    }
+}
+
+// All nb stages of a tile in rounds of up to 3 bits (GS from bit 0 up, CT from the top down),
+// with a barrier after each round. The tile must be in LDS, behind a barrier, on entry.
+template <bool GS>
+__device__ __forceinline__ void tile_stages(uint64_t *lds, uint32_t nb, uint32_t logc, uint64_t hi,
+                                            const uint64_t *__restrict__ tw) {
+    const uint32_t tid = threadIdx.x, nthreads = blockDim.x;
+    uint32_t done = 0;
+    while (done < nb) {
+        const uint32_t R = nb - done >= 3 ? 3 : nb - done;
+        const uint32_t s0 = GS ? done : nb - done - R;
+        if (R == 3)
+            stage_round<GS, 3>(lds, nb, logc, s0, hi, tw, tid, nthreads);
+        else if (R == 2)
+            stage_round<GS, 2>(lds, nb, logc, s0, hi, tw, tid, nthreads);
+        else
+            stage_round<GS, 1>(lds, nb, logc, s0, hi, tw, tid, nthreads);
+        __syncthreads();
+        done += R;
+    }
+}
+
 // Stages over bits [lo, lo+nb) of every row of length 2^L. Element e of the tile is
 // (mid = e >> logc, col = e & (C-1)); global index p = hi*2^(lo+nb) + mid*2^lo + lblk*C + col.
 template <bool GS>
@@ -53,28 +124,7 @@ k_tile_pass(const uint64_t *__restrict__ in, uint64_t *__restrict__ out, uint64_
     }
     __syncthreads();
 
-    const uint32_t bfly = tile >> 1;
-    for (uint32_t s = 0; s < nb; s++) {
-        const uint32_t sb = GS ? s : (nb - 1 - s);  // stage bit relative to lo
-        const uint32_t b = lo + sb;
-        for (uint32_t q = threadIdx.x; q < bfly; q += TILE_THREADS) {
-            uint32_t col = q & (C - 1), mq = q >> logc;
-            uint32_t mid = ((mq >> sb) << (sb + 1)) | (mq & ((1u << sb) - 1));
-            uint32_t e0 = (mid << logc) | col, e1 = e0 + (1u << (sb + logc));
-            uint64_t p = base + ((uint64_t)mid << lo) + col;
-            uint64_t w = tw[p >> (b + 1)];
-            uint64_t u = lds[e0], v = lds[e1];
-            if (GS) {
-                lds[e0] = gl::add(u, v);
-                lds[e1] = gl::mul(gl::sub(u, v), w);
-            } else {
-                uint64_t t = gl::mul(v, w);
-                lds[e0] = gl::add(u, t);
-                lds[e1] = gl::sub(u, t);
-            }
-        }
-        __syncthreads();
-    }
+    tile_stages<GS>(lds, nb, logc, hi, tw);
 
     for (uint32_t e = threadIdx.x; e < tile; e += TILE_THREADS) {
         uint64_t mid = e >> logc, col = e & (C - 1);
@@ -93,10 +143,12 @@ k_encode_mid(const uint64_t *__restrict__ in, uint64_t in_stride, uint64_t *__re
              const uint64_t *__restrict__ zk_inv, const uint64_t *__restrict__ zn,
              const uint64_t *__restrict__ spread, const uint64_t *__restrict__ masks,
              uint64_t P, const uint64_t *__restrict__ pow7, uint64_t shift_pow_k,
-             const uint64_t *const *in_rows, uint64_t *const *cw_rows) {
+             const uint64_t *const *in_rows, uint64_t *const *cw_rows, uint64_t c_off) {
     extern __shared__ uint64_t lds[];
     const uint32_t Tk = 1u << tk, Tn = Tk << log_r, s = 1u << log_r;
-    const uint64_t c = blockIdx.x, row = blockIdx.y;
+    // x: the row, y: the tile (+ c_off), so the rows of one tile run together and the tile's
+    // twiddles stay in L2.
+    const uint64_t row = blockIdx.x, c = blockIdx.y + c_off;
     const uint64_t kbase = c << tk, nbase = c << (tk + log_r);
     const uint64_t *src = in_rows ? in_rows[row] : in + row * in_stride;
 
@@ -104,17 +156,7 @@ k_encode_mid(const uint64_t *__restrict__ in, uint64_t in_stride, uint64_t *__re
     __syncthreads();
 
     // Cooley-Tukey stages b = tk-1 .. 0 of the size-k inverse transform.
-    for (int b = (int)tk - 1; b >= 0; b--) {
-        for (uint32_t q = threadIdx.x; q < (Tk >> 1); q += TILE_THREADS) {
-            uint32_t e0 = ((q >> b) << (b + 1)) | (q & ((1u << b) - 1)), e1 = e0 + (1u << b);
-            uint64_t w = zk_inv[(kbase + e0) >> (b + 1)];
-            uint64_t u = lds[e0], v = lds[e1];
-            uint64_t t = gl::mul(v, w);
-            lds[e0] = gl::add(u, t);
-            lds[e1] = gl::sub(u, t);
-        }
-        __syncthreads();
-    }
+    tile_stages<false>(lds, tk, 0, c, zk_inv);
 
     // Spread (encoding.rs:323-340) and masks (encoding.rs:303-312): entry t times
     // 7^{rev_k(t)}/k to position s*t, mask terms at s*t and s*t + s/2, zero elsewhere.
@@ -147,17 +189,7 @@ k_encode_mid(const uint64_t *__restrict__ in, uint64_t in_stride, uint64_t *__re
     __syncthreads();
 
     // Gentleman-Sande stages b = 0 .. tk+r-1 of the size-n forward transform.
-    const uint32_t tn = tk + log_r;
-    for (uint32_t b = 0; b < tn; b++) {
-        for (uint32_t q = threadIdx.x; q < (Tn >> 1); q += TILE_THREADS) {
-            uint32_t e0 = ((q >> b) << (b + 1)) | (q & ((1u << b) - 1)), e1 = e0 + (1u << b);
-            uint64_t w = zn[(nbase + e0) >> (b + 1)];
-            uint64_t u = lds[e0], v = lds[e1];
-            lds[e0] = gl::add(u, v);
-            lds[e1] = gl::mul(gl::sub(u, v), w);
-        }
-        __syncthreads();
-    }
+    tile_stages<true>(lds, tk + log_r, 0, c, zn);
 
     uint64_t *dst = cw_rows ? cw_rows[row] : cw + row * cw_stride;
     for (uint32_t e = threadIdx.x; e < Tn; e += TILE_THREADS) dst[nbase + e] = lds[e];
@@ -283,12 +315,16 @@ void Encoder::launch(const uint64_t *src, uint64_t src_stride, const uint64_t *c
         src_rows = (const uint64_t *const *)tmp_rows;
     }
     {
-        dim3 grid((unsigned)(k >> tk), (unsigned)rc);
+        const uint64_t tiles = k >> tk;
         size_t shm = sizeof(uint64_t) << (tk + log_r);
         KernelTimer timer(ctx, "k_encode_mid", stream);
-        hipLaunchKernelGGL(k_encode_mid, grid, dim3(TILE_THREADS), shm, stream, src, src_stride,
-                           cw, n, log_k, log_r, tk, zk_inv.ptr, zn_fwd.ptr, spread_tab.ptr, masks,
-                           P, pow7_tab.ptr, shift_pow_k, src_rows, cw_rows);
+        for (uint64_t c0 = 0; c0 < tiles; c0 += 65535) {
+            const uint64_t tc = tiles - c0 < 65535 ? tiles - c0 : 65535;
+            hipLaunchKernelGGL(k_encode_mid, dim3((unsigned)rc, (unsigned)tc), dim3(TILE_THREADS), shm,
+                               stream, src, src_stride, cw, n, log_k, log_r, tk, zk_inv.ptr, zn_fwd.ptr,
+                               spread_tab.ptr, masks, P, pow7_tab.ptr, shift_pow_k, src_rows, cw_rows,
+                               c0);
+        }
     }
     for (auto &ps : gs_passes) {
         uint32_t lo = ps.first, nb = ps.second, logc = lo < 4 ? lo : 4;

@@ -56,6 +56,30 @@ def kernel_bytes(name, rows, log_k, log_r):
     }.get(name)
 
 
+def kernel_butterflies(name, rows, log_k, log_r, tile_log=13):
+    """Radix-2 butterflies of one launch of each NTT kernel of the encoder's plan: the fused
+    middle runs tk CT stages over k and tk + r GS stages over n (tk = tile_log - r), the strided
+    passes the remaining log_k - tk CT stages and log_k - tk GS stages."""
+    k, n = 1 << log_k, 1 << (log_k + log_r)
+    tk = min(log_k, tile_log - log_r)
+    return {
+        "k_encode_mid": rows * (k // 2 * tk + n // 2 * (tk + log_r)),
+        "k_tile_pass<ct>": rows * k // 2 * (log_k - tk),
+        "k_tile_pass<gs>": rows * n // 2 * (log_k - tk),
+    }.get(name)
+
+
+def butterfly_peak():
+    """The device's measured register-resident butterfly rate (tsr_butterfly_peak)."""
+    import tessera_amd as T
+    from tessera_amd.lib import check, lib
+    dev = T.Device(0)
+    rate = ctypes.c_double(0)
+    check(lib().tsr_butterfly_peak(dev.handle, ctypes.byref(rate)))
+    dev.close()
+    return rate.value
+
+
 def timed(fn, reps):
     """Median seconds of `reps` calls of fn (each call ends synchronized)."""
     ts = []
@@ -294,15 +318,83 @@ def copy_probe(nbytes=2 << 30, reps=4):
     dev.close()
 
 
+def traffic_tables(fetch_head, fetch_copy, write_head, write_copy, out_json, out_md, command):
+    """HBM bytes per launch from rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE CSVs (KiB per dispatch):
+    each counter is scaled by what it reads for the calibration copy (bench.py --only copy: 2 GiB
+    fetched and written per copy); writes the per-kernel table to out_json (read by the roofline)
+    and a markdown summary to out_md."""
+    import csv
+    import re
+
+    def per_kernel(path):
+        agg = {}
+        for r in csv.DictReader(open(path)):
+            name = re.sub(r"\(.*", "", r["Kernel_Name"]).replace("void ", "").replace("tsr::", "")
+            agg.setdefault(name, []).append(float(r["Counter_Value"]) * 1024.0)
+        return agg
+
+    def copy_factor(path):
+        vals = per_kernel(path).get("__amd_rocclr_copyBuffer", [])
+        big = max(vals)
+        return (2 << 30) / big, big
+
+    f_fac, f_raw = copy_factor(fetch_copy)
+    w_fac, w_raw = copy_factor(write_copy)
+    fetch, write = per_kernel(fetch_head), per_kernel(write_head)
+    table = {}
+    lines = [f"# HBM traffic per launch, headline commit (256 x 2^20, rate 1/4)\n",
+             "Command (each pass its own run, from the repository root on the GPU box):\n",
+             "```\n" + command + "\n```\n",
+             f"Calibration: one 2 GiB device copy reads FETCH_SIZE = {f_raw / 2**30:.4f} GiB and "
+             f"WRITE_SIZE = {w_raw / 2**30:.4f} GiB, so FETCH_SIZE is scaled by {f_fac:.4f} and "
+             f"WRITE_SIZE by {w_fac:.4f}.\n",
+             "| kernel | launches | fetch (GiB/launch) | write (GiB/launch) | traffic (GiB/launch) |",
+             "|---|---|---|---|---|"]
+    for name in fetch:
+        fv = sum(fetch[name]) / len(fetch[name]) * f_fac
+        wv = sum(write.get(name, [0])) / max(1, len(write.get(name, [0]))) * w_fac
+        if name.startswith("k_"):
+            table[name] = round(fv + wv)
+        lines.append(f"| `{name}` | {len(fetch[name])} | {fv / 2**30:.4f} | {wv / 2**30:.4f} | "
+                     f"{(fv + wv) / 2**30:.4f} |")
+    with open(out_json, "w") as f:
+        json.dump(table, f, indent=1, sort_keys=True)
+    with open(out_md, "w") as f:
+        f.write("\n".join(lines) + "\n")
+
+
 def traffic_from_profiles(kernel):
     """HBM bytes per launch measured by rocprofv3 --pmc (FETCH_SIZE + WRITE_SIZE, scaled by a
     copy kernel's reading), when a committed summary holds it; else None."""
     path = os.path.join(ROOT, "profiles", "traffic.json")
+    alias = {"k_tile_pass<gs>": "k_tile_pass<true>", "k_tile_pass<ct>": "k_tile_pass<false>"}
     try:
         with open(path) as f:
-            return json.load(f).get(kernel)
+            return json.load(f).get(alias.get(kernel, kernel))
     except (OSError, ValueError):
         return None
+
+
+def trace_summary(db, title, command):
+    """Markdown of a rocprofv3 --kernel-trace --stats database: per kernel calls, total, average,
+    min and max duration, share, registers, LDS and launch shape."""
+    import re
+    import sqlite3
+    c = sqlite3.connect(db)
+    rows = list(c.execute("select * from top_kernels"))
+    extra = {r[0]: r[1:] for r in c.execute(
+        "select name, min(duration), max(duration), max(vgpr_count), max(sgpr_count), max(lds_size), "
+        "max(grid_x*grid_y*grid_z), max(workgroup_x) from kernels group by name")}
+    out = [f"# {title}\n", "Command (from the repository root on the GPU box):\n",
+           "```\n" + command + "\n```\n",
+           "| kernel | calls | total (us) | avg (us) | min (us) | max (us) | % | VGPR | SGPR | LDS (B) | grid | WG |",
+           "|---|---|---|---|---|---|---|---|---|---|---|---|"]
+    for name, calls, tot, avg, pct in rows:
+        short = re.sub(r"\(.*", "", name)
+        mn, mx, vg, sg, lds, grid, wg = extra.get(name, (0,) * 7)
+        out.append(f"| `{short}` | {calls} | {tot:.1f} | {avg:.1f} | {mn / 1000:.1f} | {mx / 1000:.1f} | "
+                   f"{pct:.1f} | {vg} | {sg} | {lds} | {grid} | {wg} |")
+    return "\n".join(out) + "\n"
 
 
 def cpu_baseline():
@@ -346,6 +438,11 @@ def main():
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--traffic", nargs=4, metavar=("FETCH_HEAD", "FETCH_COPY", "WRITE_HEAD", "WRITE_COPY"),
+                    help="rocprofv3 --pmc CSVs -> profiles/traffic.json and its summary")
+    ap.add_argument("--traffic-command", default="", help="the profiling command, for the summary")
+    ap.add_argument("--trace-summary", nargs=3, metavar=("DB", "TITLE", "COMMAND"),
+                    help="print a markdown summary of a rocprofv3 kernel-trace database")
     ap.add_argument("--only", choices=["headline", "copy"], default=None,
                     help="time the headline commit only; or run the device copy that calibrates "
                          "the HBM byte counters (4 copies of 2 GiB)")
@@ -355,6 +452,14 @@ def main():
               "multi-GPU sharding is described in DESIGN.md, not run", file=sys.stderr)
         sys.exit(2)
 
+    if a.trace_summary:
+        print(trace_summary(*a.trace_summary), end="")
+        return
+    if a.traffic:
+        traffic_tables(*a.traffic, os.path.join(ROOT, "profiles", "traffic.json"),
+                       os.path.join(ROOT, "profiles", "traffic_commit_large_256x2p20.md"),
+                       a.traffic_command)
+        return
     if a.only == "copy":
         copy_probe()
         return
@@ -371,6 +476,15 @@ def main():
                     "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
                     "traffic": traffic_from_profiles(dom), "algorithmic_bytes": b,
                     "avg_ms": round(kernels[dom]["avg_ms"], 4)}
+        bf = kernel_butterflies(dom, ROWS, LOG_K, LOG_R)
+        if bf:
+            peak = butterfly_peak()
+            rate = bf / (kernels[dom]["avg_ms"] * 1e-3)
+            # the integer NTT is bound by the vector ALUs: its butterfly rate against the measured
+            # register-resident rate of the same butterfly
+            roofline["valu"] = {"butterflies": bf, "achieved": round(rate / 1e9, 1),
+                                "peak": round(peak / 1e9, 1), "unit": "G butterflies/s",
+                                "frac": round(rate / peak, 4)}
     extra = {}
     if a.only is None:
         log("plain: 256 x 2^14 commit + Sum a*b")

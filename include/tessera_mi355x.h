@@ -296,6 +296,10 @@ tsr_status tsr_ctx_profile(tsr_ctx *ctx, int enable);
 /* Total milliseconds and launch count of the named kernel since the last clear. */
 tsr_status tsr_ctx_kernel_time(tsr_ctx *ctx, const char *kernel, double *total_ms,
                                uint64_t *launches);
+/* The device's register-resident Goldilocks butterfly rate (u + v, (u - v) w per butterfly, no
+ * memory traffic): the VALU ceiling the NTT kernels are measured against. No reference
+ * counterpart (the reference quotes butterflies per second, docs/perf/v0.1/cc-h100.md). */
+tsr_status tsr_butterfly_peak(tsr_ctx *ctx, double *butterflies_per_s);
 
 #ifdef __cplusplus
 }

@@ -92,6 +92,7 @@ _SIGS = {
     "tsr_commit_batch_tuning": (i32, [vp, vp, u32, ctypes.POINTER(u32)]),
     "tsr_commit_batch_free": (None, [vp]),
     "tsr_eq_table": (i32, [vp, vp, u32, vp]),
+    "tsr_butterfly_peak": (i32, [vp, ctypes.POINTER(ctypes.c_double)]),
     "tsr_server_create": (i32, [i32, vp, ctypes.POINTER(vp)]),
     "tsr_server_submit": (i32, [vp, vp]),
     "tsr_server_wait": (i32, [vp, vp, u32]),

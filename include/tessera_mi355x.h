@@ -84,6 +84,40 @@ tsr_status tsr_chains_absorb(tsr_chains *chains, const uint64_t *rows, uint64_t 
 tsr_status tsr_chains_read(tsr_chains *chains, uint8_t *out);
 void tsr_chains_free(tsr_chains *chains);
 
+/* ---- device sinks (BlockSink, crates/tessera-pcs/src/streaming.rs:24-38; MatrixSink and
+ *      WeightEncoding::matrix, crates/tessera-pcs/src/weights/regenerated.rs:24-62; the reference's
+ *      implementations AdviceSink / WeightSink, crates/tessera-prover/src/gpu.rs:161-196,432-667).
+ *      A sink owns its rows, masks and codewords on the device and zeroes them when freed. ---- */
+typedef struct tsr_block_sink tsr_block_sink;
+typedef struct tsr_block_sink tsr_matrix_sink;
+/* A block sink for rows of k = 2^log_k values with P = mask_coeffs masks per row, at rate
+ * 2^-log2_inv_rate, absorbing blocks of up to `block` (1..124) rows into the chains of the
+ * n = 2^(log_k + r) codeword columns of `commitment_id`. */
+tsr_status tsr_block_sink_create(tsr_ctx *ctx, uint32_t log_k, uint32_t log2_inv_rate,
+                                 uint64_t mask_coeffs, uint64_t block,
+                                 const uint8_t commitment_id[32], tsr_block_sink **out);
+/* BlockSink::absorb: `count` whole rows (1..block; a shorter block must be the last) and their
+ * masks (count x P, m_0 first) are encoded and absorbed into every column chain. */
+tsr_status tsr_block_sink_absorb(tsr_block_sink *sink, const uint64_t *rows, uint64_t count,
+                                 const uint64_t *masks);
+/* BlockSink::finish: the n column chain values, 32 bytes each, in column order. */
+tsr_status tsr_block_sink_finish(tsr_block_sink *sink, uint8_t *digests);
+void tsr_block_sink_free(tsr_block_sink *sink);
+/* WeightEncoding::matrix: a sink for unmasked rows of 2^log_k values at rate 2^-r into the chains of
+ * `columns` columns, answering the q `queries` (distinct, increasing). DECLINED (the host encodes
+ * the matrix) unless columns = 2^(log_k + r) and block is 1..124. */
+tsr_status tsr_matrix_sink_create(tsr_ctx *ctx, uint32_t log_k, uint32_t log2_inv_rate,
+                                  uint64_t block, const uint8_t commitment_id[32],
+                                  uint64_t columns, const uint64_t *queries, uint64_t q,
+                                  tsr_matrix_sink **out);
+/* MatrixSink::absorb: encodes and absorbs `count` whole rows and writes each row's symbols at the
+ * queried columns, row by row (count x q values). */
+tsr_status tsr_matrix_sink_absorb(tsr_matrix_sink *sink, const uint64_t *rows, uint64_t count,
+                                  uint64_t *symbols_out);
+/* MatrixSink::finish: the column chain values, 32 bytes each. */
+tsr_status tsr_matrix_sink_finish(tsr_matrix_sink *sink, uint8_t *digests);
+void tsr_matrix_sink_free(tsr_matrix_sink *sink);
+
 /* ---- Merkle (MerkleTree::new / root, crates/tessera-hash/src/merkle.rs:93-119) ---- */
 /* root of the binary BLAKE3 tree over n = 2^m leaves (32 bytes each). */
 tsr_status tsr_merkle_root(tsr_ctx *ctx, const uint8_t *leaves, uint64_t n, uint8_t root[32]);
@@ -283,6 +317,30 @@ tsr_status tsr_logup_prove(tsr_ctx *ctx, tsr_transcript *t, const tsr_logup_inst
  * lookup names no row. */
 tsr_status tsr_logup_multiplicities(tsr_ctx *ctx, uint64_t rows, const uint64_t *lookup_rows,
                                     uint64_t lookups, uint64_t *out);
+
+/* ---- the level offload (LevelOffload::load, DeviceLevel::{sums, bind, firsts},
+ *      crates/tessera-sumcheck/src/offload.rs:17-79; the reference's GpuLevel,
+ *      crates/tessera-gpu/src/level.rs:52-307) ---- */
+typedef struct tsr_level tsr_level;
+enum {
+    TSR_LEVEL_FRACTION = 0, /* N0 D1 + N1 D0 + lambda D0 D1 over [N0, N1, D0, D1] */
+    TSR_LEVEL_PRODUCT = 1   /* V0 V1 over [V0, V1] */
+};
+/* LevelOffload::load: `n_tables` tables (4 for a fraction level, 2 for a product level) of 2^m
+ * Fp2 entries each (host or device; copied), 1 <= m <= 32. */
+tsr_status tsr_level_load(tsr_ctx *ctx, int shape, const uint64_t lambda[2],
+                          const uint64_t *const *tables, uint32_t n_tables, uint32_t m,
+                          tsr_level **out);
+/* DeviceLevel::sums: Sum_p w(p) P(line_x(p)) for x = 0, 2, 3 (3 Fp2 in `out`) over the pairs p of
+ * the current tables, w(p) = low[p mod 2^low_bits] high[p >> low_bits] (2^low_bits and high_len
+ * Fp2), line_x(p) = t[2p] + x (t[2p+1] - t[2p]). BAD_SHAPE unless the weights cover the pairs. */
+tsr_status tsr_level_sums(tsr_level *level, const uint64_t *low, uint32_t low_bits,
+                          const uint64_t *high, uint64_t high_len, uint64_t out[6]);
+/* DeviceLevel::bind: t'[i] = t[2i] + r (t[2i+1] - t[2i]) for every table. */
+tsr_status tsr_level_bind(tsr_level *level, const uint64_t r[2]);
+/* DeviceLevel::firsts: each table's first entry (n_tables Fp2). */
+tsr_status tsr_level_firsts(tsr_level *level, uint64_t *out);
+void tsr_level_free(tsr_level *level);
 
 /* eq(r, .) over n variables (crates/tessera-poly/src/eq.rs:23: entry x = prod_i (r_i x_i +
  * (1 - r_i)(1 - x_i)), variables LSB-first): `point` n Fp2 on the host, `out` 2^n Fp2 (device or

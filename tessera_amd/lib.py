@@ -92,6 +92,19 @@ _SIGS = {
     "tsr_commit_batch_tuning": (i32, [vp, vp, u32, ctypes.POINTER(u32)]),
     "tsr_commit_batch_free": (None, [vp]),
     "tsr_eq_table": (i32, [vp, vp, u32, vp]),
+    "tsr_block_sink_create": (i32, [vp, u32, u32, u64, u64, vp, ctypes.POINTER(vp)]),
+    "tsr_block_sink_absorb": (i32, [vp, vp, u64, vp]),
+    "tsr_block_sink_finish": (i32, [vp, vp]),
+    "tsr_block_sink_free": (None, [vp]),
+    "tsr_matrix_sink_create": (i32, [vp, u32, u32, u64, vp, u64, vp, u64, ctypes.POINTER(vp)]),
+    "tsr_matrix_sink_absorb": (i32, [vp, vp, u64, vp]),
+    "tsr_matrix_sink_finish": (i32, [vp, vp]),
+    "tsr_matrix_sink_free": (None, [vp]),
+    "tsr_level_load": (i32, [vp, i32, vp, vp, u32, u32, ctypes.POINTER(vp)]),
+    "tsr_level_sums": (i32, [vp, vp, u32, vp, u64, vp]),
+    "tsr_level_bind": (i32, [vp, vp]),
+    "tsr_level_firsts": (i32, [vp, vp]),
+    "tsr_level_free": (None, [vp]),
     "tsr_butterfly_peak": (i32, [vp, ctypes.POINTER(ctypes.c_double)]),
     "tsr_server_create": (i32, [i32, vp, ctypes.POINTER(vp)]),
     "tsr_server_submit": (i32, [vp, vp]),

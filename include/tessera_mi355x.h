@@ -245,6 +245,11 @@ tsr_status tsr_hash(tsr_ctx *ctx, int domain, const uint8_t *data, uint64_t len,
 /* Transcript::new(context): state = H(transcript-init, TCE(SessionContext)). */
 tsr_status tsr_transcript_create(tsr_ctx *ctx, const uint8_t *context_tce, uint64_t len,
                                  tsr_transcript **out);
+/* A transcript continuing from a state and squeeze counter another party's Transcript reached
+ * (the fields the reference's Transcript holds, crates/tessera-hash/src/transcript.rs:92-96): the hand-over of a session
+ * between a host prover and this library. */
+tsr_status tsr_transcript_from_state(tsr_ctx *ctx, const uint8_t state[32], uint64_t counter,
+                                     tsr_transcript **out);
 void tsr_transcript_free(tsr_transcript *t);
 tsr_status tsr_transcript_state(const tsr_transcript *t, uint8_t state[32], uint64_t *counter);
 /* absorb(label, value): `tce` is the value's TCE encoding. Labels are printable ASCII. */

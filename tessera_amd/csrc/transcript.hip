@@ -294,6 +294,17 @@ tsr_status tsr_transcript_create(tsr_ctx *ctx, const uint8_t *context_tce, uint6
     return TSR_OK;
 }
 
+tsr_status tsr_transcript_from_state(tsr_ctx *ctx, const uint8_t state[32], uint64_t counter,
+                                     tsr_transcript **out) {
+    T_CHECK(out && state, TSR_BAD_ARG, "null argument");
+    std::unique_ptr<tsr_transcript> t(new tsr_transcript());
+    t->t.ctx = (const Ctx *)ctx;
+    memcpy(t->t.state, state, 32);
+    t->t.counter = counter;
+    *out = t.release();
+    return TSR_OK;
+}
+
 void tsr_transcript_free(tsr_transcript *t) { delete t; }
 
 tsr_status tsr_transcript_state(const tsr_transcript *t, uint8_t state[32], uint64_t *counter) {

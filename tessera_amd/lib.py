@@ -92,6 +92,7 @@ _SIGS = {
     "tsr_commit_batch_tuning": (i32, [vp, vp, u32, ctypes.POINTER(u32)]),
     "tsr_commit_batch_free": (None, [vp]),
     "tsr_eq_table": (i32, [vp, vp, u32, vp]),
+    "tsr_transcript_from_state": (i32, [vp, vp, u64, ctypes.POINTER(vp)]),
     "tsr_block_sink_create": (i32, [vp, u32, u32, u64, u64, vp, ctypes.POINTER(vp)]),
     "tsr_block_sink_absorb": (i32, [vp, vp, u64, vp]),
     "tsr_block_sink_finish": (i32, [vp, vp]),

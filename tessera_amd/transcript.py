@@ -47,6 +47,16 @@ class Transcript:
                                           len(context_tce), ctypes.byref(h)))
         self.handle = h.value
 
+    @classmethod
+    def from_state(cls, state: bytes, counter: int, device=None):
+        """A transcript continuing another's session from its state and squeeze counter."""
+        t = cls.__new__(cls)
+        h = vp()
+        check(lib().tsr_transcript_from_state(device.handle if device else None, bytes(state),
+                                              counter, ctypes.byref(h)))
+        t.handle = h.value
+        return t
+
     @property
     def state(self) -> bytes:
         out = ctypes.create_string_buffer(32)

@@ -52,3 +52,16 @@ def test_long_absorbs_match_oracle():
         b.absorb("x", data)
         assert a.state == b.state, length
         assert a.squeeze("y") == b.squeeze("y")
+
+
+def test_transcript_from_state_continues_the_session():
+    """A transcript handed over as (state, counter) continues exactly as the original."""
+    from tessera_amd.transcript import Transcript
+    a = Transcript(b"hand-over")
+    a.absorb("x", b"\x01\x02")
+    a.challenge_fp2("c")
+    b = Transcript.from_state(a.state, a.counter)
+    for t in (a, b):
+        t.absorb("y", b"\x03")
+    assert a.challenge_fp2("d") == b.challenge_fp2("d")
+    assert a.state == b.state and a.counter == b.counter

@@ -45,7 +45,10 @@ TSR_HD uint64_t reduce128(uint64_t lo, uint64_t hi) {
     uint32_t hi_lo = (uint32_t)hi;
     uint64_t t0 = lo - hi_hi;
     t0 -= (lo < hi_hi) ? EPS : 0;
-    uint64_t t1 = ((uint64_t)hi_lo << 32) - hi_lo;
+    // t1 = hi_lo (2^32 - 1) from its 32-bit halves (low -hi_lo, high hi_lo - [hi_lo != 0]):
+    // two cheap ALU ops instead of the multiply the compiler would otherwise pick
+    const uint32_t t1l = 0u - hi_lo, t1h = hi_lo - (hi_lo != 0u);
+    uint64_t t1 = ((uint64_t)t1h << 32) | t1l;
     uint64_t t2 = t0 + t1;
     t2 += (t2 < t1) ? EPS : 0;
     return canon(t2);

@@ -114,6 +114,36 @@ __global__ void k_count_rows(const uint64_t *__restrict__ rows, uint64_t n, uint
     else atomicOr(bad, 1u);
 }
 
+// The same counts for tables of at most 2^17 rows, without global contention on hot rows: the
+// rows are cut into slices of 2^14; workgroup (chunk c, slice s) counts the lookups of chunk c
+// that fall in slice s in an LDS histogram, then adds each nonzero bin to the global count once.
+// Integer counting is exact in any order.
+constexpr uint32_t HIST_SLICE_LOG = 14, HIST_THREADS = 512;
+constexpr uint64_t HIST_CHUNK = 1ull << 16;
+__global__ void __launch_bounds__(HIST_THREADS)
+k_count_rows_lds(const uint64_t *__restrict__ rows, uint64_t n, uint64_t limit,
+                 unsigned long long *__restrict__ counts, unsigned *__restrict__ bad) {
+    __shared__ uint32_t hist[1u << HIST_SLICE_LOG];
+    const uint64_t slice_lo = (uint64_t)blockIdx.y << HIST_SLICE_LOG;
+    for (uint32_t i = threadIdx.x; i < (1u << HIST_SLICE_LOG); i += HIST_THREADS) hist[i] = 0;
+    __syncthreads();
+    const uint64_t c0 = (uint64_t)blockIdx.x * HIST_CHUNK;
+    const uint64_t c1 = c0 + HIST_CHUNK < n ? c0 + HIST_CHUNK : n;
+    for (uint64_t i = c0 + threadIdx.x; i < c1; i += HIST_THREADS) {
+        const uint64_t r = rows[i];
+        if (r >= limit) {
+            if (blockIdx.y == 0) atomicOr(bad, 1u);
+        } else if (r - slice_lo < (1ull << HIST_SLICE_LOG) && r >= slice_lo) {
+            atomicAdd(&hist[r - slice_lo], 1u);
+        }
+    }
+    __syncthreads();
+    for (uint32_t i = threadIdx.x; i < (1u << HIST_SLICE_LOG); i += HIST_THREADS) {
+        const uint32_t v = hist[i];
+        if (v && slice_lo + i < limit) atomicAdd(&counts[slice_lo + i], (unsigned long long)v);
+    }
+}
+
 static unsigned nblocks(uint64_t n, unsigned per) {
     return (unsigned)std::max<uint64_t>(1, (n + per - 1) / per);
 }
@@ -471,7 +501,12 @@ tsr_status tsr_logup_multiplicities(tsr_ctx *ctx_, uint64_t rows, const uint64_t
     if ((st = bad.alloc(ctx, 1))) return st.code;
     hipMemsetAsync(counts.ptr, 0, 8 * leaves, s);
     hipMemsetAsync(bad.ptr, 0, 4, s);
-    if (lookups)
+    const uint64_t slices = (rows + (1ull << HIST_SLICE_LOG) - 1) >> HIST_SLICE_LOG;
+    const uint64_t chunks = (lookups + HIST_CHUNK - 1) / HIST_CHUNK;
+    if (lookups && slices <= 8 && chunks <= 65535)
+        hipLaunchKernelGGL(k_count_rows_lds, dim3((unsigned)chunks, (unsigned)slices), dim3(HIST_THREADS), 0, s,
+                           d, lookups, rows, (unsigned long long *)counts.ptr, bad.ptr);
+    else if (lookups)
         hipLaunchKernelGGL(k_count_rows, dim3(nblocks(lookups, 256)), dim3(256), 0, s, d, lookups,
                            rows, (unsigned long long *)counts.ptr, bad.ptr);
     if ((st = check_launch("multiplicities"))) return st.code;

@@ -103,6 +103,17 @@ def test_multiplicities(dev):
         multiplicities(dev, 10, np.array([3, 10], dtype=np.uint64))
 
 
+@pytest.mark.parametrize("rows,lookups", [(1, 70000), (1 << 14, 1), ((1 << 14) + 1, 300000),
+                                          (1 << 17, 200000), ((1 << 17) + 1, 200000)])
+def test_multiplicities_histogram_and_global_paths(dev, rows, lookups):
+    """Tables up to 2^17 rows count in LDS slices of 2^14 (chunks of 2^16 lookups), larger ones
+    with global atomics; both against exact counting, with Zipf-hot rows."""
+    from tessera_amd.logup import multiplicities
+    from tests.helpers import zipf_rows
+    idx = zipf_rows(lookups, rows, 1.1, rows + lookups)
+    assert np.array_equal(multiplicities(dev, rows, idx), O.multiplicities(rows, idx))
+
+
 def test_config3_logup_against_oracle(dev):
     """2^22 lookups Zipf(1.1) into 2^16 rows with values: the whole proof against the oracle."""
     from tessera_amd.logup import multiplicities

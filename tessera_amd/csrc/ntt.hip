@@ -98,9 +98,69 @@ __device__ __forceinline__ void tile_stages(uint64_t *lds, uint32_t nb, uint32_t
     }
 }
 
+// The same rounds with the tile geometry known at compile time (NB stage bits, LOGC column
+// bits, the round's first bit S0): each thread's 2^R elements and the stage's twiddles then sit
+// at constant offsets from one address, which the LDS and global loads take as immediates.
+template <bool GS, int R, int NB, int LOGC, int S0>
+__device__ __forceinline__ void stage_round_ct(uint64_t *lds, uint64_t hi, const uint64_t *__restrict__ tw,
+                                               uint32_t tid) {
+    constexpr int M = 1 << R;
+    constexpr uint32_t groups = 1u << (NB - R + LOGC);
+    constexpr uint32_t cmask = (1u << LOGC) - 1;
+    constexpr uint32_t stride = 1u << (S0 + LOGC);  // between a group's consecutive elements
+#pragma unroll
+    for (uint32_t q0 = 0; q0 < groups; q0 += TILE_THREADS) {
+        const uint32_t q = q0 + tid;
+        if (groups % TILE_THREADS != 0 && q >= groups) break;
+        const uint32_t col = q & cmask, m = q >> LOGC;
+        const uint32_t base = ((m >> S0) << (S0 + R)) | (m & ((1u << S0) - 1));
+        uint64_t *e0 = lds + ((base << LOGC) | col);
+        uint64_t x[M];
+#pragma unroll
+        for (int j = 0; j < M; j++) x[j] = e0[j * stride];
+#pragma unroll
+        for (int tt = 0; tt < R; tt++) {
+            const int t = GS ? tt : R - 1 - tt;
+            const int sb = S0 + t;
+            const uint64_t *wp = tw + ((hi << (NB - sb - 1)) | (base >> (sb + 1)));
+            uint64_t w[M / 2 > 0 ? M / 2 : 1];
+#pragma unroll
+            for (int u = 0; u < (M >> (t + 1)); u++) w[u] = wp[u];
+#pragma unroll
+            for (int j = 0; j < M; j++) {
+                if (j & (1 << t)) continue;
+                const uint64_t wj = w[j >> (t + 1)];
+                const uint64_t u = x[j], v = x[j + (1 << t)];
+                if (GS) {
+                    x[j] = gl::add(u, v);
+                    x[j + (1 << t)] = gl::mul(gl::sub(u, v), wj);
+                } else {
+                    const uint64_t tv = gl::mul(v, wj);
+                    x[j] = gl::add(u, tv);
+                    x[j + (1 << t)] = gl::sub(u, tv);
+                }
+            }
+        }
+#pragma unroll
+        for (int j = 0; j < M; j++) e0[j * stride] = x[j];
+    }
+}
+
+template <bool GS, int NB, int LOGC, int DONE = 0>
+__device__ __forceinline__ void tile_stages_ct(uint64_t *lds, uint64_t hi, const uint64_t *__restrict__ tw) {
+    if constexpr (DONE < NB) {
+        constexpr int R = NB - DONE >= 3 ? 3 : NB - DONE;
+        constexpr int S0 = GS ? DONE : NB - DONE - R;
+        stage_round_ct<GS, R, NB, LOGC, S0>(lds, hi, tw, threadIdx.x);
+        __syncthreads();
+        tile_stages_ct<GS, NB, LOGC, DONE + R>(lds, hi, tw);
+    }
+}
+
 // Stages over bits [lo, lo+nb) of every row of length 2^L. Element e of the tile is
 // (mid = e >> logc, col = e & (C-1)); global index p = hi*2^(lo+nb) + mid*2^lo + lblk*C + col.
-template <bool GS>
+// NB > 0: nb = NB and logc = LOGC, known at compile time.
+template <bool GS, int NB, int LOGC>
 __global__ void __launch_bounds__(TILE_THREADS)
 k_tile_pass(const uint64_t *__restrict__ in, uint64_t *__restrict__ out, uint64_t in_stride,
             uint64_t out_stride, uint32_t L, uint32_t lo, uint32_t nb, uint32_t logc,
@@ -124,7 +184,10 @@ k_tile_pass(const uint64_t *__restrict__ in, uint64_t *__restrict__ out, uint64_
     }
     __syncthreads();
 
-    tile_stages<GS>(lds, nb, logc, hi, tw);
+    if constexpr (NB > 0)
+        tile_stages_ct<GS, NB, LOGC>(lds, hi, tw);
+    else
+        tile_stages<GS>(lds, nb, logc, hi, tw);
 
     for (uint32_t e = threadIdx.x; e < tile; e += TILE_THREADS) {
         uint64_t mid = e >> logc, col = e & (C - 1);
@@ -137,6 +200,8 @@ __device__ __forceinline__ uint32_t rev_bits(uint64_t x, uint32_t bits) {
 }
 
 // The fused middle of the encoding, one tile of 2^tk row values -> 2^(tk+r) codeword symbols.
+// TK > 0: tk = TK and tk + log_r = TN, known at compile time.
+template <int TK, int TN>
 __global__ void __launch_bounds__(TILE_THREADS)
 k_encode_mid(const uint64_t *__restrict__ in, uint64_t in_stride, uint64_t *__restrict__ cw,
              uint64_t cw_stride, uint32_t log_k, uint32_t log_r, uint32_t tk,
@@ -156,7 +221,10 @@ k_encode_mid(const uint64_t *__restrict__ in, uint64_t in_stride, uint64_t *__re
     __syncthreads();
 
     // Cooley-Tukey stages b = tk-1 .. 0 of the size-k inverse transform.
-    tile_stages<false>(lds, tk, 0, c, zk_inv);
+    if constexpr (TK > 0)
+        tile_stages_ct<false, TK, 0>(lds, c, zk_inv);
+    else
+        tile_stages<false>(lds, tk, 0, c, zk_inv);
 
     // Spread (encoding.rs:323-340) and masks (encoding.rs:303-312): entry t times
     // 7^{rev_k(t)}/k to position s*t, mask terms at s*t and s*t + s/2, zero elsewhere.
@@ -189,13 +257,53 @@ k_encode_mid(const uint64_t *__restrict__ in, uint64_t in_stride, uint64_t *__re
     __syncthreads();
 
     // Gentleman-Sande stages b = 0 .. tk+r-1 of the size-n forward transform.
-    tile_stages<true>(lds, tk + log_r, 0, c, zn);
+    if constexpr (TK > 0)
+        tile_stages_ct<true, TN, 0>(lds, c, zn);
+    else
+        tile_stages<true>(lds, tk + log_r, 0, c, zn);
 
     uint64_t *dst = cw_rows ? cw_rows[row] : cw + row * cw_stride;
     for (uint32_t e = threadIdx.x; e < Tn; e += TILE_THREADS) dst[nbase + e] = lds[e];
 }
 
 // ---------------------------------------------------------------------------------------------
+
+using TilePassFn = void (*)(const uint64_t *, uint64_t *, uint64_t, uint64_t, uint32_t, uint32_t,
+                            uint32_t, uint32_t, const uint64_t *, const uint64_t *const *,
+                            uint64_t *const *);
+using MidFn = void (*)(const uint64_t *, uint64_t, uint64_t *, uint64_t, uint32_t, uint32_t, uint32_t,
+                       const uint64_t *, const uint64_t *, const uint64_t *, const uint64_t *, uint64_t,
+                       const uint64_t *, uint64_t, const uint64_t *const *, uint64_t *const *, uint64_t);
+
+// The compile-time tile pass for (nb, 16 columns), else the general one.
+template <bool GS>
+static TilePassFn tile_pass_fn(uint32_t nb, uint32_t logc) {
+    if (logc == 4) {
+        switch (nb) {
+            case 1: return k_tile_pass<GS, 1, 4>;
+            case 2: return k_tile_pass<GS, 2, 4>;
+            case 3: return k_tile_pass<GS, 3, 4>;
+            case 4: return k_tile_pass<GS, 4, 4>;
+            case 5: return k_tile_pass<GS, 5, 4>;
+            case 6: return k_tile_pass<GS, 6, 4>;
+            case 7: return k_tile_pass<GS, 7, 4>;
+            case 8: return k_tile_pass<GS, 8, 4>;
+            case 9: return k_tile_pass<GS, 9, 4>;
+            default: break;
+        }
+    }
+    return k_tile_pass<GS, 0, 0>;
+}
+
+// The compile-time fused middle for the full tiles of tile_log 12 and 13 at rates 1/2 and 1/4.
+static MidFn mid_fn(uint32_t tk, uint32_t log_r) {
+    const uint32_t tn = tk + log_r;
+    if (tk == 11 && tn == 13) return k_encode_mid<11, 13>;
+    if (tk == 12 && tn == 13) return k_encode_mid<12, 13>;
+    if (tk == 10 && tn == 12) return k_encode_mid<10, 12>;
+    if (tk == 11 && tn == 12) return k_encode_mid<11, 12>;
+    return k_encode_mid<0, 0>;
+}
 
 static void bit_reversed_powers(uint64_t base, uint32_t bits, uint64_t scale,
                                 std::vector<uint64_t> &out) {
@@ -308,7 +416,7 @@ void Encoder::launch(const uint64_t *src, uint64_t src_stride, const uint64_t *c
         dim3 grid((unsigned)(k >> (nb + logc)), (unsigned)rc);
         size_t shm = sizeof(uint64_t) << (nb + logc);
         KernelTimer timer(ctx, "k_tile_pass<ct>", stream);
-        hipLaunchKernelGGL(k_tile_pass<false>, grid, dim3(TILE_THREADS), shm, stream, src, tmp,
+        hipLaunchKernelGGL(tile_pass_fn<false>(nb, logc), grid, dim3(TILE_THREADS), shm, stream, src, tmp,
                            src_stride, k, log_k, lo, nb, logc, zk_inv.ptr, src_rows, tmp_rows);
         src = tmp;
         src_stride = k;
@@ -320,7 +428,7 @@ void Encoder::launch(const uint64_t *src, uint64_t src_stride, const uint64_t *c
         KernelTimer timer(ctx, "k_encode_mid", stream);
         for (uint64_t c0 = 0; c0 < tiles; c0 += 65535) {
             const uint64_t tc = tiles - c0 < 65535 ? tiles - c0 : 65535;
-            hipLaunchKernelGGL(k_encode_mid, dim3((unsigned)rc, (unsigned)tc), dim3(TILE_THREADS), shm,
+            hipLaunchKernelGGL(mid_fn(tk, log_r), dim3((unsigned)rc, (unsigned)tc), dim3(TILE_THREADS), shm,
                                stream, src, src_stride, cw, n, log_k, log_r, tk, zk_inv.ptr, zn_fwd.ptr,
                                spread_tab.ptr, masks, P, pow7_tab.ptr, shift_pow_k, src_rows, cw_rows,
                                c0);
@@ -332,7 +440,7 @@ void Encoder::launch(const uint64_t *src, uint64_t src_stride, const uint64_t *c
         dim3 grid((unsigned)(n >> (nb + logc)), (unsigned)rc);
         size_t shm = sizeof(uint64_t) << (nb + logc);
         KernelTimer timer(ctx, "k_tile_pass<gs>", stream);
-        hipLaunchKernelGGL(k_tile_pass<true>, grid, dim3(TILE_THREADS), shm, stream, cw, cw, n, n,
+        hipLaunchKernelGGL(tile_pass_fn<true>(nb, logc), grid, dim3(TILE_THREADS), shm, stream, cw, cw, n, n,
                            log_n, lo, nb, logc, zn_fwd.ptr, (const uint64_t *const *)cw_rows,
                            cw_rows);
     }

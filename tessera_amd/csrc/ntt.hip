@@ -101,6 +101,36 @@ __device__ __forceinline__ void tile_stages(uint64_t *lds, uint32_t nb, uint32_t
 // The same rounds with the tile geometry known at compile time (NB stage bits, LOGC column
 // bits, the round's first bit S0): each thread's 2^R elements and the stage's twiddles then sit
 // at constant offsets from one address, which the LDS and global loads take as immediates.
+// The R stages of one group held in registers (x[j] = element base | (j << S0)).
+template <bool GS, int R, int NB, int S0>
+__device__ __forceinline__ void round_regs(uint64_t *x, uint32_t base, uint64_t hi,
+                                           const uint64_t *__restrict__ tw) {
+    constexpr int M = 1 << R;
+#pragma unroll
+    for (int tt = 0; tt < R; tt++) {
+        const int t = GS ? tt : R - 1 - tt;
+        const int sb = S0 + t;
+        const uint64_t *wp = tw + ((hi << (NB - sb - 1)) | (base >> (sb + 1)));
+        uint64_t w[M / 2 > 0 ? M / 2 : 1];
+#pragma unroll
+        for (int u = 0; u < (M >> (t + 1)); u++) w[u] = wp[u];
+#pragma unroll
+        for (int j = 0; j < M; j++) {
+            if (j & (1 << t)) continue;
+            const uint64_t wj = w[j >> (t + 1)];
+            const uint64_t u = x[j], v = x[j + (1 << t)];
+            if (GS) {
+                x[j] = gl::add(u, v);
+                x[j + (1 << t)] = gl::mul(gl::sub(u, v), wj);
+            } else {
+                const uint64_t tv = gl::mul(v, wj);
+                x[j] = gl::add(u, tv);
+                x[j + (1 << t)] = gl::sub(u, tv);
+            }
+        }
+    }
+}
+
 template <bool GS, int R, int NB, int LOGC, int S0>
 __device__ __forceinline__ void stage_round_ct(uint64_t *lds, uint64_t hi, const uint64_t *__restrict__ tw,
                                                uint32_t tid) {
@@ -118,29 +148,7 @@ __device__ __forceinline__ void stage_round_ct(uint64_t *lds, uint64_t hi, const
         uint64_t x[M];
 #pragma unroll
         for (int j = 0; j < M; j++) x[j] = e0[j * stride];
-#pragma unroll
-        for (int tt = 0; tt < R; tt++) {
-            const int t = GS ? tt : R - 1 - tt;
-            const int sb = S0 + t;
-            const uint64_t *wp = tw + ((hi << (NB - sb - 1)) | (base >> (sb + 1)));
-            uint64_t w[M / 2 > 0 ? M / 2 : 1];
-#pragma unroll
-            for (int u = 0; u < (M >> (t + 1)); u++) w[u] = wp[u];
-#pragma unroll
-            for (int j = 0; j < M; j++) {
-                if (j & (1 << t)) continue;
-                const uint64_t wj = w[j >> (t + 1)];
-                const uint64_t u = x[j], v = x[j + (1 << t)];
-                if (GS) {
-                    x[j] = gl::add(u, v);
-                    x[j + (1 << t)] = gl::mul(gl::sub(u, v), wj);
-                } else {
-                    const uint64_t tv = gl::mul(v, wj);
-                    x[j] = gl::add(u, tv);
-                    x[j + (1 << t)] = gl::sub(u, tv);
-                }
-            }
-        }
+        round_regs<GS, R, NB, S0>(x, base, hi, tw);
 #pragma unroll
         for (int j = 0; j < M; j++) e0[j * stride] = x[j];
     }
@@ -225,6 +233,46 @@ k_encode_mid(const uint64_t *__restrict__ in, uint64_t in_stride, uint64_t *__re
         tile_stages_ct<false, TK, 0>(lds, c, zk_inv);
     else
         tile_stages<false>(lds, tk, 0, c, zk_inv);
+
+    if constexpr (TK > 0) {
+        if (P == 0) {
+            // Unmasked rows: the spread puts entry t times 7^{rev_k(t)}/k at position s t and
+            // zeros between, so the first GS round (bits 0..2, groups of 8 positions) starts from
+            // registers: each group's 8 / s entries, zeros elsewhere, which the butterflies of
+            // the round fold away. Every group writes all 8 of its positions.
+            constexpr int S = 1 << (TN - TK), PERG = 8 / S;
+            constexpr uint32_t GROUPS = 1u << (TN - 3);
+            constexpr uint32_t GPT = (GROUPS + TILE_THREADS - 1) / TILE_THREADS;
+            static_assert(TN - TK >= 1 && TN - TK <= 3 && TN >= 3, "expansion round shape");
+            uint64_t v[GPT][PERG];
+#pragma unroll
+            for (uint32_t i = 0; i < GPT; i++) {
+                const uint32_t g = threadIdx.x + i * TILE_THREADS;
+#pragma unroll
+                for (int jj = 0; jj < PERG; jj++) {
+                    const uint32_t t = g * PERG + jj;
+                    v[i][jj] = g < GROUPS ? gl::mul(lds[t], spread[kbase + t]) : 0;
+                }
+            }
+            __syncthreads();
+#pragma unroll
+            for (uint32_t i = 0; i < GPT; i++) {
+                const uint32_t g = threadIdx.x + i * TILE_THREADS;
+                if (g >= GROUPS) break;
+                uint64_t x[8];
+#pragma unroll
+                for (int j = 0; j < 8; j++) x[j] = (j % S == 0) ? v[i][j / S] : 0;
+                round_regs<true, 3, TN, 0>(x, g << 3, c, zn);
+#pragma unroll
+                for (int j = 0; j < 8; j++) lds[(g << 3) + j] = x[j];
+            }
+            __syncthreads();
+            tile_stages_ct<true, TN, 0, 3>(lds, c, zn);
+            uint64_t *dst = cw_rows ? cw_rows[row] : cw + row * cw_stride;
+            for (uint32_t e = threadIdx.x; e < Tn; e += TILE_THREADS) dst[nbase + e] = lds[e];
+            return;
+        }
+    }
 
     // Spread (encoding.rs:323-340) and masks (encoding.rs:303-312): entry t times
     // 7^{rev_k(t)}/k to position s*t, mask terms at s*t and s*t + s/2, zero elsewhere.

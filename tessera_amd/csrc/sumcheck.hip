@@ -200,12 +200,23 @@ k_sc_transcript(uint32_t j, ScInst *__restrict__ inst, uint32_t count,
                 const uint8_t *__restrict__ labels, const uint32_t *__restrict__ label_lens,
                 fp2 *__restrict__ rounds_out, fp2 *__restrict__ chals_out) {
     extern __shared__ fp2 sums[];  // count x (D+1)
-    for (uint32_t x = threadIdx.x; x < count * (D + 1); x += SC_THREADS) {
-        uint32_t i = x / (D + 1), z = x % (D + 1);
+    __shared__ uint32_t hbuf[DEV_BUF_WORDS];
+    // each wave reduces slots x = wave, wave + 4, ...: lanes stride over the partial blocks,
+    // then a shuffle tree adds the 64 lane sums (field addition: any order gives the same sum)
+    const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    for (uint32_t x = wave; x < count * (D + 1); x += SC_THREADS / 64) {
+        const uint32_t i = x / (D + 1), z = x % (D + 1);
         fp2 s{0, 0};
         const fp2 *p = inst[i].partials;
-        for (uint32_t b = 0; b < nblk[i]; b++) s = gl::add(s, p[(uint64_t)b * (D + 1) + z]);
-        sums[x] = s;
+        for (uint32_t b = lane; b < nblk[i]; b += 64) s = gl::add(s, p[(uint64_t)b * (D + 1) + z]);
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) {
+            fp2 o;
+            o.c0 = __shfl_xor(s.c0, off, 64);
+            o.c1 = __shfl_xor(s.c1, off, 64);
+            s = gl::add(s, o);
+        }
+        if (lane == 0) sums[x] = s;
     }
     __syncthreads();
     if (threadIdx.x != 0) return;
@@ -225,8 +236,8 @@ k_sc_transcript(uint32_t j, ScInst *__restrict__ inst, uint32_t count,
         }
     }
     DevTranscript t = ctl->t;
-    dev_absorb_fp2s(t, keys, labels + (2 * j) * LABEL_SLOT, label_lens[2 * j], msg, D);
-    fp2 r = dev_challenge_fp2(t, keys, labels + (2 * j + 1) * LABEL_SLOT, label_lens[2 * j + 1]);
+    dev_absorb_fp2s(t, keys, labels + (2 * j) * LABEL_SLOT, label_lens[2 * j], msg, D, hbuf);
+    fp2 r = dev_challenge_fp2(t, keys, labels + (2 * j + 1) * LABEL_SLOT, label_lens[2 * j + 1], hbuf);
     ctl->t = t;
     ctl->r = r;
     for (int z = 0; z < D; z++) rounds_out[(uint64_t)j * D + z] = msg[z];

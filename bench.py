@@ -257,17 +257,22 @@ def run_batched(reps=3):
 
 def run_served(cpu_sample):
     """SERVE_REQUESTS requests, Poisson arrivals at SERVE_RATE_HZ, prompt+output lengths
-    U[16,512] (prompt a quarter): the server (continuous batching, 64 slots, 4 prover streams,
+    U[16,512] (prompt a quarter): the server (continuous batching, 128 slots, 4 prover streams,
     chunks of 32 positions) vs plain per-request calls on the same arrivals (each starts at the
-    later of its arrival and the previous call's end). Latency = completion - arrival."""
+    later of its arrival and the previous call's end). Latency = completion - arrival. Capacity:
+    the same requests offered all at once to the server."""
     from tessera_amd.serve import Server, report, run_plain as plain_calls, run_served as served_calls, workload
     reqs = workload(SERVE_REQUESTS, seed=0x5E7E, rate_hz=SERVE_RATE_HZ)
-    with Server(slots=64, provers=4, chunk=32, model_seed=SEED) as srv:
+    with Server(slots=128, provers=4, chunk=32, model_seed=SEED) as srv:
         warm = workload(8, seed=1, rate_hz=1e6)
         served_calls(srv, warm)
         res_s, wall_s, t0 = served_calls(srv, reqs)
         served = report(res_s, reqs, wall_s, t0)
         served["max_in_flight"] = srv.stats()[0]
+        # capacity: the same requests all offered at once
+        burst = [(rid + 10_000, p, n, 0.0) for rid, p, n, _ in reqs]
+        res_b, wall_b, t0_b = served_calls(srv, burst)
+        capacity = report(res_b, burst, wall_b, t0_b)
         res_p, lat_p, wall_p = plain_calls(srv, reqs)
     p50, p99 = np.percentile(lat_p, 50), np.percentile(lat_p, 99)
     service = [r.t_done - r.t_admit for r in res_p.values()]
@@ -277,7 +282,9 @@ def run_served(cpu_sample):
     out = {"rate_hz": SERVE_RATE_HZ, "requests": len(reqs),
            "served": {k: round(v, 3) if isinstance(v, float) else v for k, v in served.items()},
            "plain": {k: round(v, 3) for k, v in plain.items()},
+           "capacity": {k: round(v, 3) if isinstance(v, float) else v for k, v in capacity.items()},
            "speedup_proofs_per_s": round(served["proofs_per_s"] / plain["proofs_per_s"], 3),
+           "capacity_vs_plain": round(capacity["proofs_per_s"] / plain["proofs_per_s"], 3),
            "results_equal": bool(equal)}
     if cpu_sample:
         out["cpu"] = cpu_served_sample(reqs[:3])
@@ -354,7 +361,9 @@ def traffic_tables(fetch_head, fetch_copy, write_head, write_copy, out_json, out
         fv = sum(fetch[name]) / len(fetch[name]) * f_fac
         wv = sum(write.get(name, [0])) / max(1, len(write.get(name, [0]))) * w_fac
         if name.startswith("k_"):
-            table[name] = round(fv + wv)
+            launches = len(fetch[name])  # one headline step (--steps 1 --warmup 0)
+            table[name] = {"rows": ROWS // launches if name.startswith(("k_tile_pass", "k_encode_mid")) else ROWS,
+                           "bytes": round(fv + wv)}
         lines.append(f"| `{name}` | {len(fetch[name])} | {fv / 2**30:.4f} | {wv / 2**30:.4f} | "
                      f"{(fv + wv) / 2**30:.4f} |")
     with open(out_json, "w") as f:
@@ -363,16 +372,20 @@ def traffic_tables(fetch_head, fetch_copy, write_head, write_copy, out_json, out
         f.write("\n".join(lines) + "\n")
 
 
-def traffic_from_profiles(kernel):
+def traffic_from_profiles(kernel, rows):
     """HBM bytes per launch measured by rocprofv3 --pmc (FETCH_SIZE + WRITE_SIZE, scaled by a
-    copy kernel's reading), when a committed summary holds it; else None."""
+    copy kernel's reading) for launches of `rows` rows, when a committed summary holds it; else
+    None. The summary maps kernel -> {"rows": rows per launch, "bytes": bytes per launch}."""
     path = os.path.join(ROOT, "profiles", "traffic.json")
     alias = {"k_tile_pass<gs>": "k_tile_pass<true>", "k_tile_pass<ct>": "k_tile_pass<false>"}
     try:
         with open(path) as f:
-            return json.load(f).get(alias.get(kernel, kernel))
+            entry = json.load(f).get(alias.get(kernel, kernel))
     except (OSError, ValueError):
         return None
+    if not isinstance(entry, dict) or entry.get("rows") != rows:
+        return None
+    return entry["bytes"]
 
 
 def trace_summary(db, title, command):
@@ -470,13 +483,16 @@ def main():
     dom = max(kernels, key=lambda x: kernels[x]["total_ms"]) if kernels else None
     roofline = None
     if dom:
-        b = kernel_bytes(dom, ROWS, LOG_K, LOG_R)
+        # the encoder may split the rows over several launches (pipelined chunks)
+        rows_per_launch = ROWS * a.steps // kernels[dom]["launches"]
+        b = kernel_bytes(dom, rows_per_launch, LOG_K, LOG_R)
         achieved = b / (kernels[dom]["avg_ms"] * 1e-3) / 1e9
         roofline = {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 1),
                     "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
-                    "traffic": traffic_from_profiles(dom), "algorithmic_bytes": b,
+                    "traffic": traffic_from_profiles(dom, rows_per_launch), "algorithmic_bytes": b,
+                    "rows_per_launch": rows_per_launch,
                     "avg_ms": round(kernels[dom]["avg_ms"], 4)}
-        bf = kernel_butterflies(dom, ROWS, LOG_K, LOG_R)
+        bf = kernel_butterflies(dom, rows_per_launch, LOG_K, LOG_R)
         if bf:
             peak = butterfly_peak()
             rate = bf / (kernels[dom]["avg_ms"] * 1e-3)

@@ -138,6 +138,25 @@ typedef struct {
  * B = 0 (BAD_SHAPE), before any work. */
 tsr_status tsr_commit_unmasked(tsr_ctx *ctx, const tsr_commitment_params *params,
                                const uint64_t *rows, uint64_t count, tsr_commitment **out);
+/* ReferenceCommitment::commit (commitment.rs:318-326): the masked advice commitment of a layout
+ * of `segments` segments holding segment_rows[s] non-zero rows each (RowLayout, commitment.rs:
+ * 186-262; the layout's row indices are the caller's to check, only the counts enter). `rows`
+ * holds the N = sum(segment_rows) rows in layout order (N x k) and `masks` their P = mask_coeffs
+ * coefficients each, m_0 first, as drawn from the MaskSource (N x P). Every segment's columns
+ * are chained in blocks of B over that segment's rows (an empty segment keeps d_j[0]), leaf_j =
+ * H(column-merge, d_j^(1) || ... || d_j^(S)), and the tree is built over the n leaves.
+ * Errors, in this order and before any work: r = 0 or log_k + r > 32, P = 0 (NoMask), P > k
+ * (MaskTooLarge), no segment (NoSegments), B = 0 (BAD_SHAPE); B > 124 (DECLINED). */
+tsr_status tsr_commit(tsr_ctx *ctx, const tsr_commitment_params *params, const uint64_t *segment_rows,
+                      uint32_t segments, const uint64_t *rows, const uint64_t *masks,
+                      tsr_commitment **out);
+/* ReferenceCommitment::from_encoded (commitment.rs:418-450): steps 2 to 4 from N x n codewords
+ * already encoded (host or device; copied and kept resident): merged != 0 gives the masked
+ * commitment's column-merge leaves over the segments, merged = 0 the chained leaves of one
+ * segment (then P must be 0). Same errors as tsr_commit / tsr_commit_unmasked. */
+tsr_status tsr_commit_from_encoded(tsr_ctx *ctx, const tsr_commitment_params *params,
+                                   const uint64_t *segment_rows, uint32_t segments, int merged,
+                                   const uint64_t *codewords, tsr_commitment **out);
 tsr_status tsr_commitment_root(const tsr_commitment *c, uint8_t root[32]);
 /* n leaves, 32 bytes each, column order. */
 tsr_status tsr_commitment_leaves(const tsr_commitment *c, uint8_t *out);

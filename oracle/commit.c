@@ -196,3 +196,39 @@ int orc_commit_unmasked(uint32_t log_k, uint32_t log_r, uint64_t block, const ui
     free(chains);
     return rc;
 }
+
+/* ReferenceCommitment::commit -> encode -> from_encoded (commitment.rs:318-446) for the masked
+ * advice commitment: the N = sum(segment_rows) rows encoded with their P masks each (m_0
+ * first, in layout order), one set of chains per segment over that segment's rows in blocks of
+ * B (an empty segment keeps d_j[0]), leaf_j = H(column-merge, d_j^(1) || ... || d_j^(S)), the
+ * Merkle tree over the n leaves. `leaves` receives the n leaves (32 bytes each). */
+int orc_commit_masked(uint32_t log_k, uint32_t log_r, uint64_t P, uint64_t block,
+                      const uint8_t id[32], const uint64_t *segment_rows, uint32_t segments,
+                      const uint64_t *rows, const uint64_t *masks, uint64_t *codewords,
+                      uint8_t *leaves, uint8_t *nodes, int threads) {
+    if (block == 0 || log_r == 0 || log_k + log_r > 32 || P == 0 || P > (1ULL << log_k) ||
+        segments == 0)
+        return -1;
+    uint64_t n = 1ULL << (log_k + log_r), count = 0;
+    for (uint32_t s = 0; s < segments; s++) count += segment_rows[s];
+    if (orc_encode_rows(log_k, log_r, P, rows, masks, count, codewords, threads)) return -1;
+    uint8_t *digests = malloc(32 * n * (size_t)segments);
+    uint64_t off = 0;
+    for (uint32_t s = 0; s < segments; s++) {
+        uint8_t *chains = digests + 32 * n * (size_t)s;
+        orc_column_init(id, n, chains);
+        for (uint64_t r = 0; r < segment_rows[s]; r += block) {
+            uint64_t c = segment_rows[s] - r < block ? segment_rows[s] - r : block;
+            orc_column_absorb(chains, n, codewords + (off + r) * n, c, threads);
+        }
+        off += segment_rows[s];
+    }
+    for (uint64_t j = 0; j < n; j++) {
+        orc_hasher h;
+        orc_hasher_init(&h, ORC_DOM_COLUMN_MERGE);
+        for (uint32_t s = 0; s < segments; s++) orc_hasher_update(&h, digests + 32 * (n * s + j), 32);
+        orc_hasher_finalize(&h, leaves + 32 * j);
+    }
+    free(digests);
+    return orc_merkle_tree(leaves, n, nodes, threads);
+}

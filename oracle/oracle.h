@@ -104,6 +104,10 @@ int orc_merkle_verify(const uint8_t root[32], uint64_t n, const uint64_t *idx,
                       uint64_t n_siblings);
 /* ReferenceCommitment::commit_unmasked (commitment.rs:343-446): encode, chain every column over
  * all rows in blocks of `block`, leaves = chains, Merkle tree. nodes: 2n-1 digests. */
+int orc_commit_masked(uint32_t log_k, uint32_t log_r, uint64_t P, uint64_t block,
+                      const uint8_t id[32], const uint64_t *segment_rows, uint32_t segments,
+                      const uint64_t *rows, const uint64_t *masks, uint64_t *codewords,
+                      uint8_t *leaves, uint8_t *nodes, int threads);
 int orc_commit_unmasked(uint32_t log_k, uint32_t log_inv_rate, uint64_t block,
                         const uint8_t id[32], const uint64_t *rows, uint64_t count,
                         uint64_t *codewords, uint8_t *nodes, int threads);

@@ -79,6 +79,7 @@ def _declare(L):
         "orc_merkle_open": (i64, [vp, u64, vp, u64, vp, u64]),
         "orc_merkle_verify": (i32, [vp, u64, vp, vp, u64, vp, u64]),
         "orc_commit_unmasked": (i32, [u32, u32, u64, vp, vp, u64, vp, vp, i32]),
+        "orc_commit_masked": (i32, [u32, u32, u64, u64, vp, vp, u32, vp, vp, vp, vp, vp, i32]),
         "orc_transcript_init": (None, [vp, vp, sz]),
         "orc_transcript_absorb": (None, [vp, cp, vp, sz]),
         "orc_transcript_squeeze": (None, [vp, cp, vp]),
@@ -255,6 +256,27 @@ def merkle_verify(root: bytes, n, pairs, siblings):
     digs = b"".join(d for _, d in pairs)
     return bool(lib().orc_merkle_verify(root, n, _buf(idx), digs, len(pairs),
                                         b"".join(siblings) if siblings else None, len(siblings)))
+
+
+def commit_masked(rows, masks, log_k, log_r, P, block, commitment_id: bytes, segment_rows, threads=1):
+    """(codewords, leaves, nodes) of ReferenceCommitment::commit: masked rows, one chain set per
+    segment, column-merge leaves."""
+    rows = np.ascontiguousarray(rows, dtype=np.uint64).reshape(-1)
+    masks = np.ascontiguousarray(masks, dtype=np.uint64).reshape(-1)
+    seg = np.ascontiguousarray(np.asarray(segment_rows, dtype=np.uint64))
+    k = 1 << log_k
+    n = k << log_r
+    count = int(seg.sum())
+    cw = np.zeros(max(1, count * n), dtype=np.uint64)
+    leaves = ctypes.create_string_buffer(32 * n)
+    nodes = ctypes.create_string_buffer(32 * (2 * n - 1))
+    rc = lib().orc_commit_masked(log_k, log_r, P, block, commitment_id, _buf(seg), seg.size,
+                                 _buf(rows) if rows.size else None, _buf(masks) if masks.size else None,
+                                 _buf(cw), leaves, nodes, threads)
+    if rc:
+        raise ValueError("bad commitment shape")
+    raw = leaves.raw
+    return cw[:count * n].reshape(count, n), [raw[32 * j:32 * j + 32] for j in range(n)], nodes.raw
 
 
 def commit_unmasked(rows, log_k, log_r, block, commitment_id: bytes, threads=1):

@@ -567,6 +567,130 @@ tsr_status tsr_commit_unmasked(tsr_ctx *ctx, const tsr_commitment_params *params
     return TSR_OK;
 }
 
+}  // extern "C"
+
+// Steps 2 to 4 of a commitment from its resident codewords (ReferenceCommitment::from_encoded,
+// commitment.rs:418-450): chained leaves (one segment, leaf = the chain's last digest) or merged
+// leaves (one chain set per segment, leaf_j = H(column-merge, d_j^(1) || ... || d_j^(S))), the
+// Merkle tree, the root.
+static Status finish_commitment(tsr_commitment *c, const uint64_t *segment_rows, uint32_t segments,
+                                bool merged) {
+    Ctx *ctx = c->ctx;
+    const uint64_t n = c->n;
+    const uint32_t block = (uint32_t)c->params.row_block;
+    Status st;
+    if (!merged) {
+        if ((st = chain_leaves(ctx, c->codewords.ptr, n, c->count, block, c->params.commitment_id,
+                               c->nodes.ptr, ctx->stream)))
+            return st;
+    } else {
+        DevBuf<uint32_t> dig;
+        if ((st = dig.alloc(ctx, 8 * n * (uint64_t)segments))) return st;
+        uint64_t off = 0;
+        for (uint32_t sgm = 0; sgm < segments; sgm++) {
+            if ((st = chain_leaves(ctx, c->codewords.ptr + off * n, n, segment_rows[sgm], block,
+                                   c->params.commitment_id, dig.ptr + 8 * n * (uint64_t)sgm, ctx->stream)))
+                return st;
+            off += segment_rows[sgm];
+        }
+        if ((st = merge_leaves(ctx, dig.ptr, segments, n, c->nodes.ptr, ctx->stream))) return st;
+    }
+    if ((st = merkle_build(ctx, c->nodes.ptr, n, ctx->stream))) return st;
+    TSR_HIP(hipMemcpyAsync(c->root, c->nodes.ptr + 8 * (2 * n - 2), 32, hipMemcpyDeviceToHost, ctx->stream));
+    TSR_HIP(hipStreamSynchronize(ctx->stream));
+    return Status::ok();
+}
+
+// The shape checks of a commitment with `segments` segments, in the reference's order
+// (CommitmentParams::shape / unmasked_shape, commitment.rs:145-185); the rows they sum to.
+static Status commitment_shape(const tsr_commitment_params *params, bool masked,
+                               const uint64_t *segment_rows, uint32_t segments, uint64_t *count) {
+    tsr_status s = check_code(params->log2_k, params->log2_inv_rate);
+    if (s) return Status{s};
+    const uint64_t k = 1ull << params->log2_k;
+    if (masked && params->mask_coeffs == 0)
+        return Status::fail(TSR_BAD_SHAPE, "NoMask: a masked commitment has P >= 1");
+    if (masked && params->mask_coeffs > k)
+        return Status::fail(TSR_BAD_SHAPE, "MaskTooLarge: mask_coeffs exceeds k");
+    if (!masked && params->mask_coeffs != 0)
+        return Status::fail(TSR_BAD_SHAPE, "UnexpectedMask: an unmasked commitment has P = 0");
+    if (segments == 0) return Status::fail(TSR_BAD_SHAPE, "NoSegments: a layout has a segment");
+    if (params->row_block == 0) return Status::fail(TSR_BAD_SHAPE, "the row block B is at least 1");
+    if (params->row_block > 124)
+        return Status::fail(TSR_DECLINED, "row blocks above 124 rows are hashed on the CPU");
+    uint64_t total = 0;
+    for (uint32_t i = 0; i < segments; i++) total += segment_rows[i];
+    *count = total;
+    return Status::ok();
+}
+
+extern "C" {
+
+tsr_status tsr_commit(tsr_ctx *ctx, const tsr_commitment_params *params, const uint64_t *segment_rows,
+                      uint32_t segments, const uint64_t *rows, const uint64_t *masks,
+                      tsr_commitment **out) {
+    CHECK_ARG(ctx && params && out && (segments == 0 || segment_rows), "null argument");
+    uint64_t count = 0;
+    TRY(commitment_shape(params, true, segment_rows, segments, &count));
+    CHECK_ARG(count == 0 || (rows && masks), "null rows or masks");
+    TRY_HIP(hipSetDevice(ctx->device));
+    const uint32_t log_k = params->log2_k, log_r = params->log2_inv_rate;
+    const uint64_t k = 1ull << log_k, n = k << log_r, P = params->mask_coeffs;
+    std::unique_ptr<tsr_commitment> c(new tsr_commitment());
+    c->ctx = ctx;
+    c->params = *params;
+    c->count = count;
+    c->n = n;
+    TRY(c->nodes.alloc(ctx, 8 * (2 * n - 1)));
+    TRY(c->codewords.alloc(ctx, count * n));
+    if (count) {
+        const uint64_t *drows = rows, *dmasks = masks;
+        DevBuf<uint64_t> in_buf, mask_buf;
+        if (!is_device_ptr(rows)) {
+            TRY(in_buf.alloc(ctx, count * k));
+            TRY_HIP(hipMemcpyAsync(in_buf.ptr, rows, 8 * count * k, hipMemcpyHostToDevice, ctx->stream));
+            drows = in_buf.ptr;
+        }
+        if (!is_device_ptr(masks)) {
+            TRY(mask_buf.alloc(ctx, count * P));
+            TRY_HIP(hipMemcpyAsync(mask_buf.ptr, masks, 8 * count * P, hipMemcpyHostToDevice, ctx->stream));
+            dmasks = mask_buf.ptr;
+        }
+        Encoder *enc = nullptr;
+        TRY(get_encoder(ctx, log_k, log_r, P, &enc));
+        TRY(ctx->scratch.reserve(ctx, enc->scratch_elems(count)));
+        TRY(enc->encode(drows, dmasks, count, c->codewords.ptr, ctx->scratch.ptr, ctx->stream));
+        TRY_HIP(hipStreamSynchronize(ctx->stream));  // the staged rows and masks are released below
+    }
+    TRY(finish_commitment(c.get(), segment_rows, segments, true));
+    *out = c.release();
+    return TSR_OK;
+}
+
+tsr_status tsr_commit_from_encoded(tsr_ctx *ctx, const tsr_commitment_params *params,
+                                   const uint64_t *segment_rows, uint32_t segments, int merged,
+                                   const uint64_t *codewords, tsr_commitment **out) {
+    CHECK_ARG(ctx && params && out && (segments == 0 || segment_rows), "null argument");
+    uint64_t count = 0;
+    TRY(commitment_shape(params, merged != 0, segment_rows, segments, &count));
+    CHECK_SHAPE(merged || segments == 1, "a commitment of chained leaves has one segment");
+    CHECK_ARG(count == 0 || codewords, "null codewords");
+    TRY_HIP(hipSetDevice(ctx->device));
+    const uint64_t n = 1ull << (params->log2_k + params->log2_inv_rate);
+    std::unique_ptr<tsr_commitment> c(new tsr_commitment());
+    c->ctx = ctx;
+    c->params = *params;
+    c->count = count;
+    c->n = n;
+    TRY(c->nodes.alloc(ctx, 8 * (2 * n - 1)));
+    TRY(c->codewords.alloc(ctx, count * n));
+    if (count)
+        TRY_HIP(hipMemcpyAsync(c->codewords.ptr, codewords, 8 * count * n, hipMemcpyDefault, ctx->stream));
+    TRY(finish_commitment(c.get(), segment_rows, segments, merged != 0));
+    *out = c.release();
+    return TSR_OK;
+}
+
 tsr_status tsr_commitment_root(const tsr_commitment *c, uint8_t root[32]) {
     CHECK_ARG(c && root, "null argument");
     memcpy(root, c->root, 32);

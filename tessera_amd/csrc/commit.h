@@ -9,6 +9,9 @@ namespace tsr {
 Status merkle_build(Ctx *ctx, uint32_t *nodes, uint64_t n, hipStream_t stream);
 Status chain_leaves(Ctx *ctx, const uint64_t *cw, uint64_t n, uint64_t rows, uint32_t block,
                     const uint8_t id[32], uint32_t *leaves, hipStream_t stream);
+// leaf_j = H(column-merge, d_j^(1) || ... || d_j^(S)) from S segments' chains (segment-major).
+Status merge_leaves(Ctx *ctx, const uint32_t *digests, uint32_t segments, uint64_t n,
+                    uint32_t *leaves, hipStream_t stream);
 Status chains_init(Ctx *ctx, const uint8_t id[32], uint64_t n, uint32_t *chains,
                    hipStream_t stream);
 Status chains_absorb(Ctx *ctx, const uint64_t *rows, uint64_t n, uint32_t count,

@@ -247,6 +247,75 @@ Status chain_leaves(Ctx *ctx, const uint64_t *cw, uint64_t n, uint64_t rows, uin
     return check_launch("column chains");
 }
 
+// leaf_j = H(column-merge, d_j^(1) || ... || d_j^(S)) (commitment.rs:434-446): BLAKE3 in
+// derive-key mode over the S digests of column j (segment s's at dig[(s n + j) 8 ..]), 1024-byte
+// chunks of 32 digests, chunk chaining values merged into the BLAKE3 tree when S > 32.
+__global__ void k_merge_leaves(const uint32_t *__restrict__ dig, uint32_t S, uint64_t n, Key8 key,
+                               uint32_t *__restrict__ leaves) {
+    const uint64_t j = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= n) return;
+    const uint32_t chunks = (S + 31) / 32;
+    uint32_t stack[32][8];
+    int depth = 0;
+    uint32_t out[8];
+    for (uint32_t c = 0; c < chunks; c++) {
+        uint32_t cv[8];
+        for (int i = 0; i < 8; i++) cv[i] = key.w[i];
+        const uint32_t d0 = 32 * c, dn = S - d0 < 32 ? S - d0 : 32, nb = (dn + 1) / 2;
+        for (uint32_t b = 0; b < nb; b++) {
+            uint32_t m[16];
+            const uint32_t *a = dig + ((uint64_t)(d0 + 2 * b) * n + j) * 8;
+            const bool pair = 2 * b + 1 < dn;
+            const uint32_t *bb = dig + ((uint64_t)(d0 + 2 * b + 1) * n + j) * 8;
+            for (int i = 0; i < 8; i++) {
+                m[i] = a[i];
+                m[8 + i] = pair ? bb[i] : 0;
+            }
+            uint32_t flags = b3::DERIVE_KEY_MATERIAL;
+            if (b == 0) flags |= b3::CHUNK_START;
+            if (b == nb - 1) flags |= b3::CHUNK_END;
+            if (chunks == 1 && b == nb - 1) flags |= b3::ROOT;
+            b3::compress(cv, m, c, pair ? 64 : 32, flags, cv);
+        }
+        if (c + 1 < chunks) {
+            // a chunk followed by more input: push its chaining value, merging complete subtrees
+            uint32_t total = c + 1;
+            while ((total & 1) == 0) {
+                uint32_t m[16];
+                depth--;
+                for (int i = 0; i < 8; i++) {
+                    m[i] = stack[depth][i];
+                    m[8 + i] = cv[i];
+                }
+                b3::compress(key.w, m, 0, 64, b3::DERIVE_KEY_MATERIAL | b3::PARENT, cv);
+                total >>= 1;
+            }
+            for (int i = 0; i < 8; i++) stack[depth][i] = cv[i];
+            depth++;
+        } else {
+            for (int i = 0; i < 8; i++) out[i] = cv[i];
+        }
+    }
+    while (depth > 0) {
+        uint32_t m[16];
+        depth--;
+        for (int i = 0; i < 8; i++) {
+            m[i] = stack[depth][i];
+            m[8 + i] = out[i];
+        }
+        b3::compress(key.w, m, 0, 64, b3::DERIVE_KEY_MATERIAL | b3::PARENT | (depth == 0 ? b3::ROOT : 0u), out);
+    }
+    for (int i = 0; i < 8; i++) leaves[j * 8 + i] = out[i];
+}
+
+Status merge_leaves(Ctx *ctx, const uint32_t *digests, uint32_t segments, uint64_t n,
+                    uint32_t *leaves, hipStream_t stream) {
+    KernelTimer timer(ctx, "k_merge_leaves", stream);
+    hipLaunchKernelGGL(k_merge_leaves, dim3(blocks_for(n, 256)), dim3(256), 0, stream, digests, segments, n,
+                       key_of(ctx, DOM_COLUMN_MERGE), leaves);
+    return check_launch("column merge");
+}
+
 Status chains_init(Ctx *ctx, const uint8_t id[32], uint64_t n, uint32_t *chains,
                    hipStream_t stream) {
     Key8 idk;

@@ -147,6 +147,40 @@ class ReferenceCommitment:
                                         ctypes.byref(h)))
         return cls(device, h.value, params, count)
 
+    @classmethod
+    def commit(cls, device: Device, params: CommitmentParamsPy, segment_rows, rows, masks):
+        """ReferenceCommitment::commit: the masked advice commitment of a layout whose segments
+        hold segment_rows[s] rows; `rows` (N x k) and `masks` (N x P, m_0 first) in layout order."""
+        seg = np.ascontiguousarray(np.asarray(segment_rows, dtype=np.uint64))
+        keep = []
+
+        def p(x):
+            if isinstance(x, DeviceBuffer):
+                return x.ptr
+            a = np.ascontiguousarray(x, dtype=np.uint64).reshape(-1)
+            keep.append(a)
+            return a.ctypes.data if a.size else None
+        h = vp()
+        cp = params.c()
+        check(lib().tsr_commit(device.handle, ctypes.byref(cp), seg.ctypes.data if seg.size else None,
+                               seg.size, p(rows), p(masks), ctypes.byref(h)))
+        return cls(device, h.value, params, int(seg.sum()))
+
+    @classmethod
+    def from_encoded(cls, device: Device, params: CommitmentParamsPy, segment_rows, codewords,
+                     merged=True):
+        """ReferenceCommitment::from_encoded: steps 2 to 4 from N x n codewords."""
+        seg = np.ascontiguousarray(np.asarray(segment_rows, dtype=np.uint64))
+        cw = codewords if isinstance(codewords, DeviceBuffer) else \
+            np.ascontiguousarray(codewords, dtype=np.uint64).reshape(-1)
+        h = vp()
+        cp = params.c()
+        ptr = cw.ptr if isinstance(cw, DeviceBuffer) else (cw.ctypes.data if cw.size else None)
+        check(lib().tsr_commit_from_encoded(device.handle, ctypes.byref(cp),
+                                            seg.ctypes.data if seg.size else None, seg.size,
+                                            int(merged), ptr, ctypes.byref(h)))
+        return cls(device, h.value, params, int(seg.sum()))
+
     def root(self) -> bytes:
         out = ctypes.create_string_buffer(32)
         check(lib().tsr_commitment_root(self.handle, out))

@@ -90,3 +90,35 @@ def test_shape_errors(dev):
     with pytest.raises(T.BadShape):
         T.ReferenceCommitment.from_encoded(dev, T.CommitmentParamsPy(3, 1, 0, 4), [1, 1],
                                            np.zeros(32, dtype=np.uint64), merged=False)
+
+
+STREAMING = load("pcs_streaming.json")["cases"]
+
+
+@pytest.mark.parametrize("case", STREAMING, ids=[c["name"] for c in STREAMING])
+def test_streaming_vectors(dev, case):
+    """The reference's streaming advice commitments: the whole commitment at once (leaves, root),
+    and the streaming path, one BlockSink per segment fed block by block (each segment's digests)."""
+    import tessera_amd as T
+    from tessera_amd.sinks import BlockSink
+    log_k = case["layout"]["log2_k"]
+    k, n = 1 << log_k, len(case["leaves"])
+    log_r = n.bit_length() - 1 - log_k
+    P_, B = case["mask_coeffs"], case["row_block"]
+    rows = np.array([h(x) for r in case["rows"] for x in r], dtype=np.uint64)
+    masks = np.array([h(x) for m in case["masks"] for x in m], dtype=np.uint64)
+    seg = [len(s) for s in case["segments"]]
+    cid = bytes.fromhex(case["commitment_id"])
+    c = T.ReferenceCommitment.commit(dev, T.CommitmentParamsPy(log_k, log_r, P_, B, cid), seg, rows, masks)
+    assert [x.hex() for x in c.leaves()] == case["leaves"]
+    assert c.root().hex() == case["root"]
+    c.free()
+    off = 0
+    for s, count in enumerate(seg):
+        sink = BlockSink(dev, log_k, log_r, P_, B, cid)
+        for r0 in range(off, off + count, B):
+            r1 = min(off + count, r0 + B)
+            sink.absorb(rows[r0 * k:r1 * k], masks[r0 * P_:r1 * P_])
+        assert [d.hex() for d in sink.finish()] == case["segment_digests"][s]
+        sink.free()
+        off += count

@@ -24,3 +24,34 @@ def test_masked_commitment_matches_vectors(case):
         assert [int(x) for x in cw[:, j]] == [h(x) for x in col]
     n = len(leaves)
     assert O.merkle_open(nodes, n, case["queries"]) == [bytes.fromhex(x) for x in case["siblings"]]
+
+
+STREAMING = load("pcs_streaming.json")["cases"]
+
+
+def streaming_inputs(case):
+    k = 1 << case["layout"]["log2_k"]
+    rows = [h(x) for r in case["rows"] for x in r]
+    masks = [h(x) for m in case["masks"] for x in m]
+    seg = [len(s) for s in case["segments"]]
+    n = len(case["leaves"])
+    return k, n.bit_length() - 1 - case["layout"]["log2_k"], rows, masks, seg
+
+
+@pytest.mark.parametrize("case", STREAMING, ids=[c["name"] for c in STREAMING])
+def test_streaming_commitments_match_vectors(case):
+    """The reference's streaming advice commitments (canonical layouts of S = 1, 2, 8 segments,
+    empty commitments, single rows): codewords, every segment's digests, leaves and root."""
+    k, log_r, rows, masks, seg = streaming_inputs(case)
+    log_k = case["layout"]["log2_k"]
+    cid = bytes.fromhex(case["commitment_id"])
+    cw, leaves, nodes = O.commit_masked(rows, masks, log_k, log_r, case["mask_coeffs"],
+                                        case["row_block"], cid, seg)
+    assert [[int(x) for x in r] for r in cw] == [[h(x) for x in r] for r in case["codewords"]]
+    off = 0
+    for s, count in enumerate(seg):
+        digests = O.column_chains(cid, cw[off:off + count].reshape(-1), len(leaves), case["row_block"])
+        assert [d.hex() for d in digests] == case["segment_digests"][s]
+        off += count
+    assert [x.hex() for x in leaves] == case["leaves"]
+    assert nodes[-32:].hex() == case["root"]

@@ -157,6 +157,21 @@ tsr_status tsr_commit(tsr_ctx *ctx, const tsr_commitment_params *params, const u
 tsr_status tsr_commit_from_encoded(tsr_ctx *ctx, const tsr_commitment_params *params,
                                    const uint64_t *segment_rows, uint32_t segments, int merged,
                                    const uint64_t *codewords, tsr_commitment **out);
+/* RowLayout::row_coefficients (crates/tessera-pcs/src/opening.rs:116-158): for each claim group g
+ * (group_sizes[g] claims, listed group after group, each a row point of row_vars Fp2 in
+ * row_points and a gamma), c_g(row_i) = Sum_claims gamma eq(row_point, row_i) over the N layout
+ * rows `rows` (row indices, layout order), then the proximity row eq(xi_point, row_i).
+ * out: (groups + 1) x N Fp2 (host or device). BAD_SHAPE for row_vars > 64. */
+tsr_status tsr_row_coefficients(tsr_ctx *ctx, uint32_t row_vars, const uint64_t *rows, uint64_t N,
+                                const uint64_t *row_points, const uint64_t *gammas,
+                                const uint32_t *group_sizes, uint32_t groups, const uint64_t *xi_point,
+                                uint64_t *out);
+/* ReferenceCommitment::open's combine (commitment.rs:489-524, opening.rs:241-330) over a masked
+ * commitment's resident rows and masks: for each of `count` coefficient rows (count x N Fp2,
+ * e.g. tsr_row_coefficients' output), u = Sum_row c(row) A_row (k Fp2) and M = Sum_row c(row)
+ * m_row (P Fp2). BAD_SHAPE for an unmasked commitment, which keeps no rows. */
+tsr_status tsr_commitment_open(const tsr_commitment *c, const uint64_t *coefficients, uint32_t count,
+                               uint64_t *u_out, uint64_t *mask_out);
 tsr_status tsr_commitment_root(const tsr_commitment *c, uint8_t root[32]);
 /* n leaves, 32 bytes each, column order. */
 tsr_status tsr_commitment_leaves(const tsr_commitment *c, uint8_t *out);

@@ -104,6 +104,12 @@ int orc_merkle_verify(const uint8_t root[32], uint64_t n, const uint64_t *idx,
                       uint64_t n_siblings);
 /* ReferenceCommitment::commit_unmasked (commitment.rs:343-446): encode, chain every column over
  * all rows in blocks of `block`, leaves = chains, Merkle tree. nodes: 2n-1 digests. */
+void orc_row_coefficients(uint32_t row_vars, const uint64_t *rows, uint64_t N,
+                          const orc_fp2 *row_points, const orc_fp2 *gammas,
+                          const uint32_t *group_sizes, uint32_t groups, const orc_fp2 *xi_point,
+                          orc_fp2 *out);
+void orc_combine(const orc_fp2 *coefficients, uint32_t count, const uint64_t *matrix, uint64_t N,
+                 uint64_t width, orc_fp2 *out);
 int orc_commit_masked(uint32_t log_k, uint32_t log_r, uint64_t P, uint64_t block,
                       const uint8_t id[32], const uint64_t *segment_rows, uint32_t segments,
                       const uint64_t *rows, const uint64_t *masks, uint64_t *codewords,

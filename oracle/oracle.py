@@ -80,6 +80,8 @@ def _declare(L):
         "orc_merkle_verify": (i32, [vp, u64, vp, vp, u64, vp, u64]),
         "orc_commit_unmasked": (i32, [u32, u32, u64, vp, vp, u64, vp, vp, i32]),
         "orc_commit_masked": (i32, [u32, u32, u64, u64, vp, vp, u32, vp, vp, vp, vp, vp, i32]),
+        "orc_row_coefficients": (None, [u32, vp, u64, vp, vp, vp, u32, vp, vp]),
+        "orc_combine": (None, [vp, u32, vp, u64, u64, vp]),
         "orc_transcript_init": (None, [vp, vp, sz]),
         "orc_transcript_absorb": (None, [vp, cp, vp, sz]),
         "orc_transcript_squeeze": (None, [vp, cp, vp]),
@@ -277,6 +279,33 @@ def commit_masked(rows, masks, log_k, log_r, P, block, commitment_id: bytes, seg
         raise ValueError("bad commitment shape")
     raw = leaves.raw
     return cw[:count * n].reshape(count, n), [raw[32 * j:32 * j + 32] for j in range(n)], nodes.raw
+
+
+def row_coefficients(row_vars, rows, groups, xi_point):
+    """RowLayout::row_coefficients: groups = [[(row_point (row_vars x 2), gamma (2,)), ...], ...];
+    returns (G + 1) x N x 2."""
+    rows = np.ascontiguousarray(np.asarray(rows, dtype=np.uint64))
+    pts = [np.asarray(p, dtype=np.uint64).reshape(-1, 2) for grp in groups for p, _ in grp]
+    gam = [np.asarray(gm, dtype=np.uint64).reshape(2) for grp in groups for _, gm in grp]
+    points = np.ascontiguousarray(np.concatenate(pts) if pts else np.zeros((1, 2), np.uint64))
+    gammas = np.ascontiguousarray(np.stack(gam) if gam else np.zeros((1, 2), np.uint64))
+    sizes = np.ascontiguousarray(np.array([len(g) for g in groups] or [0], dtype=np.uint32))
+    xi = np.ascontiguousarray(np.asarray(xi_point, dtype=np.uint64).reshape(-1, 2) if row_vars else np.zeros((1, 2), np.uint64))
+    out = np.zeros(((len(groups) + 1) * max(1, rows.size), 2), dtype=np.uint64)
+    lib().orc_row_coefficients(row_vars, _buf(rows) if rows.size else None, rows.size, _buf(points),
+                               _buf(gammas), _buf(sizes), len(groups), _buf(xi), _buf(out))
+    return out[:(len(groups) + 1) * rows.size].reshape(len(groups) + 1, rows.size, 2)
+
+
+def combine(coefficients, matrix, width):
+    """combine: out[g][i] = Sum_r coefficients[g][r] matrix[r][i]; returns count x width x 2."""
+    co = np.ascontiguousarray(np.asarray(coefficients, dtype=np.uint64).reshape(-1, 2))
+    m = np.ascontiguousarray(np.asarray(matrix, dtype=np.uint64).reshape(-1))
+    count = np.asarray(coefficients).shape[0]
+    N = m.size // width if width else 0
+    out = np.zeros((count * max(1, width), 2), dtype=np.uint64)
+    lib().orc_combine(_buf(co) if co.size else None, count, _buf(m) if m.size else None, N, width, _buf(out))
+    return out[:count * width].reshape(count, width, 2)
 
 
 def commit_unmasked(rows, log_k, log_r, block, commitment_id: bytes, threads=1):

@@ -24,6 +24,10 @@ struct tsr_commitment {
     tsr::DevBuf<uint64_t> codewords;
     tsr::DevBuf<uint32_t> nodes;  // (2n-1) x 8 words
     uint8_t root[32];
+    // A masked (advice) commitment keeps its rows (N x k) and masks (N x P) for its openings;
+    // they and the codewords are secret-class and zeroed when it is freed.
+    tsr::DevBuf<uint64_t> rows, masks;
+    bool secret = false;
 };
 
 struct tsr_chains {
@@ -643,24 +647,16 @@ tsr_status tsr_commit(tsr_ctx *ctx, const tsr_commitment_params *params, const u
     c->n = n;
     TRY(c->nodes.alloc(ctx, 8 * (2 * n - 1)));
     TRY(c->codewords.alloc(ctx, count * n));
+    c->secret = true;
+    TRY(c->rows.alloc(ctx, count * k));
+    TRY(c->masks.alloc(ctx, count * P));
     if (count) {
-        const uint64_t *drows = rows, *dmasks = masks;
-        DevBuf<uint64_t> in_buf, mask_buf;
-        if (!is_device_ptr(rows)) {
-            TRY(in_buf.alloc(ctx, count * k));
-            TRY_HIP(hipMemcpyAsync(in_buf.ptr, rows, 8 * count * k, hipMemcpyHostToDevice, ctx->stream));
-            drows = in_buf.ptr;
-        }
-        if (!is_device_ptr(masks)) {
-            TRY(mask_buf.alloc(ctx, count * P));
-            TRY_HIP(hipMemcpyAsync(mask_buf.ptr, masks, 8 * count * P, hipMemcpyHostToDevice, ctx->stream));
-            dmasks = mask_buf.ptr;
-        }
+        TRY_HIP(hipMemcpyAsync(c->rows.ptr, rows, 8 * count * k, hipMemcpyDefault, ctx->stream));
+        TRY_HIP(hipMemcpyAsync(c->masks.ptr, masks, 8 * count * P, hipMemcpyDefault, ctx->stream));
         Encoder *enc = nullptr;
         TRY(get_encoder(ctx, log_k, log_r, P, &enc));
         TRY(ctx->scratch.reserve(ctx, enc->scratch_elems(count)));
-        TRY(enc->encode(drows, dmasks, count, c->codewords.ptr, ctx->scratch.ptr, ctx->stream));
-        TRY_HIP(hipStreamSynchronize(ctx->stream));  // the staged rows and masks are released below
+        TRY(enc->encode(c->rows.ptr, c->masks.ptr, count, c->codewords.ptr, ctx->scratch.ptr, ctx->stream));
     }
     TRY(finish_commitment(c.get(), segment_rows, segments, true));
     *out = c.release();
@@ -764,9 +760,81 @@ tsr_status tsr_commitment_answer(const tsr_commitment *c, const uint64_t *querie
     return TSR_OK;
 }
 
+tsr_status tsr_row_coefficients(tsr_ctx *ctx, uint32_t row_vars, const uint64_t *rows, uint64_t N,
+                                const uint64_t *row_points, const uint64_t *gammas,
+                                const uint32_t *group_sizes, uint32_t groups, const uint64_t *xi_point,
+                                uint64_t *out) {
+    CHECK_ARG(ctx && out && (N == 0 || rows) && (groups == 0 || group_sizes) && (row_vars == 0 || xi_point),
+              "null argument");
+    CHECK_SHAPE(row_vars <= 64, "RowVars: a row index has at most 64 bits");
+    TRY_HIP(hipSetDevice(ctx->device));
+    std::vector<uint32_t> first(groups + 1, 0);
+    for (uint32_t g = 0; g < groups; g++) first[g + 1] = first[g] + group_sizes[g];
+    const uint64_t claims = first[groups];
+    CHECK_ARG(claims == 0 || (gammas && (row_vars == 0 || row_points)), "null claims");
+    if (N == 0) return TSR_OK;
+    hipStream_t s = ctx->stream;
+    DevBuf<uint64_t> d_rows, d_points, d_gammas, d_xi;
+    DevBuf<uint32_t> d_first;
+    DevBuf<gl::fp2> d_out;
+    TRY(d_rows.alloc(ctx, N));
+    TRY(d_points.alloc(ctx, std::max<uint64_t>(1, 2 * claims * row_vars)));
+    TRY(d_gammas.alloc(ctx, std::max<uint64_t>(1, 2 * claims)));
+    TRY(d_xi.alloc(ctx, std::max<uint32_t>(1, 2 * row_vars)));
+    TRY(d_first.alloc(ctx, groups + 1));
+    TRY_HIP(hipMemcpyAsync(d_rows.ptr, rows, 8 * N, hipMemcpyDefault, s));
+    if (claims && row_vars) TRY_HIP(hipMemcpyAsync(d_points.ptr, row_points, 16 * claims * row_vars, hipMemcpyDefault, s));
+    if (claims) TRY_HIP(hipMemcpyAsync(d_gammas.ptr, gammas, 16 * claims, hipMemcpyDefault, s));
+    if (row_vars) TRY_HIP(hipMemcpyAsync(d_xi.ptr, xi_point, 16 * row_vars, hipMemcpyDefault, s));
+    TRY_HIP(hipMemcpyAsync(d_first.ptr, first.data(), 4 * (groups + 1), hipMemcpyHostToDevice, s));
+    gl::fp2 *dst = (gl::fp2 *)out;
+    if (!is_device_ptr(out)) {
+        TRY(d_out.alloc(ctx, (uint64_t)(groups + 1) * N));
+        dst = d_out.ptr;
+    }
+    TRY(row_coefficients(ctx, row_vars, d_rows.ptr, N, (const gl::fp2 *)d_points.ptr,
+                         (const gl::fp2 *)d_gammas.ptr, d_first.ptr, groups, (const gl::fp2 *)d_xi.ptr,
+                         dst, s));
+    if (!is_device_ptr(out))
+        TRY_HIP(hipMemcpyAsync(out, dst, 16 * (uint64_t)(groups + 1) * N, hipMemcpyDeviceToHost, s));
+    TRY_HIP(hipStreamSynchronize(s));
+    return TSR_OK;
+}
+
+tsr_status tsr_commitment_open(const tsr_commitment *c, const uint64_t *coefficients, uint32_t count,
+                               uint64_t *u_out, uint64_t *mask_out) {
+    CHECK_ARG(c && (count == 0 || (coefficients && u_out && mask_out)), "null argument");
+    CHECK_SHAPE(c->secret, "open combines the rows of a masked (advice) commitment, which keeps them");
+    Ctx *ctx = c->ctx;
+    TRY_HIP(hipSetDevice(ctx->device));
+    if (count == 0) return TSR_OK;
+    hipStream_t s = ctx->stream;
+    const uint64_t N = c->count, k = 1ull << c->params.log2_k, P = c->params.mask_coeffs;
+    DevBuf<gl::fp2> d_coef, d_u, d_m;
+    const gl::fp2 *coef = (const gl::fp2 *)coefficients;
+    if (!is_device_ptr(coefficients)) {
+        TRY(d_coef.alloc(ctx, std::max<uint64_t>(1, (uint64_t)count * N)));
+        if (N) TRY_HIP(hipMemcpyAsync(d_coef.ptr, coefficients, 16 * (uint64_t)count * N, hipMemcpyHostToDevice, s));
+        coef = d_coef.ptr;
+    }
+    TRY(d_u.alloc(ctx, (uint64_t)count * k));
+    TRY(d_m.alloc(ctx, (uint64_t)count * P));
+    TRY(combine_rows(ctx, coef, count, c->rows.ptr, N, k, d_u.ptr, s));
+    TRY(combine_rows(ctx, coef, count, c->masks.ptr, N, P, d_m.ptr, s));
+    TRY_HIP(hipMemcpyAsync(u_out, d_u.ptr, 16 * (uint64_t)count * k, hipMemcpyDefault, s));
+    TRY_HIP(hipMemcpyAsync(mask_out, d_m.ptr, 16 * (uint64_t)count * P, hipMemcpyDefault, s));
+    TRY_HIP(hipStreamSynchronize(s));
+    return TSR_OK;
+}
+
 void tsr_commitment_free(tsr_commitment *c) {
     if (!c) return;
     (void)hipSetDevice(c->ctx->device);
+    if (c->secret) {
+        zero_words(c->rows.ptr, c->rows.len, c->ctx->stream);
+        zero_words(c->masks.ptr, c->masks.len, c->ctx->stream);
+        zero_words(c->codewords.ptr, c->codewords.len, c->ctx->stream);
+    }
     (void)hipStreamSynchronize(c->ctx->stream);
     delete c;
 }

@@ -3,6 +3,7 @@
 #pragma once
 #include <vector>
 #include "common.h"
+#include "field.h"
 
 namespace tsr {
 
@@ -12,6 +13,14 @@ Status chain_leaves(Ctx *ctx, const uint64_t *cw, uint64_t n, uint64_t rows, uin
 // leaf_j = H(column-merge, d_j^(1) || ... || d_j^(S)) from S segments' chains (segment-major).
 Status merge_leaves(Ctx *ctx, const uint32_t *digests, uint32_t segments, uint64_t n,
                     uint32_t *leaves, hipStream_t stream);
+// The opening's row coefficients ((groups + 1) x N Fp2) and combined rows (opening.hip).
+Status row_coefficients(Ctx *ctx, uint32_t row_vars, const uint64_t *rows, uint64_t N,
+                        const gl::fp2 *points, const gl::fp2 *gammas, const uint32_t *first,
+                        uint32_t groups, const gl::fp2 *xi, gl::fp2 *out, hipStream_t s);
+Status combine_rows(Ctx *ctx, const gl::fp2 *coef, uint32_t count, const uint64_t *m, uint64_t N,
+                    uint64_t width, gl::fp2 *out, hipStream_t s);
+// Zeroes n words (a fill kernel, usable where a memset node is not wanted).
+void zero_words(uint64_t *p, uint64_t n, hipStream_t s);
 Status chains_init(Ctx *ctx, const uint8_t id[32], uint64_t n, uint32_t *chains,
                    hipStream_t stream);
 Status chains_absorb(Ctx *ctx, const uint64_t *rows, uint64_t n, uint32_t count,

@@ -93,6 +93,8 @@ _SIGS = {
     "tsr_commit_batch_free": (None, [vp]),
     "tsr_eq_table": (i32, [vp, vp, u32, vp]),
     "tsr_commit": (i32, [vp, vp, vp, u32, vp, vp, ctypes.POINTER(vp)]),
+    "tsr_row_coefficients": (i32, [vp, u32, vp, u64, vp, vp, vp, u32, vp, vp]),
+    "tsr_commitment_open": (i32, [vp, vp, u32, vp, vp]),
     "tsr_commit_from_encoded": (i32, [vp, vp, vp, u32, i32, vp, ctypes.POINTER(vp)]),
     "tsr_transcript_from_state": (i32, [vp, vp, u64, ctypes.POINTER(vp)]),
     "tsr_block_sink_create": (i32, [vp, u32, u32, u64, u64, vp, ctypes.POINTER(vp)]),

@@ -181,6 +181,40 @@ class ReferenceCommitment:
                                             int(merged), ptr, ctypes.byref(h)))
         return cls(device, h.value, params, int(seg.sum()))
 
+    def open(self, groups, xi_point, row_indices, row_vars):
+        """ReferenceCommitment::open (commitment.rs:489-524) of a masked commitment: groups =
+        [(column_point, [(row_point, gamma), ...]), ...] (Fp2 values as (c0, c1)); the layout's
+        row indices in layout order and row_vars. Returns [(u (k x 2), mask (P x 2))], the groups'
+        combined rows then the proximity row's."""
+        from .lib import BadShape
+        log_k = self.params.log2_k
+        for g, (cp, claims) in enumerate(groups):
+            if len(cp) != log_k:
+                raise BadShape(2, f"ColumnPoint: group {g} has {len(cp)} coordinates, not {log_k}")
+            for c, (rp, _) in enumerate(claims):
+                if len(rp) != row_vars:
+                    raise BadShape(2, f"RowPoint: group {g} claim {c} has {len(rp)} coordinates")
+        if len(xi_point) != row_vars:
+            raise BadShape(2, f"XiPoint: {len(xi_point)} coordinates, not {row_vars}")
+        rows = np.ascontiguousarray(np.asarray(row_indices, dtype=np.uint64))
+        pts = [np.asarray(rp, dtype=np.uint64).reshape(-1, 2) for _, cl in groups for rp, _ in cl]
+        gam = [np.asarray(gm, dtype=np.uint64).reshape(2) for _, cl in groups for _, gm in cl]
+        points = np.ascontiguousarray(np.concatenate(pts) if pts else np.zeros((1, 2), np.uint64))
+        gammas = np.ascontiguousarray(np.stack(gam) if gam else np.zeros((1, 2), np.uint64))
+        sizes = np.ascontiguousarray(np.array([len(cl) for _, cl in groups] or [0], dtype=np.uint32))
+        xi = np.ascontiguousarray(np.asarray(xi_point, dtype=np.uint64).reshape(-1, 2)
+                                  if row_vars else np.zeros((1, 2), np.uint64))
+        count = len(groups) + 1
+        coef = np.zeros((count * max(1, rows.size), 2), dtype=np.uint64)
+        check(lib().tsr_row_coefficients(self.device.handle, row_vars, rows.ctypes.data if rows.size else None,
+                                         rows.size, points.ctypes.data, gammas.ctypes.data,
+                                         sizes.ctypes.data, len(groups), xi.ctypes.data, coef.ctypes.data))
+        k, P = 1 << log_k, self.params.mask_coeffs
+        u = np.zeros((count * k, 2), dtype=np.uint64)
+        m = np.zeros((count * max(1, P), 2), dtype=np.uint64)
+        check(lib().tsr_commitment_open(self.handle, coef.ctypes.data, count, u.ctypes.data, m.ctypes.data))
+        return [(u[g * k:(g + 1) * k], m[g * P:(g + 1) * P]) for g in range(count)]
+
     def root(self) -> bytes:
         out = ctypes.create_string_buffer(32)
         check(lib().tsr_commitment_root(self.handle, out))

@@ -122,3 +122,52 @@ def test_streaming_vectors(dev, case):
         assert [d.hex() for d in sink.finish()] == case["segment_digests"][s]
         sink.free()
         off += count
+
+
+def fp2s(xs):
+    from tests.helpers import fp2
+    return [fp2(x) for x in xs]
+
+
+@pytest.mark.parametrize("case", CASES, ids=[c["name"][:40] for c in CASES])
+def test_open_matches_vectors(dev, case):
+    """ReferenceCommitment::open on the device: the combined rows u, M of every group and of the
+    proximity row (RFC-0011 steps 2 to 5)."""
+    import tessera_amd as T
+    rows = np.array([h(x) for r in case["witness"]["rows"] for x in r], dtype=np.uint64)
+    masks = np.array([h(x) for m in case["witness"]["masks"] for x in m], dtype=np.uint64)
+    seg = [len(s) for s in case["segments"]]
+    params = T.CommitmentParamsPy(case["log2_k"], case["log2_inv_rate"], case["mask_coeffs"],
+                                  case["row_block"], bytes.fromhex(case["commitment_id"]))
+    c = T.ReferenceCommitment.commit(dev, params, seg, rows, masks)
+    groups = [(fp2s(g["column_point"]), [(fp2s(cl["row_point"]), fp2s([cl["gamma"]])[0]) for cl in g["claims"]])
+              for g in case["groups"]]
+    got = c.open(groups, fp2s(case["xi_point"]), [r for s in case["segments"] for r in s], case["row_vars"])
+    for (u, m), want in zip(got, case["combined"]):
+        assert [tuple(int(v) for v in x) for x in u] == fp2s(want["u"])
+        assert [tuple(int(v) for v in x) for x in m] == fp2s(want["mask"])
+    c.free()
+
+
+def test_open_against_oracle(dev):
+    """A larger opening: 300 rows of k = 2^10 over 10 row variables, 3 groups, P = 37."""
+    import tessera_amd as T
+    rng = np.random.default_rng(4)
+    log_k, P_, row_vars = 10, 37, 10
+    idx = np.sort(rng.choice(1 << row_vars, 300, replace=False)).astype(np.uint64)
+    seg = [120, 0, 180]
+    rows = splitmix_uniform(300 << log_k, 41)
+    masks = splitmix_uniform(300 * P_, 42)
+    rnd = lambda *s: rng.integers(0, (1 << 64) - (1 << 32), s, dtype=np.uint64)
+    groups = [(rnd(log_k, 2), [(rnd(row_vars, 2), rnd(2)) for _ in range(nc)]) for nc in (2, 0, 3)]
+    xi = rnd(row_vars, 2)
+    c = T.ReferenceCommitment.commit(dev, T.CommitmentParamsPy(log_k, 1, P_, 32, bytes(32)), seg, rows, masks)
+    got = c.open(groups, xi, idx, row_vars)
+    co = O.row_coefficients(row_vars, idx, [cl for _, cl in groups], xi)
+    u = O.combine(co, rows, 1 << log_k)
+    m = O.combine(co, masks, P_)
+    for g, (gu, gm) in enumerate(got):
+        assert np.array_equal(gu, u[g]) and np.array_equal(gm, m[g])
+    with pytest.raises(T.BadShape, match="XiPoint"):
+        c.open(groups, xi[:3], idx, row_vars)
+    c.free()

@@ -1,6 +1,7 @@
 """The oracle's masked advice commitment (ReferenceCommitment::commit: masked encoding, one chain
 set per segment, column-merge leaves, Merkle root) against the reference's opening vectors: the
 root, the leaves at the queried columns, the answered columns and the canonical multiproof."""
+import numpy as np
 import pytest
 
 from oracle import oracle as O
@@ -55,3 +56,29 @@ def test_streaming_commitments_match_vectors(case):
         off += count
     assert [x.hex() for x in leaves] == case["leaves"]
     assert nodes[-32:].hex() == case["root"]
+
+
+def claim_groups(case):
+    from tests.helpers import fp2
+    return [[(np.array([fp2(x) for x in c["row_point"]], dtype=np.uint64).reshape(-1, 2),
+              np.array(fp2(c["gamma"]), dtype=np.uint64)) for c in g["claims"]] for g in case["groups"]]
+
+
+@pytest.mark.parametrize("case", CASES, ids=[c["name"][:40] for c in CASES])
+def test_combined_rows_match_vectors(case):
+    """RowLayout::row_coefficients and combine (RFC-0011 steps 2 to 5): the row coefficients
+    and the combined rows u, M of every group and of the proximity row."""
+    from tests.helpers import fp2
+    rows_idx = [r for s in case["segments"] for r in s]
+    xi = np.array([fp2(x) for x in case["xi_point"]], dtype=np.uint64).reshape(-1, 2)
+    co = O.row_coefficients(case["row_vars"], rows_idx, claim_groups(case), xi)
+    want = [[fp2(x) for x in col] for col in case["intermediate"]["row_coefficients"]]
+    assert [[tuple(int(v) for v in x) for x in g] for g in co] == want
+    k = 1 << case["log2_k"]
+    rows = [h(x) for r in case["witness"]["rows"] for x in r]
+    masks = [h(x) for m in case["witness"]["masks"] for x in m]
+    u = O.combine(co, rows, k)
+    mk = O.combine(co, masks, case["mask_coeffs"])
+    for g, comb in enumerate(case["combined"]):
+        assert [tuple(int(v) for v in x) for x in u[g]] == [fp2(x) for x in comb["u"]]
+        assert [tuple(int(v) for v in x) for x in mk[g]] == [fp2(x) for x in comb["mask"]]

@@ -181,10 +181,14 @@ void orc_multiplicities(const orc_logup_instance *inst, uint64_t *out /* 2^h_t *
  * (2k = lookup tree k, 2k+1 = table tree k). Outputs alpha, beta, roots (4 Fp2 per instance),
  * values (one Fp2 per instance with values), leaf claims (4 Fp2 per instance: lookup num/den,
  * table num/den) and a per-level log (see logup.c). Returns 0. */
-int orc_prove_logup(orc_transcript *t, const orc_logup_instance *inst, uint32_t count,
-                    const uint32_t *schedule, uint64_t *alpha, uint64_t *beta, uint64_t *roots,
-                    uint64_t *values, uint64_t *n_values, uint64_t *claims, uint64_t *log,
-                    uint64_t cap, uint64_t *log_len);
+typedef struct { const uint64_t *leaves; uint64_t len; uint32_t height; } orc_forest;
+int orc_prove_trees(orc_transcript *tr, const orc_forest *forests, uint32_t nforests,
+                    uint64_t *forest_claims, uint64_t *log, uint64_t cap, uint64_t *log_len);
+int orc_prove_logup(orc_transcript *tr, const orc_logup_instance *inst, uint32_t count,
+                    const orc_forest *forests, uint32_t nforests,
+                    const uint32_t *schedule, uint64_t *alpha_out, uint64_t *beta_out,
+                    uint64_t *roots, uint64_t *values, uint64_t *n_values, uint64_t *claims,
+                    uint64_t *forest_claims, uint64_t *log, uint64_t cap, uint64_t *log_len);
 
 #ifdef __cplusplus
 }

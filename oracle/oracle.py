@@ -516,14 +516,50 @@ def multiplicities(rows, lookup_rows):
     return out
 
 
-def prove_logup(transcript, instances, schedule):
+class _Forest(ctypes.Structure):
+    _fields_ = [("leaves", vp), ("len", u64), ("height", u32)]
+
+
+def _forests(forests, keep):
+    arr = (_Forest * max(1, len(forests)))()
+    for f, (leaves, height) in enumerate(forests):
+        a = np.ascontiguousarray(np.asarray(leaves, dtype=np.uint64))
+        keep.append(a)
+        arr[f].leaves = a.ctypes.data
+        arr[f].len = a.size
+        arr[f].height = height
+    return arr
+
+
+def prove_trees(transcript, forests):
+    """prove_trees over product forests [(leaves, height), ...]: (forest leaf claim values, level
+    records)."""
+    L = lib()
+    L.orc_prove_trees.restype = i32
+    L.orc_prove_trees.argtypes = [vp, vp, u32, vp, vp, u64, vp]
+    keep = []
+    arr = _forests(forests, keep)
+    claims = np.zeros(max(1, 2 * len(forests)), np.uint64)
+    cap = 1 << 22
+    log = np.zeros(cap, np.uint64)
+    ll = ctypes.c_uint64(0)
+    rc = L.orc_prove_trees(ctypes.byref(transcript.t), arr, len(forests), claims.ctypes.data,
+                           log.ctypes.data, cap, ctypes.byref(ll))
+    if rc:
+        raise ValueError("forest shape")
+    return ([(int(claims[2 * f]), int(claims[2 * f + 1])) for f in range(len(forests))],
+            parse_level_log(log[:ll.value]))
+
+
+def prove_logup(transcript, instances, schedule, forests=()):
     """instances: list of dicts with keys, values (or None), lookup_keys, lookup_values (or None),
-    multiplicities. schedule: tree ids (2k lookup, 2k+1 table). Returns a dict with alpha, beta,
-    roots, values, claims and the per-level records."""
+    multiplicities. forests: [(leaves, height), ...]. schedule: tree ids (2k lookup, 2k+1 table,
+    2 len(instances) + f forest f). Returns a dict with alpha, beta, roots, values, claims,
+    forest_claims and the per-level records."""
     L = lib()
     L.orc_prove_logup.restype = i32
-    L.orc_prove_logup.argtypes = [vp, ctypes.POINTER(_LogUpInstance), u32, vp, vp, vp, vp, vp, vp,
-                                  vp, vp, u64, vp]
+    L.orc_prove_logup.argtypes = [vp, ctypes.POINTER(_LogUpInstance), u32, vp, u32, vp, vp, vp, vp, vp,
+                                  vp, vp, vp, vp, u64, vp]
     keep = []
     arr = (_LogUpInstance * max(1, len(instances)))()
 
@@ -553,13 +589,18 @@ def prove_logup(transcript, instances, schedule):
     cap = 1 << 22
     log = np.zeros(cap, np.uint64)
     ll = ctypes.c_uint64(0)
-    L.orc_prove_logup(ctypes.byref(transcript.t), arr, n, sched.ctypes.data if sched.size else None,
+    farr = _forests(list(forests), keep)
+    fclaims = np.zeros(max(1, 2 * len(forests)), np.uint64)
+    L.orc_prove_logup(ctypes.byref(transcript.t), arr, n, farr, len(forests),
+                      sched.ctypes.data if sched.size else None,
                       alpha.ctypes.data, beta.ctypes.data, roots.ctypes.data, values.ctypes.data,
-                      ctypes.byref(nv), claims.ctypes.data, log.ctypes.data, cap, ctypes.byref(ll))
+                      ctypes.byref(nv), claims.ctypes.data, fclaims.ctypes.data, log.ctypes.data, cap,
+                      ctypes.byref(ll))
     assert ll.value <= cap
     return {"alpha": tuple(int(x) for x in alpha), "beta": tuple(int(x) for x in beta),
             "roots": roots[:8 * n].reshape(-1, 2), "values": values[:2 * nv.value].reshape(-1, 2),
-            "claims": claims[:8 * n].reshape(-1, 2), "levels": parse_level_log(log[:ll.value])}
+            "claims": claims[:8 * n].reshape(-1, 2), "levels": parse_level_log(log[:ll.value]),
+            "forest_claims": [(int(fclaims[2 * f]), int(fclaims[2 * f + 1])) for f in range(len(forests))]}
 
 
 def parse_level_log(log):
@@ -569,8 +610,12 @@ def parse_level_log(log):
     def f2(i):
         return (log[i], log[i + 1])
     while at < len(log):
+        s_max = log[at]; at += 1
+        roots = [f2(at + 2 * c) for c in range(s_max)]; at += 2 * s_max
+        has_lambda = log[at]; at += 1
         n = log[at]; at += 1
-        lv = {"gamma": f2(at), "lambda": f2(at + 2)}; at += 4
+        lv = {"root_challenges": roots, "gamma": f2(at), "lambda": f2(at + 2) if has_lambda else None}
+        at += 4
         lv["rounds"] = [[f2(at + 6 * j + 2 * z) for z in range(3)] for j in range(n)]; at += 6 * n
         lv["challenges"] = [f2(at + 2 * j) for j in range(n)]; at += 2 * n
         nf = log[at]; at += 1

@@ -144,7 +144,3 @@ def test_forest_batches(batch):
     assert len(last) == len(claims)
     for n in last:
         assert claims[n["forest"]] == fp2(n["value"])
-    last = batch["levels"][-1]["next"]
-    assert len(last) == len(claims)
-    for n in last:
-        assert claims[n["forest"]] == fp2(n["value"])

@@ -335,21 +335,43 @@ typedef struct {
     const uint64_t *lookup_values;  /* N values; NULL exactly when `values` is NULL */
     const uint64_t *multiplicities; /* 2^ceil(log2 T) entries (Fp) */
 } tsr_logup_instance;
-/* Proves the LogUp part of a leaf stage with fraction trees only: alpha (logup/alpha), beta
- * (logup/beta), each instance's lookup tree (1, alpha - (key + beta value)) and table tree
- * (-m_j, alpha - t_j), both padded with row 0, their roots absorbed under logup/roots, the
- * levels tree/{l} in `schedule` order (2 count tree ids, 2k = lookup tree k, 2k+1 = table tree
- * k; NULL = TreeId::paired), then the values under logup/values. Outputs: alpha, beta, roots
- * (4 Fp2 per instance: n_a, d_a, n_t, d_t), values (one Fp2 per instance with values), claims
- * (4 Fp2 per instance: the lookup and table trees' leaf numerator and denominator), and, when
- * `log` is given, per level: n_rounds, gamma, lambda, rounds (n x 3 Fp2), challenges, n_evals,
- * evals, rho, transcript state (4 u64), counter. Errors before the transcript moves: NoRows,
- * NoLookups, ValuesMismatch, a missing multiplicity column, a schedule that does not name each
- * tree once (BAD_SHAPE). */
+/* A product forest (Forest::new, crates/tessera-sumcheck/src/forest.rs:30-60): 2^s stacked
+ * product trees of height `height` over `len` base leaves (len = 2^(s + height)), node z of level
+ * t the product of nodes 2z, 2z+1 of level t+1. */
+typedef struct {
+    const uint64_t *leaves; /* len leaves (Fp), host or device */
+    uint64_t len;           /* a power of two */
+    uint32_t height;        /* 1 <= height <= log2(len) */
+} tsr_forest;
+/* Proves the LogUp part of a leaf stage: alpha (logup/alpha), beta (logup/beta), each
+ * instance's lookup tree (1, alpha - (key + beta value)) and table tree (-m_j, alpha - t_j),
+ * both padded with row 0, their roots absorbed under logup/roots, then the levels tree/{l} of
+ * the trees and the product forests in `schedule` order (2 count + nforests tree ids, 2k =
+ * lookup tree k, 2k+1 = table tree k, 2 count + f = forest f; NULL = TreeId::paired followed
+ * by the forests), then the values under logup/values. A level squeezes the root challenges
+ * tree/{l}/root/{c} of the forests that join it (each takes the last s as its point), gamma,
+ * and lambda only while a fraction tree is active. Outputs: alpha, beta, roots (4 Fp2 per
+ * instance: n_a, d_a, n_t, d_t), values (one Fp2 per instance with values), claims (4 Fp2 per
+ * instance: the lookup and table trees' leaf numerator and denominator), forest_claims (one
+ * Fp2 per forest: its value on the stacked leaves at its last point), and, when `log` is given,
+ * per level: s_max, the root challenges (s_max Fp2), has_lambda, n_rounds, gamma, lambda (0
+ * without fraction trees), rounds (n x 3 Fp2), challenges, n_evals, evals (N0 N1 D0 D1 per
+ * fraction tree, V0 V1 per forest), rho, transcript state (4 u64), counter. Errors before the
+ * transcript moves, in this order: NoRows, NoLookups, ValuesMismatch, a missing multiplicity
+ * column, a forest's LeafCount or Height, a schedule that does not name each tree once
+ * (BAD_SHAPE). crates/tessera-logup/src/prover.rs:307-469. */
 tsr_status tsr_logup_prove(tsr_ctx *ctx, tsr_transcript *t, const tsr_logup_instance *instances,
-                           uint32_t count, const uint32_t *schedule, uint64_t alpha[2],
-                           uint64_t beta[2], uint64_t *roots, uint64_t *values,
-                           uint64_t *n_values, uint64_t *claims, uint64_t *log,
+                           uint32_t count, const tsr_forest *forests, uint32_t nforests,
+                           const uint32_t *schedule, uint64_t alpha[2], uint64_t beta[2],
+                           uint64_t *roots, uint64_t *values, uint64_t *n_values,
+                           uint64_t *claims, uint64_t *forest_claims, uint64_t *log,
+                           uint64_t log_cap, uint64_t *log_len);
+/* prove_trees (crates/tessera-sumcheck/src/trees.rs:761-776) over product forests alone, in
+ * order, under tree/{l}: the level loop of tsr_logup_prove without fraction trees (no lambda).
+ * forest_claims: one Fp2 per forest; `log` as tsr_logup_prove's. LeafCount or Height (BAD_SHAPE)
+ * before the transcript moves. */
+tsr_status tsr_prove_trees(tsr_ctx *ctx, tsr_transcript *t, const tsr_forest *forests,
+                           uint32_t nforests, uint64_t *forest_claims, uint64_t *log,
                            uint64_t log_cap, uint64_t *log_len);
 /* multiplicities(table, lookups): out[j] = lookups of row j over 2^ceil(log2 rows) entries, row 0
  * also counting the lookup tree's 2^ceil(log2 N) - N padding leaves. LookupRow (BAD_SHAPE) when a

@@ -154,17 +154,57 @@ static uint32_t ceil_log2(uint64_t x) {
     return h;
 }
 
-// One fraction tree on the device: level t (0..h) holds N[t], D[t] of 2^t entries; the leaf
-// level's numerators are unit (lookup tree) or base (table tree, nbase).
+// A product forest's level above (forest.rs:306-340): V[t][z] = V[t+1][2z] V[t+1][2z+1], base
+// field throughout since the leaves are base.
+__global__ void k_product_above(const uint64_t *__restrict__ below, uint64_t count,
+                                uint64_t *__restrict__ out) {
+    const uint64_t z = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (z >= count) return;
+    out[z] = gl::mul(below[2 * z], below[2 * z + 1]);
+}
+
+// A tree of the level loop on the device.
+//   fraction tree: level t (0..h) holds N[t], D[t] of 2^t entries; the leaf level's numerators
+//     are unit (lookup tree) or base (table tree, nbase).
+//   product forest: 2^s stacked trees of height h; level t holds V[t] of 2^(s+t) base values,
+//     V[h] the caller's leaves.
 struct DevTree {
-    uint32_t h = 0, join = 0;
+    uint32_t h = 0, join = 0, s = 0;
+    bool forest = false;
     bool unit_leaves = true;
     std::vector<DevBuf<fp2>> N, D;
     DevBuf<uint64_t> nbase;
-    // current claim
+    std::vector<DevBuf<uint64_t>> V;  // levels 0..h-1 of a forest
+    const uint64_t *leaves = nullptr;
+    // current claim: point and (num, den), or the product value in num; a forest has none
+    // until the level it joins
     std::vector<fp2> point;
     fp2 num{0, 0}, den{0, 0};
+    bool has_claim = true;
+    const uint64_t *level_values(uint32_t t) const { return t == h ? leaves : V[t].ptr; }
 };
+
+static Status build_forest(Ctx *ctx, DevTree &t, const uint64_t *leaves, uint64_t len,
+                           uint32_t height) {
+    hipStream_t s = ctx->stream;
+    uint32_t vars = 0;
+    while ((1ull << vars) < len) vars++;
+    t.forest = true;
+    t.has_claim = false;
+    t.h = height;
+    t.s = vars - height;
+    t.leaves = leaves;
+    t.V = std::vector<DevBuf<uint64_t>>(height);
+    Status st;
+    KernelTimer timer(ctx, "k_product_above", s);
+    for (int l = (int)height - 1; l >= 0; l--) {
+        const uint64_t m = 1ull << (t.s + l);
+        if ((st = t.V[l].alloc(ctx, m))) return st;
+        hipLaunchKernelGGL(k_product_above, dim3(nblocks(m, 256)), dim3(256), 0, s,
+                           t.level_values(l + 1), m, t.V[l].ptr);
+    }
+    return check_launch("product forest");
+}
 
 static Status build_tree(Ctx *ctx, DevTree &t, uint32_t h, const uint64_t *keys,
                          const uint64_t *vals, uint64_t count, const uint64_t *mult, fp2 alpha,
@@ -227,35 +267,225 @@ struct Log {
     }
 };
 
+// Sum_z eq[z] col[z] over 2^n entries (col base), read back to the host.
+static Status eq_dot(Ctx *ctx, const fp2 *eq, const uint64_t *col, uint64_t count, uint64_t pad,
+                     uint32_t n, fp2 *part, fp2 *val, fp2 &out) {
+    hipStream_t s = ctx->stream;
+    const uint64_t len = 1ull << n;
+    const unsigned nb = std::min<unsigned>(1024, nblocks(len, 256));
+    hipLaunchKernelGGL(k_eq_dot, dim3(nb), dim3(256), 0, s, eq, col, count, pad, len, part);
+    hipLaunchKernelGGL(k_sum_partials, dim3(1), dim3(64), 0, s, part, nb, val);
+    Status st;
+    if ((st = check_launch("eq dot"))) return st;
+    TSR_HIP(hipMemcpyAsync(&out, val, 16, hipMemcpyDeviceToHost, s));
+    TSR_HIP(hipStreamSynchronize(s));
+    return Status::ok();
+}
+
+// The level loop of prove_trees_in (trees.rs:779-930) over the scheduled trees: each joins at
+// tallest - height. Per level: the root challenges tree/l/root/{c} of the forests joining here
+// (each takes the last s of them as its point, its claim Sum_z eq(point, z) V[0][z]), gamma,
+// lambda when a fraction tree is active, the batched rounds of eq (N0 D1 + N1 D0 + lambda D0 D1)
+// per fraction tree and eq V0 V1 per forest (degree 3, padding rounds from the claims), the
+// evals, rho under tree/l/line and the next claims at rho || own point. The log record per
+// level: s_max, the root challenges, has_lambda, n_max, gamma, lambda (0 without fractions),
+// the rounds (n_max x 3), the challenges, the count of evals and the evals, rho, the state,
+// the counter.
+static Status prove_levels(Ctx *ctx, Transcript &tr, std::vector<DevTree> &trees,
+                           const std::vector<uint32_t> &schedule, Log &log) {
+    hipStream_t s = ctx->stream;
+    uint32_t tallest = 0;
+    for (uint32_t id : schedule) tallest = std::max(tallest, trees[id].h);
+    for (uint32_t id : schedule) trees[id].join = tallest - trees[id].h;
+
+    DevBuf<fp2> d_point, d_part, d_val;
+    Status st;
+    if ((st = d_point.alloc(ctx, 64)) || (st = d_part.alloc(ctx, 1024)) || (st = d_val.alloc(ctx, 1)))
+        return st;
+    for (uint32_t level = 0; level < tallest; level++) {
+        const std::string prefix = "tree/" + std::to_string(level);
+        uint32_t s_max = 0;
+        for (uint32_t id : schedule)
+            if (trees[id].forest && trees[id].join == level) s_max = std::max(s_max, trees[id].s);
+        std::vector<fp2> rootc(s_max);
+        for (uint32_t c = 0; c < s_max; c++)
+            rootc[c] = tr.challenge_fp2((prefix + "/root/" + std::to_string(c)).c_str(), nullptr);
+        for (uint32_t id : schedule) {
+            DevTree &t = trees[id];
+            if (t.forest && t.join == level)
+                t.point.assign(rootc.begin() + (s_max - t.s), rootc.end());
+        }
+        std::vector<uint32_t> act;
+        bool fractions = false;
+        for (uint32_t id : schedule)
+            if (trees[id].join <= level) {
+                act.push_back(id);
+                fractions |= !trees[id].forest;
+            }
+        const fp2 gamma = tr.challenge_fp2((prefix + "/gamma").c_str(), nullptr);
+        const fp2 lambda = fractions ? tr.challenge_fp2((prefix + "/lambda").c_str(), nullptr) : fp2{0, 0};
+        std::vector<GenInst> insts(act.size());
+        std::vector<DevBuf<fp2>> eqs(act.size());
+        for (size_t i = 0; i < act.size(); i++) {
+            DevTree &t = trees[act[i]];
+            const uint32_t below = level - t.join + 1, n = (uint32_t)t.point.size();
+            if (n > 64) return Status::fail(TSR_DECLINED, "a claim point of more than 64 variables");
+            if ((st = eqs[i].alloc(ctx, 1ull << n))) return st;
+            if (n) {
+                TSR_HIP(hipMemcpyAsync(d_point.ptr, t.point.data(), 16 * n, hipMemcpyHostToDevice, s));
+            }
+            {
+                KernelTimer timer(ctx, "k_eq_table", s);
+                hipLaunchKernelGGL(k_eq_table, dim3(nblocks(1ull << n, 256)), dim3(256), 0, s,
+                                   d_point.ptr, n, eqs[i].ptr);
+            }
+            GenInst &g = insts[i];
+            g.vars = n;
+            g.claim_given = true;
+            GenTable eq;
+            eq.src = eqs[i].ptr;
+            eq.is_public = true;
+            if (t.forest) {
+                if (!t.has_claim) {
+                    // the joining claim: V0 V1 at z is the root V[0][z] (trees.rs:638-640)
+                    if ((st = eq_dot(ctx, eqs[i].ptr, t.V[0].ptr, 1ull << n, 0, n, d_part.ptr,
+                                     d_val.ptr, t.num)))
+                        return st;
+                    t.has_claim = true;
+                }
+                g.rel = REL_PRODUCT;
+                g.claim = t.num;
+                GenTable v0, v1;
+                v0.src = v1.src = t.level_values(below);
+                v0.base = v1.base = true;
+                v0.shift = v1.shift = 1;
+                v1.off = 1;
+                g.tabs = {eq, v0, v1};
+            } else {
+                g.rel = REL_FRACTION;
+                g.claim = gl::add(t.num, gl::mul(lambda, t.den));
+                GenTable n0, n1, d0, d1;
+                const bool leaf = below == t.h;
+                if (leaf && t.unit_leaves) {
+                    n0.src = n1.src = nullptr;
+                } else if (leaf) {
+                    n0.src = n1.src = t.nbase.ptr;
+                    n0.base = n1.base = true;
+                } else {
+                    n0.src = n1.src = t.N[below].ptr;
+                }
+                d0.src = d1.src = t.D[below].ptr;
+                n0.shift = n1.shift = d0.shift = d1.shift = 1;
+                n1.off = d1.off = 1;
+                g.tabs = {eq, n0, n1, d0, d1};
+            }
+            // an awaited copy of the point into d_point must finish before the next overwrite
+            TSR_HIP(hipStreamSynchronize(s));
+        }
+        SumcheckOut out;
+        if ((st = run_batch(ctx, tr, prefix, 3, insts, gamma, lambda, out))) return st;
+        const fp2 rho = tr.challenge_fp2((prefix + "/line").c_str(), nullptr);
+        const uint32_t n_max = (uint32_t)out.challenges.size();
+        log.put(s_max);
+        for (auto &v : rootc) log.f2(v);
+        log.put(fractions ? 1 : 0);
+        log.put(n_max);
+        log.f2(gamma);
+        log.f2(lambda);
+        for (auto &v : out.rounds) log.f2(v);
+        for (auto &v : out.challenges) log.f2(v);
+        log.put(out.finals.size());
+        for (auto &v : out.finals) log.f2(v);
+        log.f2(rho);
+        for (int b = 0; b < 4; b++) {
+            uint64_t w = 0;
+            for (int j = 0; j < 8; j++) w |= (uint64_t)tr.state[8 * b + j] << (8 * j);
+            log.put(w);
+        }
+        log.put(tr.counter);
+        size_t at = 0;
+        for (size_t i = 0; i < act.size(); i++) {
+            DevTree &t = trees[act[i]];
+            const uint32_t n = (uint32_t)t.point.size();
+            std::vector<fp2> np;
+            np.push_back(rho);
+            for (uint32_t k = 0; k < n; k++) np.push_back(out.challenges[n_max - n + k]);
+            t.point = np;
+            const fp2 *f = &out.finals[at];
+            const uint32_t below = level - t.join + 1;
+            if (t.forest) {
+                t.num = gl::add(f[0], gl::mul(rho, gl::sub(f[1], f[0])));
+                at += 2;
+                // levels at and above this one are no longer needed (trees.rs drop_level)
+                if (below == 1) t.V[0].release();
+                if (below < t.h) t.V[below].release();
+            } else {
+                t.num = gl::add(f[0], gl::mul(rho, gl::sub(f[1], f[0])));
+                t.den = gl::add(f[2], gl::mul(rho, gl::sub(f[3], f[2])));
+                at += 4;
+                if (below < t.h) {
+                    t.N[below].release();
+                    t.D[below].release();
+                }
+            }
+        }
+    }
+    return Status::ok();
+}
+
+// Uploads a host column into `up` (device columns pass through); nullptr stays nullptr.
+struct Uploads {
+    Ctx *ctx;
+    std::vector<DevBuf<uint64_t>> bufs;
+    Status operator()(const uint64_t *p, uint64_t n, const uint64_t **out) {
+        if (!p || is_device_ptr(p)) {
+            *out = p;
+            return Status::ok();
+        }
+        bufs.emplace_back();
+        Status st = bufs.back().alloc(ctx, n);
+        if (st) return st;
+        TSR_HIP(hipMemcpyAsync(bufs.back().ptr, p, 8 * n, hipMemcpyHostToDevice, ctx->stream));
+        *out = bufs.back().ptr;
+        return Status::ok();
+    }
+};
+
+// prove_trees (trees.rs:761-776) over product forests alone, in order: each forest's leaf claim
+// value in forest_claims (its point is the last level's rho || challenges).
+Status prove_trees(Ctx *ctx, Transcript &tr, const tsr_forest *forests, uint32_t nforests,
+                   std::vector<fp2> &forest_claims, Log &log) {
+    Uploads dev{ctx, {}};
+    dev.bufs.reserve(nforests + 1);
+    std::vector<DevTree> trees(nforests);
+    std::vector<uint32_t> sched(nforests);
+    Status st;
+    for (uint32_t f = 0; f < nforests; f++) {
+        const uint64_t *lv;
+        if ((st = dev(forests[f].leaves, forests[f].len, &lv)) ||
+            (st = build_forest(ctx, trees[f], lv, forests[f].len, forests[f].height)))
+            return st;
+        sched[f] = f;
+    }
+    if ((st = prove_levels(ctx, tr, trees, sched, log))) return st;
+    forest_claims.clear();
+    for (auto &t : trees) forest_claims.push_back(t.num);
+    return Status::ok();
+}
+
 Status prove_logup(Ctx *ctx, Transcript &tr, const tsr_logup_instance *inst, uint32_t count,
+                   const tsr_forest *forests, uint32_t nforests,
                    const std::vector<uint32_t> &schedule, fp2 &alpha, fp2 &beta,
                    std::vector<fp2> &roots, std::vector<fp2> &values, std::vector<fp2> &claims,
-                   Log &log) {
+                   std::vector<fp2> &forest_claims, Log &log) {
     hipStream_t s = ctx->stream;
     alpha = tr.challenge_fp2("logup/alpha", nullptr);
     beta = tr.challenge_fp2("logup/beta", nullptr);
 
     // device copies of the columns
-    std::vector<DevBuf<uint64_t>> up;
-    up.reserve(8 * count + 1);
-    auto dev = [&](const uint64_t *p, uint64_t n, const uint64_t **out) -> Status {
-        if (!p) {
-            *out = nullptr;
-            return Status::ok();
-        }
-        if (is_device_ptr(p)) {
-            *out = p;
-            return Status::ok();
-        }
-        up.emplace_back();
-        Status st = up.back().alloc(ctx, n);
-        if (st) return st;
-        TSR_HIP(hipMemcpyAsync(up.back().ptr, p, 8 * n, hipMemcpyHostToDevice, s));
-        *out = up.back().ptr;
-        return Status::ok();
-    };
-
-    std::vector<DevTree> trees(2 * count);
+    Uploads up{ctx, {}};
+    up.bufs.reserve(8 * count + nforests + 1);
+    std::vector<DevTree> trees(2 * count + nforests);
     std::vector<const uint64_t *> lvals(count), tvals(count);
     roots.clear();
     Status st;
@@ -263,10 +493,10 @@ Status prove_logup(Ctx *ctx, Transcript &tr, const tsr_logup_instance *inst, uin
         const tsr_logup_instance &in = inst[k];
         const uint32_t ha = ceil_log2(in.lookups), ht = ceil_log2(in.rows);
         const uint64_t *keys, *vals, *lk, *lv, *m;
-        if ((st = dev(in.keys, in.rows, &keys)) || (st = dev(in.values, in.rows, &vals)) ||
-            (st = dev(in.lookup_keys, in.lookups, &lk)) ||
-            (st = dev(in.lookup_values, in.lookups, &lv)) ||
-            (st = dev(in.multiplicities, 1ull << ht, &m)))
+        if ((st = up(in.keys, in.rows, &keys)) || (st = up(in.values, in.rows, &vals)) ||
+            (st = up(in.lookup_keys, in.lookups, &lk)) ||
+            (st = up(in.lookup_values, in.lookups, &lv)) ||
+            (st = up(in.multiplicities, 1ull << ht, &m)))
             return st;
         lvals[k] = lv;
         tvals[k] = vals;
@@ -288,101 +518,22 @@ Status prove_logup(Ctx *ctx, Transcript &tr, const tsr_logup_instance *inst, uin
         roots.push_back(trees[2 * k + 1].den);
     }
     tr.absorb_fp2s("logup/roots", (const uint64_t *)roots.data(), roots.size());
-
-    uint32_t tallest = 0;
-    for (uint32_t id : schedule) tallest = std::max(tallest, trees[id].h);
-    for (uint32_t id : schedule) trees[id].join = tallest - trees[id].h;
-
-    DevBuf<fp2> d_point;
-    if ((st = d_point.alloc(ctx, 64))) return st;
-    for (uint32_t level = 0; level < tallest; level++) {
-        const std::string prefix = "tree/" + std::to_string(level);
-        fp2 gamma = tr.challenge_fp2((prefix + "/gamma").c_str(), nullptr);
-        fp2 lambda = tr.challenge_fp2((prefix + "/lambda").c_str(), nullptr);
-        std::vector<uint32_t> act;
-        for (uint32_t id : schedule)
-            if (trees[id].join <= level) act.push_back(id);
-        std::vector<GenInst> insts(act.size());
-        std::vector<DevBuf<fp2>> eqs(act.size());
-        for (size_t i = 0; i < act.size(); i++) {
-            DevTree &t = trees[act[i]];
-            const uint32_t tl = level - t.join, below = tl + 1, n = (uint32_t)t.point.size();
-            if ((st = eqs[i].alloc(ctx, 1ull << n))) return st;
-            if (n) {
-                TSR_HIP(hipMemcpyAsync(d_point.ptr, t.point.data(), 16 * n, hipMemcpyHostToDevice, s));
-            }
-            {
-                KernelTimer timer(ctx, "k_eq_table", s);
-                hipLaunchKernelGGL(k_eq_table, dim3(nblocks(1ull << n, 256)), dim3(256), 0, s,
-                                   d_point.ptr, n, eqs[i].ptr);
-            }
-            GenInst &g = insts[i];
-            g.vars = n;
-            g.rel = REL_FRACTION;
-            g.claim_given = true;
-            g.claim = gl::add(t.num, gl::mul(lambda, t.den));
-            GenTable eq;
-            eq.src = eqs[i].ptr;
-            eq.is_public = true;
-            GenTable n0, n1, d0, d1;
-            const bool leaf = below == t.h;
-            if (leaf && t.unit_leaves) {
-                n0.src = n1.src = nullptr;
-            } else if (leaf) {
-                n0.src = n1.src = t.nbase.ptr;
-                n0.base = n1.base = true;
-            } else {
-                n0.src = n1.src = t.N[below].ptr;
-            }
-            d0.src = d1.src = t.D[below].ptr;
-            n0.shift = n1.shift = d0.shift = d1.shift = 1;
-            n1.off = d1.off = 1;
-            g.tabs = {eq, n0, n1, d0, d1};
-            // an awaited copy of the point into d_point must finish before the next overwrite
-            TSR_HIP(hipStreamSynchronize(s));
-        }
-        SumcheckOut out;
-        if ((st = run_batch(ctx, tr, prefix, 3, insts, gamma, lambda, out))) return st;
-        fp2 rho = tr.challenge_fp2((prefix + "/line").c_str(), nullptr);
-        const uint32_t n_max = (uint32_t)out.challenges.size();
-        log.put(n_max);
-        log.f2(gamma);
-        log.f2(lambda);
-        for (auto &v : out.rounds) log.f2(v);
-        for (auto &v : out.challenges) log.f2(v);
-        log.put(out.finals.size());
-        for (auto &v : out.finals) log.f2(v);
-        log.f2(rho);
-        for (int b = 0; b < 4; b++) {
-            uint64_t w = 0;
-            for (int j = 0; j < 8; j++) w |= (uint64_t)tr.state[8 * b + j] << (8 * j);
-            log.put(w);
-        }
-        log.put(tr.counter);
-        for (size_t i = 0; i < act.size(); i++) {
-            DevTree &t = trees[act[i]];
-            const uint32_t n = (uint32_t)t.point.size();
-            std::vector<fp2> np;
-            np.push_back(rho);
-            for (uint32_t k = 0; k < n; k++) np.push_back(out.challenges[n_max - n + k]);
-            t.point = np;
-            const fp2 *f = &out.finals[4 * i];
-            t.num = gl::add(f[0], gl::mul(rho, gl::sub(f[1], f[0])));
-            t.den = gl::add(f[2], gl::mul(rho, gl::sub(f[3], f[2])));
-            // the level below this one is no longer needed (trees.rs drop_level)
-            const uint32_t below = level - t.join + 1;
-            if (below < t.h) {
-                t.N[below].release();
-                t.D[below].release();
-            }
-        }
+    for (uint32_t f = 0; f < nforests; f++) {
+        const uint64_t *lv;
+        if ((st = up(forests[f].leaves, forests[f].len, &lv)) ||
+            (st = build_forest(ctx, trees[2 * count + f], lv, forests[f].len, forests[f].height)))
+            return st;
     }
+    if ((st = prove_levels(ctx, tr, trees, schedule, log))) return st;
 
     // values: the padded lookup value column at rho_a (prover.rs:301-318)
     values.clear();
     claims.clear();
-    DevBuf<fp2> d_part, d_val;
-    if ((st = d_part.alloc(ctx, 1024)) || (st = d_val.alloc(ctx, 1))) return st;
+    forest_claims.clear();
+    for (uint32_t f = 0; f < nforests; f++) forest_claims.push_back(trees[2 * count + f].num);
+    DevBuf<fp2> d_part, d_val, d_point;
+    if ((st = d_part.alloc(ctx, 1024)) || (st = d_val.alloc(ctx, 1)) || (st = d_point.alloc(ctx, 64)))
+        return st;
     for (uint32_t k = 0; k < count; k++) {
         DevTree &la = trees[2 * k], &ta = trees[2 * k + 1];
         if (inst[k].values) {
@@ -395,15 +546,9 @@ Status prove_logup(Ctx *ctx, Transcript &tr, const tsr_logup_instance *inst, uin
             uint64_t pad = 0;  // row 0's value fills the padding slots
             TSR_HIP(hipMemcpyAsync(&pad, tvals[k], 8, hipMemcpyDeviceToHost, s));
             TSR_HIP(hipStreamSynchronize(s));
-            const uint64_t len = 1ull << n;
-            const unsigned nb = std::min<unsigned>(1024, nblocks(len, 256));
-            hipLaunchKernelGGL(k_eq_dot, dim3(nb), dim3(256), 0, s, eq.ptr, lvals[k],
-                               inst[k].lookups, pad, len, d_part.ptr);
-            hipLaunchKernelGGL(k_sum_partials, dim3(1), dim3(64), 0, s, d_part.ptr, nb, d_val.ptr);
-            if ((st = check_launch("values"))) return st;
             fp2 v{0, 0};
-            TSR_HIP(hipMemcpyAsync(&v, d_val.ptr, 16, hipMemcpyDeviceToHost, s));
-            TSR_HIP(hipStreamSynchronize(s));
+            if ((st = eq_dot(ctx, eq.ptr, lvals[k], inst[k].lookups, pad, n, d_part.ptr, d_val.ptr, v)))
+                return st;
             values.push_back(v);
         }
         claims.push_back(la.num);
@@ -421,14 +566,35 @@ using namespace tsr;
 
 extern "C" {
 
+// Forest shapes (forest.rs:269-286), forest by forest: LeafCount, then Height.
+static Status check_forests(const tsr_forest *forests, uint32_t nforests) {
+    for (uint32_t f = 0; f < nforests; f++) {
+        const uint64_t len = forests[f].len;
+        if (len == 0 || (len & (len - 1)))
+            return Status::fail(TSR_BAD_SHAPE, "LeafCount: forest " + std::to_string(f));
+        if (forests[f].height == 0 || (1ull << forests[f].height) > len)
+            return Status::fail(TSR_BAD_SHAPE, "Height: forest " + std::to_string(f));
+        if (len > (1ull << 32)) return Status::fail(TSR_DECLINED, "a forest above 2^32 leaves");
+        if (!forests[f].leaves) return Status::fail(TSR_BAD_ARG, "null forest leaves");
+    }
+    return Status::ok();
+}
+
+static void put_f2s(uint64_t *out, const std::vector<fp2> &v) {
+    if (out && !v.empty()) memcpy(out, v.data(), 16 * v.size());
+}
+
 tsr_status tsr_logup_prove(tsr_ctx *ctx_, tsr_transcript *t, const tsr_logup_instance *inst,
-                           uint32_t count, const uint32_t *schedule, uint64_t alpha_out[2],
-                           uint64_t beta_out[2], uint64_t *roots_out, uint64_t *values_out,
-                           uint64_t *n_values, uint64_t *claims_out, uint64_t *log,
+                           uint32_t count, const tsr_forest *forests, uint32_t nforests,
+                           const uint32_t *schedule, uint64_t alpha_out[2], uint64_t beta_out[2],
+                           uint64_t *roots_out, uint64_t *values_out, uint64_t *n_values,
+                           uint64_t *claims_out, uint64_t *forest_claims_out, uint64_t *log,
                            uint64_t log_cap, uint64_t *log_len) {
     Ctx *ctx = (Ctx *)ctx_;
-    if (!ctx || !t || (count && !inst)) return Status::fail(TSR_BAD_ARG, "null argument").code;
-    // LogUpError order (instance.rs:140-166): per instance the columns, then the schedule.
+    if (!ctx || !t || (count && !inst) || (nforests && !forests))
+        return Status::fail(TSR_BAD_ARG, "null argument").code;
+    // LogUpError order (prover.rs:361-375): per instance the columns, the forests' shapes, then
+    // the schedule.
     for (uint32_t k = 0; k < count; k++) {
         const tsr_logup_instance &in = inst[k];
         if (in.rows == 0 || !in.keys)
@@ -442,23 +608,28 @@ tsr_status tsr_logup_prove(tsr_ctx *ctx_, tsr_transcript *t, const tsr_logup_ins
         if (ceil_log2(in.lookups) > 32 || ceil_log2(in.rows) > 32)
             return Status::fail(TSR_DECLINED, "a tree above 2^32 leaves").code;
     }
-    std::vector<uint32_t> sched(2 * count);
+    Status st = check_forests(forests, nforests);
+    if (st) return st.code;
+    const uint32_t ntrees = 2 * count + nforests;
+    std::vector<uint32_t> sched(ntrees);
     if (schedule) {
-        std::vector<int> seen(2 * count, 0);
-        for (uint32_t i = 0; i < 2 * count; i++) {
-            if (schedule[i] >= 2 * count || seen[schedule[i]]++)
+        std::vector<int> seen(ntrees, 0);
+        for (uint32_t i = 0; i < ntrees; i++) {
+            if (schedule[i] >= ntrees || seen[schedule[i]]++)
                 return Status::fail(TSR_BAD_SHAPE, "ScheduleTree: position " + std::to_string(i)).code;
             sched[i] = schedule[i];
         }
     } else {
-        for (uint32_t i = 0; i < 2 * count; i++) sched[i] = i;  // TreeId::paired
+        for (uint32_t i = 0; i < ntrees; i++) sched[i] = i;  // TreeId::paired, then the forests
     }
+    if (forest_claims_out == nullptr && nforests) return Status::fail(TSR_BAD_ARG, "null forest claims").code;
     hipError_t e = hipSetDevice(ctx->device);
     if (e != hipSuccess) return hip_fail(e, "hipSetDevice").code;
     fp2 alpha, beta;
-    std::vector<fp2> roots, values, claims;
+    std::vector<fp2> roots, values, claims, fclaims;
     Log lg{log, log ? log_cap : 0};
-    Status st = prove_logup(ctx, t->t, inst, count, sched, alpha, beta, roots, values, claims, lg);
+    st = prove_logup(ctx, t->t, inst, count, forests, nforests, sched, alpha, beta, roots, values,
+                     claims, fclaims, lg);
     if (st) return st.code;
     if (alpha_out) {
         alpha_out[0] = alpha.c0;
@@ -468,10 +639,30 @@ tsr_status tsr_logup_prove(tsr_ctx *ctx_, tsr_transcript *t, const tsr_logup_ins
         beta_out[0] = beta.c0;
         beta_out[1] = beta.c1;
     }
-    if (roots_out) memcpy(roots_out, roots.data(), 16 * roots.size());
-    if (values_out) memcpy(values_out, values.data(), 16 * values.size());
+    put_f2s(roots_out, roots);
+    put_f2s(values_out, values);
     if (n_values) *n_values = values.size();
-    if (claims_out) memcpy(claims_out, claims.data(), 16 * claims.size());
+    put_f2s(claims_out, claims);
+    put_f2s(forest_claims_out, fclaims);
+    if (log_len) *log_len = lg.len;
+    if (log && lg.len > log_cap) return Status::fail(TSR_BAD_SHAPE, "log buffer too small").code;
+    return TSR_OK;
+}
+
+tsr_status tsr_prove_trees(tsr_ctx *ctx_, tsr_transcript *t, const tsr_forest *forests,
+                           uint32_t nforests, uint64_t *forest_claims_out, uint64_t *log,
+                           uint64_t log_cap, uint64_t *log_len) {
+    Ctx *ctx = (Ctx *)ctx_;
+    if (!ctx || !t || (nforests && (!forests || !forest_claims_out)))
+        return Status::fail(TSR_BAD_ARG, "null argument").code;
+    Status st = check_forests(forests, nforests);
+    if (st) return st.code;
+    hipError_t e = hipSetDevice(ctx->device);
+    if (e != hipSuccess) return hip_fail(e, "hipSetDevice").code;
+    std::vector<fp2> fclaims;
+    Log lg{log, log ? log_cap : 0};
+    if ((st = prove_trees(ctx, t->t, forests, nforests, fclaims, lg))) return st.code;
+    put_f2s(forest_claims_out, fclaims);
     if (log_len) *log_len = lg.len;
     if (log && lg.len > log_cap) return Status::fail(TSR_BAD_SHAPE, "log buffer too small").code;
     return TSR_OK;

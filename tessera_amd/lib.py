@@ -83,8 +83,9 @@ _SIGS = {
                                          ctypes.POINTER(i32)]),
     "tsr_sumcheck_prove": (i32, [vp, vp, ctypes.c_char_p, u32, vp, u32, vp, vp, vp, vp, u64,
                                  ctypes.POINTER(u64)]),
-    "tsr_logup_prove": (i32, [vp, vp, vp, u32, vp, vp, vp, vp, vp, ctypes.POINTER(u64), vp, vp,
-                              u64, ctypes.POINTER(u64)]),
+    "tsr_logup_prove": (i32, [vp, vp, vp, u32, vp, u32, vp, vp, vp, vp, vp, ctypes.POINTER(u64),
+                              vp, vp, vp, u64, ctypes.POINTER(u64)]),
+    "tsr_prove_trees": (i32, [vp, vp, vp, u32, vp, vp, u64, ctypes.POINTER(u64)]),
     "tsr_logup_multiplicities": (i32, [vp, u64, vp, u64, vp]),
     "tsr_commit_batch_create": (i32, [vp, vp, u32, u32, ctypes.POINTER(vp)]),
     "tsr_commit_batch_run": (i32, [vp]),

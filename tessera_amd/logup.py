@@ -1,5 +1,6 @@
-"""LogUp-GKR (crates/tessera-logup: LogUpInstance, multiplicities, prove_logup) on the MI355X
-back end: fraction trees are built and proven level by level on the device."""
+"""LogUp-GKR (crates/tessera-logup: LogUpInstance, multiplicities, prove_logup) and prove_trees
+over product forests (crates/tessera-sumcheck/src/trees.rs:761-776, forest.rs) on the MI355X back
+end: fraction trees and forests are built and proven level by level on the device."""
 import ctypes
 from dataclasses import dataclass
 
@@ -13,6 +14,18 @@ class _CInstance(ctypes.Structure):
     _fields_ = [("rows", u64), ("keys", ctypes.c_void_p), ("values", ctypes.c_void_p),
                 ("lookups", u64), ("lookup_keys", ctypes.c_void_p),
                 ("lookup_values", ctypes.c_void_p), ("multiplicities", ctypes.c_void_p)]
+
+
+class _CForest(ctypes.Structure):
+    _fields_ = [("leaves", ctypes.c_void_p), ("len", u64), ("height", ctypes.c_uint32)]
+
+
+@dataclass
+class Forest:
+    """A product forest (Forest::new, forest.rs:30-60): 2^s stacked product trees of height
+    `height` over the base leaves (a power-of-two count, host array or DeviceBuffer)."""
+    leaves: object
+    height: int
 
 
 @dataclass
@@ -34,6 +47,7 @@ class ProvenLogUp:
     values: np.ndarray
     claims: np.ndarray  # 4 per instance, x 2
     levels: list
+    forest_claims: list = None  # one (c0, c1) per forest
 
 
 def _count(x):
@@ -49,7 +63,27 @@ def multiplicities(device, rows: int, lookup_rows):
     return out
 
 
-def prove_logup(device, transcript, instances, schedule=None, log_cap=1 << 22):
+def _forest_array(forests, keep):
+    arr = (_CForest * max(1, len(forests)))()
+    for f, fo in enumerate(forests):
+        if isinstance(fo.leaves, DeviceBuffer):
+            arr[f].leaves, arr[f].len = fo.leaves.ptr, fo.leaves.count
+        else:
+            a = np.ascontiguousarray(fo.leaves, dtype=np.uint64)
+            keep.append(a)
+            arr[f].leaves, arr[f].len = a.ctypes.data, a.size
+        arr[f].height = fo.height
+    return arr
+
+
+def _pairs(a, n):
+    return [(int(a[2 * i]), int(a[2 * i + 1])) for i in range(n)]
+
+
+def prove_logup(device, transcript, instances, schedule=None, forests=(), log_cap=1 << 22):
+    """prove_logup (prover.rs:307-469): the instances' lookup and table trees beside the product
+    forests, levels in `schedule` order (tree ids 2k lookup, 2k+1 table, 2 len(instances) + f
+    forest f; None: paired, then the forests)."""
     keep = []
     arr = (_CInstance * max(1, len(instances)))()
 
@@ -69,35 +103,60 @@ def prove_logup(device, transcript, instances, schedule=None, log_cap=1 << 22):
         arr[k].lookup_keys = p(ins.lookup_keys)
         arr[k].lookup_values = p(ins.lookup_values)
         arr[k].multiplicities = p(ins.multiplicities)
-    n = len(instances)
+    forests = list(forests)
+    farr = _forest_array(forests, keep)
+    n, nf = len(instances), len(forests)
     sched = None if schedule is None else np.ascontiguousarray(schedule, dtype=np.uint32)
     alpha, beta = np.zeros(2, np.uint64), np.zeros(2, np.uint64)
     roots = np.zeros(max(1, 8 * n), np.uint64)
     values = np.zeros(max(1, 2 * n), np.uint64)
     claims = np.zeros(max(1, 8 * n), np.uint64)
+    fclaims = np.zeros(max(1, 2 * nf), np.uint64)
     nv, ll = u64(), u64()
     log = np.zeros(log_cap, np.uint64)
-    check(lib().tsr_logup_prove(device.handle, transcript.handle, arr, n,
+    check(lib().tsr_logup_prove(device.handle, transcript.handle, arr, n, farr, nf,
                                 sched.ctypes.data if sched is not None else None,
                                 alpha.ctypes.data, beta.ctypes.data, roots.ctypes.data,
                                 values.ctypes.data, ctypes.byref(nv), claims.ctypes.data,
-                                log.ctypes.data, log_cap, ctypes.byref(ll)))
+                                fclaims.ctypes.data, log.ctypes.data, log_cap, ctypes.byref(ll)))
     return ProvenLogUp((int(alpha[0]), int(alpha[1])), (int(beta[0]), int(beta[1])),
                        roots[:8 * n].reshape(-1, 2), values[:2 * nv.value].reshape(-1, 2),
-                       claims[:8 * n].reshape(-1, 2), parse_level_log(log[:ll.value]))
+                       claims[:8 * n].reshape(-1, 2), parse_level_log(log[:ll.value]),
+                       _pairs(fclaims, nf))
+
+
+def prove_trees(device, transcript, forests, log_cap=1 << 22):
+    """prove_trees (trees.rs:761-776) over product forests alone: (the forests' leaf claim
+    values, the per-level records)."""
+    keep = []
+    forests = list(forests)
+    farr = _forest_array(forests, keep)
+    fclaims = np.zeros(max(1, 2 * len(forests)), np.uint64)
+    ll = u64()
+    log = np.zeros(log_cap, np.uint64)
+    check(lib().tsr_prove_trees(device.handle, transcript.handle, farr, len(forests),
+                                fclaims.ctypes.data, log.ctypes.data, log_cap, ctypes.byref(ll)))
+    return _pairs(fclaims, len(forests)), parse_level_log(log[:ll.value])
 
 
 def parse_level_log(log):
-    """Per level: gamma, lambda, rounds, challenges, final_evals, rho, state, counter."""
+    """Per level: root_challenges, gamma, lambda (None without fraction trees), rounds,
+    challenges, final_evals, rho, state, counter."""
     out, at = [], 0
     log = [int(x) for x in log]
 
     def f2(i):
         return (log[i], log[i + 1])
     while at < len(log):
-        n = log[at]
+        s_max = log[at]
         at += 1
-        lv = {"gamma": f2(at), "lambda": f2(at + 2)}
+        lv = {"root_challenges": [f2(at + 2 * c) for c in range(s_max)]}
+        at += 2 * s_max
+        has_lambda = log[at]
+        n = log[at + 1]
+        at += 2
+        lv["gamma"] = f2(at)
+        lv["lambda"] = f2(at + 2) if has_lambda else None
         at += 4
         lv["rounds"] = [[f2(at + 6 * j + 2 * z) for z in range(3)] for j in range(n)]
         at += 6 * n

@@ -1,13 +1,15 @@
-"""Parity of the gfx950 LogUp (device-built fraction trees, level sumchecks with the device
-transcript) with the reference's logup batches and with the oracle: alpha, beta, roots, every
-level's messages, challenges, evaluations, rho and transcript state, values, leaf claims; the
-multiplicity counts; and the 2^22-lookup / 2^16-row Zipf(1.1) workload whole against the oracle."""
+"""Parity of the gfx950 LogUp (device-built fraction trees and product forests, level sumchecks
+with the device transcript) with the reference's logup, tree and forest batches and with the
+oracle: alpha, beta, roots, every level's root challenges, gamma, lambda, messages, challenges,
+evaluations, rho and transcript state, values, leaf and forest claims; the multiplicity counts;
+and the 2^22-lookup / 2^16-row Zipf(1.1) workload whole against the oracle."""
 import numpy as np
 import pytest
 
 from oracle import oracle as O
 from tests.helpers import P, context_from_json, fp2, h, load, logup_config3
-from tests.test_oracle_logup import honest_batches, logup_inputs
+from tests.test_oracle_logup import (FOREST_BATCHES, TREE_BATCHES, batch_forests,
+                                     check_tree_levels, honest_batches, logup_inputs)
 
 pytestmark = pytest.mark.gpu
 
@@ -26,6 +28,11 @@ def to_gpu_instances(insts):
                           i["multiplicities"]) for i in insts]
 
 
+def to_gpu_forests(forests):
+    from tessera_amd.logup import Forest
+    return [Forest(np.asarray(leaves, dtype=np.uint64), height) for leaves, height in forests]
+
+
 def as_tuples(a):
     return [tuple(int(x) for x in v) for v in np.asarray(a).reshape(-1, 2)]
 
@@ -37,39 +44,102 @@ def test_reference_logup_batches(dev):
     for b in honest_batches():
         insts, sched = logup_inputs(b)
         # the oracle decides which batches an honest prover reproduces (forged roots aside)
+        forests = batch_forests(b)
         ot = O.OracleTranscript(context_from_json(b["context"]))
-        want = O.prove_logup(ot, insts, sched)
+        want = O.prove_logup(ot, insts, sched, forests)
         if as_tuples(want["roots"]) != [fp2(x) for x in b["roots"]]:
             continue
         t = Transcript(context_from_json(b["context"]))
-        got = prove_logup(dev, t, to_gpu_instances(insts), sched)
+        got = prove_logup(dev, t, to_gpu_instances(insts), sched, to_gpu_forests(forests))
         assert got.alpha == fp2(b["alpha"]) and got.beta == fp2(b["beta"])
         assert as_tuples(got.roots) == [fp2(x) for x in b["roots"]]
         assert len(got.levels) == len(b["levels"])
         for g, w in zip(got.levels, b["levels"]):
-            assert g["gamma"] == fp2(w["gamma"]) and g["lambda"] == fp2(w["lambda"])
+            assert g["root_challenges"] == [fp2(c) for c in w.get("root_challenges", [])]
+            assert g["gamma"] == fp2(w["gamma"])
+            assert g["lambda"] == (fp2(w["lambda"]) if w["lambda"] is not None else None)
             assert g["rounds"] == [[fp2(v) for v in m] for m in w["rounds"]], (b["name"], w["prefix"])
             assert g["challenges"] == [fp2(c) for c in w["challenges"]]
             assert g["final_evals"] == [fp2(f) for f in w["final_evals"]]
             assert g["rho"] == fp2(w["rho"]) and g["state"] == w["state"]
         assert as_tuples(got.values) == [fp2(v) for v in b["values"]]
         assert t.state.hex() == b["state"] and t.counter == b["counter"]
+        assert got.forest_claims == want["forest_claims"]
         checked += 1
-    assert checked >= 9
+    assert checked >= 13
 
 
-def compare_with_oracle(dev, insts, sched, ctx=b"logup"):
+@pytest.mark.parametrize("batch", TREE_BATCHES, ids=[b["name"][:40] for b in TREE_BATCHES])
+def test_tree_batches(dev, batch):
+    """prove_trees over single product trees (forests of one tree), against the reference's
+    recorded levels and leaf claims."""
+    from tessera_amd.logup import prove_trees
+    from tessera_amd.transcript import Transcript
+    t = Transcript(context_from_json(batch["context"]))
+    claims, levels = prove_trees(dev, t, to_gpu_forests(
+        [([h(x) for x in tr["leaves"]], tr["height"]) for tr in batch["trees"]]))
+    check_tree_levels(levels, batch["levels"])
+    for n in batch["levels"][-1]["next"]:
+        assert claims[n["tree"]] == fp2(n["value"])
+
+
+@pytest.mark.parametrize("batch", FOREST_BATCHES, ids=[b["name"][:40] for b in FOREST_BATCHES])
+def test_forest_batches(dev, batch):
+    """prove_trees over product forests of 2^s trees (points from the root challenges)."""
+    from tessera_amd.logup import prove_trees
+    from tessera_amd.transcript import Transcript
+    t = Transcript(context_from_json(batch["context"]))
+    claims, levels = prove_trees(dev, t, to_gpu_forests(
+        [([h(x) for x in f["leaves"]], f["height"]) for f in batch["forests"]]))
+    check_tree_levels(levels, batch["levels"])
+    for n in batch["levels"][-1]["next"]:
+        assert claims[n["forest"]] == fp2(n["value"])
+
+
+@pytest.mark.parametrize("shapes", [[(12, 3)], [(10, 10), (3, 1)], [(16, 2), (16, 16), (5, 4)],
+                                    [(20, 5)]])
+def test_random_forests(dev, shapes):
+    """prove_trees over random forests of 2^(s+h) leaves, against the oracle; the larger ones
+    run the level kernels over many blocks."""
+    from tessera_amd.logup import prove_trees
+    from tessera_amd.transcript import Transcript
+    rng = np.random.default_rng(sum(v for v, _ in shapes))
+    forests = [(rng.integers(0, P, 1 << v, dtype=np.uint64), ht) for v, ht in shapes]
+    t = Transcript(b"forests")
+    claims, levels = prove_trees(dev, t, to_gpu_forests(forests))
+    ot = O.OracleTranscript(b"forests")
+    want_claims, want_levels = O.prove_trees(ot, forests)
+    assert claims == want_claims and levels == want_levels
+    assert t.state == ot.state and t.counter == ot.counter
+
+
+def test_forest_shape_errors(dev):
+    import tessera_amd as T
+    from tessera_amd.logup import prove_trees
+    from tessera_amd.transcript import Transcript
+    t = Transcript(b"forests")
+    with pytest.raises(T.BadShape, match="LeafCount: forest 1"):
+        prove_trees(dev, t, to_gpu_forests([(np.ones(4, np.uint64), 1), (np.ones(6, np.uint64), 1)]))
+    with pytest.raises(T.BadShape, match="Height: forest 0"):
+        prove_trees(dev, t, to_gpu_forests([(np.ones(4, np.uint64), 3)]))
+    with pytest.raises(T.BadShape, match="Height: forest 0"):
+        prove_trees(dev, t, to_gpu_forests([(np.ones(4, np.uint64), 0)]))
+    assert t.state == Transcript(b"forests").state  # before the transcript moves
+
+
+def compare_with_oracle(dev, insts, sched, ctx=b"logup", forests=()):
     from tessera_amd.logup import prove_logup
     from tessera_amd.transcript import Transcript
     t = Transcript(ctx)
-    got = prove_logup(dev, t, to_gpu_instances(insts), sched)
+    got = prove_logup(dev, t, to_gpu_instances(insts), sched, to_gpu_forests(forests))
     ot = O.OracleTranscript(ctx)
-    want = O.prove_logup(ot, insts, sched)
+    want = O.prove_logup(ot, insts, sched, forests)
     assert got.alpha == want["alpha"] and got.beta == want["beta"]
     assert as_tuples(got.roots) == as_tuples(want["roots"])
     assert got.levels == want["levels"]
     assert as_tuples(got.values) == as_tuples(want["values"])
     assert as_tuples(got.claims) == as_tuples(want["claims"])
+    assert got.forest_claims == want["forest_claims"]
     assert t.state == ot.state and t.counter == ot.counter
     return got
 
@@ -91,6 +161,27 @@ def test_random_batches(dev, shape):
     sched = list(range(2 * len(insts)))
     sched = sched[::-1] if len(insts) > 1 else sched
     compare_with_oracle(dev, insts, sched)
+
+
+@pytest.mark.parametrize("case", ["taller_forest", "shorter_forest", "two_forests_interleaved"])
+def test_random_batches_with_forests(dev, case):
+    """Fraction trees beside product forests in one level loop: a forest taller than every
+    tree (levels without lambda first), one joining late, and two interleaved in the schedule."""
+    rng = np.random.default_rng({"taller_forest": 1, "shorter_forest": 2, "two_forests_interleaved": 3}[case])
+    if case == "taller_forest":
+        insts = [random_instance(rng, 9, 20)]
+        forests = [(rng.integers(0, P, 1 << 9, dtype=np.uint64), 7)]
+        sched = [2, 0, 1]
+    elif case == "shorter_forest":
+        insts = [random_instance(rng, 300, 700, values=False)]
+        forests = [(rng.integers(0, P, 1 << 6, dtype=np.uint64), 2)]
+        sched = [0, 2, 1]
+    else:
+        insts = [random_instance(rng, 40, 100), random_instance(rng, 5, 3, values=False)]
+        forests = [(rng.integers(0, P, 1 << 8, dtype=np.uint64), 8),
+                   (rng.integers(0, P, 1 << 12, dtype=np.uint64), 4)]
+        sched = [4, 0, 3, 5, 2, 1]
+    compare_with_oracle(dev, insts, sched, forests=forests)
 
 
 def test_multiplicities(dev):

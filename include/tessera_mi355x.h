@@ -379,6 +379,22 @@ tsr_status tsr_prove_trees(tsr_ctx *ctx, tsr_transcript *t, const tsr_forest *fo
 tsr_status tsr_logup_multiplicities(tsr_ctx *ctx, uint64_t rows, const uint64_t *lookup_rows,
                                     uint64_t lookups, uint64_t *out);
 
+/* ---- the advice mask stream (Masks as a MaskSource, crates/tessera-prover/src/cpu/random.rs:
+ *      123-229,275-320; the tessera/v1/advice-masks domain, crates/tessera-hash/src/domain.rs:
+ *      90-127) ---- */
+typedef struct tsr_mask_stream tsr_mask_stream;
+/* A mask stream over the 32-byte secret per-proof seed: the BLAKE3 derive-key extended output of
+ * the seed under "tessera/v1/advice-masks", read as u64 little-endian words, a word >= p discarded
+ * whole (MaskSource::draw). The seed is copied; the copy is cleared by tsr_mask_stream_free. */
+tsr_status tsr_mask_stream_create(tsr_ctx *ctx, const uint8_t seed[32], tsr_mask_stream **out);
+/* The next `count` draws of the stream into `out` (host or device memory), generated on the
+ * device: the masks a commitment takes (N rows x P coefficients, m_0 first, row after row, in
+ * layout order; commitment.rs:385-391). Two draws in a row equal one draw of their sum. */
+tsr_status tsr_mask_stream_draw(tsr_mask_stream *m, uint64_t count, uint64_t *out);
+/* The number of stream words consumed so far (draws plus discarded words). */
+uint64_t tsr_mask_stream_position(const tsr_mask_stream *m);
+void tsr_mask_stream_free(tsr_mask_stream *m);
+
 /* ---- the level offload (LevelOffload::load, DeviceLevel::{sums, bind, firsts},
  *      crates/tessera-sumcheck/src/offload.rs:17-79; the reference's GpuLevel,
  *      crates/tessera-gpu/src/level.rs:52-307) ---- */

@@ -160,6 +160,29 @@ def blake3(data: bytes, mode=0, key: bytes = b"", out_len=32):
     return out.raw
 
 
+PROVER_DOMAINS = ("advice-masks", "prover-share", "ni-session")  # domain.rs:90-118
+
+
+def prover_stream(name, seed: bytes, nbytes):
+    """A per-proof stream (random.rs:123-229): the BLAKE3 derive-key extended output of the
+    32-byte seed under "tessera/v1/<name>", first `nbytes` bytes."""
+    return blake3(bytes(seed), mode=2, key=("tessera/v1/" + name).encode(), out_len=nbytes)
+
+
+def advice_masks(seed: bytes, count, first_word=0):
+    """MaskSource::draw of Masks, `count` times from stream word `first_word` (random.rs:287-320):
+    the advice-masks stream read as u64 LE words, a word >= p discarded whole. Returns (masks,
+    the stream word after the last one drawn)."""
+    n = first_word + count + 64
+    while True:
+        words = np.frombuffer(prover_stream("advice-masks", seed, 8 * n), dtype="<u8")[first_word:]
+        keep = np.flatnonzero(words < P)
+        if keep.size >= count:
+            end = first_word + (int(keep[count - 1]) + 1 if count else 0)
+            return words[keep[:count]].astype(np.uint64), end
+        n *= 2
+
+
 def domain_contexts():
     L = lib()
     return [L.orc_domain_context(i).decode() for i in range(L.orc_domain_count())]

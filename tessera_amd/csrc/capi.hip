@@ -171,23 +171,24 @@ static const char *const DOMAINS[DOM_COUNT] = {
 
 const char *domain_context(int d) { return d >= 0 && d < DOM_COUNT ? DOMAINS[d] : nullptr; }
 
-// Context keys: BLAKE3 of the context string in DERIVE_KEY_CONTEXT mode. Every context string
-// is shorter than 64 bytes, so it is one block of one chunk.
-void derive_domain_keys(uint32_t keys[DOM_COUNT][8]) {
+// Context key: BLAKE3 of the context string in DERIVE_KEY_CONTEXT mode. Every context string
+// here is shorter than 64 bytes, so it is one block of one chunk.
+void derive_context_key(const char *context, uint32_t key[8]) {
     static const uint32_t iv[8] = {B3_IV0, B3_IV1, B3_IV2, B3_IV3,
                                    B3_IV4, B3_IV5, B3_IV6, B3_IV7};
-    for (int d = 0; d < DOM_COUNT; d++) {
-        uint8_t bytes[64] = {0};
-        size_t len = strlen(DOMAINS[d]);
-        memcpy(bytes, DOMAINS[d], len);
-        uint32_t blk[16];
-        for (int i = 0; i < 16; i++)
-            blk[i] = (uint32_t)bytes[4 * i] | ((uint32_t)bytes[4 * i + 1] << 8) |
-                     ((uint32_t)bytes[4 * i + 2] << 16) | ((uint32_t)bytes[4 * i + 3] << 24);
-        b3::compress(iv, blk, 0, (uint32_t)len,
-                     b3::CHUNK_START | b3::CHUNK_END | b3::ROOT | b3::DERIVE_KEY_CONTEXT,
-                     keys[d]);
-    }
+    uint8_t bytes[64] = {0};
+    size_t len = strlen(context);
+    memcpy(bytes, context, len < 64 ? len : 63);
+    uint32_t blk[16];
+    for (int i = 0; i < 16; i++)
+        blk[i] = (uint32_t)bytes[4 * i] | ((uint32_t)bytes[4 * i + 1] << 8) |
+                 ((uint32_t)bytes[4 * i + 2] << 16) | ((uint32_t)bytes[4 * i + 3] << 24);
+    b3::compress(iv, blk, 0, (uint32_t)len,
+                 b3::CHUNK_START | b3::CHUNK_END | b3::ROOT | b3::DERIVE_KEY_CONTEXT, key);
+}
+
+void derive_domain_keys(uint32_t keys[DOM_COUNT][8]) {
+    for (int d = 0; d < DOM_COUNT; d++) derive_context_key(DOMAINS[d], keys[d]);
 }
 
 // SplitMix64 stream value i (state seed + (i+1) * 0x9E3779B97F4A7C15), before rejection.

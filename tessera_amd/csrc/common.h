@@ -147,5 +147,7 @@ enum Domain : int {
 };
 const char *domain_context(int d);
 void derive_domain_keys(uint32_t keys[DOM_COUNT][8]);
+// The BLAKE3 context key of a context string shorter than 64 bytes.
+void derive_context_key(const char *context, uint32_t key[8]);
 
 }  // namespace tsr

@@ -102,6 +102,10 @@ _SIGS = {
     "tsr_block_sink_absorb": (i32, [vp, vp, u64, vp]),
     "tsr_block_sink_finish": (i32, [vp, vp]),
     "tsr_block_sink_free": (None, [vp]),
+    "tsr_mask_stream_create": (i32, [vp, vp, ctypes.POINTER(vp)]),
+    "tsr_mask_stream_draw": (i32, [vp, u64, vp]),
+    "tsr_mask_stream_position": (u64, [vp]),
+    "tsr_mask_stream_free": (None, [vp]),
     "tsr_matrix_sink_create": (i32, [vp, u32, u32, u64, vp, u64, vp, u64, ctypes.POINTER(vp)]),
     "tsr_matrix_sink_absorb": (i32, [vp, vp, u64, vp]),
     "tsr_matrix_sink_finish": (i32, [vp, vp]),

@@ -84,6 +84,43 @@ class ColumnChains:
         return [raw[32 * j:32 * j + 32] for j in range(self.columns)]
 
 
+class MaskStream:
+    """The per-proof advice mask source (Masks as a MaskSource, crates/tessera-prover/src/cpu/
+    random.rs:123-229,275-320), generated on the device: BLAKE3 derive-key output of the 32-byte
+    seed under tessera/v1/advice-masks, u64 LE words, words >= p discarded whole."""
+
+    def __init__(self, device: Device, seed: bytes):
+        if len(seed) != 32:
+            raise ValueError("the mask seed is 32 bytes")
+        h = vp()
+        check(lib().tsr_mask_stream_create(device.handle, bytes(seed), ctypes.byref(h)))
+        self.device, self.handle = device, h.value
+
+    def draw(self, count: int, out: DeviceBuffer = None):
+        """The next `count` masks: into `out` (device) when given, else a numpy array."""
+        if out is not None:
+            if out.count < count:
+                raise ValueError("the output buffer is shorter than the draw")
+            check(lib().tsr_mask_stream_draw(self.handle, count, out.ptr))
+            return out
+        host = np.zeros(max(1, count), dtype=np.uint64)
+        check(lib().tsr_mask_stream_draw(self.handle, count, host.ctypes.data))
+        return host[:count]
+
+    @property
+    def position(self) -> int:
+        """Stream words consumed so far (draws and discarded words)."""
+        return int(lib().tsr_mask_stream_position(self.handle))
+
+    def close(self):
+        if self.handle:
+            lib().tsr_mask_stream_free(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        self.close()
+
+
 def merkle_root(device: Device, leaves):
     """MerkleTree::new(leaves).root() (crates/tessera-hash/src/merkle.rs:93-119)."""
     data = b"".join(leaves)

@@ -171,3 +171,72 @@ def test_open_against_oracle(dev):
     with pytest.raises(T.BadShape, match="XiPoint"):
         c.open(groups, xi[:3], idx, row_vars)
     c.free()
+
+
+def test_mask_stream_vectors(dev):
+    """The device mask stream against spec/vectors/hash prover_streams: the proof seed's 24 masks,
+    and a seed whose seventh word is >= p, drawn whole and in pieces across the discarded word
+    (host and device destinations)."""
+    import tessera_amd as T
+    from tessera_amd.pcs import MaskStream
+    v = load("hash.json")["prover_streams"]
+    seed = bytes.fromhex(v["proof_seed"])
+    ms = MaskStream(dev, seed)
+    assert [int(x) for x in ms.draw(len(v["advice_masks"]))] == [h(x) for x in v["advice_masks"]]
+    assert ms.position == O.advice_masks(seed, len(v["advice_masks"]))[1]
+    ms.close()
+    rej = v["rejection"]
+    want = [h(x) for x in rej["values"]]
+    ms = MaskStream(dev, bytes.fromhex(rej["seed"]))
+    assert [int(x) for x in ms.draw(len(want))] == want
+    assert ms.position == len(want) + 1
+    ms.close()
+    ms = MaskStream(dev, bytes.fromhex(rej["seed"]))
+    first = ms.draw(5)
+    assert ms.position == 5
+    buf = T.DeviceBuffer(dev, 10)
+    ms.draw(10, out=buf)
+    assert [int(x) for x in first] + [int(x) for x in buf.download()] == want
+    assert ms.position == 16
+    assert ms.draw(0).size == 0 and ms.position == 16
+    ms.close()
+
+
+def test_mask_stream_large_draws(dev):
+    """Draws of 1, 7, 2^20 + 3 and 2^22 - 5 masks in a row, starting mid-block, against the
+    oracle's single draw of their sum."""
+    import tessera_amd as T
+    from tessera_amd.pcs import MaskStream
+    seed = bytes(range(100, 132))
+    sizes = [1, 7, (1 << 20) + 3, (1 << 22) - 5]
+    want, end = O.advice_masks(seed, sum(sizes))
+    ms = MaskStream(dev, seed)
+    got = []
+    for s in sizes:
+        buf = T.DeviceBuffer(dev, s)
+        ms.draw(s, out=buf)
+        got.append(buf.download())
+    assert np.array_equal(np.concatenate(got), want)
+    assert ms.position == end
+    ms.close()
+
+
+def test_commit_with_stream_masks(dev):
+    """A masked commitment whose masks the device stream draws (N x P, m_0 first, row after row)
+    equals the oracle's commitment with the oracle's draws of the same seed."""
+    import tessera_amd as T
+    from tessera_amd.pcs import MaskStream
+    log_k, log_r, P_, B, seg = 9, 1, 17, 32, [40, 0, 25]
+    k, N = 1 << log_k, sum(seg)
+    seed = bytes([9] * 32)
+    rows = splitmix_uniform(N * k, 77)
+    masks, _ = O.advice_masks(seed, N * P_)
+    cid = bytes(range(32))
+    _, leaves, nodes = O.commit_masked(rows, masks, log_k, log_r, P_, B, cid, seg, threads=8)
+    ms = MaskStream(dev, seed)
+    dmasks = T.DeviceBuffer(dev, N * P_)
+    ms.draw(N * P_, out=dmasks)
+    c = T.ReferenceCommitment.commit(dev, T.CommitmentParamsPy(log_k, log_r, P_, B, cid), seg, rows, dmasks)
+    assert c.leaves() == leaves and c.root() == nodes[-32:]
+    c.free()
+    ms.close()

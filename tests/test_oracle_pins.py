@@ -86,6 +86,36 @@ def test_domain_digests():
             assert O.hash_domain(i, data).hex() == want, (dom["name"], length)
 
 
+def test_prover_streams():
+    """The per-proof streams of spec/vectors/hash prover_streams: the three prover domains'
+    derive-key digests, the prover share (32 stream bytes), the NI session (16 bytes), 24 advice
+    masks of the proof seed, and a seed whose seventh word is >= p (discarded whole)."""
+    d = load("hash.json")
+    v = d["prover_streams"]
+    assert [x["name"] for x in v["domains"]] == list(O.PROVER_DOMAINS)
+    for dom in v["domains"]:
+        for length, want in zip(d["input_lengths"], dom["digests"]):
+            data = bytes(j % 251 for j in range(length))
+            assert O.blake3(data, mode=2, key=dom["context"].encode()).hex() == want, (dom["name"], length)
+    seed = bytes.fromhex(v["proof_seed"])
+    assert O.prover_stream("prover-share", seed, 32).hex() == v["prover_share"]
+    assert O.prover_stream("ni-session", seed, 16).hex() == v["ni_session"]
+    masks, end = O.advice_masks(seed, len(v["advice_masks"]))
+    assert [int(m) for m in masks] == [h(x) for x in v["advice_masks"]]
+    rej = v["rejection"]
+    rseed = bytes.fromhex(rej["seed"])
+    words = np.frombuffer(O.prover_stream("advice-masks", rseed, 8 * len(rej["words"])), dtype="<u8")
+    assert [int(w) for w in words] == [h(x) for x in rej["words"]]
+    assert int(words[rej["rejected_word_index"]]) >= P
+    values, end = O.advice_masks(rseed, len(rej["values"]))
+    assert [int(m) for m in values] == [h(x) for x in rej["values"]]
+    assert end == len(rej["values"]) + 1  # one word discarded
+    # draws in pieces continue the stream across the discarded word
+    a, e1 = O.advice_masks(rseed, 5)
+    b, e2 = O.advice_masks(rseed, 10, e1)
+    assert [int(x) for x in np.concatenate([a, b])] == [int(m) for m in values] and e2 == end
+
+
 # ---------------- NTT / encoding ----------------
 
 def direct_eval(coeffs, x):

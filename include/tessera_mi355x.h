@@ -306,10 +306,21 @@ typedef struct {
     uint32_t ext;         /* 1: Fp2 entries */
     uint32_t is_public;   /* 1: a public factor, left out of the final evaluations (Table::public) */
 } tsr_table;
+/* One term of a sum-of-products instance (Term, crates/tessera-sumcheck/src/terms.rs:10-40):
+ * weight * the product of the tables at `factors`, positions in the instance's table list,
+ * which may repeat (a squared table). */
+typedef struct {
+    uint64_t weight[2];       /* Fp2 */
+    const uint32_t *factors;
+    uint32_t nfactors;        /* 1 ..= the batch's degree */
+} tsr_term;
 typedef struct {
     uint32_t vars;            /* n_i: every table of the instance holds 2^n_i entries */
-    uint32_t factors;         /* the instance's relation is the product of its tables */
+    uint32_t factors;         /* the number of tables */
     const tsr_table *tables;
+    const tsr_term *terms;    /* NULL: the product of every table (Instance::new); else the sum
+                                 of the terms (Instance::sum_of_products, prover.rs:300-312) */
+    uint32_t nterms;
 } tsr_instance;
 /* Proves one batched product sumcheck of `degree` under the labels of `prefix` (e.g. "stage/0")
  * on transcript `t`: gamma from <prefix>/gamma, instance i weighted gamma^i and front-padded to
@@ -317,8 +328,12 @@ typedef struct {
  * <prefix>/round/{j}, r_j squeezed under <prefix>/r/{j}, the final evaluations (public factors
  * left out) absorbed under <prefix>/evals. Outputs: gamma (1 Fp2), rounds (n_max x degree Fp2),
  * challenges (n_max Fp2), finals (n_finals Fp2; capacity finals_cap). The rounds and their
- * transcript run on the device. Errors, before the transcript moves: degree outside 1..=10,
- * no instance, an instance with no table or more tables than the degree (BAD_SHAPE). */
+ * transcript run on the device. An instance is a product of its tables or a sum of products
+ * (terms). Errors, before the transcript moves, in the order of check_instance
+ * (prover.rs:739-771): degree outside 1..=10, no instance, then per instance NoTables,
+ * TooManyTables (a product of more tables than the degree) or NoTerms (terms given, none),
+ * TermFactors (a term of 0 or more than `degree` factors), TermFactor (a factor naming no
+ * table) (BAD_SHAPE). DECLINED above the device's limits: 10 tables, 16 terms per instance. */
 tsr_status tsr_sumcheck_prove(tsr_ctx *ctx, tsr_transcript *t, const char *prefix,
                               uint32_t degree, const tsr_instance *instances, uint32_t count,
                               uint64_t *gamma_out, uint64_t *rounds_out, uint64_t *challenges_out,

@@ -142,10 +142,19 @@ typedef struct {
     int ext;      /* 1: Fp2 entries (c0,c1), 0: Fp entries */
     int is_public;/* 1: omitted from the final evaluations (Table::public) */
 } orc_table;
+/* One term of a sum-of-products instance (Term, crates/tessera-sumcheck/src/terms.rs:10-40):
+ * weight * prod of the tables at `factors` (positions in the instance's list; may repeat). */
+typedef struct {
+    uint64_t weight[2];
+    uint32_t nfactors;
+    const uint32_t *factors;
+} orc_term;
 typedef struct {
     uint32_t vars;
-    uint32_t factors;
+    uint32_t factors;          /* the number of tables */
     const orc_table *tables;
+    const orc_term *terms;     /* NULL: the product of every table (Instance::new) */
+    uint32_t nterms;           /* with terms: Instance::sum_of_products (prover.rs:300-312) */
 } orc_instance;
 /* prove_batch of one stage (prover.rs:1122-1421): gamma, rounds, finals. Outputs:
  *   rounds: n_max * degree Fp2 (p_j(0), p_j(2..d)), challenges: n_max Fp2,

@@ -420,8 +420,21 @@ class _Table(ctypes.Structure):
     _fields_ = [("data", vp), ("ext", i32), ("is_public", i32)]
 
 
+class _Term(ctypes.Structure):
+    _fields_ = [("weight", u64 * 2), ("nfactors", u32), ("factors", ctypes.POINTER(u32))]
+
+
 class _Instance(ctypes.Structure):
-    _fields_ = [("vars", u32), ("factors", u32), ("tables", ctypes.POINTER(_Table))]
+    _fields_ = [("vars", u32), ("factors", u32), ("tables", ctypes.POINTER(_Table)),
+                ("terms", ctypes.POINTER(_Term)), ("nterms", u32)]
+
+
+class SumOfProducts:
+    """Instance::sum_of_products(tables, terms): terms are (weight, [factor positions]) with the
+    weight an Fp2 pair or an int."""
+
+    def __init__(self, tables, terms):
+        self.tables, self.terms = list(tables), list(terms)
 
 
 def _sumcheck_decl():
@@ -447,8 +460,11 @@ def prove_batch(transcript, prefix, degree, instances, threads=1):
     n_max = 0
     n_final = 0
     for i, tabs in enumerate(instances):
-        arr = (_Table * len(tabs))()
-        vars_ = None
+        terms = None
+        if isinstance(tabs, SumOfProducts):
+            tabs, terms = tabs.tables, tabs.terms
+        arr = (_Table * max(1, len(tabs)))()
+        vars_ = 0
         for k, (vals, ext, public) in enumerate(tabs):
             v = np.ascontiguousarray(np.asarray(vals, dtype=np.uint64).reshape(-1))
             keep.append(v)
@@ -462,6 +478,18 @@ def prove_batch(transcript, prefix, degree, instances, threads=1):
         insts[i].vars = vars_
         insts[i].factors = len(tabs)
         insts[i].tables = arr
+        if terms is not None:
+            tarr = (_Term * max(1, len(terms)))()
+            for t, (w, factors) in enumerate(terms):
+                w = (w, 0) if isinstance(w, int) else w
+                f = (u32 * max(1, len(factors)))(*factors)
+                keep.append(f)
+                tarr[t].weight[0], tarr[t].weight[1] = int(w[0]) % P, int(w[1]) % P
+                tarr[t].nfactors = len(factors)
+                tarr[t].factors = f
+            keep.append(tarr)
+            insts[i].terms = tarr
+            insts[i].nterms = len(terms)
         n_max = max(n_max, vars_)
     gamma = np.zeros(2, dtype=np.uint64)
     claim = np.zeros(2, dtype=np.uint64)

@@ -5,6 +5,9 @@
  * final_evals}, Bindings::bind, padding_scale) with the kernels of kernel.rs: round j's message
  * is p_j(0), p_j(2), ..., p_j(d) of sum_i gamma^i P_i (P_i front-padded), absorbed as TCE
  * Vec<Fp2>; the challenge binds the lowest variable, t[k] = t[2k] + r (t[2k+1] - t[2k]).
+ * A sum-of-products instance (prover.rs:243-312, kernel.rs:502-633 term_sums) has the relation
+ * sum_t w_t prod_{f in t} T_f over its table list; the reference groups terms by weight and
+ * factors out common tables, which changes no value in the field.
  */
 #include <stdlib.h>
 #include <string.h>
@@ -80,6 +83,8 @@ void orc_round_eval(const uint64_t *evals, uint32_t d, const uint64_t claim[2], 
 typedef struct {
     uint32_t vars, pad, factors;
     const orc_table *tables;
+    const orc_term *terms; /* NULL: product */
+    uint32_t nterms;
     F2 **cur;      /* current (bound) values per table, NULL until the first own bind */
     F2 weight;     /* gamma^i */
     F2 wsum;       /* gamma^i * S_i for padding rounds */
@@ -109,6 +114,31 @@ static void sums_range(void *arg, int slot, uint64_t lo, uint64_t hi) {
     for (uint32_t z = 0; z <= d; z++) acc[z] = f2(0, 0);
     for (uint64_t x = lo; x < hi; x++) {
         F2 prod[16];
+        if (in->terms) {
+            /* every table's line at every node, then sum_t w_t prod_{f in t} line_f(z) */
+            F2 line[16][16];
+            for (uint32_t k = 0; k < in->factors; k++) {
+                F2 a = entry(in, k, 2 * x), b = entry(in, k, 2 * x + 1), diff = sub2(b, a);
+                line[k][0] = a;
+                line[k][d] = b;
+                F2 v = add2(b, diff);
+                for (uint32_t z = 2; z <= d; z++) {
+                    line[k][z - 1] = v;
+                    v = add2(v, diff);
+                }
+            }
+            for (uint32_t z = 0; z <= d; z++) {
+                F2 sum = f2(0, 0);
+                for (uint32_t t = 0; t < in->nterms; t++) {
+                    const orc_term *tm = &in->terms[t];
+                    F2 p = f2(tm->weight[0], tm->weight[1]);
+                    for (uint32_t f = 0; f < tm->nfactors; f++) p = mul2(p, line[tm->factors[f]][z]);
+                    sum = add2(sum, p);
+                }
+                acc[z] = add2(acc[z], sum);
+            }
+            continue;
+        }
         for (uint32_t z = 0; z <= d; z++) prod[z] = f2(1, 0);
         for (uint32_t k = 0; k < in->factors; k++) {
             F2 a = entry(in, k, 2 * x), b = entry(in, k, 2 * x + 1), diff = sub2(b, a);
@@ -179,8 +209,22 @@ int64_t orc_prove_batch(orc_transcript *t, const char *prefix, uint32_t degree,
         in[i].factors = instances[i].factors;
         in[i].tables = instances[i].tables;
         in[i].vars = instances[i].vars;
+        in[i].terms = instances[i].terms;
+        in[i].nterms = instances[i].nterms;
         in[i].cur = calloc(instances[i].factors ? instances[i].factors : 1, sizeof(F2 *));
-        if (in[i].factors == 0 || in[i].factors > degree) return -1;
+        /* check_instance (prover.rs:739-771): NoTables, TooManyTables or NoTerms, TermFactors,
+         * TermFactor */
+        if (in[i].factors == 0 || in[i].factors > 16) return -1;
+        if (!in[i].terms && in[i].factors > degree) return -1;
+        if (in[i].terms) {
+            if (in[i].nterms == 0) return -1;
+            for (uint32_t t = 0; t < in[i].nterms; t++) {
+                const orc_term *tm = &in[i].terms[t];
+                if (tm->nfactors == 0 || tm->nfactors > degree) return -1;
+                for (uint32_t f = 0; f < tm->nfactors; f++)
+                    if (tm->factors[f] >= in[i].factors) return -1;
+            }
+        }
         if (in[i].vars > n_max) n_max = in[i].vars;
     }
     snprintf(label, sizeof label, "%s/gamma", prefix);
@@ -191,7 +235,15 @@ int64_t orc_prove_batch(orc_transcript *t, const char *prefix, uint32_t degree,
         in[i].pad = n_max - in[i].vars;
         /* S_i, the sum over the own hypercube (State::new, prover.rs:924-980) */
         F2 S;
-        if (in[i].vars == 0) {
+        if (in[i].vars == 0 && in[i].terms) {
+            S = f2(0, 0);
+            for (uint32_t t = 0; t < in[i].nterms; t++) {
+                const orc_term *tm = &in[i].terms[t];
+                F2 p = f2(tm->weight[0], tm->weight[1]);
+                for (uint32_t f = 0; f < tm->nfactors; f++) p = mul2(p, entry(&in[i], tm->factors[f], 0));
+                S = add2(S, p);
+            }
+        } else if (in[i].vars == 0) {
             S = f2(1, 0);
             for (uint32_t k = 0; k < in[i].factors; k++) S = mul2(S, entry(&in[i], k, 0));
         } else {

@@ -9,7 +9,7 @@
 
 namespace tsr {
 
-enum { REL_PRODUCT = 0, REL_FRACTION = 1 };
+enum { REL_PRODUCT = 0, REL_FRACTION = 1, REL_TERMS = 2 };
 
 struct SumcheckOut {
     gl::fp2 gamma;
@@ -27,10 +27,18 @@ struct GenTable {
     uint8_t shift = 0, off = 0;
 };
 
+// A term of a REL_TERMS instance: weight times the product of its factors, with mult[k] the
+// number of times table k appears among them.
+struct GenTerm {
+    gl::fp2 weight{1, 0};
+    std::vector<uint8_t> mult;
+};
+
 struct GenInst {
     uint32_t vars = 0;
     int rel = REL_PRODUCT;  // REL_FRACTION: tables eq, N0, N1, D0, D1 (degree 3)
     std::vector<GenTable> tabs;
+    std::vector<GenTerm> terms;  // REL_TERMS: sum_t w_t prod_k T_k^mult[t][k]
     bool claim_given = false;  // padding rounds use `claim` (a tree level's) instead of S_i
     gl::fp2 claim{0, 0};
 };

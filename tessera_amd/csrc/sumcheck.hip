@@ -9,8 +9,10 @@
 // tree level: equal for honest tables); binding is t[k] = t[2k] + r (t[2k+1] - t[2k])
 // (tessera-poly multilinear.rs:70-121).
 //
-// Two relations: a product of the instance's tables (prove_batch's Instance::new), and a
-// fraction-tree level eq (N0 D1 + N1 D0 + lambda D0 D1) over [eq, N0, N1, D0, D1].
+// Three relations: a product of the instance's tables (prove_batch's Instance::new), a
+// fraction-tree level eq (N0 D1 + N1 D0 + lambda D0 D1) over [eq, N0, N1, D0, D1], and a sum of
+// weighted products over the instance's tables (Instance::sum_of_products, prover.rs:243-312;
+// kernel.rs:502-633 term_sums, whose grouping by weight and common factors changes no value).
 //
 // Execution: the caller squeezes gamma (and lambda) on the host transcript, then run_batch
 // enqueues the whole loop on the context's stream: the first-round sums of every instance
@@ -41,6 +43,16 @@ constexpr int SC_MAXD = 10;
 constexpr int SC_THREADS = 256;
 constexpr int SC_MAX_BLOCKS = 1024;
 
+constexpr int SC_MAXT = 16;
+
+// The terms of a REL_TERMS instance: term t is weight[t] * prod_k T_k^mult[t][k].
+struct ScTerms {
+    uint32_t nterms;
+    uint32_t unit;  // bit t: weight[t] is 1
+    fp2 weight[SC_MAXT];
+    uint8_t mult[SC_MAXT][SC_MAXF];
+};
+
 struct ScTables {
     const void *src[SC_MAXF];  // nullptr: the constant 1 (a fraction tree's unit numerators)
     fp2 *dst[SC_MAXF];
@@ -56,8 +68,44 @@ __device__ __forceinline__ fp2 ld_entry(const void *p, bool base, uint64_t x) {
 // Node values of one pair, slots [p(0), p(2), ..., p(D), p(1)] (D+1 of them).
 template <int D, int REL>
 __device__ __forceinline__ void pair_values(const fp2 *a, const fp2 *b, uint32_t F,
-                                            const fp2 &lambda, fp2 out[D + 1]) {
-    if (REL == REL_PRODUCT) {
+                                            const fp2 &lambda, const ScTerms &terms,
+                                            fp2 out[D + 1]) {
+    if (REL == REL_TERMS) {
+        // line values of every table at z = 0, 2, ..., D, then 1; the tables are indexed at
+        // compile time (unrolled), the terms and multiplicities are uniform across the wave
+        fp2 line[SC_MAXF];
+#pragma unroll
+        for (int s = 0; s <= D; s++) {
+            fp2 v[SC_MAXF];
+#pragma unroll
+            for (int k = 0; k < SC_MAXF; k++) {
+                if (k >= (int)F) break;
+                if (s == 0) {
+                    v[k] = a[k];
+                } else if (s == D) {
+                    v[k] = b[k];
+                } else {
+                    line[k] = s == 1 ? gl::add(b[k], gl::sub(b[k], a[k])) : gl::add(line[k], gl::sub(b[k], a[k]));
+                    v[k] = line[k];
+                }
+            }
+            fp2 sum{0, 0};
+            for (uint32_t t = 0; t < terms.nterms; t++) {
+                fp2 p = terms.weight[t];
+                bool first = (terms.unit >> t) & 1;
+#pragma unroll
+                for (int k = 0; k < SC_MAXF; k++) {
+                    if (k >= (int)F) break;
+                    for (uint32_t m = 0; m < terms.mult[t][k]; m++) {
+                        p = first ? v[k] : gl::mul(p, v[k]);
+                        first = false;
+                    }
+                }
+                sum = gl::add(sum, p);
+            }
+            out[s] = sum;
+        }
+    } else if (REL == REL_PRODUCT) {
         for (uint32_t k = 0; k < F; k++) {
             fp2 diff = gl::sub(b[k], a[k]);
             fp2 line = gl::add(b[k], diff);  // z = 2
@@ -116,7 +164,8 @@ __device__ __forceinline__ uint64_t shfl_xor_u64(uint64_t v, int mask) {
 template <int D, int REL>
 __global__ void __launch_bounds__(SC_THREADS)
 k_sc_round(ScTables t, uint32_t F, uint32_t base_mask, bool first, uint64_t out_len,
-           uint64_t pairs, const fp2 *__restrict__ r_ptr, fp2 lambda, fp2 *__restrict__ partials) {
+           uint64_t pairs, const fp2 *__restrict__ r_ptr, fp2 lambda, fp2 *__restrict__ partials,
+           ScTerms terms) {
     const bool bind = out_len > 0;
     const fp2 r = bind ? *r_ptr : fp2{0, 0};
     const uint64_t items = pairs > 0 ? pairs : out_len;
@@ -149,7 +198,7 @@ k_sc_round(ScTables t, uint32_t F, uint32_t base_mask, bool first, uint64_t out_
         }
         if (pairs > 0) {
             fp2 val[D + 1];
-            pair_values<D, REL>(a, b, F, lambda, val);
+            pair_values<D, REL>(a, b, F, lambda, terms, val);
 #pragma unroll
             for (int z = 0; z <= D; z++) acc[z] = gl::add(acc[z], val[z]);
         }
@@ -256,24 +305,29 @@ __global__ void k_sc_gather(const void *const *__restrict__ ptrs, const uint32_t
 template <int D, int REL>
 static void launch_round(dim3 grid, hipStream_t s, const ScTables &t, uint32_t F,
                          uint32_t base_mask, bool first, uint64_t out_len, uint64_t pairs,
-                         const fp2 *r, fp2 lambda, fp2 *partials) {
+                         const fp2 *r, fp2 lambda, fp2 *partials, const ScTerms &terms) {
     hipLaunchKernelGGL((k_sc_round<D, REL>), grid, dim3(SC_THREADS), 0, s, t, F, base_mask,
-                       first, out_len, pairs, r, lambda, partials);
+                       first, out_len, pairs, r, lambda, partials, terms);
 }
 
 static void launch_round_d(uint32_t d, int rel, dim3 grid, hipStream_t s, const ScTables &t,
                            uint32_t F, uint32_t base_mask, bool first, uint64_t out_len,
-                           uint64_t pairs, const fp2 *r, fp2 lambda, fp2 *partials) {
+                           uint64_t pairs, const fp2 *r, fp2 lambda, fp2 *partials,
+                           const ScTerms &terms) {
     if (rel == REL_FRACTION) {
         launch_round<3, REL_FRACTION>(grid, s, t, F, base_mask, first, out_len, pairs, r, lambda,
-                                      partials);
+                                      partials, terms);
         return;
     }
     switch (d) {
-#define CASE(D)                                                                           \
-    case D:                                                                               \
-        launch_round<D, REL_PRODUCT>(grid, s, t, F, base_mask, first, out_len, pairs, r,  \
-                                     lambda, partials);                                   \
+#define CASE(D)                                                                              \
+    case D:                                                                                  \
+        if (rel == REL_TERMS)                                                                \
+            launch_round<D, REL_TERMS>(grid, s, t, F, base_mask, first, out_len, pairs, r,   \
+                                       lambda, partials, terms);                             \
+        else                                                                                 \
+            launch_round<D, REL_PRODUCT>(grid, s, t, F, base_mask, first, out_len, pairs, r, \
+                                         lambda, partials, terms);                           \
         break;
         CASE(1) CASE(2) CASE(3) CASE(4) CASE(5) CASE(6) CASE(7) CASE(8) CASE(9) CASE(10)
 #undef CASE
@@ -357,10 +411,27 @@ Status run_batch(Ctx *ctx, Transcript &tr, const std::string &p, uint32_t D,
         hinst[i].partials = partials[i].ptr;
     }
 
+    // the terms of REL_TERMS instances, as the round kernel reads them
+    std::vector<ScTerms> sterms(count);
+    for (uint32_t i = 0; i < count; i++) {
+        ScTerms &st_ = sterms[i];
+        memset(&st_, 0, sizeof st_);
+        if (insts[i].rel != REL_TERMS) continue;
+        if (insts[i].terms.size() > SC_MAXT || insts[i].tabs.size() > SC_MAXF)
+            return Status::fail(TSR_DECLINED, "more than 16 terms or 10 tables in an instance");
+        st_.nterms = (uint32_t)insts[i].terms.size();
+        for (uint32_t t = 0; t < st_.nterms; t++) {
+            const GenTerm &g = insts[i].terms[t];
+            st_.weight[t] = g.weight;
+            if (g.weight.c0 == 1 && g.weight.c1 == 0) st_.unit |= 1u << t;
+            for (size_t k = 0; k < g.mult.size() && k < SC_MAXF; k++) st_.mult[t][k] = g.mult[k];
+        }
+    }
+
     // S_i of instances on no variable (no claim given): the relation at the single entries
     for (uint32_t i = 0; i < count; i++) {
         if (insts[i].vars != 0 || hinst[i].pad == 0 || insts[i].claim_given) continue;
-        fp2 prod{1, 0};
+        std::vector<fp2> single;
         for (auto &tab : insts[i].tabs) {
             fp2 v{1, 0};
             if (tab.src) {
@@ -369,9 +440,21 @@ Status run_batch(Ctx *ctx, Transcript &tr, const std::string &p, uint32_t D,
                                        tab.base ? 8 : 16, hipMemcpyDeviceToHost, s));
                 TSR_HIP(hipStreamSynchronize(s));
             }
-            prod = gl::mul(prod, v);
+            single.push_back(v);
         }
-        hinst[i].wsum = gl::mul(hinst[i].weight, prod);
+        fp2 rel{1, 0};
+        if (insts[i].rel == REL_TERMS) {
+            rel = fp2{0, 0};
+            for (const GenTerm &g : insts[i].terms) {
+                fp2 p = g.weight;
+                for (size_t k = 0; k < g.mult.size(); k++)
+                    for (uint32_t m = 0; m < g.mult[k]; m++) p = gl::mul(p, single[k]);
+                rel = gl::add(rel, p);
+            }
+        } else {
+            for (auto &v : single) rel = gl::mul(rel, v);
+        }
+        hinst[i].wsum = gl::mul(hinst[i].weight, rel);
         hinst[i].wsum_given = 1;
     }
 
@@ -454,7 +537,7 @@ Status run_batch(Ctx *ctx, Transcript &tr, const std::string &p, uint32_t D,
             uint64_t pairs = 1ull << (insts[i].vars - 1);
             launch_round_d(D, insts[i].rel, dim3(blocks_for_pairs(pairs)), s, t,
                            (uint32_t)insts[i].tabs.size(), bm, true, 0, pairs, nullptr, lambda,
-                           partials[i].ptr);
+                           partials[i].ptr, sterms[i]);
         }
     }
     for (uint32_t j = 0; j < n_max; j++) {
@@ -475,7 +558,7 @@ Status run_batch(Ctx *ctx, Transcript &tr, const std::string &p, uint32_t D,
             uint64_t pairs = out_len / 2;  // next own round's pairs (0 after the last)
             launch_round_d(D, insts[i].rel, dim3(blocks_for_pairs(pairs ? pairs : out_len)), s,
                            tt, (uint32_t)insts[i].tabs.size(), bm, t == 0, out_len, pairs,
-                           &d_ctl.ptr->r, lambda, partials[i].ptr);
+                           &d_ctl.ptr->r, lambda, partials[i].ptr, sterms[i]);
         }
     }
     if ((st = check_launch("sumcheck rounds"))) return st;
@@ -549,7 +632,15 @@ Status prove_batch(Ctx *ctx, Transcript &tr, const char *prefix, uint32_t degree
     uploads.reserve(64);
     for (uint32_t i = 0; i < count; i++) {
         insts[i].vars = instances[i].vars;
-        insts[i].rel = REL_PRODUCT;
+        insts[i].rel = instances[i].terms ? REL_TERMS : REL_PRODUCT;
+        for (uint32_t t = 0; instances[i].terms && t < instances[i].nterms; t++) {
+            const tsr_term &tm = instances[i].terms[t];
+            GenTerm g;
+            g.weight = fp2{tm.weight[0], tm.weight[1]};
+            g.mult.assign(instances[i].factors, 0);
+            for (uint32_t f = 0; f < tm.nfactors; f++) g.mult[tm.factors[f]]++;
+            insts[i].terms.push_back(g);
+        }
         const uint64_t len = 1ull << instances[i].vars;
         for (uint32_t k = 0; k < instances[i].factors; k++) {
             const tsr_table &tt = instances[i].tables[k];
@@ -596,12 +687,31 @@ extern "C" tsr_status tsr_sumcheck_prove(tsr_ctx *ctx_, tsr_transcript *t, const
         const tsr_instance &in = instances[i];
         if (in.factors == 0 || !in.tables)
             return Status::fail(TSR_BAD_SHAPE, "NoTables: instance " + std::to_string(i)).code;
-        if (in.factors > degree)
+        if (!in.terms && in.factors > degree)
             return Status::fail(TSR_BAD_SHAPE, "TooManyTables: instance " + std::to_string(i) +
                                                    " has " + std::to_string(in.factors) +
                                                    " tables at degree " + std::to_string(degree)).code;
+        if (in.terms && in.nterms == 0)
+            return Status::fail(TSR_BAD_SHAPE, "NoTerms: instance " + std::to_string(i)).code;
+        for (uint32_t t = 0; in.terms && t < in.nterms; t++) {
+            const tsr_term &tm = in.terms[t];
+            if (tm.nfactors == 0 || tm.nfactors > degree)
+                return Status::fail(TSR_BAD_SHAPE, "TermFactors: instance " + std::to_string(i) +
+                                                       " term " + std::to_string(t) + " has " +
+                                                       std::to_string(tm.nfactors) + " factors at degree " +
+                                                       std::to_string(degree)).code;
+            if (!tm.factors) return Status::fail(TSR_BAD_ARG, "null term factors").code;
+            for (uint32_t f = 0; f < tm.nfactors; f++)
+                if (tm.factors[f] >= in.factors)
+                    return Status::fail(TSR_BAD_SHAPE, "TermFactor: instance " + std::to_string(i) +
+                                                           " term " + std::to_string(t) + " names table " +
+                                                           std::to_string(tm.factors[f]) + " of " +
+                                                           std::to_string(in.factors)).code;
+        }
         if (in.vars > 40)
             return Status::fail(TSR_DECLINED, "tables above 2^40 entries").code;
+        if (in.factors > SC_MAXF || (in.terms && in.nterms > SC_MAXT))
+            return Status::fail(TSR_DECLINED, "more than 10 tables or 16 terms in an instance").code;
         for (uint32_t k = 0; k < in.factors; k++) {
             if (!in.tables[k].data) return Status::fail(TSR_BAD_ARG, "null table").code;
             if (!in.tables[k].is_public) n_final++;

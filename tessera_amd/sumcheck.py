@@ -9,13 +9,29 @@ import numpy as np
 from .device import DeviceBuffer
 from .lib import check, lib, u32, u64, vp
 
+_P = 0xFFFFFFFF00000001
+
 
 class _CTable(ctypes.Structure):
     _fields_ = [("data", vp), ("ext", u32), ("is_public", u32)]
 
 
+class _CTerm(ctypes.Structure):
+    _fields_ = [("weight", u64 * 2), ("factors", ctypes.POINTER(u32)), ("nfactors", u32)]
+
+
 class _CInstance(ctypes.Structure):
-    _fields_ = [("vars", u32), ("factors", u32), ("tables", ctypes.POINTER(_CTable))]
+    _fields_ = [("vars", u32), ("factors", u32), ("tables", ctypes.POINTER(_CTable)),
+                ("terms", ctypes.POINTER(_CTerm)), ("nterms", u32)]
+
+
+@dataclass
+class SumOfProducts:
+    """Instance::sum_of_products(tables, terms) (prover.rs:300-312): the relation
+    sum_t weight_t * prod_{k in factors_t} tables[k]; terms are (weight, [positions]) with the
+    weight an Fp2 pair or an int (Term::new, terms.rs:24-28)."""
+    tables: list
+    terms: list
 
 
 @dataclass
@@ -56,8 +72,8 @@ class ProvenBatch:
 
 
 def prove_batch(device, transcript, prefix: str, degree: int, instances) -> ProvenBatch:
-    """prove_batch(transcript, prefix, degree, instances) (prover.rs:1122-1145); `instances` is a
-    list of lists of Table (each an Instance::new product)."""
+    """prove_batch(transcript, prefix, degree, instances) (prover.rs:1122-1145); each instance is
+    a list of Table (an Instance::new product) or a SumOfProducts."""
     _SIG = lib().tsr_sumcheck_prove
     _SIG.restype = ctypes.c_int
     _SIG.argtypes = [vp, vp, ctypes.c_char_p, u32, ctypes.POINTER(_CInstance), u32, vp, vp, vp, vp,
@@ -66,6 +82,9 @@ def prove_batch(device, transcript, prefix: str, degree: int, instances) -> Prov
     cinst = (_CInstance * max(1, len(instances)))()
     n_max, n_final = 0, 0
     for i, tabs in enumerate(instances):
+        terms = None
+        if isinstance(tabs, SumOfProducts):
+            tabs, terms = tabs.tables, tabs.terms
         arr = (_CTable * max(1, len(tabs)))()
         vars_ = tabs[0].n_vars() if tabs else 0
         for k, t in enumerate(tabs):
@@ -83,6 +102,18 @@ def prove_batch(device, transcript, prefix: str, degree: int, instances) -> Prov
         cinst[i].vars = vars_
         cinst[i].factors = len(tabs)
         cinst[i].tables = arr
+        if terms is not None:
+            tarr = (_CTerm * max(1, len(terms)))()
+            for t, (w, factors) in enumerate(terms):
+                w = (w, 0) if isinstance(w, int) else w
+                f = (u32 * max(1, len(factors)))(*factors)
+                keep.append(f)
+                tarr[t].weight[0], tarr[t].weight[1] = int(w[0]) % _P, int(w[1]) % _P
+                tarr[t].factors = f
+                tarr[t].nfactors = len(factors)
+            keep.append(tarr)
+            cinst[i].terms = tarr
+            cinst[i].nterms = len(terms)
         n_max = max(n_max, vars_)
     gamma = np.zeros(2, dtype=np.uint64)
     rounds = np.zeros(max(1, n_max * degree) * 2, dtype=np.uint64)

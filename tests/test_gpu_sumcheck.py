@@ -157,3 +157,68 @@ def test_eq_table_matches_oracle(dev, n):
     d = dev.buffer(2 << n)
     eq_table(dev, r, out=d)
     assert np.array_equal(d.download().reshape(-1, 2), want)
+
+
+def run_terms(dev, prefix, degree, spec, ctx=b"terms"):
+    """spec: list of (tables [(values, ext, public)], terms [(weight, [factors])] or None)."""
+    from tessera_amd.sumcheck import SumOfProducts, Table, prove_batch
+    from tessera_amd.transcript import Transcript
+    dev_insts, orc_insts = [], []
+    for tabs, terms in spec:
+        dt = [Table(v, e, p) for v, e, p in tabs]
+        dev_insts.append(dt if terms is None else SumOfProducts(dt, terms))
+        orc_insts.append(tabs if terms is None else O.SumOfProducts(tabs, terms))
+    t = Transcript(ctx)
+    g = prove_batch(dev, t, prefix, degree, dev_insts)
+    ot = O.OracleTranscript(ctx)
+    want = O.prove_batch(ot, prefix, degree, orc_insts, threads=8)
+    assert g.gamma == want["gamma"]
+    assert np.array_equal(g.rounds, want["rounds"])
+    assert np.array_equal(g.challenges, want["challenges"])
+    assert np.array_equal(g.final_evals, want["finals"])
+    assert t.state == ot.state and t.counter == ot.counter
+
+
+def rand_tables(rng, n, kinds):
+    return [(rng.integers(0, P, ((1 << n), 2) if ext else (1 << n,), dtype=np.uint64), ext, public)
+            for ext, public in kinds]
+
+
+@pytest.mark.parametrize("degree", [1, 2, 3, 5, 7, 10])
+def test_sums_of_products_against_oracle(dev, degree):
+    """Sum-of-products instances beside products: a tree-level shape (eq public, four tables,
+    three terms, a non-unit weight), repeated factors, more tables than the degree, padded
+    instances down to no variable, base and Fp2 tables, weights with both coordinates."""
+    rng = np.random.default_rng(100 + degree)
+    d = degree
+    spec = []
+    kinds5 = [(True, True), (False, False), (True, False), (False, False), (True, False)]
+    terms5 = [((1, 0), [0, 1, 4][:d]), ((1, 0), [0, 2, 3][:d]), ((12345, 678), [0, 3, 4][:d])]
+    spec.append((rand_tables(rng, 11, kinds5), terms5))
+    spec.append((rand_tables(rng, 6, [(False, False), (True, False)]),
+                 [((2, 0), ([0, 1] * d)[:d]), ((7, 9), [1]), ((P - 1, P - 2), [0] * d)]))
+    spec.append((rand_tables(rng, 0, [(True, False), (False, False)]),
+                 [((5, 5), [0, 1][:d]), ((1, 0), [1] * d)]))
+    spec.append((rand_tables(rng, 9, [(False, False)] * min(d, 3)), None))
+    kinds10 = [(i % 2 == 0, i == 3) for i in range(10)]
+    spec.append((rand_tables(rng, 13, kinds10),
+                 [((i + 1, i), [(i + m) % 10 for m in range(1 + i % d)]) for i in range(16)]))
+    run_terms(dev, "stage/3", degree, spec)
+
+
+def test_sum_of_products_shape_errors(dev):
+    import tessera_amd as T
+    from tessera_amd.sumcheck import SumOfProducts, Table, prove_batch
+    from tessera_amd.transcript import Transcript
+    a = Table(np.ones(4, np.uint64), False, False)
+    t = Transcript(b"x")
+    for terms, name in [([], "NoTerms: instance 1"), ([((1, 0), [])], "TermFactors: instance 1 term 0"),
+                        ([((1, 0), [0]), ((1, 0), [0, 0, 0, 1])], "TermFactors: instance 1 term 1"),
+                        ([((1, 0), [2])], "TermFactor: instance 1 term 0")]:
+        with pytest.raises(T.BadShape, match=name):
+            prove_batch(dev, t, "stage/0", 3, [[a], SumOfProducts([a, a], terms)])
+    assert t.state == Transcript(b"x").state
+    # more tables than the degree is a product's refusal, not a sum's
+    with pytest.raises(T.BadShape, match="TooManyTables"):
+        prove_batch(dev, t, "stage/0", 1, [[a, a]])
+    prove_batch(dev, t, "stage/0", 1, [SumOfProducts([a, a, a], [((1, 0), [2])])])

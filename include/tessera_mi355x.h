@@ -394,6 +394,66 @@ tsr_status tsr_prove_trees(tsr_ctx *ctx, tsr_transcript *t, const tsr_forest *fo
 tsr_status tsr_logup_multiplicities(tsr_ctx *ctx, uint64_t rows, const uint64_t *lookup_rows,
                                     uint64_t lookups, uint64_t *out);
 
+/* ---- the weight commitment (RFC-0040: WeightLayout::new / check, WeightCommitment::commit,
+ *      WeightCommitmentPublic; crates/tessera-pcs/src/weights/{layout.rs:55-260,466-560,
+ *      767-870, commit.rs:61-135, public.rs:22-48}, descriptor.rs:118-160) ---- */
+typedef struct {
+    uint8_t log2_k;          /* k_W = 2^log2_k (WeightLayoutParams) */
+    uint8_t log2_inv_rate;   /* r >= 1 */
+    uint32_t mask_coeffs;    /* P_W (carried in the descriptor and the public artifact) */
+    uint32_t blinding_rows;  /* B_W (likewise) */
+} tsr_weight_layout_params;
+typedef struct {
+    uint32_t out, inputs;    /* WeightShape: output channels (rows) x input channels */
+} tsr_weight_shape;
+typedef struct {             /* WeightTensorRef */
+    uint32_t matrix;         /* WeightMatrix: Embed 0, Q, K, V, O, Gate, Up, Down, LmHead 8 */
+    uint32_t layer;          /* 0 for Embed and LmHead */
+    const int8_t *data;      /* out x inputs INT8 entries, row-major; host or device memory */
+    uint64_t len;
+} tsr_weight_tensor;
+typedef struct {             /* ChannelParamsRef */
+    uint32_t matrix, layer;
+    uint32_t kind;           /* 0: class indices (u8), 1: multipliers (u16) */
+    uint64_t count;          /* one per output channel */
+    const void *values;      /* host memory */
+} tsr_channel_params;
+typedef struct tsr_weight_commitment tsr_weight_commitment;
+/* Commits a model's weights: the layout of `layers` layers and the nine weight types' shapes
+ * (channel_weights: n_channel_weights (matrix, layer) pairs, the weights that have a
+ * ChannelParams), then each of the ten matrices' committed rows, laid out from the INT8 entries
+ * on the device, committed unmasked at rate 2^-r with chains of `row_block` rows from
+ * commitment_id_T = H("commitment-id", TCE(CommitmentDescriptor::WeightMatrix)); the public
+ * artifact TCE(layout) || the ten roots || u32 epoch and R_W = H("weights-root", it). Errors
+ * before any work, in the reference's order: EmptyWeight, InverseRate, TooLong, WeightPlace,
+ * WeightTwice (of the layout's channel list), then WeightPlace, WeightLength, WeightTwice,
+ * WeightMissing of the tensors and of the channel parameters (BAD_SHAPE). */
+tsr_status tsr_weight_commit(tsr_ctx *ctx, const tsr_weight_layout_params *params, uint32_t layers,
+                             const tsr_weight_shape shapes[9], const uint32_t *channel_weights,
+                             uint32_t n_channel_weights, uint32_t row_block, uint32_t epoch,
+                             const tsr_weight_tensor *tensors, uint32_t ntensors,
+                             const tsr_channel_params *channels, uint32_t nchannels,
+                             tsr_weight_commitment **out);
+/* TCE(WeightCommitmentPublic), 334 bytes (public.rs:22-31). */
+tsr_status tsr_weight_commitment_public(const tsr_weight_commitment *w, uint8_t out[334]);
+/* R_W = H("weights-root", TCE(WeightCommitmentPublic)) (public.rs:33-36). */
+tsr_status tsr_weight_commitment_weights_root(const tsr_weight_commitment *w, uint8_t out[32]);
+/* The .twr artifact: the public part sealed in its envelope (schema 0x0301, version 1), 386
+ * bytes (WeightCommitmentPublic::write, public.rs:48). */
+tsr_status tsr_weight_commitment_artifact(const tsr_weight_commitment *w, uint8_t out[386]);
+/* WeightCommitment::matrix (commit.rs:148): matrix T's commitment (borrowed; for root, leaves,
+ * answer), its log2 k_T, row bits and committed rows m_T. */
+tsr_status tsr_weight_commitment_matrix(const tsr_weight_commitment *w, uint32_t matrix,
+                                        const tsr_commitment **commitment, uint32_t *log2_k,
+                                        uint32_t *row_vars, uint64_t *rows);
+void tsr_weight_commitment_free(tsr_weight_commitment *w);
+/* Envelope::seal (crates/tessera-codec/src/envelope.rs; RFC-0003 'Envelope'): b"TSRA", u16 1,
+ * u16 schema_id, u16 schema_version, u16 0, u64 payload length, the payload, then
+ * H("envelope", every preceding byte): len + 52 bytes into `out` (capacity `cap`). */
+tsr_status tsr_envelope_seal(tsr_ctx *ctx, uint16_t schema_id, uint16_t schema_version,
+                             const uint8_t *payload, uint64_t len, uint8_t *out, uint64_t cap,
+                             uint64_t *out_len);
+
 /* ---- the advice mask stream (Masks as a MaskSource, crates/tessera-prover/src/cpu/random.rs:
  *      123-229,275-320; the tessera/v1/advice-masks domain, crates/tessera-hash/src/domain.rs:
  *      90-127) ---- */

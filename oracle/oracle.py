@@ -730,3 +730,116 @@ def served_proof(weights, request_id, prompt, n_output, threads=8):
     out = prove_batch(t, "stage/0", 3, [[(eq, True, True), (A, False, False), (B, False, False)]],
                       threads=threads)
     return toks, root, t.state, t.counter, out
+
+
+# ---------------- the weight commitment (RFC-0040) ----------------
+# Restates spec RFC-0040 'The matrices', 'Encoding, leaves and roots', 'WeightCommitmentPublic'
+# as crates/tessera-pcs/src/weights/*.rs implement it (the layout of weights/layout.rs, the
+# descriptor of crates/tessera-pcs/src/commitment_id.rs, the envelope of
+# crates/tessera-codec/src/envelope.rs): integer index arithmetic in numpy, the commitment by the
+# C oracle's commit_unmasked.
+
+WEIGHT_MATRICES = ("Embed", "Q", "K", "V", "O", "Gate", "Up", "Down", "LmHead", "ChannelParams")
+
+
+def _pad2(x):
+    return 1 if x <= 1 else 1 << (x - 1).bit_length()
+
+
+def _log2(x):
+    return x.bit_length() - 1
+
+
+def weight_layout(out, inn, layers, log2_kw):
+    """A weight type's message width, row bits and committed rows (RFC-0040 'The matrices')."""
+    out_pad, in_pad, l_pad = _pad2(out), _pad2(inn), _pad2(layers)
+    k = max(in_pad, min(1 << log2_kw, out_pad * in_pad))
+    a = _log2(k // in_pad)
+    h = _log2(out_pad) - a
+    committed = [l * (1 << h) + oh for l in range(layers) for oh in range(1 << h) if (oh << a) < out]
+    return {"log2_k": _log2(k), "a": a, "h": h, "row_vars": _log2(l_pad) + h,
+            "rows": len(committed), "committed_rows": committed, "in_pad": in_pad}
+
+
+def weight_rows(w, out, inn, layers, lay):
+    """The committed rows (m_T x k_T, Fp) of a weight of layers x out x in integers: entry
+    W_l[o][i] at row l 2^h + floor(o / 2^a), column (o mod 2^a) in_pad + i; negatives as p + w."""
+    w = np.asarray(w, dtype=np.int64).reshape(layers, out, inn)
+    a, h, in_pad, k = lay["a"], lay["h"], lay["in_pad"], 1 << lay["log2_k"]
+    rows = np.zeros((lay["rows"], k), dtype=np.int64)
+    for m, r in enumerate(lay["committed_rows"]):
+        l, oh = r >> h, r & ((1 << h) - 1)
+        for ol in range(1 << a):
+            o = (oh << a) + ol
+            if o < out:
+                rows[m, ol * in_pad: ol * in_pad + inn] = w[l, o]
+    out = rows.astype(np.uint64)
+    neg = rows < 0
+    out[neg] = np.uint64(P) - (-rows[neg]).astype(np.uint64)
+    return out
+
+
+def channel_layout(channels, n_layers, log2_kw):
+    """ChannelParams (RFC-0040): channels are (type index t, layer, values, padded output width);
+    the tensor [16, L_pad, C] flattened row-major at width k_CP = min(k_W, 16 L_pad C)."""
+    l_pad = _pad2(n_layers)
+    C = max([c[3] for c in channels], default=1)
+    total = 16 * l_pad * C
+    k = min(1 << log2_kw, total)
+    flat = np.zeros(total, dtype=np.uint64)
+    live = np.zeros(total, dtype=bool)
+    for t, l, values, _ in channels:
+        f0 = (t * l_pad + l) * C
+        flat[f0:f0 + len(values)] = np.asarray(values, dtype=np.uint64)
+        live[f0:f0 + len(values)] = True
+    committed = [r for r in range(total // k) if live[r * k:(r + 1) * k].any()]
+    rows = flat.reshape(-1, k)[committed] if committed else np.zeros((0, k), np.uint64)
+    return {"log2_k": _log2(k), "row_vars": _log2(total // k), "rows": len(committed),
+            "committed_rows": committed, "channel_width": C}, rows
+
+
+def weight_layout_tce(layout):
+    """TCE(WeightLayoutParams): u8 log2_k, u8 log2_inv_rate, u32 mask_coeffs, u32 blinding_rows."""
+    return (bytes([layout["log2_k"], layout["log2_inv_rate"]]) + layout["mask_coeffs"].to_bytes(4, "little")
+            + layout["blinding_rows"].to_bytes(4, "little"))
+
+
+def weight_descriptor(layout, matrix, log2_k, row_block, rows, epoch):
+    """TCE(CommitmentDescriptor::WeightMatrix) (RFC-0031): u16 tag 1, the layout, u16 matrix,
+    u8 log2_k, u32 row_block, u64 rows, u32 epoch."""
+    return ((1).to_bytes(2, "little") + weight_layout_tce(layout) + matrix.to_bytes(2, "little")
+            + bytes([log2_k]) + row_block.to_bytes(4, "little") + rows.to_bytes(8, "little")
+            + epoch.to_bytes(4, "little"))
+
+
+DOMAIN_COMMITMENT_ID, DOMAIN_WEIGHTS_ROOT, DOMAIN_ENVELOPE = 1, 6, 0
+
+
+def envelope(schema_id, schema_version, payload: bytes):
+    """Envelope (RFC-0003): b"TSRA", u16 1, u16 schema, u16 version, u16 0, u64 payload length,
+    the payload, then H("envelope", every preceding byte)."""
+    head = (b"TSRA" + (1).to_bytes(2, "little") + schema_id.to_bytes(2, "little")
+            + schema_version.to_bytes(2, "little") + (0).to_bytes(2, "little")
+            + len(payload).to_bytes(8, "little") + payload)
+    return head + hash_domain(DOMAIN_ENVELOPE, head)
+
+
+def weight_commit(tensors, channels, n_layers, layout, row_block, epoch, threads=8):
+    """The weight commitment of a model: tensors[t] = (layers x out x in integers, out, in,
+    layers) for the nine weight types; channels as channel_layout takes them. Returns per
+    matrix (layout, descriptor, commitment_id, root), the public TCE, R_W and the .twr bytes."""
+    out = []
+    log2_kw, log_r = layout["log2_k"], layout["log2_inv_rate"]
+    for t in range(10):
+        if t < 9:
+            w, o, i, L = tensors[t]
+            lay = weight_layout(o, i, L, log2_kw)
+            rows = weight_rows(w, o, i, L, lay)
+        else:
+            lay, rows = channel_layout(channels, n_layers, log2_kw)
+        desc = weight_descriptor(layout, t, lay["log2_k"], row_block, lay["rows"], epoch)
+        cid = hash_domain(DOMAIN_COMMITMENT_ID, desc)
+        _, nodes = commit_unmasked(rows, lay["log2_k"], log_r, row_block, cid, threads=threads)
+        out.append((lay, desc, cid, nodes[-32:]))
+    public = weight_layout_tce(layout) + b"".join(m[3] for m in out) + epoch.to_bytes(4, "little")
+    return out, public, hash_domain(DOMAIN_WEIGHTS_ROOT, public), envelope(0x0301, 1, public)

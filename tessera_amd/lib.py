@@ -102,6 +102,16 @@ _SIGS = {
     "tsr_block_sink_absorb": (i32, [vp, vp, u64, vp]),
     "tsr_block_sink_finish": (i32, [vp, vp]),
     "tsr_block_sink_free": (None, [vp]),
+    "tsr_weight_commit": (i32, [vp, vp, u32, vp, vp, u32, u32, u32, vp, u32, vp, u32,
+                                ctypes.POINTER(vp)]),
+    "tsr_weight_commitment_public": (i32, [vp, vp]),
+    "tsr_weight_commitment_weights_root": (i32, [vp, vp]),
+    "tsr_weight_commitment_artifact": (i32, [vp, vp]),
+    "tsr_weight_commitment_matrix": (i32, [vp, u32, ctypes.POINTER(vp), ctypes.POINTER(u32),
+                                           ctypes.POINTER(u32), ctypes.POINTER(u64)]),
+    "tsr_weight_commitment_free": (None, [vp]),
+    "tsr_envelope_seal": (i32, [vp, ctypes.c_uint16, ctypes.c_uint16, vp, u64, vp, u64,
+                                ctypes.POINTER(u64)]),
     "tsr_mask_stream_create": (i32, [vp, vp, ctypes.POINTER(vp)]),
     "tsr_mask_stream_draw": (i32, [vp, u64, vp]),
     "tsr_mask_stream_position": (u64, [vp]),

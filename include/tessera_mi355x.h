@@ -454,6 +454,50 @@ tsr_status tsr_envelope_seal(tsr_ctx *ctx, uint16_t schema_id, uint16_t schema_v
                              const uint8_t *payload, uint64_t len, uint8_t *out, uint64_t cap,
                              uint64_t *out_len);
 
+/* ---- wire formats: TCE (RFC-0003; crates/tessera-codec/src/tce.rs:306-441) of SumcheckProof
+ *      (crates/tessera-sumcheck/src/proof.rs:9-24) and the commitment descriptors
+ *      (crates/tessera-pcs/src/descriptor.rs:10-160). Host code: no context or device needed. ---- */
+typedef struct {
+    uint64_t max_total_bytes; /* DecodeLimits (RFC-0003 'Limits') */
+    uint64_t max_vec_len;
+    uint32_t max_depth;       /* above 256 counts as 256 */
+} tsr_decode_limits;
+enum {
+    TSR_CODEC_OK = 0, TSR_CODEC_MAGIC = 1, TSR_CODEC_VERSION = 2, TSR_CODEC_NONCANONICAL = 3,
+    TSR_CODEC_TRUNCATED = 4, TSR_CODEC_DIGEST = 5, TSR_CODEC_LIMIT = 6
+};
+/* TCE(SumcheckProof { rounds, final_evals }): u64 n_rounds, each round u64 degree then its
+ * degree Fp2 (p(0), p(2), ..., p(d)), then u64 n_finals and the finals. `rounds` as the prover
+ * writes them (n_rounds x degree Fp2). The size needed is written to out_len; BAD_SHAPE when
+ * `cap` is short, BAD_ARG for a non-canonical value. */
+tsr_status tsr_sumcheck_proof_encode(const uint64_t *rounds, uint32_t n_rounds, uint32_t degree,
+                                     const uint64_t *finals, uint64_t n_finals, uint8_t *out,
+                                     uint64_t cap, uint64_t *out_len);
+/* Decodes a SumcheckProof under `limits` with RFC-0003's checks and offsets: round j's length in
+ * round_lens[j], the round values one after another in `values` (Fp2), the finals in `finals`.
+ * A malformed input is BAD_SHAPE with codec_error (TSR_CODEC_*) and error_offset set. */
+tsr_status tsr_sumcheck_proof_decode(const uint8_t *in, uint64_t len, const tsr_decode_limits *limits,
+                                     uint32_t *n_rounds, uint32_t *round_lens, uint32_t round_lens_cap,
+                                     uint64_t *values, uint64_t values_cap, uint64_t *n_values,
+                                     uint64_t *finals, uint64_t finals_cap, uint64_t *n_finals,
+                                     int *codec_error, uint64_t *error_offset);
+typedef struct {             /* CommitmentDescriptor::Advice */
+    uint8_t statement_digest[32];
+    uint32_t unit;
+    uint8_t log2_k, log2_inv_rate;  /* AdviceLayoutParams */
+    uint32_t mask_coeffs, row_block, segments;
+    uint64_t rows;           /* the unit's non-zero rows, every segment's */
+} tsr_advice_descriptor;
+/* TCE(CommitmentDescriptor::Advice) (60 bytes, may be NULL) and the commitment_id
+ * H("commitment-id", it) the masked commitment's chains start from (RFC-0031). */
+tsr_status tsr_advice_commitment_id(const tsr_advice_descriptor *d, uint8_t tce_out[60],
+                                    uint8_t commitment_id[32]);
+/* TCE(CommitmentDescriptor::WeightMatrix) (31 bytes, may be NULL) and its commitment_id. */
+tsr_status tsr_weight_matrix_commitment_id(const tsr_weight_layout_params *layout, uint32_t matrix,
+                                           uint8_t log2_k, uint32_t row_block, uint64_t rows,
+                                           uint32_t epoch, uint8_t tce_out[31],
+                                           uint8_t commitment_id[32]);
+
 /* ---- the advice mask stream (Masks as a MaskSource, crates/tessera-prover/src/cpu/random.rs:
  *      123-229,275-320; the tessera/v1/advice-masks domain, crates/tessera-hash/src/domain.rs:
  *      90-127) ---- */

@@ -198,22 +198,13 @@ Status weight_commit(Ctx *ctx, const tsr_weight_layout_params &params, uint32_t 
     uint8_t roots[10][32];
     for (uint32_t t = 0; t < 10; t++) {
         const uint64_t k = 1ull << ms[t].log2_k, m = ms[t].rows;
-        std::vector<uint8_t> desc;
-        desc.push_back(1);
-        desc.push_back(0);  // CommitmentDescriptor::WeightMatrix
-        put_layout(desc, params);
-        desc.push_back((uint8_t)t);
-        desc.push_back(0);
-        desc.push_back((uint8_t)ms[t].log2_k);
-        put_u32(desc, row_block);
-        put_u64(desc, m);
-        put_u32(desc, epoch);
         tsr_commitment_params cp{};
         cp.log2_k = ms[t].log2_k;
         cp.log2_inv_rate = params.log2_inv_rate;
         cp.mask_coeffs = 0;
         cp.row_block = row_block;
-        hash_domain(ctx, DOM_COMMITMENT_ID, desc.data(), desc.size(), cp.commitment_id);
+        tsr_weight_matrix_commitment_id(&params, t, (uint8_t)ms[t].log2_k, row_block, m, epoch, nullptr,
+                                        cp.commitment_id);
         DevBuf<uint64_t> rows;
         if (m && (st = rows.alloc(ctx, m * k))) return st;
         if (t < 9 && m) {

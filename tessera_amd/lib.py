@@ -112,6 +112,12 @@ _SIGS = {
     "tsr_weight_commitment_free": (None, [vp]),
     "tsr_envelope_seal": (i32, [vp, ctypes.c_uint16, ctypes.c_uint16, vp, u64, vp, u64,
                                 ctypes.POINTER(u64)]),
+    "tsr_sumcheck_proof_encode": (i32, [vp, u32, u32, vp, u64, vp, u64, ctypes.POINTER(u64)]),
+    "tsr_sumcheck_proof_decode": (i32, [vp, u64, vp, ctypes.POINTER(u32), vp, u32, vp, u64,
+                                        ctypes.POINTER(u64), vp, u64, ctypes.POINTER(u64),
+                                        ctypes.POINTER(ctypes.c_int), ctypes.POINTER(u64)]),
+    "tsr_advice_commitment_id": (i32, [vp, vp, vp]),
+    "tsr_weight_matrix_commitment_id": (i32, [vp, u32, ctypes.c_uint8, u32, u64, u32, vp, vp]),
     "tsr_mask_stream_create": (i32, [vp, vp, ctypes.POINTER(vp)]),
     "tsr_mask_stream_draw": (i32, [vp, u64, vp]),
     "tsr_mask_stream_position": (u64, [vp]),

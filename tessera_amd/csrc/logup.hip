@@ -80,6 +80,15 @@ __global__ void k_eq_table(const fp2 *__restrict__ point, uint32_t n, fp2 *__res
     out[x] = acc;
 }
 
+// eq(r, x) = eq(r_lo, x_lo) eq(r_hi, x_hi): out[x] = lo[x mod 2^a] hi[x >> a], one Fp2 product
+// per entry from two tables of about 2^(n/2) entries each.
+__global__ void k_eq_outer(const fp2 *__restrict__ lo, uint32_t a, const fp2 *__restrict__ hi,
+                           uint64_t len, fp2 *__restrict__ out) {
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    for (uint64_t x = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; x < len; x += stride)
+        out[x] = gl::mul(lo[x & ((1ull << a) - 1)], hi[x >> a]);
+}
+
 // Partial sums of eq[x] * col[x] (col padded with `pad` from `count` on), one per block.
 __global__ void __launch_bounds__(256)
 k_eq_dot(const fp2 *__restrict__ eq, const uint64_t *__restrict__ col, uint64_t count,
@@ -334,11 +343,7 @@ static Status prove_levels(Ctx *ctx, Transcript &tr, std::vector<DevTree> &trees
             if (n) {
                 TSR_HIP(hipMemcpyAsync(d_point.ptr, t.point.data(), 16 * n, hipMemcpyHostToDevice, s));
             }
-            {
-                KernelTimer timer(ctx, "k_eq_table", s);
-                hipLaunchKernelGGL(k_eq_table, dim3(nblocks(1ull << n, 256)), dim3(256), 0, s,
-                                   d_point.ptr, n, eqs[i].ptr);
-            }
+            if ((st = eq_table(ctx, d_point.ptr, n, eqs[i].ptr, s))) return st;
             GenInst &g = insts[i];
             g.vars = n;
             g.claim_given = true;
@@ -541,8 +546,7 @@ Status prove_logup(Ctx *ctx, Transcript &tr, const tsr_logup_instance *inst, uin
             DevBuf<fp2> eq;
             if ((st = eq.alloc(ctx, 1ull << n))) return st;
             if (n) TSR_HIP(hipMemcpyAsync(d_point.ptr, la.point.data(), 16 * n, hipMemcpyHostToDevice, s));
-            hipLaunchKernelGGL(k_eq_table, dim3(nblocks(1ull << n, 256)), dim3(256), 0, s,
-                               d_point.ptr, n, eq.ptr);
+            if ((st = eq_table(ctx, d_point.ptr, n, eq.ptr, s))) return st;
             uint64_t pad = 0;  // row 0's value fills the padding slots
             TSR_HIP(hipMemcpyAsync(&pad, tvals[k], 8, hipMemcpyDeviceToHost, s));
             TSR_HIP(hipStreamSynchronize(s));
@@ -723,10 +727,28 @@ tsr_status tsr_logup_multiplicities(tsr_ctx *ctx_, uint64_t rows, const uint64_t
 }  // extern "C"
 
 namespace tsr {
+// eq(point, .) over n variables. Up to 2^12 entries each entry is its own product; above, the
+// table is the outer product of the tables over the low n/2 and high n - n/2 variables.
 Status eq_table(Ctx *ctx, const gl::fp2 *point, uint32_t n, gl::fp2 *out, hipStream_t s) {
     KernelTimer timer(ctx, "k_eq_table", s);
-    hipLaunchKernelGGL(k_eq_table, dim3(nblocks(1ull << n, 256)), dim3(256), 0, s, point, n, out);
-    return check_launch("eq table");
+    if (n <= 12) {
+        hipLaunchKernelGGL(k_eq_table, dim3(nblocks(1ull << n, 256)), dim3(256), 0, s, point, n, out);
+        return check_launch("eq table");
+    }
+    const uint32_t a = n / 2;
+    DevBuf<gl::fp2> lo, hi;
+    Status st;
+    if ((st = lo.alloc(ctx, 1ull << a)) || (st = hi.alloc(ctx, 1ull << (n - a)))) return st;
+    hipLaunchKernelGGL(k_eq_table, dim3(nblocks(1ull << a, 256)), dim3(256), 0, s, point, a, lo.ptr);
+    hipLaunchKernelGGL(k_eq_table, dim3(nblocks(1ull << (n - a), 256)), dim3(256), 0, s, point + a,
+                       n - a, hi.ptr);
+    const uint64_t len = 1ull << n;
+    hipLaunchKernelGGL(k_eq_outer, dim3((unsigned)std::min<uint64_t>(nblocks(len, 256), 1u << 20)), dim3(256),
+                       0, s, lo.ptr, a, hi.ptr, len, out);
+    if ((st = check_launch("eq table"))) return st;
+    // lo and hi go back to the context's pool, which other streams may draw from
+    TSR_HIP(hipStreamSynchronize(s));
+    return Status::ok();
 }
 }  // namespace tsr
 

@@ -9,8 +9,9 @@
 //
 //   * k_tile_pass: a run of up to 9 radix-2 stages over index bits [lo, lo+nb) of every row,
 //     on LDS tiles of 2^(nb+logc) elements: 2^nb strided points x 2^logc consecutive columns,
-//     so each wave reads and writes 64-byte runs (logc = 3) from HBM. Eight columns (32 KiB
-//     tiles) measured faster than sixteen (64 KiB, half the workgroups per CU) and than four.
+//     2^12 elements (32 KiB) in all, so each wave reads and writes runs of 2^logc words (64 B
+//     for the 9-bit passes). 32 KiB tiles of eight columns measured faster than 64 KiB tiles of
+//     sixteen (half the workgroups per CU) and than 16 KiB tiles of four.
 //   * k_encode_mid: the fused middle — the low CT stages of the size-k inverse transform on a
 //     contiguous tile of 2^tk values, the spread and mask, and the low GS stages of the size-n
 //     forward transform on the 2^(tk+r) symbols that tile expands to. One HBM read of the row
@@ -26,6 +27,9 @@
 namespace tsr {
 
 constexpr int TILE_THREADS = 512;
+// A strided pass's tile: 2^PASS_TILE_LOG elements (32 KiB), 2^nb strided points x 2^(12 - nb)
+// consecutive columns, so a pass of few stage bits still has 512 radix-8 groups per round.
+constexpr uint32_t PASS_TILE_LOG = 12;
 // Threads of the fused middle's workgroup. Its 64 KiB tile allows two workgroups per CU, so
 // 1024 threads give the CU its full 32 waves (16 with 512 threads): 8% faster at the
 // 256 x 2^20 rate-1/4 headline, measured side by side in one process.
@@ -337,20 +341,20 @@ using MidFn = void (*)(const uint64_t *, uint64_t, uint64_t *, uint64_t, uint32_
                        const uint64_t *, const uint64_t *, const uint64_t *, const uint64_t *, uint64_t,
                        const uint64_t *, uint64_t, const uint64_t *const *, uint64_t *const *, uint64_t);
 
-// The compile-time tile pass for (nb, 8 columns), else the general one. top: the pass covers
-// the row's top bits (its tiles have hi = 0).
+// The compile-time tile pass for (nb, 2^(12 - nb) columns), else the general one. top: the pass
+// covers the row's top bits (its tiles have hi = 0).
 template <bool GS, bool TOP>
 static TilePassFn tile_pass_nb(uint32_t nb) {
     switch (nb) {
-        case 1: return k_tile_pass<GS, 1, 3, TOP>;
-        case 2: return k_tile_pass<GS, 2, 3, TOP>;
-        case 3: return k_tile_pass<GS, 3, 3, TOP>;
-        case 4: return k_tile_pass<GS, 4, 3, TOP>;
-        case 5: return k_tile_pass<GS, 5, 3, TOP>;
-        case 6: return k_tile_pass<GS, 6, 3, TOP>;
-        case 7: return k_tile_pass<GS, 7, 3, TOP>;
-        case 8: return k_tile_pass<GS, 8, 3, TOP>;
-        case 9: return k_tile_pass<GS, 9, 3, TOP>;
+        case 1: return k_tile_pass<GS, 1, PASS_TILE_LOG - 1, TOP>;
+        case 2: return k_tile_pass<GS, 2, PASS_TILE_LOG - 2, TOP>;
+        case 3: return k_tile_pass<GS, 3, PASS_TILE_LOG - 3, TOP>;
+        case 4: return k_tile_pass<GS, 4, PASS_TILE_LOG - 4, TOP>;
+        case 5: return k_tile_pass<GS, 5, PASS_TILE_LOG - 5, TOP>;
+        case 6: return k_tile_pass<GS, 6, PASS_TILE_LOG - 6, TOP>;
+        case 7: return k_tile_pass<GS, 7, PASS_TILE_LOG - 7, TOP>;
+        case 8: return k_tile_pass<GS, 8, PASS_TILE_LOG - 8, TOP>;
+        case 9: return k_tile_pass<GS, 9, PASS_TILE_LOG - 9, TOP>;
         default: return nullptr;
     }
 }
@@ -358,8 +362,15 @@ static TilePassFn tile_pass_nb(uint32_t nb) {
 template <bool GS>
 static TilePassFn tile_pass_fn(uint32_t nb, uint32_t logc, bool top) {
     TilePassFn f = nullptr;
-    if (logc == 3) f = top ? tile_pass_nb<GS, true>(nb) : tile_pass_nb<GS, false>(nb);
+    if (nb <= 9 && logc == PASS_TILE_LOG - nb) f = top ? tile_pass_nb<GS, true>(nb) : tile_pass_nb<GS, false>(nb);
     return f ? f : k_tile_pass<GS, 0, 0>;
+}
+
+// Column bits of a pass over bits [lo, lo + nb): the tile's 2^12 elements, or as many columns
+// as lie below lo.
+static uint32_t pass_logc(uint32_t lo, uint32_t nb) {
+    const uint32_t want = nb < PASS_TILE_LOG ? PASS_TILE_LOG - nb : 0;
+    return lo < want ? lo : want;
 }
 
 // The compile-time fused middle for the full tiles of tile_log 12 and 13 at rates 1/2 and 1/4.
@@ -479,7 +490,7 @@ void Encoder::launch(const uint64_t *src, uint64_t src_stride, const uint64_t *c
                      hipStream_t stream) const {
     const uint64_t k = 1ull << log_k, n = k << log_r;
     for (auto &ps : ct_passes) {
-        uint32_t lo = ps.first, nb = ps.second, logc = lo < 3 ? lo : 3;
+        uint32_t lo = ps.first, nb = ps.second, logc = pass_logc(lo, nb);
         dim3 grid((unsigned)(k >> (nb + logc)), (unsigned)rc);
         size_t shm = sizeof(uint64_t) << (nb + logc);
         KernelTimer timer(ctx, "k_tile_pass<ct>", stream);
@@ -502,7 +513,7 @@ void Encoder::launch(const uint64_t *src, uint64_t src_stride, const uint64_t *c
         }
     }
     for (auto &ps : gs_passes) {
-        uint32_t lo = ps.first, nb = ps.second, logc = lo < 3 ? lo : 3;
+        uint32_t lo = ps.first, nb = ps.second, logc = pass_logc(lo, nb);
         uint32_t log_n = log_k + log_r;
         dim3 grid((unsigned)(n >> (nb + logc)), (unsigned)rc);
         size_t shm = sizeof(uint64_t) << (nb + logc);

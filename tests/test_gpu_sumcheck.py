@@ -146,8 +146,9 @@ def test_config3_eq_a_b_invariants(dev):
     assert claim == O.fp2_mul(O.fp2_mul(e, ea), eb)
 
 
-@pytest.mark.parametrize("n", [0, 1, 5, 12])
+@pytest.mark.parametrize("n", [0, 1, 5, 12, 13, 20])
 def test_eq_table_matches_oracle(dev, n):
+    """Direct products up to 12 variables, the outer product of two half tables above."""
     from tessera_amd.sumcheck import eq_table
     rng = np.random.default_rng(n)
     r = rng.integers(0, P, (n, 2), dtype=np.uint64)

@@ -339,24 +339,25 @@ static Status prove_levels(Ctx *ctx, Transcript &tr, std::vector<DevTree> &trees
             DevTree &t = trees[act[i]];
             const uint32_t below = level - t.join + 1, n = (uint32_t)t.point.size();
             if (n > 64) return Status::fail(TSR_DECLINED, "a claim point of more than 64 variables");
-            if ((st = eqs[i].alloc(ctx, 1ull << n))) return st;
-            if (n) {
-                TSR_HIP(hipMemcpyAsync(d_point.ptr, t.point.data(), 16 * n, hipMemcpyHostToDevice, s));
-            }
-            if ((st = eq_table(ctx, d_point.ptr, n, eqs[i].ptr, s))) return st;
             GenInst &g = insts[i];
             g.vars = n;
             g.claim_given = true;
-            GenTable eq;
-            eq.src = eqs[i].ptr;
-            eq.is_public = true;
+            g.eq_factor = true;
+            g.eq_point = t.point;
             if (t.forest) {
                 if (!t.has_claim) {
+                    if ((st = eqs[i].alloc(ctx, 1ull << n))) return st;
+                    if (n) {
+                        TSR_HIP(hipMemcpyAsync(d_point.ptr, t.point.data(), 16 * n, hipMemcpyHostToDevice, s));
+                    }
+                    if ((st = eq_table(ctx, d_point.ptr, n, eqs[i].ptr, s))) return st;
                     // the joining claim: V0 V1 at z is the root V[0][z] (trees.rs:638-640)
                     if ((st = eq_dot(ctx, eqs[i].ptr, t.V[0].ptr, 1ull << n, 0, n, d_part.ptr,
                                      d_val.ptr, t.num)))
                         return st;
                     t.has_claim = true;
+                    // the copy of the point into d_point is done before the next overwrites it
+                    TSR_HIP(hipStreamSynchronize(s));
                 }
                 g.rel = REL_PRODUCT;
                 g.claim = t.num;
@@ -365,7 +366,7 @@ static Status prove_levels(Ctx *ctx, Transcript &tr, std::vector<DevTree> &trees
                 v0.base = v1.base = true;
                 v0.shift = v1.shift = 1;
                 v1.off = 1;
-                g.tabs = {eq, v0, v1};
+                g.tabs = {v0, v1};
             } else {
                 g.rel = REL_FRACTION;
                 g.claim = gl::add(t.num, gl::mul(lambda, t.den));
@@ -382,10 +383,8 @@ static Status prove_levels(Ctx *ctx, Transcript &tr, std::vector<DevTree> &trees
                 d0.src = d1.src = t.D[below].ptr;
                 n0.shift = n1.shift = d0.shift = d1.shift = 1;
                 n1.off = d1.off = 1;
-                g.tabs = {eq, n0, n1, d0, d1};
+                g.tabs = {n0, n1, d0, d1};
             }
-            // an awaited copy of the point into d_point must finish before the next overwrite
-            TSR_HIP(hipStreamSynchronize(s));
         }
         SumcheckOut out;
         if ((st = run_batch(ctx, tr, prefix, 3, insts, gamma, lambda, out))) return st;

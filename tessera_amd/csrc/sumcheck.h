@@ -41,6 +41,11 @@ struct GenInst {
     std::vector<GenTerm> terms;  // REL_TERMS: sum_t w_t prod_k T_k^mult[t][k]
     bool claim_given = false;  // padding rounds use `claim` (a tree level's) instead of S_i
     gl::fp2 claim{0, 0};
+    // eq(eq_point, x) multiplies the relation without being a table (degree 3, product or
+    // fraction relations with a claim): the rounds weight each pair by the eq of the variables
+    // not yet bound and fold the bound ones into a running factor.
+    bool eq_factor = false;
+    std::vector<gl::fp2> eq_point;  // vars coordinates, bit i of the index is coordinate i
 };
 
 // The rounds and final evaluations of a batch whose gamma (and lambda) the caller squeezed.

@@ -66,10 +66,28 @@ __device__ __forceinline__ fp2 ld_entry(const void *p, bool base, uint64_t x) {
 }
 
 // Node values of one pair, slots [p(0), p(2), ..., p(D), p(1)] (D+1 of them).
-template <int D, int REL>
+template <int D, int REL, bool EQF = false>
 __device__ __forceinline__ void pair_values(const fp2 *a, const fp2 *b, uint32_t F,
                                             const fp2 &lambda, const ScTerms &terms,
                                             fp2 out[D + 1]) {
+    if (REL == REL_FRACTION && EQF) {
+        // tables N0, N1, D0, D1 (the eq factor is applied per pair by the caller)
+#pragma unroll
+        for (int s = 0; s <= D; s++) {
+            const int z = s == D ? 1 : (s == 0 ? 0 : s + 1);
+            fp2 v[4];
+#pragma unroll
+            for (int k = 0; k < 4; k++) {
+                fp2 diff = gl::sub(b[k], a[k]);
+                fp2 at = a[k];
+                for (int m = 0; m < z; m++) at = gl::add(at, diff);
+                v[k] = at;
+            }
+            out[s] = gl::add(gl::add(gl::mul(v[0], v[3]), gl::mul(v[1], v[2])),
+                             gl::mul(lambda, gl::mul(v[2], v[3])));
+        }
+        return;
+    }
     if (REL == REL_TERMS) {
         // line values of every table at z = 0, 2, ..., D, then 1; the tables are indexed at
         // compile time (unrolled), the terms and multiplicities are uniform across the wave
@@ -161,11 +179,15 @@ __device__ __forceinline__ uint64_t shfl_xor_u64(uint64_t v, int mask) {
 //   bind + sums: dst[2q], dst[2q+1] from src[4q..4q+3]; the pair (dst[2q], dst[2q+1]) summed
 //   bind only:   dst[q] from src[2q], src[2q+1] for q < out_len
 // `first`: the sources are the instance's original tables (base entries, strides, offsets).
-template <int D, int REL>
+// EQF: the relation carries eq(r, x) as a factor, not a table: each pair q of this round is
+// weighted by E[q] = eq_lo[q mod 2^eq_a] eq_hi[q >> eq_a], the eq of the instance's point on the
+// variables above the one this round binds (the transcript kernel applies the rest).
+template <int D, int REL, bool EQF = false>
 __global__ void __launch_bounds__(SC_THREADS)
 k_sc_round(ScTables t, uint32_t F, uint32_t base_mask, bool first, uint64_t out_len,
            uint64_t pairs, const fp2 *__restrict__ r_ptr, fp2 lambda, fp2 *__restrict__ partials,
-           ScTerms terms) {
+           ScTerms terms, const fp2 *__restrict__ eq_lo, const fp2 *__restrict__ eq_hi,
+           uint32_t eq_a) {
     const bool bind = out_len > 0;
     const fp2 r = bind ? *r_ptr : fp2{0, 0};
     const uint64_t items = pairs > 0 ? pairs : out_len;
@@ -198,7 +220,12 @@ k_sc_round(ScTables t, uint32_t F, uint32_t base_mask, bool first, uint64_t out_
         }
         if (pairs > 0) {
             fp2 val[D + 1];
-            pair_values<D, REL>(a, b, F, lambda, terms, val);
+            pair_values<D, REL, EQF>(a, b, F, lambda, terms, val);
+            if (EQF) {
+                const fp2 e = gl::mul(eq_lo[q & ((1ull << eq_a) - 1)], eq_hi[q >> eq_a]);
+#pragma unroll
+                for (int z = 0; z <= D; z++) val[z] = gl::mul(val[z], e);
+            }
 #pragma unroll
             for (int z = 0; z <= D; z++) acc[z] = gl::add(acc[z], val[z]);
         }
@@ -229,10 +256,12 @@ k_sc_round(ScTables t, uint32_t F, uint32_t base_mask, bool first, uint64_t out_
 struct ScInst {
     uint32_t vars, pad;
     uint32_t wsum_given;  // 1: wsum set by the host (claims, or instances on no variable)
-    uint32_t pad_;
+    uint32_t eqf;         // 1: eq(eq_r, x) is a factor of the relation (see k_sc_round)
     fp2 weight;    // gamma^i
     fp2 wsum;      // gamma^i S_i
     fp2 *partials; // (D+1) per block of its last k_sc_round
+    const fp2 *eq_r;  // the eq point, one coordinate per own round
+    fp2 eq_k;         // prod over bound rounds t of eq(r_t, c_t)
 };
 
 struct ScCtl {
@@ -273,7 +302,19 @@ k_sc_transcript(uint32_t j, ScInst *__restrict__ inst, uint32_t count,
     for (int z = 0; z < D; z++) msg[z] = fp2{0, 0};
     for (uint32_t i = 0; i < count; i++) {
         ScInst &in = inst[i];
-        const fp2 *s = sums + (uint64_t)i * (D + 1);
+        fp2 *s = sums + (uint64_t)i * (D + 1);
+        if (in.eqf && j >= in.pad) {
+            // the eq line of this round: K ((1 - r_t)(1 - z) + r_t z) at the nodes 0, 2..D, 1
+            const fp2 rt = in.eq_r[j - in.pad];
+            const fp2 l0 = gl::sub(fp2{1, 0}, rt), step = gl::sub(rt, l0);
+            fp2 lz = gl::add(l0, gl::add(step, step));  // z = 2
+            s[0] = gl::mul(s[0], gl::mul(in.eq_k, l0));
+            for (int z = 1; z < D; z++) {
+                s[z] = gl::mul(s[z], gl::mul(in.eq_k, lz));
+                lz = gl::add(lz, step);
+            }
+            s[D] = gl::mul(s[D], gl::mul(in.eq_k, rt));
+        }
         if (j == 0 && in.pad > 0 && !in.wsum_given)
             in.wsum = gl::mul(in.weight, gl::add(s[0], s[D]));  // S_i = p(0) + p(1)
         if (j < in.pad) {
@@ -289,6 +330,14 @@ k_sc_transcript(uint32_t j, ScInst *__restrict__ inst, uint32_t count,
     fp2 r = dev_challenge_fp2(t, keys, labels + (2 * j + 1) * LABEL_SLOT, label_lens[2 * j + 1], hbuf);
     ctl->t = t;
     ctl->r = r;
+    for (uint32_t i = 0; i < count; i++) {
+        ScInst &in = inst[i];
+        if (!in.eqf || j < in.pad) continue;
+        // K *= eq(r_t, c) = r_t c + (1 - r_t)(1 - c)
+        const fp2 rt = in.eq_r[j - in.pad];
+        const fp2 e = gl::add(gl::mul(rt, r), gl::mul(gl::sub(fp2{1, 0}, rt), gl::sub(fp2{1, 0}, r)));
+        in.eq_k = gl::mul(in.eq_k, e);
+    }
     for (int z = 0; z < D; z++) rounds_out[(uint64_t)j * D + z] = msg[z];
     chals_out[j] = r;
 }
@@ -300,20 +349,57 @@ __global__ void k_sc_gather(const void *const *__restrict__ ptrs, const uint32_t
     if (e < n) out[e] = ld_entry(ptrs[e], base[e] != 0, off[e]);
 }
 
+// eq tables of segments of one point: table g holds eq(point[start_g .. start_g + n_g), x) for
+// x < 2^n_g at out + off_g (blockIdx.y = g).
+struct EqSeg {
+    uint64_t off;
+    uint32_t start, n;
+};
+
+__global__ void __launch_bounds__(256)
+k_eq_segments(const fp2 *__restrict__ point, const EqSeg *__restrict__ segs, fp2 *__restrict__ out) {
+    const EqSeg g = segs[blockIdx.y];
+    const uint64_t len = 1ull << g.n;
+    for (uint64_t x = (uint64_t)blockIdx.x * 256 + threadIdx.x; x < len; x += (uint64_t)gridDim.x * 256) {
+        fp2 e{1, 0};
+        for (uint32_t i = 0; i < g.n; i++) {
+            const fp2 r = point[g.start + i];
+            e = gl::mul(e, ((x >> i) & 1) ? r : gl::sub(fp2{1, 0}, r));
+        }
+        out[g.off + x] = e;
+    }
+}
+
 // ---------------------------------------------------------------------------------------------
 
-template <int D, int REL>
+// The eq factor of one launch: the pair weights' half tables (lo == nullptr: none).
+struct EqSplit {
+    const fp2 *lo = nullptr, *hi = nullptr;
+    uint32_t a = 0;
+};
+
+template <int D, int REL, bool EQF = false>
 static void launch_round(dim3 grid, hipStream_t s, const ScTables &t, uint32_t F,
                          uint32_t base_mask, bool first, uint64_t out_len, uint64_t pairs,
-                         const fp2 *r, fp2 lambda, fp2 *partials, const ScTerms &terms) {
-    hipLaunchKernelGGL((k_sc_round<D, REL>), grid, dim3(SC_THREADS), 0, s, t, F, base_mask,
-                       first, out_len, pairs, r, lambda, partials, terms);
+                         const fp2 *r, fp2 lambda, fp2 *partials, const ScTerms &terms,
+                         const EqSplit &eq = EqSplit()) {
+    hipLaunchKernelGGL((k_sc_round<D, REL, EQF>), grid, dim3(SC_THREADS), 0, s, t, F, base_mask,
+                       first, out_len, pairs, r, lambda, partials, terms, eq.lo, eq.hi, eq.a);
 }
 
 static void launch_round_d(uint32_t d, int rel, dim3 grid, hipStream_t s, const ScTables &t,
                            uint32_t F, uint32_t base_mask, bool first, uint64_t out_len,
                            uint64_t pairs, const fp2 *r, fp2 lambda, fp2 *partials,
-                           const ScTerms &terms) {
+                           const ScTerms &terms, const EqSplit &eq = EqSplit()) {
+    if (eq.lo) {  // tree levels: degree 3
+        if (rel == REL_FRACTION)
+            launch_round<3, REL_FRACTION, true>(grid, s, t, F, base_mask, first, out_len, pairs, r,
+                                                lambda, partials, terms, eq);
+        else
+            launch_round<3, REL_PRODUCT, true>(grid, s, t, F, base_mask, first, out_len, pairs, r,
+                                               lambda, partials, terms, eq);
+        return;
+    }
     if (rel == REL_FRACTION) {
         launch_round<3, REL_FRACTION>(grid, s, t, F, base_mask, first, out_len, pairs, r, lambda,
                                       partials, terms);
@@ -390,13 +476,66 @@ Status run_batch(Ctx *ctx, Transcript &tr, const std::string &p, uint32_t D,
         hinst[i].weight = w;
         hinst[i].wsum_given = insts[i].claim_given ? 1 : 0;
         hinst[i].wsum = insts[i].claim_given ? gl::mul(w, insts[i].claim) : fp2{0, 0};
+        hinst[i].eqf = 0;
+        hinst[i].eq_r = nullptr;
+        hinst[i].eq_k = fp2{1, 0};
         w = gl::mul(w, gamma);
+    }
+    Status st;
+
+    // eq factors: the points on the device, and for each own round t of each such instance the
+    // half tables of E_t = eq(point[t+1 ..], .) over the pairs of round t (lo: the low half of
+    // the pair index, hi: the rest)
+    std::vector<fp2> eq_pts;
+    std::vector<EqSeg> segs;
+    std::vector<std::vector<EqSplit>> splits(count);
+    std::vector<uint32_t> pt_off(count, 0);
+    uint64_t seg_len = 0;
+    for (uint32_t i = 0; i < count; i++) {
+        if (!insts[i].eq_factor) continue;
+        const GenInst &g = insts[i];
+        if (D != 3 || (g.rel != REL_PRODUCT && g.rel != REL_FRACTION) || !g.claim_given ||
+            g.eq_point.size() != g.vars)
+            return Status::fail(TSR_DECLINED, "an eq factor needs degree 3, a claim and a point of vars coordinates");
+        pt_off[i] = (uint32_t)eq_pts.size();
+        eq_pts.insert(eq_pts.end(), g.eq_point.begin(), g.eq_point.end());
+        for (uint32_t t = 0; t < g.vars; t++) {
+            const uint32_t m = g.vars - t - 1, a = m / 2;
+            segs.push_back(EqSeg{seg_len, pt_off[i] + t + 1, a});
+            seg_len += 1ull << a;
+            segs.push_back(EqSeg{seg_len, pt_off[i] + t + 1 + a, m - a});
+            seg_len += 1ull << (m - a);
+        }
+    }
+    DevBuf<fp2> d_eqpt, d_eqtab;
+    DevBuf<EqSeg> d_segs;
+    if (!segs.empty()) {
+        if ((st = d_eqpt.alloc(ctx, eq_pts.size())) || (st = d_eqtab.alloc(ctx, seg_len)) ||
+            (st = d_segs.alloc(ctx, segs.size())))
+            return st;
+        TSR_HIP(hipMemcpyAsync(d_eqpt.ptr, eq_pts.data(), 16 * eq_pts.size(), hipMemcpyHostToDevice, s));
+        TSR_HIP(hipMemcpyAsync(d_segs.ptr, segs.data(), sizeof(EqSeg) * segs.size(), hipMemcpyHostToDevice, s));
+        uint32_t widest = 0;
+        for (auto &g : segs) widest = std::max(widest, g.n);
+        for (size_t g0 = 0; g0 < segs.size(); g0 += 65535) {
+            const unsigned ny = (unsigned)std::min<size_t>(65535, segs.size() - g0);
+            hipLaunchKernelGGL(k_eq_segments, dim3((unsigned)std::min<uint64_t>(((1ull << widest) + 255) / 256, 1024), ny), dim3(256), 0,
+                               s, d_eqpt.ptr, d_segs.ptr + g0, d_eqtab.ptr);
+        }
+        if ((st = check_launch("eq segments"))) return st;
+        size_t g = 0;
+        for (uint32_t i = 0; i < count; i++) {
+            if (!insts[i].eq_factor) continue;
+            hinst[i].eqf = 1;
+            hinst[i].eq_r = d_eqpt.ptr + pt_off[i];
+            for (uint32_t t = 0; t < insts[i].vars; t++, g += 2)
+                splits[i].push_back(EqSplit{d_eqtab.ptr + segs[g].off, d_eqtab.ptr + segs[g + 1].off, segs[g].n});
+        }
     }
 
     // working buffers: per table a (2^{n-1}) and b (2^{n-2}) Fp2
     std::vector<std::vector<DevBuf<fp2>>> bufa(count), bufb(count);
     std::vector<DevBuf<fp2>> partials(count);
-    Status st;
     for (uint32_t i = 0; i < count; i++) {
         const uint32_t v = insts[i].vars;
         const uint64_t len = 1ull << v;
@@ -537,7 +676,8 @@ Status run_batch(Ctx *ctx, Transcript &tr, const std::string &p, uint32_t D,
             uint64_t pairs = 1ull << (insts[i].vars - 1);
             launch_round_d(D, insts[i].rel, dim3(blocks_for_pairs(pairs)), s, t,
                            (uint32_t)insts[i].tabs.size(), bm, true, 0, pairs, nullptr, lambda,
-                           partials[i].ptr, sterms[i]);
+                           partials[i].ptr, sterms[i],
+                           splits[i].empty() ? EqSplit() : splits[i][0]);
         }
     }
     for (uint32_t j = 0; j < n_max; j++) {
@@ -556,9 +696,11 @@ Status run_batch(Ctx *ctx, Transcript &tr, const std::string &p, uint32_t D,
             tables_of(i, t, tt, bm);
             uint64_t out_len = 1ull << (insts[i].vars - t - 1);
             uint64_t pairs = out_len / 2;  // next own round's pairs (0 after the last)
+            // the pairs summed are those of own round t + 1 (none after the last round)
+            const EqSplit eq = splits[i].empty() ? EqSplit() : splits[i][std::min<uint32_t>(t + 1, insts[i].vars - 1)];
             launch_round_d(D, insts[i].rel, dim3(blocks_for_pairs(pairs ? pairs : out_len)), s,
                            tt, (uint32_t)insts[i].tabs.size(), bm, t == 0, out_len, pairs,
-                           &d_ctl.ptr->r, lambda, partials[i].ptr, sterms[i]);
+                           &d_ctl.ptr->r, lambda, partials[i].ptr, sterms[i], eq);
         }
     }
     if ((st = check_launch("sumcheck rounds"))) return st;

@@ -66,28 +66,10 @@ __device__ __forceinline__ fp2 ld_entry(const void *p, bool base, uint64_t x) {
 }
 
 // Node values of one pair, slots [p(0), p(2), ..., p(D), p(1)] (D+1 of them).
-template <int D, int REL, bool EQF = false>
+template <int D, int REL>
 __device__ __forceinline__ void pair_values(const fp2 *a, const fp2 *b, uint32_t F,
                                             const fp2 &lambda, const ScTerms &terms,
                                             fp2 out[D + 1]) {
-    if (REL == REL_FRACTION && EQF) {
-        // tables N0, N1, D0, D1 (the eq factor is applied per pair by the caller)
-#pragma unroll
-        for (int s = 0; s <= D; s++) {
-            const int z = s == D ? 1 : (s == 0 ? 0 : s + 1);
-            fp2 v[4];
-#pragma unroll
-            for (int k = 0; k < 4; k++) {
-                fp2 diff = gl::sub(b[k], a[k]);
-                fp2 at = a[k];
-                for (int m = 0; m < z; m++) at = gl::add(at, diff);
-                v[k] = at;
-            }
-            out[s] = gl::add(gl::add(gl::mul(v[0], v[3]), gl::mul(v[1], v[2])),
-                             gl::mul(lambda, gl::mul(v[2], v[3])));
-        }
-        return;
-    }
     if (REL == REL_TERMS) {
         // line values of every table at z = 0, 2, ..., D, then 1; the tables are indexed at
         // compile time (unrolled), the terms and multiplicities are uniform across the wave
@@ -179,15 +161,11 @@ __device__ __forceinline__ uint64_t shfl_xor_u64(uint64_t v, int mask) {
 //   bind + sums: dst[2q], dst[2q+1] from src[4q..4q+3]; the pair (dst[2q], dst[2q+1]) summed
 //   bind only:   dst[q] from src[2q], src[2q+1] for q < out_len
 // `first`: the sources are the instance's original tables (base entries, strides, offsets).
-// EQF: the relation carries eq(r, x) as a factor, not a table: each pair q of this round is
-// weighted by E[q] = eq_lo[q mod 2^eq_a] eq_hi[q >> eq_a], the eq of the instance's point on the
-// variables above the one this round binds (the transcript kernel applies the rest).
-template <int D, int REL, bool EQF = false>
+template <int D, int REL>
 __global__ void __launch_bounds__(SC_THREADS)
 k_sc_round(ScTables t, uint32_t F, uint32_t base_mask, bool first, uint64_t out_len,
            uint64_t pairs, const fp2 *__restrict__ r_ptr, fp2 lambda, fp2 *__restrict__ partials,
-           ScTerms terms, const fp2 *__restrict__ eq_lo, const fp2 *__restrict__ eq_hi,
-           uint32_t eq_a) {
+           ScTerms terms) {
     const bool bind = out_len > 0;
     const fp2 r = bind ? *r_ptr : fp2{0, 0};
     const uint64_t items = pairs > 0 ? pairs : out_len;
@@ -220,12 +198,7 @@ k_sc_round(ScTables t, uint32_t F, uint32_t base_mask, bool first, uint64_t out_
         }
         if (pairs > 0) {
             fp2 val[D + 1];
-            pair_values<D, REL, EQF>(a, b, F, lambda, terms, val);
-            if (EQF) {
-                const fp2 e = gl::mul(eq_lo[q & ((1ull << eq_a) - 1)], eq_hi[q >> eq_a]);
-#pragma unroll
-                for (int z = 0; z <= D; z++) val[z] = gl::mul(val[z], e);
-            }
+            pair_values<D, REL>(a, b, F, lambda, terms, val);
 #pragma unroll
             for (int z = 0; z <= D; z++) acc[z] = gl::add(acc[z], val[z]);
         }
@@ -249,6 +222,108 @@ k_sc_round(ScTables t, uint32_t F, uint32_t base_mask, bool first, uint64_t out_
         fp2 s = red[0][threadIdx.x];
         for (int w = 1; w < SC_THREADS / 64; w++) s = gl::add(s, red[w][threadIdx.x]);
         partials[(uint64_t)blockIdx.x * (D + 1) + threadIdx.x] = s;
+    }
+}
+
+// One own round of a level instance whose relation is eq(r, x) F(x) with eq a factor, not a
+// table (degree 3). Modes as k_sc_round's. Pair q of the round is weighted by
+// E[q] = eq_lo[q mod 2^eq_a] eq_hi[q >> eq_a], the eq of the point on the variables above the one
+// the round binds; the transcript kernel multiplies in the eq of the bound variables and the
+// line of the current one. Without that line, E F(z) is quadratic in z: each pair gives its
+// values at z = 0, 1, 2, and each block's node 3 follows as f(3) = f(0) + 3 (f(2) - f(1)).
+//   REL_PRODUCT:  F = T0 T1 (tables T0, T1), E folded into T0's line
+//   REL_FRACTION: F = N0 D1 + N1 D0 + lambda D0 D1 (tables N0, N1, D0, D1), computed as
+//                 (E N0) D1 + (E D0)(N1 + lambda D1); UNIT (N0 = N1 = 1, tables 0 and 1 absent):
+//                 E (D0 + D1) + (lambda E D0) D1
+// Line values: X(0) = a, X(1) = b, X(2) = 2b - a.
+template <int REL, bool UNIT>
+__global__ void __launch_bounds__(SC_THREADS)
+k_sc_round_eq(ScTables t, uint32_t base_mask, bool first, uint64_t out_len, uint64_t pairs,
+              const fp2 *__restrict__ r_ptr, fp2 lambda, fp2 *__restrict__ partials,
+              const fp2 *__restrict__ eq_lo, const fp2 *__restrict__ eq_hi, uint32_t eq_a) {
+    constexpr int F = REL == REL_PRODUCT ? 2 : 4;
+    constexpr int K0 = (REL == REL_FRACTION && UNIT) ? 2 : 0;
+    const bool bind = out_len > 0;
+    const fp2 r = bind ? *r_ptr : fp2{0, 0};
+    const uint64_t items = pairs > 0 ? pairs : out_len;
+    const uint64_t stride = (uint64_t)gridDim.x * SC_THREADS;
+    fp2 acc[3] = {{0, 0}, {0, 0}, {0, 0}};
+    auto line2 = [](const fp2 &a, const fp2 &b) { return gl::sub(gl::add(b, b), a); };
+
+    for (uint64_t q = (uint64_t)blockIdx.x * SC_THREADS + threadIdx.x; q < items; q += stride) {
+        fp2 a[F], b[F];
+#pragma unroll
+        for (int k = K0; k < F; k++) {
+            const bool base = first && ((base_mask >> k) & 1);
+            const uint32_t sh = first ? t.shift[k] : 0, of = first ? t.off[k] : 0;
+            auto at = [&](uint64_t x) { return ld_entry(t.src[k], base, (x << sh) + of); };
+            if (!bind) {
+                a[k] = at(2 * q);
+                b[k] = at(2 * q + 1);
+            } else if (pairs > 0) {
+                fp2 e0 = at(4 * q), e1 = at(4 * q + 1), e2 = at(4 * q + 2), e3 = at(4 * q + 3);
+                a[k] = gl::add(e0, gl::mul(r, gl::sub(e1, e0)));
+                b[k] = gl::add(e2, gl::mul(r, gl::sub(e3, e2)));
+                t.dst[k][2 * q] = a[k];
+                t.dst[k][2 * q + 1] = b[k];
+            } else {
+                fp2 e0 = at(2 * q), e1 = at(2 * q + 1);
+                t.dst[k][q] = gl::add(e0, gl::mul(r, gl::sub(e1, e0)));
+            }
+        }
+        if (pairs == 0) continue;
+        const fp2 e = gl::mul(eq_lo[q & ((1ull << eq_a) - 1)], eq_hi[q >> eq_a]);
+        fp2 v0, v1, v2;
+        if (REL == REL_PRODUCT) {
+            const fp2 x0 = gl::mul(e, a[0]), x1 = gl::mul(e, b[0]);
+            v0 = gl::mul(x0, a[1]);
+            v1 = gl::mul(x1, b[1]);
+            v2 = gl::mul(line2(x0, x1), line2(a[1], b[1]));
+        } else if (UNIT) {
+            const fp2 u = gl::mul(lambda, e);
+            const fp2 s0 = gl::mul(e, gl::add(a[2], a[3])), s1 = gl::mul(e, gl::add(b[2], b[3]));
+            const fp2 p0 = gl::mul(u, a[2]), p1 = gl::mul(u, b[2]);
+            v0 = gl::add(s0, gl::mul(p0, a[3]));
+            v1 = gl::add(s1, gl::mul(p1, b[3]));
+            v2 = gl::add(line2(s0, s1), gl::mul(line2(p0, p1), line2(a[3], b[3])));
+        } else {
+            const fp2 n0 = gl::mul(e, a[0]), n1 = gl::mul(e, b[0]);
+            const fp2 d0 = gl::mul(e, a[2]), d1 = gl::mul(e, b[2]);
+            const fp2 w0 = gl::add(a[1], gl::mul(lambda, a[3])), w1 = gl::add(b[1], gl::mul(lambda, b[3]));
+            v0 = gl::add(gl::mul(n0, a[3]), gl::mul(d0, w0));
+            v1 = gl::add(gl::mul(n1, b[3]), gl::mul(d1, w1));
+            v2 = gl::add(gl::mul(line2(n0, n1), line2(a[3], b[3])), gl::mul(line2(d0, d1), line2(w0, w1)));
+        }
+        acc[0] = gl::add(acc[0], v0);
+        acc[1] = gl::add(acc[1], v1);
+        acc[2] = gl::add(acc[2], v2);
+    }
+    if (pairs == 0) return;
+
+    __shared__ fp2 red[SC_THREADS / 64][3];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+#pragma unroll
+    for (int z = 0; z < 3; z++) {
+        fp2 v = acc[z];
+        for (int m = 32; m >= 1; m >>= 1) {
+            fp2 o{shfl_xor_u64(v.c0, m), shfl_xor_u64(v.c1, m)};
+            v = gl::add(v, o);
+        }
+        if (lane == 0) red[wave][z] = v;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        fp2 f[3];
+        for (int z = 0; z < 3; z++) {
+            f[z] = red[0][z];
+            for (int w = 1; w < SC_THREADS / 64; w++) f[z] = gl::add(f[z], red[w][z]);
+        }
+        const fp2 d = gl::sub(f[2], f[1]);
+        fp2 *out = partials + (uint64_t)blockIdx.x * 4;  // slots z = 0, 2, 3, 1
+        out[0] = f[0];
+        out[1] = f[2];
+        out[2] = gl::add(f[0], gl::add(gl::add(d, d), d));
+        out[3] = f[1];
     }
 }
 
@@ -378,26 +453,36 @@ struct EqSplit {
     uint32_t a = 0;
 };
 
-template <int D, int REL, bool EQF = false>
+template <int D, int REL>
 static void launch_round(dim3 grid, hipStream_t s, const ScTables &t, uint32_t F,
                          uint32_t base_mask, bool first, uint64_t out_len, uint64_t pairs,
-                         const fp2 *r, fp2 lambda, fp2 *partials, const ScTerms &terms,
-                         const EqSplit &eq = EqSplit()) {
-    hipLaunchKernelGGL((k_sc_round<D, REL, EQF>), grid, dim3(SC_THREADS), 0, s, t, F, base_mask,
-                       first, out_len, pairs, r, lambda, partials, terms, eq.lo, eq.hi, eq.a);
+                         const fp2 *r, fp2 lambda, fp2 *partials, const ScTerms &terms) {
+    hipLaunchKernelGGL((k_sc_round<D, REL>), grid, dim3(SC_THREADS), 0, s, t, F, base_mask,
+                       first, out_len, pairs, r, lambda, partials, terms);
+}
+
+template <int REL, bool UNIT>
+static void launch_round_eq(dim3 grid, hipStream_t s, const ScTables &t, uint32_t base_mask,
+                            bool first, uint64_t out_len, uint64_t pairs, const fp2 *r, fp2 lambda,
+                            fp2 *partials, const EqSplit &eq) {
+    hipLaunchKernelGGL((k_sc_round_eq<REL, UNIT>), grid, dim3(SC_THREADS), 0, s, t, base_mask,
+                       first, out_len, pairs, r, lambda, partials, eq.lo, eq.hi, eq.a);
 }
 
 static void launch_round_d(uint32_t d, int rel, dim3 grid, hipStream_t s, const ScTables &t,
                            uint32_t F, uint32_t base_mask, bool first, uint64_t out_len,
                            uint64_t pairs, const fp2 *r, fp2 lambda, fp2 *partials,
                            const ScTerms &terms, const EqSplit &eq = EqSplit()) {
-    if (eq.lo) {  // tree levels: degree 3
-        if (rel == REL_FRACTION)
-            launch_round<3, REL_FRACTION, true>(grid, s, t, F, base_mask, first, out_len, pairs, r,
-                                                lambda, partials, terms, eq);
+    if (eq.lo) {  // tree levels: degree 3, tables T0 T1 or N0 N1 D0 D1
+        if (rel == REL_PRODUCT)
+            launch_round_eq<REL_PRODUCT, false>(grid, s, t, base_mask, first, out_len, pairs, r,
+                                                lambda, partials, eq);
+        else if (!t.src[0])
+            launch_round_eq<REL_FRACTION, true>(grid, s, t, base_mask, first, out_len, pairs, r,
+                                                lambda, partials, eq);
         else
-            launch_round<3, REL_PRODUCT, true>(grid, s, t, F, base_mask, first, out_len, pairs, r,
-                                               lambda, partials, terms, eq);
+            launch_round_eq<REL_FRACTION, false>(grid, s, t, base_mask, first, out_len, pairs, r,
+                                                 lambda, partials, eq);
         return;
     }
     if (rel == REL_FRACTION) {
@@ -494,9 +579,16 @@ Status run_batch(Ctx *ctx, Transcript &tr, const std::string &p, uint32_t D,
     for (uint32_t i = 0; i < count; i++) {
         if (!insts[i].eq_factor) continue;
         const GenInst &g = insts[i];
-        if (D != 3 || (g.rel != REL_PRODUCT && g.rel != REL_FRACTION) || !g.claim_given ||
-            g.eq_point.size() != g.vars)
-            return Status::fail(TSR_DECLINED, "an eq factor needs degree 3, a claim and a point of vars coordinates");
+        const size_t nt = g.rel == REL_PRODUCT ? 2 : 4;
+        bool shape = D == 3 && (g.rel == REL_PRODUCT || g.rel == REL_FRACTION) && g.claim_given &&
+                     g.eq_point.size() == g.vars && g.tabs.size() == nt;
+        // tables present, except both numerators of a fraction with unit numerators
+        for (size_t k = 0; shape && k < nt; k++)
+            shape = g.tabs[k].src || (g.rel == REL_FRACTION && k < 2 && !g.tabs[0].src && !g.tabs[1].src);
+        if (g.rel == REL_FRACTION && shape) shape = !g.tabs[0].src == !g.tabs[1].src;
+        if (!shape)
+            return Status::fail(TSR_DECLINED, "an eq factor needs degree 3, a claim, a point of vars "
+                                              "coordinates and tables T0 T1 or N0 N1 D0 D1");
         pt_off[i] = (uint32_t)eq_pts.size();
         eq_pts.insert(eq_pts.end(), g.eq_point.begin(), g.eq_point.end());
         for (uint32_t t = 0; t < g.vars; t++) {

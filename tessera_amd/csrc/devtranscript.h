@@ -1,4 +1,4 @@
-// The Fiat-Shamir transcript on the device: one lane runs the transcript steps of a round loop
+// The Fiat-Shamir transcript on the device: one wave runs the transcript steps of a round loop
 // so that a whole sumcheck needs no host round trip. Byte conventions are those of
 // crates/tessera-hash/src/transcript.rs (absorb: state || u64LE(|label|) || label || TCE(value);
 // squeeze: state || Ascii(label) || u64LE(counter), then the ratchet state || out). Every input
@@ -23,92 +23,155 @@ struct DevTranscript {
 
 constexpr int LABEL_SLOT = 64;  // bytes per label in a label table
 
-// Hash inputs are assembled in a caller-provided LDS buffer of DEV_BUF_WORDS words (the
-// byte-addressed writes and the word reads of the message blocks then stay in LDS instead of
-// private scratch memory).
+// Hash inputs are assembled in an LDS buffer of DEV_BUF_WORDS words.
 constexpr int DEV_BUF_WORDS = 96;  // 384 bytes: state, label, count and 11 Fp2 fit
 
-// H(key, buf[0..len)) for len < 1024: one chunk, blocks of 64 bytes, the last one ROOT. The
-// bytes of buf from len up to the next multiple of 64 are zeroed here.
-__device__ inline void dev_hash_chunk(const uint32_t key[8], uint32_t *buf, uint32_t len,
-                                      uint32_t out[8]) {
-    uint8_t *bytes = (uint8_t *)buf;
-    const uint32_t nblocks = len == 0 ? 1 : (len + 63) / 64;
-    for (uint32_t at = len; at < 64 * nblocks; at++) bytes[at] = 0;
-    uint32_t cv[8];
-    for (int i = 0; i < 8; i++) cv[i] = key[i];
-    for (uint32_t b = 0; b < nblocks; b++) {
-        uint32_t m[16];
+// One wave runs the transcript: every lane of the wave calls these functions with the same
+// arguments (uniform control flow). Lane q = lane mod 4 of each quad holds column q of the
+// BLAKE3 state (words q, 4 + q, 8 + q, 12 + q); the diagonal step rotates rows 1-3 across the
+// quad with DPP quad permutations, so a compression is 7 x 2 G functions deep instead of 7 x 8.
+// Every quad computes the same values. Inputs are assembled in LDS by all 64 lanes at once.
+
+// Orders this wave's LDS accesses (LDS operations of one wave complete in issue order).
+__device__ __forceinline__ void wave_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// Lane q of each quad reads lane (q + k) mod 4: k = 1, 2, 3.
+template <int K>
+__device__ __forceinline__ uint32_t quad_rot(uint32_t v) {
+    constexpr int ctrl = K == 1 ? 0x39 : K == 2 ? 0x4E : 0x93;  // quad_perm selectors
+    return (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, ctrl, 0xF, 0xF, false);
+}
+
+// Nibble i of B3_SCHED[r] is the index of the i-th message word of round r (the message
+// permutation 2,6,3,10,7,0,4,13,1,11,12,5,9,14,15,8 applied r times).
+constexpr uint64_t B3_SCHED[7] = {0xfedcba9876543210ull, 0x8fe95cb1d407a362ull,
+                                  0x18fb0956e72dca43ull, 0x61852b04fd3e9c7aull,
+                                  0x461035278eafb9dcull, 0x7462a03d1fc85be9ull,
+                                  0xd743c2ae689105fbull};
+
+__device__ __forceinline__ void quad_g(uint32_t &a, uint32_t &b, uint32_t &c, uint32_t &d,
+                                       uint32_t mx, uint32_t my) {
+    a = a + b + mx;
+    d = b3::rotr(d ^ a, 16);
+    c = c + d;
+    b = b3::rotr(b ^ c, 12);
+    a = a + b + my;
+    d = b3::rotr(d ^ a, 8);
+    c = c + d;
+    b = b3::rotr(b ^ c, 7);
+}
+
+// The compression of the 16 LDS words at blk (counter 0): (a, b) hold chaining value words
+// (q, 4 + q) and receive the output's.
+__device__ __forceinline__ void quad_compress(uint32_t &a, uint32_t &b, const uint32_t *blk,
+                                              uint32_t blen, uint32_t flags) {
+    const uint32_t q = threadIdx.x & 3;
+    uint32_t c = q == 0 ? B3_IV0 : q == 1 ? B3_IV1 : q == 2 ? B3_IV2 : B3_IV3;
+    uint32_t d = q == 2 ? blen : q == 3 ? flags : 0u;
+    uint32_t m[28];
 #pragma unroll
-        for (int i = 0; i < 16; i++) m[i] = buf[16 * b + i];
-        uint32_t flags = b3::DERIVE_KEY_MATERIAL;
-        if (b == 0) flags |= b3::CHUNK_START;
+    for (int r = 0; r < 7; r++) {
+        m[4 * r + 0] = blk[(B3_SCHED[r] >> (8 * q)) & 15];
+        m[4 * r + 1] = blk[(B3_SCHED[r] >> (8 * q + 4)) & 15];
+        m[4 * r + 2] = blk[(B3_SCHED[r] >> (32 + 8 * q)) & 15];
+        m[4 * r + 3] = blk[(B3_SCHED[r] >> (36 + 8 * q)) & 15];
+    }
+#pragma unroll
+    for (int r = 0; r < 7; r++) {
+        quad_g(a, b, c, d, m[4 * r], m[4 * r + 1]);
+        b = quad_rot<1>(b);
+        c = quad_rot<2>(c);
+        d = quad_rot<3>(d);
+        quad_g(a, b, c, d, m[4 * r + 2], m[4 * r + 3]);
+        b = quad_rot<3>(b);
+        c = quad_rot<2>(c);
+        d = quad_rot<1>(d);
+    }
+    a ^= c;
+    b ^= d;
+}
+
+// H(key, buf[0..len)) for len < 1024: one chunk of derive-key material blocks, the last one
+// ROOT; the bytes of buf from len to the next multiple of 64 are zero. The 8 output words go to
+// out (LDS).
+__device__ inline void wave_hash_chunk(const uint32_t key[8], const uint32_t *buf, uint32_t len,
+                                       uint32_t *out) {
+    const uint32_t q = threadIdx.x & 3;
+    const uint32_t nb = len == 0 ? 1 : (len + 63) / 64;
+    uint32_t a = q == 0 ? key[0] : q == 1 ? key[1] : q == 2 ? key[2] : key[3];
+    uint32_t b = q == 0 ? key[4] : q == 1 ? key[5] : q == 2 ? key[6] : key[7];
+    for (uint32_t i = 0; i < nb; i++) {
+        uint32_t flags = b3::DERIVE_KEY_MATERIAL | (i == 0 ? b3::CHUNK_START : 0u);
         uint32_t blen = 64;
-        if (b == nblocks - 1) {
+        if (i == nb - 1) {
             flags |= b3::CHUNK_END | b3::ROOT;
-            blen = len - 64 * b;
+            blen = len - 64 * i;
         }
-        b3::compress(cv, m, 0, blen, flags, cv);
+        quad_compress(a, b, buf + 16 * i, blen, flags);
     }
-    for (int i = 0; i < 8; i++) out[i] = cv[i];
-}
-
-__device__ inline uint32_t put_words(uint32_t *buf, uint32_t at, const uint32_t *w, int n) {
-    uint8_t *b = (uint8_t *)buf;
-    for (int i = 0; i < n; i++)
-        for (int j = 0; j < 4; j++) b[at++] = (uint8_t)(w[i] >> (8 * j));
-    return at;
-}
-__device__ inline uint32_t put_u64(uint32_t *buf, uint32_t at, uint64_t v) {
-    uint8_t *b = (uint8_t *)buf;
-    for (int j = 0; j < 8; j++) b[at++] = (uint8_t)(v >> (8 * j));
-    return at;
-}
-__device__ inline uint32_t put_label(uint32_t *buf, uint32_t at, const uint8_t *label,
-                                     uint32_t len) {
-    at = put_u64(buf, at, len);
-    uint8_t *b = (uint8_t *)buf;
-    for (uint32_t i = 0; i < len; i++) b[at++] = label[i];
-    return at;
-}
-
-// absorb(label, Vec<Fp2>): TCE = u64LE(count) then c0 || c1 of each element (count <= 11).
-__device__ inline void dev_absorb_fp2s(DevTranscript &t, const DevKeys &k, const uint8_t *label,
-                                       uint32_t label_len, const gl::fp2 *v, uint32_t count,
-                                       uint32_t *buf) {
-    uint32_t at = put_words(buf, 0, t.state, 8);
-    at = put_label(buf, at, label, label_len);
-    at = put_u64(buf, at, count);
-    for (uint32_t i = 0; i < count; i++) {
-        at = put_u64(buf, at, v[i].c0);
-        at = put_u64(buf, at, v[i].c1);
+    if ((threadIdx.x & 63) < 4) {
+        out[q] = a;
+        out[4 + q] = b;
     }
-    dev_hash_chunk(k.absorb, buf, at, t.state);
+    wave_sync();
 }
 
-__device__ inline void dev_squeeze(DevTranscript &t, const DevKeys &k, const uint8_t *label,
-                                   uint32_t label_len, uint32_t out[8], uint32_t *buf) {
-    uint32_t at = put_words(buf, 0, t.state, 8);
-    at = put_label(buf, at, label, label_len);
-    at = put_u64(buf, at, t.counter);
-    dev_hash_chunk(k.squeeze, buf, at, out);
-    t.counter++;
-    for (int i = 0; i < 8; i++) {
-        buf[i] = t.state[i];
-        buf[8 + i] = out[i];
+// buf = st (8 words) || u64LE(L) || label[0..L) || u64LE(x) || tail[0..tail_len), zero to the
+// next multiple of 64 bytes; returns the length. The absorb input with x = count and the TCE of
+// the values as tail, or the squeeze input with x = counter and no tail.
+__device__ inline uint32_t wave_put(uint32_t *buf, const uint32_t *st, const uint8_t *label,
+                                    uint32_t L, uint64_t x, const uint8_t *tail, uint32_t tail_len) {
+    const uint32_t len = 48 + L + tail_len;
+    const uint32_t words = 16 * ((len + 63) / 64);
+    for (uint32_t w = threadIdx.x & 63; w < words; w += 64) {
+        uint32_t v = 0;
+#pragma unroll
+        for (uint32_t k = 0; k < 4; k++) {
+            const uint32_t p = 4 * w + k;
+            uint32_t byte = 0;
+            if (p < 32) byte = st[p >> 2] >> (8 * (p & 3));
+            else if (p < 36) byte = L >> (8 * (p - 32));
+            else if (p < 40) byte = 0;
+            else if (p < 40 + L) byte = label[p - 40];
+            else if (p < 48 + L) byte = (uint32_t)(x >> (8 * (p - 40 - L)));
+            else if (p < len) byte = tail[p - 48 - L];
+            v |= (byte & 0xFF) << (8 * k);
+        }
+        buf[w] = v;
     }
-    dev_hash_chunk(k.ratchet, buf, 64, t.state);
+    wave_sync();
+    return len;
 }
 
-// challenge_fp2: the first two LE words of a squeeze, squeezing again unless both are below p.
-__device__ inline gl::fp2 dev_challenge_fp2(DevTranscript &t, const DevKeys &k,
-                                            const uint8_t *label, uint32_t label_len,
-                                            uint32_t *buf) {
+// absorb(label, Vec<Fp2>) on the state st (LDS, updated): TCE = u64LE(count) then c0 || c1 of
+// each element (count <= 11; vals in LDS).
+__device__ inline void wave_absorb_fp2s(uint32_t *st, const DevKeys &k, const uint8_t *label,
+                                        uint32_t label_len, const gl::fp2 *vals, uint32_t count,
+                                        uint32_t *buf) {
+    const uint32_t len = wave_put(buf, st, label, label_len, count, (const uint8_t *)vals, 16 * count);
+    wave_hash_chunk(k.absorb, buf, len, st);
+}
+
+// challenge_fp2: squeeze (state || Ascii(label) || u64LE(counter), then the ratchet
+// state || out), taking the first two LE words pairs of the output, again unless both are
+// below p.
+__device__ inline gl::fp2 wave_challenge_fp2(uint32_t *st, uint64_t &counter, const DevKeys &k,
+                                             const uint8_t *label, uint32_t label_len,
+                                             uint32_t *buf, uint32_t *out) {
     for (;;) {
-        uint32_t o[8];
-        dev_squeeze(t, k, label, label_len, o, buf);
-        uint64_t c0 = (uint64_t)o[0] | ((uint64_t)o[1] << 32);
-        uint64_t c1 = (uint64_t)o[2] | ((uint64_t)o[3] << 32);
+        const uint32_t len = wave_put(buf, st, label, label_len, counter, nullptr, 0);
+        wave_hash_chunk(k.squeeze, buf, len, out);
+        counter++;
+        const uint32_t lane = threadIdx.x & 63;
+        if (lane < 16) buf[lane] = lane < 8 ? st[lane] : out[lane - 8];
+        wave_sync();
+        wave_hash_chunk(k.ratchet, buf, 64, st);
+        const uint64_t c0 = (uint64_t)out[0] | ((uint64_t)out[1] << 32);
+        const uint64_t c1 = (uint64_t)out[2] | ((uint64_t)out[3] << 32);
         if (c0 < gl::P && c1 < gl::P) return gl::fp2{c0, c1};
     }
 }

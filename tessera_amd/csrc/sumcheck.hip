@@ -346,6 +346,9 @@ struct ScCtl {
 
 // Round j: reduce the partial sums, form the message, absorb, squeeze r_j.
 // nblk[i]: the number of blocks whose partials instance i left for this round.
+// The per-instance steps run on one thread per instance and the transcript on wave 0 reads
+// only LDS and registers: the labels and the transcript state are loaded at the start by
+// independent loads, so the hashes wait on no global memory round trip.
 template <int D>
 __global__ void __launch_bounds__(SC_THREADS)
 k_sc_transcript(uint32_t j, ScInst *__restrict__ inst, uint32_t count,
@@ -354,14 +357,40 @@ k_sc_transcript(uint32_t j, ScInst *__restrict__ inst, uint32_t count,
                 fp2 *__restrict__ rounds_out, fp2 *__restrict__ chals_out) {
     extern __shared__ fp2 sums[];  // count x (D+1)
     __shared__ uint32_t hbuf[DEV_BUF_WORDS];
-    // each wave reduces slots x = wave, wave + 4, ...: lanes stride over the partial blocks,
-    // then a shuffle tree adds the 64 lane sums (field addition: any order gives the same sum)
+    __shared__ uint32_t lab[2 * LABEL_SLOT / 4];  // the labels of round j (absorb, then squeeze)
+    __shared__ uint32_t st[8], outw[8];
+    __shared__ fp2 msg[D];
+    __shared__ fp2 chal;
     const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    uint64_t counter = 0;
+    uint32_t len_absorb = 0, len_squeeze = 0;
+    if (wave == 0) {
+        counter = ctl->t.counter;
+        len_absorb = label_lens[2 * j];
+        len_squeeze = label_lens[2 * j + 1];
+        if (lane < 8) st[lane] = ctl->t.state[lane];
+    }
+    if (threadIdx.x < 2 * LABEL_SLOT / 4)
+        lab[threadIdx.x] = ((const uint32_t *)(labels + (size_t)2 * j * LABEL_SLOT))[threadIdx.x];
+    // each wave reduces slots x = wave, wave + 4, ...: lanes stride over the partial blocks with
+    // four loads in flight, then a shuffle tree adds the 64 lane sums (field addition: any order
+    // gives the same sum)
     for (uint32_t x = wave; x < count * (D + 1); x += SC_THREADS / 64) {
         const uint32_t i = x / (D + 1), z = x % (D + 1);
-        fp2 s{0, 0};
         const fp2 *p = inst[i].partials;
-        for (uint32_t b = lane; b < nblk[i]; b += 64) s = gl::add(s, p[(uint64_t)b * (D + 1) + z]);
+        const uint32_t nb = nblk[i];
+        fp2 s0{0, 0}, s1{0, 0}, s2{0, 0}, s3{0, 0};
+        uint32_t b = lane;
+        for (; b + 192 < nb; b += 256) {
+            const fp2 v0 = p[(uint64_t)b * (D + 1) + z], v1 = p[(uint64_t)(b + 64) * (D + 1) + z];
+            const fp2 v2 = p[(uint64_t)(b + 128) * (D + 1) + z], v3 = p[(uint64_t)(b + 192) * (D + 1) + z];
+            s0 = gl::add(s0, v0);
+            s1 = gl::add(s1, v1);
+            s2 = gl::add(s2, v2);
+            s3 = gl::add(s3, v3);
+        }
+        for (; b < nb; b += 64) s0 = gl::add(s0, p[(uint64_t)b * (D + 1) + z]);
+        fp2 s = gl::add(gl::add(s0, s1), gl::add(s2, s3));
 #pragma unroll
         for (int off = 32; off > 0; off >>= 1) {
             fp2 o;
@@ -372,11 +401,9 @@ k_sc_transcript(uint32_t j, ScInst *__restrict__ inst, uint32_t count,
         if (lane == 0) sums[x] = s;
     }
     __syncthreads();
-    if (threadIdx.x != 0) return;
-    fp2 msg[D];
-    for (int z = 0; z < D; z++) msg[z] = fp2{0, 0};
-    for (uint32_t i = 0; i < count; i++) {
-        ScInst &in = inst[i];
+    // each instance's share of the message, in place of its sums (slots 0..D-1)
+    for (uint32_t i = threadIdx.x; i < count; i += SC_THREADS) {
+        const ScInst in = inst[i];
         fp2 *s = sums + (uint64_t)i * (D + 1);
         if (in.eqf && j >= in.pad) {
             // the eq line of this round: K ((1 - r_t)(1 - z) + r_t z) at the nodes 0, 2..D, 1
@@ -390,22 +417,41 @@ k_sc_transcript(uint32_t j, ScInst *__restrict__ inst, uint32_t count,
             }
             s[D] = gl::mul(s[D], gl::mul(in.eq_k, rt));
         }
-        if (j == 0 && in.pad > 0 && !in.wsum_given)
-            in.wsum = gl::mul(in.weight, gl::add(s[0], s[D]));  // S_i = p(0) + p(1)
+        fp2 wsum = in.wsum;
+        if (j == 0 && in.pad > 0 && !in.wsum_given) {
+            wsum = gl::mul(in.weight, gl::add(s[0], s[D]));  // S_i = p(0) + p(1)
+            inst[i].wsum = wsum;
+        }
         if (j < in.pad) {
-            uint64_t scale = 1ull << (in.pad - j - 1);
-            fp2 v = gl::mul_base(in.wsum, scale);
-            for (int z = 0; z < D; z++) msg[z] = gl::add(msg[z], v);
+            const fp2 v = gl::mul_base(wsum, 1ull << (in.pad - j - 1));
+            for (int z = 0; z < D; z++) s[z] = v;
         } else {
-            for (int z = 0; z < D; z++) msg[z] = gl::add(msg[z], gl::mul(in.weight, s[z]));
+            for (int z = 0; z < D; z++) s[z] = gl::mul(in.weight, s[z]);
         }
     }
-    DevTranscript t = ctl->t;
-    dev_absorb_fp2s(t, keys, labels + (2 * j) * LABEL_SLOT, label_lens[2 * j], msg, D, hbuf);
-    fp2 r = dev_challenge_fp2(t, keys, labels + (2 * j + 1) * LABEL_SLOT, label_lens[2 * j + 1], hbuf);
-    ctl->t = t;
-    ctl->r = r;
-    for (uint32_t i = 0; i < count; i++) {
+    __syncthreads();
+    if (wave == 0) {
+        if (lane < D) {
+            fp2 m{0, 0};
+            for (uint32_t i = 0; i < count; i++) m = gl::add(m, sums[(uint64_t)i * (D + 1) + lane]);
+            msg[lane] = m;
+            rounds_out[(uint64_t)j * D + lane] = m;
+        }
+        wave_sync();
+        const uint8_t *lb = (const uint8_t *)lab;
+        wave_absorb_fp2s(st, keys, lb, len_absorb, msg, D, hbuf);
+        const fp2 r = wave_challenge_fp2(st, counter, keys, lb + LABEL_SLOT, len_squeeze, hbuf, outw);
+        if (lane < 8) ctl->t.state[lane] = st[lane];
+        if (lane == 0) {
+            ctl->t.counter = counter;
+            ctl->r = r;
+            chal = r;
+            chals_out[j] = r;
+        }
+    }
+    __syncthreads();
+    const fp2 r = chal;
+    for (uint32_t i = threadIdx.x; i < count; i += SC_THREADS) {
         ScInst &in = inst[i];
         if (!in.eqf || j < in.pad) continue;
         // K *= eq(r_t, c) = r_t c + (1 - r_t)(1 - c)
@@ -413,8 +459,6 @@ k_sc_transcript(uint32_t j, ScInst *__restrict__ inst, uint32_t count,
         const fp2 e = gl::add(gl::mul(rt, r), gl::mul(gl::sub(fp2{1, 0}, rt), gl::sub(fp2{1, 0}, r)));
         in.eq_k = gl::mul(in.eq_k, e);
     }
-    for (int z = 0; z < D; z++) rounds_out[(uint64_t)j * D + z] = msg[z];
-    chals_out[j] = r;
 }
 
 // Gathers each final evaluation: entry off[e] of the buffer at ptrs[e] (base or Fp2).

@@ -643,32 +643,6 @@ Status run_batch(Ctx *ctx, Transcript &tr, const std::string &p, uint32_t D,
             seg_len += 1ull << (m - a);
         }
     }
-    DevBuf<fp2> d_eqpt, d_eqtab;
-    DevBuf<EqSeg> d_segs;
-    if (!segs.empty()) {
-        if ((st = d_eqpt.alloc(ctx, eq_pts.size())) || (st = d_eqtab.alloc(ctx, seg_len)) ||
-            (st = d_segs.alloc(ctx, segs.size())))
-            return st;
-        TSR_HIP(hipMemcpyAsync(d_eqpt.ptr, eq_pts.data(), 16 * eq_pts.size(), hipMemcpyHostToDevice, s));
-        TSR_HIP(hipMemcpyAsync(d_segs.ptr, segs.data(), sizeof(EqSeg) * segs.size(), hipMemcpyHostToDevice, s));
-        uint32_t widest = 0;
-        for (auto &g : segs) widest = std::max(widest, g.n);
-        for (size_t g0 = 0; g0 < segs.size(); g0 += 65535) {
-            const unsigned ny = (unsigned)std::min<size_t>(65535, segs.size() - g0);
-            hipLaunchKernelGGL(k_eq_segments, dim3((unsigned)std::min<uint64_t>(((1ull << widest) + 255) / 256, 1024), ny), dim3(256), 0,
-                               s, d_eqpt.ptr, d_segs.ptr + g0, d_eqtab.ptr);
-        }
-        if ((st = check_launch("eq segments"))) return st;
-        size_t g = 0;
-        for (uint32_t i = 0; i < count; i++) {
-            if (!insts[i].eq_factor) continue;
-            hinst[i].eqf = 1;
-            hinst[i].eq_r = d_eqpt.ptr + pt_off[i];
-            for (uint32_t t = 0; t < insts[i].vars; t++, g += 2)
-                splits[i].push_back(EqSplit{d_eqtab.ptr + segs[g].off, d_eqtab.ptr + segs[g + 1].off, segs[g].n});
-        }
-    }
-
     // working buffers: per table a (2^{n-1}) and b (2^{n-2}) Fp2
     std::vector<std::vector<DevBuf<fp2>>> bufa(count), bufb(count);
     std::vector<DevBuf<fp2>> partials(count);
@@ -763,22 +737,101 @@ Status run_batch(Ctx *ctx, Transcript &tr, const std::string &p, uint32_t D,
     hctl.t.counter = tr.counter;
     hctl.r = fp2{0, 0};
 
-    DevBuf<uint8_t> d_labels;
-    DevBuf<uint32_t> d_lens, d_nblk;
-    DevBuf<ScInst> d_inst;
-    DevBuf<ScCtl> d_ctl;
-    DevBuf<fp2> d_rounds, d_chals;
-    if ((st = d_labels.alloc(ctx, labels.size())) || (st = d_lens.alloc(ctx, lens.size())) ||
-        (st = d_nblk.alloc(ctx, nblk.size())) || (st = d_inst.alloc(ctx, count)) ||
-        (st = d_ctl.alloc(ctx, 1)) || (st = d_rounds.alloc(ctx, (uint64_t)nr * D)) ||
-        (st = d_chals.alloc(ctx, nr)))
-        return st;
-    TSR_HIP(hipMemcpyAsync(d_labels.ptr, labels.data(), labels.size(), hipMemcpyHostToDevice, s));
-    TSR_HIP(hipMemcpyAsync(d_lens.ptr, lens.data(), 4 * lens.size(), hipMemcpyHostToDevice, s));
-    TSR_HIP(hipMemcpyAsync(d_nblk.ptr, nblk.data(), 4 * nblk.size(), hipMemcpyHostToDevice, s));
-    TSR_HIP(hipMemcpyAsync(d_inst.ptr, hinst.data(), sizeof(ScInst) * count, hipMemcpyHostToDevice, s));
-    TSR_HIP(hipMemcpyAsync(d_ctl.ptr, &hctl, sizeof(ScCtl), hipMemcpyHostToDevice, s));
+    // final evaluations: each non-public table's single remaining entry (1 for a constant table)
+    std::vector<const void *> fptr;
+    std::vector<uint32_t> fbase, foff;
+    for (uint32_t i = 0; i < count; i++) {
+        for (size_t k = 0; k < insts[i].tabs.size(); k++) {
+            const GenTable &tab = insts[i].tabs[k];
+            if (tab.is_public) continue;
+            if (!tab.src) {
+                fptr.push_back(nullptr);
+                fbase.push_back(0);
+                foff.push_back(0);
+            } else if (insts[i].vars == 0) {
+                fptr.push_back(tab.src);
+                fbase.push_back(tab.base ? 1 : 0);
+                foff.push_back(tab.off);
+            } else {
+                // the last own round t = vars-1 wrote into a (t even) or b (t odd)
+                fptr.push_back(((insts[i].vars - 1) % 2 == 0) ? (const void *)bufa[i][k].ptr
+                                                               : (const void *)bufb[i][k].ptr);
+                fbase.push_back(0);
+                foff.push_back(0);
+            }
+        }
+    }
+    const size_t nf = fptr.size();
+
+    // Everything the rounds read from the host goes up in one copy and everything the host reads
+    // back comes down in one: one device block laid out as
+    //   [eq points | eq segments | labels | label lengths | nblk | instances | finals' sources]
+    //   [control block | rounds | challenges | finals]
+    // (the control block is in both copies: its initial state up, its final state down).
+    size_t at = 0;
+    auto place = [&](size_t bytes) {
+        at = (at + 255) & ~(size_t)255;
+        const size_t o = at;
+        at += bytes;
+        return o;
+    };
+    const size_t o_pts = place(16 * eq_pts.size()), o_segs = place(sizeof(EqSeg) * segs.size());
+    const size_t o_labels = place(labels.size()), o_lens = place(4 * lens.size());
+    const size_t o_nblk = place(4 * nblk.size()), o_inst = place(sizeof(ScInst) * count);
+    const size_t o_fptr = place(8 * nf), o_fbase = place(4 * nf), o_foff = place(4 * nf);
+    const size_t o_ctl = place(sizeof(ScCtl));
+    const size_t up_end = at;
+    const size_t o_rounds = place(16 * (size_t)nr * D), o_chals = place(16 * (size_t)nr);
+    const size_t o_fin = place(16 * std::max<size_t>(nf, 1));
+    DevBuf<uint8_t> blob;
+    if ((st = blob.alloc(ctx, at))) return st;
+    uint8_t *const dev = blob.ptr;
+    fp2 *const d_eqpt = (fp2 *)(dev + o_pts);
+    EqSeg *const d_segs = (EqSeg *)(dev + o_segs);
+    ScInst *const d_inst = (ScInst *)(dev + o_inst);
+    ScCtl *const d_ctl = (ScCtl *)(dev + o_ctl);
+
+    // the eq half tables of every own round (written by k_eq_segments after the upload)
+    DevBuf<fp2> d_eqtab;
+    if (!segs.empty()) {
+        if ((st = d_eqtab.alloc(ctx, seg_len))) return st;
+        size_t g = 0;
+        for (uint32_t i = 0; i < count; i++) {
+            if (!insts[i].eq_factor) continue;
+            hinst[i].eqf = 1;
+            hinst[i].eq_r = d_eqpt + pt_off[i];
+            for (uint32_t t = 0; t < insts[i].vars; t++, g += 2)
+                splits[i].push_back(EqSplit{d_eqtab.ptr + segs[g].off, d_eqtab.ptr + segs[g + 1].off, segs[g].n});
+        }
+    }
+    std::vector<uint8_t> up(up_end, 0);
+    auto put = [&](size_t o, const void *src, size_t bytes) {
+        if (bytes) memcpy(&up[o], src, bytes);
+    };
+    put(o_pts, eq_pts.data(), 16 * eq_pts.size());
+    put(o_segs, segs.data(), sizeof(EqSeg) * segs.size());
+    put(o_labels, labels.data(), labels.size());
+    put(o_lens, lens.data(), 4 * lens.size());
+    put(o_nblk, nblk.data(), 4 * nblk.size());
+    put(o_inst, hinst.data(), sizeof(ScInst) * count);
+    put(o_fptr, fptr.data(), 8 * nf);
+    put(o_fbase, fbase.data(), 4 * nf);
+    put(o_foff, foff.data(), 4 * nf);
+    put(o_ctl, &hctl, sizeof(ScCtl));
+    TSR_HIP(hipMemcpyAsync(dev, up.data(), up_end, hipMemcpyHostToDevice, s));
     const DevKeys keys = dev_keys(ctx);
+
+    if (!segs.empty()) {
+        uint32_t widest = 0;
+        for (auto &g : segs) widest = std::max(widest, g.n);
+        const unsigned gx = (unsigned)std::min<uint64_t>(((1ull << widest) + 255) / 256, 1024);
+        for (size_t g0 = 0; g0 < segs.size(); g0 += 65535) {
+            const unsigned ny = (unsigned)std::min<size_t>(65535, segs.size() - g0);
+            hipLaunchKernelGGL(k_eq_segments, dim3(gx, ny), dim3(256), 0, s, d_eqpt, d_segs + g0,
+                               d_eqtab.ptr);
+        }
+        if ((st = check_launch("eq segments"))) return st;
+    }
 
     // own round t binds from (t == 0: original, odd t: a, even t: b) into (even t: a, odd: b)
     auto tables_of = [&](uint32_t i, uint32_t t, ScTables &tt, uint32_t &base_mask) {
@@ -819,9 +872,9 @@ Status run_batch(Ctx *ctx, Transcript &tr, const std::string &p, uint32_t D,
     for (uint32_t j = 0; j < n_max; j++) {
         {
             KernelTimer timer(ctx, "k_sc_transcript", s);
-            launch_transcript_d(D, s, j, d_inst.ptr, count, d_nblk.ptr + (size_t)j * count,
-                                d_ctl.ptr, keys, d_labels.ptr, d_lens.ptr, d_rounds.ptr,
-                                d_chals.ptr);
+            launch_transcript_d(D, s, j, d_inst, count, (const uint32_t *)(dev + o_nblk) + (size_t)j * count,
+                                d_ctl, keys, dev + o_labels, (const uint32_t *)(dev + o_lens),
+                                (fp2 *)(dev + o_rounds), (fp2 *)(dev + o_chals));
         }
         KernelTimer timer(ctx, "k_sc_round", s);
         for (uint32_t i = 0; i < count; i++) {
@@ -836,63 +889,28 @@ Status run_batch(Ctx *ctx, Transcript &tr, const std::string &p, uint32_t D,
             const EqSplit eq = splits[i].empty() ? EqSplit() : splits[i][std::min<uint32_t>(t + 1, insts[i].vars - 1)];
             launch_round_d(D, insts[i].rel, dim3(blocks_for_pairs(pairs ? pairs : out_len)), s,
                            tt, (uint32_t)insts[i].tabs.size(), bm, t == 0, out_len, pairs,
-                           &d_ctl.ptr->r, lambda, partials[i].ptr, sterms[i], eq);
+                           &d_ctl->r, lambda, partials[i].ptr, sterms[i], eq);
         }
     }
     if ((st = check_launch("sumcheck rounds"))) return st;
-
-    // final evaluations: each non-public table's single remaining entry (1 for a constant table)
-    std::vector<const void *> fptr;
-    std::vector<uint32_t> fbase, foff;
-    for (uint32_t i = 0; i < count; i++) {
-        for (size_t k = 0; k < insts[i].tabs.size(); k++) {
-            const GenTable &tab = insts[i].tabs[k];
-            if (tab.is_public) continue;
-            if (!tab.src) {
-                fptr.push_back(nullptr);
-                fbase.push_back(0);
-                foff.push_back(0);
-            } else if (insts[i].vars == 0) {
-                fptr.push_back(tab.src);
-                fbase.push_back(tab.base ? 1 : 0);
-                foff.push_back(tab.off);
-            } else {
-                // the last own round t = vars-1 wrote into a (t even) or b (t odd)
-                fptr.push_back(((insts[i].vars - 1) % 2 == 0) ? (const void *)bufa[i][k].ptr
-                                                               : (const void *)bufb[i][k].ptr);
-                fbase.push_back(0);
-                foff.push_back(0);
-            }
-        }
-    }
-    out.finals.resize(fptr.size());
-    DevBuf<uint64_t> d_ptrs;
-    DevBuf<uint32_t> d_base, d_off;
-    DevBuf<fp2> d_fin;
-    if (!fptr.empty()) {
-        if ((st = d_ptrs.alloc(ctx, fptr.size())) || (st = d_base.alloc(ctx, fbase.size())) ||
-            (st = d_off.alloc(ctx, foff.size())) || (st = d_fin.alloc(ctx, fptr.size())))
-            return st;
-        TSR_HIP(hipMemcpyAsync(d_ptrs.ptr, fptr.data(), 8 * fptr.size(), hipMemcpyHostToDevice, s));
-        TSR_HIP(hipMemcpyAsync(d_base.ptr, fbase.data(), 4 * fbase.size(), hipMemcpyHostToDevice, s));
-        TSR_HIP(hipMemcpyAsync(d_off.ptr, foff.data(), 4 * foff.size(), hipMemcpyHostToDevice, s));
-        hipLaunchKernelGGL(k_sc_gather, dim3((unsigned)((fptr.size() + 63) / 64)), dim3(64), 0, s,
-                           (const void *const *)d_ptrs.ptr, d_base.ptr, d_off.ptr,
-                           (uint32_t)fptr.size(), d_fin.ptr);
+    if (nf) {
+        hipLaunchKernelGGL(k_sc_gather, dim3((unsigned)((nf + 63) / 64)), dim3(64), 0, s,
+                           (const void *const *)(dev + o_fptr), (const uint32_t *)(dev + o_fbase),
+                           (const uint32_t *)(dev + o_foff), (uint32_t)nf, (fp2 *)(dev + o_fin));
         if ((st = check_launch("sumcheck finals"))) return st;
-        TSR_HIP(hipMemcpyAsync(out.finals.data(), d_fin.ptr, 16 * fptr.size(),
-                               hipMemcpyDeviceToHost, s));
     }
+    std::vector<uint8_t> down(at - o_ctl);
+    TSR_HIP(hipMemcpyAsync(down.data(), dev + o_ctl, down.size(), hipMemcpyDeviceToHost, s));
+    TSR_HIP(hipStreamSynchronize(s));
+    memcpy(&hctl, &down[0], sizeof(ScCtl));
     out.rounds.resize((size_t)n_max * D);
     out.challenges.resize(n_max);
+    out.finals.resize(nf);
     if (n_max) {
-        TSR_HIP(hipMemcpyAsync(out.rounds.data(), d_rounds.ptr, 16 * (size_t)n_max * D,
-                               hipMemcpyDeviceToHost, s));
-        TSR_HIP(hipMemcpyAsync(out.challenges.data(), d_chals.ptr, 16 * (size_t)n_max,
-                               hipMemcpyDeviceToHost, s));
+        memcpy(out.rounds.data(), &down[o_rounds - o_ctl], 16 * (size_t)n_max * D);
+        memcpy(out.challenges.data(), &down[o_chals - o_ctl], 16 * (size_t)n_max);
     }
-    TSR_HIP(hipMemcpyAsync(&hctl, d_ctl.ptr, sizeof(ScCtl), hipMemcpyDeviceToHost, s));
-    TSR_HIP(hipStreamSynchronize(s));
+    if (nf) memcpy(out.finals.data(), &down[o_fin - o_ctl], 16 * nf);
     store_state_words(hctl.t.state, tr.state);
     tr.counter = hctl.t.counter;
     tr.absorb_fp2s((p + "/evals").c_str(), (const uint64_t *)out.finals.data(), out.finals.size());

@@ -21,6 +21,7 @@ Also on the same JSON line:
                  plain per-request call at the same arrival times; results compared
 
 python bench.py [--steps K] [--warmup W] [--gpus 1] [--no-cpu-baseline] [--only headline]
+python bench.py --only refshapes   (the reference's three H100 unit-cost workloads, ~1 minute)
 """
 import argparse
 import ctypes
@@ -472,6 +473,85 @@ def cpu_baseline():
     }
 
 
+# The reference's published H100 unit costs for the same three workloads (1x H100 SXM 80 GB,
+# docs/perf/v0.1/cc-h100.md:9,13, pi-h100.md:9-11,19, l-h100.md:18-21).
+H100_PS = {"encode_and_chains": 134.8, "forest": 84.6, "logup": 297.0}
+
+
+def run_refshapes(reps=3):
+    """The reference's three unit-cost workloads on this back end, per unit as the reference
+    quotes them: (i) encode at rate 1/2 with 200 mask coefficients + BLAKE3 column chains of
+    B = 32 rows, rows of k = 2^22 (8 row blocks, 2^30 committed elements; the rows are drawn on
+    the device each block and that time is subtracted); (ii) a product forest of 2^16 trees of
+    2^14 leaves (2^30 leaves): build + every level sumcheck; (iii) LogUp of 2^30 lookups into a
+    2^24-row table with values: both fraction trees + every level sumcheck (multiplicities
+    counted beforehand). Best of `reps` after a warm-up."""
+    import tessera_amd as T
+    from tessera_amd.lib import lib
+    from tessera_amd.logup import Forest, LogUpInstance, multiplicities, prove_logup, prove_trees
+    from tessera_amd.pcs import ColumnChains, RowEncoder
+    from tessera_amd.transcript import Transcript
+    dev = T.Device(0)
+
+    def best(fn):
+        fn()
+        dev.synchronize()
+        out = []
+        for _ in range(reps):
+            t0 = time.perf_counter()
+            fn()
+            dev.synchronize()
+            out.append(time.perf_counter() - t0)
+        return min(out)
+
+    res = {}
+    log_k, masks_n, B, blocks = 22, 200, 32, 8
+    k, n = 1 << log_k, 1 << (log_k + 1)
+    rows, masks, cw = dev.buffer(B * k), dev.buffer(B * masks_n), dev.buffer(B * n)
+    enc = RowEncoder(dev, k, 1, masks_n)
+
+    def cc(work=True):
+        ch = ColumnChains(dev, bytes([7] * 32), n, B) if work else None
+        for i in range(blocks):
+            rows.fill_uniform(SEED + 1000 + i)
+            masks.fill_uniform(SEED + 2000 + i)
+            if work:
+                enc.encode_rows(rows, masks, out=cw)
+                ch.absorb_block(cw)
+        if ch:  # the chain digests stay on the device
+            dev.synchronize()
+            lib().tsr_chains_free(ch.handle)
+            ch.handle = None
+    t_all, t_gen = best(cc), best(lambda: cc(False))
+    elems = blocks * B * k
+    res["encode_and_chains"] = {"ps_per_elem": round(1e12 * (t_all - t_gen) / elems, 2),
+                                "elements": elems}
+    rows.free(); masks.free(); cw.free()
+
+    leaves = dev.buffer(1 << 30).fill_uniform(SEED + 77)
+    t = best(lambda: prove_trees(dev, Transcript(b"forest"), [Forest(leaves, 14)]))
+    res["forest"] = {"ps_per_leaf": round(1e12 * t / (1 << 30), 2), "trees": 1 << 16, "leaves_per_tree": 1 << 14}
+    leaves.free()
+
+    L, R = 30, 24
+    rng = np.random.default_rng(SEED)
+    vals = rng.integers(0, 1 << 62, 1 << R, dtype=np.uint64)
+    d_keys, d_vals = dev.upload(np.arange(1 << R, dtype=np.uint64)), dev.upload(vals)
+    idx = rng.integers(0, 1 << R, 1 << L, dtype=np.uint64)
+    d_idx = dev.upload(idx)  # the keys are the row indices
+    d_lv = dev.upload(vals[idx])
+    del idx
+    m = dev.upload(multiplicities(dev, 1 << R, d_idx))
+    inst = [LogUpInstance(d_keys, d_vals, d_idx, d_lv, m)]
+    t = best(lambda: prove_logup(dev, Transcript(b"logup"), inst))
+    res["logup"] = {"ps_per_lookup": round(1e12 * t / (1 << L), 2), "lookups": 1 << L, "rows": 1 << R}
+    dev.close()
+    for key, unit in (("encode_and_chains", "ps_per_elem"), ("forest", "ps_per_leaf"), ("logup", "ps_per_lookup")):
+        res[key]["h100_ps"] = H100_PS[key]
+        res[key]["speedup_vs_h100"] = round(H100_PS[key] / res[key][unit], 3)
+    return res
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -483,10 +563,11 @@ def main():
     ap.add_argument("--traffic-command", default="", help="the profiling command, for the summary")
     ap.add_argument("--trace-summary", nargs=3, metavar=("DB", "TITLE", "COMMAND"),
                     help="print a markdown summary of a rocprofv3 kernel-trace database")
-    ap.add_argument("--only", choices=["headline", "copy", "hostfed"], default=None,
+    ap.add_argument("--only", choices=["headline", "copy", "hostfed", "refshapes"], default=None,
                     help="time the headline commit only; run the device copy that calibrates "
-                         "the HBM byte counters (4 copies of 2 GiB); or time the headline commit "
-                         "from host rows (the PCIe-inclusive rate)")
+                         "the HBM byte counters (4 copies of 2 GiB); time the headline commit "
+                         "from host rows (the PCIe-inclusive rate); or time the reference's three "
+                         "H100 unit-cost workloads")
     a = ap.parse_args()
     if a.gpus != 1:
         print("bench.py measures one GPU: this build runs on a single MI355X; "
@@ -506,6 +587,9 @@ def main():
         return
     if a.only == "hostfed":
         run_hostfed()
+        return
+    if a.only == "refshapes":
+        print(json.dumps({"workload": "the reference's H100 unit-cost workloads", **run_refshapes()}), flush=True)
         return
     log("headline: 256 x 2^20 commit + 64 openings")
     sec, kernels, root, n_sib = run_headline(a.steps, a.warmup)

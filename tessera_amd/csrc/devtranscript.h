@@ -96,14 +96,14 @@ __device__ __forceinline__ void quad_compress(uint32_t &a, uint32_t &b, const ui
 }
 
 // H(key, buf[0..len)) for len < 1024: one chunk of derive-key material blocks, the last one
-// ROOT; the bytes of buf from len to the next multiple of 64 are zero. The 8 output words go to
-// out (LDS).
-__device__ inline void wave_hash_chunk(const uint32_t key[8], const uint32_t *buf, uint32_t len,
+// ROOT; the bytes of buf from len to the next multiple of 64 are zero. key (8 words), buf and
+// out (8 words) are in LDS.
+__device__ inline void wave_hash_chunk(const uint32_t *key, const uint32_t *buf, uint32_t len,
                                        uint32_t *out) {
     const uint32_t q = threadIdx.x & 3;
     const uint32_t nb = len == 0 ? 1 : (len + 63) / 64;
-    uint32_t a = q == 0 ? key[0] : q == 1 ? key[1] : q == 2 ? key[2] : key[3];
-    uint32_t b = q == 0 ? key[4] : q == 1 ? key[5] : q == 2 ? key[6] : key[7];
+    uint32_t a = key[q];
+    uint32_t b = key[4 + q];
     for (uint32_t i = 0; i < nb; i++) {
         uint32_t flags = b3::DERIVE_KEY_MATERIAL | (i == 0 ? b3::CHUNK_START : 0u);
         uint32_t blen = 64;
@@ -147,9 +147,14 @@ __device__ inline uint32_t wave_put(uint32_t *buf, const uint32_t *st, const uin
     return len;
 }
 
+// The three transcript keys in LDS, as wave_hash_chunk reads them.
+struct WaveKeys {
+    uint32_t absorb[8], squeeze[8], ratchet[8];
+};
+
 // absorb(label, Vec<Fp2>) on the state st (LDS, updated): TCE = u64LE(count) then c0 || c1 of
 // each element (count <= 11; vals in LDS).
-__device__ inline void wave_absorb_fp2s(uint32_t *st, const DevKeys &k, const uint8_t *label,
+__device__ inline void wave_absorb_fp2s(uint32_t *st, const WaveKeys &k, const uint8_t *label,
                                         uint32_t label_len, const gl::fp2 *vals, uint32_t count,
                                         uint32_t *buf) {
     const uint32_t len = wave_put(buf, st, label, label_len, count, (const uint8_t *)vals, 16 * count);
@@ -159,7 +164,7 @@ __device__ inline void wave_absorb_fp2s(uint32_t *st, const DevKeys &k, const ui
 // challenge_fp2: squeeze (state || Ascii(label) || u64LE(counter), then the ratchet
 // state || out), taking the first two LE words pairs of the output, again unless both are
 // below p.
-__device__ inline gl::fp2 wave_challenge_fp2(uint32_t *st, uint64_t &counter, const DevKeys &k,
+__device__ inline gl::fp2 wave_challenge_fp2(uint32_t *st, uint64_t &counter, const WaveKeys &k,
                                              const uint8_t *label, uint32_t label_len,
                                              uint32_t *buf, uint32_t *out) {
     for (;;) {

@@ -327,17 +327,27 @@ k_sc_round_eq(ScTables t, uint32_t base_mask, bool first, uint64_t out_len, uint
     }
 }
 
-// Per-instance facts the transcript kernel needs.
-struct ScInst {
-    uint32_t vars, pad;
+// The facts of one instance in one round (a table of rounds x instances, built on the host).
+struct ScRoundInst {
+    uint32_t nblk;        // blocks of partials this round's message reads
+    uint32_t pad;         // padding rounds in front of the instance's own rounds
+    uint32_t eqf;         // 1: eq(point, x) is a factor of the relation (see k_sc_round_eq)
     uint32_t wsum_given;  // 1: wsum set by the host (claims, or instances on no variable)
-    uint32_t eqf;         // 1: eq(eq_r, x) is a factor of the relation (see k_sc_round)
-    fp2 weight;    // gamma^i
-    fp2 wsum;      // gamma^i S_i
-    fp2 *partials; // (D+1) per block of its last k_sc_round
-    const fp2 *eq_r;  // the eq point, one coordinate per own round
-    fp2 eq_k;         // prod over bound rounds t of eq(r_t, c_t)
+    fp2 weight;           // gamma^i
+    fp2 rt;               // eqf and an own round t: coordinate t of the eq point
 };
+
+// The per-instance state the rounds update.
+struct ScInstState {
+    fp2 wsum;  // gamma^i S_i
+    fp2 eq_k;  // eqf: prod over the bound own rounds t of eq(point_t, c_t)
+};
+
+// Batches of up to SC_STAGED instances have their facts staged in LDS by the transcript kernel.
+constexpr uint32_t SC_STAGED = 64;
+// The transcript kernel's LDS: its static arrays and count x (D+1) sums (one workgroup).
+constexpr size_t SC_TRANSCRIPT_LDS = 64 * 1024;
+constexpr size_t SC_TRANSCRIPT_STATIC = 8 * 1024;
 
 struct ScCtl {
     DevTranscript t;
@@ -345,52 +355,61 @@ struct ScCtl {
 };
 
 // Round j: reduce the partial sums, form the message, absorb, squeeze r_j.
-// nblk[i]: the number of blocks whose partials instance i left for this round.
-// The per-instance steps run on one thread per instance and the transcript on wave 0 reads
-// only LDS and registers: the labels and the transcript state are loaded at the start by
-// independent loads, so the hashes wait on no global memory round trip.
+// Instance i's partials are partials[(i * SC_MAX_BLOCKS + b) * (D + 1) + slot] for
+// b < ri[i].nblk <= nblk_max. Every global load the kernel makes has an address known at launch
+// (the round's instance table, the instance states, the partials up to nblk_max, the labels, the
+// transcript state), so all of them are in flight at once; the transcript runs on wave 0 from LDS
+// and registers.
 template <int D>
 __global__ void __launch_bounds__(SC_THREADS)
-k_sc_transcript(uint32_t j, ScInst *__restrict__ inst, uint32_t count,
-                const uint32_t *__restrict__ nblk, ScCtl *__restrict__ ctl, DevKeys keys,
-                const uint8_t *__restrict__ labels, const uint32_t *__restrict__ label_lens,
-                fp2 *__restrict__ rounds_out, fp2 *__restrict__ chals_out) {
+k_sc_transcript(uint32_t j, const ScRoundInst *__restrict__ ri, ScInstState *__restrict__ ist,
+                uint32_t count, uint32_t nblk_max, const fp2 *__restrict__ partials,
+                ScCtl *__restrict__ ctl, DevKeys keys, const uint8_t *__restrict__ labels,
+                const uint32_t *__restrict__ label_lens, fp2 *__restrict__ rounds_out,
+                fp2 *__restrict__ chals_out) {
     extern __shared__ fp2 sums[];  // count x (D+1)
+    __shared__ ScRoundInst sri[SC_STAGED];
+    __shared__ ScInstState sis[SC_STAGED];
     __shared__ uint32_t hbuf[DEV_BUF_WORDS];
     __shared__ uint32_t lab[2 * LABEL_SLOT / 4];  // the labels of round j (absorb, then squeeze)
     __shared__ uint32_t st[8], outw[8];
     __shared__ fp2 msg[D];
     __shared__ fp2 chal;
+    __shared__ WaveKeys wk;
     const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    uint64_t counter = 0;
-    uint32_t len_absorb = 0, len_squeeze = 0;
-    if (wave == 0) {
-        counter = ctl->t.counter;
-        len_absorb = label_lens[2 * j];
-        len_squeeze = label_lens[2 * j + 1];
-        if (lane < 8) st[lane] = ctl->t.state[lane];
-    }
-    if (threadIdx.x < 2 * LABEL_SLOT / 4)
-        lab[threadIdx.x] = ((const uint32_t *)(labels + (size_t)2 * j * LABEL_SLOT))[threadIdx.x];
-    // each wave reduces slots x = wave, wave + 4, ...: lanes stride over the partial blocks with
-    // four loads in flight, then a shuffle tree adds the 64 lane sums (field addition: any order
-    // gives the same sum)
+    // Loads into registers first, unconditionally (count >= 1; addresses clamped), LDS stores
+    // after the partials' loads are issued: no wave waits on memory before the reduction.
+    const uint32_t ci = threadIdx.x < count ? threadIdx.x : count - 1;
+    const ScRoundInst my = ri[ci];
+    const ScInstState mys = ist[ci];
+    // vector loads (lane-indexed) only: a scalar load from global memory would make the first
+    // wait on the kernel arguments (scalar loads complete out of order) wait for it too
+    const uint32_t t_word = ((const uint32_t *)&ctl->t)[lane % 10];  // state[0..8), counter
+    const uint32_t len_word = label_lens[2 * j + (lane & 1)];
+    const uint32_t key_word = (&keys.absorb[0])[lane < 24 ? lane : 23];
+    const uint32_t lab_word = ((const uint32_t *)(labels + (size_t)2 * j * LABEL_SLOT))[threadIdx.x & 31];
+    // each wave reduces slots x = wave, wave + 4, ...: lane l loads blocks l, l + 64, ... below
+    // nblk_max (all at once), keeps those below the instance's own count, then a shuffle tree
+    // adds the 64 lane sums (field addition: any order gives the same sum)
+    constexpr int PER_LANE = SC_MAX_BLOCKS / 64;
     for (uint32_t x = wave; x < count * (D + 1); x += SC_THREADS / 64) {
         const uint32_t i = x / (D + 1), z = x % (D + 1);
-        const fp2 *p = inst[i].partials;
-        const uint32_t nb = nblk[i];
-        fp2 s0{0, 0}, s1{0, 0}, s2{0, 0}, s3{0, 0};
-        uint32_t b = lane;
-        for (; b + 192 < nb; b += 256) {
-            const fp2 v0 = p[(uint64_t)b * (D + 1) + z], v1 = p[(uint64_t)(b + 64) * (D + 1) + z];
-            const fp2 v2 = p[(uint64_t)(b + 128) * (D + 1) + z], v3 = p[(uint64_t)(b + 192) * (D + 1) + z];
-            s0 = gl::add(s0, v0);
-            s1 = gl::add(s1, v1);
-            s2 = gl::add(s2, v2);
-            s3 = gl::add(s3, v3);
+        const fp2 *p = partials + (uint64_t)i * SC_MAX_BLOCKS * (D + 1) + z;
+        const uint32_t nb = ri[i].nblk;
+        fp2 v[PER_LANE];
+#pragma unroll
+        for (int k = 0; k < PER_LANE; k++) {
+            const uint32_t b = lane + 64 * k;
+            v[k] = b < nblk_max ? p[(uint64_t)b * (D + 1)] : fp2{0, 0};
         }
-        for (; b < nb; b += 64) s0 = gl::add(s0, p[(uint64_t)b * (D + 1) + z]);
-        fp2 s = gl::add(gl::add(s0, s1), gl::add(s2, s3));
+#pragma unroll
+        for (int k = 0; k < PER_LANE; k++)
+            if (lane + 64 * k >= nb) v[k] = fp2{0, 0};
+#pragma unroll
+        for (int w = PER_LANE / 2; w >= 1; w /= 2)
+#pragma unroll
+            for (int k = 0; k < w; k++) v[k] = gl::add(v[k], v[k + w]);
+        fp2 s = v[0];
 #pragma unroll
         for (int off = 32; off > 0; off >>= 1) {
             fp2 o;
@@ -400,33 +419,51 @@ k_sc_transcript(uint32_t j, ScInst *__restrict__ inst, uint32_t count,
         }
         if (lane == 0) sums[x] = s;
     }
+    if (wave == 0 && lane < 8) st[lane] = t_word;
+    if (wave == 0 && lane < 24) (&wk.absorb[0])[lane] = key_word;
+    if (threadIdx.x < 2 * LABEL_SLOT / 4) lab[threadIdx.x] = lab_word;
+    if (count <= SC_STAGED && threadIdx.x < count) {
+        sri[threadIdx.x] = my;
+        sis[threadIdx.x] = mys;
+    }
     __syncthreads();
-    // each instance's share of the message, in place of its sums (slots 0..D-1)
-    for (uint32_t i = threadIdx.x; i < count; i += SC_THREADS) {
-        const ScInst in = inst[i];
-        fp2 *s = sums + (uint64_t)i * (D + 1);
-        if (in.eqf && j >= in.pad) {
-            // the eq line of this round: K ((1 - r_t)(1 - z) + r_t z) at the nodes 0, 2..D, 1
-            const fp2 rt = in.eq_r[j - in.pad];
-            const fp2 l0 = gl::sub(fp2{1, 0}, rt), step = gl::sub(rt, l0);
-            fp2 lz = gl::add(l0, gl::add(step, step));  // z = 2
-            s[0] = gl::mul(s[0], gl::mul(in.eq_k, l0));
-            for (int z = 1; z < D; z++) {
-                s[z] = gl::mul(s[z], gl::mul(in.eq_k, lz));
-                lz = gl::add(lz, step);
-            }
-            s[D] = gl::mul(s[D], gl::mul(in.eq_k, rt));
-        }
-        fp2 wsum = in.wsum;
-        if (j == 0 && in.pad > 0 && !in.wsum_given) {
-            wsum = gl::mul(in.weight, gl::add(s[0], s[D]));  // S_i = p(0) + p(1)
-            inst[i].wsum = wsum;
-        }
+    // S_i of the padded instances that have no claim: p(0) + p(1) of their first sums
+    auto first_sum = [&](uint32_t i, const ScRoundInst &in, ScInstState &is) {
+        if (j != 0 || in.pad == 0 || in.wsum_given) return;
+        const fp2 *s = sums + (uint64_t)i * (D + 1);
+        is.wsum = gl::mul(in.weight, gl::add(s[0], s[D]));
+        ist[i].wsum = is.wsum;
+    };
+    // each instance's share of the message, in place of its sums at the nodes 0, 2, ..., D (the
+    // message slots): weight (times K L(node) for an eq factor) times the sum, or its padding
+    // share 2^(pad-j-1) gamma^i S_i
+    auto share = [&](uint32_t i, uint32_t z, const ScRoundInst &in, const ScInstState &is) {
+        fp2 *s = sums + (uint64_t)i * (D + 1) + z;
         if (j < in.pad) {
-            const fp2 v = gl::mul_base(wsum, 1ull << (in.pad - j - 1));
-            for (int z = 0; z < D; z++) s[z] = v;
-        } else {
-            for (int z = 0; z < D; z++) s[z] = gl::mul(in.weight, s[z]);
+            *s = gl::mul_base(is.wsum, 1ull << (in.pad - j - 1));
+            return;
+        }
+        fp2 f = in.weight;
+        if (in.eqf) {
+            // the eq line of this round: K ((1 - r_t)(1 - x) + r_t x) at x = node
+            const fp2 l0 = gl::sub(fp2{1, 0}, in.rt), step = gl::sub(in.rt, l0);
+            const uint32_t node = z == 0 ? 0 : z + 1;
+            f = gl::mul(f, gl::mul(is.eq_k, gl::add(l0, gl::mul_base(step, node))));
+        }
+        *s = gl::mul(*s, f);
+    };
+    if (count <= SC_STAGED) {
+        // the facts staged in LDS; one thread per instance and slot (each reads its own slot)
+        if (threadIdx.x < count) first_sum(threadIdx.x, sri[threadIdx.x], sis[threadIdx.x]);
+        __syncthreads();
+        for (uint32_t x = threadIdx.x; x < count * D; x += SC_THREADS)
+            share(x / D, x % D, sri[x / D], sis[x / D]);
+    } else {
+        for (uint32_t i = threadIdx.x; i < count; i += SC_THREADS) {
+            const ScRoundInst in = ri[i];
+            ScInstState is = ist[i];
+            first_sum(i, in, is);
+            for (uint32_t z = 0; z < D; z++) share(i, z, in, is);
         }
     }
     __syncthreads();
@@ -438,9 +475,13 @@ k_sc_transcript(uint32_t j, ScInst *__restrict__ inst, uint32_t count,
             rounds_out[(uint64_t)j * D + lane] = m;
         }
         wave_sync();
+        uint64_t counter = ((uint64_t)(uint32_t)__builtin_amdgcn_readlane((int)t_word, 9) << 32) |
+                           (uint32_t)__builtin_amdgcn_readlane((int)t_word, 8);
+        const uint32_t len_absorb = (uint32_t)__builtin_amdgcn_readlane((int)len_word, 0);
+        const uint32_t len_squeeze = (uint32_t)__builtin_amdgcn_readlane((int)len_word, 1);
         const uint8_t *lb = (const uint8_t *)lab;
-        wave_absorb_fp2s(st, keys, lb, len_absorb, msg, D, hbuf);
-        const fp2 r = wave_challenge_fp2(st, counter, keys, lb + LABEL_SLOT, len_squeeze, hbuf, outw);
+        wave_absorb_fp2s(st, wk, lb, len_absorb, msg, D, hbuf);
+        const fp2 r = wave_challenge_fp2(st, counter, wk, lb + LABEL_SLOT, len_squeeze, hbuf, outw);
         if (lane < 8) ctl->t.state[lane] = st[lane];
         if (lane == 0) {
             ctl->t.counter = counter;
@@ -451,14 +492,14 @@ k_sc_transcript(uint32_t j, ScInst *__restrict__ inst, uint32_t count,
     }
     __syncthreads();
     const fp2 r = chal;
-    for (uint32_t i = threadIdx.x; i < count; i += SC_THREADS) {
-        ScInst &in = inst[i];
-        if (!in.eqf || j < in.pad) continue;
-        // K *= eq(r_t, c) = r_t c + (1 - r_t)(1 - c)
-        const fp2 rt = in.eq_r[j - in.pad];
-        const fp2 e = gl::add(gl::mul(rt, r), gl::mul(gl::sub(fp2{1, 0}, rt), gl::sub(fp2{1, 0}, r)));
-        in.eq_k = gl::mul(in.eq_k, e);
-    }
+    // K *= eq(r_t, c) = r_t c + (1 - r_t)(1 - c)
+    auto fold = [&](uint32_t i, const ScRoundInst &in, const fp2 &k) {
+        if (!in.eqf || j < in.pad) return;
+        const fp2 e = gl::add(gl::mul(in.rt, r), gl::mul(gl::sub(fp2{1, 0}, in.rt), gl::sub(fp2{1, 0}, r)));
+        ist[i].eq_k = gl::mul(k, e);
+    };
+    if (threadIdx.x < count) fold(threadIdx.x, my, mys.eq_k);
+    for (uint32_t i = threadIdx.x + SC_THREADS; i < count; i += SC_THREADS) fold(i, ri[i], ist[i].eq_k);
 }
 
 // Gathers each final evaluation: entry off[e] of the buffer at ptrs[e] (base or Fp2).
@@ -549,16 +590,16 @@ static void launch_round_d(uint32_t d, int rel, dim3 grid, hipStream_t s, const 
     }
 }
 
-static void launch_transcript_d(uint32_t d, hipStream_t s, uint32_t j, ScInst *inst,
-                                uint32_t count, const uint32_t *nblk, ScCtl *ctl,
-                                const DevKeys &keys, const uint8_t *labels,
-                                const uint32_t *lens, fp2 *rounds, fp2 *chals) {
+static void launch_transcript_d(uint32_t d, hipStream_t s, uint32_t j, const ScRoundInst *ri,
+                                ScInstState *ist, uint32_t count, uint32_t nblk_max,
+                                const fp2 *partials, ScCtl *ctl, const DevKeys &keys,
+                                const uint8_t *labels, const uint32_t *lens, fp2 *rounds, fp2 *chals) {
     size_t shm = sizeof(fp2) * count * (d + 1);
     switch (d) {
 #define CASE(D)                                                                               \
     case D:                                                                                   \
-        hipLaunchKernelGGL(k_sc_transcript<D>, dim3(1), dim3(SC_THREADS), shm, s, j, inst,     \
-                           count, nblk, ctl, keys, labels, lens, rounds, chals);              \
+        hipLaunchKernelGGL(k_sc_transcript<D>, dim3(1), dim3(SC_THREADS), shm, s, j, ri, ist,  \
+                           count, nblk_max, partials, ctl, keys, labels, lens, rounds, chals); \
         break;
         CASE(1) CASE(2) CASE(3) CASE(4) CASE(5) CASE(6) CASE(7) CASE(8) CASE(9) CASE(10)
 #undef CASE
@@ -596,18 +637,22 @@ Status run_batch(Ctx *ctx, Transcript &tr, const std::string &p, uint32_t D,
     uint32_t n_max = 0;
     for (auto &in : insts) n_max = std::max(n_max, in.vars);
     out.gamma = gamma;
+    if (count == 0) return Status::fail(TSR_BAD_SHAPE, "NoInstances");
+    if (SC_TRANSCRIPT_STATIC + sizeof(fp2) * count * (D + 1) > SC_TRANSCRIPT_LDS)
+        return Status::fail(TSR_DECLINED, "a batch of " + std::to_string(count) +
+                                              " instances at degree " + std::to_string(D) +
+                                              " exceeds the transcript kernel's LDS");
 
-    std::vector<ScInst> hinst(count);
+    std::vector<uint32_t> pad(count), wgiven(count);
+    std::vector<fp2> weight(count);
+    std::vector<ScInstState> hstate(count);
     fp2 w{1, 0};
     for (uint32_t i = 0; i < count; i++) {
-        hinst[i].vars = insts[i].vars;
-        hinst[i].pad = n_max - insts[i].vars;
-        hinst[i].weight = w;
-        hinst[i].wsum_given = insts[i].claim_given ? 1 : 0;
-        hinst[i].wsum = insts[i].claim_given ? gl::mul(w, insts[i].claim) : fp2{0, 0};
-        hinst[i].eqf = 0;
-        hinst[i].eq_r = nullptr;
-        hinst[i].eq_k = fp2{1, 0};
+        pad[i] = n_max - insts[i].vars;
+        weight[i] = w;
+        wgiven[i] = insts[i].claim_given ? 1 : 0;
+        hstate[i].wsum = insts[i].claim_given ? gl::mul(w, insts[i].claim) : fp2{0, 0};
+        hstate[i].eq_k = fp2{1, 0};
         w = gl::mul(w, gamma);
     }
     Status st;
@@ -645,7 +690,10 @@ Status run_batch(Ctx *ctx, Transcript &tr, const std::string &p, uint32_t D,
     }
     // working buffers: per table a (2^{n-1}) and b (2^{n-2}) Fp2
     std::vector<std::vector<DevBuf<fp2>>> bufa(count), bufb(count);
-    std::vector<DevBuf<fp2>> partials(count);
+    // instance i's partials: SC_MAX_BLOCKS x (D+1) from partials + i SC_MAX_BLOCKS (D+1)
+    DevBuf<fp2> partials;
+    const uint64_t pstride = (uint64_t)SC_MAX_BLOCKS * (D + 1);
+    if ((st = partials.alloc(ctx, pstride * std::max(count, 1u)))) return st;
     for (uint32_t i = 0; i < count; i++) {
         const uint32_t v = insts[i].vars;
         const uint64_t len = 1ull << v;
@@ -656,8 +704,6 @@ Status run_batch(Ctx *ctx, Transcript &tr, const std::string &p, uint32_t D,
             if ((st = bufa[i][k].alloc(ctx, len / 2))) return st;
             if (v >= 2 && (st = bufb[i][k].alloc(ctx, len / 4))) return st;
         }
-        if ((st = partials[i].alloc(ctx, (uint64_t)SC_MAX_BLOCKS * (D + 1)))) return st;
-        hinst[i].partials = partials[i].ptr;
     }
 
     // the terms of REL_TERMS instances, as the round kernel reads them
@@ -679,7 +725,7 @@ Status run_batch(Ctx *ctx, Transcript &tr, const std::string &p, uint32_t D,
 
     // S_i of instances on no variable (no claim given): the relation at the single entries
     for (uint32_t i = 0; i < count; i++) {
-        if (insts[i].vars != 0 || hinst[i].pad == 0 || insts[i].claim_given) continue;
+        if (insts[i].vars != 0 || pad[i] == 0 || insts[i].claim_given) continue;
         std::vector<fp2> single;
         for (auto &tab : insts[i].tabs) {
             fp2 v{1, 0};
@@ -703,8 +749,8 @@ Status run_batch(Ctx *ctx, Transcript &tr, const std::string &p, uint32_t D,
         } else {
             for (auto &v : single) rel = gl::mul(rel, v);
         }
-        hinst[i].wsum = gl::mul(hinst[i].weight, rel);
-        hinst[i].wsum_given = 1;
+        hstate[i].wsum = gl::mul(weight[i], rel);
+        wgiven[i] = 1;
     }
 
     // labels of every round, the control block, per-round block counts
@@ -720,16 +766,24 @@ Status run_batch(Ctx *ctx, Transcript &tr, const std::string &p, uint32_t D,
         lens[2 * j] = (uint32_t)a.size();
         lens[2 * j + 1] = (uint32_t)b.size();
     }
-    // nblk[j][i]: blocks of instance i's partials that round j's message reads (round 0 also
-    // reads a padded instance's first sums for S_i)
-    std::vector<uint32_t> nblk((size_t)nr * count, 0);
-    for (uint32_t i = 0; i < count; i++) {
-        const uint32_t v = insts[i].vars, pad = hinst[i].pad;
-        if (v == 0) continue;
-        for (uint32_t j = 0; j < n_max; j++) {
-            if (j < pad && j != 0) continue;
-            uint32_t t = j < pad ? 0 : j - pad;
-            nblk[(size_t)j * count + i] = blocks_for_pairs(1ull << (v - t - 1));
+    // the round table: per round j and instance i the blocks of partials round j's message
+    // reads (round 0 also reads a padded instance's first sums for S_i) and the eq coordinate
+    std::vector<ScRoundInst> rtab((size_t)nr * count);
+    std::vector<uint32_t> nblk_max(nr, 0);
+    for (uint32_t j = 0; j < nr; j++) {
+        for (uint32_t i = 0; i < count; i++) {
+            ScRoundInst &e = rtab[(size_t)j * count + i];
+            const uint32_t v = insts[i].vars;
+            e.nblk = 0;
+            e.pad = pad[i];
+            e.eqf = insts[i].eq_factor ? 1 : 0;
+            e.wsum_given = wgiven[i];
+            e.weight = weight[i];
+            e.rt = (e.eqf && j >= pad[i] && j < n_max) ? insts[i].eq_point[j - pad[i]] : fp2{0, 0};
+            if (v == 0 || j >= n_max || (j < pad[i] && j != 0)) continue;
+            const uint32_t t = j < pad[i] ? 0 : j - pad[i];
+            e.nblk = blocks_for_pairs(1ull << (v - t - 1));
+            nblk_max[j] = std::max(nblk_max[j], e.nblk);
         }
     }
     ScCtl hctl;
@@ -777,7 +831,7 @@ Status run_batch(Ctx *ctx, Transcript &tr, const std::string &p, uint32_t D,
     };
     const size_t o_pts = place(16 * eq_pts.size()), o_segs = place(sizeof(EqSeg) * segs.size());
     const size_t o_labels = place(labels.size()), o_lens = place(4 * lens.size());
-    const size_t o_nblk = place(4 * nblk.size()), o_inst = place(sizeof(ScInst) * count);
+    const size_t o_rtab = place(sizeof(ScRoundInst) * rtab.size()), o_state = place(sizeof(ScInstState) * count);
     const size_t o_fptr = place(8 * nf), o_fbase = place(4 * nf), o_foff = place(4 * nf);
     const size_t o_ctl = place(sizeof(ScCtl));
     const size_t up_end = at;
@@ -788,7 +842,8 @@ Status run_batch(Ctx *ctx, Transcript &tr, const std::string &p, uint32_t D,
     uint8_t *const dev = blob.ptr;
     fp2 *const d_eqpt = (fp2 *)(dev + o_pts);
     EqSeg *const d_segs = (EqSeg *)(dev + o_segs);
-    ScInst *const d_inst = (ScInst *)(dev + o_inst);
+    const ScRoundInst *const d_rtab = (const ScRoundInst *)(dev + o_rtab);
+    ScInstState *const d_state = (ScInstState *)(dev + o_state);
     ScCtl *const d_ctl = (ScCtl *)(dev + o_ctl);
 
     // the eq half tables of every own round (written by k_eq_segments after the upload)
@@ -798,8 +853,6 @@ Status run_batch(Ctx *ctx, Transcript &tr, const std::string &p, uint32_t D,
         size_t g = 0;
         for (uint32_t i = 0; i < count; i++) {
             if (!insts[i].eq_factor) continue;
-            hinst[i].eqf = 1;
-            hinst[i].eq_r = d_eqpt + pt_off[i];
             for (uint32_t t = 0; t < insts[i].vars; t++, g += 2)
                 splits[i].push_back(EqSplit{d_eqtab.ptr + segs[g].off, d_eqtab.ptr + segs[g + 1].off, segs[g].n});
         }
@@ -812,8 +865,8 @@ Status run_batch(Ctx *ctx, Transcript &tr, const std::string &p, uint32_t D,
     put(o_segs, segs.data(), sizeof(EqSeg) * segs.size());
     put(o_labels, labels.data(), labels.size());
     put(o_lens, lens.data(), 4 * lens.size());
-    put(o_nblk, nblk.data(), 4 * nblk.size());
-    put(o_inst, hinst.data(), sizeof(ScInst) * count);
+    put(o_rtab, rtab.data(), sizeof(ScRoundInst) * rtab.size());
+    put(o_state, hstate.data(), sizeof(ScInstState) * count);
     put(o_fptr, fptr.data(), 8 * nf);
     put(o_fbase, fbase.data(), 4 * nf);
     put(o_foff, foff.data(), 4 * nf);
@@ -865,21 +918,22 @@ Status run_batch(Ctx *ctx, Transcript &tr, const std::string &p, uint32_t D,
             uint64_t pairs = 1ull << (insts[i].vars - 1);
             launch_round_d(D, insts[i].rel, dim3(blocks_for_pairs(pairs)), s, t,
                            (uint32_t)insts[i].tabs.size(), bm, true, 0, pairs, nullptr, lambda,
-                           partials[i].ptr, sterms[i],
+                           partials.ptr + pstride * i, sterms[i],
                            splits[i].empty() ? EqSplit() : splits[i][0]);
         }
     }
     for (uint32_t j = 0; j < n_max; j++) {
         {
             KernelTimer timer(ctx, "k_sc_transcript", s);
-            launch_transcript_d(D, s, j, d_inst, count, (const uint32_t *)(dev + o_nblk) + (size_t)j * count,
-                                d_ctl, keys, dev + o_labels, (const uint32_t *)(dev + o_lens),
-                                (fp2 *)(dev + o_rounds), (fp2 *)(dev + o_chals));
+            launch_transcript_d(D, s, j, d_rtab + (size_t)j * count, d_state, count, nblk_max[j],
+                                partials.ptr, d_ctl, keys, dev + o_labels,
+                                (const uint32_t *)(dev + o_lens), (fp2 *)(dev + o_rounds),
+                                (fp2 *)(dev + o_chals));
         }
         KernelTimer timer(ctx, "k_sc_round", s);
         for (uint32_t i = 0; i < count; i++) {
-            if (j < hinst[i].pad) continue;
-            uint32_t t = j - hinst[i].pad;
+            if (j < pad[i]) continue;
+            uint32_t t = j - pad[i];
             ScTables tt{};
             uint32_t bm;
             tables_of(i, t, tt, bm);
@@ -889,7 +943,7 @@ Status run_batch(Ctx *ctx, Transcript &tr, const std::string &p, uint32_t D,
             const EqSplit eq = splits[i].empty() ? EqSplit() : splits[i][std::min<uint32_t>(t + 1, insts[i].vars - 1)];
             launch_round_d(D, insts[i].rel, dim3(blocks_for_pairs(pairs ? pairs : out_len)), s,
                            tt, (uint32_t)insts[i].tabs.size(), bm, t == 0, out_len, pairs,
-                           &d_ctl->r, lambda, partials[i].ptr, sterms[i], eq);
+                           &d_ctl->r, lambda, partials.ptr + pstride * i, sterms[i], eq);
         }
     }
     if ((st = check_launch("sumcheck rounds"))) return st;

@@ -405,13 +405,17 @@ k_sc_transcript(uint32_t j, const ScRoundInst *__restrict__ ri, ScInstState *__r
 #pragma unroll
         for (int k = 0; k < PER_LANE; k++)
             if (lane + 64 * k >= nb) v[k] = fp2{0, 0};
+        if (nblk_max > 64) {  // uniform: more than one partial per lane
 #pragma unroll
-        for (int w = PER_LANE / 2; w >= 1; w /= 2)
+            for (int w = PER_LANE / 2; w >= 1; w /= 2)
 #pragma unroll
-            for (int k = 0; k < w; k++) v[k] = gl::add(v[k], v[k + w]);
+                for (int k = 0; k < w; k++) v[k] = gl::add(v[k], v[k + w]);
+        }
         fp2 s = v[0];
+        // lanes at or above nblk_max hold zero: only the steps that reach a lower lane
 #pragma unroll
         for (int off = 32; off > 0; off >>= 1) {
+            if ((uint32_t)off >= nblk_max) continue;
             fp2 o;
             o.c0 = __shfl_xor(s.c0, off, 64);
             o.c1 = __shfl_xor(s.c1, off, 64);

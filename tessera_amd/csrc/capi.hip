@@ -106,6 +106,27 @@ Status dev_alloc(Ctx *ctx, uint64_t bytes, void **out) {
     return Status::ok();
 }
 
+Status pinned_host(Ctx *ctx, size_t bytes, uint8_t **out) {
+    if (bytes > ctx->pinned_cap) {
+        if (ctx->pinned) {
+            (void)hipStreamSynchronize(ctx->stream);
+            (void)hipHostFree(ctx->pinned);
+            ctx->pinned = nullptr;
+            ctx->pinned_cap = 0;
+        }
+        const size_t cap = std::max<size_t>(bytes, 64 * 1024);
+        hipError_t e = hipHostMalloc(&ctx->pinned, cap, hipHostMallocDefault);
+        if (e != hipSuccess) {
+            (void)hipGetLastError();
+            ctx->pinned = nullptr;
+            return hip_fail(e, "hipHostMalloc");
+        }
+        ctx->pinned_cap = cap;
+    }
+    *out = (uint8_t *)ctx->pinned;
+    return Status::ok();
+}
+
 void dev_free(Ctx *ctx, void *p, uint64_t bytes) {
     if (!p) return;
     if (!ctx) {
@@ -277,6 +298,7 @@ void tsr_ctx_destroy(tsr_ctx *ctx) {
     ctx->scratch.release();
     ctx->staging_in.release();
     ctx->staging_out.release();
+    if (ctx->pinned) (void)hipHostFree(ctx->pinned);
     pool_trim(ctx);
     (void)hipStreamDestroy(ctx->stream);
     delete ctx;

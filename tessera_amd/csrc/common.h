@@ -113,7 +113,14 @@ struct Ctx {
     DevBuf<uint64_t> scratch;      // reusable per-call scratch (encode temporaries)
     DevBuf<uint64_t> staging_in;   // host-input staging
     DevBuf<uint64_t> staging_out;  // host-output staging
+    // Pinned host memory for the small per-call uploads and readbacks of the sumcheck rounds
+    // (grown on demand, freed with the context; a context's calls run one at a time).
+    void *pinned = nullptr;
+    size_t pinned_cap = 0;
 };
+
+// At least `bytes` of the context's pinned host buffer (hipHostMalloc, grown when short).
+Status pinned_host(Ctx *ctx, size_t bytes, uint8_t **out);
 
 template <typename T>
 Status DevBuf<T>::upload(Ctx *c, const std::vector<T> &v) {

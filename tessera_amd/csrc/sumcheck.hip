@@ -861,9 +861,12 @@ Status run_batch(Ctx *ctx, Transcript &tr, const std::string &p, uint32_t D,
                 splits[i].push_back(EqSplit{d_eqtab.ptr + segs[g].off, d_eqtab.ptr + segs[g + 1].off, segs[g].n});
         }
     }
-    std::vector<uint8_t> up(up_end, 0);
+    // one pinned host block for both copies: the upload image, then the readback
+    uint8_t *up;
+    if ((st = pinned_host(ctx, std::max(up_end, at - o_ctl), &up))) return st;
+    memset(up, 0, up_end);
     auto put = [&](size_t o, const void *src, size_t bytes) {
-        if (bytes) memcpy(&up[o], src, bytes);
+        if (bytes) memcpy(up + o, src, bytes);
     };
     put(o_pts, eq_pts.data(), 16 * eq_pts.size());
     put(o_segs, segs.data(), sizeof(EqSeg) * segs.size());
@@ -875,7 +878,7 @@ Status run_batch(Ctx *ctx, Transcript &tr, const std::string &p, uint32_t D,
     put(o_fbase, fbase.data(), 4 * nf);
     put(o_foff, foff.data(), 4 * nf);
     put(o_ctl, &hctl, sizeof(ScCtl));
-    TSR_HIP(hipMemcpyAsync(dev, up.data(), up_end, hipMemcpyHostToDevice, s));
+    TSR_HIP(hipMemcpyAsync(dev, up, up_end, hipMemcpyHostToDevice, s));
     const DevKeys keys = dev_keys(ctx);
 
     if (!segs.empty()) {
@@ -957,8 +960,9 @@ Status run_batch(Ctx *ctx, Transcript &tr, const std::string &p, uint32_t D,
                            (const uint32_t *)(dev + o_foff), (uint32_t)nf, (fp2 *)(dev + o_fin));
         if ((st = check_launch("sumcheck finals"))) return st;
     }
-    std::vector<uint8_t> down(at - o_ctl);
-    TSR_HIP(hipMemcpyAsync(down.data(), dev + o_ctl, down.size(), hipMemcpyDeviceToHost, s));
+    // the readback reuses the pinned block: the stream has finished the upload before it
+    uint8_t *const down = up;
+    TSR_HIP(hipMemcpyAsync(down, dev + o_ctl, at - o_ctl, hipMemcpyDeviceToHost, s));
     TSR_HIP(hipStreamSynchronize(s));
     memcpy(&hctl, &down[0], sizeof(ScCtl));
     out.rounds.resize((size_t)n_max * D);

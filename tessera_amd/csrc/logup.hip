@@ -89,13 +89,24 @@ __global__ void k_eq_outer(const fp2 *__restrict__ lo, uint32_t a, const fp2 *__
         out[x] = gl::mul(lo[x & ((1ull << a) - 1)], hi[x >> a]);
 }
 
-// Partial sums of eq[x] * col[x] (col padded with `pad` from `count` on), one per block.
+// Partial sums of eq(point, x) col[x] over x < groups 2^a (col padded with `pad` from `count`
+// on), one per block, with eq(point, x) = lo[x mod 2^a] hi[x >> a]: a block takes whole groups
+// g of 2^a consecutive x, sums lo[l] col[g 2^a + l] (two base products per entry) and weights
+// the group's sum by hi[g] once. No eq table of 2^n entries is built or read.
 __global__ void __launch_bounds__(256)
-k_eq_dot(const fp2 *__restrict__ eq, const uint64_t *__restrict__ col, uint64_t count,
-         uint64_t pad, uint64_t len, fp2 *__restrict__ partials) {
+k_eq_dot_split(const fp2 *__restrict__ lo, uint32_t a, const fp2 *__restrict__ hi,
+               uint64_t groups, const uint64_t *__restrict__ col, uint64_t count, uint64_t pad,
+               fp2 *__restrict__ partials) {
     fp2 acc{0, 0};
-    for (uint64_t x = (uint64_t)blockIdx.x * 256 + threadIdx.x; x < len; x += (uint64_t)gridDim.x * 256)
-        acc = gl::add(acc, gl::mul_base(eq[x], x < count ? col[x] : pad));
+    const uint64_t glen = 1ull << a;
+    for (uint64_t g = blockIdx.x; g < groups; g += gridDim.x) {
+        fp2 sg{0, 0};
+        for (uint64_t l = threadIdx.x; l < glen; l += 256) {
+            const uint64_t x = (g << a) + l;
+            sg = gl::add(sg, gl::mul_base(lo[l], x < count ? col[x] : pad));
+        }
+        acc = gl::add(acc, gl::mul(sg, hi[g]));
+    }
     __shared__ fp2 red[256];
     red[threadIdx.x] = acc;
     __syncthreads();
@@ -106,11 +117,19 @@ k_eq_dot(const fp2 *__restrict__ eq, const uint64_t *__restrict__ col, uint64_t 
     if (threadIdx.x == 0) partials[blockIdx.x] = red[0];
 }
 
-__global__ void k_sum_partials(const fp2 *__restrict__ partials, uint32_t n, fp2 *__restrict__ out) {
-    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+// The sum of n partials (one block of 256 threads; the loads of a thread are independent).
+__global__ void __launch_bounds__(256)
+k_sum_partials(const fp2 *__restrict__ partials, uint32_t n, fp2 *__restrict__ out) {
     fp2 s{0, 0};
-    for (uint32_t i = 0; i < n; i++) s = gl::add(s, partials[i]);
-    *out = s;
+    for (uint32_t i = threadIdx.x; i < n; i += 256) s = gl::add(s, partials[i]);
+    __shared__ fp2 red[256];
+    red[threadIdx.x] = s;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) red[threadIdx.x] = gl::add(red[threadIdx.x], red[threadIdx.x + w]);
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) *out = red[0];
 }
 
 // multiplicities (instance.rs:225-253): exact counts over 2^{h_t} rows.
@@ -276,15 +295,23 @@ struct Log {
     }
 };
 
-// Sum_z eq[z] col[z] over 2^n entries (col base), read back to the host.
-static Status eq_dot(Ctx *ctx, const fp2 *eq, const uint64_t *col, uint64_t count, uint64_t pad,
-                     uint32_t n, fp2 *part, fp2 *val, fp2 &out) {
+// Sum_x eq(point, x) col[x] over x < 2^n (col base, padded with `pad` from `count` on), read back
+// to the host; point: n Fp2 on the device. The eq half tables over point[0, a) and point[a, n)
+// take 2^a + 2^(n-a) entries.
+static Status eq_dot(Ctx *ctx, const fp2 *point, uint32_t n, const uint64_t *col, uint64_t count,
+                     uint64_t pad, fp2 *part, fp2 *val, fp2 &out) {
     hipStream_t s = ctx->stream;
-    const uint64_t len = 1ull << n;
-    const unsigned nb = std::min<unsigned>(1024, nblocks(len, 256));
-    hipLaunchKernelGGL(k_eq_dot, dim3(nb), dim3(256), 0, s, eq, col, count, pad, len, part);
-    hipLaunchKernelGGL(k_sum_partials, dim3(1), dim3(64), 0, s, part, nb, val);
+    const uint32_t a = n <= 12 ? n : n / 2;
+    const uint64_t groups = 1ull << (n - a);
+    DevBuf<fp2> lo, hi;
     Status st;
+    if ((st = lo.alloc(ctx, 1ull << a)) || (st = hi.alloc(ctx, groups))) return st;
+    hipLaunchKernelGGL(k_eq_table, dim3(nblocks(1ull << a, 256)), dim3(256), 0, s, point, a, lo.ptr);
+    hipLaunchKernelGGL(k_eq_table, dim3(nblocks(groups, 256)), dim3(256), 0, s, point + a, n - a, hi.ptr);
+    const unsigned nb = (unsigned)std::min<uint64_t>(1024, groups);
+    hipLaunchKernelGGL(k_eq_dot_split, dim3(nb), dim3(256), 0, s, lo.ptr, a, hi.ptr, groups, col,
+                       count, pad, part);
+    hipLaunchKernelGGL(k_sum_partials, dim3(1), dim3(256), 0, s, part, nb, val);
     if ((st = check_launch("eq dot"))) return st;
     TSR_HIP(hipMemcpyAsync(&out, val, 16, hipMemcpyDeviceToHost, s));
     TSR_HIP(hipStreamSynchronize(s));
@@ -334,7 +361,6 @@ static Status prove_levels(Ctx *ctx, Transcript &tr, std::vector<DevTree> &trees
         const fp2 gamma = tr.challenge_fp2((prefix + "/gamma").c_str(), nullptr);
         const fp2 lambda = fractions ? tr.challenge_fp2((prefix + "/lambda").c_str(), nullptr) : fp2{0, 0};
         std::vector<GenInst> insts(act.size());
-        std::vector<DevBuf<fp2>> eqs(act.size());
         for (size_t i = 0; i < act.size(); i++) {
             DevTree &t = trees[act[i]];
             const uint32_t below = level - t.join + 1, n = (uint32_t)t.point.size();
@@ -346,18 +372,15 @@ static Status prove_levels(Ctx *ctx, Transcript &tr, std::vector<DevTree> &trees
             g.eq_point = t.point;
             if (t.forest) {
                 if (!t.has_claim) {
-                    if ((st = eqs[i].alloc(ctx, 1ull << n))) return st;
                     if (n) {
                         TSR_HIP(hipMemcpyAsync(d_point.ptr, t.point.data(), 16 * n, hipMemcpyHostToDevice, s));
                     }
-                    if ((st = eq_table(ctx, d_point.ptr, n, eqs[i].ptr, s))) return st;
-                    // the joining claim: V0 V1 at z is the root V[0][z] (trees.rs:638-640)
-                    if ((st = eq_dot(ctx, eqs[i].ptr, t.V[0].ptr, 1ull << n, 0, n, d_part.ptr,
+                    // the joining claim: V0 V1 at z is the root V[0][z] (trees.rs:638-640); eq_dot
+                    // ends synchronized, so d_point is free for the next point
+                    if ((st = eq_dot(ctx, d_point.ptr, n, t.V[0].ptr, 1ull << n, 0, d_part.ptr,
                                      d_val.ptr, t.num)))
                         return st;
                     t.has_claim = true;
-                    // the copy of the point into d_point is done before the next overwrites it
-                    TSR_HIP(hipStreamSynchronize(s));
                 }
                 g.rel = REL_PRODUCT;
                 g.claim = t.num;
@@ -542,15 +565,12 @@ Status prove_logup(Ctx *ctx, Transcript &tr, const tsr_logup_instance *inst, uin
         DevTree &la = trees[2 * k], &ta = trees[2 * k + 1];
         if (inst[k].values) {
             const uint32_t n = la.h;
-            DevBuf<fp2> eq;
-            if ((st = eq.alloc(ctx, 1ull << n))) return st;
             if (n) TSR_HIP(hipMemcpyAsync(d_point.ptr, la.point.data(), 16 * n, hipMemcpyHostToDevice, s));
-            if ((st = eq_table(ctx, d_point.ptr, n, eq.ptr, s))) return st;
             uint64_t pad = 0;  // row 0's value fills the padding slots
             TSR_HIP(hipMemcpyAsync(&pad, tvals[k], 8, hipMemcpyDeviceToHost, s));
             TSR_HIP(hipStreamSynchronize(s));
             fp2 v{0, 0};
-            if ((st = eq_dot(ctx, eq.ptr, lvals[k], inst[k].lookups, pad, n, d_part.ptr, d_val.ptr, v)))
+            if ((st = eq_dot(ctx, d_point.ptr, n, lvals[k], inst[k].lookups, pad, d_part.ptr, d_val.ptr, v)))
                 return st;
             values.push_back(v);
         }

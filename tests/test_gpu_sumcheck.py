@@ -223,3 +223,33 @@ def test_sum_of_products_shape_errors(dev):
     with pytest.raises(T.BadShape, match="TooManyTables"):
         prove_batch(dev, t, "stage/0", 1, [[a, a]])
     prove_batch(dev, t, "stage/0", 1, [SumOfProducts([a, a, a], [((1, 0), [2])])])
+
+
+@pytest.mark.parametrize("count", [65, 256])
+def test_many_instances(dev, count):
+    """Batches past the transcript kernel's staged instance facts (64) up to the largest batch
+    (256): front padding from 0 to 6 variables, base and Fp2 tables, public factors, products of
+    two and three factors at degree 3."""
+    rng = np.random.default_rng(count)
+    spec = []
+    for i in range(count):
+        n = int(rng.integers(0, 7))
+        k = 2 + (i % 2)
+        inst = []
+        for f in range(k):
+            ext = (i + f) % 3 == 0
+            v = rng.integers(0, P, (1 << n) * (2 if ext else 1), dtype=np.uint64)
+            inst.append((v.reshape(-1, 2) if ext else v, ext, f == 2))
+        spec.append(inst)
+    run_both(dev, "stage/many", 3, spec)
+
+
+def test_more_than_256_instances_are_declined(dev):
+    import tessera_amd as T
+    from tessera_amd.sumcheck import Table, prove_batch
+    from tessera_amd.transcript import Transcript
+    t = Transcript(b"x")
+    s0 = t.state
+    with pytest.raises(T.Declined, match="256"):
+        prove_batch(dev, t, "stage/0", 2, [[Table(np.ones(2, np.uint64))] for _ in range(257)])
+    assert t.state == s0 and t.counter == 0

@@ -45,28 +45,101 @@ __global__ void k_leaves(const uint64_t *__restrict__ keys, const uint64_t *__re
     if (mult) nout[x] = gl::neg(mult[x]);
 }
 
-// Level above: node x from children 2x, 2x+1. Numerators below: unit (nb == nullptr and
-// nbase == nullptr), base (nbase) or Fp2 (nb).
-__global__ void k_level_above(const fp2 *__restrict__ nb, const uint64_t *__restrict__ nbase,
-                              const fp2 *__restrict__ db, uint64_t count, fp2 *__restrict__ n,
-                              fp2 *__restrict__ d) {
-    uint64_t x = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (x >= count) return;
-    fp2 dl = db[2 * x], dr = db[2 * x + 1];
-    fp2 nl, nr;
-    if (nb) {
-        nl = nb[2 * x];
-        nr = nb[2 * x + 1];
-    } else if (nbase) {
-        nl = fp2{nbase[2 * x], 0};
-        nr = fp2{nbase[2 * x + 1], 0};
-    } else {
-        nl = fp2{1, 0};
-        nr = fp2{1, 0};
+// Up to three tree levels per pass. A workgroup of 512 threads takes 256 * 2^L consecutive
+// nodes of a level into LDS (loads and stores coalesced: thread t touches t, t + 512, ...) and
+// writes the L levels above them, so a level is read once for every three built.
+constexpr int LEVELS_PER_PASS = 3;
+constexpr int LEVEL_THREADS = 512;
+constexpr uint32_t level_block(int L) { return 256u << L; }  // inputs per workgroup
+struct LevelsOut {
+    fp2 *n[LEVELS_PER_PASS], *d[LEVELS_PER_PASS];  // level k above the inputs: n[k-1], d[k-1]
+};
+
+// The block's inputs are sn/sd[0, cnt) (input index base + i); writes levels 1..L above.
+// UNIT: the inputs' numerators are all 1 (level 1 is D_l + D_r over D_l D_r; sn unused).
+template <int L, bool UNIT>
+__device__ __forceinline__ void block_levels(fp2 *sn, fp2 *sd, uint64_t base, uint32_t cnt,
+                                             const LevelsOut &out) {
+    const uint32_t t = threadIdx.x;
+#pragma unroll
+    for (int k = 1; k <= L; k++) {
+        const uint32_t m = cnt >> k;  // nodes of level k in this block
+        // level-1 nodes per thread (at least one; higher levels have fewer, j < m guards them)
+        constexpr int PER = (int)(level_block(L) / 2 / LEVEL_THREADS) > 0 ? (int)(level_block(L) / 2 / LEVEL_THREADS) : 1;
+        fp2 nn[PER], dd[PER];
+#pragma unroll
+        for (int i = 0; i < PER; i++) {
+            const uint32_t j = t + LEVEL_THREADS * i;
+            if (j < m) {
+                const fp2 dl = sd[2 * j], dr = sd[2 * j + 1];
+                nn[i] = (UNIT && k == 1) ? gl::add(dl, dr)
+                                         : gl::add(gl::mul(sn[2 * j], dr), gl::mul(sn[2 * j + 1], dl));
+                dd[i] = gl::mul(dl, dr);
+            }
+        }
+        __syncthreads();  // every child read before any parent overwrites it
+#pragma unroll
+        for (int i = 0; i < PER; i++) {
+            const uint32_t j = t + LEVEL_THREADS * i;
+            if (j < m) {
+                sn[j] = nn[i];
+                sd[j] = dd[i];
+                out.n[k - 1][(base >> k) + j] = nn[i];
+                out.d[k - 1][(base >> k) + j] = dd[i];
+            }
+        }
+        __syncthreads();
     }
-    n[x] = gl::add(gl::mul(nl, dr), gl::mul(nr, dl));
-    d[x] = gl::mul(dl, dr);
 }
+
+// The leaves of a tree (as k_leaves) and the L levels above them: workgroup b owns leaves
+// [b 256 2^L, (b + 1) 256 2^L) of the 2^h. Unit leaves keep no numerators: sn holds level 1 up.
+template <int L, bool UNIT>
+__global__ void __launch_bounds__(LEVEL_THREADS)
+k_leaves_above(const uint64_t *__restrict__ keys, const uint64_t *__restrict__ vals,
+               uint64_t count, uint64_t leaves, fp2 alpha, fp2 beta, fp2 pad,
+               const uint64_t *__restrict__ mult, uint64_t *__restrict__ nout,
+               fp2 *__restrict__ dout, LevelsOut out) {
+    __shared__ fp2 sn[UNIT ? level_block(L) / 2 : level_block(L)], sd[level_block(L)];
+    const uint64_t base = (uint64_t)blockIdx.x * level_block(L);
+    const uint32_t cnt = (uint32_t)std::min<uint64_t>(leaves - base, level_block(L));
+#pragma unroll
+    for (int i = 0; i < (int)(level_block(L) / LEVEL_THREADS); i++) {
+        const uint32_t j = threadIdx.x + LEVEL_THREADS * i;
+        if (j >= cnt) break;
+        const uint64_t y = base + j;
+        const fp2 d = y < count ? leaf_den(alpha, beta, keys[y], vals, y) : pad;
+        dout[y] = d;
+        sd[j] = d;
+        if (!UNIT) {
+            const uint64_t m = gl::neg(mult[y]);
+            nout[y] = m;
+            sn[j] = fp2{m, 0};
+        }
+    }
+    __syncthreads();
+    block_levels<L, UNIT>(sn, sd, base, cnt, out);
+}
+
+// The L levels above a stored level of 2^below Fp2 numerators and denominators.
+template <int L>
+__global__ void __launch_bounds__(LEVEL_THREADS)
+k_levels_above(const fp2 *__restrict__ nb, const fp2 *__restrict__ db, uint64_t inputs,
+               LevelsOut out) {
+    __shared__ fp2 sn[level_block(L)], sd[level_block(L)];
+    const uint64_t base = (uint64_t)blockIdx.x * level_block(L);
+    const uint32_t cnt = (uint32_t)std::min<uint64_t>(inputs - base, level_block(L));
+#pragma unroll
+    for (int i = 0; i < (int)(level_block(L) / LEVEL_THREADS); i++) {
+        const uint32_t j = threadIdx.x + LEVEL_THREADS * i;
+        if (j >= cnt) break;
+        sn[j] = nb[base + j];
+        sd[j] = db[base + j];
+    }
+    __syncthreads();
+    block_levels<L, false>(sn, sd, base, cnt, out);
+}
+
 
 // eq(r, x) = prod_i (x_i ? r_i : 1 - r_i), x_1 the lowest bit (tessera-poly eq.rs:23).
 __global__ void k_eq_table(const fp2 *__restrict__ point, uint32_t n, fp2 *__restrict__ out) {
@@ -246,19 +319,58 @@ static Status build_tree(Ctx *ctx, DevTree &t, uint32_t h, const uint64_t *keys,
     Status st;
     if ((st = t.D[h].alloc(ctx, leaves))) return st;
     if (mult && (st = t.nbase.alloc(ctx, leaves))) return st;
-    {
+    for (int l = (int)h - 1; l >= 0; l--)
+        if ((st = t.N[l].alloc(ctx, 1ull << l)) || (st = t.D[l].alloc(ctx, 1ull << l))) return st;
+    if (h == 0) {
         KernelTimer timer(ctx, "k_leaves", s);
-        hipLaunchKernelGGL(k_leaves, dim3(nblocks(leaves, 256)), dim3(256), 0, s, keys, vals,
-                           count, leaves, alpha, beta, pad, mult, t.nbase.ptr, t.D[h].ptr);
+        hipLaunchKernelGGL(k_leaves, dim3(1), dim3(256), 0, s, keys, vals, count, leaves, alpha,
+                           beta, pad, mult, t.nbase.ptr, t.D[0].ptr);
+        return check_launch("fraction tree");
     }
-    KernelTimer timer(ctx, "k_level_above", s);
-    for (int l = (int)h - 1; l >= 0; l--) {
-        const uint64_t m = 1ull << l;
-        if ((st = t.N[l].alloc(ctx, m)) || (st = t.D[l].alloc(ctx, m))) return st;
-        const fp2 *nb = (l + 1 == (int)h) ? nullptr : t.N[l + 1].ptr;
-        const uint64_t *nbb = (l + 1 == (int)h && mult) ? t.nbase.ptr : nullptr;
-        hipLaunchKernelGGL(k_level_above, dim3(nblocks(m, 256)), dim3(256), 0, s, nb, nbb,
-                           t.D[l + 1].ptr, m, t.N[l].ptr, t.D[l].ptr);
+    // the leaves and up to three levels above them, then three levels per pass
+    auto outs = [&](uint32_t below, uint32_t L) {
+        LevelsOut o{};
+        for (uint32_t k = 1; k <= L; k++) {
+            o.n[k - 1] = t.N[below - k].ptr;
+            o.d[k - 1] = t.D[below - k].ptr;
+        }
+        return o;
+    };
+    {
+        KernelTimer timer(ctx, "k_leaves_above", s);
+        const uint32_t L = std::min<uint32_t>(h, LEVELS_PER_PASS);
+        const LevelsOut o = outs(h, L);
+        const dim3 grid(nblocks(leaves, level_block(L)));
+#define LEAVES(LL)                                                                               \
+    if (L == LL) {                                                                               \
+        if (mult)                                                                                \
+            hipLaunchKernelGGL((k_leaves_above<LL, false>), grid, dim3(LEVEL_THREADS), 0, s,      \
+                               keys, vals, count, leaves, alpha, beta, pad, mult, t.nbase.ptr,   \
+                               t.D[h].ptr, o);                                                   \
+        else                                                                                     \
+            hipLaunchKernelGGL((k_leaves_above<LL, true>), grid, dim3(LEVEL_THREADS), 0, s, keys, \
+                               vals, count, leaves, alpha, beta, pad, mult, nullptr, t.D[h].ptr, \
+                               o);                                                               \
+    }
+        LEAVES(1) LEAVES(2) LEAVES(3)
+#undef LEAVES
+    }
+    KernelTimer timer(ctx, "k_levels_above", s);
+    for (uint32_t below = h - std::min<uint32_t>(h, LEVELS_PER_PASS); below > 0;) {
+        const uint32_t L = std::min<uint32_t>(below, LEVELS_PER_PASS);
+        const uint64_t inputs = 1ull << below;
+        const LevelsOut o = outs(below, L);
+        const dim3 grid(nblocks(inputs, level_block(L)));
+        if (L == 1)
+            hipLaunchKernelGGL(k_levels_above<1>, grid, dim3(LEVEL_THREADS), 0, s, t.N[below].ptr,
+                               t.D[below].ptr, inputs, o);
+        else if (L == 2)
+            hipLaunchKernelGGL(k_levels_above<2>, grid, dim3(LEVEL_THREADS), 0, s, t.N[below].ptr,
+                               t.D[below].ptr, inputs, o);
+        else
+            hipLaunchKernelGGL(k_levels_above<3>, grid, dim3(LEVEL_THREADS), 0, s, t.N[below].ptr,
+                               t.D[below].ptr, inputs, o);
+        below -= L;
     }
     return check_launch("fraction tree");
 }

@@ -236,7 +236,10 @@ k_sc_round(ScTables t, uint32_t F, uint32_t base_mask, bool first, uint64_t out_
 //                 (E N0) D1 + (E D0)(N1 + lambda D1); UNIT (N0 = N1 = 1, tables 0 and 1 absent):
 //                 E (D0 + D1) + (lambda E D0) D1
 // Line values: X(0) = a, X(1) = b, X(2) = 2b - a.
-template <int REL, bool UNIT>
+// BASE (REL_PRODUCT, first round of a level, both tables base, as a forest's level values): the
+// sums multiply in the base field and scale by E once per node (3 base products + 3 Fp2 x Fp
+// per pair instead of 6 Fp2 products), the binding takes 2 base products per entry.
+template <int REL, bool UNIT, bool BASE = false>
 __global__ void __launch_bounds__(SC_THREADS)
 k_sc_round_eq(ScTables t, uint32_t base_mask, bool first, uint64_t out_len, uint64_t pairs,
               const fp2 *__restrict__ r_ptr, fp2 lambda, fp2 *__restrict__ partials,
@@ -252,8 +255,44 @@ k_sc_round_eq(ScTables t, uint32_t base_mask, bool first, uint64_t out_len, uint
 
     for (uint64_t q = (uint64_t)blockIdx.x * SC_THREADS + threadIdx.x; q < items; q += stride) {
         fp2 a[F], b[F];
+        if (BASE) {
+            const uint64_t *s0 = (const uint64_t *)t.src[0], *s1 = (const uint64_t *)t.src[1];
+            auto at0 = [&](uint64_t x) { return s0[(x << t.shift[0]) + t.off[0]]; };
+            auto at1 = [&](uint64_t x) { return s1[(x << t.shift[1]) + t.off[1]]; };
+            // e0 + r (e1 - e0) for base e0, e1
+            auto bindb = [&](uint64_t e0, uint64_t e1) {
+                fp2 v = gl::mul_base(r, gl::sub(e1, e0));
+                v.c0 = gl::add(v.c0, e0);
+                return v;
+            };
+            if (!bind) {
+                const uint64_t a0 = at0(2 * q), b0 = at0(2 * q + 1), a1 = at1(2 * q), b1 = at1(2 * q + 1);
+                const fp2 e = gl::mul(eq_lo[q & ((1ull << eq_a) - 1)], eq_hi[q >> eq_a]);
+                const uint64_t f0 = gl::mul(a0, a1), f1 = gl::mul(b0, b1);
+                const uint64_t f2 = gl::mul(gl::sub(gl::add(b0, b0), a0), gl::sub(gl::add(b1, b1), a1));
+                acc[0] = gl::add(acc[0], gl::mul_base(e, f0));
+                acc[1] = gl::add(acc[1], gl::mul_base(e, f1));
+                acc[2] = gl::add(acc[2], gl::mul_base(e, f2));
+                continue;
+            }
+            if (pairs == 0) {
+                t.dst[0][q] = bindb(at0(2 * q), at0(2 * q + 1));
+                t.dst[1][q] = bindb(at1(2 * q), at1(2 * q + 1));
+                continue;
+            }
+            a[0] = bindb(at0(4 * q), at0(4 * q + 1));
+            b[0] = bindb(at0(4 * q + 2), at0(4 * q + 3));
+            a[1] = bindb(at1(4 * q), at1(4 * q + 1));
+            b[1] = bindb(at1(4 * q + 2), at1(4 * q + 3));
+#pragma unroll
+            for (int k = 0; k < 2; k++) {
+                t.dst[k][2 * q] = a[k];
+                t.dst[k][2 * q + 1] = b[k];
+            }
+        }
 #pragma unroll
         for (int k = K0; k < F; k++) {
+            if (BASE) break;
             const bool base = first && ((base_mask >> k) & 1);
             const uint32_t sh = first ? t.shift[k] : 0, of = first ? t.off[k] : 0;
             auto at = [&](uint64_t x) { return ld_entry(t.src[k], base, (x << sh) + of); };
@@ -550,12 +589,12 @@ static void launch_round(dim3 grid, hipStream_t s, const ScTables &t, uint32_t F
                        first, out_len, pairs, r, lambda, partials, terms);
 }
 
-template <int REL, bool UNIT>
+template <int REL, bool UNIT, bool BASE = false>
 static void launch_round_eq(dim3 grid, hipStream_t s, const ScTables &t, uint32_t base_mask,
                             bool first, uint64_t out_len, uint64_t pairs, const fp2 *r, fp2 lambda,
                             fp2 *partials, const EqSplit &eq) {
-    hipLaunchKernelGGL((k_sc_round_eq<REL, UNIT>), grid, dim3(SC_THREADS), 0, s, t, base_mask,
-                       first, out_len, pairs, r, lambda, partials, eq.lo, eq.hi, eq.a);
+    hipLaunchKernelGGL((k_sc_round_eq<REL, UNIT, BASE>), grid, dim3(SC_THREADS), 0, s, t,
+                       base_mask, first, out_len, pairs, r, lambda, partials, eq.lo, eq.hi, eq.a);
 }
 
 static void launch_round_d(uint32_t d, int rel, dim3 grid, hipStream_t s, const ScTables &t,
@@ -563,7 +602,10 @@ static void launch_round_d(uint32_t d, int rel, dim3 grid, hipStream_t s, const 
                            uint64_t pairs, const fp2 *r, fp2 lambda, fp2 *partials,
                            const ScTerms &terms, const EqSplit &eq = EqSplit()) {
     if (eq.lo) {  // tree levels: degree 3, tables T0 T1 or N0 N1 D0 D1
-        if (rel == REL_PRODUCT)
+        if (rel == REL_PRODUCT && first && (base_mask & 3) == 3)
+            launch_round_eq<REL_PRODUCT, false, true>(grid, s, t, base_mask, first, out_len, pairs,
+                                                      r, lambda, partials, eq);
+        else if (rel == REL_PRODUCT)
             launch_round_eq<REL_PRODUCT, false>(grid, s, t, base_mask, first, out_len, pairs, r,
                                                 lambda, partials, eq);
         else if (!t.src[0])

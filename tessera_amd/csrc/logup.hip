@@ -51,6 +51,9 @@ __global__ void k_leaves(const uint64_t *__restrict__ keys, const uint64_t *__re
 constexpr int LEVELS_PER_PASS = 3;
 constexpr int LEVEL_THREADS = 512;
 constexpr uint32_t level_block(int L) { return 256u << L; }  // inputs per workgroup
+struct LevelsOutBase {
+    uint64_t *v[3];  // level k above the inputs: v[k-1]
+};
 struct LevelsOut {
     fp2 *n[LEVELS_PER_PASS], *d[LEVELS_PER_PASS];  // level k above the inputs: n[k-1], d[k-1]
 };
@@ -257,11 +260,42 @@ static uint32_t ceil_log2(uint64_t x) {
 
 // A product forest's level above (forest.rs:306-340): V[t][z] = V[t+1][2z] V[t+1][2z+1], base
 // field throughout since the leaves are base.
-__global__ void k_product_above(const uint64_t *__restrict__ below, uint64_t count,
-                                uint64_t *__restrict__ out) {
-    const uint64_t z = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (z >= count) return;
-    out[z] = gl::mul(below[2 * z], below[2 * z + 1]);
+// L levels per pass as the fraction trees': a 512-thread workgroup stages 256 2^L consecutive
+// values in LDS and writes their ancestors on each level (out[k-1]: level k above the input).
+template <int L>
+__global__ void __launch_bounds__(LEVEL_THREADS)
+k_products_above(const uint64_t *__restrict__ below, uint64_t inputs, LevelsOutBase out) {
+    __shared__ uint64_t sv[level_block(L)];
+    const uint64_t base = (uint64_t)blockIdx.x * level_block(L);
+    const uint32_t cnt = (uint32_t)std::min<uint64_t>(inputs - base, level_block(L));
+    const uint32_t t = threadIdx.x;
+#pragma unroll
+    for (int i = 0; i < (int)(level_block(L) / LEVEL_THREADS); i++) {
+        const uint32_t j = t + LEVEL_THREADS * i;
+        if (j < cnt) sv[j] = below[base + j];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int k = 1; k <= L; k++) {
+        const uint32_t m = cnt >> k;
+        constexpr int PER = (int)(level_block(L) / 2 / LEVEL_THREADS) > 0 ? (int)(level_block(L) / 2 / LEVEL_THREADS) : 1;
+        uint64_t v[PER];
+#pragma unroll
+        for (int i = 0; i < PER; i++) {
+            const uint32_t j = t + LEVEL_THREADS * i;
+            if (j < m) v[i] = gl::mul(sv[2 * j], sv[2 * j + 1]);
+        }
+        __syncthreads();
+#pragma unroll
+        for (int i = 0; i < PER; i++) {
+            const uint32_t j = t + LEVEL_THREADS * i;
+            if (j < m) {
+                sv[j] = v[i];
+                out.v[k - 1][(base >> k) + j] = v[i];
+            }
+        }
+        __syncthreads();
+    }
 }
 
 // A tree of the level loop on the device.
@@ -297,12 +331,22 @@ static Status build_forest(Ctx *ctx, DevTree &t, const uint64_t *leaves, uint64_
     t.leaves = leaves;
     t.V = std::vector<DevBuf<uint64_t>>(height);
     Status st;
-    KernelTimer timer(ctx, "k_product_above", s);
-    for (int l = (int)height - 1; l >= 0; l--) {
-        const uint64_t m = 1ull << (t.s + l);
-        if ((st = t.V[l].alloc(ctx, m))) return st;
-        hipLaunchKernelGGL(k_product_above, dim3(nblocks(m, 256)), dim3(256), 0, s,
-                           t.level_values(l + 1), m, t.V[l].ptr);
+    for (int l = (int)height - 1; l >= 0; l--)
+        if ((st = t.V[l].alloc(ctx, 1ull << (t.s + l)))) return st;
+    KernelTimer timer(ctx, "k_products_above", s);
+    for (uint32_t below = height; below > 0;) {
+        const uint32_t L = std::min<uint32_t>(below, LEVELS_PER_PASS);
+        const uint64_t inputs = 1ull << (t.s + below);
+        LevelsOutBase o{};
+        for (uint32_t k = 1; k <= L; k++) o.v[k - 1] = t.V[below - k].ptr;
+        const dim3 grid(nblocks(inputs, level_block(L)));
+        if (L == 1)
+            hipLaunchKernelGGL(k_products_above<1>, grid, dim3(LEVEL_THREADS), 0, s, t.level_values(below), inputs, o);
+        else if (L == 2)
+            hipLaunchKernelGGL(k_products_above<2>, grid, dim3(LEVEL_THREADS), 0, s, t.level_values(below), inputs, o);
+        else
+            hipLaunchKernelGGL(k_products_above<3>, grid, dim3(LEVEL_THREADS), 0, s, t.level_values(below), inputs, o);
+        below -= L;
     }
     return check_launch("product forest");
 }

@@ -155,6 +155,41 @@ __device__ __forceinline__ uint64_t shfl_xor_u64(uint64_t v, int mask) {
     return ((uint64_t)hi << 32) | lo;
 }
 
+// One item q of an own round in k_sc_round's modes: binds the tables at q and/or adds the
+// pair's node values to acc.
+template <int D, int REL>
+__device__ __forceinline__ void sc_item(const ScTables &t, uint32_t F, uint32_t base_mask, bool first,
+                                        bool bind, uint64_t pairs, const fp2 &r, const fp2 &lambda,
+                                        const ScTerms &terms, uint64_t q, fp2 (&acc)[D + 1]) {
+    fp2 a[SC_MAXF], b[SC_MAXF];
+    for (uint32_t k = 0; k < F; k++) {
+        const bool base = first && ((base_mask >> k) & 1);
+        const uint32_t sh = first ? t.shift[k] : 0, of = first ? t.off[k] : 0;
+        auto at = [&](uint64_t x) { return ld_entry(t.src[k], base, (x << sh) + of); };
+        if (!bind) {
+            a[k] = at(2 * q);
+            b[k] = at(2 * q + 1);
+        } else if (pairs > 0) {
+            fp2 e0 = at(4 * q), e1 = at(4 * q + 1), e2 = at(4 * q + 2), e3 = at(4 * q + 3);
+            a[k] = gl::add(e0, gl::mul(r, gl::sub(e1, e0)));
+            b[k] = gl::add(e2, gl::mul(r, gl::sub(e3, e2)));
+            if (t.dst[k]) {
+                t.dst[k][2 * q] = a[k];
+                t.dst[k][2 * q + 1] = b[k];
+            }
+        } else {
+            fp2 e0 = at(2 * q), e1 = at(2 * q + 1);
+            if (t.dst[k]) t.dst[k][q] = gl::add(e0, gl::mul(r, gl::sub(e1, e0)));
+        }
+    }
+    if (pairs > 0) {
+        fp2 val[D + 1];
+        pair_values<D, REL>(a, b, F, lambda, terms, val);
+#pragma unroll
+        for (int z = 0; z <= D; z++) acc[z] = gl::add(acc[z], val[z]);
+    }
+}
+
 // One own round of one instance: bind every table with r (when out_len > 0) and/or sum the node
 // values over `pairs` pairs (when pairs > 0), writing one partial per block.
 //   sums only:   pairs of the unbound tables (entries 2q, 2q+1)
@@ -174,35 +209,8 @@ k_sc_round(ScTables t, uint32_t F, uint32_t base_mask, bool first, uint64_t out_
 #pragma unroll
     for (int z = 0; z <= D; z++) acc[z] = fp2{0, 0};
 
-    for (uint64_t q = (uint64_t)blockIdx.x * SC_THREADS + threadIdx.x; q < items; q += stride) {
-        fp2 a[SC_MAXF], b[SC_MAXF];
-        for (uint32_t k = 0; k < F; k++) {
-            const bool base = first && ((base_mask >> k) & 1);
-            const uint32_t sh = first ? t.shift[k] : 0, of = first ? t.off[k] : 0;
-            auto at = [&](uint64_t x) { return ld_entry(t.src[k], base, (x << sh) + of); };
-            if (!bind) {
-                a[k] = at(2 * q);
-                b[k] = at(2 * q + 1);
-            } else if (pairs > 0) {
-                fp2 e0 = at(4 * q), e1 = at(4 * q + 1), e2 = at(4 * q + 2), e3 = at(4 * q + 3);
-                a[k] = gl::add(e0, gl::mul(r, gl::sub(e1, e0)));
-                b[k] = gl::add(e2, gl::mul(r, gl::sub(e3, e2)));
-                if (t.dst[k]) {
-                    t.dst[k][2 * q] = a[k];
-                    t.dst[k][2 * q + 1] = b[k];
-                }
-            } else {
-                fp2 e0 = at(2 * q), e1 = at(2 * q + 1);
-                if (t.dst[k]) t.dst[k][q] = gl::add(e0, gl::mul(r, gl::sub(e1, e0)));
-            }
-        }
-        if (pairs > 0) {
-            fp2 val[D + 1];
-            pair_values<D, REL>(a, b, F, lambda, terms, val);
-#pragma unroll
-            for (int z = 0; z <= D; z++) acc[z] = gl::add(acc[z], val[z]);
-        }
-    }
+    for (uint64_t q = (uint64_t)blockIdx.x * SC_THREADS + threadIdx.x; q < items; q += stride)
+        sc_item<D, REL>(t, F, base_mask, first, bind, pairs, r, lambda, terms, q, acc);
     if (pairs == 0) return;
 
     // wave reduction, then across the block's waves through LDS
@@ -225,6 +233,101 @@ k_sc_round(ScTables t, uint32_t F, uint32_t base_mask, bool first, uint64_t out_
     }
 }
 
+// One item q of a level round in k_sc_round_eq's modes: binds the tables at q and/or adds E F
+// at z = 0, 1, 2 to acc.
+template <int REL, bool UNIT, bool BASE>
+__device__ __forceinline__ void sc_item_eq(const ScTables &t, uint32_t base_mask, bool first,
+                                           bool bind, uint64_t pairs, const fp2 &r,
+                                           const fp2 &lambda, const fp2 *__restrict__ eq_lo,
+                                           const fp2 *__restrict__ eq_hi, uint32_t eq_a,
+                                           uint64_t q, fp2 (&acc)[3]) {
+    constexpr int F = REL == REL_PRODUCT ? 2 : 4;
+    constexpr int K0 = (REL == REL_FRACTION && UNIT) ? 2 : 0;
+    auto line2 = [](const fp2 &a, const fp2 &b) { return gl::sub(gl::add(b, b), a); };
+    fp2 a[F], b[F];
+    if (BASE) {
+        const uint64_t *s0 = (const uint64_t *)t.src[0], *s1 = (const uint64_t *)t.src[1];
+        auto at0 = [&](uint64_t x) { return s0[(x << t.shift[0]) + t.off[0]]; };
+        auto at1 = [&](uint64_t x) { return s1[(x << t.shift[1]) + t.off[1]]; };
+        // e0 + r (e1 - e0) for base e0, e1
+        auto bindb = [&](uint64_t e0, uint64_t e1) {
+            fp2 v = gl::mul_base(r, gl::sub(e1, e0));
+            v.c0 = gl::add(v.c0, e0);
+            return v;
+        };
+        if (!bind) {
+            const uint64_t a0 = at0(2 * q), b0 = at0(2 * q + 1), a1 = at1(2 * q), b1 = at1(2 * q + 1);
+            const fp2 e = gl::mul(eq_lo[q & ((1ull << eq_a) - 1)], eq_hi[q >> eq_a]);
+            const uint64_t f0 = gl::mul(a0, a1), f1 = gl::mul(b0, b1);
+            const uint64_t f2 = gl::mul(gl::sub(gl::add(b0, b0), a0), gl::sub(gl::add(b1, b1), a1));
+            acc[0] = gl::add(acc[0], gl::mul_base(e, f0));
+            acc[1] = gl::add(acc[1], gl::mul_base(e, f1));
+            acc[2] = gl::add(acc[2], gl::mul_base(e, f2));
+            return;
+        }
+        if (pairs == 0) {
+            t.dst[0][q] = bindb(at0(2 * q), at0(2 * q + 1));
+            t.dst[1][q] = bindb(at1(2 * q), at1(2 * q + 1));
+            return;
+        }
+        a[0] = bindb(at0(4 * q), at0(4 * q + 1));
+        b[0] = bindb(at0(4 * q + 2), at0(4 * q + 3));
+        a[1] = bindb(at1(4 * q), at1(4 * q + 1));
+        b[1] = bindb(at1(4 * q + 2), at1(4 * q + 3));
+#pragma unroll
+        for (int k = 0; k < 2; k++) {
+            t.dst[k][2 * q] = a[k];
+            t.dst[k][2 * q + 1] = b[k];
+        }
+    }
+#pragma unroll
+    for (int k = K0; k < F; k++) {
+        if (BASE) break;
+        const bool base = first && ((base_mask >> k) & 1);
+        const uint32_t sh = first ? t.shift[k] : 0, of = first ? t.off[k] : 0;
+        auto at = [&](uint64_t x) { return ld_entry(t.src[k], base, (x << sh) + of); };
+        if (!bind) {
+            a[k] = at(2 * q);
+            b[k] = at(2 * q + 1);
+        } else if (pairs > 0) {
+            fp2 e0 = at(4 * q), e1 = at(4 * q + 1), e2 = at(4 * q + 2), e3 = at(4 * q + 3);
+            a[k] = gl::add(e0, gl::mul(r, gl::sub(e1, e0)));
+            b[k] = gl::add(e2, gl::mul(r, gl::sub(e3, e2)));
+            t.dst[k][2 * q] = a[k];
+            t.dst[k][2 * q + 1] = b[k];
+        } else {
+            fp2 e0 = at(2 * q), e1 = at(2 * q + 1);
+            t.dst[k][q] = gl::add(e0, gl::mul(r, gl::sub(e1, e0)));
+        }
+    }
+    if (pairs == 0) return;
+    const fp2 e = gl::mul(eq_lo[q & ((1ull << eq_a) - 1)], eq_hi[q >> eq_a]);
+    fp2 v0, v1, v2;
+    if (REL == REL_PRODUCT) {
+        const fp2 x0 = gl::mul(e, a[0]), x1 = gl::mul(e, b[0]);
+        v0 = gl::mul(x0, a[1]);
+        v1 = gl::mul(x1, b[1]);
+        v2 = gl::mul(line2(x0, x1), line2(a[1], b[1]));
+    } else if (UNIT) {
+        const fp2 u = gl::mul(lambda, e);
+        const fp2 s0 = gl::mul(e, gl::add(a[2], a[3])), s1 = gl::mul(e, gl::add(b[2], b[3]));
+        const fp2 p0 = gl::mul(u, a[2]), p1 = gl::mul(u, b[2]);
+        v0 = gl::add(s0, gl::mul(p0, a[3]));
+        v1 = gl::add(s1, gl::mul(p1, b[3]));
+        v2 = gl::add(line2(s0, s1), gl::mul(line2(p0, p1), line2(a[3], b[3])));
+    } else {
+        const fp2 n0 = gl::mul(e, a[0]), n1 = gl::mul(e, b[0]);
+        const fp2 d0 = gl::mul(e, a[2]), d1 = gl::mul(e, b[2]);
+        const fp2 w0 = gl::add(a[1], gl::mul(lambda, a[3])), w1 = gl::add(b[1], gl::mul(lambda, b[3]));
+        v0 = gl::add(gl::mul(n0, a[3]), gl::mul(d0, w0));
+        v1 = gl::add(gl::mul(n1, b[3]), gl::mul(d1, w1));
+        v2 = gl::add(gl::mul(line2(n0, n1), line2(a[3], b[3])), gl::mul(line2(d0, d1), line2(w0, w1)));
+    }
+    acc[0] = gl::add(acc[0], v0);
+    acc[1] = gl::add(acc[1], v1);
+    acc[2] = gl::add(acc[2], v2);
+}
+
 // One own round of a level instance whose relation is eq(r, x) F(x) with eq a factor, not a
 // table (degree 3). Modes as k_sc_round's. Pair q of the round is weighted by
 // E[q] = eq_lo[q mod 2^eq_a] eq_hi[q >> eq_a], the eq of the point on the variables above the one
@@ -244,99 +347,14 @@ __global__ void __launch_bounds__(SC_THREADS)
 k_sc_round_eq(ScTables t, uint32_t base_mask, bool first, uint64_t out_len, uint64_t pairs,
               const fp2 *__restrict__ r_ptr, fp2 lambda, fp2 *__restrict__ partials,
               const fp2 *__restrict__ eq_lo, const fp2 *__restrict__ eq_hi, uint32_t eq_a) {
-    constexpr int F = REL == REL_PRODUCT ? 2 : 4;
-    constexpr int K0 = (REL == REL_FRACTION && UNIT) ? 2 : 0;
     const bool bind = out_len > 0;
     const fp2 r = bind ? *r_ptr : fp2{0, 0};
     const uint64_t items = pairs > 0 ? pairs : out_len;
     const uint64_t stride = (uint64_t)gridDim.x * SC_THREADS;
     fp2 acc[3] = {{0, 0}, {0, 0}, {0, 0}};
-    auto line2 = [](const fp2 &a, const fp2 &b) { return gl::sub(gl::add(b, b), a); };
 
-    for (uint64_t q = (uint64_t)blockIdx.x * SC_THREADS + threadIdx.x; q < items; q += stride) {
-        fp2 a[F], b[F];
-        if (BASE) {
-            const uint64_t *s0 = (const uint64_t *)t.src[0], *s1 = (const uint64_t *)t.src[1];
-            auto at0 = [&](uint64_t x) { return s0[(x << t.shift[0]) + t.off[0]]; };
-            auto at1 = [&](uint64_t x) { return s1[(x << t.shift[1]) + t.off[1]]; };
-            // e0 + r (e1 - e0) for base e0, e1
-            auto bindb = [&](uint64_t e0, uint64_t e1) {
-                fp2 v = gl::mul_base(r, gl::sub(e1, e0));
-                v.c0 = gl::add(v.c0, e0);
-                return v;
-            };
-            if (!bind) {
-                const uint64_t a0 = at0(2 * q), b0 = at0(2 * q + 1), a1 = at1(2 * q), b1 = at1(2 * q + 1);
-                const fp2 e = gl::mul(eq_lo[q & ((1ull << eq_a) - 1)], eq_hi[q >> eq_a]);
-                const uint64_t f0 = gl::mul(a0, a1), f1 = gl::mul(b0, b1);
-                const uint64_t f2 = gl::mul(gl::sub(gl::add(b0, b0), a0), gl::sub(gl::add(b1, b1), a1));
-                acc[0] = gl::add(acc[0], gl::mul_base(e, f0));
-                acc[1] = gl::add(acc[1], gl::mul_base(e, f1));
-                acc[2] = gl::add(acc[2], gl::mul_base(e, f2));
-                continue;
-            }
-            if (pairs == 0) {
-                t.dst[0][q] = bindb(at0(2 * q), at0(2 * q + 1));
-                t.dst[1][q] = bindb(at1(2 * q), at1(2 * q + 1));
-                continue;
-            }
-            a[0] = bindb(at0(4 * q), at0(4 * q + 1));
-            b[0] = bindb(at0(4 * q + 2), at0(4 * q + 3));
-            a[1] = bindb(at1(4 * q), at1(4 * q + 1));
-            b[1] = bindb(at1(4 * q + 2), at1(4 * q + 3));
-#pragma unroll
-            for (int k = 0; k < 2; k++) {
-                t.dst[k][2 * q] = a[k];
-                t.dst[k][2 * q + 1] = b[k];
-            }
-        }
-#pragma unroll
-        for (int k = K0; k < F; k++) {
-            if (BASE) break;
-            const bool base = first && ((base_mask >> k) & 1);
-            const uint32_t sh = first ? t.shift[k] : 0, of = first ? t.off[k] : 0;
-            auto at = [&](uint64_t x) { return ld_entry(t.src[k], base, (x << sh) + of); };
-            if (!bind) {
-                a[k] = at(2 * q);
-                b[k] = at(2 * q + 1);
-            } else if (pairs > 0) {
-                fp2 e0 = at(4 * q), e1 = at(4 * q + 1), e2 = at(4 * q + 2), e3 = at(4 * q + 3);
-                a[k] = gl::add(e0, gl::mul(r, gl::sub(e1, e0)));
-                b[k] = gl::add(e2, gl::mul(r, gl::sub(e3, e2)));
-                t.dst[k][2 * q] = a[k];
-                t.dst[k][2 * q + 1] = b[k];
-            } else {
-                fp2 e0 = at(2 * q), e1 = at(2 * q + 1);
-                t.dst[k][q] = gl::add(e0, gl::mul(r, gl::sub(e1, e0)));
-            }
-        }
-        if (pairs == 0) continue;
-        const fp2 e = gl::mul(eq_lo[q & ((1ull << eq_a) - 1)], eq_hi[q >> eq_a]);
-        fp2 v0, v1, v2;
-        if (REL == REL_PRODUCT) {
-            const fp2 x0 = gl::mul(e, a[0]), x1 = gl::mul(e, b[0]);
-            v0 = gl::mul(x0, a[1]);
-            v1 = gl::mul(x1, b[1]);
-            v2 = gl::mul(line2(x0, x1), line2(a[1], b[1]));
-        } else if (UNIT) {
-            const fp2 u = gl::mul(lambda, e);
-            const fp2 s0 = gl::mul(e, gl::add(a[2], a[3])), s1 = gl::mul(e, gl::add(b[2], b[3]));
-            const fp2 p0 = gl::mul(u, a[2]), p1 = gl::mul(u, b[2]);
-            v0 = gl::add(s0, gl::mul(p0, a[3]));
-            v1 = gl::add(s1, gl::mul(p1, b[3]));
-            v2 = gl::add(line2(s0, s1), gl::mul(line2(p0, p1), line2(a[3], b[3])));
-        } else {
-            const fp2 n0 = gl::mul(e, a[0]), n1 = gl::mul(e, b[0]);
-            const fp2 d0 = gl::mul(e, a[2]), d1 = gl::mul(e, b[2]);
-            const fp2 w0 = gl::add(a[1], gl::mul(lambda, a[3])), w1 = gl::add(b[1], gl::mul(lambda, b[3]));
-            v0 = gl::add(gl::mul(n0, a[3]), gl::mul(d0, w0));
-            v1 = gl::add(gl::mul(n1, b[3]), gl::mul(d1, w1));
-            v2 = gl::add(gl::mul(line2(n0, n1), line2(a[3], b[3])), gl::mul(line2(d0, d1), line2(w0, w1)));
-        }
-        acc[0] = gl::add(acc[0], v0);
-        acc[1] = gl::add(acc[1], v1);
-        acc[2] = gl::add(acc[2], v2);
-    }
+    for (uint64_t q = (uint64_t)blockIdx.x * SC_THREADS + threadIdx.x; q < items; q += stride)
+        sc_item_eq<REL, UNIT, BASE>(t, base_mask, first, bind, pairs, r, lambda, eq_lo, eq_hi, eq_a, q, acc);
     if (pairs == 0) return;
 
     __shared__ fp2 red[SC_THREADS / 64][3];
@@ -545,6 +563,258 @@ k_sc_transcript(uint32_t j, const ScRoundInst *__restrict__ ri, ScInstState *__r
     for (uint32_t i = threadIdx.x + SC_THREADS; i < count; i += SC_THREADS) fold(i, ri[i], ist[i].eq_k);
 }
 
+// ---------------------------------------------------------------------------------------------
+// The tail of a batch: once every remaining round has at most SC_TAIL_PAIRS pairs per instance,
+// one workgroup runs all of them (the message of the first from the partials of the grid
+// launches, then per round: the instances' bind + sums in the block, the shares, the hashes on
+// wave 0, the eq folds) and the final binds. Nothing leaves the workgroup between rounds: no
+// kernel boundary and no trip through another XCD's cache per round.
+
+constexpr uint32_t SC_TAIL_LOG = 8;
+constexpr uint64_t SC_TAIL_PAIRS = 1ull << SC_TAIL_LOG;
+constexpr int SC_TAIL_THREADS = 512;  // the instances' items run side by side in thread groups
+constexpr uint32_t SC_TAIL_MAX = 8;   // instances a tail takes (its steps are staged in LDS)
+
+enum ScTailKind : uint32_t {
+    TAIL_NONE = 0,        // padding round (or an instance on no variable at the final bind)
+    TAIL_PRODUCT,         // k_sc_round<D, REL_PRODUCT>
+    TAIL_TERMS,           // k_sc_round<D, REL_TERMS>
+    TAIL_FRACTION,        // k_sc_round<3, REL_FRACTION> (an eq table as table 0)
+    TAIL_EQ_PRODUCT,      // k_sc_round_eq<REL_PRODUCT, false, false>
+    TAIL_EQ_PRODUCT_BASE, // k_sc_round_eq<REL_PRODUCT, false, true>
+    TAIL_EQ_UNIT,         // k_sc_round_eq<REL_FRACTION, true, false>
+    TAIL_EQ_FRACTION,     // k_sc_round_eq<REL_FRACTION, false, false>
+};
+
+// One instance's work in one tail round (or the final bind): as the grid launch would do it.
+struct ScTailStep {
+    ScTables t;
+    const fp2 *eq_lo, *eq_hi;
+    uint32_t eq_a, kind, F, base_mask;
+    uint32_t first, pad_;
+    uint64_t out_len, pairs;
+};
+
+static_assert(sizeof(ScTailStep) % 16 == 0, "tail steps are staged in LDS as 16-byte words");
+
+// This thread's items of one step, summed into acc (slots 0, 2..D, 1).
+template <int D>
+__device__ __forceinline__ void tail_items(const ScTailStep &sp, const ScTerms &terms, const fp2 &r,
+                                           const fp2 &lambda, uint32_t q0, uint32_t stride,
+                                           fp2 (&acc)[D + 1]) {
+    const bool bind = sp.out_len > 0, first = sp.first != 0;
+    const uint64_t items = sp.pairs > 0 ? sp.pairs : sp.out_len;
+    switch (sp.kind) {
+    case TAIL_PRODUCT:
+        for (uint64_t q = q0; q < items; q += stride)
+            sc_item<D, REL_PRODUCT>(sp.t, sp.F, sp.base_mask, first, bind, sp.pairs, r, lambda, terms, q, acc);
+        return;
+    case TAIL_TERMS:
+        for (uint64_t q = q0; q < items; q += stride)
+            sc_item<D, REL_TERMS>(sp.t, sp.F, sp.base_mask, first, bind, sp.pairs, r, lambda, terms, q, acc);
+        return;
+    default:
+        break;
+    }
+    if constexpr (D == 3) {
+        fp2 a3[3] = {{0, 0}, {0, 0}, {0, 0}};
+        switch (sp.kind) {
+        case TAIL_FRACTION:
+            for (uint64_t q = q0; q < items; q += stride)
+                sc_item<3, REL_FRACTION>(sp.t, sp.F, sp.base_mask, first, bind, sp.pairs, r, lambda, terms, q, acc);
+            return;
+        case TAIL_EQ_PRODUCT:
+            for (uint64_t q = q0; q < items; q += stride)
+                sc_item_eq<REL_PRODUCT, false, false>(sp.t, sp.base_mask, first, bind, sp.pairs, r, lambda, sp.eq_lo, sp.eq_hi, sp.eq_a, q, a3);
+            break;
+        case TAIL_EQ_PRODUCT_BASE:
+            for (uint64_t q = q0; q < items; q += stride)
+                sc_item_eq<REL_PRODUCT, false, true>(sp.t, sp.base_mask, first, bind, sp.pairs, r, lambda, sp.eq_lo, sp.eq_hi, sp.eq_a, q, a3);
+            break;
+        case TAIL_EQ_UNIT:
+            for (uint64_t q = q0; q < items; q += stride)
+                sc_item_eq<REL_FRACTION, true, false>(sp.t, sp.base_mask, first, bind, sp.pairs, r, lambda, sp.eq_lo, sp.eq_hi, sp.eq_a, q, a3);
+            break;
+        case TAIL_EQ_FRACTION:
+            for (uint64_t q = q0; q < items; q += stride)
+                sc_item_eq<REL_FRACTION, false, false>(sp.t, sp.base_mask, first, bind, sp.pairs, r, lambda, sp.eq_lo, sp.eq_hi, sp.eq_a, q, a3);
+            break;
+        default:
+            return;
+        }
+        // E F is quadratic in the bound variable: node 3 = f(0) + 3 (f(2) - f(1)) (linear, so
+        // per thread as well as per block)
+        const fp2 d = gl::sub(a3[2], a3[1]);
+        acc[0] = gl::add(acc[0], a3[0]);
+        acc[1] = gl::add(acc[1], a3[2]);
+        acc[2] = gl::add(acc[2], gl::add(a3[0], gl::add(gl::add(d, d), d)));
+        acc[3] = gl::add(acc[3], a3[1]);
+    }
+}
+
+// One workgroup of SC_TAIL_THREADS: instance i's items run on the thread group
+// [i G, (i + 1) G) (G = SC_TAIL_THREADS / the power of two >= count), so the instances' chains of
+// products overlap; the steps are staged in LDS at the start.
+template <int D>
+__global__ void __launch_bounds__(SC_TAIL_THREADS)
+k_sc_tail(uint32_t j0, uint32_t n_max, const ScRoundInst *__restrict__ rtab,
+          ScInstState *__restrict__ ist, uint32_t count, uint32_t nblk_max0,
+          const fp2 *__restrict__ partials, const ScTailStep *__restrict__ steps_g,
+          const ScTerms *__restrict__ terms, fp2 lambda, ScCtl *__restrict__ ctl, DevKeys keys,
+          const uint8_t *__restrict__ labels, const uint32_t *__restrict__ label_lens,
+          fp2 *__restrict__ rounds_out, fp2 *__restrict__ chals_out) {
+    extern __shared__ uint4 dyn[];  // the steps, then count x (D+1) sums
+    constexpr int WAVES = SC_TAIL_THREADS / 64;
+    __shared__ ScRoundInst sri[SC_TAIL_MAX];
+    __shared__ ScInstState sis[SC_TAIL_MAX];
+    __shared__ uint32_t hbuf[DEV_BUF_WORDS];
+    __shared__ uint32_t lab[2 * LABEL_SLOT / 4];
+    __shared__ uint32_t st[8], outw[8];
+    __shared__ fp2 msg[D];
+    __shared__ fp2 chal;
+    __shared__ WaveKeys wk;
+    __shared__ fp2 red[WAVES][D + 1];
+    const uint32_t n_steps = (n_max - j0) * count;
+    ScTailStep *steps = (ScTailStep *)dyn;
+    fp2 *sums = (fp2 *)(steps + n_steps);
+    for (uint32_t w = threadIdx.x; w < n_steps * (sizeof(ScTailStep) / 16); w += SC_TAIL_THREADS)
+        dyn[w] = ((const uint4 *)steps_g)[w];
+    const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const uint32_t t_word = ((const uint32_t *)&ctl->t)[lane % 10];
+    if (wave == 0 && lane < 8) st[lane] = t_word;
+    if (wave == 0 && lane < 24) (&wk.absorb[0])[lane] = (&keys.absorb[0])[lane];
+    if (threadIdx.x < count) sis[threadIdx.x] = ist[threadIdx.x];
+    uint64_t counter = ((uint64_t)(uint32_t)__builtin_amdgcn_readlane((int)t_word, 9) << 32) |
+                       (uint32_t)__builtin_amdgcn_readlane((int)t_word, 8);
+    // the thread groups: G threads (whole waves) per instance
+    uint32_t groups = 1;
+    while (groups < count) groups <<= 1;
+    const uint32_t G = SC_TAIL_THREADS / groups, gi = threadIdx.x / G, gt = threadIdx.x % G;
+    const uint32_t gw = G / 64;  // waves per group
+    fp2 r{0, 0};
+    for (uint32_t j = j0; j < n_max; j++) {
+        const ScRoundInst *ri = rtab + (size_t)j * count;
+        if (threadIdx.x < count) sri[threadIdx.x] = ri[threadIdx.x];
+        if (threadIdx.x < 2 * LABEL_SLOT / 4)
+            lab[threadIdx.x] = ((const uint32_t *)(labels + (size_t)2 * j * LABEL_SLOT))[threadIdx.x];
+        const uint32_t len_word = label_lens[2 * j + (lane & 1)];
+        if (j == j0) {
+            __syncthreads();  // the staged steps
+            // the grid launches' partials, as k_sc_transcript reduces them
+            for (uint32_t x = wave; x < count * (D + 1); x += WAVES) {
+                const uint32_t i = x / (D + 1), z = x % (D + 1);
+                const fp2 *p = partials + (uint64_t)i * SC_MAX_BLOCKS * (D + 1) + z;
+                const uint32_t nb = ri[i].nblk;
+                fp2 s{0, 0};
+                for (uint32_t b = lane; b < nblk_max0; b += 64)
+                    if (b < nb) s = gl::add(s, p[(uint64_t)b * (D + 1)]);
+#pragma unroll
+                for (int off = 32; off > 0; off >>= 1) {
+                    fp2 o;
+                    o.c0 = __shfl_xor(s.c0, off, 64);
+                    o.c1 = __shfl_xor(s.c1, off, 64);
+                    s = gl::add(s, o);
+                }
+                if (lane == 0) sums[x] = s;
+            }
+        } else {
+            // bind at r_{j-1} and the sums of round j: instance gi on this thread's group
+            fp2 acc[D + 1];
+#pragma unroll
+            for (int z = 0; z <= D; z++) acc[z] = fp2{0, 0};
+            if (gi < count)
+                tail_items<D>(steps[(size_t)(j - j0 - 1) * count + gi], terms[gi], r, lambda, gt, G, acc);
+#pragma unroll
+            for (int z = 0; z <= D; z++) {
+                fp2 v = acc[z];
+                for (int m = 32; m >= 1; m >>= 1) {
+                    fp2 o{shfl_xor_u64(v.c0, m), shfl_xor_u64(v.c1, m)};
+                    v = gl::add(v, o);
+                }
+                if (lane == 0) red[wave][z] = v;
+            }
+            __syncthreads();
+            for (uint32_t x = threadIdx.x; x < count * (D + 1); x += SC_TAIL_THREADS) {
+                const uint32_t i = x / (D + 1), z = x % (D + 1);
+                fp2 v{0, 0};
+                for (uint32_t w = i * gw; w < (i + 1) * gw; w++) v = gl::add(v, red[w][z]);
+                sums[x] = v;
+            }
+        }
+        __syncthreads();
+        // S_i of padded instances without a claim (round 0 only), then the shares in place
+        if (threadIdx.x < count) {
+            const uint32_t i = threadIdx.x;
+            const ScRoundInst &in = sri[i];
+            if (j == 0 && in.pad > 0 && !in.wsum_given) {
+                const fp2 *s = sums + (uint64_t)i * (D + 1);
+                sis[i].wsum = gl::mul(in.weight, gl::add(s[0], s[D]));
+            }
+        }
+        __syncthreads();
+        for (uint32_t x = threadIdx.x; x < count * D; x += SC_TAIL_THREADS) {
+            const uint32_t i = x / D, z = x % D;
+            const ScRoundInst &in = sri[i];
+            fp2 *s = sums + (uint64_t)i * (D + 1) + z;
+            if (j < in.pad) {
+                *s = gl::mul_base(sis[i].wsum, 1ull << (in.pad - j - 1));
+            } else {
+                fp2 f = in.weight;
+                if (in.eqf) {
+                    const fp2 l0 = gl::sub(fp2{1, 0}, in.rt), step = gl::sub(in.rt, l0);
+                    const uint32_t node = z == 0 ? 0 : z + 1;
+                    f = gl::mul(f, gl::mul(sis[i].eq_k, gl::add(l0, gl::mul_base(step, node))));
+                }
+                *s = gl::mul(*s, f);
+            }
+        }
+        __syncthreads();
+        if (wave == 0) {
+            if (lane < D) {
+                fp2 m{0, 0};
+                for (uint32_t i = 0; i < count; i++) m = gl::add(m, sums[(uint64_t)i * (D + 1) + lane]);
+                msg[lane] = m;
+                rounds_out[(uint64_t)j * D + lane] = m;
+            }
+            wave_sync();
+            const uint32_t len_absorb = (uint32_t)__builtin_amdgcn_readlane((int)len_word, 0);
+            const uint32_t len_squeeze = (uint32_t)__builtin_amdgcn_readlane((int)len_word, 1);
+            const uint8_t *lb = (const uint8_t *)lab;
+            wave_absorb_fp2s(st, wk, lb, len_absorb, msg, D, hbuf);
+            const fp2 c = wave_challenge_fp2(st, counter, wk, lb + LABEL_SLOT, len_squeeze, hbuf, outw);
+            if (lane == 0) {
+                chal = c;
+                chals_out[j] = c;
+            }
+        }
+        __syncthreads();
+        r = chal;
+        if (threadIdx.x < count) {
+            const ScRoundInst &in = sri[threadIdx.x];
+            if (in.eqf && j >= in.pad) {
+                const fp2 e = gl::add(gl::mul(in.rt, r), gl::mul(gl::sub(fp2{1, 0}, in.rt), gl::sub(fp2{1, 0}, r)));
+                sis[threadIdx.x].eq_k = gl::mul(sis[threadIdx.x].eq_k, e);
+            }
+        }
+        __syncthreads();
+    }
+    // the last binds: every instance's last own round at r_{n_max-1}
+    if (gi < count) {
+        fp2 acc[D + 1];
+#pragma unroll
+        for (int z = 0; z <= D; z++) acc[z] = fp2{0, 0};
+        tail_items<D>(steps[(size_t)(n_max - 1 - j0) * count + gi], terms[gi], r, lambda, gt, G, acc);
+    }
+    if (threadIdx.x < count) ist[threadIdx.x] = sis[threadIdx.x];
+    if (wave == 0) {
+        if (lane < 8) ctl->t.state[lane] = st[lane];
+        if (lane == 0) {
+            ctl->t.counter = counter;
+            ctl->r = r;
+        }
+    }
+}
+
 // Gathers each final evaluation: entry off[e] of the buffer at ptrs[e] (base or Fp2).
 __global__ void k_sc_gather(const void *const *__restrict__ ptrs, const uint32_t *__restrict__ base,
                             const uint32_t *__restrict__ off, uint32_t n, fp2 *__restrict__ out) {
@@ -646,6 +916,25 @@ static void launch_transcript_d(uint32_t d, hipStream_t s, uint32_t j, const ScR
     case D:                                                                                   \
         hipLaunchKernelGGL(k_sc_transcript<D>, dim3(1), dim3(SC_THREADS), shm, s, j, ri, ist,  \
                            count, nblk_max, partials, ctl, keys, labels, lens, rounds, chals); \
+        break;
+        CASE(1) CASE(2) CASE(3) CASE(4) CASE(5) CASE(6) CASE(7) CASE(8) CASE(9) CASE(10)
+#undef CASE
+    }
+}
+
+static void launch_tail_d(uint32_t d, hipStream_t s, uint32_t j0, uint32_t n_max,
+                          const ScRoundInst *rtab, ScInstState *ist, uint32_t count,
+                          uint32_t nblk_max0, const fp2 *partials, const ScTailStep *steps,
+                          const ScTerms *terms, fp2 lambda, ScCtl *ctl, const DevKeys &keys,
+                          const uint8_t *labels, const uint32_t *lens, fp2 *rounds, fp2 *chals) {
+    size_t shm = sizeof(ScTailStep) * (size_t)(n_max - j0) * count + sizeof(fp2) * count * (d + 1);
+    switch (d) {
+#define CASE(D)                                                                               \
+    case D:                                                                                   \
+        hipLaunchKernelGGL(k_sc_tail<D>, dim3(1), dim3(SC_TAIL_THREADS), shm, s, j0, n_max,    \
+                           rtab, ist, count, nblk_max0, partials, steps, terms, lambda, ctl,   \
+                           keys,                                                               \
+                           labels, lens, rounds, chals);                                       \
         break;
         CASE(1) CASE(2) CASE(3) CASE(4) CASE(5) CASE(6) CASE(7) CASE(8) CASE(9) CASE(10)
 #undef CASE
@@ -863,6 +1152,12 @@ Status run_batch(Ctx *ctx, Transcript &tr, const std::string &p, uint32_t D,
     }
     const size_t nf = fptr.size();
 
+    // The tail: from round j0 on, every round has at most SC_TAIL_PAIRS pairs per instance
+    // (round j has 2^(n_max - j - 1) in every instance it binds), and one workgroup runs it all.
+    const bool tail = count <= SC_TAIL_MAX && n_max >= 1;
+    const uint32_t j0 = !tail ? n_max : (n_max >= SC_TAIL_LOG + 2 ? n_max - SC_TAIL_LOG - 2 : 0);
+    const size_t n_steps = tail ? (size_t)(n_max - j0) * count : 0;
+
     // Everything the rounds read from the host goes up in one copy and everything the host reads
     // back comes down in one: one device block laid out as
     //   [eq points | eq segments | labels | label lengths | nblk | instances | finals' sources]
@@ -879,6 +1174,8 @@ Status run_batch(Ctx *ctx, Transcript &tr, const std::string &p, uint32_t D,
     const size_t o_labels = place(labels.size()), o_lens = place(4 * lens.size());
     const size_t o_rtab = place(sizeof(ScRoundInst) * rtab.size()), o_state = place(sizeof(ScInstState) * count);
     const size_t o_fptr = place(8 * nf), o_fbase = place(4 * nf), o_foff = place(4 * nf);
+    const size_t o_steps = place(sizeof(ScTailStep) * n_steps);
+    const size_t o_terms = place(tail ? sizeof(ScTerms) * count : 0);
     const size_t o_ctl = place(sizeof(ScCtl));
     const size_t up_end = at;
     const size_t o_rounds = place(16 * (size_t)nr * D), o_chals = place(16 * (size_t)nr);
@@ -903,38 +1200,6 @@ Status run_batch(Ctx *ctx, Transcript &tr, const std::string &p, uint32_t D,
                 splits[i].push_back(EqSplit{d_eqtab.ptr + segs[g].off, d_eqtab.ptr + segs[g + 1].off, segs[g].n});
         }
     }
-    // one pinned host block for both copies: the upload image, then the readback
-    uint8_t *up;
-    if ((st = pinned_host(ctx, std::max(up_end, at - o_ctl), &up))) return st;
-    memset(up, 0, up_end);
-    auto put = [&](size_t o, const void *src, size_t bytes) {
-        if (bytes) memcpy(up + o, src, bytes);
-    };
-    put(o_pts, eq_pts.data(), 16 * eq_pts.size());
-    put(o_segs, segs.data(), sizeof(EqSeg) * segs.size());
-    put(o_labels, labels.data(), labels.size());
-    put(o_lens, lens.data(), 4 * lens.size());
-    put(o_rtab, rtab.data(), sizeof(ScRoundInst) * rtab.size());
-    put(o_state, hstate.data(), sizeof(ScInstState) * count);
-    put(o_fptr, fptr.data(), 8 * nf);
-    put(o_fbase, fbase.data(), 4 * nf);
-    put(o_foff, foff.data(), 4 * nf);
-    put(o_ctl, &hctl, sizeof(ScCtl));
-    TSR_HIP(hipMemcpyAsync(dev, up, up_end, hipMemcpyHostToDevice, s));
-    const DevKeys keys = dev_keys(ctx);
-
-    if (!segs.empty()) {
-        uint32_t widest = 0;
-        for (auto &g : segs) widest = std::max(widest, g.n);
-        const unsigned gx = (unsigned)std::min<uint64_t>(((1ull << widest) + 255) / 256, 1024);
-        for (size_t g0 = 0; g0 < segs.size(); g0 += 65535) {
-            const unsigned ny = (unsigned)std::min<size_t>(65535, segs.size() - g0);
-            hipLaunchKernelGGL(k_eq_segments, dim3(gx, ny), dim3(256), 0, s, d_eqpt, d_segs + g0,
-                               d_eqtab.ptr);
-        }
-        if ((st = check_launch("eq segments"))) return st;
-    }
-
     // own round t binds from (t == 0: original, odd t: a, even t: b) into (even t: a, odd: b)
     auto tables_of = [&](uint32_t i, uint32_t t, ScTables &tt, uint32_t &base_mask) {
         base_mask = 0;
@@ -957,6 +1222,83 @@ Status run_batch(Ctx *ctx, Transcript &tr, const std::string &p, uint32_t D,
         }
     };
 
+    // the tail's steps: for in-block round j (j0 < j < n_max) instance i binds its own round
+    // t - 1 at r_{j-1} and sums own round t = j - pad_i (sums only when t == 0), as the grid
+    // launch after round j - 1 would; the last step binds own round vars_i - 1
+    std::vector<ScTailStep> steps(n_steps);
+    auto kind_of = [&](uint32_t i, bool first, uint32_t bm, const ScTables &tt) -> uint32_t {
+        const GenInst &g = insts[i];
+        if (g.eq_factor) {
+            if (g.rel == REL_PRODUCT) return (first && (bm & 3) == 3) ? TAIL_EQ_PRODUCT_BASE : TAIL_EQ_PRODUCT;
+            return tt.src[0] ? TAIL_EQ_FRACTION : TAIL_EQ_UNIT;
+        }
+        return g.rel == REL_TERMS ? TAIL_TERMS : g.rel == REL_FRACTION ? TAIL_FRACTION : TAIL_PRODUCT;
+    };
+    for (uint32_t j = j0 + 1; tail && j <= n_max; j++) {
+        for (uint32_t i = 0; i < count; i++) {
+            ScTailStep &sp = steps[(size_t)(j - j0 - 1) * count + i];
+            memset(&sp, 0, sizeof sp);
+            const uint32_t v = insts[i].vars;
+            if (v == 0 || j < pad[i]) continue;  // TAIL_NONE
+            const uint32_t t = j - pad[i];       // own round summed (t == v: the last bind only)
+            uint32_t bm = 0;
+            if (t == 0) {
+                tables_of(i, 0, sp.t, bm);
+                sp.first = 1;
+                sp.out_len = 0;
+                sp.pairs = 1ull << (v - 1);
+            } else {
+                tables_of(i, t - 1, sp.t, bm);
+                sp.first = t - 1 == 0;
+                sp.out_len = 1ull << (v - t);
+                sp.pairs = t < v ? sp.out_len / 2 : 0;
+            }
+            sp.F = (uint32_t)insts[i].tabs.size();
+            sp.base_mask = bm;
+            sp.kind = kind_of(i, sp.first != 0, bm, sp.t);
+            if (!splits[i].empty()) {
+                const EqSplit &e = splits[i][std::min<uint32_t>(t, v - 1)];
+                sp.eq_lo = e.lo;
+                sp.eq_hi = e.hi;
+                sp.eq_a = e.a;
+            }
+        }
+    }
+    // one pinned host block for both copies: the upload image, then the readback
+    uint8_t *up;
+    if ((st = pinned_host(ctx, std::max(up_end, at - o_ctl), &up))) return st;
+    memset(up, 0, up_end);
+    auto put = [&](size_t o, const void *src, size_t bytes) {
+        if (bytes) memcpy(up + o, src, bytes);
+    };
+    put(o_pts, eq_pts.data(), 16 * eq_pts.size());
+    put(o_segs, segs.data(), sizeof(EqSeg) * segs.size());
+    put(o_labels, labels.data(), labels.size());
+    put(o_lens, lens.data(), 4 * lens.size());
+    put(o_rtab, rtab.data(), sizeof(ScRoundInst) * rtab.size());
+    put(o_state, hstate.data(), sizeof(ScInstState) * count);
+    put(o_fptr, fptr.data(), 8 * nf);
+    put(o_fbase, fbase.data(), 4 * nf);
+    put(o_foff, foff.data(), 4 * nf);
+    put(o_steps, steps.data(), sizeof(ScTailStep) * n_steps);
+    if (tail) put(o_terms, sterms.data(), sizeof(ScTerms) * count);
+    put(o_ctl, &hctl, sizeof(ScCtl));
+    TSR_HIP(hipMemcpyAsync(dev, up, up_end, hipMemcpyHostToDevice, s));
+    const DevKeys keys = dev_keys(ctx);
+
+    if (!segs.empty()) {
+        uint32_t widest = 0;
+        for (auto &g : segs) widest = std::max(widest, g.n);
+        const unsigned gx = (unsigned)std::min<uint64_t>(((1ull << widest) + 255) / 256, 1024);
+        for (size_t g0 = 0; g0 < segs.size(); g0 += 65535) {
+            const unsigned ny = (unsigned)std::min<size_t>(65535, segs.size() - g0);
+            hipLaunchKernelGGL(k_eq_segments, dim3(gx, ny), dim3(256), 0, s, d_eqpt, d_segs + g0,
+                               d_eqtab.ptr);
+        }
+        if ((st = check_launch("eq segments"))) return st;
+    }
+
+
     {
         KernelTimer timer(ctx, "k_sc_round", s);
         for (uint32_t i = 0; i < count; i++) {
@@ -971,7 +1313,7 @@ Status run_batch(Ctx *ctx, Transcript &tr, const std::string &p, uint32_t D,
                            splits[i].empty() ? EqSplit() : splits[i][0]);
         }
     }
-    for (uint32_t j = 0; j < n_max; j++) {
+    for (uint32_t j = 0; j < j0; j++) {
         {
             KernelTimer timer(ctx, "k_sc_transcript", s);
             launch_transcript_d(D, s, j, d_rtab + (size_t)j * count, d_state, count, nblk_max[j],
@@ -994,6 +1336,13 @@ Status run_batch(Ctx *ctx, Transcript &tr, const std::string &p, uint32_t D,
                            tt, (uint32_t)insts[i].tabs.size(), bm, t == 0, out_len, pairs,
                            &d_ctl->r, lambda, partials.ptr + pstride * i, sterms[i], eq);
         }
+    }
+    if (tail) {
+        KernelTimer timer(ctx, "k_sc_tail", s);
+        launch_tail_d(D, s, j0, n_max, d_rtab, d_state, count, nblk_max[j0], partials.ptr,
+                      (const ScTailStep *)(dev + o_steps), (const ScTerms *)(dev + o_terms), lambda,
+                      d_ctl, keys, dev + o_labels, (const uint32_t *)(dev + o_lens),
+                      (fp2 *)(dev + o_rounds), (fp2 *)(dev + o_chals));
     }
     if ((st = check_launch("sumcheck rounds"))) return st;
     if (nf) {

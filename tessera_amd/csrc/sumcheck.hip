@@ -233,6 +233,10 @@ k_sc_round(ScTables t, uint32_t F, uint32_t base_mask, bool first, uint64_t out_
     }
 }
 
+template <int REL, bool UNIT>
+__device__ __forceinline__ void sc_eq_nodes(const fp2 *a, const fp2 *b, const fp2 &e,
+                                            const fp2 &lambda, fp2 (&acc)[3]);
+
 // One item q of a level round in k_sc_round_eq's modes: binds the tables at q and/or adds E F
 // at z = 0, 1, 2 to acc.
 template <int REL, bool UNIT, bool BASE>
@@ -280,28 +284,60 @@ __device__ __forceinline__ void sc_item_eq(const ScTables &t, uint32_t base_mask
             t.dst[k][2 * q + 1] = b[k];
         }
     }
+    if (!BASE) {
+        // every table's entries are loaded before any bound value is stored (the tables are
+        // separate arrays, but the compiler cannot know it: a store between two tables' loads
+        // would hold the second table's loads back)
+        fp2 ent[F][4];
+        const fp2 e = pairs > 0 ? gl::mul(eq_lo[q & ((1ull << eq_a) - 1)], eq_hi[q >> eq_a]) : fp2{0, 0};
 #pragma unroll
-    for (int k = K0; k < F; k++) {
-        if (BASE) break;
-        const bool base = first && ((base_mask >> k) & 1);
-        const uint32_t sh = first ? t.shift[k] : 0, of = first ? t.off[k] : 0;
-        auto at = [&](uint64_t x) { return ld_entry(t.src[k], base, (x << sh) + of); };
-        if (!bind) {
-            a[k] = at(2 * q);
-            b[k] = at(2 * q + 1);
-        } else if (pairs > 0) {
-            fp2 e0 = at(4 * q), e1 = at(4 * q + 1), e2 = at(4 * q + 2), e3 = at(4 * q + 3);
-            a[k] = gl::add(e0, gl::mul(r, gl::sub(e1, e0)));
-            b[k] = gl::add(e2, gl::mul(r, gl::sub(e3, e2)));
-            t.dst[k][2 * q] = a[k];
-            t.dst[k][2 * q + 1] = b[k];
-        } else {
-            fp2 e0 = at(2 * q), e1 = at(2 * q + 1);
-            t.dst[k][q] = gl::add(e0, gl::mul(r, gl::sub(e1, e0)));
+        for (int k = K0; k < F; k++) {
+            const bool base = first && ((base_mask >> k) & 1);
+            const uint32_t sh = first ? t.shift[k] : 0, of = first ? t.off[k] : 0;
+            auto at = [&](uint64_t x) { return ld_entry(t.src[k], base, (x << sh) + of); };
+            if (bind && pairs > 0) {
+#pragma unroll
+                for (int m = 0; m < 4; m++) ent[k][m] = at(4 * q + m);
+            } else {
+                ent[k][0] = at(2 * q);
+                ent[k][1] = at(2 * q + 1);
+            }
         }
+#pragma unroll
+        for (int k = K0; k < F; k++) {
+            if (!bind) {
+                a[k] = ent[k][0];
+                b[k] = ent[k][1];
+            } else if (pairs > 0) {
+                a[k] = gl::add(ent[k][0], gl::mul(r, gl::sub(ent[k][1], ent[k][0])));
+                b[k] = gl::add(ent[k][2], gl::mul(r, gl::sub(ent[k][3], ent[k][2])));
+            } else {
+                a[k] = gl::add(ent[k][0], gl::mul(r, gl::sub(ent[k][1], ent[k][0])));
+            }
+        }
+#pragma unroll
+        for (int k = K0; k < F; k++) {
+            if (!bind) break;
+            if (pairs > 0) {
+                t.dst[k][2 * q] = a[k];
+                t.dst[k][2 * q + 1] = b[k];
+            } else {
+                t.dst[k][q] = a[k];
+            }
+        }
+        if (pairs == 0) return;
+        sc_eq_nodes<REL, UNIT>(a, b, e, lambda, acc);
+        return;
     }
-    if (pairs == 0) return;
     const fp2 e = gl::mul(eq_lo[q & ((1ull << eq_a) - 1)], eq_hi[q >> eq_a]);
+    sc_eq_nodes<REL, UNIT>(a, b, e, lambda, acc);
+}
+
+// E F(z) at z = 0, 1, 2 of one pair added to acc (see k_sc_round_eq).
+template <int REL, bool UNIT>
+__device__ __forceinline__ void sc_eq_nodes(const fp2 *a, const fp2 *b, const fp2 &e,
+                                            const fp2 &lambda, fp2 (&acc)[3]) {
+    auto line2 = [](const fp2 &x, const fp2 &y) { return gl::sub(gl::add(y, y), x); };
     fp2 v0, v1, v2;
     if (REL == REL_PRODUCT) {
         const fp2 x0 = gl::mul(e, a[0]), x1 = gl::mul(e, b[0]);

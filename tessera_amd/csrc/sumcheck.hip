@@ -633,13 +633,95 @@ struct ScTailStep {
 
 static_assert(sizeof(ScTailStep) % 16 == 0, "tail steps are staged in LDS as 16-byte words");
 
+// A level round's pairs in the tail, eight lanes per pair: lane 2k + h of the pair binds (or
+// loads) half h of table K0 + k and leaves it in wbuf; lanes z = 0, 1, 2 then take the line
+// values at z, E and the relation, so one pair's products are spread over four lanes and no
+// single lane runs the whole chain (a wave executes an instruction in four cycles whatever the
+// number of its active lanes). acc: slots 0, 2, 3, 1 (node 3 = f(0) - 3 f(1) + 3 f(2)).
+template <int REL, bool UNIT>
+__device__ __forceinline__ void tail_eq_wide(const ScTailStep &sp, const fp2 &r, const fp2 &lambda,
+                                             uint32_t gt, uint32_t G, fp2 *wbuf, fp2 (&acc)[4]) {
+    constexpr int F = REL == REL_PRODUCT ? 2 : 4;
+    constexpr int K0 = (REL == REL_FRACTION && UNIT) ? 2 : 0;
+    constexpr uint32_t NV = 2 * (F - K0);
+    const bool bind = sp.out_len > 0, first = sp.first != 0;
+    const uint32_t role = gt & 7, slots = G / 8;
+    fp2 *my = wbuf + (threadIdx.x & ~7u);
+    for (uint64_t q = gt >> 3; q < sp.pairs; q += slots) {
+        if (role < NV) {
+            const uint32_t k = K0 + (role >> 1), h = role & 1;
+            const bool base = first && ((sp.base_mask >> k) & 1);
+            const uint32_t sh = first ? sp.t.shift[k] : 0, of = first ? sp.t.off[k] : 0;
+            const void *src = sp.t.src[k];
+            fp2 v;
+            if (!bind) {
+                v = ld_entry(src, base, ((2 * q + h) << sh) + of);
+            } else {
+                const fp2 e0 = ld_entry(src, base, ((4 * q + 2 * h) << sh) + of);
+                const fp2 e1 = ld_entry(src, base, ((4 * q + 2 * h + 1) << sh) + of);
+                v = gl::add(e0, gl::mul(r, gl::sub(e1, e0)));
+                sp.t.dst[k][2 * q + h] = v;
+            }
+            my[role] = v;
+        }
+        wave_sync();
+        if (role < 3) {
+            fp2 x[F];
+#pragma unroll
+            for (int k = K0; k < F; k++) {
+                const fp2 a = my[2 * (k - K0)], b = my[2 * (k - K0) + 1];
+                x[k] = role == 0 ? a : role == 1 ? b : gl::sub(gl::add(b, b), a);
+            }
+            const fp2 e = gl::mul(sp.eq_lo[q & ((1ull << sp.eq_a) - 1)], sp.eq_hi[q >> sp.eq_a]);
+            fp2 f;
+            if (REL == REL_PRODUCT)
+                f = gl::mul(gl::mul(e, x[0]), x[1]);
+            else if (UNIT)
+                f = gl::mul(e, gl::add(gl::add(x[2], x[3]), gl::mul(lambda, gl::mul(x[2], x[3]))));
+            else
+                f = gl::mul(e, gl::add(gl::add(gl::mul(x[0], x[3]), gl::mul(x[1], x[2])),
+                                       gl::mul(lambda, gl::mul(x[2], x[3]))));
+            const fp2 f3 = gl::add(gl::add(f, f), f);
+            if (role == 0) {
+                acc[0] = gl::add(acc[0], f);
+                acc[2] = gl::add(acc[2], f);
+            } else if (role == 1) {
+                acc[3] = gl::add(acc[3], f);
+                acc[2] = gl::sub(acc[2], f3);
+            } else {
+                acc[1] = gl::add(acc[1], f);
+                acc[2] = gl::add(acc[2], f3);
+            }
+        }
+        wave_sync();  // the pair's values are read before the next pair's overwrite them
+    }
+}
+
 // This thread's items of one step, summed into acc (slots 0, 2..D, 1).
 template <int D>
 __device__ __forceinline__ void tail_items(const ScTailStep &sp, const ScTerms &terms, const fp2 &r,
                                            const fp2 &lambda, uint32_t q0, uint32_t stride,
-                                           fp2 (&acc)[D + 1]) {
+                                           fp2 *wbuf, fp2 (&acc)[D + 1]) {
     const bool bind = sp.out_len > 0, first = sp.first != 0;
     const uint64_t items = sp.pairs > 0 ? sp.pairs : sp.out_len;
+    if constexpr (D == 3) {
+        if (sp.pairs > 0) {  // level rounds with sums: eight lanes per pair
+            switch (sp.kind) {
+            case TAIL_EQ_PRODUCT:
+            case TAIL_EQ_PRODUCT_BASE:
+                tail_eq_wide<REL_PRODUCT, false>(sp, r, lambda, q0, stride, wbuf, acc);
+                return;
+            case TAIL_EQ_UNIT:
+                tail_eq_wide<REL_FRACTION, true>(sp, r, lambda, q0, stride, wbuf, acc);
+                return;
+            case TAIL_EQ_FRACTION:
+                tail_eq_wide<REL_FRACTION, false>(sp, r, lambda, q0, stride, wbuf, acc);
+                return;
+            default:
+                break;
+            }
+        }
+    }
     switch (sp.kind) {
     case TAIL_PRODUCT:
         for (uint64_t q = q0; q < items; q += stride)
@@ -710,6 +792,7 @@ k_sc_tail(uint32_t j0, uint32_t n_max, const ScRoundInst *__restrict__ rtab,
     __shared__ fp2 chal;
     __shared__ WaveKeys wk;
     __shared__ fp2 red[WAVES][D + 1];
+    __shared__ fp2 wbuf[SC_TAIL_THREADS];  // eight lanes per pair: the pair's bound values
     const uint32_t n_steps = (n_max - j0) * count;
     ScTailStep *steps = (ScTailStep *)dyn;
     fp2 *sums = (fp2 *)(steps + n_steps);
@@ -759,7 +842,7 @@ k_sc_tail(uint32_t j0, uint32_t n_max, const ScRoundInst *__restrict__ rtab,
 #pragma unroll
             for (int z = 0; z <= D; z++) acc[z] = fp2{0, 0};
             if (gi < count)
-                tail_items<D>(steps[(size_t)(j - j0 - 1) * count + gi], terms[gi], r, lambda, gt, G, acc);
+                tail_items<D>(steps[(size_t)(j - j0 - 1) * count + gi], terms[gi], r, lambda, gt, G, wbuf, acc);
 #pragma unroll
             for (int z = 0; z <= D; z++) {
                 fp2 v = acc[z];
@@ -839,7 +922,7 @@ k_sc_tail(uint32_t j0, uint32_t n_max, const ScRoundInst *__restrict__ rtab,
         fp2 acc[D + 1];
 #pragma unroll
         for (int z = 0; z <= D; z++) acc[z] = fp2{0, 0};
-        tail_items<D>(steps[(size_t)(n_max - 1 - j0) * count + gi], terms[gi], r, lambda, gt, G, acc);
+        tail_items<D>(steps[(size_t)(n_max - 1 - j0) * count + gi], terms[gi], r, lambda, gt, G, wbuf, acc);
     }
     if (threadIdx.x < count) ist[threadIdx.x] = sis[threadIdx.x];
     if (wave == 0) {

@@ -705,7 +705,9 @@ __device__ __forceinline__ void tail_items(const ScTailStep &sp, const ScTerms &
     const bool bind = sp.out_len > 0, first = sp.first != 0;
     const uint64_t items = sp.pairs > 0 ? sp.pairs : sp.out_len;
     if constexpr (D == 3) {
-        if (sp.pairs > 0) {  // level rounds with sums: eight lanes per pair
+        // level rounds with sums whose pairs fit the group eight lanes each (more pairs: one
+        // lane per pair, the group is busy either way)
+        if (sp.pairs > 0 && sp.pairs * 8 <= stride) {
             switch (sp.kind) {
             case TAIL_EQ_PRODUCT:
             case TAIL_EQ_PRODUCT_BASE:

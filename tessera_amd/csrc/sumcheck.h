@@ -36,7 +36,7 @@ struct GenTerm {
 
 struct GenInst {
     uint32_t vars = 0;
-    int rel = REL_PRODUCT;  // REL_FRACTION: tables eq, N0, N1, D0, D1 (degree 3)
+    int rel = REL_PRODUCT;  // REL_FRACTION: tables N0, N1, D0, D1 with the eq factor (degree 3)
     std::vector<GenTable> tabs;
     std::vector<GenTerm> terms;  // REL_TERMS: sum_t w_t prod_k T_k^mult[t][k]
     bool claim_given = false;  // padding rounds use `claim` (a tree level's) instead of S_i

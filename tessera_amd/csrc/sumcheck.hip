@@ -127,24 +127,6 @@ __device__ __forceinline__ void pair_values(const fp2 *a, const fp2 *b, uint32_t
                 }
             }
         }
-    } else {
-        // tables eq, N0, N1, D0, D1 at the nodes z = 0, 1, 2, ..., D
-#pragma unroll
-        for (int s = 0; s <= D; s++) {
-            const int z = s == D ? 1 : (s == 0 ? 0 : s + 1);
-            fp2 v[5];
-#pragma unroll
-            for (int k = 0; k < 5; k++) {
-                fp2 diff = gl::sub(b[k], a[k]);
-                fp2 at = a[k];
-                for (int m = 0; m < z; m++) at = gl::add(at, diff);
-                v[k] = at;
-            }
-            fp2 d01 = gl::mul(v[3], v[4]);
-            fp2 inner = gl::add(gl::add(gl::mul(v[1], v[4]), gl::mul(v[2], v[3])),
-                                gl::mul(lambda, d01));
-            out[s] = gl::mul(v[0], inner);
-        }
     }
 }
 
@@ -615,7 +597,6 @@ enum ScTailKind : uint32_t {
     TAIL_NONE = 0,        // padding round (or an instance on no variable at the final bind)
     TAIL_PRODUCT,         // k_sc_round<D, REL_PRODUCT>
     TAIL_TERMS,           // k_sc_round<D, REL_TERMS>
-    TAIL_FRACTION,        // k_sc_round<3, REL_FRACTION> (an eq table as table 0)
     TAIL_EQ_PRODUCT,      // k_sc_round_eq<REL_PRODUCT, false, false>
     TAIL_EQ_PRODUCT_BASE, // k_sc_round_eq<REL_PRODUCT, false, true>
     TAIL_EQ_UNIT,         // k_sc_round_eq<REL_FRACTION, true, false>
@@ -739,10 +720,6 @@ __device__ __forceinline__ void tail_items(const ScTailStep &sp, const ScTerms &
     if constexpr (D == 3) {
         fp2 a3[3] = {{0, 0}, {0, 0}, {0, 0}};
         switch (sp.kind) {
-        case TAIL_FRACTION:
-            for (uint64_t q = q0; q < items; q += stride)
-                sc_item<3, REL_FRACTION>(sp.t, sp.F, sp.base_mask, first, bind, sp.pairs, r, lambda, terms, q, acc);
-            return;
         case TAIL_EQ_PRODUCT:
             for (uint64_t q = q0; q < items; q += stride)
                 sc_item_eq<REL_PRODUCT, false, false>(sp.t, sp.base_mask, first, bind, sp.pairs, r, lambda, sp.eq_lo, sp.eq_hi, sp.eq_a, q, a3);
@@ -1007,11 +984,6 @@ static void launch_round_d(uint32_t d, int rel, dim3 grid, hipStream_t s, const 
                                                  lambda, partials, eq);
         return;
     }
-    if (rel == REL_FRACTION) {
-        launch_round<3, REL_FRACTION>(grid, s, t, F, base_mask, first, out_len, pairs, r, lambda,
-                                      partials, terms);
-        return;
-    }
     switch (d) {
 #define CASE(D)                                                                              \
     case D:                                                                                  \
@@ -1122,6 +1094,8 @@ Status run_batch(Ctx *ctx, Transcript &tr, const std::string &p, uint32_t D,
     std::vector<uint32_t> pt_off(count, 0);
     uint64_t seg_len = 0;
     for (uint32_t i = 0; i < count; i++) {
+        if (insts[i].rel == REL_FRACTION && !insts[i].eq_factor)
+            return Status::fail(TSR_DECLINED, "a fraction relation carries its eq factor");
         if (!insts[i].eq_factor) continue;
         const GenInst &g = insts[i];
         const size_t nt = g.rel == REL_PRODUCT ? 2 : 4;
@@ -1353,7 +1327,7 @@ Status run_batch(Ctx *ctx, Transcript &tr, const std::string &p, uint32_t D,
             if (g.rel == REL_PRODUCT) return (first && (bm & 3) == 3) ? TAIL_EQ_PRODUCT_BASE : TAIL_EQ_PRODUCT;
             return tt.src[0] ? TAIL_EQ_FRACTION : TAIL_EQ_UNIT;
         }
-        return g.rel == REL_TERMS ? TAIL_TERMS : g.rel == REL_FRACTION ? TAIL_FRACTION : TAIL_PRODUCT;
+        return g.rel == REL_TERMS ? TAIL_TERMS : TAIL_PRODUCT;
     };
     for (uint32_t j = j0 + 1; tail && j <= n_max; j++) {
         for (uint32_t i = 0; i < count; i++) {

@@ -66,10 +66,11 @@ __device__ __forceinline__ fp2 ld_entry(const void *p, bool base, uint64_t x) {
 }
 
 // Node values of one pair, slots [p(0), p(2), ..., p(D), p(1)] (D+1 of them).
-template <int D, int REL>
+// FT > 0: exactly FT tables, known at compile time (else F at run time, at most SC_MAXF).
+template <int D, int REL, int FT = 0>
 __device__ __forceinline__ void pair_values(const fp2 *a, const fp2 *b, uint32_t F,
                                             const fp2 &lambda, const ScTerms &terms,
-                                            fp2 out[D + 1]) {
+                                            fp2 out[D + 1], uint32_t basem = 0) {
     if (REL == REL_TERMS) {
         // line values of every table at z = 0, 2, ..., D, then 1; the tables are indexed at
         // compile time (unrolled), the terms and multiplicities are uniform across the wave
@@ -106,9 +107,14 @@ __device__ __forceinline__ void pair_values(const fp2 *a, const fp2 *b, uint32_t
             out[s] = sum;
         }
     } else if (REL == REL_PRODUCT) {
-        for (uint32_t k = 0; k < F; k++) {
+        // basem bit k: table k's values are base (c1 = 0), so its products are base products
+#pragma unroll
+        for (int k = 0; k < (FT > 0 ? FT : SC_MAXF); k++) {
+            if (FT == 0 && k >= (int)F) break;
+            const bool bk = (basem >> k) & 1;
             fp2 diff = gl::sub(b[k], a[k]);
             fp2 line = gl::add(b[k], diff);  // z = 2
+            auto times = [&](const fp2 &x, const fp2 &v) { return bk ? gl::mul_base(x, v.c0) : gl::mul(x, v); };
             if (k == 0) {
                 out[0] = a[k];
                 out[D] = b[k];
@@ -118,11 +124,11 @@ __device__ __forceinline__ void pair_values(const fp2 *a, const fp2 *b, uint32_t
                     line = gl::add(line, diff);
                 }
             } else {
-                out[0] = gl::mul(out[0], a[k]);
-                out[D] = gl::mul(out[D], b[k]);
+                out[0] = times(out[0], a[k]);
+                out[D] = times(out[D], b[k]);
 #pragma unroll
                 for (int z = 2; z <= D; z++) {
-                    out[z - 1] = gl::mul(out[z - 1], line);
+                    out[z - 1] = times(out[z - 1], line);
                     line = gl::add(line, diff);
                 }
             }
@@ -139,12 +145,26 @@ __device__ __forceinline__ uint64_t shfl_xor_u64(uint64_t v, int mask) {
 
 // One item q of an own round in k_sc_round's modes: binds the tables at q and/or adds the
 // pair's node values to acc.
-template <int D, int REL>
+template <int D, int REL, int FT = 0>
 __device__ __forceinline__ void sc_item(const ScTables &t, uint32_t F, uint32_t base_mask, bool first,
                                         bool bind, uint64_t pairs, const fp2 &r, const fp2 &lambda,
                                         const ScTerms &terms, uint64_t q, fp2 (&acc)[D + 1]) {
-    fp2 a[SC_MAXF], b[SC_MAXF];
-    for (uint32_t k = 0; k < F; k++) {
+    // the tables are indexed at compile time (unrolled up to SC_MAXF, F uniform), so a and b
+    // stay in registers; a base table of the first round is bound with base products
+    constexpr int KMAX = FT > 0 ? FT : SC_MAXF;
+    fp2 a[KMAX], b[KMAX];
+    // e0 + r (e1 - e0), with two base products when e0, e1 are base
+    auto bindv = [&](const fp2 &e0, const fp2 &e1, bool base) {
+        if (base) {
+            fp2 v = gl::mul_base(r, gl::sub(e1.c0, e0.c0));
+            v.c0 = gl::add(v.c0, e0.c0);
+            return v;
+        }
+        return gl::add(e0, gl::mul(r, gl::sub(e1, e0)));
+    };
+#pragma unroll
+    for (int k = 0; k < KMAX; k++) {
+        if (FT == 0 && k >= (int)F) break;
         const bool base = first && ((base_mask >> k) & 1);
         const uint32_t sh = first ? t.shift[k] : 0, of = first ? t.off[k] : 0;
         auto at = [&](uint64_t x) { return ld_entry(t.src[k], base, (x << sh) + of); };
@@ -153,20 +173,21 @@ __device__ __forceinline__ void sc_item(const ScTables &t, uint32_t F, uint32_t 
             b[k] = at(2 * q + 1);
         } else if (pairs > 0) {
             fp2 e0 = at(4 * q), e1 = at(4 * q + 1), e2 = at(4 * q + 2), e3 = at(4 * q + 3);
-            a[k] = gl::add(e0, gl::mul(r, gl::sub(e1, e0)));
-            b[k] = gl::add(e2, gl::mul(r, gl::sub(e3, e2)));
+            a[k] = bindv(e0, e1, base);
+            b[k] = bindv(e2, e3, base);
             if (t.dst[k]) {
                 t.dst[k][2 * q] = a[k];
                 t.dst[k][2 * q + 1] = b[k];
             }
         } else {
             fp2 e0 = at(2 * q), e1 = at(2 * q + 1);
-            if (t.dst[k]) t.dst[k][q] = gl::add(e0, gl::mul(r, gl::sub(e1, e0)));
+            if (t.dst[k]) t.dst[k][q] = bindv(e0, e1, base);
         }
     }
     if (pairs > 0) {
         fp2 val[D + 1];
-        pair_values<D, REL>(a, b, F, lambda, terms, val);
+        // first round without binding: base tables multiply as base values
+        pair_values<D, REL, FT>(a, b, F, lambda, terms, val, bind ? 0u : (first ? base_mask : 0u));
 #pragma unroll
         for (int z = 0; z <= D; z++) acc[z] = gl::add(acc[z], val[z]);
     }
@@ -178,7 +199,7 @@ __device__ __forceinline__ void sc_item(const ScTables &t, uint32_t F, uint32_t 
 //   bind + sums: dst[2q], dst[2q+1] from src[4q..4q+3]; the pair (dst[2q], dst[2q+1]) summed
 //   bind only:   dst[q] from src[2q], src[2q+1] for q < out_len
 // `first`: the sources are the instance's original tables (base entries, strides, offsets).
-template <int D, int REL>
+template <int D, int REL, int FT = 0>
 __global__ void __launch_bounds__(SC_THREADS)
 k_sc_round(ScTables t, uint32_t F, uint32_t base_mask, bool first, uint64_t out_len,
            uint64_t pairs, const fp2 *__restrict__ r_ptr, fp2 lambda, fp2 *__restrict__ partials,
@@ -192,7 +213,7 @@ k_sc_round(ScTables t, uint32_t F, uint32_t base_mask, bool first, uint64_t out_
     for (int z = 0; z <= D; z++) acc[z] = fp2{0, 0};
 
     for (uint64_t q = (uint64_t)blockIdx.x * SC_THREADS + threadIdx.x; q < items; q += stride)
-        sc_item<D, REL>(t, F, base_mask, first, bind, pairs, r, lambda, terms, q, acc);
+        sc_item<D, REL, FT>(t, F, base_mask, first, bind, pairs, r, lambda, terms, q, acc);
     if (pairs == 0) return;
 
     // wave reduction, then across the block's waves through LDS
@@ -949,11 +970,11 @@ struct EqSplit {
     uint32_t a = 0;
 };
 
-template <int D, int REL>
+template <int D, int REL, int FT = 0>
 static void launch_round(dim3 grid, hipStream_t s, const ScTables &t, uint32_t F,
                          uint32_t base_mask, bool first, uint64_t out_len, uint64_t pairs,
                          const fp2 *r, fp2 lambda, fp2 *partials, const ScTerms &terms) {
-    hipLaunchKernelGGL((k_sc_round<D, REL>), grid, dim3(SC_THREADS), 0, s, t, F, base_mask,
+    hipLaunchKernelGGL((k_sc_round<D, REL, FT>), grid, dim3(SC_THREADS), 0, s, t, F, base_mask,
                        first, out_len, pairs, r, lambda, partials, terms);
 }
 
@@ -990,6 +1011,9 @@ static void launch_round_d(uint32_t d, int rel, dim3 grid, hipStream_t s, const 
         if (rel == REL_TERMS)                                                                \
             launch_round<D, REL_TERMS>(grid, s, t, F, base_mask, first, out_len, pairs, r,   \
                                        lambda, partials, terms);                             \
+        else if (F == D) /* a product of D tables: the tables known at compile time */      \
+            launch_round<D, REL_PRODUCT, D>(grid, s, t, F, base_mask, first, out_len, pairs, \
+                                            r, lambda, partials, terms);                     \
         else                                                                                 \
             launch_round<D, REL_PRODUCT>(grid, s, t, F, base_mask, first, out_len, pairs, r, \
                                          lambda, partials, terms);                           \

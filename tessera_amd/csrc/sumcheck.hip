@@ -728,6 +728,11 @@ __device__ __forceinline__ void tail_items(const ScTailStep &sp, const ScTerms &
     }
     switch (sp.kind) {
     case TAIL_PRODUCT:
+        if (sp.F == D) {  // a product of D tables: the tables known at compile time
+            for (uint64_t q = q0; q < items; q += stride)
+                sc_item<D, REL_PRODUCT, D>(sp.t, sp.F, sp.base_mask, first, bind, sp.pairs, r, lambda, terms, q, acc);
+            return;
+        }
         for (uint64_t q = q0; q < items; q += stride)
             sc_item<D, REL_PRODUCT>(sp.t, sp.F, sp.base_mask, first, bind, sp.pairs, r, lambda, terms, q, acc);
         return;

@@ -222,17 +222,68 @@ void launch_gather_roots(const MatDesc *mats, uint32_t count, uint32_t *roots, h
 
 static unsigned blocks_for(uint64_t n, unsigned per) { return (unsigned)((n + per - 1) / per); }
 
+// The top of a Merkle tree in one workgroup: from the level of `width` (<= 2 MERKLE_TOP) nodes at
+// node index `start`, every level above it up to the root, each appended after the one below
+// as k_merkle_level writes them; the level being built stays in LDS.
+constexpr uint32_t MERKLE_TOP = 1024;
+__global__ void __launch_bounds__(MERKLE_TOP)
+k_merkle_top(uint32_t *__restrict__ nodes, uint64_t start, uint64_t width, Key8 key) {
+    __shared__ uint32_t d[MERKLE_TOP][8];
+    const uint32_t t = threadIdx.x;
+    bool from_global = true;
+    while (width > 1) {
+        const uint64_t count = width / 2;
+        uint32_t out[8];
+        if (t < count) {
+            uint32_t L[8], R[8];
+            if (from_global) {
+                const uint4 *l = (const uint4 *)(nodes + 8 * (start + 2 * t));
+                const uint4 a = l[0], b = l[1], c = l[2], e = l[3];
+                const uint32_t l8[8] = {a.x, a.y, a.z, a.w, b.x, b.y, b.z, b.w};
+                const uint32_t r8[8] = {c.x, c.y, c.z, c.w, e.x, e.y, e.z, e.w};
+#pragma unroll
+                for (int i = 0; i < 8; i++) {
+                    L[i] = l8[i];
+                    R[i] = r8[i];
+                }
+            } else {
+#pragma unroll
+                for (int i = 0; i < 8; i++) {
+                    L[i] = d[2 * t][i];
+                    R[i] = d[2 * t + 1][i];
+                }
+            }
+            b3::hash64(key.w, L, R, out);
+        }
+        __syncthreads();  // the level below is read before it is overwritten
+        if (t < count) {
+#pragma unroll
+            for (int i = 0; i < 8; i++) d[t][i] = out[i];
+            uint4 *o = (uint4 *)(nodes + 8 * (start + width + t));
+            o[0] = make_uint4(out[0], out[1], out[2], out[3]);
+            o[1] = make_uint4(out[4], out[5], out[6], out[7]);
+        }
+        __syncthreads();
+        start += width;
+        width = count;
+        from_global = false;
+    }
+}
+
 Status merkle_build(Ctx *ctx, uint32_t *nodes, uint64_t n, hipStream_t stream) {
     Key8 key = key_of(ctx, DOM_MERKLE_NODE);
     uint64_t start = 0, width = n;
     KernelTimer timer(ctx, "k_merkle_level", stream);
-    while (width > 1) {
+    // one launch per level while a level has more than 2 MERKLE_TOP nodes, then the top at once
+    while (width > 2 * MERKLE_TOP) {
         uint64_t count = width / 2;
         hipLaunchKernelGGL(k_merkle_level, dim3(blocks_for(count, 256)), dim3(256), 0, stream,
                            nodes + 8 * start, nodes + 8 * (start + width), count, key);
         start += width;
         width /= 2;
     }
+    if (width > 1)
+        hipLaunchKernelGGL(k_merkle_top, dim3(1), dim3(MERKLE_TOP), 0, stream, nodes, start, width, key);
     return check_launch("merkle levels");
 }
 

@@ -436,7 +436,7 @@ struct ScRoundInst {
 // The per-instance state the rounds update.
 struct ScInstState {
     fp2 wsum;  // gamma^i S_i
-    fp2 eq_k;  // eqf: prod over the bound own rounds t of eq(point_t, c_t)
+    fp2 eq_k;  // eqf: gamma^i times the product over the bound own rounds t of eq(point_t, c_t)
 };
 
 // Batches of up to SC_STAGED instances have their facts staged in LDS by the transcript kernel.
@@ -444,6 +444,20 @@ constexpr uint32_t SC_STAGED = 64;
 // The transcript kernel's LDS: its static arrays and count x (D+1) sums (one workgroup).
 constexpr size_t SC_TRANSCRIPT_LDS = 64 * 1024;
 constexpr size_t SC_TRANSCRIPT_STATIC = 8 * 1024;
+
+// W L(x) for the eq line of round t: W ((1 - r_t)(1 - x) + r_t x) at the node x (additions only).
+__device__ __forceinline__ fp2 eq_line_at(const fp2 &w, const fp2 &rt, uint32_t node) {
+    const fp2 l0 = gl::sub(fp2{1, 0}, rt), step = gl::sub(rt, l0);
+    fp2 l = l0;
+    for (uint32_t n = 0; n < node; n++) l = gl::add(l, step);
+    return gl::mul(w, l);
+}
+
+// eq(r_t, c) = r_t c + (1 - r_t)(1 - c) = 1 - r_t - c + 2 r_t c (one product).
+__device__ __forceinline__ fp2 eq_at(const fp2 &rt, const fp2 &c) {
+    const fp2 m = gl::mul(rt, c);
+    return gl::sub(gl::sub(gl::add(fp2{1, 0}, gl::add(m, m)), rt), c);
+}
 
 struct ScCtl {
     DevTranscript t;
@@ -543,13 +557,8 @@ k_sc_transcript(uint32_t j, const ScRoundInst *__restrict__ ri, ScInstState *__r
             *s = gl::mul_base(is.wsum, 1ull << (in.pad - j - 1));
             return;
         }
-        fp2 f = in.weight;
-        if (in.eqf) {
-            // the eq line of this round: K ((1 - r_t)(1 - x) + r_t x) at x = node
-            const fp2 l0 = gl::sub(fp2{1, 0}, in.rt), step = gl::sub(in.rt, l0);
-            const uint32_t node = z == 0 ? 0 : z + 1;
-            f = gl::mul(f, gl::mul(is.eq_k, gl::add(l0, gl::mul_base(step, node))));
-        }
+        // weight, or for an eq factor gamma^i K times the eq line of this round at the node
+        const fp2 f = in.eqf ? eq_line_at(is.eq_k, in.rt, z == 0 ? 0 : z + 1) : in.weight;
         *s = gl::mul(*s, f);
     };
     if (count <= SC_STAGED) {
@@ -592,11 +601,10 @@ k_sc_transcript(uint32_t j, const ScRoundInst *__restrict__ ri, ScInstState *__r
     }
     __syncthreads();
     const fp2 r = chal;
-    // K *= eq(r_t, c) = r_t c + (1 - r_t)(1 - c)
+    // gamma^i K *= eq(r_t, c)
     auto fold = [&](uint32_t i, const ScRoundInst &in, const fp2 &k) {
         if (!in.eqf || j < in.pad) return;
-        const fp2 e = gl::add(gl::mul(in.rt, r), gl::mul(gl::sub(fp2{1, 0}, in.rt), gl::sub(fp2{1, 0}, r)));
-        ist[i].eq_k = gl::mul(k, e);
+        ist[i].eq_k = gl::mul(k, eq_at(in.rt, r));
     };
     if (threadIdx.x < count) fold(threadIdx.x, my, mys.eq_k);
     for (uint32_t i = threadIdx.x + SC_THREADS; i < count; i += SC_THREADS) fold(i, ri[i], ist[i].eq_k);
@@ -867,15 +875,17 @@ k_sc_tail(uint32_t j0, uint32_t n_max, const ScRoundInst *__restrict__ rtab,
         }
         __syncthreads();
         // S_i of padded instances without a claim (round 0 only), then the shares in place
-        if (threadIdx.x < count) {
-            const uint32_t i = threadIdx.x;
-            const ScRoundInst &in = sri[i];
-            if (j == 0 && in.pad > 0 && !in.wsum_given) {
-                const fp2 *s = sums + (uint64_t)i * (D + 1);
-                sis[i].wsum = gl::mul(in.weight, gl::add(s[0], s[D]));
+        if (j == 0) {
+            if (threadIdx.x < count) {
+                const uint32_t i = threadIdx.x;
+                const ScRoundInst &in = sri[i];
+                if (in.pad > 0 && !in.wsum_given) {
+                    const fp2 *s = sums + (uint64_t)i * (D + 1);
+                    sis[i].wsum = gl::mul(in.weight, gl::add(s[0], s[D]));
+                }
             }
+            __syncthreads();
         }
-        __syncthreads();
         for (uint32_t x = threadIdx.x; x < count * D; x += SC_TAIL_THREADS) {
             const uint32_t i = x / D, z = x % D;
             const ScRoundInst &in = sri[i];
@@ -883,12 +893,7 @@ k_sc_tail(uint32_t j0, uint32_t n_max, const ScRoundInst *__restrict__ rtab,
             if (j < in.pad) {
                 *s = gl::mul_base(sis[i].wsum, 1ull << (in.pad - j - 1));
             } else {
-                fp2 f = in.weight;
-                if (in.eqf) {
-                    const fp2 l0 = gl::sub(fp2{1, 0}, in.rt), step = gl::sub(in.rt, l0);
-                    const uint32_t node = z == 0 ? 0 : z + 1;
-                    f = gl::mul(f, gl::mul(sis[i].eq_k, gl::add(l0, gl::mul_base(step, node))));
-                }
+                const fp2 f = in.eqf ? eq_line_at(sis[i].eq_k, in.rt, z == 0 ? 0 : z + 1) : in.weight;
                 *s = gl::mul(*s, f);
             }
         }
@@ -915,10 +920,7 @@ k_sc_tail(uint32_t j0, uint32_t n_max, const ScRoundInst *__restrict__ rtab,
         r = chal;
         if (threadIdx.x < count) {
             const ScRoundInst &in = sri[threadIdx.x];
-            if (in.eqf && j >= in.pad) {
-                const fp2 e = gl::add(gl::mul(in.rt, r), gl::mul(gl::sub(fp2{1, 0}, in.rt), gl::sub(fp2{1, 0}, r)));
-                sis[threadIdx.x].eq_k = gl::mul(sis[threadIdx.x].eq_k, e);
-            }
+            if (in.eqf && j >= in.pad) sis[threadIdx.x].eq_k = gl::mul(sis[threadIdx.x].eq_k, eq_at(in.rt, r));
         }
         __syncthreads();
     }
@@ -1109,7 +1111,7 @@ Status run_batch(Ctx *ctx, Transcript &tr, const std::string &p, uint32_t D,
         weight[i] = w;
         wgiven[i] = insts[i].claim_given ? 1 : 0;
         hstate[i].wsum = insts[i].claim_given ? gl::mul(w, insts[i].claim) : fp2{0, 0};
-        hstate[i].eq_k = fp2{1, 0};
+        hstate[i].eq_k = w;  // gamma^i: the eq factor's K starts at 1
         w = gl::mul(w, gamma);
     }
     Status st;

@@ -41,7 +41,10 @@ using gl::fp2;
 constexpr int SC_MAXF = 10;
 constexpr int SC_MAXD = 10;
 constexpr int SC_THREADS = 256;
-constexpr int SC_MAX_BLOCKS = 1024;
+#ifndef TSR_SC_MAX_BLOCKS
+#define TSR_SC_MAX_BLOCKS 1024
+#endif
+constexpr int SC_MAX_BLOCKS = TSR_SC_MAX_BLOCKS;  // workgroups of a round launch at most
 
 constexpr int SC_MAXT = 16;
 

@@ -45,6 +45,9 @@ constexpr int SC_THREADS = 256;
 #define TSR_SC_MAX_BLOCKS 1024
 #endif
 constexpr int SC_MAX_BLOCKS = TSR_SC_MAX_BLOCKS;  // workgroups of a round launch at most
+// A level round over 2^m pairs splits eq into eq_lo (2^a entries) and eq_hi (2^(m - a)); a = m
+// up to this many variables (eq_lo at most 128 KiB), else max(this, m / 2).
+constexpr uint32_t SC_EQ_LO_BITS = 13;
 
 constexpr int SC_MAXT = 16;
 
@@ -239,21 +242,27 @@ k_sc_round(ScTables t, uint32_t F, uint32_t base_mask, bool first, uint64_t out_
     }
 }
 
-template <int REL, bool UNIT>
+template <int REL, bool UNIT, int NA>
 __device__ __forceinline__ void sc_eq_nodes(const fp2 *a, const fp2 *b, const fp2 &e,
-                                            const fp2 &lambda, fp2 (&acc)[3]);
+                                            const fp2 &lambda, fp2 (&acc)[NA]);
 
 // One item q of a level round in k_sc_round_eq's modes: binds the tables at q and/or adds E F
-// at z = 0, 1, 2 to acc.
-template <int REL, bool UNIT, bool BASE>
+// at z = 0, 1, 2 to acc. LO: the pair's weight is eq_lo[q mod 2^eq_a] alone (the caller applies
+// eq_hi[q >> eq_a] to the sums of a run of pairs that share it, or eq_hi is the single entry 1).
+// NA = 6 (unit numerators only): acc[0..2] take E (D0 + D1) and acc[3..5] E D0 D1, so that
+// lambda multiplies the sums instead of every pair.
+template <int REL, bool UNIT, bool BASE, bool LO = false, int NA = 3>
 __device__ __forceinline__ void sc_item_eq(const ScTables &t, uint32_t base_mask, bool first,
                                            bool bind, uint64_t pairs, const fp2 &r,
                                            const fp2 &lambda, const fp2 *__restrict__ eq_lo,
                                            const fp2 *__restrict__ eq_hi, uint32_t eq_a,
-                                           uint64_t q, fp2 (&acc)[3]) {
+                                           uint64_t q, fp2 (&acc)[NA]) {
     constexpr int F = REL == REL_PRODUCT ? 2 : 4;
     constexpr int K0 = (REL == REL_FRACTION && UNIT) ? 2 : 0;
-    auto line2 = [](const fp2 &a, const fp2 &b) { return gl::sub(gl::add(b, b), a); };
+    auto weight = [&](uint64_t x) {
+        const fp2 lo = eq_lo[x & ((1ull << eq_a) - 1)];
+        return LO ? lo : gl::mul(lo, eq_hi[x >> eq_a]);
+    };
     fp2 a[F], b[F];
     if (BASE) {
         const uint64_t *s0 = (const uint64_t *)t.src[0], *s1 = (const uint64_t *)t.src[1];
@@ -267,7 +276,7 @@ __device__ __forceinline__ void sc_item_eq(const ScTables &t, uint32_t base_mask
         };
         if (!bind) {
             const uint64_t a0 = at0(2 * q), b0 = at0(2 * q + 1), a1 = at1(2 * q), b1 = at1(2 * q + 1);
-            const fp2 e = gl::mul(eq_lo[q & ((1ull << eq_a) - 1)], eq_hi[q >> eq_a]);
+            const fp2 e = weight(q);
             const uint64_t f0 = gl::mul(a0, a1), f1 = gl::mul(b0, b1);
             const uint64_t f2 = gl::mul(gl::sub(gl::add(b0, b0), a0), gl::sub(gl::add(b1, b1), a1));
             acc[0] = gl::add(acc[0], gl::mul_base(e, f0));
@@ -295,7 +304,7 @@ __device__ __forceinline__ void sc_item_eq(const ScTables &t, uint32_t base_mask
         // separate arrays, but the compiler cannot know it: a store between two tables' loads
         // would hold the second table's loads back)
         fp2 ent[F][4];
-        const fp2 e = pairs > 0 ? gl::mul(eq_lo[q & ((1ull << eq_a) - 1)], eq_hi[q >> eq_a]) : fp2{0, 0};
+        const fp2 e = pairs > 0 ? weight(q) : fp2{0, 0};
 #pragma unroll
         for (int k = K0; k < F; k++) {
             const bool base = first && ((base_mask >> k) & 1);
@@ -332,18 +341,30 @@ __device__ __forceinline__ void sc_item_eq(const ScTables &t, uint32_t base_mask
             }
         }
         if (pairs == 0) return;
-        sc_eq_nodes<REL, UNIT>(a, b, e, lambda, acc);
+        sc_eq_nodes<REL, UNIT, NA>(a, b, e, lambda, acc);
         return;
     }
-    const fp2 e = gl::mul(eq_lo[q & ((1ull << eq_a) - 1)], eq_hi[q >> eq_a]);
-    sc_eq_nodes<REL, UNIT>(a, b, e, lambda, acc);
+    sc_eq_nodes<REL, UNIT, NA>(a, b, weight(q), lambda, acc);
 }
 
 // E F(z) at z = 0, 1, 2 of one pair added to acc (see k_sc_round_eq).
-template <int REL, bool UNIT>
+template <int REL, bool UNIT, int NA>
 __device__ __forceinline__ void sc_eq_nodes(const fp2 *a, const fp2 *b, const fp2 &e,
-                                            const fp2 &lambda, fp2 (&acc)[3]) {
+                                            const fp2 &lambda, fp2 (&acc)[NA]) {
     auto line2 = [](const fp2 &x, const fp2 &y) { return gl::sub(gl::add(y, y), x); };
+    if constexpr (NA == 6) {
+        static_assert(REL == REL_FRACTION && UNIT, "split sums are for unit numerators");
+        // E (D0 + D1) is linear in z; (E D0) D1 at z = 0, 1, 2: 7 products
+        const fp2 s0 = gl::mul(e, gl::add(a[2], a[3])), s1 = gl::mul(e, gl::add(b[2], b[3]));
+        const fp2 x0 = gl::mul(e, a[2]), x1 = gl::mul(e, b[2]);
+        acc[0] = gl::add(acc[0], s0);
+        acc[1] = gl::add(acc[1], s1);
+        acc[2] = gl::add(acc[2], line2(s0, s1));
+        acc[3] = gl::add(acc[3], gl::mul(x0, a[3]));
+        acc[4] = gl::add(acc[4], gl::mul(x1, b[3]));
+        acc[5] = gl::add(acc[5], gl::mul(line2(x0, x1), line2(a[3], b[3])));
+        return;
+    }
     fp2 v0, v1, v2;
     if (REL == REL_PRODUCT) {
         const fp2 x0 = gl::mul(e, a[0]), x1 = gl::mul(e, b[0]);
@@ -384,7 +405,12 @@ __device__ __forceinline__ void sc_eq_nodes(const fp2 *a, const fp2 *b, const fp
 // BASE (REL_PRODUCT, first round of a level, both tables base, as a forest's level values): the
 // sums multiply in the base field and scale by E once per node (3 base products + 3 Fp2 x Fp
 // per pair instead of 6 Fp2 products), the binding takes 2 base products per entry.
-template <int REL, bool UNIT, bool BASE = false>
+// LO (rounds with sums; eq_hi a single entry 1, or 2^eq_a >= SC_THREADS): block b takes the
+// pairs [b c, (b + 1) c), c a multiple of SC_THREADS, so that the pairs of one pass of the block
+// share eq_hi[q >> eq_a]. A pair is weighted by eq_lo alone and each run of pairs with one eq_hi
+// entry scales its sums by it once: one Fp2 product less per pair. Unit numerators also keep
+// E (D0 + D1) and E D0 D1 apart and multiply by lambda once per thread.
+template <int REL, bool UNIT, bool BASE = false, bool LO = false>
 __global__ void __launch_bounds__(SC_THREADS)
 k_sc_round_eq(ScTables t, uint32_t base_mask, bool first, uint64_t out_len, uint64_t pairs,
               const fp2 *__restrict__ r_ptr, fp2 lambda, fp2 *__restrict__ partials,
@@ -392,12 +418,55 @@ k_sc_round_eq(ScTables t, uint32_t base_mask, bool first, uint64_t out_len, uint
     const bool bind = out_len > 0;
     const fp2 r = bind ? *r_ptr : fp2{0, 0};
     const uint64_t items = pairs > 0 ? pairs : out_len;
-    const uint64_t stride = (uint64_t)gridDim.x * SC_THREADS;
     fp2 acc[3] = {{0, 0}, {0, 0}, {0, 0}};
 
-    for (uint64_t q = (uint64_t)blockIdx.x * SC_THREADS + threadIdx.x; q < items; q += stride)
-        sc_item_eq<REL, UNIT, BASE>(t, base_mask, first, bind, pairs, r, lambda, eq_lo, eq_hi, eq_a, q, acc);
-    if (pairs == 0) return;
+    if constexpr (!LO) {
+        const uint64_t stride = (uint64_t)gridDim.x * SC_THREADS;
+        for (uint64_t q = (uint64_t)blockIdx.x * SC_THREADS + threadIdx.x; q < items; q += stride)
+            sc_item_eq<REL, UNIT, BASE>(t, base_mask, first, bind, pairs, r, lambda, eq_lo, eq_hi, eq_a, q, acc);
+        if (pairs == 0) return;
+    } else {
+        constexpr int NA = (REL == REL_FRACTION && UNIT) ? 6 : 3;
+        const uint64_t per = (pairs + gridDim.x - 1) / gridDim.x;
+        const uint64_t chunk = (per + SC_THREADS - 1) / SC_THREADS * SC_THREADS;
+        const uint64_t q0 = (uint64_t)blockIdx.x * chunk;
+        const uint64_t q1 = q0 + chunk < pairs ? q0 + chunk : pairs;
+        const bool hi_one = (pairs >> eq_a) <= 1;
+        // a run's sums, folded into acc (times lambda for the E D0 D1 half, times its eq_hi)
+        fp2 run[NA];
+#pragma unroll
+        for (int z = 0; z < NA; z++) run[z] = fp2{0, 0};
+        auto fold = [&](const fp2 *w) {
+#pragma unroll
+            for (int z = 0; z < 3; z++) {
+                fp2 v = run[z];
+                if constexpr (NA == 6) v = gl::add(v, gl::mul(lambda, run[3 + z]));
+                acc[z] = gl::add(acc[z], w ? gl::mul(*w, v) : v);
+            }
+#pragma unroll
+            for (int z = 0; z < NA; z++) run[z] = fp2{0, 0};
+        };
+        uint64_t h = q0 >> eq_a;
+#pragma unroll 1
+        for (uint64_t qb = q0; qb < q1; qb += SC_THREADS) {
+            if (!hi_one && (qb >> eq_a) != h) {
+                const fp2 w = eq_hi[h];
+                fold(&w);
+                h = qb >> eq_a;
+            }
+            const uint64_t q = qb + threadIdx.x;
+            if (q < q1)
+                sc_item_eq<REL, UNIT, BASE, true, NA>(t, base_mask, first, bind, pairs, r, lambda, eq_lo, eq_hi, eq_a, q, run);
+        }
+        if (q0 < q1) {
+            if (hi_one) {
+                fold(nullptr);
+            } else {
+                const fp2 w = eq_hi[h];
+                fold(&w);
+            }
+        }
+    }
 
     __shared__ fp2 red[SC_THREADS / 64][3];
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
@@ -685,7 +754,7 @@ __device__ __forceinline__ void tail_eq_wide(const ScTailStep &sp, const fp2 &r,
                 const fp2 a = my[2 * (k - K0)], b = my[2 * (k - K0) + 1];
                 x[k] = role == 0 ? a : role == 1 ? b : gl::sub(gl::add(b, b), a);
             }
-            const fp2 e = gl::mul(sp.eq_lo[q & ((1ull << sp.eq_a) - 1)], sp.eq_hi[q >> sp.eq_a]);
+            const fp2 e = sp.eq_lo[q];  // eq_hi is [1] in the tail's rounds
             fp2 f;
             if (REL == REL_PRODUCT)
                 f = gl::mul(gl::mul(e, x[0]), x[1]);
@@ -759,19 +828,19 @@ __device__ __forceinline__ void tail_items(const ScTailStep &sp, const ScTerms &
         switch (sp.kind) {
         case TAIL_EQ_PRODUCT:
             for (uint64_t q = q0; q < items; q += stride)
-                sc_item_eq<REL_PRODUCT, false, false>(sp.t, sp.base_mask, first, bind, sp.pairs, r, lambda, sp.eq_lo, sp.eq_hi, sp.eq_a, q, a3);
+                sc_item_eq<REL_PRODUCT, false, false, true>(sp.t, sp.base_mask, first, bind, sp.pairs, r, lambda, sp.eq_lo, sp.eq_hi, sp.eq_a, q, a3);
             break;
         case TAIL_EQ_PRODUCT_BASE:
             for (uint64_t q = q0; q < items; q += stride)
-                sc_item_eq<REL_PRODUCT, false, true>(sp.t, sp.base_mask, first, bind, sp.pairs, r, lambda, sp.eq_lo, sp.eq_hi, sp.eq_a, q, a3);
+                sc_item_eq<REL_PRODUCT, false, true, true>(sp.t, sp.base_mask, first, bind, sp.pairs, r, lambda, sp.eq_lo, sp.eq_hi, sp.eq_a, q, a3);
             break;
         case TAIL_EQ_UNIT:
             for (uint64_t q = q0; q < items; q += stride)
-                sc_item_eq<REL_FRACTION, true, false>(sp.t, sp.base_mask, first, bind, sp.pairs, r, lambda, sp.eq_lo, sp.eq_hi, sp.eq_a, q, a3);
+                sc_item_eq<REL_FRACTION, true, false, true>(sp.t, sp.base_mask, first, bind, sp.pairs, r, lambda, sp.eq_lo, sp.eq_hi, sp.eq_a, q, a3);
             break;
         case TAIL_EQ_FRACTION:
             for (uint64_t q = q0; q < items; q += stride)
-                sc_item_eq<REL_FRACTION, false, false>(sp.t, sp.base_mask, first, bind, sp.pairs, r, lambda, sp.eq_lo, sp.eq_hi, sp.eq_a, q, a3);
+                sc_item_eq<REL_FRACTION, false, false, true>(sp.t, sp.base_mask, first, bind, sp.pairs, r, lambda, sp.eq_lo, sp.eq_hi, sp.eq_a, q, a3);
             break;
         default:
             return;
@@ -992,8 +1061,17 @@ template <int REL, bool UNIT, bool BASE = false>
 static void launch_round_eq(dim3 grid, hipStream_t s, const ScTables &t, uint32_t base_mask,
                             bool first, uint64_t out_len, uint64_t pairs, const fp2 *r, fp2 lambda,
                             fp2 *partials, const EqSplit &eq) {
-    hipLaunchKernelGGL((k_sc_round_eq<REL, UNIT, BASE>), grid, dim3(SC_THREADS), 0, s, t,
-                       base_mask, first, out_len, pairs, r, lambda, partials, eq.lo, eq.hi, eq.a);
+    // LO when eq_hi is the single entry 1, or when its runs span whole passes of a block and
+    // every thread has at least 8 pairs (one run's scaling, 3 products, then costs less than the
+    // product per pair it saves)
+    const bool lo = pairs > 0 && ((pairs >> eq.a) <= 1 ||
+                                  ((1ull << eq.a) >= SC_THREADS && pairs >= 8ull * grid.x * SC_THREADS));
+    if (lo)
+        hipLaunchKernelGGL((k_sc_round_eq<REL, UNIT, BASE, true>), grid, dim3(SC_THREADS), 0, s, t,
+                           base_mask, first, out_len, pairs, r, lambda, partials, eq.lo, eq.hi, eq.a);
+    else
+        hipLaunchKernelGGL((k_sc_round_eq<REL, UNIT, BASE>), grid, dim3(SC_THREADS), 0, s, t,
+                           base_mask, first, out_len, pairs, r, lambda, partials, eq.lo, eq.hi, eq.a);
 }
 
 static void launch_round_d(uint32_t d, int rel, dim3 grid, hipStream_t s, const ScTables &t,
@@ -1145,7 +1223,10 @@ Status run_batch(Ctx *ctx, Transcript &tr, const std::string &p, uint32_t D,
         pt_off[i] = (uint32_t)eq_pts.size();
         eq_pts.insert(eq_pts.end(), g.eq_point.begin(), g.eq_point.end());
         for (uint32_t t = 0; t < g.vars; t++) {
-            const uint32_t m = g.vars - t - 1, a = m / 2;
+            // round t weights its 2^m pairs by eq_lo[q mod 2^a] eq_hi[q >> a]; a = m (eq_hi = [1])
+            // up to SC_EQ_LO_BITS variables, so that small rounds and the tail use eq_lo alone
+            const uint32_t m = g.vars - t - 1;
+            const uint32_t a = m <= SC_EQ_LO_BITS ? m : std::max<uint32_t>(SC_EQ_LO_BITS, m / 2);
             segs.push_back(EqSeg{seg_len, pt_off[i] + t + 1, a});
             seg_len += 1ull << a;
             segs.push_back(EqSeg{seg_len, pt_off[i] + t + 1 + a, m - a});
@@ -1390,6 +1471,9 @@ Status run_batch(Ctx *ctx, Transcript &tr, const std::string &p, uint32_t D,
                 sp.eq_lo = e.lo;
                 sp.eq_hi = e.hi;
                 sp.eq_a = e.a;
+                // the tail weights a pair by eq_lo alone: its rounds have at most 2^9 pairs
+                if (sp.pairs > (1ull << e.a))
+                    return Status::fail(TSR_BAD_SHAPE, "a tail round with more than one eq_hi entry");
             }
         }
     }

@@ -410,12 +410,13 @@ __device__ __forceinline__ void sc_eq_nodes(const fp2 *a, const fp2 *b, const fp
 // share eq_hi[q >> eq_a]. A pair is weighted by eq_lo alone and each run of pairs with one eq_hi
 // entry scales its sums by it once: one Fp2 product less per pair. Unit numerators also keep
 // E (D0 + D1) and E D0 D1 apart and multiply by lambda once per thread.
-template <int REL, bool UNIT, bool BASE = false, bool LO = false>
+// BIND: out_len > 0 (a round without binding holds half the loaded entries: fewer registers).
+template <int REL, bool UNIT, bool BASE = false, bool LO = false, bool BIND = true>
 __global__ void __launch_bounds__(SC_THREADS)
 k_sc_round_eq(ScTables t, uint32_t base_mask, bool first, uint64_t out_len, uint64_t pairs,
               const fp2 *__restrict__ r_ptr, fp2 lambda, fp2 *__restrict__ partials,
               const fp2 *__restrict__ eq_lo, const fp2 *__restrict__ eq_hi, uint32_t eq_a) {
-    const bool bind = out_len > 0;
+    constexpr bool bind = BIND;
     const fp2 r = bind ? *r_ptr : fp2{0, 0};
     const uint64_t items = pairs > 0 ? pairs : out_len;
     fp2 acc[3] = {{0, 0}, {0, 0}, {0, 0}};
@@ -1066,9 +1067,15 @@ static void launch_round_eq(dim3 grid, hipStream_t s, const ScTables &t, uint32_
     // product per pair it saves)
     const bool lo = pairs > 0 && ((pairs >> eq.a) <= 1 ||
                                   ((1ull << eq.a) >= SC_THREADS && pairs >= 8ull * grid.x * SC_THREADS));
-    if (lo)
+    if (lo && out_len == 0)
+        hipLaunchKernelGGL((k_sc_round_eq<REL, UNIT, BASE, true, false>), grid, dim3(SC_THREADS), 0, s,
+                           t, base_mask, first, out_len, pairs, r, lambda, partials, eq.lo, eq.hi, eq.a);
+    else if (lo)
         hipLaunchKernelGGL((k_sc_round_eq<REL, UNIT, BASE, true>), grid, dim3(SC_THREADS), 0, s, t,
                            base_mask, first, out_len, pairs, r, lambda, partials, eq.lo, eq.hi, eq.a);
+    else if (out_len == 0)
+        hipLaunchKernelGGL((k_sc_round_eq<REL, UNIT, BASE, false, false>), grid, dim3(SC_THREADS), 0, s,
+                           t, base_mask, first, out_len, pairs, r, lambda, partials, eq.lo, eq.hi, eq.a);
     else
         hipLaunchKernelGGL((k_sc_round_eq<REL, UNIT, BASE>), grid, dim3(SC_THREADS), 0, s, t,
                            base_mask, first, out_len, pairs, r, lambda, partials, eq.lo, eq.hi, eq.a);

@@ -550,6 +550,11 @@ tsr_status tsr_ctx_profile(tsr_ctx *ctx, int enable);
 /* Total milliseconds and launch count of the named kernel since the last clear. */
 tsr_status tsr_ctx_kernel_time(tsr_ctx *ctx, const char *kernel, double *total_ms,
                                uint64_t *launches);
+/* The most workgroups one sumcheck round launch uses (1..1024; 0 restores the default, 1024).
+ * Results do not depend on it. No reference counterpart: a tuning knob of this back end (fewer
+ * workgroups give each thread more pairs of a round, which changes the kernels' code paths at
+ * smaller sizes). BAD_ARG above 1024. */
+tsr_status tsr_ctx_set_round_blocks(tsr_ctx *ctx, uint32_t blocks);
 /* The device's register-resident Goldilocks butterfly rate (u + v, (u - v) w per butterfly, no
  * memory traffic): the VALU ceiling the NTT kernels are measured against. No reference
  * counterpart (the reference quotes butterflies per second, docs/perf/v0.1/cc-h100.md). */

@@ -311,6 +311,13 @@ tsr_status tsr_ctx_synchronize(tsr_ctx *ctx) {
     return TSR_OK;
 }
 
+tsr_status tsr_ctx_set_round_blocks(tsr_ctx *ctx, uint32_t blocks) {
+    CHECK_ARG(ctx, "null context");
+    CHECK_ARG(blocks <= 1024, "at most 1024 workgroups per round launch");
+    ctx->round_blocks = blocks;
+    return TSR_OK;
+}
+
 tsr_status tsr_ctx_profile(tsr_ctx *ctx, int enable) {
     CHECK_ARG(ctx, "null context");
     if (enable == 2) {

@@ -104,6 +104,7 @@ struct Ctx {
     std::map<std::tuple<uint32_t, uint32_t, uint64_t, uint32_t>, std::shared_ptr<Encoder>> encoders;
     uint32_t keys[19][8];  // BLAKE3 context keys of the reference's 19 domains
     bool profile = false;
+    uint32_t round_blocks = 0;  // workgroups of a sumcheck round launch at most (0: the default)
     struct Timed {
         const char *name;
         hipEvent_t start, stop;

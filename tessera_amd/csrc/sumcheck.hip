@@ -1153,9 +1153,10 @@ static void launch_tail_d(uint32_t d, hipStream_t s, uint32_t j0, uint32_t n_max
     }
 }
 
-static uint32_t blocks_for_pairs(uint64_t items) {
+static uint32_t blocks_for_pairs(const Ctx *ctx, uint64_t items) {
+    const uint64_t cap = ctx->round_blocks ? ctx->round_blocks : SC_MAX_BLOCKS;
     uint64_t b = (items + SC_THREADS - 1) / SC_THREADS;
-    return (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(b, SC_MAX_BLOCKS));
+    return (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(b, cap));
 }
 
 DevKeys dev_keys(const Ctx *ctx) {
@@ -1334,7 +1335,7 @@ Status run_batch(Ctx *ctx, Transcript &tr, const std::string &p, uint32_t D,
             e.rt = (e.eqf && j >= pad[i] && j < n_max) ? insts[i].eq_point[j - pad[i]] : fp2{0, 0};
             if (v == 0 || j >= n_max || (j < pad[i] && j != 0)) continue;
             const uint32_t t = j < pad[i] ? 0 : j - pad[i];
-            e.nblk = blocks_for_pairs(1ull << (v - t - 1));
+            e.nblk = blocks_for_pairs(ctx, 1ull << (v - t - 1));
             nblk_max[j] = std::max(nblk_max[j], e.nblk);
         }
     }
@@ -1527,7 +1528,7 @@ Status run_batch(Ctx *ctx, Transcript &tr, const std::string &p, uint32_t D,
             uint32_t bm;
             tables_of(i, 0, t, bm);
             uint64_t pairs = 1ull << (insts[i].vars - 1);
-            launch_round_d(D, insts[i].rel, dim3(blocks_for_pairs(pairs)), s, t,
+            launch_round_d(D, insts[i].rel, dim3(blocks_for_pairs(ctx, pairs)), s, t,
                            (uint32_t)insts[i].tabs.size(), bm, true, 0, pairs, nullptr, lambda,
                            partials.ptr + pstride * i, sterms[i],
                            splits[i].empty() ? EqSplit() : splits[i][0]);
@@ -1552,7 +1553,7 @@ Status run_batch(Ctx *ctx, Transcript &tr, const std::string &p, uint32_t D,
             uint64_t pairs = out_len / 2;  // next own round's pairs (0 after the last)
             // the pairs summed are those of own round t + 1 (none after the last round)
             const EqSplit eq = splits[i].empty() ? EqSplit() : splits[i][std::min<uint32_t>(t + 1, insts[i].vars - 1)];
-            launch_round_d(D, insts[i].rel, dim3(blocks_for_pairs(pairs ? pairs : out_len)), s,
+            launch_round_d(D, insts[i].rel, dim3(blocks_for_pairs(ctx, pairs ? pairs : out_len)), s,
                            tt, (uint32_t)insts[i].tabs.size(), bm, t == 0, out_len, pairs,
                            &d_ctl->r, lambda, partials.ptr + pstride * i, sterms[i], eq);
         }

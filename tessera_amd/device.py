@@ -72,6 +72,11 @@ class Device:
     def set_budget(self, nbytes):
         check(lib().tsr_ctx_set_budget(self.handle, nbytes))
 
+    def set_round_blocks(self, blocks):
+        """The most workgroups a sumcheck round launch uses (1..1024, 0 = default 1024). Proofs
+        do not depend on it; fewer workgroups give each thread more pairs of a round."""
+        check(lib().tsr_ctx_set_round_blocks(self.handle, blocks))
+
     def memory(self):
         return int(lib().tsr_ctx_memory(self.handle))
 

@@ -141,6 +141,7 @@ _SIGS = {
     "tsr_server_clock": (ctypes.c_double, [vp]),
     "tsr_server_destroy": (None, [vp]),
     "tsr_ctx_profile": (i32, [vp, i32]),
+    "tsr_ctx_set_round_blocks": (i32, [vp, u32]),
     "tsr_ctx_kernel_time": (i32, [vp, ctypes.c_char_p, ctypes.POINTER(ctypes.c_double),
                                   ctypes.POINTER(u64)]),
 }

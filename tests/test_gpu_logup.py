@@ -218,3 +218,31 @@ def test_config3_logup_against_oracle(dev):
     na, da, nt, dt = as_tuples(got.roots)
     s1, s2 = O.fp2_mul(na, dt), O.fp2_mul(nt, da)
     assert ((s1[0] + s2[0]) % P, (s1[1] + s2[1]) % P) == (0, 0)
+
+
+@pytest.mark.parametrize("blocks", [16, 7, 1])
+def test_narrow_round_launches(dev, blocks):
+    """With at most `blocks` workgroups per round launch, a thread takes 8 or more pairs of a
+    round from 2^15 pairs on (2^14 with 7, every round with 1), so the level rounds weight each
+    pair by eq_lo alone and scale each run of pairs by its eq_hi entry, with runs that cross
+    block chunks (7) and several runs per block (1): LogUp of 2^19 lookups with values and a
+    2^18-leaf forest, against the oracle. The launch width must not change a proof."""
+    from tessera_amd.lib import TsrError
+    from tessera_amd.logup import prove_trees
+    from tessera_amd.transcript import Transcript
+    with pytest.raises(TsrError):
+        dev.set_round_blocks(1025)
+    dev.set_round_blocks(blocks)
+    try:
+        rng = np.random.default_rng(900 + blocks)
+        compare_with_oracle(dev, [random_instance(rng, 1 << 10, 1 << 19)], [0, 1],
+                            ctx=b"narrow-rounds")
+        forests = [(rng.integers(0, P, 1 << 18, dtype=np.uint64), 4)]
+        t = Transcript(b"narrow-forest")
+        claims, levels = prove_trees(dev, t, to_gpu_forests(forests))
+        ot = O.OracleTranscript(b"narrow-forest")
+        want_claims, want_levels = O.prove_trees(ot, forests)
+        assert claims == want_claims and levels == want_levels
+        assert t.state == ot.state and t.counter == ot.counter
+    finally:
+        dev.set_round_blocks(0)

@@ -253,3 +253,18 @@ def test_more_than_256_instances_are_declined(dev):
     with pytest.raises(T.Declined, match="256"):
         prove_batch(dev, t, "stage/0", 2, [[Table(np.ones(2, np.uint64))] for _ in range(257)])
     assert t.state == s0 and t.counter == 0
+
+
+@pytest.mark.parametrize("blocks", [1, 7, 64])
+def test_narrow_round_launches(dev, blocks):
+    """At most `blocks` workgroups per round launch (each thread then takes many pairs of a round,
+    and a round's partials come from a block count that is not a multiple of the wave): products
+    of degree 3 and 2 over 16 and 13 variables, base and Fp2 tables, against the oracle."""
+    rng = np.random.default_rng(700 + blocks)
+    spec = [[(v, e, False) for v, e, _ in rand_tables(rng, 16, [(False, False), (True, False), (False, False)])],
+            [(v, e, False) for v, e, _ in rand_tables(rng, 13, [(True, False), (False, False), (False, False)])]]
+    dev.set_round_blocks(blocks)
+    try:
+        run_both(dev, "stage/narrow", 3, spec)
+    finally:
+        dev.set_round_blocks(0)

@@ -11,7 +11,8 @@
 //     on LDS tiles of 2^(nb+logc) elements: 2^nb strided points x 2^logc consecutive columns,
 //     2^12 elements (32 KiB) in all, so each wave reads and writes runs of 2^logc words (64 B
 //     for the 9-bit passes). 32 KiB tiles of eight columns measured faster than 64 KiB tiles of
-//     sixteen (half the workgroups per CU) and than 16 KiB tiles of four.
+//     sixteen (half the workgroups per CU) and than 16 KiB tiles of four. Ten bits run as one
+//     pass on 64 KiB tiles of eight columns (1024 threads) rather than two passes of five.
 //   * k_encode_mid: the fused middle — the low CT stages of the size-k inverse transform on a
 //     contiguous tile of 2^tk values, the spread and mask, and the low GS stages of the size-n
 //     forward transform on the 2^(tk+r) symbols that tile expands to. One HBM read of the row
@@ -183,9 +184,9 @@ __device__ __forceinline__ void tile_stages_ct(uint64_t *lds, uint64_t hi, const
 
 // Stages over bits [lo, lo+nb) of every row of length 2^L. Element e of the tile is
 // (mid = e >> logc, col = e & (C-1)); global index p = hi*2^(lo+nb) + mid*2^lo + lblk*C + col.
-// NB > 0: nb = NB and logc = LOGC, known at compile time.
-template <bool GS, int NB, int LOGC, bool TOP = false>
-__global__ void __launch_bounds__(TILE_THREADS)
+// NB > 0: nb = NB and logc = LOGC, known at compile time. THREADS: the workgroup size.
+template <bool GS, int NB, int LOGC, bool TOP = false, int THREADS = TILE_THREADS>
+__global__ void __launch_bounds__(THREADS)
 k_tile_pass(const uint64_t *__restrict__ in, uint64_t *__restrict__ out, uint64_t in_stride,
             uint64_t out_stride, uint32_t L, uint32_t lo, uint32_t nb, uint32_t logc,
             const uint64_t *__restrict__ tw, const uint64_t *const *in_rows,
@@ -208,18 +209,18 @@ k_tile_pass(const uint64_t *__restrict__ in, uint64_t *__restrict__ out, uint64_
     const uint64_t *src = in_rows ? in_rows[row] : in + row * in_stride;
     uint64_t *dst = out_rows ? out_rows[row] : out + row * out_stride;
 
-    for (uint32_t e = threadIdx.x; e < tile; e += TILE_THREADS) {
+    for (uint32_t e = threadIdx.x; e < tile; e += THREADS) {
         uint64_t mid = e >> logc, col = e & (C - 1);
         lds[e] = src[base + (mid << lo) + col];
     }
     __syncthreads();
 
     if constexpr (NB > 0)
-        tile_stages_ct<GS, NB, LOGC, 0, TOP>(lds, hi, tw);
+        tile_stages_ct<GS, NB, LOGC, 0, TOP, THREADS>(lds, hi, tw);
     else
         tile_stages<GS>(lds, nb, logc, hi, tw);
 
-    for (uint32_t e = threadIdx.x; e < tile; e += TILE_THREADS) {
+    for (uint32_t e = threadIdx.x; e < tile; e += THREADS) {
         uint64_t mid = e >> logc, col = e & (C - 1);
         dst[base + (mid << lo) + col] = lds[e];
     }
@@ -365,17 +366,33 @@ static TilePassFn tile_pass_nb(uint32_t nb) {
     }
 }
 
+// A 10-bit pass: 2^10 strided points x 8 columns, 64 KiB of LDS (two workgroups per CU), so
+// 1024 threads give the CU its 32 waves. Ten stage bits then cost one read and one write of
+// the rows instead of two of each.
+constexpr uint32_t WIDE_PASS_NB = 10, WIDE_PASS_LOGC = 3, WIDE_PASS_THREADS = 1024;
+
+static bool wide_pass(uint32_t nb, uint32_t logc) {
+    return nb == WIDE_PASS_NB && logc == WIDE_PASS_LOGC;
+}
+
 template <bool GS>
 static TilePassFn tile_pass_fn(uint32_t nb, uint32_t logc, bool top) {
+    if (wide_pass(nb, logc))
+        return top ? k_tile_pass<GS, WIDE_PASS_NB, WIDE_PASS_LOGC, true, WIDE_PASS_THREADS>
+                   : k_tile_pass<GS, WIDE_PASS_NB, WIDE_PASS_LOGC, false, WIDE_PASS_THREADS>;
     TilePassFn f = nullptr;
     if (nb <= 9 && logc == PASS_TILE_LOG - nb) f = top ? tile_pass_nb<GS, true>(nb) : tile_pass_nb<GS, false>(nb);
     return f ? f : k_tile_pass<GS, 0, 0>;
 }
 
-// Column bits of a pass over bits [lo, lo + nb): the tile's 2^12 elements, or as many columns
-// as lie below lo.
+static uint32_t tile_pass_threads(uint32_t nb, uint32_t logc) {
+    return wide_pass(nb, logc) ? WIDE_PASS_THREADS : TILE_THREADS;
+}
+
+// Column bits of a pass over bits [lo, lo + nb): the tile's 2^12 elements (2^13 for a 10-bit
+// pass), or as many columns as lie below lo.
 static uint32_t pass_logc(uint32_t lo, uint32_t nb) {
-    const uint32_t want = nb < PASS_TILE_LOG ? PASS_TILE_LOG - nb : 0;
+    const uint32_t want = nb == WIDE_PASS_NB ? WIDE_PASS_LOGC : nb < PASS_TILE_LOG ? PASS_TILE_LOG - nb : 0;
     return lo < want ? lo : want;
 }
 
@@ -441,10 +458,12 @@ Status Encoder::init(Ctx *ctx, uint32_t log_k, uint32_t log_r, uint64_t P, uint3
     const uint32_t log_n = log_k + log_r;
     const uint64_t k = 1ull << log_k;
     tk = log_k < tile_log - log_r ? log_k : tile_log - log_r;
-    // CT passes over k-bits [tk, log_k), run from the top bits down.
-    auto ct = split_bits(tk, log_k, tile_log - 4);
+    // CT passes over k-bits [tk, log_k), run from the top bits down; up to 10 bits per pass
+    // with 64 KiB tiles (tile_log 13), else tile_log - 4
+    const uint32_t max_nb = tile_log >= 13 ? WIDE_PASS_NB : tile_log - 4;
+    auto ct = split_bits(tk, log_k, max_nb);
     ct_passes.assign(ct.rbegin(), ct.rend());
-    gs_passes = split_bits(tk + log_r, log_n, tile_log - 4);
+    gs_passes = split_bits(tk + log_r, log_n, max_nb);
 
     std::vector<uint64_t> zki = twiddle_table(log_k, true);
     std::vector<uint64_t> zn = twiddle_table(log_n, false);
@@ -500,7 +519,7 @@ void Encoder::launch(const uint64_t *src, uint64_t src_stride, const uint64_t *c
         dim3 grid((unsigned)(k >> (nb + logc)), (unsigned)rc);
         size_t shm = sizeof(uint64_t) << (nb + logc);
         KernelTimer timer(ctx, "k_tile_pass<ct>", stream);
-        hipLaunchKernelGGL(tile_pass_fn<false>(nb, logc, lo + nb == log_k), grid, dim3(TILE_THREADS), shm, stream, src, tmp,
+        hipLaunchKernelGGL(tile_pass_fn<false>(nb, logc, lo + nb == log_k), grid, dim3(tile_pass_threads(nb, logc)), shm, stream, src, tmp,
                            src_stride, k, log_k, lo, nb, logc, zk_inv.ptr, src_rows, tmp_rows);
         src = tmp;
         src_stride = k;
@@ -524,7 +543,7 @@ void Encoder::launch(const uint64_t *src, uint64_t src_stride, const uint64_t *c
         dim3 grid((unsigned)(n >> (nb + logc)), (unsigned)rc);
         size_t shm = sizeof(uint64_t) << (nb + logc);
         KernelTimer timer(ctx, "k_tile_pass<gs>", stream);
-        hipLaunchKernelGGL(tile_pass_fn<true>(nb, logc, lo + nb == log_n), grid, dim3(TILE_THREADS), shm, stream, cw, cw, n, n,
+        hipLaunchKernelGGL(tile_pass_fn<true>(nb, logc, lo + nb == log_n), grid, dim3(tile_pass_threads(nb, logc)), shm, stream, cw, cw, n, n,
                            log_n, lo, nb, logc, zn_fwd.ptr, (const uint64_t *const *)cw_rows,
                            cw_rows);
     }

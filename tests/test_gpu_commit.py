@@ -40,9 +40,11 @@ def test_encoding_golden_rate_quarter(dev):
         assert list(cw[0]) == list(hexes(case["codeword"])), (k, P_)
 
 
-@pytest.mark.parametrize("log_k", [0, 1, 2, 5, 8, 10, 11, 12, 13, 14, 16, 18])
+@pytest.mark.parametrize("log_k", [0, 1, 2, 5, 8, 10, 11, 12, 13, 14, 16, 18, 21, 22])
 @pytest.mark.parametrize("log_r", [1, 2])
 def test_encoding_matches_oracle(dev, log_k, log_r):
+    """Rows of 2^log_k against the oracle; 2^21 at rate 1/4 and 2^22 at rate 1/2 run ten stage
+    bits in one strided pass each side, 2^22 at rate 1/4 eleven bits in two."""
     import tessera_amd as T
     k = 1 << log_k
     rows = 3 if log_k <= 14 else 2

@@ -563,6 +563,11 @@ static Status prove_levels(Ctx *ctx, Transcript &tr, std::vector<DevTree> &trees
                 n0.shift = n1.shift = d0.shift = d1.shift = 1;
                 n1.off = d1.off = 1;
                 g.tabs = {n0, n1, d0, d1};
+                // the level this instance proves: its first round's sums at z = 0, 1
+                if (n0.src && t.N[below - 1].ptr && t.D[below - 1].ptr) {
+                    g.parent_n = t.N[below - 1].ptr;
+                    g.parent_d = t.D[below - 1].ptr;
+                }
             }
         }
         SumcheckOut out;
@@ -606,7 +611,13 @@ static Status prove_levels(Ctx *ctx, Transcript &tr, std::vector<DevTree> &trees
                 t.num = gl::add(f[0], gl::mul(rho, gl::sub(f[1], f[0])));
                 t.den = gl::add(f[2], gl::mul(rho, gl::sub(f[3], f[2])));
                 at += 4;
-                if (below < t.h) {
+                // the level above this one is done with; this one stays as the next level's
+                // parents, unless the next is a leaf level with unit numerators (no parents)
+                if (below >= 2) {
+                    t.N[below - 1].release();
+                    t.D[below - 1].release();
+                }
+                if (below + 1 == t.h && t.unit_leaves) {
                     t.N[below].release();
                     t.D[below].release();
                 }

@@ -46,6 +46,10 @@ struct GenInst {
     // not yet bound and fold the bound ones into a running factor.
     bool eq_factor = false;
     std::vector<gl::fp2> eq_point;  // vars coordinates, bit i of the index is coordinate i
+    // REL_FRACTION with stored numerators: the values of the level the instance proves, entry x
+    // = (parent_n[x], parent_d[x]) = (N0 D1 + N1 D0, D0 D1) at x (nullptr: not available). The
+    // first round's sums at z = 0, 1 then take them instead of the tables' products.
+    const gl::fp2 *parent_n = nullptr, *parent_d = nullptr;
 };
 
 // The rounds and final evaluations of a batch whose gamma (and lambda) the caller squeezed.

@@ -391,6 +391,35 @@ __device__ __forceinline__ void sc_eq_nodes(const fp2 *a, const fp2 *b, const fp
     acc[2] = gl::add(acc[2], v2);
 }
 
+// The first round's sums of a fraction level with stored numerators, from the level it proves:
+// F(z) at z = 0, 1 is n + lambda d of the parent nodes 2q, 2q + 1 (the level's values are
+// N0 D1 + N1 D0 and D0 D1 of these very tables), and only z = 2 multiplies the tables' line
+// values: 8 Fp2 products per pair instead of 12.
+template <bool LO>
+__device__ __forceinline__ void sc_item_eq_parents(const ScTables &t, uint32_t base_mask,
+                                                   const fp2 &lambda, const fp2 *__restrict__ eq_lo,
+                                                   const fp2 *__restrict__ eq_hi, uint32_t eq_a,
+                                                   const fp2 *__restrict__ pn,
+                                                   const fp2 *__restrict__ pd, uint64_t q,
+                                                   fp2 (&acc)[3]) {
+    auto line2 = [](const fp2 &x, const fp2 &y) { return gl::sub(gl::add(y, y), x); };
+    fp2 x2[4];
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+        const bool base = (base_mask >> k) & 1;
+        const fp2 a = ld_entry(t.src[k], base, ((2 * q) << t.shift[k]) + t.off[k]);
+        const fp2 b = ld_entry(t.src[k], base, ((2 * q + 1) << t.shift[k]) + t.off[k]);
+        x2[k] = line2(a, b);
+    }
+    const fp2 n0 = pn[2 * q], n1 = pn[2 * q + 1], d0 = pd[2 * q], d1 = pd[2 * q + 1];
+    const fp2 lo = eq_lo[q & ((1ull << eq_a) - 1)];
+    const fp2 e = LO ? lo : gl::mul(lo, eq_hi[q >> eq_a]);
+    const fp2 f2 = gl::add(gl::mul(x2[0], x2[3]), gl::mul(x2[2], gl::add(x2[1], gl::mul(lambda, x2[3]))));
+    acc[0] = gl::add(acc[0], gl::mul(e, gl::add(n0, gl::mul(lambda, d0))));
+    acc[1] = gl::add(acc[1], gl::mul(e, gl::add(n1, gl::mul(lambda, d1))));
+    acc[2] = gl::add(acc[2], gl::mul(e, f2));
+}
+
 // One own round of a level instance whose relation is eq(r, x) F(x) with eq a factor, not a
 // table (degree 3). Modes as k_sc_round's. Pair q of the round is weighted by
 // E[q] = eq_lo[q mod 2^eq_a] eq_hi[q >> eq_a], the eq of the point on the variables above the one
@@ -411,11 +440,15 @@ __device__ __forceinline__ void sc_eq_nodes(const fp2 *a, const fp2 *b, const fp
 // entry scales its sums by it once: one Fp2 product less per pair. Unit numerators also keep
 // E (D0 + D1) and E D0 D1 apart and multiply by lambda once per thread.
 // BIND: out_len > 0 (a round without binding holds half the loaded entries: fewer registers).
-template <int REL, bool UNIT, bool BASE = false, bool LO = false, bool BIND = true>
+// PAR (a first round of a fraction level with stored numerators, no binding): the sums at
+// z = 0, 1 from the parent level's values pn, pd (sc_item_eq_parents).
+template <int REL, bool UNIT, bool BASE = false, bool LO = false, bool BIND = true, bool PAR = false>
 __global__ void __launch_bounds__(SC_THREADS)
 k_sc_round_eq(ScTables t, uint32_t base_mask, bool first, uint64_t out_len, uint64_t pairs,
               const fp2 *__restrict__ r_ptr, fp2 lambda, fp2 *__restrict__ partials,
-              const fp2 *__restrict__ eq_lo, const fp2 *__restrict__ eq_hi, uint32_t eq_a) {
+              const fp2 *__restrict__ eq_lo, const fp2 *__restrict__ eq_hi, uint32_t eq_a,
+              const fp2 *__restrict__ pn, const fp2 *__restrict__ pd) {
+    static_assert(!PAR || (REL == REL_FRACTION && !UNIT && !BASE && !BIND), "parents: fraction sums");
     constexpr bool bind = BIND;
     const fp2 r = bind ? *r_ptr : fp2{0, 0};
     const uint64_t items = pairs > 0 ? pairs : out_len;
@@ -423,8 +456,12 @@ k_sc_round_eq(ScTables t, uint32_t base_mask, bool first, uint64_t out_len, uint
 
     if constexpr (!LO) {
         const uint64_t stride = (uint64_t)gridDim.x * SC_THREADS;
-        for (uint64_t q = (uint64_t)blockIdx.x * SC_THREADS + threadIdx.x; q < items; q += stride)
-            sc_item_eq<REL, UNIT, BASE>(t, base_mask, first, bind, pairs, r, lambda, eq_lo, eq_hi, eq_a, q, acc);
+        for (uint64_t q = (uint64_t)blockIdx.x * SC_THREADS + threadIdx.x; q < items; q += stride) {
+            if constexpr (PAR)
+                sc_item_eq_parents<false>(t, base_mask, lambda, eq_lo, eq_hi, eq_a, pn, pd, q, acc);
+            else
+                sc_item_eq<REL, UNIT, BASE>(t, base_mask, first, bind, pairs, r, lambda, eq_lo, eq_hi, eq_a, q, acc);
+        }
         if (pairs == 0) return;
     } else {
         constexpr int NA = (REL == REL_FRACTION && UNIT) ? 6 : 3;
@@ -456,8 +493,12 @@ k_sc_round_eq(ScTables t, uint32_t base_mask, bool first, uint64_t out_len, uint
                 h = qb >> eq_a;
             }
             const uint64_t q = qb + threadIdx.x;
-            if (q < q1)
-                sc_item_eq<REL, UNIT, BASE, true, NA>(t, base_mask, first, bind, pairs, r, lambda, eq_lo, eq_hi, eq_a, q, run);
+            if (q < q1) {
+                if constexpr (PAR)
+                    sc_item_eq_parents<true>(t, base_mask, lambda, eq_lo, eq_hi, eq_a, pn, pd, q, run);
+                else
+                    sc_item_eq<REL, UNIT, BASE, true, NA>(t, base_mask, first, bind, pairs, r, lambda, eq_lo, eq_hi, eq_a, q, run);
+            }
         }
         if (q0 < q1) {
             if (hi_one) {
@@ -1061,30 +1102,42 @@ static void launch_round(dim3 grid, hipStream_t s, const ScTables &t, uint32_t F
 template <int REL, bool UNIT, bool BASE = false>
 static void launch_round_eq(dim3 grid, hipStream_t s, const ScTables &t, uint32_t base_mask,
                             bool first, uint64_t out_len, uint64_t pairs, const fp2 *r, fp2 lambda,
-                            fp2 *partials, const EqSplit &eq) {
+                            fp2 *partials, const EqSplit &eq, const fp2 *pn = nullptr,
+                            const fp2 *pd = nullptr) {
     // LO when eq_hi is the single entry 1, or when its runs span whole passes of a block and
     // every thread has at least 8 pairs (one run's scaling, 3 products, then costs less than the
     // product per pair it saves)
     const bool lo = pairs > 0 && ((pairs >> eq.a) <= 1 ||
                                   ((1ull << eq.a) >= SC_THREADS && pairs >= 8ull * grid.x * SC_THREADS));
+#define TSR_ROUND_EQ(LO_, BIND_, PAR_)                                                              \
+    hipLaunchKernelGGL((k_sc_round_eq<REL, UNIT, BASE, LO_, BIND_, PAR_>), grid, dim3(SC_THREADS), 0, s, \
+                       t, base_mask, first, out_len, pairs, r, lambda, partials, eq.lo, eq.hi, eq.a, \
+                       pn, pd)
+    if constexpr (REL == REL_FRACTION && !UNIT && !BASE) {
+        if (pn && pd && first && out_len == 0 && pairs > 0) {
+            if (lo)
+                TSR_ROUND_EQ(true, false, true);
+            else
+                TSR_ROUND_EQ(false, false, true);
+            return;
+        }
+    }
     if (lo && out_len == 0)
-        hipLaunchKernelGGL((k_sc_round_eq<REL, UNIT, BASE, true, false>), grid, dim3(SC_THREADS), 0, s,
-                           t, base_mask, first, out_len, pairs, r, lambda, partials, eq.lo, eq.hi, eq.a);
+        TSR_ROUND_EQ(true, false, false);
     else if (lo)
-        hipLaunchKernelGGL((k_sc_round_eq<REL, UNIT, BASE, true>), grid, dim3(SC_THREADS), 0, s, t,
-                           base_mask, first, out_len, pairs, r, lambda, partials, eq.lo, eq.hi, eq.a);
+        TSR_ROUND_EQ(true, true, false);
     else if (out_len == 0)
-        hipLaunchKernelGGL((k_sc_round_eq<REL, UNIT, BASE, false, false>), grid, dim3(SC_THREADS), 0, s,
-                           t, base_mask, first, out_len, pairs, r, lambda, partials, eq.lo, eq.hi, eq.a);
+        TSR_ROUND_EQ(false, false, false);
     else
-        hipLaunchKernelGGL((k_sc_round_eq<REL, UNIT, BASE>), grid, dim3(SC_THREADS), 0, s, t,
-                           base_mask, first, out_len, pairs, r, lambda, partials, eq.lo, eq.hi, eq.a);
+        TSR_ROUND_EQ(false, true, false);
+#undef TSR_ROUND_EQ
 }
 
 static void launch_round_d(uint32_t d, int rel, dim3 grid, hipStream_t s, const ScTables &t,
                            uint32_t F, uint32_t base_mask, bool first, uint64_t out_len,
                            uint64_t pairs, const fp2 *r, fp2 lambda, fp2 *partials,
-                           const ScTerms &terms, const EqSplit &eq = EqSplit()) {
+                           const ScTerms &terms, const EqSplit &eq = EqSplit(),
+                           const fp2 *pn = nullptr, const fp2 *pd = nullptr) {
     if (eq.lo) {  // tree levels: degree 3, tables T0 T1 or N0 N1 D0 D1
         if (rel == REL_PRODUCT && first && (base_mask & 3) == 3)
             launch_round_eq<REL_PRODUCT, false, true>(grid, s, t, base_mask, first, out_len, pairs,
@@ -1097,7 +1150,7 @@ static void launch_round_d(uint32_t d, int rel, dim3 grid, hipStream_t s, const 
                                                 lambda, partials, eq);
         else
             launch_round_eq<REL_FRACTION, false>(grid, s, t, base_mask, first, out_len, pairs, r,
-                                                 lambda, partials, eq);
+                                                 lambda, partials, eq, pn, pd);
         return;
     }
     switch (d) {
@@ -1531,7 +1584,8 @@ Status run_batch(Ctx *ctx, Transcript &tr, const std::string &p, uint32_t D,
             launch_round_d(D, insts[i].rel, dim3(blocks_for_pairs(ctx, pairs)), s, t,
                            (uint32_t)insts[i].tabs.size(), bm, true, 0, pairs, nullptr, lambda,
                            partials.ptr + pstride * i, sterms[i],
-                           splits[i].empty() ? EqSplit() : splits[i][0]);
+                           splits[i].empty() ? EqSplit() : splits[i][0], insts[i].parent_n,
+                           insts[i].parent_d);
         }
     }
     for (uint32_t j = 0; j < j0; j++) {

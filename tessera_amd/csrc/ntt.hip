@@ -195,9 +195,9 @@ k_tile_pass(const uint64_t *__restrict__ in, uint64_t *__restrict__ out, uint64_
     const uint32_t C = 1u << logc;
     const uint32_t tiles_per_hi = 1u << (lo - logc);
     // Workgroups go to the 8 XCDs round-robin by linear id. Adjacent tiles read the two halves
-    // of the same 128-byte lines (runs of 64 bytes), so tile t runs on XCD t / (tiles / 8):
-    // each XCD takes a contiguous range of tiles, and the two halves of a line meet in its L2
-    // (6% faster for the 9-bit pass over 256 rows of 2^22).
+    // of the same 128-byte lines (runs of 64 bytes); tile t runs on XCD t / (tiles / 8), so each
+    // XCD takes a contiguous range of tiles. The 9-bit pass over 256 rows of 2^22 runs 6% faster
+    // so; its HBM bytes (FETCH_SIZE, WRITE_SIZE) are the algorithmic 8 + 8 GiB either way.
     uint32_t bx = blockIdx.x;
     if ((gridDim.x & 7) == 0) bx = (bx & 7) * (gridDim.x >> 3) + (bx >> 3);
     const uint64_t hi = bx / tiles_per_hi;

@@ -224,8 +224,10 @@ def run_sumcheck(reps=3):
     d_lk, d_lv = dev.upload(keys[idx]), dev.upload(vals[idx])
     d_idx = dev.upload(idx.astype(np.uint64))
 
+    d_m = dev.buffer(keys.size)  # 2^16 rows: the counts stay on the device
+
     def lookups():
-        m = multiplicities(dev, keys.size, d_idx)
+        m = multiplicities(dev, keys.size, d_idx, out=d_m)
         t = Transcript(ctx_tce)
         prove_logup(dev, t, [LogUpInstance(d_keys, d_vals, d_lk, d_lv, m)])
 

@@ -209,6 +209,11 @@ k_sum_partials(const fp2 *__restrict__ partials, uint32_t n, fp2 *__restrict__ o
 }
 
 // multiplicities (instance.rs:225-253): exact counts over 2^{h_t} rows.
+// counts[0] += x (the padding lookups of row 0), on the device.
+__global__ void k_add_first(uint64_t *__restrict__ counts, uint64_t x) {
+    if (threadIdx.x == 0) counts[0] += x;
+}
+
 __global__ void k_count_rows(const uint64_t *__restrict__ rows, uint64_t n, uint64_t limit,
                              unsigned long long *__restrict__ counts, unsigned *__restrict__ bad) {
     uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -894,6 +899,21 @@ tsr_status tsr_logup_multiplicities(tsr_ctx *ctx_, uint64_t rows, const uint64_t
     uint32_t hbad = 0;
     e = hipMemcpyAsync(&hbad, bad.ptr, 4, hipMemcpyDeviceToHost, s);
     if (e != hipSuccess) return hip_fail(e, "copy").code;
+    if (is_device_ptr(out)) {
+        // device output: the counts stay on the device (only the row check comes back first,
+        // so that a bad lookup leaves `out` untouched)
+        e = hipStreamSynchronize(s);
+        if (e != hipSuccess) return hip_fail(e, "sync").code;
+        if (hbad) return Status::fail(TSR_BAD_SHAPE, "LookupRow: a lookup names no row of the table").code;
+        if (lookups)
+            hipLaunchKernelGGL(k_add_first, dim3(1), dim3(64), 0, s, counts.ptr,
+                               (1ull << ceil_log2(lookups)) - lookups);
+        e = hipMemcpyAsync(out, counts.ptr, 8 * leaves, hipMemcpyDeviceToDevice, s);
+        if (e != hipSuccess) return hip_fail(e, "copy").code;
+        e = hipStreamSynchronize(s);
+        if (e != hipSuccess) return hip_fail(e, "sync").code;
+        return check_launch("multiplicities").code;
+    }
     std::vector<uint64_t> h(leaves);
     e = hipMemcpyAsync(h.data(), counts.ptr, 8 * leaves, hipMemcpyDeviceToHost, s);
     if (e != hipSuccess) return hip_fail(e, "copy").code;

@@ -54,12 +54,20 @@ def _count(x):
     return x.count if isinstance(x, DeviceBuffer) else np.asarray(x).size
 
 
-def multiplicities(device, rows: int, lookup_rows):
-    """Exact counts over 2^ceil(log2 rows) entries, row 0 counting the lookup padding."""
+def multiplicities(device, rows: int, lookup_rows, out=None):
+    """Exact counts over 2^ceil(log2 rows) entries, row 0 counting the lookup padding. out: a
+    DeviceBuffer of at least that many u64 to count into on the device (returned), else a new
+    host array."""
     lr = lookup_rows if isinstance(lookup_rows, DeviceBuffer) else np.ascontiguousarray(lookup_rows, np.uint64)
     h = (rows - 1).bit_length() if rows > 1 else 0
-    out = np.zeros(1 << h, dtype=np.uint64)
-    check(lib().tsr_logup_multiplicities(device.handle, rows, ptr_of(lr), _count(lr), out.ctypes.data))
+    if out is None:
+        out = np.zeros(1 << h, dtype=np.uint64)
+        dst = out.ctypes.data
+    else:
+        if not isinstance(out, DeviceBuffer) or out.count < (1 << h):
+            raise ValueError(f"out must be a DeviceBuffer of at least {1 << h} entries")
+        dst = out.ptr
+    check(lib().tsr_logup_multiplicities(device.handle, rows, ptr_of(lr), _count(lr), dst))
     return out
 
 

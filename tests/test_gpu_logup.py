@@ -192,6 +192,17 @@ def test_multiplicities(dev):
     import tessera_amd as T
     with pytest.raises(T.BadShape, match="LookupRow"):
         multiplicities(dev, 10, np.array([3, 10], dtype=np.uint64))
+    # into a device buffer: the same counts; a bad row leaves the buffer as it was
+    out = dev.buffer(1024)
+    d_rows = dev.upload(rows)
+    assert multiplicities(dev, 1000, d_rows, out=out) is out
+    assert np.array_equal(out.download(), O.multiplicities(1000, rows))
+    before = out.download()
+    with pytest.raises(T.BadShape, match="LookupRow"):
+        multiplicities(dev, 1000, dev.upload(np.array([3, 1000], dtype=np.uint64)), out=out)
+    assert np.array_equal(out.download(), before)
+    with pytest.raises(ValueError):
+        multiplicities(dev, 1000, d_rows, out=dev.buffer(1000))
 
 
 @pytest.mark.parametrize("rows,lookups", [(1, 70000), (1 << 14, 1), ((1 << 14) + 1, 300000),

@@ -390,7 +390,8 @@ tsr_status tsr_prove_trees(tsr_ctx *ctx, tsr_transcript *t, const tsr_forest *fo
                            uint64_t log_cap, uint64_t *log_len);
 /* multiplicities(table, lookups): out[j] = lookups of row j over 2^ceil(log2 rows) entries, row 0
  * also counting the lookup tree's 2^ceil(log2 N) - N padding leaves. LookupRow (BAD_SHAPE) when a
- * lookup names no row. */
+ * lookup names no row, `out` untouched. lookup_rows and out may each be host or device memory; a
+ * device `out` is written on the device. */
 tsr_status tsr_logup_multiplicities(tsr_ctx *ctx, uint64_t rows, const uint64_t *lookup_rows,
                                     uint64_t lookups, uint64_t *out);
 

@@ -39,6 +39,12 @@ LOG_K, LOG_R, ROWS, BLOCK, QUERIES = 20, 2, 256, 32, 64
 SEED = 0x7E55E7A
 HBM_PEAK_GBS = 8000.0
 SERVE_RATE_HZ, SERVE_REQUESTS = 400.0, 256
+SC_BYTES_PER_INDEX = 80  # the n = 24 eq*a*b sumcheck's algorithmic HBM bytes per index
+LOGUP_ROUNDS = 22 * 21 // 2  # level l of the 2^22-leaf lookup tree has l variables
+PROFILE_STEPS = 3  # headline steps of the separate, instrumented pass that times each kernel
+# BLAKE3 compressions of one column of the headline: the column-init input (40 B, one block) and
+# ceil(256 / 32) = 8 column-step inputs of 32 + 32 * 8 = 288 B (5 blocks each)
+CHAIN_COMPRESSIONS_PER_COLUMN = 1 + (ROWS // BLOCK) * 5
 
 
 def log(msg):
@@ -115,14 +121,18 @@ def run_headline(steps, warmup):
     for _ in range(warmup):
         step()
     dev.synchronize()
-    lib().tsr_ctx_profile(dev.handle, 2)
-    lib().tsr_ctx_profile(dev.handle, 1)
-    dev.synchronize()
+    # the timed steps run with no instrumentation
     t0 = time.perf_counter()
     for _ in range(steps):
         root, ans = step()
     dev.synchronize()
     t1 = time.perf_counter()
+    # per-kernel times in a separate pass: HIP events around every launch on the library's stream
+    lib().tsr_ctx_profile(dev.handle, 2)
+    lib().tsr_ctx_profile(dev.handle, 1)
+    for _ in range(PROFILE_STEPS):
+        step()
+    dev.synchronize()
     lib().tsr_ctx_profile(dev.handle, 0)
     kernels = {}
     for name in ("k_tile_pass<ct>", "k_encode_mid", "k_tile_pass<gs>", "k_chain_leaves",
@@ -323,7 +333,7 @@ def cpu_served_sample(reqs):
     """The oracle (C: decode, commit, sumcheck; all host threads) on the first requests of the
     same workload: proofs/s for one request at a time."""
     from oracle import oracle as O
-    threads = os.cpu_count() or 1
+    threads = host_threads()
     w = O.model_weights(SEED)
     t0 = time.perf_counter()
     tokens = 0
@@ -440,38 +450,136 @@ def trace_summary(db, title, command):
     return "\n".join(out) + "\n"
 
 
+def rfc0020(fn, units, warmups=3, runs=10):
+    """RFC-0020's measurement protocol (crates/tessera-bench/src/measure.rs:16-33): warm-up runs,
+    measured runs, ns per unit as median, min, max and the coefficient of variation (sample
+    standard deviation over the mean); a run whose CV is above 3% is repeated once."""
+    def once():
+        for _ in range(warmups):
+            fn()
+        ts = []
+        for _ in range(runs):
+            t0 = time.perf_counter()
+            fn()
+            ts.append(1e9 * (time.perf_counter() - t0) / units)
+        ts = np.array(ts)
+        cv = float(ts.std(ddof=1) / ts.mean()) if runs > 1 and ts.mean() > 0 else 0.0
+        return {"median": round(float(np.median(ts)), 4), "min": round(float(ts.min()), 4),
+                "max": round(float(ts.max()), 4), "cv": round(cv, 4), "runs": runs, "warmups": warmups}
+    st = once()
+    if st["cv"] > 0.03:
+        st = once()
+        st["repeated"] = True
+    st["noisy"] = st["cv"] > 0.03
+    return st
+
+
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for ln in f:
+                if ln.startswith("model name"):
+                    return ln.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def host_threads():
+    try:
+        return len(os.sched_getaffinity(0))
+    except AttributeError:
+        return os.cpu_count() or 1
+
+
+# The reference's single-thread CPU microbenchmarks (docs/perf/baselines/github-ubuntu-24.04.json,
+# GitHub ubuntu-24.04 x86_64 runner; sizes from crates/tessera-bench/benches/primitives.rs:106-113).
+REF_CPU_NS = {"field/fp-mul": 1.5947, "ntt/fp": 29.4802, "hash/column-chains": 27.9673,
+              "sumcheck/product-3": 66.5614, "logup/prove": 207.9136, "trees/forest": 53.1683}
+
+
+def cpu_calibration():
+    """The oracle single-threaded at the reference's microbenchmark sizes, beside the reference's
+    published numbers, so that the CPU baseline can be judged against the reference's own CPU
+    path: 2^20 Fp products, a forward NTT of 2^20 points, one block of 32 rows into 2^14 column
+    chains, a degree-3 sumcheck of three Fp tables of 2^20, the LogUp stage of 2^18 lookups into
+    2^16 rows with values, a product forest of 2^10 trees of 2^12 leaves. Runs above 0.2 s use
+    one warm-up and three runs."""
+    from oracle import oracle as O
+    from tests.helpers import context_tce, splitmix_uniform
+    out = {}
+    ctx = context_tce(statement_digest=bytes([0x65] * 32), session_id=bytes([0x65] * 16))
+
+    def add(name, fn, units):
+        t0 = time.perf_counter()
+        fn()
+        long = time.perf_counter() - t0 > 0.2
+        st = rfc0020(fn, units, *((1, 3) if long else (3, 10)))
+        st["reference_ns"] = REF_CPU_NS[name]
+        st["ratio_to_reference"] = round(st["median"] / REF_CPU_NS[name], 3)
+        out[name] = st
+
+    n = 1 << 20
+    a, b, c = splitmix_uniform(n, 1), splitmix_uniform(n, 2), splitmix_uniform(n, 3)
+    add("field/fp-mul", lambda: O.fp_mul_vec(a, b), n)
+    add("ntt/fp", lambda: O.ntt(a.copy(), 20), n)
+    rows = splitmix_uniform(32 << 14, 4)
+    add("hash/column-chains", lambda: O.column_chains(bytes([0x65] * 32), rows, 1 << 14, 32), 32 << 14)
+    add("sumcheck/product-3", lambda: O.prove_batch(O.OracleTranscript(ctx), "stage/0", 3,
+                                                    [[(a, 0, 0), (b, 0, 0), (c, 0, 0)]], threads=1), n)
+    R, N = 1 << 16, 1 << 18
+    keys = np.arange(R, dtype=np.uint64)
+    vals = splitmix_uniform(R, 5)
+    picks = (splitmix_uniform(N, 6) % np.uint64(R)).astype(np.uint64)
+    inst = {"keys": keys, "values": vals, "lookup_keys": keys[picks], "lookup_values": vals[picks],
+            "multiplicities": O.multiplicities(R, picks)}
+    add("logup/prove", lambda: O.prove_logup(O.OracleTranscript(ctx), [inst], [0, 1]), N)
+    leaves = splitmix_uniform(1 << 22, 7)
+    add("trees/forest", lambda: O.prove_trees(O.OracleTranscript(ctx), [(leaves, 12)]), 1 << 22)
+    return out
+
+
 def cpu_baseline():
-    """The oracle (C, all host threads) on a bounded sample of the same workload, scaled per unit:
-    encode 64 of the 256 rows (per-row cost is uniform), chain 2^18 of the 2^22 columns over all 256
-    rows (per-column cost is uniform), and a Merkle tree over 2^18 leaves (per-node cost uniform)."""
+    """The oracle (C, all host threads) on a bounded sample of the same workload, scaled per unit,
+    measured with RFC-0020's protocol: encode 32 of the 256 rows (per-row cost is uniform), chain
+    2^17 of the 2^22 columns over all 256 rows (per-column cost is uniform), and a Merkle tree over
+    2^17 leaves (per-node cost uniform). value = input elements per second of the whole commit at
+    the sample's median rate. Also the single-thread calibration against the reference's own
+    CPU microbenchmarks."""
     from oracle import oracle as O
     from tests.helpers import splitmix_uniform
 
-    threads = os.cpu_count() or 1
+    threads = host_threads()
     k, n = 1 << LOG_K, 1 << (LOG_K + LOG_R)
-    enc_rows, cols = 64, 1 << 18
+    enc_rows, cols = 32, 1 << 17
     rows = splitmix_uniform(enc_rows * k, SEED)
-    t0 = time.perf_counter()
     cw = O.encode_rows(rows, LOG_K, LOG_R, threads=threads)
-    t_enc = (time.perf_counter() - t0) * (ROWS / enc_rows)
     # chains over all 256 rows for `cols` columns: a (256 x cols) block of codeword symbols
     block = np.ascontiguousarray(np.tile(cw[:, :cols], (ROWS // enc_rows, 1)))
-    t0 = time.perf_counter()
     leaves = O.column_chains(bytes([7] * 32), block, cols, BLOCK, threads=threads)
-    t_chain = (time.perf_counter() - t0) * (n / cols)
-    t0 = time.perf_counter()
-    O.merkle_tree(leaves, threads=threads)
-    t_merkle = (time.perf_counter() - t0) * (n / cols)
-    total = t_enc + t_chain + t_merkle
+    parts = {
+        "encode": (lambda: O.encode_rows(rows, LOG_K, LOG_R, threads=threads), ROWS / enc_rows),
+        "chains": (lambda: O.column_chains(bytes([7] * 32), block, cols, BLOCK, threads=threads), n / cols),
+        "merkle": (lambda: O.merkle_tree(leaves, threads=threads), n / cols),
+    }
+    stats, total = {}, 0.0
+    for name, (fn, scale) in parts.items():
+        st = rfc0020(fn, 1)  # ns per run of the sample
+        stats[name] = {"sample_ms": round(st["median"] / 1e6, 3), "cv": st["cv"], "noisy": st["noisy"],
+                       "scaled_s": round(st["median"] * 1e-9 * scale, 3)}
+        total += st["median"] * 1e-9 * scale
     return {
-        "value": ROWS * k / total,
+        "value": round(ROWS * k / total, 1),
         "unit": "elem/s",
         "cores": threads,
         "kind": "port",
-        "sample": (f"oracle C restatement, {threads} threads: encode {enc_rows}/256 rows "
-                   f"({t_enc:.2f} s scaled), chains over 256 rows for 2^18/2^22 columns "
-                   f"({t_chain:.2f} s scaled), Merkle over 2^18 leaves ({t_merkle:.2f} s scaled); "
-                   f"est. {total:.1f} s per commit"),
+        "cpu_model": cpu_model(),
+        "protocol": "RFC-0020: 3 warm-ups + 10 measured runs per part, median (CV > 3% repeated once)",
+        "sample": (f"oracle C restatement, {threads} threads: encode {enc_rows}/256 rows, chains over "
+                   f"256 rows for 2^17/2^22 columns, Merkle over 2^17 leaves, each scaled per unit; "
+                   f"est. {total:.2f} s per commit"),
+        "parts": stats,
+        "single_thread_calibration": cpu_calibration(),
     }
 
 
@@ -554,6 +662,55 @@ def run_refshapes(reps=3):
     return res
 
 
+def headline_roofline(dom, kernels, rows_profiled):
+    """The dominant kernel's roofline. The encoder's kernels are integer-VALU bound (64-bit
+    modular multiplies from 32-bit multiply-adds), so the binding roof is the butterfly rate
+    against the rate the same butterfly reaches register-resident on this device
+    (tsr_butterfly_peak); the HBM fraction (algorithmic bytes and measured traffic) is beside it."""
+    launches = kernels[dom]["launches"]
+    rows_per_launch = rows_profiled // launches  # the encoder may split rows over launches
+    avg_s = kernels[dom]["avg_ms"] * 1e-3
+    b = kernel_bytes(dom, rows_per_launch, LOG_K, LOG_R)
+    gbs = b / avg_s / 1e9
+    hbm = {"achieved": round(gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+           "frac": round(gbs / HBM_PEAK_GBS, 4), "algorithmic_bytes": b,
+           "traffic": traffic_from_profiles(dom, rows_per_launch)}
+    bf = kernel_butterflies(dom, rows_per_launch, LOG_K, LOG_R)
+    out = {"kernel": dom, "rows_per_launch": rows_per_launch, "avg_ms": round(kernels[dom]["avg_ms"], 4)}
+    if bf:
+        peak = butterfly_peak()
+        rate = bf / avg_s
+        out.update({"bound": "valu", "achieved": round(rate / 1e9, 1), "peak": round(peak / 1e9, 1),
+                    "unit": "G butterflies/s", "frac": round(rate / peak, 4),
+                    "peak_kind": "measured: the same Goldilocks butterfly register-resident "
+                                 "(tsr_butterfly_peak), ~42 VALU instructions per butterfly",
+                    "butterflies": bf, "traffic": hbm["traffic"], "hbm": hbm})
+    else:
+        out.update({"bound": "hbm", **{k: hbm[k] for k in ("achieved", "peak", "unit", "frac", "traffic")},
+                    "algorithmic_bytes": b})
+    return out
+
+
+def chain_roofline(kernels):
+    """k_chain_leaves of the headline: BLAKE3 compressions per second and its HBM fraction (it
+    reads every codeword symbol once and writes one leaf per column)."""
+    c = kernels.get("k_chain_leaves")
+    if not c:
+        return None
+    n = 1 << (LOG_K + LOG_R)
+    launches_per_step = max(1, c["launches"] // PROFILE_STEPS)
+    per_step_s = c["avg_ms"] * 1e-3 * launches_per_step
+    comp = CHAIN_COMPRESSIONS_PER_COLUMN * n
+    b = kernel_bytes("k_chain_leaves", ROWS, LOG_K, LOG_R)
+    gbs = b / per_step_s / 1e9
+    return {"kernel": "k_chain_leaves", "bound": "valu", "ms_per_step": round(per_step_s * 1e3, 4),
+            "compressions": comp, "achieved": round(comp / per_step_s / 1e9, 2),
+            "unit": "G BLAKE3 compressions/s",
+            "hbm": {"achieved": round(gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                    "frac": round(gbs / HBM_PEAK_GBS, 4), "algorithmic_bytes": b,
+                    "traffic": traffic_from_profiles("k_chain_leaves", ROWS)}}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -600,24 +757,7 @@ def main():
     dom = max(kernels, key=lambda x: kernels[x]["total_ms"]) if kernels else None
     roofline = None
     if dom:
-        # the encoder may split the rows over several launches (pipelined chunks)
-        rows_per_launch = ROWS * a.steps // kernels[dom]["launches"]
-        b = kernel_bytes(dom, rows_per_launch, LOG_K, LOG_R)
-        achieved = b / (kernels[dom]["avg_ms"] * 1e-3) / 1e9
-        roofline = {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 1),
-                    "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
-                    "traffic": traffic_from_profiles(dom, rows_per_launch), "algorithmic_bytes": b,
-                    "rows_per_launch": rows_per_launch,
-                    "avg_ms": round(kernels[dom]["avg_ms"], 4)}
-        bf = kernel_butterflies(dom, rows_per_launch, LOG_K, LOG_R)
-        if bf:
-            peak = butterfly_peak()
-            rate = bf / (kernels[dom]["avg_ms"] * 1e-3)
-            # the integer NTT is bound by the vector ALUs: its butterfly rate against the measured
-            # register-resident rate of the same butterfly
-            roofline["valu"] = {"butterflies": bf, "achieved": round(rate / 1e9, 1),
-                                "peak": round(peak / 1e9, 1), "unit": "G butterflies/s",
-                                "frac": round(rate / peak, 4)}
+        roofline = headline_roofline(dom, kernels, ROWS * PROFILE_STEPS)
     extra = {}
     if a.only is None:
         log("plain: 256 x 2^14 commit + Sum a*b")
@@ -655,6 +795,19 @@ def main():
         "cpu_baseline": cpu,
     }
     line.update(extra)
+    line["rooflines"] = {"k_chain_leaves": chain_roofline(kernels)}
+    if "sumcheck" in extra:
+        sc = extra["sumcheck"]
+        gbs = SC_BYTES_PER_INDEX * (1 << 24) / (sc["eq_a_b_n24_ms"] * 1e-3) / 1e9
+        line["rooflines"]["sumcheck_eq_a_b_n24"] = {
+            "bound": "hbm", "achieved": round(gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": round(gbs / HBM_PEAK_GBS, 4), "algorithmic_bytes": SC_BYTES_PER_INDEX * (1 << 24),
+            "note": f"{SC_BYTES_PER_INDEX} B per hypercube index: two Fp tables read in round 0, "
+                    "then fused bind + sums over the folded Fp2 halves, eq(r, x) as a point"}
+        line["rooflines"]["logup_2p22_into_2p16"] = {
+            "bound": "latency", "rounds": LOGUP_ROUNDS,
+            "us_per_round": round(1e3 * sc["logup_2p22_into_2p16_ms"] / LOGUP_ROUNDS, 2),
+            "note": "22 levels of the lookup tree, a level of l variables proven in l rounds"}
     print(json.dumps(line), flush=True)
 
 

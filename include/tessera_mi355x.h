@@ -391,7 +391,10 @@ tsr_status tsr_prove_trees(tsr_ctx *ctx, tsr_transcript *t, const tsr_forest *fo
 /* multiplicities(table, lookups): out[j] = lookups of row j over 2^ceil(log2 rows) entries, row 0
  * also counting the lookup tree's 2^ceil(log2 N) - N padding leaves. LookupRow (BAD_SHAPE) when a
  * lookup names no row, `out` untouched. lookup_rows and out may each be host or device memory; a
- * device `out` is written on the device. */
+ * device `out` is written on the device. A table of no row has no leaf: with no lookup that is
+ * OK and nothing is written (`out` may be NULL), with any lookup it is LookupRow
+ * (crates/tessera-logup/src/instance.rs:225-253). The device copies of the rows and counts are
+ * cleared before the library reuses their memory. */
 tsr_status tsr_logup_multiplicities(tsr_ctx *ctx, uint64_t rows, const uint64_t *lookup_rows,
                                     uint64_t lookups, uint64_t *out);
 

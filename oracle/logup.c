@@ -17,9 +17,9 @@
 
 typedef orc_fp2 F2;
 static F2 f2(uint64_t a, uint64_t b) { F2 r = {a, b}; return r; }
-static F2 add2(F2 a, F2 b) { return orc_fp2_add(a, b); }
-static F2 sub2(F2 a, F2 b) { return orc_fp2_sub(a, b); }
-static F2 mul2(F2 a, F2 b) { return orc_fp2_mul(a, b); }
+static F2 add2(F2 a, F2 b) { return fp2_add(a, b); }
+static F2 sub2(F2 a, F2 b) { return fp2_sub(a, b); }
+static F2 mul2(F2 a, F2 b) { return fp2_mul(a, b); }
 
 static uint32_t ceil_log2(uint64_t x) {
     uint32_t h = 0;
@@ -112,7 +112,7 @@ static void free_tree(tree_t *t) {
 /* compress: key + beta value, or the key alone (prover.rs:201-207). */
 static F2 compress_pair(F2 beta, uint64_t key, const uint64_t *value) {
     if (!value) return f2(key, 0);
-    return add2(orc_fp2_mul_base(beta, *value), f2(key, 0));
+    return add2(fp2_mul_base(beta, *value), f2(key, 0));
 }
 
 static void log_put(uint64_t *log, uint64_t cap, uint64_t *len, uint64_t v) {
@@ -215,7 +215,7 @@ static void prove_level(orc_transcript *tr, uint32_t level, tree_t **sched, uint
         for (uint32_t i = 0; i < count; i++) {
             uint32_t pad = n_max - act[i]->plen;
             if (j < pad) {
-                F2 v = orc_fp2_mul_base(mul2(w[i], c[i]), 1ULL << (pad - j - 1));
+                F2 v = fp2_mul_base(mul2(w[i], c[i]), 1ULL << (pad - j - 1));
                 for (int z = 0; z < 3; z++) msg[z] = add2(msg[z], v);
                 continue;
             }
@@ -377,7 +377,7 @@ int orc_prove_logup(orc_transcript *tr, const orc_logup_instance *inst, uint32_t
         }
         build_tree(&trees[2 * k], ha, N, D);
         for (uint64_t x = 0; x < nt; x++) {
-            N[x] = f2(orc_fp_sub(0, in->multiplicities[x]), 0);
+            N[x] = f2(fp_sub(0, in->multiplicities[x]), 0);
             D[x] = x < in->rows
                        ? sub2(alpha, compress_pair(beta, in->keys[x], in->values ? &in->values[x] : NULL))
                        : pad;

@@ -25,8 +25,8 @@ void orc_row_coefficients(uint32_t row_vars, const uint64_t *rows, uint64_t N,
             for (uint64_t i = 0; i < N; i++) {
                 orc_fp2 e = scale;
                 for (uint32_t t = 0; t < row_vars; t++)
-                    e = orc_fp2_mul(e, (rows[i] >> t) & 1 ? pt[t] : orc_fp2_sub(one, pt[t]));
-                out[(uint64_t)g * N + i] = orc_fp2_add(out[(uint64_t)g * N + i], e);
+                    e = fp2_mul(e, (rows[i] >> t) & 1 ? pt[t] : fp2_sub(one, pt[t]));
+                out[(uint64_t)g * N + i] = fp2_add(out[(uint64_t)g * N + i], e);
             }
         }
         if (g < groups) claim += nc;
@@ -41,7 +41,7 @@ void orc_combine(const orc_fp2 *coefficients, uint32_t count, const uint64_t *ma
         for (uint64_t i = 0; i < width; i++) {
             orc_fp2 acc = {0, 0};
             for (uint64_t r = 0; r < N; r++)
-                acc = orc_fp2_add(acc, orc_fp2_mul_base(coefficients[(uint64_t)g * N + r], matrix[r * width + i]));
+                acc = fp2_add(acc, fp2_mul_base(coefficients[(uint64_t)g * N + r], matrix[r * width + i]));
             out[(uint64_t)g * width + i] = acc;
         }
 }

@@ -30,14 +30,13 @@
 
 #include <stddef.h>
 #include <stdint.h>
+#include "fp.h"
 
 #ifdef __cplusplus
 extern "C" {
 #endif
 
 #define ORC_P 0xFFFFFFFF00000001ULL
-
-typedef struct { uint64_t c0, c1; } orc_fp2;
 
 /* ---- field (crates/tessera-field/src/goldilocks.rs, ext2.rs) ---- */
 uint64_t orc_fp_add(uint64_t a, uint64_t b);

@@ -18,9 +18,9 @@
 typedef orc_fp2 F2;
 
 static F2 f2(uint64_t a, uint64_t b) { F2 r = {a, b}; return r; }
-static F2 add2(F2 a, F2 b) { return orc_fp2_add(a, b); }
-static F2 sub2(F2 a, F2 b) { return orc_fp2_sub(a, b); }
-static F2 mul2(F2 a, F2 b) { return orc_fp2_mul(a, b); }
+static F2 add2(F2 a, F2 b) { return fp2_add(a, b); }
+static F2 sub2(F2 a, F2 b) { return fp2_sub(a, b); }
+static F2 mul2(F2 a, F2 b) { return fp2_mul(a, b); }
 
 /* eq_table (tessera-poly eq.rs:23-45): doubling per coordinate, x_1 the lowest bit. */
 void orc_eq_table(const uint64_t *point, uint32_t n, uint64_t *out) {
@@ -70,9 +70,9 @@ void orc_round_eval(const uint64_t *evals, uint32_t d, const uint64_t claim[2], 
         for (uint32_t m = 0; m <= d; m++) {
             if (m == k) continue;
             num = mul2(num, sub2(rr, f2(m, 0)));
-            den = orc_fp_mul(den, orc_fp_sub(k, m));
+            den = fp_mul(den, fp_sub(k, m));
         }
-        value = add2(value, mul2(at[k], orc_fp2_mul_base(num, orc_fp_inv(den))));
+        value = add2(value, mul2(at[k], fp2_mul_base(num, orc_fp_inv(den))));
     }
     out[0] = value.c0;
     out[1] = value.c1;
@@ -88,6 +88,8 @@ typedef struct {
     F2 **cur;      /* current (bound) values per table, NULL until the first own bind */
     F2 weight;     /* gamma^i */
     F2 wsum;       /* gamma^i * S_i for padding rounds */
+    F2 first[17];  /* node sums of the first own round, from the pass that gave S_i */
+    int have_first;
 } inst_t;
 
 static F2 entry(const inst_t *in, uint32_t k, uint64_t x) {
@@ -102,7 +104,49 @@ typedef struct {
     uint64_t pairs;
     F2 *partial;    /* per thread: d+1 sums */
     int threads;
+    int need_one;   /* also sum z = 1 (only S_i needs it; a message sends p(0), p(2..d)) */
 } sums_t;
+
+/* Every table still unbound and in the base field: the first round of Table::base inputs, whose
+ * line values and node products stay in Fp (kernel.rs single/zip over base slices). */
+static int all_base(const inst_t *in) {
+    if (in->terms) return 0;
+    for (uint32_t k = 0; k < in->factors; k++)
+        if (in->cur[k] || in->tables[k].ext) return 0;
+    return 1;
+}
+
+static void sums_range_base(sums_t *s, F2 *acc, uint64_t lo, uint64_t hi) {
+    const inst_t *in = s->in;
+    uint32_t d = s->nodes, F = in->factors;
+    uint64_t sum[17] = {0};
+    for (uint64_t x = lo; x < hi; x++) {
+        uint64_t prod[17];
+        for (uint32_t k = 0; k < F; k++) {
+            const uint64_t *t = in->tables[k].data;
+            uint64_t a = t[2 * x], b = t[2 * x + 1], diff = fp_sub(b, a);
+            if (k == 0) {
+                uint64_t line = fp_add(b, diff);
+                prod[0] = a;
+                prod[d] = b;
+                for (uint32_t z = 2; z <= d; z++) {
+                    prod[z - 1] = line;
+                    line = fp_add(line, diff);
+                }
+                continue;
+            }
+            prod[0] = fp_mul(prod[0], a);
+            if (s->need_one) prod[d] = fp_mul(prod[d], b);
+            uint64_t line = fp_add(b, diff);
+            for (uint32_t z = 2; z <= d; z++) {
+                prod[z - 1] = fp_mul(prod[z - 1], line);
+                line = fp_add(line, diff);
+            }
+        }
+        for (uint32_t z = 0; z <= d; z++) sum[z] = fp_add(sum[z], prod[z]);
+    }
+    for (uint32_t z = 0; z <= d; z++) acc[z] = f2(sum[z], 0);
+}
 
 /* kernel::product_sums (kernel.rs:292): sum over pairs of prod_k line_k(z). Slot layout:
  * [p(0), p(2), ..., p(d), p(1)]. */
@@ -112,6 +156,10 @@ static void sums_range(void *arg, int slot, uint64_t lo, uint64_t hi) {
     uint32_t d = s->nodes;
     F2 *acc = s->partial + (uint64_t)slot * (d + 1);
     for (uint32_t z = 0; z <= d; z++) acc[z] = f2(0, 0);
+    if (all_base(in)) {
+        sums_range_base(s, acc, lo, hi);
+        return;
+    }
     for (uint64_t x = lo; x < hi; x++) {
         F2 prod[16];
         if (in->terms) {
@@ -139,13 +187,22 @@ static void sums_range(void *arg, int slot, uint64_t lo, uint64_t hi) {
             }
             continue;
         }
-        for (uint32_t z = 0; z <= d; z++) prod[z] = f2(1, 0);
         for (uint32_t k = 0; k < in->factors; k++) {
             F2 a = entry(in, k, 2 * x), b = entry(in, k, 2 * x + 1), diff = sub2(b, a);
-            /* z = 0 -> a; z = 1 -> b; z >= 2 -> a + z (b - a) */
-            prod[0] = mul2(prod[0], a);
-            prod[d] = mul2(prod[d], b);
+            /* z = 0 -> a; z = 1 -> b; z >= 2 -> a + z (b - a); the first table seeds the
+             * products */
             F2 line = add2(b, diff); /* z = 2 */
+            if (k == 0) {
+                prod[0] = a;
+                prod[d] = b;
+                for (uint32_t z = 2; z <= d; z++) {
+                    prod[z - 1] = line;
+                    line = add2(line, diff);
+                }
+                continue;
+            }
+            prod[0] = mul2(prod[0], a);
+            if (s->need_one) prod[d] = mul2(prod[d], b);
             for (uint32_t z = 2; z <= d; z++) {
                 prod[z - 1] = mul2(prod[z - 1], line);
                 line = add2(line, diff);
@@ -157,12 +214,13 @@ static void sums_range(void *arg, int slot, uint64_t lo, uint64_t hi) {
 
 /* All d+1 node sums of an instance's current round: out[0] = p(0), out[1..d-1] = p(2..d),
  * out[d] = p(1). */
-static void node_sums(const inst_t *in, uint32_t d, uint64_t pairs, F2 *out, int threads) {
+static void node_sums(const inst_t *in, uint32_t d, uint64_t pairs, F2 *out, int threads,
+                      int need_one) {
     int th = threads;
     if (pairs < (1u << 12)) th = 1; /* prover.rs:502-536: at least 2^12 pairs per thread */
     else if ((uint64_t)th > pairs >> 12) th = (int)(pairs >> 12);
     if (th < 1) th = 1;
-    sums_t s = {in, d, pairs, calloc((size_t)th * (d + 1), sizeof(F2)), th};
+    sums_t s = {in, d, pairs, calloc((size_t)th * (d + 1), sizeof(F2)), th, need_one};
     orc_parallel_for_t(pairs, th, sums_range, &s);
     for (uint32_t z = 0; z <= d; z++) out[z] = f2(0, 0);
     for (int t = 0; t < th; t++)
@@ -178,6 +236,15 @@ typedef struct {
 } bind_t;
 static void bind_range(void *arg, uint64_t lo, uint64_t hi) {
     bind_t *b = arg;
+    const orc_table *t = &b->in->tables[b->k];
+    if (!b->in->cur[b->k] && !t->ext) {
+        /* a base table: a + r (c - a) = (a + r0 (c - a), r1 (c - a)) */
+        for (uint64_t x = lo; x < hi; x++) {
+            uint64_t a = t->data[2 * x], dlt = fp_sub(t->data[2 * x + 1], a);
+            b->dst[x] = f2(fp_add(a, fp_mul(b->r.c0, dlt)), fp_mul(b->r.c1, dlt));
+        }
+        return;
+    }
     for (uint64_t x = lo; x < hi; x++) {
         F2 a = entry(b->in, b->k, 2 * x), c = entry(b->in, b->k, 2 * x + 1);
         b->dst[x] = add2(a, mul2(b->r, sub2(c, a)));
@@ -247,12 +314,12 @@ int64_t orc_prove_batch(orc_transcript *t, const char *prefix, uint32_t degree,
             S = f2(1, 0);
             for (uint32_t k = 0; k < in[i].factors; k++) S = mul2(S, entry(&in[i], k, 0));
         } else {
-            F2 sums[16];
-            node_sums(&in[i], degree, 1ULL << (in[i].vars - 1), sums, threads);
-            S = add2(sums[0], sums[degree]);
+            node_sums(&in[i], degree, 1ULL << (in[i].vars - 1), in[i].first, threads, 1);
+            in[i].have_first = 1;
+            S = add2(in[i].first[0], in[i].first[degree]);
         }
         in[i].wsum = mul2(w, S);
-        claim = add2(claim, orc_fp2_mul_base(in[i].wsum, orc_fp_pow(2, in[i].pad)));
+        claim = add2(claim, fp2_mul_base(in[i].wsum, fp_pow(2, in[i].pad)));
         w = mul2(w, gamma);
     }
     for (uint32_t j = 0; j < n_max; j++) {
@@ -262,13 +329,18 @@ int64_t orc_prove_batch(orc_transcript *t, const char *prefix, uint32_t degree,
             inst_t *x = &in[i];
             if (j < x->pad) {
                 /* padding round: 2^{pad - j - 1} S_i at every node (prover.rs:1020-1031) */
-                F2 v = orc_fp2_mul_base(x->wsum, 1ULL << (x->pad - j - 1));
+                F2 v = fp2_mul_base(x->wsum, 1ULL << (x->pad - j - 1));
                 for (uint32_t z = 0; z < degree; z++) msg[z] = add2(msg[z], v);
                 continue;
             }
             uint64_t pairs = 1ULL << (x->vars - (j - x->pad) - 1);
-            F2 sums[16];
-            node_sums(x, degree, pairs, sums, threads);
+            F2 sums[17];
+            if (x->have_first) {
+                memcpy(sums, x->first, sizeof sums); /* no table has been bound since */
+                x->have_first = 0;
+            } else {
+                node_sums(x, degree, pairs, sums, threads, 0);
+            }
             msg[0] = add2(msg[0], mul2(x->weight, sums[0]));
             for (uint32_t z = 1; z < degree; z++) msg[z] = add2(msg[z], mul2(x->weight, sums[z]));
         }

@@ -299,6 +299,8 @@ void tsr_ctx_destroy(tsr_ctx *ctx) {
     ctx->staging_in.release();
     ctx->staging_out.release();
     if (ctx->pinned) (void)hipHostFree(ctx->pinned);
+    for (auto &kv : ctx->user_allocs) (void)hipFree(kv.first);  // blocks never given back
+    ctx->user_allocs.clear();
     pool_trim(ctx);
     (void)hipStreamDestroy(ctx->stream);
     delete ctx;
@@ -365,22 +367,29 @@ tsr_status tsr_alloc(tsr_ctx *ctx, uint64_t bytes, void **out) {
     CHECK_ARG(ctx && out, "null argument");
     TRY_HIP(hipSetDevice(ctx->device));
     TRY(dev_alloc(ctx, bytes ? bytes : 1, out));
+    std::lock_guard<std::mutex> lk(ctx->pool_mu);
+    ctx->user_allocs[*out] = size_class(bytes ? bytes : 1);
     return TSR_OK;
 }
 
+// The block goes back to the context's pool, like the library's own buffers, with the bytes
+// dev_alloc charged for it; a pointer tsr_alloc did not return is refused untouched.
 tsr_status tsr_free(tsr_ctx *ctx, void *ptr) {
     CHECK_ARG(ctx, "null context");
     if (!ptr) return TSR_OK;
-    TRY_HIP(hipSetDevice(ctx->device));
-    size_t size = 0;
-    void *base = nullptr;
-    if (hipMemGetAddressRange(&base, &size, ptr) != hipSuccess) {
-        (void)hipGetLastError();
-        size = 0;
+    uint64_t charged = 0;
+    {
+        std::lock_guard<std::mutex> lk(ctx->pool_mu);
+        auto it = ctx->user_allocs.find(ptr);
+        if (it == ctx->user_allocs.end())
+            return Status::fail(TSR_BAD_ARG, "tsr_free: not a pointer from tsr_alloc").code;
+        charged = it->second;
+        ctx->user_allocs.erase(it);
     }
+    TRY_HIP(hipSetDevice(ctx->device));
+    // work queued on the stream may still read or write the block
     TRY_HIP(hipStreamSynchronize(ctx->stream));
-    (void)hipFree(ptr);
-    ctx->in_use -= size < ctx->in_use ? size : ctx->in_use;
+    dev_free(ctx, ptr, charged);
     return TSR_OK;
 }
 

@@ -101,6 +101,7 @@ struct Ctx {
     uint64_t pooled = 0;  // freed blocks kept for reuse
     std::multimap<uint64_t, void *> pool;
     std::mutex pool_mu;  // dev_alloc / dev_free may come from several threads (served proving)
+    std::map<void *, uint64_t> user_allocs;  // tsr_alloc pointers and the bytes charged for them
     std::map<std::tuple<uint32_t, uint32_t, uint64_t, uint32_t>, std::shared_ptr<Encoder>> encoders;
     uint32_t keys[19][8];  // BLAKE3 context keys of the reference's 19 domains
     bool profile = false;

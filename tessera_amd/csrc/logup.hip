@@ -16,6 +16,7 @@
 #include <string.h>
 #include <string>
 #include <vector>
+#include "commit.h"
 #include "common.h"
 #include "field.h"
 #include "sumcheck.h"
@@ -823,9 +824,12 @@ tsr_status tsr_logup_prove(tsr_ctx *ctx_, tsr_transcript *t, const tsr_logup_ins
     fp2 alpha, beta;
     std::vector<fp2> roots, values, claims, fclaims;
     Log lg{log, log ? log_cap : 0};
-    st = prove_logup(ctx, t->t, inst, count, forests, nforests, sched, alpha, beta, roots, values,
+    // on a copy of the transcript, kept only when the whole proof succeeded
+    Transcript work = t->t;
+    st = prove_logup(ctx, work, inst, count, forests, nforests, sched, alpha, beta, roots, values,
                      claims, fclaims, lg);
     if (st) return st.code;
+    t->t = work;
     if (alpha_out) {
         alpha_out[0] = alpha.c0;
         alpha_out[1] = alpha.c1;
@@ -856,37 +860,62 @@ tsr_status tsr_prove_trees(tsr_ctx *ctx_, tsr_transcript *t, const tsr_forest *f
     if (e != hipSuccess) return hip_fail(e, "hipSetDevice").code;
     std::vector<fp2> fclaims;
     Log lg{log, log ? log_cap : 0};
-    if ((st = prove_trees(ctx, t->t, forests, nforests, fclaims, lg))) return st.code;
+    Transcript work = t->t;  // kept only when the whole proof succeeded
+    if ((st = prove_trees(ctx, work, forests, nforests, fclaims, lg))) return st.code;
+    t->t = work;
     put_f2s(forest_claims_out, fclaims);
     if (log_len) *log_len = lg.len;
     if (log && lg.len > log_cap) return Status::fail(TSR_BAD_SHAPE, "log buffer too small").code;
     return TSR_OK;
 }
 
+// The lookup rows and the counts are secret-class (instance.rs:217-219: SecretBuf, cleared on
+// drop): the device copies are cleared before their blocks go back to the context's pool.
+struct ClearOnExit {
+    uint64_t *p;
+    uint64_t n;
+    hipStream_t s;
+    ~ClearOnExit() {
+        if (p && n) {
+            zero_words(p, n, s);
+            (void)hipStreamSynchronize(s);
+        }
+    }
+};
+
 tsr_status tsr_logup_multiplicities(tsr_ctx *ctx_, uint64_t rows, const uint64_t *lookup_rows,
                                     uint64_t lookups, uint64_t *out) {
     Ctx *ctx = (Ctx *)ctx_;
-    if (!ctx || !out || (lookups && !lookup_rows))
+    if (!ctx || (lookups && !lookup_rows) || (rows && !out))
         return Status::fail(TSR_BAD_ARG, "null argument").code;
-    if (rows == 0) return Status::fail(TSR_BAD_SHAPE, "NoRows").code;
+    // A table of no row has no leaf, so nothing is written; any lookup then names no row
+    // (instance.rs:228-241).
+    if (rows == 0) {
+        if (lookups)
+            return Status::fail(TSR_BAD_SHAPE, "LookupRow: a lookup names no row of the table").code;
+        return TSR_OK;
+    }
     hipError_t e = hipSetDevice(ctx->device);
     if (e != hipSuccess) return hip_fail(e, "hipSetDevice").code;
     hipStream_t s = ctx->stream;
     const uint64_t leaves = 1ull << ceil_log2(rows);
     DevBuf<uint64_t> counts, in;
+    DevBuf<uint32_t> bad;
     Status st;
     if ((st = counts.alloc(ctx, leaves))) return st.code;
+    ClearOnExit clear_counts{counts.ptr, leaves, s};
     const uint64_t *d = lookup_rows;
+    if ((st = bad.alloc(ctx, 1))) return st.code;
     if (lookups && !is_device_ptr(lookup_rows)) {
         if ((st = in.alloc(ctx, lookups))) return st.code;
         e = hipMemcpyAsync(in.ptr, lookup_rows, 8 * lookups, hipMemcpyHostToDevice, s);
         if (e != hipSuccess) return hip_fail(e, "copy").code;
         d = in.ptr;
     }
-    DevBuf<uint32_t> bad;
-    if ((st = bad.alloc(ctx, 1))) return st.code;
-    hipMemsetAsync(counts.ptr, 0, 8 * leaves, s);
-    hipMemsetAsync(bad.ptr, 0, 4, s);
+    ClearOnExit clear_in{in.ptr, in.ptr ? lookups : 0, s};
+    zero_words(counts.ptr, leaves, s);
+    e = hipMemsetAsync(bad.ptr, 0, 4, s);
+    if (e != hipSuccess) return hip_fail(e, "hipMemsetAsync").code;
     const uint64_t slices = (rows + (1ull << HIST_SLICE_LOG) - 1) >> HIST_SLICE_LOG;
     const uint64_t chunks = (lookups + HIST_CHUNK - 1) / HIST_CHUNK;
     if (lookups && slices <= 8 && chunks <= 65535)
@@ -899,34 +928,19 @@ tsr_status tsr_logup_multiplicities(tsr_ctx *ctx_, uint64_t rows, const uint64_t
     uint32_t hbad = 0;
     e = hipMemcpyAsync(&hbad, bad.ptr, 4, hipMemcpyDeviceToHost, s);
     if (e != hipSuccess) return hip_fail(e, "copy").code;
-    if (is_device_ptr(out)) {
-        // device output: the counts stay on the device (only the row check comes back first,
-        // so that a bad lookup leaves `out` untouched)
-        e = hipStreamSynchronize(s);
-        if (e != hipSuccess) return hip_fail(e, "sync").code;
-        if (hbad) return Status::fail(TSR_BAD_SHAPE, "LookupRow: a lookup names no row of the table").code;
-        if (lookups)
-            hipLaunchKernelGGL(k_add_first, dim3(1), dim3(64), 0, s, counts.ptr,
-                               (1ull << ceil_log2(lookups)) - lookups);
-        e = hipMemcpyAsync(out, counts.ptr, 8 * leaves, hipMemcpyDeviceToDevice, s);
-        if (e != hipSuccess) return hip_fail(e, "copy").code;
-        e = hipStreamSynchronize(s);
-        if (e != hipSuccess) return hip_fail(e, "sync").code;
-        return check_launch("multiplicities").code;
-    }
-    std::vector<uint64_t> h(leaves);
-    e = hipMemcpyAsync(h.data(), counts.ptr, 8 * leaves, hipMemcpyDeviceToHost, s);
-    if (e != hipSuccess) return hip_fail(e, "copy").code;
+    // the row check comes back first, so that a bad lookup leaves `out` untouched
     e = hipStreamSynchronize(s);
     if (e != hipSuccess) return hip_fail(e, "sync").code;
     if (hbad) return Status::fail(TSR_BAD_SHAPE, "LookupRow: a lookup names no row of the table").code;
-    if (lookups) h[0] += (1ull << ceil_log2(lookups)) - lookups;
-    if (is_device_ptr(out)) {
-        e = hipMemcpy(out, h.data(), 8 * leaves, hipMemcpyHostToDevice);
-        if (e != hipSuccess) return hip_fail(e, "copy").code;
-    } else {
-        memcpy(out, h.data(), 8 * leaves);
-    }
+    if (lookups)
+        hipLaunchKernelGGL(k_add_first, dim3(1), dim3(64), 0, s, counts.ptr,
+                           (1ull << ceil_log2(lookups)) - lookups);
+    if ((st = check_launch("multiplicities"))) return st.code;
+    // device or host output alike: one copy of the counts
+    e = hipMemcpyAsync(out, counts.ptr, 8 * leaves, hipMemcpyDefault, s);
+    if (e != hipSuccess) return hip_fail(e, "copy").code;
+    e = hipStreamSynchronize(s);
+    if (e != hipSuccess) return hip_fail(e, "sync").code;
     return TSR_OK;
 }
 

@@ -1749,8 +1749,12 @@ extern "C" tsr_status tsr_sumcheck_prove(tsr_ctx *ctx_, tsr_transcript *t, const
     if (e != hipSuccess) return hip_fail(e, "hipSetDevice").code;
     SumcheckOut out;
     std::vector<DevBuf<uint64_t>> uploads;
-    Status st = prove_batch(ctx, t->t, prefix, degree, instances, count, out, uploads);
+    // the proof runs on a copy of the transcript, which replaces the caller's only when every
+    // step succeeded: a refusal (no memory, a decline) leaves the transcript as it was
+    Transcript work = t->t;
+    Status st = prove_batch(ctx, work, prefix, degree, instances, count, out, uploads);
     if (st) return st.code;
+    t->t = work;
     if (gamma_out) {
         gamma_out[0] = out.gamma.c0;
         gamma_out[1] = out.gamma.c1;

@@ -59,13 +59,14 @@ def multiplicities(device, rows: int, lookup_rows, out=None):
     DeviceBuffer of at least that many u64 to count into on the device (returned), else a new
     host array."""
     lr = lookup_rows if isinstance(lookup_rows, DeviceBuffer) else np.ascontiguousarray(lookup_rows, np.uint64)
-    h = (rows - 1).bit_length() if rows > 1 else 0
+    # a table of no row has no leaf (instance.rs:228-231)
+    leaves = 0 if rows == 0 else 1 << ((rows - 1).bit_length() if rows > 1 else 0)
     if out is None:
-        out = np.zeros(1 << h, dtype=np.uint64)
-        dst = out.ctypes.data
+        out = np.zeros(leaves, dtype=np.uint64)
+        dst = out.ctypes.data if leaves else None
     else:
-        if not isinstance(out, DeviceBuffer) or out.count < (1 << h):
-            raise ValueError(f"out must be a DeviceBuffer of at least {1 << h} entries")
+        if not isinstance(out, DeviceBuffer) or out.count < leaves:
+            raise ValueError(f"out must be a DeviceBuffer of at least {leaves} entries")
         dst = out.ptr
     check(lib().tsr_logup_multiplicities(device.handle, rows, ptr_of(lr), _count(lr), dst))
     return out

@@ -63,3 +63,53 @@ def test_graph_replays_match_eager(dev):
         c = T.ReferenceCommitment.commit_unmasked(dev, T.CommitmentParamsPy(lk, lr, 0, blk, cid), rows)
         assert c.root() == root
         c.free()
+
+
+@pytest.fixture(scope="module")
+def config4(dev):
+    """The batched-commit workload at its real shapes: 64 matrices, log2 k uniform in [10, 20],
+    N uniform in [16, 512] (about 2.0 G elements, 64 GiB of codewords), filled on the device,
+    committed by one autotuned, graph-captured batch. Yields (shapes, row buffers, batch roots)."""
+    from tessera_amd.batch import CommitBatch
+    shapes = batched_shapes()
+    bufs = [dev.buffer(n << lk).fill_uniform(0xC4 + i) for i, (lk, n) in enumerate(shapes)]
+    cid = bytes([7] * 32)
+    mats = [(buf, lk, 2, 32, cid) for buf, (lk, n) in zip(bufs, shapes)]
+    print(f"config 4: {len(shapes)} matrices, {sum(n << lk for lk, n in shapes)} elements", flush=True)
+    b = CommitBatch(dev, mats, autotune=True, graph=True)
+    b.run()
+    roots = b.roots()
+    b.run()  # a replay gives the same roots
+    assert b.roots() == roots
+    b.free()
+    yield shapes, bufs, cid, roots
+    for buf in bufs:
+        buf.free()
+
+
+def test_config4_batch_equals_plain_commit(dev, config4):
+    """Every batch root equals the plain per-matrix commit_unmasked root, including the
+    matrices of log2 k 18..20 with N above 256 that run the 9- and 10-bit strided passes."""
+    import tessera_amd as T
+    shapes, bufs, cid, roots = config4
+    assert sorted(set(lk for lk, _ in shapes)) == list(range(10, 21))
+    assert any(lk >= 18 and n > 256 for lk, n in shapes)
+    for (lk, n), buf, root in zip(shapes, bufs, roots):
+        c = T.ReferenceCommitment.commit_unmasked(dev, T.CommitmentParamsPy(lk, 2, 0, 32, cid), buf)
+        assert c.root() == root, (lk, n)
+        c.free()
+
+
+@pytest.mark.parametrize("lo,hi", [(0, 22), (22, 24), (24, 26)])
+def test_config4_batch_equals_oracle(dev, config4, lo, hi):
+    """Batch roots against the oracle's commit_unmasked for every matrix of 2^lo < N k <= 2^hi
+    elements (57 of the 64, log2 k 10..18 and 20)."""
+    shapes, bufs, cid, roots = config4
+    checked = 0
+    for (lk, n), buf, root in zip(shapes, bufs, roots):
+        if not (1 << lo) < (n << lk) <= (1 << hi):
+            continue
+        _, nodes = O.commit_unmasked(buf.download(), lk, 2, 32, cid, threads=16)
+        assert root == nodes[-32:], (lk, n)
+        checked += 1
+    assert checked > 0

@@ -203,6 +203,10 @@ def test_multiplicities(dev):
     assert np.array_equal(out.download(), before)
     with pytest.raises(ValueError):
         multiplicities(dev, 1000, d_rows, out=dev.buffer(1000))
+    # a table of no row: no leaf, so no lookup is an empty result and any lookup names no row
+    assert multiplicities(dev, 0, np.zeros(0, dtype=np.uint64)).size == 0
+    with pytest.raises(T.BadShape, match="LookupRow"):
+        multiplicities(dev, 0, np.array([0], dtype=np.uint64))
 
 
 @pytest.mark.parametrize("rows,lookups", [(1, 70000), (1 << 14, 1), ((1 << 14) + 1, 300000),

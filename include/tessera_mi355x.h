@@ -121,6 +121,14 @@ void tsr_matrix_sink_free(tsr_matrix_sink *sink);
 /* ---- Merkle (MerkleTree::new / root, crates/tessera-hash/src/merkle.rs:93-119) ---- */
 /* root of the binary BLAKE3 tree over n = 2^m leaves (32 bytes each). */
 tsr_status tsr_merkle_root(tsr_ctx *ctx, const uint8_t *leaves, uint64_t n, uint8_t root[32]);
+/* tsr_merkle_root with no allocation of any kind. No counterpart in the reference: it is for
+ * callers that cannot allocate device memory (code under HIP graph capture, tight serving
+ * loops, harnesses that forbid hipMalloc). The tree is built in a fixed array of the library's
+ * own (4 MiB per GPU, held until the process ends), so n is at most 2^16 leaves (DECLINED above
+ * that); leaves and root may be host or device memory. Every call on one GPU shares that array:
+ * calls that overlap on one GPU race. */
+tsr_status tsr_merkle_root_noalloc(tsr_ctx *ctx, const uint8_t *leaves, uint64_t n,
+                                   uint8_t root[32]);
 
 /* ---- the unmasked reference commitment (ReferenceCommitment::commit_unmasked / root /
  *      leaves / answer, crates/tessera-pcs/src/commitment.rs:343-598) ---- */

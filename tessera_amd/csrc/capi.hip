@@ -280,6 +280,9 @@ tsr_status tsr_ctx_create(int device, tsr_ctx **out) {
     c->device = device;
     TRY_HIP(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
     derive_domain_keys(c->keys);
+    // the no-allocation entry points' scratch address on this device (the lookup's result is
+    // kept and reported by those entry points)
+    noalloc_setup(device);
     *out = c.release();
     g_error.clear();
     return TSR_OK;

@@ -7,7 +7,12 @@
 
 namespace tsr {
 
-Status merkle_build(Ctx *ctx, uint32_t *nodes, uint64_t n, hipStream_t stream);
+// Takes the device address of the no-allocation scratch array on `device` (current), once per
+// device; tsr_ctx_create calls it.
+void noalloc_setup(int device);
+// Every level above the n leaves at nodes[0 .. 8n) (leaves first, root last). `timed`: record
+// the launches when the context profiles (false: no event is taken, nothing is allocated).
+Status merkle_build(Ctx *ctx, uint32_t *nodes, uint64_t n, hipStream_t stream, bool timed = true);
 Status chain_leaves(Ctx *ctx, const uint64_t *cw, uint64_t n, uint64_t rows, uint32_t block,
                     const uint8_t id[32], uint32_t *leaves, hipStream_t stream);
 // leaf_j = H(column-merge, d_j^(1) || ... || d_j^(S)) from S segments' chains (segment-major).

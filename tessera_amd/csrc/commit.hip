@@ -270,10 +270,10 @@ k_merkle_top(uint32_t *__restrict__ nodes, uint64_t start, uint64_t width, Key8 
     }
 }
 
-Status merkle_build(Ctx *ctx, uint32_t *nodes, uint64_t n, hipStream_t stream) {
+Status merkle_build(Ctx *ctx, uint32_t *nodes, uint64_t n, hipStream_t stream, bool timed) {
     Key8 key = key_of(ctx, DOM_MERKLE_NODE);
     uint64_t start = 0, width = n;
-    KernelTimer timer(ctx, "k_merkle_level", stream);
+    KernelTimer timer(timed ? ctx : nullptr, "k_merkle_level", stream);
     // one launch per level while a level has more than 2 MERKLE_TOP nodes, then the top at once
     while (width > 2 * MERKLE_TOP) {
         uint64_t count = width / 2;

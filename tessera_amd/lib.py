@@ -60,6 +60,7 @@ _SIGS = {
     "tsr_chains_read": (i32, [vp, vp]),
     "tsr_chains_free": (None, [vp]),
     "tsr_merkle_root": (i32, [vp, vp, u64, vp]),
+    "tsr_merkle_root_noalloc": (i32, [vp, vp, u64, vp]),
     "tsr_commit_unmasked": (i32, [vp, ctypes.POINTER(CommitmentParams), vp, u64,
                                   ctypes.POINTER(vp)]),
     "tsr_commitment_root": (i32, [vp, vp]),

@@ -204,30 +204,29 @@ def run_plain(reps=5):
 
 
 def run_sumcheck(reps=3):
-    """(i) stage/0 Sum eq(r,x) a(x) b(x), n = 24, degree 3, eq public (built on the device from
+    """(i) stage/0 Sum eq(r,x) a(x) b(x), n = 24, degree 3, eq public (given by its point
     r = 24 challenge_fp2("bench/r/{i}")), a and b uniform and resident; (ii) prove_logup of 2^22
     lookups Zipf(1.1) into a 2^16-row table with values, multiplicities counted on the device."""
     import tessera_amd as T
     from tessera_amd.logup import LogUpInstance, multiplicities, prove_logup
-    from tessera_amd.sumcheck import Table, eq_table, prove_batch
+    from tessera_amd.sumcheck import Table, prove_batch
     from tessera_amd.transcript import Transcript, session_context
     from tests.helpers import logup_config3
     dev = T.Device(0)
     n = 24
     a = dev.buffer(1 << n).fill_uniform(SEED + 4)
     b = dev.buffer(1 << n).fill_uniform(SEED + 5)
-    eq = dev.buffer(2 << n)
     ctx_tce = session_context()
 
     def prod3():
         t = Transcript(ctx_tce)
         r = [t.challenge_fp2(f"bench/r/{i}") for i in range(n)]
-        eq_table(dev, r, out=eq)
-        prove_batch(dev, t, "stage/0", 3, [[Table(eq, ext=True).public(), Table(a), Table(b)]])
+        # eq(r, .) public, given by its point (the rounds never build its 2^24 entries)
+        prove_batch(dev, t, "stage/0", 3, [[Table.eq(r).public(), Table(a), Table(b)]])
 
     prod3()
     s_prod = timed(prod3, reps)
-    a.free(); b.free(); eq.free()
+    a.free(); b.free()
 
     keys, vals, idx = logup_config3()
     d_keys, d_vals = dev.upload(keys), dev.upload(vals)
@@ -722,7 +721,7 @@ def main():
     ap.add_argument("--traffic-command", default="", help="the profiling command, for the summary")
     ap.add_argument("--trace-summary", nargs=3, metavar=("DB", "TITLE", "COMMAND"),
                     help="print a markdown summary of a rocprofv3 kernel-trace database")
-    ap.add_argument("--only", choices=["headline", "copy", "hostfed", "refshapes"], default=None,
+    ap.add_argument("--only", choices=["headline", "copy", "hostfed", "refshapes", "sumcheck"], default=None,
                     help="time the headline commit only; run the device copy that calibrates "
                          "the HBM byte counters (4 copies of 2 GiB); time the headline commit "
                          "from host rows (the PCIe-inclusive rate); or time the reference's three "
@@ -746,6 +745,9 @@ def main():
         return
     if a.only == "hostfed":
         run_hostfed()
+        return
+    if a.only == "sumcheck":
+        print(json.dumps({"workload": "sumcheck/lookup configuration", **run_sumcheck()}), flush=True)
         return
     if a.only == "refshapes":
         print(json.dumps({"workload": "the reference's H100 unit-cost workloads", **run_refshapes()}), flush=True)

@@ -310,10 +310,15 @@ tsr_status tsr_transcript_check_grind(tsr_transcript *t, const char *label, uint
 /* ---- the batched product sumcheck (prove_batch, crates/tessera-sumcheck/src/prover.rs:1122-1145;
  *      Table / Instance::new, prover.rs:78-305; SumcheckProof, proof.rs:16-24) ---- */
 typedef struct {
-    const uint64_t *data; /* 2^vars entries: u64 each (Fp, Table::base) or 2 u64 (Fp2, Table::ext) */
-    uint32_t ext;         /* 1: Fp2 entries */
+    const uint64_t *data; /* 2^vars entries: u64 each (Fp, Table::base) or 2 u64 (Fp2, Table::ext);
+                             for ext = TSR_TABLE_EQ the point of eq(point, .): vars Fp2
+                             coordinates, coordinate i for bit i of the index */
+    uint32_t ext;         /* 1: Fp2 entries; TSR_TABLE_EQ: the table eq_table(point)
+                             (crates/tessera-poly/src/eq.rs:23) given by its point instead of its
+                             2^vars entries, the same proof either way */
     uint32_t is_public;   /* 1: a public factor, left out of the final evaluations (Table::public) */
 } tsr_table;
+#define TSR_TABLE_EQ 2u
 /* One term of a sum-of-products instance (Term, crates/tessera-sumcheck/src/terms.rs:10-40):
  * weight * the product of the tables at `factors`, positions in the instance's table list,
  * which may repeat (a squared table). */
@@ -341,7 +346,11 @@ typedef struct {
  * (prover.rs:739-771): degree outside 1..=10, no instance, then per instance NoTables,
  * TooManyTables (a product of more tables than the degree) or NoTerms (terms given, none),
  * TermFactors (a term of 0 or more than `degree` factors), TermFactor (a factor naming no
- * table) (BAD_SHAPE). DECLINED above the device's limits: 10 tables, 16 terms per instance. */
+ * table) (BAD_SHAPE). DECLINED above the device's limits: 10 tables, 16 terms per instance.
+ * A public TSR_TABLE_EQ table in a product of three tables at degree 3 (the Sum eq(r, x) a b of a
+ * claim reduction, prover.rs:186-192) is never built: the rounds weight each pair by the eq of
+ * the unbound variables and fold the bound ones into a running factor. In every other use the
+ * table is built on the device from its point. */
 tsr_status tsr_sumcheck_prove(tsr_ctx *ctx, tsr_transcript *t, const char *prefix,
                               uint32_t degree, const tsr_instance *instances, uint32_t count,
                               uint64_t *gamma_out, uint64_t *rounds_out, uint64_t *challenges_out,

@@ -573,6 +573,15 @@ __device__ __forceinline__ fp2 eq_at(const fp2 &rt, const fp2 &c) {
     return gl::sub(gl::sub(gl::add(fp2{1, 0}, gl::add(m, m)), rt), c);
 }
 
+// gamma^i S_i of a padded instance without a claim, from the sums of its first own round
+// (slot 0: node 0, slot D: node 1): p(0) + p(1), with an eq factor's line (1 - r_0, r_0) at the
+// two nodes (the host puts coordinate 0 of its point in rt for round 0).
+__device__ __forceinline__ fp2 first_wsum(const ScRoundInst &in, const fp2 *s, uint32_t D) {
+    if (!in.eqf) return gl::mul(in.weight, gl::add(s[0], s[D]));
+    return gl::add(gl::mul(eq_line_at(in.weight, in.rt, 0), s[0]),
+                   gl::mul(eq_line_at(in.weight, in.rt, 1), s[D]));
+}
+
 struct ScCtl {
     DevTranscript t;
     fp2 r;  // the current round's challenge
@@ -658,8 +667,7 @@ k_sc_transcript(uint32_t j, const ScRoundInst *__restrict__ ri, ScInstState *__r
     // S_i of the padded instances that have no claim: p(0) + p(1) of their first sums
     auto first_sum = [&](uint32_t i, const ScRoundInst &in, ScInstState &is) {
         if (j != 0 || in.pad == 0 || in.wsum_given) return;
-        const fp2 *s = sums + (uint64_t)i * (D + 1);
-        is.wsum = gl::mul(in.weight, gl::add(s[0], s[D]));
+        is.wsum = first_wsum(in, sums + (uint64_t)i * (D + 1), D);
         ist[i].wsum = is.wsum;
     };
     // each instance's share of the message, in place of its sums at the nodes 0, 2, ..., D (the
@@ -994,8 +1002,7 @@ k_sc_tail(uint32_t j0, uint32_t n_max, const ScRoundInst *__restrict__ rtab,
                 const uint32_t i = threadIdx.x;
                 const ScRoundInst &in = sri[i];
                 if (in.pad > 0 && !in.wsum_given) {
-                    const fp2 *s = sums + (uint64_t)i * (D + 1);
-                    sis[i].wsum = gl::mul(in.weight, gl::add(s[0], s[D]));
+                    sis[i].wsum = first_wsum(in, sums + (uint64_t)i * (D + 1), D);
                 }
             }
             __syncthreads();
@@ -1272,8 +1279,11 @@ Status run_batch(Ctx *ctx, Transcript &tr, const std::string &p, uint32_t D,
         if (!insts[i].eq_factor) continue;
         const GenInst &g = insts[i];
         const size_t nt = g.rel == REL_PRODUCT ? 2 : 4;
-        bool shape = D == 3 && (g.rel == REL_PRODUCT || g.rel == REL_FRACTION) && g.claim_given &&
-                     g.eq_point.size() == g.vars && g.tabs.size() == nt;
+        // a fraction level always brings its claim; a product (prove_batch's Sum eq a b) may
+        // leave S_i to its first sums
+        bool shape = D == 3 && (g.rel == REL_PRODUCT || g.rel == REL_FRACTION) &&
+                     (g.claim_given || g.rel == REL_PRODUCT) && g.eq_point.size() == g.vars &&
+                     g.tabs.size() == nt;
         // tables present, except both numerators of a fraction with unit numerators
         for (size_t k = 0; shape && k < nt; k++)
             shape = g.tabs[k].src || (g.rel == REL_FRACTION && k < 2 && !g.tabs[0].src && !g.tabs[1].src);
@@ -1386,6 +1396,9 @@ Status run_batch(Ctx *ctx, Transcript &tr, const std::string &p, uint32_t D,
             e.wsum_given = wgiven[i];
             e.weight = weight[i];
             e.rt = (e.eqf && j >= pad[i] && j < n_max) ? insts[i].eq_point[j - pad[i]] : fp2{0, 0};
+            // a padded eq instance without a claim: S_i in round 0 takes the line of its first
+            // own round (coordinate 0 of the point)
+            if (e.eqf && j == 0 && pad[i] > 0 && !wgiven[i]) e.rt = insts[i].eq_point[0];
             if (v == 0 || j >= n_max || (j < pad[i] && j != 0)) continue;
             const uint32_t t = j < pad[i] ? 0 : j - pad[i];
             e.nblk = blocks_for_pairs(ctx, 1ull << (v - t - 1));
@@ -1645,32 +1658,72 @@ Status run_batch(Ctx *ctx, Transcript &tr, const std::string &p, uint32_t D,
     return Status::ok();
 }
 
+// The point of a TSR_TABLE_EQ table (host or device memory) on the host.
+static Status eq_point_of(Ctx *ctx, const tsr_table &tt, uint32_t vars, std::vector<fp2> &pt) {
+    pt.assign(vars, fp2{0, 0});
+    if (vars == 0) return Status::ok();
+    TSR_HIP(hipMemcpyAsync(pt.data(), tt.data, 16ull * vars, hipMemcpyDefault, ctx->stream));
+    TSR_HIP(hipStreamSynchronize(ctx->stream));
+    for (auto &c : pt)
+        if (c.c0 >= gl::P || c.c1 >= gl::P)
+            return Status::fail(TSR_BAD_ARG, "an eq point coordinate is not canonical");
+    return Status::ok();
+}
+
 Status prove_batch(Ctx *ctx, Transcript &tr, const char *prefix, uint32_t degree,
                    const tsr_instance *instances, uint32_t count, SumcheckOut &out,
                    std::vector<DevBuf<uint64_t>> &uploads) {
     hipStream_t s = ctx->stream;
     std::string p(prefix);
-    // gamma on the host transcript (prover.rs:1147-1150)
-    fp2 gamma = tr.challenge_fp2((p + "/gamma").c_str(), nullptr);
     std::vector<GenInst> insts(count);
     uploads.reserve(64);
     for (uint32_t i = 0; i < count; i++) {
-        insts[i].vars = instances[i].vars;
-        insts[i].rel = instances[i].terms ? REL_TERMS : REL_PRODUCT;
-        for (uint32_t t = 0; instances[i].terms && t < instances[i].nterms; t++) {
-            const tsr_term &tm = instances[i].terms[t];
+        const tsr_instance &ti = instances[i];
+        insts[i].vars = ti.vars;
+        insts[i].rel = ti.terms ? REL_TERMS : REL_PRODUCT;
+        for (uint32_t t = 0; ti.terms && t < ti.nterms; t++) {
+            const tsr_term &tm = ti.terms[t];
             GenTerm g;
             g.weight = fp2{tm.weight[0], tm.weight[1]};
-            g.mult.assign(instances[i].factors, 0);
+            g.mult.assign(ti.factors, 0);
             for (uint32_t f = 0; f < tm.nfactors; f++) g.mult[tm.factors[f]]++;
             insts[i].terms.push_back(g);
         }
-        const uint64_t len = 1ull << instances[i].vars;
-        for (uint32_t k = 0; k < instances[i].factors; k++) {
-            const tsr_table &tt = instances[i].tables[k];
+        // Sum eq(r, x) a b with eq public: eq stays a point (k_sc_round_eq's product relation)
+        uint32_t n_eq = 0, eq_at_k = 0;
+        for (uint32_t k = 0; k < ti.factors; k++)
+            if (ti.tables[k].ext == TSR_TABLE_EQ) n_eq++, eq_at_k = k;
+        const bool eq_path = !ti.terms && ti.factors == 3 && n_eq == 1 && degree == 3 &&
+                             ti.vars >= 1 && ti.tables[eq_at_k].is_public;
+        if (eq_path) {
+            insts[i].eq_factor = true;
+            Status st = eq_point_of(ctx, ti.tables[eq_at_k], ti.vars, insts[i].eq_point);
+            if (st) return st;
+        }
+        const uint64_t len = 1ull << ti.vars;
+        for (uint32_t k = 0; k < ti.factors; k++) {
+            const tsr_table &tt = ti.tables[k];
+            if (eq_path && k == eq_at_k) continue;
             GenTable tab;
-            tab.base = !tt.ext;
             tab.is_public = tt.is_public != 0;
+            if (tt.ext == TSR_TABLE_EQ) {
+                // any other use of an eq table: built on the device from its point
+                std::vector<fp2> pt;
+                Status st = eq_point_of(ctx, tt, ti.vars, pt);
+                if (st) return st;
+                DevBuf<uint64_t> dpt;
+                if ((st = dpt.alloc(ctx, 2 * std::max<uint64_t>(ti.vars, 1)))) return st;
+                if (ti.vars)
+                    TSR_HIP(hipMemcpyAsync(dpt.ptr, pt.data(), 16ull * ti.vars, hipMemcpyHostToDevice, s));
+                uploads.emplace_back();
+                if ((st = uploads.back().alloc(ctx, 2 * len))) return st;
+                if ((st = eq_table(ctx, (const fp2 *)dpt.ptr, ti.vars, (fp2 *)uploads.back().ptr, s))) return st;
+                tab.base = false;
+                tab.src = uploads.back().ptr;
+                insts[i].tabs.push_back(tab);
+                continue;
+            }
+            tab.base = !tt.ext;
             const uint64_t words = len * (tt.ext ? 2 : 1);
             if (is_device_ptr(tt.data)) {
                 tab.src = tt.data;
@@ -1685,6 +1738,8 @@ Status prove_batch(Ctx *ctx, Transcript &tr, const char *prefix, uint32_t degree
             insts[i].tabs.push_back(tab);
         }
     }
+    // gamma on the host transcript (prover.rs:1147-1150), once the tables are in place
+    fp2 gamma = tr.challenge_fp2((p + "/gamma").c_str(), nullptr);
     return run_batch(ctx, tr, p, degree, insts, gamma, fp2{0, 0}, out);
 }
 
@@ -1738,6 +1793,8 @@ extern "C" tsr_status tsr_sumcheck_prove(tsr_ctx *ctx_, tsr_transcript *t, const
             return Status::fail(TSR_DECLINED, "more than 10 tables or 16 terms in an instance").code;
         for (uint32_t k = 0; k < in.factors; k++) {
             if (!in.tables[k].data) return Status::fail(TSR_BAD_ARG, "null table").code;
+            if (in.tables[k].ext > TSR_TABLE_EQ)
+                return Status::fail(TSR_BAD_ARG, "a table is base (0), ext (1) or an eq point (2)").code;
             if (!in.tables[k].is_public) n_final++;
         }
     }

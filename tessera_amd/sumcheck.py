@@ -34,13 +34,19 @@ class SumOfProducts:
     terms: list
 
 
+TABLE_EQ = 2  # tsr_table.ext of an eq table given by its point
+
+
 @dataclass
 class Table:
-    """Table::base / Table::ext, optionally .public() (prover.rs:78-230)."""
+    """Table::base / Table::ext, optionally .public() (prover.rs:78-230). Table.eq(point) is the
+    table eq_table(point) (tessera-poly eq.rs:23) given by its point: the device never builds it
+    in a product of three tables at degree 3, and builds it from the point elsewhere."""
     values: object
     ext: bool = False
     is_public: bool = False
     vars: int = None
+    eq_point: bool = False
 
     @staticmethod
     def base(values):
@@ -50,8 +56,13 @@ class Table:
     def ext_(values):
         return Table(values, ext=True)
 
+    @staticmethod
+    def eq(point):
+        pt = np.ascontiguousarray(np.asarray(point, dtype=np.uint64).reshape(-1, 2))
+        return Table(pt, ext=True, vars=pt.shape[0], eq_point=True)
+
     def public(self):
-        return Table(self.values, self.ext, True, self.vars)
+        return Table(self.values, self.ext, True, self.vars, self.eq_point)
 
     def n_vars(self):
         if self.vars is not None:
@@ -95,7 +106,7 @@ def prove_batch(device, transcript, prefix: str, degree: int, instances) -> Prov
                 keep.append(v)
                 ptr = v.ctypes.data
             arr[k].data = ptr
-            arr[k].ext = int(t.ext)
+            arr[k].ext = TABLE_EQ if t.eq_point else int(t.ext)
             arr[k].is_public = int(t.is_public)
             n_final += 0 if t.is_public else 1
         keep.append(arr)

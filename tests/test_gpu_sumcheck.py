@@ -146,6 +146,96 @@ def test_config3_eq_a_b_invariants(dev):
     assert claim == O.fp2_mul(O.fp2_mul(e, ea), eb)
 
 
+def oracle_eq_proof(ctx, prefix, degree, specs, threads=16):
+    """The oracle's proof over materialized eq tables: specs are lists of ("eq", point, public) or
+    (values, ext, public)."""
+    t = O.OracleTranscript(ctx)
+    insts = []
+    for spec in specs:
+        tabs = []
+        for x in spec:
+            if isinstance(x[0], str):
+                tabs.append((O.eq_table(np.asarray(x[1], dtype=np.uint64).reshape(-1)), True, x[2]))
+            else:
+                tabs.append(x)
+        insts.append(tabs)
+    return O.prove_batch(t, prefix, degree, insts, threads=threads), t
+
+
+def device_eq_proof(dev, ctx, prefix, degree, specs):
+    from tessera_amd.sumcheck import Table, prove_batch
+    from tessera_amd.transcript import Transcript
+    t = Transcript(ctx)
+    insts = []
+    for spec in specs:
+        tabs = []
+        for x in spec:
+            if isinstance(x[0], str):
+                tab = Table.eq(x[1])
+            else:
+                tab = Table(x[0] if not hasattr(x[0], "ptr") else x[0], ext=bool(x[1]))
+            tabs.append(tab.public() if x[2] else tab)
+        insts.append(tabs)
+    return prove_batch(dev, t, prefix, degree, insts), t
+
+
+def assert_same_proof(want, wt, got, gt):
+    assert got.gamma == want["gamma"]
+    assert np.array_equal(got.rounds.reshape(-1), want["rounds"].reshape(-1))
+    assert np.array_equal(got.challenges.reshape(-1), want["challenges"].reshape(-1))
+    assert np.array_equal(got.final_evals.reshape(-1), want["finals"].reshape(-1))
+    assert gt.state == wt.state and gt.counter == wt.counter
+
+
+@pytest.mark.parametrize("n", [1, 2, 9, 12, 16, 20, 24])
+def test_eq_point_sum_eq_a_b_matches_oracle(dev, n):
+    """Sum eq(r, x) a(x) b(x), degree 3, eq public and given by its point: no eq table on the
+    device, the same proof bytes as the oracle's prove_batch over the materialized table. n = 24
+    is the sumcheck/lookup configuration's sumcheck at its full size, with r squeezed from the
+    transcript as the configuration draws it."""
+    from tessera_amd.transcript import Transcript
+    ctx = b"config-3"
+    t0 = Transcript(ctx)
+    r = np.array([t0.challenge_fp2(f"bench/r/{i}") for i in range(n)], dtype=np.uint64)
+    a, b = splitmix_uniform(1 << n, 40 + n), splitmix_uniform(1 << n, 80 + n)
+    print(f"eq point sumcheck n = {n}", flush=True)
+    want, wt = oracle_eq_proof(ctx, "stage/0", 3, [[("eq", r, True), (a, False, False), (b, False, False)]])
+    src = [dev.upload(a), dev.upload(b)] if n >= 16 else [a, b]
+    got, gt = device_eq_proof(dev, ctx, "stage/0", 3, [[("eq", r, True), (src[0], False, False),
+                                                        (src[1], False, False)]])
+    assert_same_proof(want, wt, got, gt)
+
+
+def test_eq_point_positions_padding_and_fallbacks(dev):
+    """The eq point anywhere in the table list, beside Fp2 tables, in a batch where it is padded
+    (its S from its first sums with the eq line), and the uses the round kernels do not take as
+    a point (a non-public eq, a product of two, degree 4: the table is built on the device) -- all equal to the oracle over materialized tables."""
+    rng = np.random.default_rng(5)
+
+    def pt(n):
+        return rng.integers(0, P, (n, 2), dtype=np.uint64)
+
+    def fp(n):
+        return rng.integers(0, P, 1 << n, dtype=np.uint64)
+
+    def f2(n):
+        return rng.integers(0, P, (1 << n, 2), dtype=np.uint64)
+
+    cases = [
+        (3, [[(fp(10), False, False), ("eq", pt(10), True), (f2(10), True, False)]]),
+        (3, [[(f2(9), True, False), (fp(9), False, False), ("eq", pt(9), True)],
+             [("eq", pt(13), True), (fp(13), False, False), (fp(13), False, False)],
+             [("eq", pt(6), True), (fp(6), False, False), (fp(6), False, False)]]),
+        (3, [[("eq", pt(8), False), (fp(8), False, False), (fp(8), False, False)]]),
+        (3, [[("eq", pt(8), True), (fp(8), False, False)]]),
+        (4, [[("eq", pt(7), True), (fp(7), False, False), (fp(7), False, False)]]),
+    ]
+    for degree, specs in cases:
+        want, wt = oracle_eq_proof(b"eq-cases", "stage/3", degree, specs)
+        got, gt = device_eq_proof(dev, b"eq-cases", "stage/3", degree, specs)
+        assert_same_proof(want, wt, got, gt)
+
+
 @pytest.mark.parametrize("n", [0, 1, 5, 12, 13, 20])
 def test_eq_table_matches_oracle(dev, n):
     """Direct products up to 12 variables, the outer product of two half tables above."""

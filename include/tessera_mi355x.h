@@ -62,6 +62,24 @@ tsr_status tsr_copy(tsr_ctx *ctx, void *dst, const void *src, uint64_t bytes);
  * crates/tessera-gpu/tests/commit.rs:19-30 with rejection). Device-side bench input generator. */
 tsr_status tsr_fill_uniform(tsr_ctx *ctx, uint64_t *dst, uint64_t count, uint64_t seed);
 
+/* ---- NTT plans (NttPlan::{new, size, ntt, intt, coset_ntt, coset_intt},
+ *      crates/tessera-pcs/src/ntt.rs:397-516) ---- */
+typedef struct tsr_ntt_plan tsr_ntt_plan;
+/* NttPlan::new(size): the forward and inverse twiddles of size n on the device. NotPowerOfTwo
+ * (size 0 or not a power of two), TooLong (above 2^32) (BAD_SHAPE). */
+tsr_status tsr_ntt_plan_create(tsr_ctx *ctx, uint64_t size, tsr_ntt_plan **out);
+uint64_t tsr_ntt_plan_size(const tsr_ntt_plan *plan);
+/* In place, natural order on both sides, over len / n rows of n canonical values (host or device
+ * memory; the reference transforms one row, len = n). ntt: values[j] = f(w_n^j) for
+ * f = sum_i values[i] X^i; intt its inverse; coset_ntt: values[j] = f(shift w_n^j); coset_intt
+ * its inverse. Errors before any value moves: TransformLength (len not a non-zero multiple of
+ * n), then ZeroShift (a zero shift) (BAD_SHAPE); a shift >= p is BAD_ARG. */
+tsr_status tsr_ntt(tsr_ntt_plan *plan, uint64_t *values, uint64_t len);
+tsr_status tsr_intt(tsr_ntt_plan *plan, uint64_t *values, uint64_t len);
+tsr_status tsr_coset_ntt(tsr_ntt_plan *plan, uint64_t *values, uint64_t len, uint64_t shift);
+tsr_status tsr_coset_intt(tsr_ntt_plan *plan, uint64_t *values, uint64_t len, uint64_t shift);
+void tsr_ntt_plan_free(tsr_ntt_plan *plan);
+
 /* ---- row encoding (RowEncoder, crates/tessera-pcs/src/encoding.rs:126-290;
  *      Device::encoder / encode_rows, crates/tessera-gpu/src/commit.rs:178,277) ---- */
 /* Encode `count` rows of k = 2^log_k values (row-major) into codewords of n = 2^(log_k+r)

@@ -6,7 +6,7 @@ This package is the Python mirror of the reference crates' API over that ABI.
 """
 from .device import Device, DeviceBuffer  # noqa: F401
 from .lib import BadShape, Declined, TsrError, declared_symbols  # noqa: F401
-from .pcs import (ColumnAnswer, ColumnChains, CommitmentParamsPy, ReferenceCommitment,  # noqa: F401
+from .pcs import (ColumnAnswer, ColumnChains, CommitmentParamsPy, NttPlan, ReferenceCommitment,  # noqa: F401
                   RowEncoder, merkle_root)
 
 __version__ = "0.1.0"

@@ -22,6 +22,10 @@
 // L2/MALL; the group of a butterfly at stage bit b is g = p >> (b + 1) for the low index p.
 #include <hip/hip_runtime.h>
 #include <stdint.h>
+#include <algorithm>
+#include <memory>
+#include <string>
+#include <vector>
 #include "field.h"
 #include "ntt.h"
 
@@ -566,3 +570,264 @@ Status get_encoder(Ctx *ctx, uint32_t log_k, uint32_t log_r, uint64_t P, Encoder
 }
 
 }  // namespace tsr
+
+// ---------------------------------------------------------------------------------------------
+// NttPlan (crates/tessera-pcs/src/ntt.rs:397-516) on the device: transforms of size n = 2^L, in
+// place, natural order on both sides, over a batch of rows. The same CT / GS stage passes as the
+// encoder (k_tile_pass), the bit reversal as tiles through LDS, the coset shift as a scaling by
+// s^i:
+//   ntt        = CT(forward table), then bit reversal
+//   intt       = bit reversal times 1/n, then GS(inverse table)
+//   coset_ntt  = values[i] *= s^i, then ntt          (f(s w^j) = sum_i (f_i s^i) w^{ij})
+//   coset_intt = intt, then values[i] *= s^{-i}      (the coefficients of f(sX) are f_i s^i)
+// equal, value for value, to the reference's Coset twiddles (ntt.rs:157-180), which apply the
+// same powers of s inside the butterflies.
+namespace tsr {
+
+constexpr uint32_t BR_BITS = 5, BR_T = 1u << BR_BITS;  // bit-reversal tiles of 32 x 32
+
+__device__ __forceinline__ uint64_t rev_low(uint64_t x, uint32_t bits) {
+    return bits ? (__brevll(x) >> (64 - bits)) : 0;
+}
+
+// Rows of 2^L >= 2^(2 BR_BITS) values, in place. Index i = (h, m, l) with h and l of BR_BITS bits
+// around a middle m of M = L - 2 BR_BITS bits: rev(i) = (rev(l), rev(m), rev(h)), so the 32 x 32
+// block of middle m (rows h, columns l: runs of 32 consecutive words) lands transposed, both
+// parts reversed, in the block of rev(m). Workgroup m < 2^M handles the pair (m, rev(m)) when
+// m <= rev(m); the pair owns both blocks, so no other workgroup reads or writes them.
+__global__ void __launch_bounds__(256)
+k_bitrev_tiles(uint64_t *__restrict__ v, uint64_t row_stride, uint32_t L, uint64_t scale, int scaled) {
+    const uint32_t M = L - 2 * BR_BITS;
+    const uint64_t m = blockIdx.x, rm = rev_low(m, M);
+    if (rm < m) return;
+    uint64_t *row = v + (uint64_t)blockIdx.y * row_stride;
+    __shared__ uint64_t A[BR_T][BR_T + 1], Bt[BR_T][BR_T + 1];
+    const uint32_t hs = M + BR_BITS;  // the shift of h (and of the output row r)
+    for (uint32_t e = threadIdx.x; e < BR_T * BR_T; e += blockDim.x) {
+        const uint32_t h = e >> BR_BITS, l = e & (BR_T - 1);
+        A[h][l] = row[((uint64_t)h << hs) | (m << BR_BITS) | l];
+        if (rm != m) Bt[h][l] = row[((uint64_t)h << hs) | (rm << BR_BITS) | l];
+    }
+    __syncthreads();
+    for (uint32_t e = threadIdx.x; e < BR_T * BR_T; e += blockDim.x) {
+        const uint32_t r = e >> BR_BITS, c = e & (BR_T - 1);  // output row rev(l), column rev(h)
+        const uint32_t l = (uint32_t)rev_low(r, BR_BITS), h = (uint32_t)rev_low(c, BR_BITS);
+        const uint64_t x = A[h][l];
+        row[((uint64_t)r << hs) | (rm << BR_BITS) | c] = scaled ? gl::mul(x, scale) : x;
+        if (rm != m) {
+            const uint64_t y = Bt[h][l];
+            row[((uint64_t)r << hs) | (m << BR_BITS) | c] = scaled ? gl::mul(y, scale) : y;
+        }
+    }
+}
+
+// Rows of 2^L < 2^(2 BR_BITS) values: one workgroup per row through LDS.
+__global__ void __launch_bounds__(256)
+k_bitrev_small(uint64_t *__restrict__ v, uint64_t row_stride, uint32_t L, uint64_t scale, int scaled) {
+    __shared__ uint64_t s[BR_T * BR_T];
+    uint64_t *row = v + (uint64_t)blockIdx.x * row_stride;
+    const uint32_t n = 1u << L;
+    for (uint32_t i = threadIdx.x; i < n; i += blockDim.x) s[i] = row[i];
+    __syncthreads();
+    for (uint32_t i = threadIdx.x; i < n; i += blockDim.x) {
+        const uint64_t x = s[rev_low(i, L)];
+        row[i] = scaled ? gl::mul(x, scale) : x;
+    }
+}
+
+// values[i] *= s^i within each row of 2^L: s^i = lo[i mod 2^a] * hi[i >> a].
+__global__ void __launch_bounds__(256)
+k_scale_powers(uint64_t *__restrict__ v, uint64_t total, uint32_t L, const uint64_t *__restrict__ lo,
+               const uint64_t *__restrict__ hi, uint32_t a) {
+    const uint64_t mask_n = (1ull << L) - 1, mask_a = (1ull << a) - 1;
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < total;
+         i += (uint64_t)gridDim.x * blockDim.x) {
+        const uint64_t j = i & mask_n;
+        v[i] = gl::mul(v[i], gl::mul(lo[j & mask_a], hi[j >> a]));
+    }
+}
+
+struct NttPlanDev {
+    Ctx *ctx = nullptr;
+    uint32_t L = 0;
+    DevBuf<uint64_t> fwd, inv;  // Z[t] = w^{rev(t)}, w^{-rev(t)}, t < n/2
+    uint64_t size_inv = 1;
+    std::vector<std::pair<uint32_t, uint32_t>> passes;  // stage-bit ranges, ascending
+
+    Status init(Ctx *c, uint32_t log_n) {
+        ctx = c;
+        L = log_n;
+        Status st;
+        if ((st = fwd.upload(ctx, twiddle_table(L, false)))) return st;
+        if ((st = inv.upload(ctx, twiddle_table(L, true)))) return st;
+        size_inv = gl::pow(gl::P / 2 + 1, L);
+        // the low bits in one pass of contiguous tiles (2^12 values), the rest in passes of up
+        // to 10 strided bits
+        const uint32_t low = L < 12 ? L : 12;
+        if (low) passes.push_back({0, low});
+        for (auto &p : split_bits(low, L, WIDE_PASS_NB)) passes.push_back(p);
+        return Status::ok();
+    }
+
+    // CT stages top-down (GS = false) or GS stages bottom-up (GS = true) over `rows` rows.
+    void stages(bool gs, uint64_t *v, uint64_t rows, hipStream_t s) const {
+        const uint64_t n = 1ull << L;
+        for (size_t i = 0; i < passes.size(); i++) {
+            const auto &ps = gs ? passes[i] : passes[passes.size() - 1 - i];
+            const uint32_t lo = ps.first, nb = ps.second, logc = pass_logc(lo, nb);
+            const size_t shm = sizeof(uint64_t) << (nb + logc);
+            const bool top = lo + nb == L;
+            TilePassFn f = gs ? tile_pass_fn<true>(nb, logc, top) : tile_pass_fn<false>(nb, logc, top);
+            for (uint64_t r0 = 0; r0 < rows; r0 += 65535) {
+                const uint64_t rc = rows - r0 < 65535 ? rows - r0 : 65535;
+                KernelTimer timer(ctx, gs ? "k_tile_pass<gs>" : "k_tile_pass<ct>", s);
+                hipLaunchKernelGGL(f, dim3((unsigned)(n >> (nb + logc)), (unsigned)rc),
+                                   dim3(tile_pass_threads(nb, logc)), shm, s, v + r0 * n, v + r0 * n, n, n,
+                                   L, lo, nb, logc, gs ? inv.ptr : fwd.ptr, nullptr, nullptr);
+            }
+        }
+    }
+
+    void bitrev(uint64_t *v, uint64_t rows, bool scaled, hipStream_t s) const {
+        const uint64_t n = 1ull << L;
+        for (uint64_t r0 = 0; r0 < rows; r0 += 65535) {
+            const uint64_t rc = rows - r0 < 65535 ? rows - r0 : 65535;
+            KernelTimer timer(ctx, "k_bitrev", s);
+            if (L >= 2 * BR_BITS)
+                hipLaunchKernelGGL(k_bitrev_tiles, dim3((unsigned)(1ull << (L - 2 * BR_BITS)), (unsigned)rc),
+                                   dim3(256), 0, s, v + r0 * n, n, L, size_inv, scaled ? 1 : 0);
+            else
+                hipLaunchKernelGGL(k_bitrev_small, dim3((unsigned)rc), dim3(256), 0, s, v + r0 * n, n, L,
+                                   size_inv, scaled ? 1 : 0);
+        }
+    }
+
+    // values[i] *= shift^i in every row; the two power tables go up with the call
+    Status scale_powers(uint64_t *v, uint64_t rows, uint64_t shift, hipStream_t s) const {
+        const uint32_t a = (L + 1) / 2;
+        std::vector<uint64_t> lo(1ull << a), hi(1ull << (L - a));
+        lo[0] = 1;
+        for (size_t i = 1; i < lo.size(); i++) lo[i] = gl::mul(lo[i - 1], shift);
+        const uint64_t step = gl::pow(shift, 1ull << a);
+        hi[0] = 1;
+        for (size_t i = 1; i < hi.size(); i++) hi[i] = gl::mul(hi[i - 1], step);
+        DevBuf<uint64_t> dlo, dhi;
+        Status st;
+        if ((st = dlo.upload(ctx, lo)) || (st = dhi.upload(ctx, hi))) return st;
+        const uint64_t total = rows << L;
+        const unsigned blocks = (unsigned)std::min<uint64_t>((total + 255) / 256, 1u << 16);
+        KernelTimer timer(ctx, "k_scale_powers", s);
+        hipLaunchKernelGGL(k_scale_powers, dim3(blocks), dim3(256), 0, s, v, total, L, dlo.ptr, dhi.ptr, a);
+        if ((st = check_launch("scale powers"))) return st;
+        // the tables go back to the context's pool, which other streams may draw from
+        TSR_HIP(hipStreamSynchronize(s));
+        return Status::ok();
+    }
+};
+
+}  // namespace tsr
+
+struct tsr_ntt_plan {
+    tsr::NttPlanDev p;
+};
+
+using namespace tsr;
+
+namespace {
+enum NttKind { NTT_FWD, NTT_INV, NTT_COSET_FWD, NTT_COSET_INV };
+
+tsr_status run_ntt(tsr_ntt_plan *plan, uint64_t *values, uint64_t len, NttKind kind, uint64_t shift) {
+    if (!plan || (len && !values)) return Status::fail(TSR_BAD_ARG, "null argument").code;
+    const NttPlanDev &p = plan->p;
+    const uint64_t n = 1ull << p.L;
+    // check (ntt.rs:441-450), then the shift (ZeroShift), before the values are touched
+    if (len == 0 || len % n)
+        return Status::fail(TSR_BAD_SHAPE, "TransformLength: expected a non-zero multiple of " +
+                                               std::to_string(n) + " values, found " + std::to_string(len)).code;
+    if ((kind == NTT_COSET_FWD || kind == NTT_COSET_INV) && shift == 0)
+        return Status::fail(TSR_BAD_SHAPE, "ZeroShift").code;
+    if (shift >= gl::P) return Status::fail(TSR_BAD_ARG, "the shift is not canonical").code;
+    Ctx *ctx = p.ctx;
+    hipError_t e = hipSetDevice(ctx->device);
+    if (e != hipSuccess) return hip_fail(e, "hipSetDevice").code;
+    hipStream_t s = ctx->stream;
+    const uint64_t rows = len / n;
+    uint64_t *v = values;
+    DevBuf<uint64_t> staged;
+    Status st;
+    const bool host = !is_device_ptr(values);
+    if (host) {
+        if ((st = staged.alloc(ctx, len))) return st.code;
+        e = hipMemcpyAsync(staged.ptr, values, 8 * len, hipMemcpyHostToDevice, s);
+        if (e != hipSuccess) return hip_fail(e, "copy").code;
+        v = staged.ptr;
+    }
+    if (p.L > 0) {
+        switch (kind) {
+            case NTT_COSET_FWD:
+                if ((st = p.scale_powers(v, rows, shift, s))) return st.code;
+                // fall through
+            case NTT_FWD:
+                p.stages(false, v, rows, s);
+                p.bitrev(v, rows, false, s);
+                break;
+            case NTT_INV:
+            case NTT_COSET_INV:
+                p.bitrev(v, rows, true, s);
+                p.stages(true, v, rows, s);
+                if (kind == NTT_COSET_INV && (st = p.scale_powers(v, rows, gl::inv(shift), s))) return st.code;
+                break;
+        }
+        if ((st = check_launch("ntt"))) return st.code;
+    }
+    if (host) {
+        e = hipMemcpyAsync(values, staged.ptr, 8 * len, hipMemcpyDeviceToHost, s);
+        if (e != hipSuccess) return hip_fail(e, "copy").code;
+    }
+    e = hipStreamSynchronize(s);
+    if (e != hipSuccess) return hip_fail(e, "sync").code;
+    return TSR_OK;
+}
+}  // namespace
+
+extern "C" {
+
+tsr_status tsr_ntt_plan_create(tsr_ctx *ctx_, uint64_t size, tsr_ntt_plan **out) {
+    Ctx *ctx = (Ctx *)ctx_;
+    if (!ctx || !out) return Status::fail(TSR_BAD_ARG, "null argument").code;
+    // NttPlan::new (ntt.rs:430-437): log2_len
+    if (size == 0 || (size & (size - 1)))
+        return Status::fail(TSR_BAD_SHAPE, "NotPowerOfTwo: " + std::to_string(size)).code;
+    if (size > (1ull << 32)) return Status::fail(TSR_BAD_SHAPE, "TooLong: " + std::to_string(size)).code;
+    hipError_t e = hipSetDevice(ctx->device);
+    if (e != hipSuccess) return hip_fail(e, "hipSetDevice").code;
+    std::unique_ptr<tsr_ntt_plan> p(new tsr_ntt_plan());
+    Status st = p->p.init(ctx, (uint32_t)__builtin_ctzll(size));
+    if (st) return st.code;
+    *out = p.release();
+    return TSR_OK;
+}
+
+uint64_t tsr_ntt_plan_size(const tsr_ntt_plan *plan) { return plan ? 1ull << plan->p.L : 0; }
+
+tsr_status tsr_ntt(tsr_ntt_plan *plan, uint64_t *values, uint64_t len) {
+    return run_ntt(plan, values, len, NTT_FWD, 0);
+}
+tsr_status tsr_intt(tsr_ntt_plan *plan, uint64_t *values, uint64_t len) {
+    return run_ntt(plan, values, len, NTT_INV, 0);
+}
+tsr_status tsr_coset_ntt(tsr_ntt_plan *plan, uint64_t *values, uint64_t len, uint64_t shift) {
+    return run_ntt(plan, values, len, NTT_COSET_FWD, shift);
+}
+tsr_status tsr_coset_intt(tsr_ntt_plan *plan, uint64_t *values, uint64_t len, uint64_t shift) {
+    return run_ntt(plan, values, len, NTT_COSET_INV, shift);
+}
+
+void tsr_ntt_plan_free(tsr_ntt_plan *plan) {
+    if (!plan) return;
+    (void)hipSetDevice(plan->p.ctx->device);
+    (void)hipStreamSynchronize(plan->p.ctx->stream);
+    delete plan;
+}
+
+}  // extern "C"

@@ -55,6 +55,13 @@ _SIGS = {
     "tsr_copy": (i32, [vp, vp, vp, u64]),
     "tsr_fill_uniform": (i32, [vp, vp, u64, u64]),
     "tsr_encode_rows": (i32, [vp, u32, u32, u64, vp, vp, u64, vp]),
+    "tsr_ntt_plan_create": (i32, [vp, u64, ctypes.POINTER(vp)]),
+    "tsr_ntt_plan_size": (u64, [vp]),
+    "tsr_ntt": (i32, [vp, vp, u64]),
+    "tsr_intt": (i32, [vp, vp, u64]),
+    "tsr_coset_ntt": (i32, [vp, vp, u64, u64]),
+    "tsr_coset_intt": (i32, [vp, vp, u64, u64]),
+    "tsr_ntt_plan_free": (None, [vp]),
     "tsr_chains_create": (i32, [vp, vp, u64, u64, ctypes.POINTER(vp)]),
     "tsr_chains_absorb": (i32, [vp, vp, u64]),
     "tsr_chains_read": (i32, [vp, vp]),

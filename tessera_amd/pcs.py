@@ -16,6 +16,54 @@ from .lib import CommitmentParams as _CParams
 from .lib import check, lib, u64, vp
 
 
+class NttPlan:
+    """NttPlan (crates/tessera-pcs/src/ntt.rs:397-516) on the device: transforms of `size` points
+    in place, natural order on both sides. `values` is a numpy uint64 array of n values or of
+    rows x n (each row one transform; numpy arrays are transformed in place) or a DeviceBuffer of
+    a multiple of n values."""
+
+    def __init__(self, device: Device, size: int):
+        h = vp()
+        check(lib().tsr_ntt_plan_create(device.handle, size, ctypes.byref(h)))
+        self.handle = h.value
+        self.device = device
+
+    def size(self):
+        return int(lib().tsr_ntt_plan_size(self.handle))
+
+    def _run(self, fn, values, *shift):
+        if isinstance(values, DeviceBuffer):
+            check(fn(self.handle, values.ptr, values.count, *shift))
+            return values
+        if not (isinstance(values, np.ndarray) and values.dtype == np.uint64 and values.flags.c_contiguous):
+            raise TypeError("values: a C-contiguous numpy uint64 array or a DeviceBuffer")
+        check(fn(self.handle, values.ctypes.data if values.size else None, values.size, *shift))
+        return values
+
+    def ntt(self, values):
+        return self._run(lib().tsr_ntt, values)
+
+    def intt(self, values):
+        return self._run(lib().tsr_intt, values)
+
+    def coset_ntt(self, values, shift):
+        return self._run(lib().tsr_coset_ntt, values, shift)
+
+    def coset_intt(self, values, shift):
+        return self._run(lib().tsr_coset_intt, values, shift)
+
+    def free(self):
+        if self.handle:
+            lib().tsr_ntt_plan_free(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
 class RowEncoder:
     """RowEncoder::new(message_len, log2_inv_rate, mask_coeffs) (encoding.rs:126-170)."""
 

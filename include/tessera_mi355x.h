@@ -136,6 +136,35 @@ tsr_status tsr_matrix_sink_absorb(tsr_matrix_sink *sink, const uint64_t *rows, u
 tsr_status tsr_matrix_sink_finish(tsr_matrix_sink *sink, uint8_t *digests);
 void tsr_matrix_sink_free(tsr_matrix_sink *sink);
 
+/* ---- ColumnSink (crates/tessera-pcs/src/streaming/opening.rs:845-890; the reference's device
+ *      sink crates/tessera-prover/src/gpu.rs:195-360): P5's answers re-encoded block by block,
+ *      P4's combined rows summed on the device as the blocks pass ---- */
+typedef struct tsr_column_sink tsr_column_sink;
+/* A sink for codewords of RowEncoder(2^log_k, r, mask_coeffs) in blocks of up to `block` rows,
+ * answering the m distinct increasing `columns`. NoQueries (m = 0), QueryRange (a column >= n),
+ * TooLong / MaskTooLarge (BAD_SHAPE). */
+tsr_status tsr_column_sink_create(tsr_ctx *ctx, uint32_t log_k, uint32_t log2_inv_rate,
+                                  uint64_t mask_coeffs, uint64_t block, const uint64_t *columns,
+                                  uint64_t m, tsr_column_sink **out);
+/* ColumnSink::select: encodes `count` whole rows (count x k) with their masks (count x P, m_0
+ * first) and writes each row's symbols at the sink's columns, row by row (count x m). */
+tsr_status tsr_column_sink_select(tsr_column_sink *s, const uint64_t *rows, const uint64_t *masks,
+                                  uint64_t count, uint64_t *symbols_out);
+/* ColumnSink::combines: 1 (the sums of select_combining stay on the device). */
+int tsr_column_sink_combines(const tsr_column_sink *s);
+/* ColumnSink::select_combining: as select, and adds sum_r coefficients[g][r] rows[r] to the
+ * sink's g-th sum (coefficients: groups x count Fp2, group-major; the same group count for
+ * every block of an opening). */
+tsr_status tsr_column_sink_select_combining(tsr_column_sink *s, const uint64_t *rows,
+                                            const uint64_t *masks, uint64_t count,
+                                            const uint64_t *coefficients, uint32_t groups,
+                                            uint64_t *symbols_out);
+/* ColumnSink::combined: adds the g-th sum into out[g] (groups x k Fp2, host memory) and clears
+ * the sums; OK with nothing added when no block was combined. The host copy is cleared too. */
+tsr_status tsr_column_sink_combined(tsr_column_sink *s, uint64_t *out, uint32_t groups);
+/* Rows, masks, codewords and sums are cleared before the memory is released. */
+void tsr_column_sink_free(tsr_column_sink *s);
+
 /* ---- Merkle (MerkleTree::new / root, crates/tessera-hash/src/merkle.rs:93-119) ---- */
 /* root of the binary BLAKE3 tree over n = 2^m leaves (32 bytes each). */
 tsr_status tsr_merkle_root(tsr_ctx *ctx, const uint8_t *leaves, uint64_t n, uint8_t root[32]);

@@ -214,3 +214,183 @@ tsr_status tsr_matrix_sink_finish(tsr_block_sink *s, uint8_t *digests) {
 void tsr_matrix_sink_free(tsr_block_sink *s) { sink_free(s); }
 
 }  // extern "C"
+
+// ---- ColumnSink (crates/tessera-pcs/src/streaming/opening.rs:845-890): P5's column answers from
+// re-encoded blocks of rows, and P4's combined rows summed on the device as the blocks pass.
+struct tsr_column_sink {
+    tsr::Ctx *ctx = nullptr;
+    uint32_t log_k = 0, log_r = 0;
+    uint64_t P = 0, block = 0, k = 0, n = 0, m = 0;
+    uint32_t groups = 0;  // of the combined rows being summed (0: none yet)
+    tsr::DevBuf<uint64_t> rows, masks, codewords, columns, out;
+    tsr::DevBuf<gl::fp2> coef, sums, part;  // a block's coefficients, the running sums, a block's sums
+};
+
+namespace tsr {
+
+__global__ void k_add_fp2(gl::fp2 *__restrict__ acc, const gl::fp2 *__restrict__ x, uint64_t n) {
+    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) acc[i] = gl::add(acc[i], x[i]);
+}
+
+static void column_sink_clear(tsr_column_sink *s) {
+    hipStream_t st = s->ctx->stream;
+    zero(s->rows, st);
+    zero(s->masks, st);
+    zero(s->codewords, st);
+    zero(s->out, st);
+    if (s->sums.ptr) zero_words((uint64_t *)s->sums.ptr, 2 * s->sums.len, st);
+    if (s->part.ptr) zero_words((uint64_t *)s->part.ptr, 2 * s->part.len, st);
+}
+
+// One block: optionally the combined rows' sums (before the encode, as the reference's device
+// sink orders them), the encode, the symbols at the sink's columns (row by row) into `symbols`.
+static Status column_sink_run(tsr_column_sink *s, const uint64_t *rows, const uint64_t *masks,
+                              uint64_t count, const uint64_t *coefficients, uint32_t groups,
+                              uint64_t *symbols) {
+    Ctx *ctx = s->ctx;
+    if (count == 0 || count > s->block)
+        return Status::fail(TSR_BAD_SHAPE, "a block of 1 to " + std::to_string(s->block) + " whole rows");
+    if (!rows || (s->P && !masks) || !symbols || (groups && !coefficients))
+        return Status::fail(TSR_BAD_ARG, "null argument");
+    if (groups && s->groups && groups != s->groups)
+        return Status::fail(TSR_BAD_SHAPE, "one group count per opening");
+    hipError_t e = hipSetDevice(ctx->device);
+    if (e != hipSuccess) return hip_fail(e, "hipSetDevice");
+    hipStream_t st_ = ctx->stream;
+    Status st;
+    TSR_HIP(hipMemcpyAsync(s->rows.ptr, rows, 8 * count * s->k, hipMemcpyDefault, st_));
+    if (s->P) TSR_HIP(hipMemcpyAsync(s->masks.ptr, masks, 8 * count * s->P, hipMemcpyDefault, st_));
+    if (groups) {
+        for (uint64_t i = 0; i < 2ull * groups * count; i++)
+            if (coefficients[i] >= gl::P) return Status::fail(TSR_BAD_ARG, "a coefficient is not canonical");
+        if (!s->groups) {
+            if ((st = s->sums.alloc(ctx, (uint64_t)groups * s->k)) || (st = s->part.alloc(ctx, (uint64_t)groups * s->k)) ||
+                (st = s->coef.alloc(ctx, (uint64_t)groups * s->block)))
+                return st;
+            zero_words((uint64_t *)s->sums.ptr, 2 * s->sums.len, st_);
+            s->groups = groups;
+        }
+        // coefficients[g][r] of the block's rows, group-major: combine_rows reads coef[g * N + r]
+        TSR_HIP(hipMemcpyAsync(s->coef.ptr, coefficients, 16ull * groups * count, hipMemcpyDefault, st_));
+        if ((st = combine_rows(ctx, s->coef.ptr, groups, s->rows.ptr, count, s->k, s->part.ptr, st_))) return st;
+        const uint64_t len = (uint64_t)groups * s->k;
+        hipLaunchKernelGGL(k_add_fp2, dim3((unsigned)((len + 255) / 256)), dim3(256), 0, st_, s->sums.ptr,
+                           s->part.ptr, len);
+    }
+    Encoder *enc = nullptr;
+    if ((st = get_encoder(ctx, s->log_k, s->log_r, s->P, &enc))) return st;
+    if ((st = ctx->scratch.reserve(ctx, enc->scratch_elems(count)))) return st;
+    if ((st = enc->encode(s->rows.ptr, s->P ? s->masks.ptr : nullptr, count, s->codewords.ptr,
+                          ctx->scratch.ptr, st_)))
+        return st;
+    const uint64_t total = count * s->m;
+    hipLaunchKernelGGL(k_gather_row_symbols, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, st_,
+                       s->codewords.ptr, s->n, count, s->columns.ptr, s->m, s->out.ptr);
+    if ((st = check_launch("column sink"))) return st;
+    TSR_HIP(hipMemcpyAsync(symbols, s->out.ptr, 8 * total, hipMemcpyDefault, st_));
+    TSR_HIP(hipStreamSynchronize(st_));
+    return Status::ok();
+}
+
+}  // namespace tsr
+
+extern "C" {
+
+tsr_status tsr_column_sink_create(tsr_ctx *ctx_, uint32_t log_k, uint32_t log2_inv_rate,
+                                  uint64_t mask_coeffs, uint64_t block, const uint64_t *columns,
+                                  uint64_t m, tsr_column_sink **out) {
+    Ctx *ctx = (Ctx *)ctx_;
+    if (!ctx || !out || (m && !columns)) return Status::fail(TSR_BAD_ARG, "null argument").code;
+    if ((uint64_t)log_k + log2_inv_rate > 32 || log2_inv_rate == 0)
+        return Status::fail(TSR_BAD_SHAPE, "TooLong or rate: n = 2^(log_k + r) with 1 <= r, at most 2^32").code;
+    if (mask_coeffs > (1ull << log_k)) return Status::fail(TSR_BAD_SHAPE, "MaskTooLarge: mask_coeffs exceeds k").code;
+    if (block == 0 || block > 65535) return Status::fail(TSR_BAD_SHAPE, "a block of 1 to 65535 rows").code;
+    if (m == 0) return Status::fail(TSR_BAD_SHAPE, "NoQueries").code;
+    const uint64_t n = 1ull << (log_k + log2_inv_rate);
+    for (uint64_t i = 0; i < m; i++) {
+        if (columns[i] >= n) return Status::fail(TSR_BAD_SHAPE, "QueryRange: a column beyond the codeword").code;
+        if (i && columns[i] <= columns[i - 1])
+            return Status::fail(TSR_BAD_SHAPE, "columns must be distinct and increasing").code;
+    }
+    hipError_t e = hipSetDevice(ctx->device);
+    if (e != hipSuccess) return hip_fail(e, "hipSetDevice").code;
+    std::unique_ptr<tsr_column_sink> s(new tsr_column_sink());
+    s->ctx = ctx;
+    s->log_k = log_k;
+    s->log_r = log2_inv_rate;
+    s->P = mask_coeffs;
+    s->block = block;
+    s->k = 1ull << log_k;
+    s->n = n;
+    s->m = m;
+    Encoder *enc = nullptr;
+    Status st;
+    if ((st = get_encoder(ctx, log_k, log2_inv_rate, mask_coeffs, &enc))) return st.code;
+    if ((st = s->rows.alloc(ctx, block * s->k)) || (st = s->masks.alloc(ctx, std::max<uint64_t>(1, block * mask_coeffs))) ||
+        (st = s->codewords.alloc(ctx, block * n)) || (st = s->columns.alloc(ctx, m)) ||
+        (st = s->out.alloc(ctx, block * m)))
+        return st.code;
+    e = hipMemcpyAsync(s->columns.ptr, columns, 8 * m, hipMemcpyDefault, ctx->stream);
+    if (e != hipSuccess) return hip_fail(e, "columns").code;
+    e = hipStreamSynchronize(ctx->stream);
+    if (e != hipSuccess) return hip_fail(e, "columns").code;
+    *out = s.release();
+    return TSR_OK;
+}
+
+tsr_status tsr_column_sink_select(tsr_column_sink *s, const uint64_t *rows, const uint64_t *masks,
+                                  uint64_t count, uint64_t *symbols_out) {
+    if (!s) return Status::fail(TSR_BAD_ARG, "null sink").code;
+    return column_sink_run(s, rows, masks, count, nullptr, 0, symbols_out).code;
+}
+
+int tsr_column_sink_combines(const tsr_column_sink *s) { return s ? 1 : 0; }
+
+tsr_status tsr_column_sink_select_combining(tsr_column_sink *s, const uint64_t *rows,
+                                            const uint64_t *masks, uint64_t count,
+                                            const uint64_t *coefficients, uint32_t groups,
+                                            uint64_t *symbols_out) {
+    if (!s) return Status::fail(TSR_BAD_ARG, "null sink").code;
+    if (groups == 0) return column_sink_run(s, rows, masks, count, nullptr, 0, symbols_out).code;
+    return column_sink_run(s, rows, masks, count, coefficients, groups, symbols_out).code;
+}
+
+tsr_status tsr_column_sink_combined(tsr_column_sink *s, uint64_t *out, uint32_t groups) {
+    if (!s) return Status::fail(TSR_BAD_ARG, "null sink").code;
+    if (!s->groups) return TSR_OK;  // nothing was combined
+    if (groups != s->groups || !out) return Status::fail(TSR_BAD_SHAPE, "one combined row per group").code;
+    hipError_t e = hipSetDevice(s->ctx->device);
+    if (e != hipSuccess) return hip_fail(e, "hipSetDevice").code;
+    const uint64_t words = 2ull * groups * s->k;
+    std::vector<uint64_t> h(words);
+    e = hipMemcpyAsync(h.data(), s->sums.ptr, 8 * words, hipMemcpyDeviceToHost, s->ctx->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(s->ctx->stream);
+    // the sums are secret-class: cleared on the device whatever happened, then on the host
+    zero_words((uint64_t *)s->sums.ptr, words, s->ctx->stream);
+    (void)hipStreamSynchronize(s->ctx->stream);
+    s->groups = 0;
+    if (e != hipSuccess) {
+        memset(h.data(), 0, 8 * words);
+        return hip_fail(e, "combined rows").code;
+    }
+    for (uint64_t i = 0; i < words; i += 2) {
+        gl::fp2 a{out[i], out[i + 1]}, b{h[i], h[i + 1]};
+        gl::fp2 c = gl::add(a, b);
+        out[i] = c.c0;
+        out[i + 1] = c.c1;
+    }
+    volatile uint64_t *vh = h.data();
+    for (uint64_t i = 0; i < words; i++) vh[i] = 0;
+    return TSR_OK;
+}
+
+void tsr_column_sink_free(tsr_column_sink *s) {
+    if (!s) return;
+    (void)hipSetDevice(s->ctx->device);
+    column_sink_clear(s);
+    (void)hipStreamSynchronize(s->ctx->stream);
+    delete s;
+}
+
+}  // extern "C"

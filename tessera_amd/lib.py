@@ -66,6 +66,12 @@ _SIGS = {
     "tsr_chains_absorb": (i32, [vp, vp, u64]),
     "tsr_chains_read": (i32, [vp, vp]),
     "tsr_chains_free": (None, [vp]),
+    "tsr_column_sink_create": (i32, [vp, u32, u32, u64, u64, vp, u64, ctypes.POINTER(vp)]),
+    "tsr_column_sink_select": (i32, [vp, vp, vp, u64, vp]),
+    "tsr_column_sink_combines": (i32, [vp]),
+    "tsr_column_sink_select_combining": (i32, [vp, vp, vp, u64, vp, u32, vp]),
+    "tsr_column_sink_combined": (i32, [vp, vp, u32]),
+    "tsr_column_sink_free": (None, [vp]),
     "tsr_merkle_root": (i32, [vp, vp, u64, vp]),
     "tsr_merkle_root_noalloc": (i32, [vp, vp, u64, vp]),
     "tsr_commit_unmasked": (i32, [vp, ctypes.POINTER(CommitmentParams), vp, u64,

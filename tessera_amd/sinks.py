@@ -102,3 +102,62 @@ class WeightEncoding:
         except Declined:
             return None
         return MatrixSink(h.value, 1 << log_k, columns, q.size)
+
+
+class ColumnSink:
+    """ColumnSink (crates/tessera-pcs/src/streaming/opening.rs:845-890): P5's symbols at the sink's
+    columns from re-encoded blocks of rows, and P4's combined rows summed on the device."""
+
+    def __init__(self, device, log_k, log2_inv_rate, mask_coeffs, block, columns):
+        self.k, self.P = 1 << log_k, mask_coeffs
+        cols = np.ascontiguousarray(columns, dtype=np.uint64)
+        self.m = cols.size
+        h = vp()
+        check(lib().tsr_column_sink_create(device.handle, log_k, log2_inv_rate, mask_coeffs, block,
+                                           cols.ctypes.data if cols.size else None, cols.size,
+                                           ctypes.byref(h)))
+        self.handle = h.value
+
+    def combines(self):
+        return bool(lib().tsr_column_sink_combines(self.handle))
+
+    def _args(self, rows, masks, keep):
+        count = (rows.count if isinstance(rows, DeviceBuffer) else np.asarray(rows).size) // self.k
+        m = _ptr(masks, keep) if self.P else None
+        out = np.zeros(count * self.m, dtype=np.uint64)
+        return count, m, out
+
+    def select(self, rows, masks=None):
+        """Each row's symbols at the columns: an array of count x m."""
+        keep = []
+        count, m, out = self._args(rows, masks, keep)
+        check(lib().tsr_column_sink_select(self.handle, _ptr(rows, keep), m, count, out.ctypes.data))
+        return out.reshape(count, self.m)
+
+    def select_combining(self, rows, masks, coefficients):
+        """coefficients: groups x count Fp2 (array (groups, count, 2)), each group's coefficient of
+        each row of the block."""
+        keep = []
+        count, m, out = self._args(rows, masks, keep)
+        c = np.ascontiguousarray(coefficients, dtype=np.uint64).reshape(-1, count, 2)
+        check(lib().tsr_column_sink_select_combining(self.handle, _ptr(rows, keep), m, count,
+                                                     c.ctypes.data, c.shape[0], out.ctypes.data))
+        return out.reshape(count, self.m)
+
+    def combined(self, out):
+        """Adds the summed combined rows into out (groups x k Fp2, a uint64 array (groups, k, 2),
+        updated in place) and clears them."""
+        assert out.dtype == np.uint64 and out.flags.c_contiguous
+        check(lib().tsr_column_sink_combined(self.handle, out.ctypes.data, out.size // (2 * self.k)))
+        return out
+
+    def free(self):
+        if self.handle:
+            lib().tsr_column_sink_free(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass

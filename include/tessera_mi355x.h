@@ -549,6 +549,46 @@ tsr_status tsr_sumcheck_proof_decode(const uint8_t *in, uint64_t len, const tsr_
                                      uint64_t *values, uint64_t values_cap, uint64_t *n_values,
                                      uint64_t *finals, uint64_t finals_cap, uint64_t *n_finals,
                                      int *codec_error, uint64_t *error_offset);
+/* One opened weight matrix of a Round3Response (MatrixColumns,
+ * crates/tessera-protocol/src/round3.rs:150-175). */
+typedef struct {
+    uint8_t matrix;           /* WeightMatrix id 0..9; strictly increasing across the entries */
+    uint64_t columns;         /* distinct queried columns, in increasing order */
+    uint64_t symbols;         /* symbols per column */
+    const uint64_t *values;   /* columns x symbols, column after column */
+    uint64_t siblings;        /* the canonical multiproof's siblings */
+    const uint8_t *proof;     /* siblings x 32 bytes */
+} tsr_matrix_columns;
+/* TCE(Round3Response { advice_columns, advice_proof, weight_columns }) (round3.rs:100-175,
+ * schema 0x0513): Vec<Vec<Fp>> of the advice's answered columns (advice_values: columns x
+ * symbols, column after column, as tsr_commitment_answer gives them), the advice multiproof
+ * (Vec<Digest>), then Vec<MatrixColumns>. Non-canonical symbols or unordered matrix ids are
+ * BAD_ARG; a short `out` is BAD_SHAPE with *out_len the size needed. */
+tsr_status tsr_round3_encode(const uint64_t *advice_values, uint64_t advice_columns,
+                             uint64_t advice_symbols, const uint8_t *advice_proof,
+                             uint64_t advice_siblings, const tsr_matrix_columns *weights,
+                             uint32_t n_weights, uint8_t *out, uint64_t cap, uint64_t *out_len);
+/* Decodes and checks a Round3Response with the decoder's checks in order (limits, depth with
+ * structs, sequences and digests one level each, lengths against the bytes left, canonical
+ * symbols, matrix ids below 10 and strictly increasing, no trailing byte): BAD_SHAPE with
+ * codec_error and error_offset on a malformed input, else the counts of advice columns and
+ * opened matrices. */
+tsr_status tsr_round3_decode(const uint8_t *in, uint64_t len, const tsr_decode_limits *limits,
+                             uint64_t *advice_columns, uint32_t *n_matrices, int *codec_error,
+                             uint64_t *error_offset);
+/* TCE(ColumnQueries { advice: Vec<u32>, weights: Vec<MatrixQueries { matrix, indices }> })
+ * (round3.rs:19-95, schema 0x0512): the queries in draw order; matrix m's counts[m] indices
+ * follow one another in `indices`. */
+tsr_status tsr_column_queries_encode(const uint32_t *advice, uint64_t n_advice, const uint8_t *matrices,
+                                     const uint64_t *counts, const uint32_t *indices, uint32_t n_matrices,
+                                     uint8_t *out, uint64_t cap, uint64_t *out_len);
+/* Decodes the verifier's ColumnQueries (the checks as tsr_round3_decode's) into the advice
+ * indices, the matrix ids with their index counts, and every matrix's indices one after another. */
+tsr_status tsr_column_queries_decode(const uint8_t *in, uint64_t len, const tsr_decode_limits *limits,
+                                     uint32_t *advice, uint64_t advice_cap, uint64_t *n_advice,
+                                     uint8_t *matrices, uint64_t *counts, uint32_t matrices_cap,
+                                     uint32_t *n_matrices, uint32_t *indices, uint64_t indices_cap,
+                                     uint64_t *n_indices, int *codec_error, uint64_t *error_offset);
 typedef struct {             /* CommitmentDescriptor::Advice */
     uint8_t statement_digest[32];
     uint32_t unit;

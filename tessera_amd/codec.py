@@ -90,3 +90,95 @@ def weight_matrix_commitment_id(layout: dict, matrix: int, log2_k, row_block, ro
     check(lib().tsr_weight_matrix_commitment_id(ctypes.byref(p), matrix, log2_k, row_block, rows,
                                                 epoch, tce, cid))
     return tce.raw, cid.raw
+
+
+class MatrixColumns(ctypes.Structure):
+    """MatrixColumns (crates/tessera-protocol/src/round3.rs:150-175) as tsr_round3_encode takes it."""
+    _fields_ = [("matrix", ctypes.c_uint8), ("columns", u64), ("symbols", u64), ("values", ctypes.c_void_p),
+                ("siblings", u64), ("proof", ctypes.c_void_p)]
+
+
+def _columns_array(columns):
+    cols = [list(c) for c in columns]
+    syms = len(cols[0]) if cols else 0
+    if any(len(c) != syms for c in cols):
+        raise ValueError("every answered column holds the same number of symbols")
+    return np.ascontiguousarray(np.array(cols, dtype=np.uint64).reshape(-1)), len(cols), syms
+
+
+def encode_round3(advice_columns, advice_siblings, weights=()) -> bytes:
+    """TCE(Round3Response): the advice's answered columns (each a list of symbols, one per
+    committed row) and its multiproof siblings (32-byte digests), then (matrix id, columns,
+    siblings) per opened weight matrix in increasing id order."""
+    keep = []
+    av, ac, asym = _columns_array(advice_columns)
+    ap = b"".join(advice_siblings)
+    ws = (MatrixColumns * max(1, len(weights)))()
+    for i, (matrix, cols, sib) in enumerate(weights):
+        v, c, sy = _columns_array(cols)
+        pr = ctypes.create_string_buffer(b"".join(sib), max(1, 32 * len(sib)))
+        keep += [v, pr]
+        ws[i] = MatrixColumns(matrix, c, sy, v.ctypes.data if v.size else None, len(sib), ctypes.addressof(pr))
+    need = u64()
+    args = (av.ctypes.data if av.size else None, ac, asym, ap or None, len(advice_siblings), ws, len(weights))
+    lib().tsr_round3_encode(*args, None, 0, ctypes.byref(need))
+    out = ctypes.create_string_buffer(max(1, need.value))
+    check(lib().tsr_round3_encode(*args, out, need.value, ctypes.byref(need)))
+    return out.raw[:need.value]
+
+
+def round3_from_answers(advice, weights=()) -> bytes:
+    """Round3Response bytes from ColumnAnswers: the advice commitment's, then (matrix id,
+    answer) per opened weight matrix."""
+    return encode_round3(advice.columns, advice.siblings,
+                         [(m, a.columns, a.siblings) for m, a in weights])
+
+
+def decode_round3(data: bytes, limits: DecodeLimits = DEFAULT_LIMITS):
+    """Checks a Round3Response: (advice columns, opened matrices), or CodecError."""
+    cols, mats, err, off = u64(), u32(), ctypes.c_int(), u64()
+    rc = lib().tsr_round3_decode(data, len(data), ctypes.byref(limits), ctypes.byref(cols), ctypes.byref(mats),
+                                 ctypes.byref(err), ctypes.byref(off))
+    if rc and err.value:
+        raise CodecError(CODEC_ERRORS[err.value], off.value, lib().tsr_last_error().decode())
+    check(rc)
+    return cols.value, mats.value
+
+
+def encode_column_queries(advice, weights=()) -> bytes:
+    """TCE(ColumnQueries): the advice indices, then (matrix id, indices) per opened matrix."""
+    a = np.ascontiguousarray(np.asarray(advice, dtype=np.uint32))
+    ids = np.array([m for m, _ in weights], dtype=np.uint8)
+    counts = np.array([len(q) for _, q in weights], dtype=np.uint64)
+    idx = np.ascontiguousarray(np.concatenate([np.asarray(q, dtype=np.uint32) for _, q in weights])
+                               if weights else np.zeros(0, np.uint32))
+    args = (a.ctypes.data if a.size else None, a.size, ids.ctypes.data if ids.size else None,
+            counts.ctypes.data if counts.size else None, idx.ctypes.data if idx.size else None, len(weights))
+    need = u64()
+    lib().tsr_column_queries_encode(*args, None, 0, ctypes.byref(need))
+    out = ctypes.create_string_buffer(max(1, need.value))
+    check(lib().tsr_column_queries_encode(*args, out, need.value, ctypes.byref(need)))
+    return out.raw[:need.value]
+
+
+def decode_column_queries(data: bytes, limits: DecodeLimits = DEFAULT_LIMITS):
+    """(advice indices, [(matrix id, indices)]) of a ColumnQueries, or CodecError."""
+    cap = len(data) // 4 + 1
+    adv = np.zeros(cap, np.uint32)
+    ids = np.zeros(cap, np.uint8)
+    cnt = np.zeros(cap, np.uint64)
+    idx = np.zeros(cap, np.uint32)
+    na, nm, ni, err, off = u64(), u32(), u64(), ctypes.c_int(), u64()
+    rc = lib().tsr_column_queries_decode(data, len(data), ctypes.byref(limits), adv.ctypes.data, cap,
+                                         ctypes.byref(na), ids.ctypes.data, cnt.ctypes.data, cap,
+                                         ctypes.byref(nm), idx.ctypes.data, cap, ctypes.byref(ni),
+                                         ctypes.byref(err), ctypes.byref(off))
+    if rc and err.value:
+        raise CodecError(CODEC_ERRORS[err.value], off.value, lib().tsr_last_error().decode())
+    check(rc)
+    out, at = [], 0
+    for m in range(nm.value):
+        c = int(cnt[m])
+        out.append((int(ids[m]), [int(x) for x in idx[at:at + c]]))
+        at += c
+    return [int(x) for x in adv[:na.value]], out

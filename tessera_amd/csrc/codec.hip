@@ -86,6 +86,70 @@ struct Reader {
         leave();
         return true;
     }
+    bool u8(uint8_t &v) {
+        if (len - at < 1) return fail(CODEC_TRUNCATED, at);
+        v = p[at++];
+        return true;
+    }
+    bool u32(uint32_t &v) {
+        if (len - at < 4) return fail(CODEC_TRUNCATED, at);
+        v = 0;
+        for (int i = 0; i < 4; i++) v |= (uint32_t)p[at + i] << (8 * i);
+        at += 4;
+        return true;
+    }
+    // a WeightMatrix id inside a by-matrix sequence: below 10 and above the previous entry's
+    // (crates/tessera-protocol/src/matrix.rs, RFC-0036 'Matrix ids')
+    bool matrix_id(uint8_t &v, int previous) {
+        const uint64_t start = at;
+        if (!u8(v)) return false;
+        if (v >= 10 || (int)v <= previous) return fail(CODEC_NONCANONICAL, start);
+        return true;
+    }
+    // Vec<Vec<Fp>>: the columns of an answer, each column's symbols after its length
+    bool columns(std::vector<std::vector<uint64_t>> &out) {
+        if (!enter()) return false;
+        uint64_t n;
+        if (!seq(n, 8)) return false;
+        out.resize(n);
+        for (uint64_t c = 0; c < n; c++) {
+            if (!enter()) return false;
+            uint64_t m;
+            if (!seq(m, 8)) return false;
+            out[c].resize(m);
+            for (uint64_t i = 0; i < m; i++)
+                if (!fp(out[c][i])) return false;
+            leave();
+        }
+        leave();
+        return true;
+    }
+    // MultiProof { siblings: Vec<Digest> }, a Digest a [u8; 32]
+    bool multiproof(std::vector<uint8_t> &sib) {
+        if (!enter() || !enter()) return false;
+        uint64_t n;
+        if (!seq(n, 32)) return false;
+        sib.resize(32 * n);
+        for (uint64_t i = 0; i < n; i++) {
+            if (!enter()) return false;
+            memcpy(&sib[32 * i], p + at, 32);  // seq checked that n digests fit
+            at += 32;
+            leave();
+        }
+        leave();
+        leave();
+        return true;
+    }
+    bool vec_u32(std::vector<uint32_t> &out) {
+        if (!enter()) return false;
+        uint64_t n;
+        if (!seq(n, 4)) return false;
+        out.resize(n);
+        for (uint64_t i = 0; i < n; i++)
+            if (!u32(out[i])) return false;
+        leave();
+        return true;
+    }
     bool vec_fp2(std::vector<uint64_t> &out) {
         if (!enter()) return false;
         uint64_t n;
@@ -103,6 +167,41 @@ static void put64(std::vector<uint8_t> &b, uint64_t v) {
 }
 static void put32(std::vector<uint8_t> &b, uint32_t v) {
     for (int i = 0; i < 4; i++) b.push_back((uint8_t)(v >> (8 * i)));
+}
+
+// Columns (Vec<Vec<Fp>>, column after column) then a MultiProof, as Round3Response and
+// MatrixColumns carry them.
+static Status put_columns_proof(std::vector<uint8_t> &b, const uint64_t *values, uint64_t columns,
+                                uint64_t symbols, const uint8_t *proof, uint64_t siblings) {
+    if ((columns && symbols && !values) || (siblings && !proof)) return Status::fail(TSR_BAD_ARG, "null argument");
+    put64(b, columns);
+    for (uint64_t c = 0; c < columns; c++) {
+        put64(b, symbols);
+        for (uint64_t i = 0; i < symbols; i++) {
+            const uint64_t v = values[c * symbols + i];
+            if (v >= gl::P) return Status::fail(TSR_BAD_ARG, "a symbol is not canonical");
+            put64(b, v);
+        }
+    }
+    put64(b, siblings);
+    b.insert(b.end(), proof, proof + 32 * siblings);
+    return Status::ok();
+}
+
+static tsr_status emit(const std::vector<uint8_t> &b, uint8_t *out, uint64_t cap, uint64_t *out_len) {
+    if (out_len) *out_len = b.size();
+    if (!out || cap < b.size())
+        return Status::fail(TSR_BAD_SHAPE, "the message needs " + std::to_string(b.size()) + " bytes").code;
+    memcpy(out, b.data(), b.size());
+    return TSR_OK;
+}
+
+static tsr_status decoded(Reader &r, bool ok, int *codec_error, uint64_t *error_offset) {
+    if (codec_error) *codec_error = r.err;
+    if (error_offset) *error_offset = r.err_at;
+    if (!ok)
+        return Status::fail(TSR_BAD_SHAPE, std::string(codec_name(r.err)) + " at offset " + std::to_string(r.err_at)).code;
+    return TSR_OK;
 }
 
 }  // namespace tsr
@@ -181,6 +280,139 @@ tsr_status tsr_sumcheck_proof_decode(const uint8_t *in, uint64_t len, const tsr_
         at += rounds[j].size() / 2;
     }
     if (!fin.empty()) memcpy(finals, fin.data(), 8 * fin.size());
+    return TSR_OK;
+}
+
+// Round3Response (crates/tessera-protocol/src/round3.rs:100-175): advice_columns, advice_proof,
+// weight_columns (Vec<MatrixColumns { matrix: u8, columns, proof }>).
+tsr_status tsr_round3_encode(const uint64_t *advice_values, uint64_t advice_columns,
+                             uint64_t advice_symbols, const uint8_t *advice_proof,
+                             uint64_t advice_siblings, const tsr_matrix_columns *weights,
+                             uint32_t n_weights, uint8_t *out, uint64_t cap, uint64_t *out_len) {
+    if (n_weights && !weights) return Status::fail(TSR_BAD_ARG, "null argument").code;
+    std::vector<uint8_t> b;
+    Status st = put_columns_proof(b, advice_values, advice_columns, advice_symbols, advice_proof, advice_siblings);
+    if (st) return st.code;
+    put64(b, n_weights);
+    for (uint32_t i = 0; i < n_weights; i++) {
+        const tsr_matrix_columns &w = weights[i];
+        if (w.matrix >= 10 || (i && w.matrix <= weights[i - 1].matrix))
+            return Status::fail(TSR_BAD_ARG, "matrix ids name a WeightMatrix (0..9) and strictly increase").code;
+        b.push_back(w.matrix);
+        if ((st = put_columns_proof(b, w.values, w.columns, w.symbols, w.proof, w.siblings))) return st.code;
+    }
+    return emit(b, out, cap, out_len);
+}
+
+tsr_status tsr_round3_decode(const uint8_t *in, uint64_t len, const tsr_decode_limits *limits,
+                             uint64_t *advice_columns, uint32_t *n_matrices, int *codec_error,
+                             uint64_t *error_offset) {
+    if ((len && !in) || !limits) return Status::fail(TSR_BAD_ARG, "null argument").code;
+    Reader r{in, len};
+    r.max_vec_len = limits->max_vec_len;
+    r.max_depth = limits->max_depth > 256 ? 256 : limits->max_depth;
+    std::vector<std::vector<uint64_t>> cols;
+    std::vector<uint8_t> sib;
+    uint64_t nw = 0;
+    bool ok = true;
+    if (len > limits->max_total_bytes) {
+        ok = r.fail(CODEC_LIMIT, 0);
+    } else if ((ok = r.enter())) {
+        ok = r.columns(cols) && r.multiproof(sib);
+        // weight_columns: MatrixColumns of at least u8 + two lengths
+        ok = ok && r.enter() && r.seq(nw, 1 + 8 + 8);
+        int prev = -1;
+        for (uint64_t i = 0; ok && i < nw; i++) {
+            std::vector<std::vector<uint64_t>> wc;
+            std::vector<uint8_t> ws;
+            uint8_t id = 0;
+            ok = r.enter() && r.matrix_id(id, prev) && r.columns(wc) && r.multiproof(ws);
+            if (ok) r.leave();
+            prev = id;
+        }
+        if (ok) r.leave();
+        if (ok) r.leave();
+        if (ok && r.at != len) ok = r.fail(CODEC_NONCANONICAL, r.at);
+    }
+    if (ok) {
+        if (advice_columns) *advice_columns = cols.size();
+        if (n_matrices) *n_matrices = (uint32_t)nw;
+    }
+    return decoded(r, ok, codec_error, error_offset);
+}
+
+// ColumnQueries (round3.rs:19-95): advice: Vec<u32>, weights: Vec<MatrixQueries { matrix: u8,
+// indices: Vec<u32> }>.
+tsr_status tsr_column_queries_encode(const uint32_t *advice, uint64_t n_advice, const uint8_t *matrices,
+                                     const uint64_t *counts, const uint32_t *indices, uint32_t n_matrices,
+                                     uint8_t *out, uint64_t cap, uint64_t *out_len) {
+    if ((n_advice && !advice) || (n_matrices && (!matrices || !counts)))
+        return Status::fail(TSR_BAD_ARG, "null argument").code;
+    std::vector<uint8_t> b;
+    put64(b, n_advice);
+    for (uint64_t i = 0; i < n_advice; i++) put32(b, advice[i]);
+    put64(b, n_matrices);
+    uint64_t at = 0;
+    for (uint32_t m = 0; m < n_matrices; m++) {
+        if (matrices[m] >= 10 || (m && matrices[m] <= matrices[m - 1]))
+            return Status::fail(TSR_BAD_ARG, "matrix ids name a WeightMatrix (0..9) and strictly increase").code;
+        if (counts[m] && !indices) return Status::fail(TSR_BAD_ARG, "null indices").code;
+        b.push_back(matrices[m]);
+        put64(b, counts[m]);
+        for (uint64_t i = 0; i < counts[m]; i++) put32(b, indices[at + i]);
+        at += counts[m];
+    }
+    return emit(b, out, cap, out_len);
+}
+
+tsr_status tsr_column_queries_decode(const uint8_t *in, uint64_t len, const tsr_decode_limits *limits,
+                                     uint32_t *advice, uint64_t advice_cap, uint64_t *n_advice,
+                                     uint8_t *matrices, uint64_t *counts, uint32_t matrices_cap,
+                                     uint32_t *n_matrices, uint32_t *indices, uint64_t indices_cap,
+                                     uint64_t *n_indices, int *codec_error, uint64_t *error_offset) {
+    if ((len && !in) || !limits) return Status::fail(TSR_BAD_ARG, "null argument").code;
+    Reader r{in, len};
+    r.max_vec_len = limits->max_vec_len;
+    r.max_depth = limits->max_depth > 256 ? 256 : limits->max_depth;
+    std::vector<uint32_t> adv, idx;
+    std::vector<uint8_t> ids;
+    std::vector<uint64_t> cnt;
+    bool ok = true;
+    if (len > limits->max_total_bytes) {
+        ok = r.fail(CODEC_LIMIT, 0);
+    } else if ((ok = r.enter())) {
+        uint64_t nw = 0;
+        ok = r.vec_u32(adv) && r.enter() && r.seq(nw, 1 + 8);
+        int prev = -1;
+        for (uint64_t i = 0; ok && i < nw; i++) {
+            uint8_t id = 0;
+            std::vector<uint32_t> q;
+            ok = r.enter() && r.matrix_id(id, prev) && r.vec_u32(q);
+            if (!ok) break;
+            r.leave();
+            prev = id;
+            ids.push_back(id);
+            cnt.push_back(q.size());
+            idx.insert(idx.end(), q.begin(), q.end());
+        }
+        if (ok) r.leave();
+        if (ok) r.leave();
+        if (ok && r.at != len) ok = r.fail(CODEC_NONCANONICAL, r.at);
+    }
+    tsr_status s = decoded(r, ok, codec_error, error_offset);
+    if (s) return s;
+    if (n_advice) *n_advice = adv.size();
+    if (n_matrices) *n_matrices = (uint32_t)ids.size();
+    if (n_indices) *n_indices = idx.size();
+    if (adv.size() > advice_cap || ids.size() > matrices_cap || idx.size() > indices_cap ||
+        (adv.size() && !advice) || (ids.size() && (!matrices || !counts)) || (idx.size() && !indices))
+        return Status::fail(TSR_BAD_SHAPE, "output capacity too small").code;
+    if (!adv.empty()) memcpy(advice, adv.data(), 4 * adv.size());
+    if (!ids.empty()) {
+        memcpy(matrices, ids.data(), ids.size());
+        memcpy(counts, cnt.data(), 8 * cnt.size());
+    }
+    if (!idx.empty()) memcpy(indices, idx.data(), 4 * idx.size());
     return TSR_OK;
 }
 

@@ -126,6 +126,13 @@ _SIGS = {
     "tsr_weight_commitment_free": (None, [vp]),
     "tsr_envelope_seal": (i32, [vp, ctypes.c_uint16, ctypes.c_uint16, vp, u64, vp, u64,
                                 ctypes.POINTER(u64)]),
+    "tsr_round3_encode": (i32, [vp, u64, u64, vp, u64, vp, u32, vp, u64, ctypes.POINTER(u64)]),
+    "tsr_round3_decode": (i32, [ctypes.c_char_p, u64, vp, ctypes.POINTER(u64), ctypes.POINTER(u32),
+                                ctypes.POINTER(i32), ctypes.POINTER(u64)]),
+    "tsr_column_queries_encode": (i32, [vp, u64, vp, vp, vp, u32, vp, u64, ctypes.POINTER(u64)]),
+    "tsr_column_queries_decode": (i32, [ctypes.c_char_p, u64, vp, vp, u64, ctypes.POINTER(u64), vp, vp, u32,
+                                        ctypes.POINTER(u32), vp, u64, ctypes.POINTER(u64), ctypes.POINTER(i32),
+                                        ctypes.POINTER(u64)]),
     "tsr_sumcheck_proof_encode": (i32, [vp, u32, u32, vp, u64, vp, u64, ctypes.POINTER(u64)]),
     "tsr_sumcheck_proof_decode": (i32, [vp, u64, vp, ctypes.POINTER(u32), vp, u32, vp, u64,
                                         ctypes.POINTER(u64), vp, u64, ctypes.POINTER(u64),

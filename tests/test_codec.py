@@ -158,3 +158,91 @@ def test_weight_matrix_commitment_ids():
             assert tce_.hex() == mat["descriptor"] and cid.hex() == mat["commitment_id"]
             n += 1
     assert n == 52
+
+
+# ---- Round3Response and ColumnQueries (crates/tessera-protocol/src/round3.rs:19-175) against
+# spec/vectors/protocol/protocol.json: the payload bytes of the recorded messages, every
+# truncation's code and offset, every refusal, and the accepted limits.
+
+PROTO = load("protocol.json")
+
+
+def _limits(d):
+    from tessera_amd.codec import DEFAULT_LIMITS, DecodeLimits
+    if not d:
+        return DEFAULT_LIMITS
+    return DecodeLimits(int(d["max_total_bytes"]), int(d["max_vec_len"]), int(d["max_depth"]))
+
+
+def _messages(typ):
+    return [m for m in PROTO["messages"] if m["type"] == typ]
+
+
+def test_round3_payloads_from_values():
+    from tessera_amd.codec import decode_round3, encode_round3
+    msgs = _messages("Round3Response")
+    assert msgs
+    for m in msgs:
+        v = m["value"]
+        adv = [[int(x) for x in col] for col in v["advice_columns"]]
+        sib = [bytes.fromhex(s) for s in v["advice_proof"]["siblings"]]
+        ws = [(int(w["matrix"]), [[int(x) for x in col] for col in w["columns"]],
+               [bytes.fromhex(s) for s in w["proof"]["siblings"]]) for w in v["weight_columns"]]
+        got = encode_round3(adv, sib, ws)
+        assert got.hex() == m["payload"], m["name"]
+        assert decode_round3(got) == (len(adv), len(ws))
+
+
+def test_column_queries_payloads_from_values():
+    from tessera_amd.codec import decode_column_queries, encode_column_queries
+    msgs = _messages("ColumnQueries")
+    assert len(msgs) == 2
+    for m in msgs:
+        v = m["value"]
+        adv = [int(x) for x in v["advice"]]
+        ws = [(int(w["matrix"]), [int(x) for x in w["indices"]]) for w in v["weights"]]
+        got = encode_column_queries(adv, ws)
+        assert got.hex() == m["payload"], m["name"]
+        assert decode_column_queries(got) == (adv, ws)
+
+
+def _decode(typ, data, limits):
+    from tessera_amd.codec import decode_column_queries, decode_round3
+    return (decode_round3 if typ == "Round3Response" else decode_column_queries)(data, limits)
+
+
+@pytest.mark.parametrize("typ", ["Round3Response", "ColumnQueries"])
+def test_truncations(typ):
+    from tessera_amd.codec import CodecError
+    for m in _messages(typ):
+        payload = bytes.fromhex(m["payload"])
+        for cut, offset in m["truncated"]:
+            with pytest.raises(CodecError) as e:
+                _decode(typ, payload[:cut], _limits(None))
+            assert (e.value.code, e.value.offset) == ("ERR-CODEC-TRUNCATED", offset), (m["name"], cut)
+
+
+@pytest.mark.parametrize("typ", ["Round3Response", "ColumnQueries"])
+def test_refused_and_accepted(typ):
+    from tessera_amd.codec import CodecError
+    refused = [r for r in PROTO["refused"] if r["type"] == typ]
+    assert len(refused) >= 9
+    for r in refused:
+        with pytest.raises(CodecError) as e:
+            _decode(typ, bytes.fromhex(r["hex"]), _limits(r.get("limits")))
+        assert (e.value.code, e.value.offset) == (r["code"], r["offset"]), r["why"]
+    accepted = [a for a in PROTO["accepted"] if a["type"] == typ]
+    assert accepted
+    for a in accepted:
+        _decode(typ, bytes.fromhex(a["hex"]), _limits(a.get("limits")))
+
+
+def test_round3_encoder_refuses_bad_input():
+    from tessera_amd.codec import encode_round3
+    from tessera_amd.lib import TsrError
+    with pytest.raises(TsrError):
+        encode_round3([[P]], [])  # a symbol that is not canonical
+    with pytest.raises(TsrError):
+        encode_round3([[1]], [], [(3, [[1]], []), (3, [[2]], [])])  # repeated matrix id
+    with pytest.raises(TsrError):
+        encode_round3([[1]], [], [(10, [[1]], [])])  # no such matrix

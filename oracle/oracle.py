@@ -11,7 +11,8 @@ import subprocess
 import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(HERE, "liboracle.so")
+# TSR_ORACLE_LIB: another build of the same sources (tests/sanitize.sh loads the ASan/UBSan one)
+LIB_PATH = os.environ.get("TSR_ORACLE_LIB") or os.path.join(HERE, "liboracle.so")
 P = 0xFFFFFFFF00000001
 
 _lib = None

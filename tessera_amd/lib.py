@@ -8,7 +8,9 @@ import os
 import re
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(HERE, "libtessera_mi355x.so")
+# TSR_LIB: another build of the same sources (tests/sanitize.sh loads the one whose host code is
+# built with ASan and UBSan)
+LIB_PATH = os.environ.get("TSR_LIB") or os.path.join(HERE, "libtessera_mi355x.so")
 HEADER = os.path.join(os.path.dirname(HERE), "include", "tessera_mi355x.h")
 
 OK, DECLINED, BAD_SHAPE, DEVICE, OOM, BAD_ARG = range(6)

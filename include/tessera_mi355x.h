@@ -31,7 +31,8 @@ typedef enum {
     TSR_BAD_SHAPE = 2, /* a length or parameter check failed; no side effects */
     TSR_DEVICE = 3,    /* a HIP runtime error */
     TSR_OOM = 4,       /* device memory exhausted or above the context's budget */
-    TSR_BAD_ARG = 5    /* null pointer or unusable argument */
+    TSR_BAD_ARG = 5,   /* null pointer or unusable argument */
+    TSR_CANCELLED = 6  /* a served request ended by tsr_server_cancel */
 } tsr_status;
 
 typedef struct tsr_ctx tsr_ctx;
@@ -310,8 +311,15 @@ typedef struct {
     double t_submit, t_admit, t_done; /* seconds on the server's clock */
 } tsr_result;
 tsr_status tsr_server_create(int device, const tsr_server_config *cfg, tsr_server **out);
-/* Queues a request (copied); returns at once. */
+/* Queues a request (copied); returns at once. The ids of requests in flight are distinct
+ * (BAD_ARG for a repeat). */
 tsr_status tsr_server_submit(tsr_server *srv, const tsr_request *req);
+/* Cancels a request that is waiting for admission, decoding, or decoded and waiting for a
+ * prover: its result arrives with status TSR_CANCELLED, and its slot, budget and device memory
+ * are released (a decoding request at the end of the decode launch in progress). DECLINED when
+ * the request is being proved, is done or was never submitted. The other requests' results are
+ * unchanged. */
+tsr_status tsr_server_cancel(tsr_server *srv, uint64_t id);
 /* Takes one finished result, waiting up to timeout_ms (DECLINED when none arrived). */
 tsr_status tsr_server_wait(tsr_server *srv, tsr_result *out, uint32_t timeout_ms);
 /* The most requests ever in flight together, and the footprint in flight now. */

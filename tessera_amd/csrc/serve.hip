@@ -29,7 +29,9 @@
 #include <condition_variable>
 #include <deque>
 #include <memory>
+#include <map>
 #include <mutex>
+#include <set>
 #include <stdint.h>
 #include <string.h>
 #include <string>
@@ -168,21 +170,16 @@ static Status prove_decoded(Ctx *ctx, const tsr_request &req, const ReqBufs &b, 
     std::vector<gl::fp2> r(n);
     for (uint32_t i = 0; i < n; i++) r[i] = tr.challenge_fp2(("serve/r/" + std::to_string(i)).c_str(), nullptr);
 
-    // tables: eq(r, .), layer 0's a and m of every token
+    // tables: eq(r, .) given by its point (never built), layer 0's a and m of every token
     const uint64_t len = 1ull << n;
     DevBuf<uint64_t> A, B;
-    DevBuf<gl::fp2> eq, d_r;
-    if ((st = A.alloc(ctx, len)) || (st = B.alloc(ctx, len)) || (st = eq.alloc(ctx, len)) ||
-        (st = d_r.alloc(ctx, std::max(n, 1u))))
-        return st;
-    if (n) TSR_HIP(hipMemcpyAsync(d_r.ptr, r.data(), 16 * n, hipMemcpyHostToDevice, s));
-    if ((st = eq_table(ctx, d_r.ptr, n, eq.ptr, s))) return st;
+    if ((st = A.alloc(ctx, len)) || (st = B.alloc(ctx, len))) return st;
     hipLaunchKernelGGL(k_gather_base, dim3((unsigned)((len + 255) / 256)), dim3(256), 0, s, b.wit.ptr,
                        total, WIT_A0, len, A.ptr);
     hipLaunchKernelGGL(k_gather_base, dim3((unsigned)((len + 255) / 256)), dim3(256), 0, s, b.wit.ptr,
                        total, WIT_M0, len, B.ptr);
     if ((st = check_launch("serve tables"))) return st;
-    tsr_table tabs[3] = {{(const uint64_t *)eq.ptr, 1, 1}, {A.ptr, 0, 0}, {B.ptr, 0, 0}};
+    tsr_table tabs[3] = {{(const uint64_t *)r.data(), TSR_TABLE_EQ, 1}, {A.ptr, 0, 0}, {B.ptr, 0, 0}};
     tsr_instance inst{n, 3, tabs};
     SumcheckOut so;
     std::vector<DevBuf<uint64_t>> uploads;
@@ -201,6 +198,9 @@ static Status prove_decoded(Ctx *ctx, const tsr_request &req, const ReqBufs &b, 
     out.final_a[1] = so.finals.size() > 0 ? so.finals[0].c1 : 0;
     return Status::ok();
 }
+
+// Where a request is, for tsr_server_cancel.
+enum JobState { JOB_QUEUED, JOB_DECODING, JOB_DECODED, JOB_PROVING };
 
 struct Job {
     tsr_request req;
@@ -231,6 +231,8 @@ struct tsr_server {
     std::deque<std::unique_ptr<tsr::Job>> queue;    // waiting for admission
     std::deque<std::unique_ptr<tsr::Job>> decoded;  // waiting for a prover
     std::deque<tsr_result> done;
+    std::map<uint64_t, int> state;  // JobState of every request in flight, by id
+    std::set<uint64_t> cancel;      // decoding requests to end after the current decode launch
     bool stop = false, decoder_done = false;
     uint32_t in_flight = 0, max_active = 0;
 };
@@ -251,6 +253,8 @@ static void finish(tsr_server *srv, Job &job, const tsr_result *filled) {
         std::lock_guard<std::mutex> lk(srv->mu);
         srv->in_flight_bytes -= job.foot;
         srv->in_flight--;
+        srv->state.erase(job.req.id);
+        srv->cancel.erase(job.req.id);
         srv->done.push_back(res);
     }
     srv->cv_done.notify_all();
@@ -282,6 +286,7 @@ static void decoder_loop(tsr_server *srv) {
                 srv->in_flight++;
                 srv->max_active = std::max(srv->max_active, srv->in_flight);
                 job->t_admit = now_s(srv->t0);
+                srv->state[job->req.id] = JOB_DECODING;
                 admitted.push_back(std::move(job));
             }
             if (active.empty() && admitted.empty() && srv->stop && srv->queue.empty()) break;
@@ -311,13 +316,19 @@ static void decoder_loop(tsr_server *srv) {
         for (size_t i = 0; i < active.size(); i++) {
             std::unique_ptr<Job> &job = active[i];
             job->pos = host_tasks[i].end;
-            if (ls) {
-                job->status = ls;
+            bool cancelled;
+            {
+                std::lock_guard<std::mutex> lk(srv->mu);
+                cancelled = srv->cancel.count(job->req.id) != 0;
+            }
+            if (ls || cancelled) {
+                job->status = ls ? ls : Status::fail(TSR_CANCELLED, "cancelled");
                 finish(srv, *job, nullptr);
             } else if (job->pos == job->bufs.total) {
                 job->bufs.kv = DevBuf<int32_t>();  // the KV cache is done with
                 {
                     std::lock_guard<std::mutex> lk(srv->mu);
+                    srv->state[job->req.id] = JOB_DECODED;
                     srv->decoded.push_back(std::move(job));
                 }
                 srv->cv_prove.notify_one();
@@ -345,6 +356,7 @@ static void prover_loop(tsr_server *srv, uint32_t slot) {
             if (srv->decoded.empty()) return;
             job = std::move(srv->decoded.front());
             srv->decoded.pop_front();
+            srv->state[job->req.id] = JOB_PROVING;
         }
         tsr_result res{};
         res.slot = slot;
@@ -415,10 +427,60 @@ tsr_status tsr_server_submit(tsr_server *srv, const tsr_request *req) {
     job->foot = request_footprint(req->n_prompt + req->n_output);
     {
         std::lock_guard<std::mutex> lk(srv->mu);
+        if (srv->state.count(req->id))
+            return Status::fail(TSR_BAD_ARG, "a request with id " + std::to_string(req->id) + " is in flight").code;
         job->t_submit = now_s(srv->t0);
+        srv->state[req->id] = JOB_QUEUED;
         srv->queue.push_back(std::move(job));
     }
     srv->cv_work.notify_all();
+    return TSR_OK;
+}
+
+tsr_status tsr_server_cancel(tsr_server *srv, uint64_t id) {
+    if (!srv) return Status::fail(TSR_BAD_ARG, "null server").code;
+    std::unique_ptr<Job> job;
+    {
+        std::lock_guard<std::mutex> lk(srv->mu);
+        auto it = srv->state.find(id);
+        if (it == srv->state.end())
+            return Status::fail(TSR_DECLINED, "no request " + std::to_string(id) + " in flight").code;
+        switch (it->second) {
+            case JOB_QUEUED: {
+                // never admitted: no slot, budget or buffer to give back
+                for (auto q = srv->queue.begin(); q != srv->queue.end(); ++q) {
+                    if ((*q)->req.id != id) continue;
+                    tsr_result res{};
+                    res.id = id;
+                    res.status = TSR_CANCELLED;
+                    res.t_submit = (*q)->t_submit;
+                    res.t_done = now_s(srv->t0);
+                    srv->queue.erase(q);
+                    srv->state.erase(it);
+                    srv->done.push_back(res);
+                    break;
+                }
+                srv->cv_done.notify_all();
+                return TSR_OK;
+            }
+            case JOB_DECODING:
+                srv->cancel.insert(id);  // the decoder ends it after its current launch
+                return TSR_OK;
+            case JOB_DECODED:
+                for (auto q = srv->decoded.begin(); q != srv->decoded.end(); ++q) {
+                    if ((*q)->req.id != id) continue;
+                    job = std::move(*q);
+                    srv->decoded.erase(q);
+                    break;
+                }
+                break;
+            default:
+                return Status::fail(TSR_DECLINED, "request " + std::to_string(id) + " is being proved").code;
+        }
+    }
+    if (!job) return Status::fail(TSR_DECLINED, "request " + std::to_string(id) + " is not waiting").code;
+    job->status = Status::fail(TSR_CANCELLED, "cancelled");
+    finish(srv, *job, nullptr);
     return TSR_OK;
 }
 

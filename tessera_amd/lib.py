@@ -13,8 +13,8 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.environ.get("TSR_LIB") or os.path.join(HERE, "libtessera_mi355x.so")
 HEADER = os.path.join(os.path.dirname(HERE), "include", "tessera_mi355x.h")
 
-OK, DECLINED, BAD_SHAPE, DEVICE, OOM, BAD_ARG = range(6)
-STATUS_NAMES = {0: "OK", 1: "DECLINED", 2: "BAD_SHAPE", 3: "DEVICE", 4: "OOM", 5: "BAD_ARG"}
+OK, DECLINED, BAD_SHAPE, DEVICE, OOM, BAD_ARG, CANCELLED = range(7)
+STATUS_NAMES = {0: "OK", 1: "DECLINED", 2: "BAD_SHAPE", 3: "DEVICE", 4: "OOM", 5: "BAD_ARG", 6: "CANCELLED"}
 
 
 class TsrError(RuntimeError):
@@ -157,6 +157,7 @@ _SIGS = {
     "tsr_butterfly_peak": (i32, [vp, ctypes.POINTER(ctypes.c_double)]),
     "tsr_server_create": (i32, [i32, vp, ctypes.POINTER(vp)]),
     "tsr_server_submit": (i32, [vp, vp]),
+    "tsr_server_cancel": (i32, [vp, u64]),
     "tsr_server_wait": (i32, [vp, vp, u32]),
     "tsr_server_stats": (i32, [vp, ctypes.POINTER(u32), ctypes.POINTER(u64)]),
     "tsr_server_prove": (i32, [vp, vp, vp]),

@@ -42,6 +42,8 @@ def _declare():
     L.tsr_server_wait.argtypes = [vp, ctypes.POINTER(Result), u32]
     L.tsr_server_prove.restype = ctypes.c_int
     L.tsr_server_prove.argtypes = [vp, ctypes.POINTER(Request), ctypes.POINTER(Result)]
+    L.tsr_server_cancel.restype = ctypes.c_int
+    L.tsr_server_cancel.argtypes = [vp, u64]
     L.tsr_server_stats.restype = ctypes.c_int
     L.tsr_server_stats.argtypes = [vp, ctypes.POINTER(u32), ctypes.POINTER(u64)]
     L.tsr_server_footprint.restype = u64
@@ -73,6 +75,16 @@ class Server:
     def submit(self, rid, prompt, n_output):
         req, p = self._request(rid, prompt, n_output)
         check(lib().tsr_server_submit(self.handle, ctypes.byref(req)))
+
+    def cancel(self, rid):
+        """True when the request was cancelled (its result arrives with status CANCELLED), False
+        when it is being proved or is no longer in flight."""
+        from .lib import DECLINED
+        rc = lib().tsr_server_cancel(self.handle, rid)
+        if rc == DECLINED:
+            return False
+        check(rc)
+        return True
 
     def wait(self, timeout_ms=600000):
         r = Result()

@@ -58,3 +58,42 @@ def test_admission_by_budget():
     res.sort(key=lambda r: r.t_admit)
     for a, b in zip(res, res[1:]):
         assert b.t_admit >= a.t_done - 1e-3
+
+
+def test_cancel_releases_slots_and_leaves_others_unchanged():
+    """Cancelled requests (one still waiting for admission behind a long one, others decoding or
+    waiting for the single prover) end with status CANCELLED and give back their slot, budget
+    and memory; the remaining requests' results equal their plain calls, and the server goes on
+    serving new requests."""
+    from tessera_amd.lib import CANCELLED
+    from tessera_amd.serve import Server, footprint
+    rng = np.random.default_rng(9)
+    reqs = [(100 + i, rng.integers(0, 4096, 4).tolist(), n) for i, n in enumerate([400, 60, 300, 30, 200, 50])]
+    with Server(slots=2, provers=1, chunk=2, hbm_budget=4 * footprint(512), model_seed=SEED) as srv:
+        alone = {rid: srv.prove(rid, p, n).key() for rid, p, n in reqs}
+        for rid, p, n in reqs:
+            srv.submit(rid, p, n)
+        # the last request waits behind the first two (two slots): cancelled while queued
+        assert srv.cancel(105)
+        cancelled = {105}
+        for rid in (100, 102):  # decoding, or already further along
+            if srv.cancel(rid):
+                cancelled.add(rid)
+        assert not srv.cancel(999)  # never submitted
+        res = {}
+        for _ in reqs:
+            r = srv.wait()
+            res[r.id] = r
+        assert set(res) == {rid for rid, _, _ in reqs}
+        for rid, r in res.items():
+            if rid in cancelled:
+                assert r.status == CANCELLED, rid
+            else:
+                assert r.status == 0 and r.key() == alone[rid], rid
+        max_active, in_flight = srv.stats()
+        assert in_flight == 0 and max_active <= 2
+        assert not srv.cancel(101)  # done
+        # the freed slots serve a new request
+        srv.submit(200, [5, 6], 20)
+        r = srv.wait()
+        assert r.status == 0 and r.id == 200 and r.n_tokens == 22

@@ -33,7 +33,7 @@ static_assert(DOFF_W2 + (uint64_t)D_MODEL * D_FF == LAYER_BYTES, "device layer l
 // K transposed, [layer][64 words of 4 int8][tcap] (word w of position u at (l * 64 + w) * tcap + u);
 // V packed by 4 positions, [layer][tcap / 4][256] (v(u, i) in byte u % 4 of word
 // (l * tcap / 4 + u / 4) * 256 + i).
-inline uint32_t kv_tcap(uint32_t total) { return (total + 3) & ~3u; }
+inline __host__ __device__ uint32_t kv_tcap(uint32_t total) { return (total + 3) & ~3u; }
 inline uint64_t kv_words(uint32_t total) {
     return (uint64_t)N_LAYERS * (64ull * kv_tcap(total) + (uint64_t)kv_tcap(total) / 4 * D_MODEL);
 }

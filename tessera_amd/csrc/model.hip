@@ -19,8 +19,9 @@
 // crates/tessera-field/src/goldilocks.rs:374: x >= 0 -> x, x < 0 -> p - |x|), WIT_STRIDE values:
 // per layer a, q, k, v, att, m (256 each), h (1024), x (256); then f (256) and the next token.
 //
-// Decoding: one workgroup of 1024 threads per request; a launch advances every request of a
-// task table by its own range of positions, so a serving loop can let requests join and leave
+// Decoding: one workgroup of 1024 threads per group of R requests (R = DECODE_GROUP), whose
+// matrix products read each weight word once for the group; a launch advances every request of
+// a task table by its own range of positions, so a serving loop can let requests join and leave
 // between launches (iteration-level batching) without changing any request's arithmetic.
 // Weight reads are packed (model.h) so each wavefront load is 256 contiguous bytes; the 7.3 MB
 // of weights are shared by all requests and stay in the L2 / last-level cache.
@@ -86,203 +87,280 @@ __global__ void k_model_weights(int8_t *__restrict__ w, uint64_t n, uint64_t see
     if (b < n) w[b] = weight_at(seed, logical_index(b));
 }
 
-// part[s * O + o] = sum over the s-th 1/S of the input words of w(o, .) . act, for a packed
-// [I4][O] matrix and an activation of I4 packed words in LDS. O * S is a multiple of MT.
-template <int O, int I4, int S>
-__device__ __forceinline__ void matvec_partials(const int32_t *__restrict__ Wp, const int32_t *act,
-                                                int32_t *part, int tid) {
+// part[j][s * O + o] = sum over the s-th 1/S of the input words of w(o, .) . act[j], for a packed
+// [I4][O] matrix and R activations of I4 packed words in LDS: each weight word is read once for
+// the R requests of the workgroup. O * S is a multiple of MT.
+template <int R, int O, int I4, int S>
+__device__ __forceinline__ void matvec_partials(const int32_t *__restrict__ Wp, const int32_t (*act)[D_MODEL],
+                                                int32_t (*part)[4096], int tid) {
     constexpr int WPS = I4 / S;
-    static_assert((O * S) % MT == 0 && I4 % S == 0, "matvec split");
+    static_assert((O * S) % MT == 0 && I4 % S == 0 && O * S <= 4096, "matvec split");
 #pragma unroll 1
     for (int idx = tid; idx < O * S; idx += MT) {
         const int o = idx % O, s = idx / O;
         const int32_t *col = Wp + (uint64_t)(s * WPS) * O + o;
-        const int32_t *a = act + s * WPS;
-        int32_t acc = 0;
+        int32_t acc[R];
+#pragma unroll
+        for (int j = 0; j < R; j++) acc[j] = 0;
 #pragma unroll 16
-        for (int w = 0; w < WPS; w++) acc = __builtin_amdgcn_sdot4(col[(uint64_t)w * O], a[w], acc, false);
-        part[idx] = acc;
+        for (int w = 0; w < WPS; w++) {
+            const int32_t word = col[(uint64_t)w * O];
+#pragma unroll
+            for (int j = 0; j < R; j++) acc[j] = __builtin_amdgcn_sdot4(word, act[j][s * WPS + w], acc[j], false);
+        }
+#pragma unroll
+        for (int j = 0; j < R; j++) part[j][idx] = acc[j];
     }
 }
 
+// (value, index) pairs: the larger value wins, the lower index on equal values (the first maximum).
+__device__ __forceinline__ void argmax_merge(int32_t &v, uint32_t &i, int32_t ov, uint32_t oi) {
+    if (ov > v || (ov == v && oi < i)) {
+        v = ov;
+        i = oi;
+    }
+}
+
+// One workgroup advances the R tasks tasks[R b .. R b + R) (those below `count`), each by its own
+// range of positions, in lockstep: step s is position start + s of every task that has one. The
+// weight reads of every matrix product are shared by the R tasks; attention, the KV caches and the
+// witness are each task's own. A task's arithmetic is the same whatever R and its neighbours are.
+template <int R>
 __global__ void __launch_bounds__(MT)
-k_decode(const int8_t *__restrict__ W, const DecodeTask *__restrict__ tasks) {
-    const DecodeTask task = tasks[blockIdx.x];
+k_decode(const int8_t *__restrict__ W, const DecodeTask *__restrict__ tasks, uint32_t count) {
     const int tid = threadIdx.x;
-    const uint32_t tcap = (task.total + 3) & ~3u;
     const int32_t *embed = (const int32_t *)(W + MODEL_OFF_EMBED);
     const int32_t *wout = (const int32_t *)(W + MODEL_OFF_OUT);
-    int32_t *kt = task.kv;                                     // [L][64][tcap]
-    int32_t *vp = task.kv + (uint64_t)N_LAYERS * 64 * tcap;    // [L][tcap / 4][256]
 
-    __shared__ int32_t x[D_MODEL];
-    __shared__ int32_t a_pk[D_MODEL / 4], q_pk[D_MODEL / 4], k_pk[D_MODEL / 4], att_pk[D_MODEL / 4];
-    __shared__ int32_t h_pk[D_FF / 4];
-    __shared__ int32_t part[4096], partn[MT];
-    __shared__ int32_t scores[N_HEADS][MAX_TOKENS];
-    __shared__ int32_t red_v[MT];
-    __shared__ uint32_t red_i[MT];
-    __shared__ uint32_t cur;
-    int8_t *a8 = (int8_t *)a_pk, *q8 = (int8_t *)q_pk, *k8 = (int8_t *)k_pk, *att8 = (int8_t *)att_pk,
-           *h8 = (int8_t *)h_pk;
+    __shared__ int32_t x[R][D_MODEL];
+    // packed int8 activations: a (also m and f), q, k, att of D_MODEL / 4 words; h of D_FF / 4
+    __shared__ int32_t a_pk[R][D_MODEL], q_pk[R][D_MODEL / 4], k_pk[R][D_MODEL / 4], att_pk[R][D_MODEL];
+    __shared__ int32_t h_pk[R][D_MODEL];
+    __shared__ int32_t part[R][4096], partn[MT];
+    __shared__ int32_t scores[R][N_HEADS][MAX_TOKENS];
+    __shared__ int32_t red_v[R][MT / 64];
+    __shared__ uint32_t red_i[R][MT / 64];
+    __shared__ uint32_t cur[R];
 
-    if (tid == 0)
-        cur = task.start < task.n_prompt ? task.prompt[task.start]
-                                         : (uint32_t)task.wit[(uint64_t)(task.start - 1) * WIT_STRIDE + WIT_NEXT];
+    // the tasks and each step's facts live in LDS (uniform reads), not in every lane's registers
+    __shared__ DecodeTask task[R];
+    __shared__ uint32_t steps_s, t[R];
+    __shared__ int on[R];
+    if (tid < R) {
+        const uint32_t at = blockIdx.x * R + tid;
+        DecodeTask tk = tasks[at < count ? at : 0];
+        if (at >= count) tk.end = tk.start;  // no position: never on
+        task[tid] = tk;
+        cur[tid] = tk.end == tk.start ? 0
+                   : tk.start < tk.n_prompt ? tk.prompt[tk.start]
+                                            : (uint32_t)tk.wit[(uint64_t)(tk.start - 1) * WIT_STRIDE + WIT_NEXT];
+    }
     __syncthreads();
-    for (uint32_t t = task.start; t < task.end; t++) {
-        const uint32_t tok = cur;
-        uint64_t *wt = task.wit + (uint64_t)t * WIT_STRIDE;
-        if (tid == 0) task.tokens[t] = tok;
-        if (tid < D_MODEL / 4) {
-            const int32_t e = embed[(uint64_t)tok * (D_MODEL / 4) + tid];
-            for (int b = 0; b < 4; b++) x[4 * tid + b] = (int8_t)(e >> (8 * b));
+    if (tid == 0) {
+        uint32_t m = 0;
+        for (int j = 0; j < R; j++) m = task[j].end - task[j].start > m ? task[j].end - task[j].start : m;
+        steps_s = m;
+    }
+    __syncthreads();
+    const uint32_t steps = steps_s;
+    for (uint32_t st = 0; st < steps; st++) {
+        if (tid < R) {
+            t[tid] = task[tid].start + st;
+            on[tid] = t[tid] < task[tid].end;
+        }
+        __syncthreads();
+        auto wit_at = [&](int j) { return task[j].wit + (uint64_t)t[j] * WIT_STRIDE; };
+        for (int e = tid; e < R * (D_MODEL / 4); e += MT) {
+            const int j = e / (D_MODEL / 4), w = e % (D_MODEL / 4);
+            if (!on[j]) continue;
+            const int32_t v = embed[(uint64_t)cur[j] * (D_MODEL / 4) + w];
+            for (int b = 0; b < 4; b++) x[j][4 * w + b] = (int8_t)(v >> (8 * b));
+            if (w == 0) task[j].tokens[t[j]] = cur[j];
         }
         __syncthreads();
         for (int l = 0; l < N_LAYERS; l++) {
             const int8_t *Lw = W + MODEL_OFF_LAYER + (uint64_t)l * LAYER_BYTES;
-            uint64_t *wl = wt + (uint64_t)l * WIT_LAYER;
-            int32_t *ktl = kt + (uint64_t)l * 64 * tcap;
-            int32_t *vpl = vp + (uint64_t)l * (tcap / 4) * D_MODEL;
             // a = rq(x)
-            if (tid < D_MODEL) {
-                const int32_t av = clamp127(x[tid] >> 2);
-                a8[tid] = (int8_t)av;
-                wl[tid] = from_i64(av);
+            for (int e = tid; e < R * D_MODEL; e += MT) {
+                const int j = e / D_MODEL, i = e % D_MODEL;
+                if (!on[j]) continue;
+                const int32_t av = clamp127(x[j][i] >> 2);
+                ((int8_t *)a_pk[j])[i] = (int8_t)av;
+                wit_at(j)[(uint64_t)l * WIT_LAYER + i] = from_i64(av);
             }
             __syncthreads();
             // q, k, v
-            matvec_partials<3 * D_MODEL, D_MODEL / 4, 4>((const int32_t *)(Lw + DOFF_QKV), a_pk, part, tid);
+            matvec_partials<R, 3 * D_MODEL, D_MODEL / 4, 4>((const int32_t *)(Lw + DOFF_QKV), a_pk, part, tid);
             __syncthreads();
-            if (tid < 3 * D_MODEL) {
-                const int32_t y = part[tid] + part[768 + tid] + part[1536 + tid] + part[2304 + tid];
+            for (int e = tid; e < R * 3 * D_MODEL; e += MT) {
+                const int j = e / (3 * D_MODEL), o = e % (3 * D_MODEL);
+                if (!on[j]) continue;
+                const int32_t y = part[j][o] + part[j][768 + o] + part[j][1536 + o] + part[j][2304 + o];
                 const int32_t v = clamp127(y >> 8);
-                wl[D_MODEL + tid] = from_i64(v);  // q, k, v are consecutive in the witness
-                if (tid < D_MODEL) {
-                    q8[tid] = (int8_t)v;
-                } else if (tid < 2 * D_MODEL) {
-                    k8[tid - D_MODEL] = (int8_t)v;
+                wit_at(j)[(uint64_t)l * WIT_LAYER + D_MODEL + o] = from_i64(v);  // q, k, v in a row
+                if (o < D_MODEL) {
+                    ((int8_t *)q_pk[j])[o] = (int8_t)v;
+                } else if (o < 2 * D_MODEL) {
+                    ((int8_t *)k_pk[j])[o - D_MODEL] = (int8_t)v;
                 } else {
-                    const int i = tid - 2 * D_MODEL;
-                    ((int8_t *)vpl)[((uint64_t)(t / 4) * D_MODEL + i) * 4 + (t & 3)] = (int8_t)v;
+                    const uint32_t tcap = kv_tcap(task[j].total);
+                    int32_t *vpl = task[j].kv + (uint64_t)N_LAYERS * 64 * tcap + (uint64_t)l * (tcap / 4) * D_MODEL;
+                    const int i = o - 2 * D_MODEL;
+                    ((int8_t *)vpl)[((uint64_t)(t[j] / 4) * D_MODEL + i) * 4 + (t[j] & 3)] = (int8_t)v;
                 }
             }
             __syncthreads();
-            if (tid < 64) ktl[(uint64_t)tid * tcap + t] = k_pk[tid];
+            for (int e = tid; e < R * 64; e += MT) {
+                const int j = e / 64, w = e % 64;
+                if (!on[j]) continue;
+                const uint32_t tcap = kv_tcap(task[j].total);
+                task[j].kv[((uint64_t)l * 64 + w) * tcap + t[j]] = k_pk[j][w];
+            }
             __threadfence_block();
             __syncthreads();
             // scores s_u = relu((q . k_u) >> 6), u <= t, per head; zeros up to a multiple of 4
-            const uint32_t n = t + 1, n4 = (n + 3) & ~3u;
-            for (uint32_t e = tid; e < N_HEADS * n; e += MT) {
-                const uint32_t hd = e / n, u = e % n;
-                const int32_t *kc = ktl + (uint64_t)(hd * 16) * tcap + u;
-                int32_t acc = 0;
 #pragma unroll
-                for (int w = 0; w < 16; w++) acc = __builtin_amdgcn_sdot4(q_pk[hd * 16 + w], kc[(uint64_t)w * tcap], acc, false);
-                const int32_t s = acc >> 6;
-                scores[hd][u] = s > 0 ? s : 0;
-            }
-            if (tid < 16) {
-                const uint32_t u = n + (tid & 3);
-                if (u < n4) scores[tid >> 2][u] = 0;
+            for (int j = 0; j < R; j++) {
+                if (!on[j]) continue;
+                const uint32_t tcap = kv_tcap(task[j].total), n = t[j] + 1, n4 = (n + 3) & ~3u;
+                const int32_t *ktl = task[j].kv + (uint64_t)l * 64 * tcap;
+                for (uint32_t e = tid; e < N_HEADS * n; e += MT) {
+                    const uint32_t hd = e / n, u = e % n;
+                    const int32_t *kc = ktl + (uint64_t)(hd * 16) * tcap + u;
+                    int32_t acc = 0;
+#pragma unroll
+                    for (int w = 0; w < 16; w++) acc = __builtin_amdgcn_sdot4(q_pk[j][hd * 16 + w], kc[(uint64_t)w * tcap], acc, false);
+                    const int32_t sc = acc >> 6;
+                    scores[j][hd][u] = sc > 0 ? sc : 0;
+                }
+                if (tid < 16) {
+                    const uint32_t u = n + (tid & 3);
+                    if (u < n4) scores[j][tid >> 2][u] = 0;
+                }
             }
             __syncthreads();
-            // o_i = (sum_u s_u v_u(i)) / (1 + sum_u s_u): element i = tid % 256, quarter s = tid / 256
-            {
-                const int i = tid & (D_MODEL - 1), s = tid >> 8, hd = i / D_HEAD;
+            // o_i = (sum_u s_u v_u(i)) / (1 + sum_u s_u): element i = tid % 256, quarter q = tid / 256
+#pragma unroll
+            for (int j = 0; j < R; j++) {
+                if (!on[j]) continue;
+                const uint32_t tcap = kv_tcap(task[j].total), n4 = (t[j] + 1 + 3) & ~3u;
+                const int32_t *vpl = task[j].kv + (uint64_t)N_LAYERS * 64 * tcap + (uint64_t)l * (tcap / 4) * D_MODEL;
+                const int i = tid & (D_MODEL - 1), q = tid >> 8, hd = i / D_HEAD;
                 const int32_t *vc = vpl + i;
                 int32_t num = 0, nsum = 0;
-                for (uint32_t wd = s; wd < n4 / 4; wd += 4) {
+                for (uint32_t wd = q; wd < n4 / 4; wd += 4) {
                     const int32_t word = vc[(uint64_t)wd * D_MODEL];
 #pragma unroll
                     for (int b = 0; b < 4; b++) {
-                        const int32_t sc = scores[hd][4 * wd + b];
+                        const int32_t sc = scores[j][hd][4 * wd + b];
                         num += sc * (int32_t)(int8_t)(word >> (8 * b));
                         nsum += sc;
                     }
                 }
-                part[tid] = num;
+                part[0][tid] = num;
                 partn[tid] = nsum;
+                __syncthreads();
+                if (tid < D_MODEL) {
+                    const int32_t nm = part[0][tid] + part[0][256 + tid] + part[0][512 + tid] + part[0][768 + tid];
+                    const int32_t norm = 1 + partn[tid] + partn[256 + tid] + partn[512 + tid] + partn[768 + tid];
+                    const int32_t o = clamp127(nm / norm);
+                    ((int8_t *)att_pk[j])[tid] = (int8_t)o;
+                    wit_at(j)[(uint64_t)l * WIT_LAYER + 4 * D_MODEL + tid] = from_i64(o);
+                }
+                __syncthreads();
             }
-            __syncthreads();
-            if (tid < D_MODEL) {
-                const int32_t num = part[tid] + part[256 + tid] + part[512 + tid] + part[768 + tid];
-                const int32_t norm = 1 + partn[tid] + partn[256 + tid] + partn[512 + tid] + partn[768 + tid];
-                const int32_t o = clamp127(num / norm);
-                att8[tid] = (int8_t)o;
-                wl[4 * D_MODEL + tid] = from_i64(o);
-            }
-            __syncthreads();
             // x += Wo att; m = rq(x)
-            matvec_partials<D_MODEL, D_MODEL / 4, 4>((const int32_t *)(Lw + DOFF_O), att_pk, part, tid);
+            matvec_partials<R, D_MODEL, D_MODEL / 4, 4>((const int32_t *)(Lw + DOFF_O), att_pk, part, tid);
             __syncthreads();
-            if (tid < D_MODEL) {
-                x[tid] += part[tid] + part[256 + tid] + part[512 + tid] + part[768 + tid];
-                const int32_t mv = clamp127(x[tid] >> 2);
-                a8[tid] = (int8_t)mv;
-                wl[5 * D_MODEL + tid] = from_i64(mv);
+            for (int e = tid; e < R * D_MODEL; e += MT) {
+                const int j = e / D_MODEL, i = e % D_MODEL;
+                if (!on[j]) continue;
+                x[j][i] += part[j][i] + part[j][256 + i] + part[j][512 + i] + part[j][768 + i];
+                const int32_t mv = clamp127(x[j][i] >> 2);
+                ((int8_t *)a_pk[j])[i] = (int8_t)mv;
+                wit_at(j)[(uint64_t)l * WIT_LAYER + 5 * D_MODEL + i] = from_i64(mv);
             }
             __syncthreads();
             // h = clamp(relu((W1 m) >> 8), 0, 127)
-            matvec_partials<D_FF, D_MODEL / 4, 4>((const int32_t *)(Lw + DOFF_W1), a_pk, part, tid);
+            matvec_partials<R, D_FF, D_MODEL / 4, 4>((const int32_t *)(Lw + DOFF_W1), a_pk, part, tid);
             __syncthreads();
-            {
-                int32_t hv = (part[tid] + part[1024 + tid] + part[2048 + tid] + part[3072 + tid]) >> 8;
+            for (int e = tid; e < R * D_FF; e += MT) {
+                const int j = e / D_FF, i = e % D_FF;
+                if (!on[j]) continue;
+                int32_t hv = (part[j][i] + part[j][1024 + i] + part[j][2048 + i] + part[j][3072 + i]) >> 8;
                 hv = hv < 0 ? 0 : (hv > 127 ? 127 : hv);
-                h8[tid] = (int8_t)hv;
-                wl[6 * D_MODEL + tid] = from_i64(hv);
+                ((int8_t *)h_pk[j])[i] = (int8_t)hv;
+                wit_at(j)[(uint64_t)l * WIT_LAYER + 6 * D_MODEL + i] = from_i64(hv);
             }
             __syncthreads();
             // x += (W2 h) >> 8
-            matvec_partials<D_MODEL, D_FF / 4, 16>((const int32_t *)(Lw + DOFF_W2), h_pk, part, tid);
+            matvec_partials<R, D_MODEL, D_FF / 4, 16>((const int32_t *)(Lw + DOFF_W2), h_pk, part, tid);
             __syncthreads();
-            if (tid < D_MODEL) {
+            for (int e = tid; e < R * D_MODEL; e += MT) {
+                const int j = e / D_MODEL, i = e % D_MODEL;
+                if (!on[j]) continue;
                 int32_t y = 0;
 #pragma unroll
-                for (int s = 0; s < 16; s++) y += part[s * 256 + tid];
-                x[tid] += y >> 8;
-                wl[6 * D_MODEL + D_FF + tid] = from_i64(x[tid]);
+                for (int q = 0; q < 16; q++) y += part[j][q * 256 + i];
+                x[j][i] += y >> 8;
+                wit_at(j)[(uint64_t)l * WIT_LAYER + 6 * D_MODEL + D_FF + i] = from_i64(x[j][i]);
             }
             __syncthreads();
         }
         // f = rq(x); logits and the greedy choice
-        if (tid < D_MODEL) {
-            const int32_t fv = clamp127(x[tid] >> 2);
-            a8[tid] = (int8_t)fv;
-            wt[N_LAYERS * WIT_LAYER + tid] = from_i64(fv);
+        for (int e = tid; e < R * D_MODEL; e += MT) {
+            const int j = e / D_MODEL, i = e % D_MODEL;
+            if (!on[j]) continue;
+            const int32_t fv = clamp127(x[j][i] >> 2);
+            ((int8_t *)a_pk[j])[i] = (int8_t)fv;
+            wit_at(j)[N_LAYERS * WIT_LAYER + i] = from_i64(fv);
         }
         __syncthreads();
-        int32_t best = INT32_MIN;
-        uint32_t best_i = 0;
+        int32_t best[R];
+        uint32_t best_i[R];
+#pragma unroll
+        for (int j = 0; j < R; j++) {
+            best[j] = INT32_MIN;
+            best_i[j] = 0;
+        }
 #pragma unroll 1
-        for (int j = tid; j < VOCAB; j += MT) {  // j increases: the first maximum is kept
-            const int32_t *col = wout + j;
-            int32_t acc = 0;
+        for (int v = tid; v < VOCAB; v += MT) {  // v increases: a thread keeps its first maximum
+            const int32_t *col = wout + v;
+            int32_t acc[R];
+#pragma unroll
+            for (int j = 0; j < R; j++) acc[j] = 0;
 #pragma unroll 16
-            for (int w = 0; w < D_MODEL / 4; w++) acc = __builtin_amdgcn_sdot4(col[(uint64_t)w * VOCAB], a_pk[w], acc, false);
-            if (acc > best) {
-                best = acc;
-                best_i = j;
+            for (int w = 0; w < D_MODEL / 4; w++) {
+                const int32_t word = col[(uint64_t)w * VOCAB];
+#pragma unroll
+                for (int j = 0; j < R; j++) acc[j] = __builtin_amdgcn_sdot4(word, a_pk[j][w], acc[j], false);
             }
-        }
-        red_v[tid] = best;
-        red_i[tid] = best_i;
-        __syncthreads();
-        for (int s = MT / 2; s > 0; s >>= 1) {
-            if (tid < s) {
-                const int32_t ov = red_v[tid + s];
-                const uint32_t oi = red_i[tid + s];
-                if (ov > red_v[tid] || (ov == red_v[tid] && oi < red_i[tid])) {
-                    red_v[tid] = ov;
-                    red_i[tid] = oi;
+#pragma unroll
+            for (int j = 0; j < R; j++)
+                if (acc[j] > best[j]) {
+                    best[j] = acc[j];
+                    best_i[j] = v;
                 }
-            }
-            __syncthreads();
         }
-        if (tid == 0) {
-            const uint32_t next = red_i[0];
-            wt[WIT_NEXT] = next;
-            cur = (t + 1 < task.n_prompt) ? task.prompt[t + 1] : next;
+        const int lane = tid & 63, wave = tid >> 6;
+#pragma unroll
+        for (int j = 0; j < R; j++) {
+            int32_t bv = best[j];
+            uint32_t bi = best_i[j];
+            for (int m = 32; m >= 1; m >>= 1) argmax_merge(bv, bi, __shfl_xor(bv, m, 64), __shfl_xor(bi, m, 64));
+            if (lane == 0) {
+                red_v[j][wave] = bv;
+                red_i[j][wave] = bi;
+            }
+        }
+        __syncthreads();
+        if (tid < R && on[tid]) {
+            const int j = tid;
+            int32_t bv = red_v[j][0];
+            uint32_t bi = red_i[j][0];
+            for (int w = 1; w < MT / 64; w++) argmax_merge(bv, bi, red_v[j][w], red_i[j][w]);
+            wit_at(j)[WIT_NEXT] = bi;
+            cur[j] = (t[j] + 1 < task[j].n_prompt) ? task[j].prompt[t[j] + 1] : bi;
         }
         __syncthreads();
     }
@@ -294,11 +372,20 @@ Status model_weights(Ctx *ctx, int8_t *w, uint64_t seed, hipStream_t s) {
     return check_launch("model weights");
 }
 
+// Requests per decode workgroup. One: measured on the served workload (256 requests, up to 73
+// in flight), four requests per workgroup took a request-token from 69 to 110 us and served
+// capacity from 2014 to 591 proofs/s. A token's time is the chain of dependent weight loads
+// from the last-level cache and the barriers between the steps, which grow with the requests a
+// workgroup carries, while the shared weight reads saved little; and four per workgroup leave
+// three quarters of the CUs idle when fewer requests than CUs are decoding.
+constexpr uint32_t DECODE_GROUP = 1;
+
 Status model_decode(Ctx *ctx, const int8_t *w, const DecodeTask *tasks, uint32_t count,
                     hipStream_t s) {
     if (count == 0) return Status::ok();
     KernelTimer timer(ctx, "k_decode", s);
-    hipLaunchKernelGGL(k_decode, dim3(count), dim3(MT), 0, s, w, tasks);
+    hipLaunchKernelGGL(k_decode<DECODE_GROUP>, dim3((count + DECODE_GROUP - 1) / DECODE_GROUP), dim3(MT), 0, s,
+                       w, tasks, count);
     return check_launch("decode");
 }
 

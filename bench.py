@@ -251,8 +251,11 @@ def run_sumcheck(reps=3):
 
 def run_batched(reps=3):
     """64 matrices (the batched-commit workload, seed 0xBA7C), uniform entries generated on the
-    device, rate 1/4, B = 32: one CommitBatch (plan, autotune, graph capture at creation; each
-    run replays the graph) vs 64 sequential commit_unmasked calls; every root compared."""
+    device, rate 1/4, B = 32: one CommitBatch (plan, autotune, graph capture at creation, shape
+    groups as parallel graph branches; each run replays the graph) vs 64 sequential
+    commit_unmasked calls; every root compared. Also: a second batch of the same shapes created
+    from the context's cached tuning, the serial variant (one stream), and each matrix's own
+    sequential time by log2 k, which bounds what any batching can gain."""
     import tessera_amd as T
     from tessera_amd.batch import CommitBatch
     from tests.helpers import batched_shapes
@@ -263,33 +266,63 @@ def run_batched(reps=3):
     cid = bytes([7] * 32)
     mats = [(buf, lk, 2, 32, cid) for buf, (lk, n) in zip(bufs, shapes)]
     t0 = time.perf_counter()
-    batch = CommitBatch(dev, mats, autotune=True, graph=True)
+    first = CommitBatch(dev, mats, autotune=True, graph=True)
+    t_create_first = time.perf_counter() - t0
+    tuning = first.tuning()
+    first.free()
+    t0 = time.perf_counter()
+    batch = CommitBatch(dev, mats, autotune=True, graph=True)  # the tuning comes from the cache
     t_create = time.perf_counter() - t0
     batch.run()
     dev.synchronize()
     s_batch = timed(lambda: (batch.run(), dev.synchronize()), reps)
     roots_b = batch.roots()
-    tuning = batch.tuning()
     batch.free()
+    serial = CommitBatch(dev, mats, autotune=True, graph=True, serial=True)
+    serial.run()
+    s_serial = timed(lambda: (serial.run(), dev.synchronize()), reps)
+    roots_serial = serial.roots()
+    serial.free()
+
+    def one(buf, lk):
+        c = T.ReferenceCommitment.commit_unmasked(dev, T.CommitmentParamsPy(lk, 2, 0, 32, cid), buf)
+        r = c.root()
+        c.free()
+        return r
 
     def sequential():
-        out = []
-        for buf, lk, _, _, _ in mats:
-            c = T.ReferenceCommitment.commit_unmasked(dev, T.CommitmentParamsPy(lk, 2, 0, 32, cid), buf)
-            out.append(c.root())
-            c.free()
-        return out
+        return [one(buf, lk) for buf, lk, _, _, _ in mats]
 
     roots_s = sequential()
     s_seq = timed(sequential, reps)
+    # each matrix alone, by log2 k: where the sequential time goes
+    by_k = {}
+    for (buf, lk, _, _, _), (_, n) in zip(mats, shapes):
+        sec = timed(lambda: one(buf, lk), 3)
+        e = by_k.setdefault(lk, {"matrices": 0, "elements": 0, "ms": 0.0})
+        e["matrices"] += 1
+        e["elements"] += n << lk
+        e["ms"] += 1e3 * sec
+    per_matrix_ms = sum(e["ms"] for e in by_k.values())
+    small_ms = sum(e["ms"] for lk, e in by_k.items() if lk <= 15)
+    large = {lk: e for lk, e in by_k.items() if lk > 15}
+    large_rate = sum(e["elements"] for e in large.values()) / (1e-3 * sum(e["ms"] for e in large.values()))
     dev.close()
     return {"matrices": len(shapes), "elements": total,
             "batched_ms": round(1e3 * s_batch, 3), "sequential_ms": round(1e3 * s_seq, 3),
+            "batched_serial_ms": round(1e3 * s_serial, 3),
             "batched_elems_per_s": round(total / s_batch, 1),
             "sequential_elems_per_s": round(total / s_seq, 1),
-            "speedup": round(s_seq / s_batch, 3), "roots_equal": roots_b == roots_s,
-            "create_ms": round(1e3 * t_create, 1),
-            "tiles": sorted({t[2] for t in tuning}) if tuning else []}
+            "speedup": round(s_seq / s_batch, 3),
+            "roots_equal": roots_b == roots_s and roots_serial == roots_s,
+            "create_ms_first": round(1e3 * t_create_first, 1), "create_ms": round(1e3 * t_create, 1),
+            "plans": [list(t) for t in tuning],
+            "by_log2_k": {str(lk): {k: round(v, 3) if isinstance(v, float) else v for k, v in e.items()}
+                          for lk, e in sorted(by_k.items())},
+            "bound": {"log2_k_le_15_ms": round(small_ms, 3), "per_matrix_total_ms": round(per_matrix_ms, 3),
+                      "large_k_elems_per_s": round(large_rate, 1),
+                      "max_speedup_if_small_k_were_free": round(per_matrix_ms / (per_matrix_ms - small_ms), 3),
+                      "max_speedup_at_large_k_rate": round(per_matrix_ms / (1e3 * total / large_rate), 3)}}
 
 
 def run_served(cpu_sample):
@@ -721,7 +754,7 @@ def main():
     ap.add_argument("--traffic-command", default="", help="the profiling command, for the summary")
     ap.add_argument("--trace-summary", nargs=3, metavar=("DB", "TITLE", "COMMAND"),
                     help="print a markdown summary of a rocprofv3 kernel-trace database")
-    ap.add_argument("--only", choices=["headline", "copy", "hostfed", "refshapes", "sumcheck"], default=None,
+    ap.add_argument("--only", choices=["headline", "copy", "hostfed", "refshapes", "sumcheck", "batched"], default=None,
                     help="time the headline commit only; run the device copy that calibrates "
                          "the HBM byte counters (4 copies of 2 GiB); time the headline commit "
                          "from host rows (the PCIe-inclusive rate); or time the reference's three "
@@ -745,6 +778,9 @@ def main():
         return
     if a.only == "hostfed":
         run_hostfed()
+        return
+    if a.only == "batched":
+        print(json.dumps({"workload": "batched commit", **run_batched()}), flush=True)
         return
     if a.only == "sumcheck":
         print(json.dumps({"workload": "sumcheck/lookup configuration", **run_sumcheck()}), flush=True)

@@ -258,8 +258,14 @@ typedef struct {
     uint64_t count;                 /* rows of the matrix (may be 0) */
 } tsr_commit_desc;
 enum {
-    TSR_BATCH_AUTOTUNE = 1, /* time the tile-size candidates of each shape group on its rows */
-    TSR_BATCH_GRAPH = 2     /* capture the launch sequence once into a hipGraph and replay it */
+    TSR_BATCH_AUTOTUNE = 1, /* time each shape group's plans (LDS tile 2^13 or 2^12, strided passes
+                               of at most the default or fewer stage bits) on its rows; the choice
+                               is kept in the context per (log2 k, r, log2 rows) for later batches */
+    TSR_BATCH_GRAPH = 2,    /* capture the launch sequence once into a hipGraph and replay it */
+    TSR_BATCH_SERIAL = 4,   /* every shape group on the context's stream, one after another (by
+                               default each group's encode is a parallel branch) */
+    TSR_BATCH_PLAN = 8      /* every group uses the plan in bits 8..15 (tile_log, 12 or 13) and
+                               16..23 (most stage bits per strided pass, 0: the default) of flags */
 };
 /* Plans a batch: device buffers for every matrix's codewords and tree, the shape groups, the
  * block tables. Host rows are uploaded once; device rows are used in place on every run. */
@@ -269,8 +275,10 @@ tsr_status tsr_commit_batch_create(tsr_ctx *ctx, const tsr_commit_desc *descs, u
 tsr_status tsr_commit_batch_run(tsr_batch *b);
 /* The count roots, 32 bytes each, in descriptor order, of the last run. */
 tsr_status tsr_commit_batch_roots(const tsr_batch *b, uint8_t *roots);
-/* The autotuner's choice per shape group: (log2_k, r, tile_log) triples, `cap` groups at most. */
-tsr_status tsr_commit_batch_tuning(const tsr_batch *b, uint32_t *tiles, uint32_t cap,
+/* The plan per shape group, five words each, `cap` groups at most: log2_k, r, tile_log, the most
+ * stage bits of a strided pass (0: the default), and the tuned plan's encode time in
+ * microseconds (0 when not timed by this batch: no autotune, or a plan from the context's cache). */
+tsr_status tsr_commit_batch_tuning(const tsr_batch *b, uint32_t *plans, uint32_t cap,
                                    uint32_t *n_groups);
 void tsr_commit_batch_free(tsr_batch *b);
 

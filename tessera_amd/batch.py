@@ -7,7 +7,7 @@ import numpy as np
 from .device import DeviceBuffer
 from .lib import check, lib, u32, u64, vp
 
-AUTOTUNE, GRAPH = 1, 2
+AUTOTUNE, GRAPH, SERIAL, PLAN = 1, 2, 4, 8
 
 
 class CommitDesc(ctypes.Structure):
@@ -19,7 +19,8 @@ class CommitBatch:
     """matrices: list of (rows, log2_k, log2_inv_rate, row_block, commitment_id); rows a numpy
     array or a DeviceBuffer of count x 2^log2_k values."""
 
-    def __init__(self, device, matrices, autotune=True, graph=True):
+    def __init__(self, device, matrices, autotune=True, graph=True, serial=False, plan=None):
+        """plan: (tile_log, most stage bits per strided pass) for every group, instead of tuning."""
         L = lib()
         L.tsr_commit_batch_create.restype = ctypes.c_int
         L.tsr_commit_batch_create.argtypes = [vp, ctypes.POINTER(CommitDesc), u32, u32, ctypes.POINTER(vp)]
@@ -49,7 +50,9 @@ class CommitBatch:
             descs[i].rows = ptr
             descs[i].count = n_vals >> log_k
         h = vp()
-        flags = (AUTOTUNE if autotune else 0) | (GRAPH if graph else 0)
+        flags = (AUTOTUNE if autotune else 0) | (GRAPH if graph else 0) | (SERIAL if serial else 0)
+        if plan is not None:
+            flags |= PLAN | (plan[0] << 8) | (plan[1] << 16)
         check(L.tsr_commit_batch_create(device.handle, descs, self.count, flags, ctypes.byref(h)))
         self.handle = h.value
 
@@ -63,10 +66,12 @@ class CommitBatch:
         return [raw[32 * i:32 * i + 32] for i in range(self.count)]
 
     def tuning(self):
-        buf = np.zeros(3 * 64, dtype=np.uint32)
+        """Per shape group: (log2_k, r, tile_log, most stage bits per strided pass (0: default),
+        tuned encode time in us (0: not timed by this batch))."""
+        buf = np.zeros(5 * 64, dtype=np.uint32)
         n = u32()
         check(lib().tsr_commit_batch_tuning(self.handle, buf.ctypes.data, 64, ctypes.byref(n)))
-        return [tuple(int(x) for x in buf[3 * g:3 * g + 3]) for g in range(min(n.value, 64))]
+        return [tuple(int(x) for x in buf[5 * g:5 * g + 5]) for g in range(min(n.value, 64))]
 
     def free(self):
         if self.handle:

@@ -32,11 +32,15 @@ struct tsr_batch {
     tsr::DevBuf<tsr::MatDesc> mats;
     // per (log k, rate) group
     struct Group {
-        uint32_t log_k, log_r, tile_log;
+        uint32_t log_k, log_r, tile_log, max_nb = 0;
         uint64_t rows;
+        double ms = 0;  // the chosen plan's time when tuned
         tsr::DevBuf<uint64_t> src, tmp, cw;  // row pointer tables
         tsr::Encoder *enc = nullptr;
+        hipStream_t side = nullptr;  // the group's branch (parallel batches)
+        hipEvent_t done = nullptr;
     };
+    hipEvent_t fork = nullptr;
     std::vector<std::unique_ptr<Group>> groups;
     tsr::DevBuf<tsr::BlockRef> chain_refs;
     uint32_t chain_blocks = 0;
@@ -54,14 +58,24 @@ static Status upload_ptrs(Ctx *ctx, DevBuf<uint64_t> &buf, const std::vector<uin
     return buf.upload(ctx, v);
 }
 
-// Enqueues the whole batch on the stream (no synchronization).
+// Enqueues the whole batch on the stream (no synchronization). The shape groups' encodes are
+// independent: unless the batch is serial, each runs on a branch of its own (a side stream
+// forked from and joined back to `s`, so a captured graph holds them as parallel branches), and
+// small-k groups overlap the large ones. Chains and Merkle levels follow on `s`.
 static Status enqueue(tsr_batch *b, hipStream_t s) {
+    const bool par = !(b->flags & TSR_BATCH_SERIAL) && b->groups.size() > 1;
+    if (par) TSR_HIP(hipEventRecord(b->fork, s));
     for (auto &g : b->groups) {
+        hipStream_t gs = par ? g->side : s;
+        if (par) TSR_HIP(hipStreamWaitEvent(gs, b->fork, 0));
         Status st = g->enc->encode_rowmap((const uint64_t *const *)g->src.ptr,
                                           (uint64_t *const *)g->tmp.ptr,
-                                          (uint64_t *const *)g->cw.ptr, g->rows, s);
+                                          (uint64_t *const *)g->cw.ptr, g->rows, gs);
         if (st) return st;
+        if (par) TSR_HIP(hipEventRecord(g->done, gs));
     }
+    if (par)
+        for (auto &g : b->groups) TSR_HIP(hipStreamWaitEvent(s, g->done, 0));
     launch_chain_leaves_multi(b->ctx, b->mats.ptr, b->chain_refs.ptr, b->chain_blocks, s);
     for (size_t l = 0; l < b->level_refs.size(); l++)
         launch_merkle_level_multi(b->ctx, b->mats.ptr, b->level_refs[l].ptr, b->level_blocks[l],
@@ -78,7 +92,7 @@ static Status time_group(tsr_batch *b, tsr_batch::Group &g, double *ms) {
     Status st = g.enc->encode_rowmap((const uint64_t *const *)g.src.ptr, (uint64_t *const *)g.tmp.ptr,
                                      (uint64_t *const *)g.cw.ptr, g.rows, s);  // warm-up
     TSR_HIP(hipEventRecord(e0, s));
-    for (int i = 0; i < 3 && !st; i++)
+    for (int i = 0; i < 2 && !st; i++)
         st = g.enc->encode_rowmap((const uint64_t *const *)g.src.ptr, (uint64_t *const *)g.tmp.ptr,
                                   (uint64_t *const *)g.cw.ptr, g.rows, s);
     TSR_HIP(hipEventRecord(e1, s));
@@ -87,7 +101,7 @@ static Status time_group(tsr_batch *b, tsr_batch::Group &g, double *ms) {
     TSR_HIP(hipEventElapsedTime(&t, e0, e1));
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
-    *ms = t / 3;
+    *ms = t / 2;
     return st;
 }
 
@@ -178,24 +192,46 @@ tsr_status tsr_commit_batch_create(tsr_ctx *ctx_, const tsr_commit_desc *descs, 
             return st.code;
         g->tile_log = 13;
         if ((st = get_encoder(ctx, g->log_k, g->log_r, 0, &g->enc, 13))) return st.code;
-        if (flags & TSR_BATCH_AUTOTUNE) {
-            // candidates: LDS tiles of 2^13 and 2^12 elements (different pass splits)
-            double best = 1e30;
-            uint32_t best_tile = 13;
-            for (uint32_t tile : {13u, 12u}) {
-                Encoder *enc = nullptr;
-                if ((st = get_encoder(ctx, g->log_k, g->log_r, 0, &enc, tile))) return st.code;
-                g->enc = enc;
-                double ms = 0;
-                if ((st = time_group(b.get(), *g, &ms))) return st.code;
-                b->tune_ms.push_back(ms);
-                if (ms < best) {
-                    best = ms;
-                    best_tile = tile;
+        if (flags & TSR_BATCH_PLAN) {
+            g->tile_log = (flags >> 8) & 0xFF;
+            g->max_nb = (flags >> 16) & 0xFF;
+            if (g->tile_log != 12 && g->tile_log != 13)
+                return Status::fail(TSR_BAD_ARG, "a plan's tile_log is 12 or 13").code;
+            if ((st = get_encoder(ctx, g->log_k, g->log_r, 0, &g->enc, g->tile_log, g->max_nb))) return st.code;
+        } else if (flags & TSR_BATCH_AUTOTUNE) {
+            // plans: the LDS tile (2^13 or 2^12 elements) and the most stage bits of a strided
+            // pass (the pass split), timed on the group's own rows; the choice is kept in the
+            // context for later batches of the same shape and height
+            const auto key = std::make_tuple(g->log_k, g->log_r, (uint32_t)(63 - __builtin_clzll(g->rows)));
+            auto hit = ctx->batch_tunings.find(key);
+            if (hit != ctx->batch_tunings.end()) {
+                g->tile_log = hit->second.first;
+                g->max_nb = hit->second.second;
+            } else {
+                double best = 1e30;
+                const std::pair<uint32_t, uint32_t> plans[] = {{13, 0}, {13, 7}, {12, 0}, {12, 6}};
+                for (const auto &pl : plans) {
+                    Encoder *enc = nullptr;
+                    if ((st = get_encoder(ctx, g->log_k, g->log_r, 0, &enc, pl.first, pl.second))) return st.code;
+                    g->enc = enc;
+                    double ms = 0;
+                    if ((st = time_group(b.get(), *g, &ms))) return st.code;
+                    b->tune_ms.push_back(ms);
+                    if (ms < best) {
+                        best = ms;
+                        g->tile_log = pl.first;
+                        g->max_nb = pl.second;
+                        g->ms = ms;
+                    }
                 }
+                ctx->batch_tunings[key] = {g->tile_log, g->max_nb};
             }
-            g->tile_log = best_tile;
-            if ((st = get_encoder(ctx, g->log_k, g->log_r, 0, &g->enc, best_tile))) return st.code;
+            if ((st = get_encoder(ctx, g->log_k, g->log_r, 0, &g->enc, g->tile_log, g->max_nb))) return st.code;
+        }
+        if (!(flags & TSR_BATCH_SERIAL)) {
+            e = hipStreamCreateWithFlags(&g->side, hipStreamNonBlocking);
+            if (e == hipSuccess) e = hipEventCreateWithFlags(&g->done, hipEventDisableTiming);
+            if (e != hipSuccess) return hip_fail(e, "branch stream").code;
         }
         b->groups.push_back(std::move(g));
     }
@@ -231,6 +267,10 @@ tsr_status tsr_commit_batch_create(tsr_ctx *ctx_, const tsr_commit_desc *descs, 
         }
     }
     if ((st = b->roots.alloc(ctx, 8 * count))) return st.code;
+    if (!(flags & TSR_BATCH_SERIAL)) {
+        e = hipEventCreateWithFlags(&b->fork, hipEventDisableTiming);
+        if (e != hipSuccess) return hip_fail(e, "fork event").code;
+    }
     e = hipStreamSynchronize(s);
     if (e != hipSuccess) return hip_fail(e, "sync").code;
     *out = b.release();
@@ -288,14 +328,17 @@ tsr_status tsr_commit_batch_roots(const tsr_batch *b, uint8_t *roots) {
     return TSR_OK;
 }
 
-tsr_status tsr_commit_batch_tuning(const tsr_batch *b, uint32_t *tiles, uint32_t cap,
+tsr_status tsr_commit_batch_tuning(const tsr_batch *b, uint32_t *plans, uint32_t cap,
                                    uint32_t *n_groups) {
     if (!b) return Status::fail(TSR_BAD_ARG, "null batch").code;
     if (n_groups) *n_groups = (uint32_t)b->groups.size();
-    for (uint32_t g = 0; g < b->groups.size() && g < cap && tiles; g++) {
-        tiles[3 * g] = b->groups[g]->log_k;
-        tiles[3 * g + 1] = b->groups[g]->log_r;
-        tiles[3 * g + 2] = b->groups[g]->tile_log;
+    for (uint32_t g = 0; g < b->groups.size() && g < cap && plans; g++) {
+        const tsr_batch::Group &gr = *b->groups[g];
+        plans[5 * g] = gr.log_k;
+        plans[5 * g + 1] = gr.log_r;
+        plans[5 * g + 2] = gr.tile_log;
+        plans[5 * g + 3] = gr.max_nb;
+        plans[5 * g + 4] = (uint32_t)(gr.ms * 1000.0);  // the tuned plan's encode time (us)
     }
     return TSR_OK;
 }
@@ -306,6 +349,14 @@ void tsr_commit_batch_free(tsr_batch *b) {
     (void)hipStreamSynchronize(b->ctx->stream);
     if (b->exec) (void)hipGraphExecDestroy(b->exec);
     if (b->graph) (void)hipGraphDestroy(b->graph);
+    for (auto &g : b->groups) {
+        if (g->side) {
+            (void)hipStreamSynchronize(g->side);
+            (void)hipStreamDestroy(g->side);
+        }
+        if (g->done) (void)hipEventDestroy(g->done);
+    }
+    if (b->fork) (void)hipEventDestroy(b->fork);
     delete b;
 }
 

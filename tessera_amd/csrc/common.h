@@ -102,6 +102,9 @@ struct Ctx {
     std::multimap<uint64_t, void *> pool;
     std::mutex pool_mu;  // dev_alloc / dev_free may come from several threads (served proving)
     std::map<void *, uint64_t> user_allocs;  // tsr_alloc pointers and the bytes charged for them
+    // batched commitments: the tuned (tile_log, max pass bits) per (log k, rate, log2 rows),
+    // so that a later batch of the same shapes skips the tuner
+    std::map<std::tuple<uint32_t, uint32_t, uint32_t>, std::pair<uint32_t, uint32_t>> batch_tunings;
     std::map<std::tuple<uint32_t, uint32_t, uint64_t, uint32_t>, std::shared_ptr<Encoder>> encoders;
     uint32_t keys[19][8];  // BLAKE3 context keys of the reference's 19 domains
     bool profile = false;

@@ -15,8 +15,10 @@ struct Encoder {
     std::vector<std::pair<uint32_t, uint32_t>> ct_passes, gs_passes;
     DevBuf<uint64_t> zk_inv, zn_fwd, spread_tab, pow7_tab;
 
-    // tile_log: log2 of the LDS tile (u64 elements) every pass works on, at most 13 (64 KiB)
-    Status init(Ctx *ctx, uint32_t log_k, uint32_t log_r, uint64_t P, uint32_t tile_log = 13);
+    // tile_log: log2 of the LDS tile (u64 elements) every pass works on, at most 13 (64 KiB);
+    // max_nb: the most stage bits of one strided pass (0: 10 with tile_log 13, else tile_log - 4)
+    Status init(Ctx *ctx, uint32_t log_k, uint32_t log_r, uint64_t P, uint32_t tile_log = 13,
+                uint32_t max_nb = 0);
     // u64 elements of scratch encode() needs for `count` rows.
     uint64_t scratch_elems(uint64_t count) const;
     // All pointers are device pointers; launches on `stream`, does not synchronize.
@@ -35,6 +37,6 @@ struct Encoder {
 
 // The context's cached encoder for (log_k, r, P): tables are built and uploaded once.
 Status get_encoder(Ctx *ctx, uint32_t log_k, uint32_t log_r, uint64_t P, Encoder **out,
-                   uint32_t tile_log = 13);
+                   uint32_t tile_log = 13, uint32_t max_nb = 0);
 
 }  // namespace tsr

@@ -451,7 +451,8 @@ static std::vector<std::pair<uint32_t, uint32_t>> split_bits(uint32_t lo, uint32
     return passes;
 }
 
-Status Encoder::init(Ctx *ctx, uint32_t log_k, uint32_t log_r, uint64_t P, uint32_t tile_log) {
+Status Encoder::init(Ctx *ctx, uint32_t log_k, uint32_t log_r, uint64_t P, uint32_t tile_log,
+                     uint32_t max_nb_in) {
     this->ctx = ctx;
     if (tile_log < log_r + 1) tile_log = log_r + 1;
     if (tile_log > TILE_MAX_LOG) tile_log = TILE_MAX_LOG;
@@ -464,7 +465,8 @@ Status Encoder::init(Ctx *ctx, uint32_t log_k, uint32_t log_r, uint64_t P, uint3
     tk = log_k < tile_log - log_r ? log_k : tile_log - log_r;
     // CT passes over k-bits [tk, log_k), run from the top bits down; up to 10 bits per pass
     // with 64 KiB tiles (tile_log 13), else tile_log - 4
-    const uint32_t max_nb = tile_log >= 13 ? WIDE_PASS_NB : tile_log - 4;
+    const uint32_t max_nb_default = tile_log >= 13 ? WIDE_PASS_NB : tile_log - 4;
+    const uint32_t max_nb = max_nb_in && max_nb_in < max_nb_default ? max_nb_in : max_nb_default;
     auto ct = split_bits(tk, log_k, max_nb);
     ct_passes.assign(ct.rbegin(), ct.rend());
     gs_passes = split_bits(tk + log_r, log_n, max_nb);
@@ -554,15 +556,15 @@ void Encoder::launch(const uint64_t *src, uint64_t src_stride, const uint64_t *c
 }
 
 Status get_encoder(Ctx *ctx, uint32_t log_k, uint32_t log_r, uint64_t P, Encoder **out,
-                   uint32_t tile_log) {
-    auto key = std::make_tuple(log_k, log_r, P, tile_log);
+                   uint32_t tile_log, uint32_t max_nb) {
+    auto key = std::make_tuple(log_k, log_r, P, tile_log | (max_nb << 8));
     auto it = ctx->encoders.find(key);
     if (it != ctx->encoders.end()) {
         *out = it->second.get();
         return Status::ok();
     }
     std::shared_ptr<Encoder> e = std::make_shared<Encoder>();
-    Status st = e->init(ctx, log_k, log_r, P, tile_log);
+    Status st = e->init(ctx, log_k, log_r, P, tile_log, max_nb);
     if (st) return st;
     *out = e.get();
     ctx->encoders.emplace(key, std::move(e));

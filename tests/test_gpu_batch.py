@@ -27,17 +27,34 @@ def make_batch(shapes, seed):
     return mats
 
 
-@pytest.mark.parametrize("autotune,graph", [(False, False), (True, False), (True, True)])
-def test_batch_roots_match_oracle(dev, autotune, graph):
+@pytest.mark.parametrize("autotune,graph,serial", [(False, False, False), (True, False, False), (True, True, False),
+                                                   (False, False, True), (True, True, True)])
+def test_batch_roots_match_oracle(dev, autotune, graph, serial):
     from tessera_amd.batch import CommitBatch
     shapes = batched_shapes(12, seed=7, log_k=(1, 13), cols=(1, 70)) + [(0, 3), (12, 0)]
     mats = make_batch(shapes, 100)
-    b = CommitBatch(dev, mats, autotune=autotune, graph=graph)
+    b = CommitBatch(dev, mats, autotune=autotune, graph=graph, serial=serial)
     b.run()
     got = b.roots()
     for (rows, lk, lr, blk, cid), root in zip(mats, got):
         _, nodes = O.commit_unmasked(rows, lk, lr, blk, cid, threads=8)
         assert root == nodes[-32:], (lk, rows.size >> lk)
+    b.free()
+
+
+@pytest.mark.parametrize("plan", [(13, 0), (13, 7), (12, 0), (12, 6), (13, 3)])
+def test_every_plan_matches_oracle(dev, plan):
+    """Each plan the tuner chooses among (the LDS tile and the pass split) gives the oracle's
+    roots, on shapes whose passes it splits differently (log2 k up to 18)."""
+    from tessera_amd.batch import CommitBatch
+    shapes = [(18, 3), (16, 5), (14, 9), (11, 17), (6, 4)]
+    mats = make_batch(shapes, 300)
+    b = CommitBatch(dev, mats, autotune=False, graph=True, plan=plan)
+    b.run()
+    for (rows, lk, lr, blk, cid), root in zip(mats, b.roots()):
+        _, nodes = O.commit_unmasked(rows, lk, lr, blk, cid, threads=8)
+        assert root == nodes[-32:], (plan, lk)
+    assert {p[2:4] for p in b.tuning()} == {plan}
     b.free()
 
 

@@ -73,6 +73,19 @@ TSR_HD uint64_t mul(uint64_t a, uint64_t b) {
 #endif
 }
 
+// x * 2^E for E in {24, 48, 72}: the 8th and 4th roots of unity are +-2^24, +-2^72 and +-2^48
+// (2^96 = -1), so the radix-8 rounds' constant twiddles are shifts. E < 64: the 128-bit product
+// is (x << E, x >> (64 - E)). E = 72: x 2^72 = (x << 8 mod 2^64) 2^64 + (x >> 56) 2^128, and
+// 2^128 = -2^32 (mod p).
+template <int E>
+TSR_HD uint64_t mul_pow2(uint64_t x) {
+    static_assert(E == 24 || E == 48 || E == 72, "a root-of-unity shift");
+    if constexpr (E < 64)
+        return reduce128(x << E, x >> (64 - E));
+    else
+        return sub(reduce128(0, x << (E - 64)), (x >> (128 - E)) << 32);
+}
+
 TSR_HD uint64_t sqr(uint64_t a) { return mul(a, a); }
 
 inline __host__ __device__ uint64_t pow(uint64_t a, uint64_t e) {

@@ -14,6 +14,7 @@ struct Encoder {
     // (lo, nb) stage-bit ranges of the strided passes, in launch order.
     std::vector<std::pair<uint32_t, uint32_t>> ct_passes, gs_passes;
     DevBuf<uint64_t> zk_inv, zn_fwd, spread_tab, pow7_tab;
+    DevBuf<uint64_t> vk_inv, vn_fwd;  // the V powers of zk_inv and zn_fwd for the radix-8 rounds
 
     // tile_log: log2 of the LDS tile (u64 elements) every pass works on, at most 13 (64 KiB);
     // max_nb: the most stage bits of one strided pass (0: 10 with tile_log 13, else tile_log - 4)

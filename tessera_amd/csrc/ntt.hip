@@ -151,9 +151,156 @@ __device__ __forceinline__ void round_regs(uint64_t *x, uint32_t base, uint64_t 
     }
 }
 
-template <bool GS, int R, int NB, int LOGC, int S0, bool TOP = false, int THREADS = TILE_THREADS>
+// ---- radix-8 rounds: a constant-root network and seven twiddle products --------------------
+// Three stages at global bits b, b+1, b+2 over x[j] = element base + j 2^b use the twiddles
+// Z[4G + u] = V w8^{rev2(u)} (u < 4), Z[2G + u] = V^2 w4^u (u < 2) and Z[G] = V^4, with
+// G = p >> (b + 3), V = Z[4G] and w8, w4 the table's 8th and 4th roots of unity (rev over the
+// table's bits puts u's bits on top). Pulling V out, the GS round is the 8-point network with
+// the twiddles w8^{rev2(u)}, w4^u and 1 followed by x[m] *= V^m; the CT round (the transpose)
+// is x[m] *= V^m followed by the same network run top stage first. The network's twiddles are
+// +-2^24, +-2^48, +-2^72 (2^96 = -1): shifts, not products. A forward table has w8 = 2^120 =
+// -2^24, w4 = 2^48, w8^3 = -2^72; an inverse one w8 = 2^72, w4 = -2^48, w8^3 = 2^24. Twelve
+// general products per eight values become seven plus five shifts (none at all in a pass's top
+// round, where G = 0 and V = 1), the same field values either way. The seven powers of V come
+// from a table of eight words per G (vpow_table), read as one 64-byte row.
+
+// The network twiddle K (1..3: w4, w8, w8^3; w4 is also the middle stage's) as (shift, sign).
+template <bool INV, int K>
+struct RootShift;
+template <> struct RootShift<false, 1> { static constexpr int E = 48; static constexpr bool NEG = false; };
+template <> struct RootShift<false, 2> { static constexpr int E = 24; static constexpr bool NEG = true; };
+template <> struct RootShift<false, 3> { static constexpr int E = 72; static constexpr bool NEG = true; };
+template <> struct RootShift<true, 1> { static constexpr int E = 48; static constexpr bool NEG = true; };
+template <> struct RootShift<true, 2> { static constexpr int E = 72; static constexpr bool NEG = false; };
+template <> struct RootShift<true, 3> { static constexpr int E = 24; static constexpr bool NEG = false; };
+
+// GS butterfly (u + v, (u - v) c) with network twiddle K; ZU / ZV: u / v known to be zero.
+template <bool INV, int K, bool ZU, bool ZV>
+__device__ __forceinline__ void gs_pair(uint64_t &a, uint64_t &b) {
+    if constexpr (ZU && ZV) {
+        return;
+    } else if constexpr (K == 0) {
+        const uint64_t u = a, v = b;
+        if constexpr (ZV) {
+            b = u;
+        } else if constexpr (ZU) {
+            a = v;
+            b = gl::neg(v);
+        } else {
+            a = gl::add(u, v);
+            b = gl::sub(u, v);
+        }
+    } else {
+        using S = RootShift<INV, K>;
+        const uint64_t u = a, v = b;
+        if constexpr (ZV) {  // (u, +-u 2^E)
+            const uint64_t t = gl::mul_pow2<S::E>(u);
+            b = S::NEG ? gl::neg(t) : t;
+        } else if constexpr (ZU) {  // (v, -+v 2^E)
+            const uint64_t t = gl::mul_pow2<S::E>(v);
+            a = v;
+            b = S::NEG ? t : gl::neg(t);
+        } else {
+            a = gl::add(u, v);
+            b = gl::mul_pow2<S::E>(S::NEG ? gl::sub(v, u) : gl::sub(u, v));
+        }
+    }
+}
+
+// CT butterfly (u + c v, u - c v) with network twiddle K.
+template <bool INV, int K>
+__device__ __forceinline__ void ct_pair(uint64_t &a, uint64_t &b) {
+    const uint64_t u = a;
+    if constexpr (K == 0) {
+        const uint64_t t = b;
+        a = gl::add(u, t);
+        b = gl::sub(u, t);
+    } else {
+        using S = RootShift<INV, K>;
+        const uint64_t t = gl::mul_pow2<S::E>(b);
+        a = S::NEG ? gl::sub(u, t) : gl::add(u, t);
+        b = S::NEG ? gl::add(u, t) : gl::sub(u, t);
+    }
+}
+
+// Zero flags after a stage at local bit t: a pair stays zero only if both of its values were.
+constexpr uint32_t zero_after(uint32_t z, int t) {
+    uint32_t out = 0;
+    for (int j = 0; j < 8; j++) {
+        if (j & (1 << t)) continue;
+        const int k = j + (1 << t);
+        if (((z >> j) & 1) && ((z >> k) & 1)) out |= (1u << j) | (1u << k);
+    }
+    return out;
+}
+constexpr bool zbit(uint32_t z, int j) { return ((z >> j) & 1) != 0; }
+
+// The GS network; Z0: the inputs known to be zero (bit j for x[j]).
+template <bool INV, uint32_t Z0>
+__device__ __forceinline__ void gs8_network(uint64_t *x) {
+    constexpr uint32_t Z1 = zero_after(Z0, 0), Z2 = zero_after(Z1, 1);
+    gs_pair<INV, 0, zbit(Z0, 0), zbit(Z0, 1)>(x[0], x[1]);
+    gs_pair<INV, 1, zbit(Z0, 2), zbit(Z0, 3)>(x[2], x[3]);
+    gs_pair<INV, 2, zbit(Z0, 4), zbit(Z0, 5)>(x[4], x[5]);
+    gs_pair<INV, 3, zbit(Z0, 6), zbit(Z0, 7)>(x[6], x[7]);
+    gs_pair<INV, 0, zbit(Z1, 0), zbit(Z1, 2)>(x[0], x[2]);
+    gs_pair<INV, 0, zbit(Z1, 1), zbit(Z1, 3)>(x[1], x[3]);
+    gs_pair<INV, 1, zbit(Z1, 4), zbit(Z1, 6)>(x[4], x[6]);
+    gs_pair<INV, 1, zbit(Z1, 5), zbit(Z1, 7)>(x[5], x[7]);
+    gs_pair<INV, 0, zbit(Z2, 0), zbit(Z2, 4)>(x[0], x[4]);
+    gs_pair<INV, 0, zbit(Z2, 1), zbit(Z2, 5)>(x[1], x[5]);
+    gs_pair<INV, 0, zbit(Z2, 2), zbit(Z2, 6)>(x[2], x[6]);
+    gs_pair<INV, 0, zbit(Z2, 3), zbit(Z2, 7)>(x[3], x[7]);
+}
+
+template <bool INV>
+__device__ __forceinline__ void ct8_network(uint64_t *x) {
+    ct_pair<INV, 0>(x[0], x[4]);
+    ct_pair<INV, 0>(x[1], x[5]);
+    ct_pair<INV, 0>(x[2], x[6]);
+    ct_pair<INV, 0>(x[3], x[7]);
+    ct_pair<INV, 0>(x[0], x[2]);
+    ct_pair<INV, 0>(x[1], x[3]);
+    ct_pair<INV, 1>(x[4], x[6]);
+    ct_pair<INV, 1>(x[5], x[7]);
+    ct_pair<INV, 0>(x[0], x[1]);
+    ct_pair<INV, 1>(x[2], x[3]);
+    ct_pair<INV, 2>(x[4], x[5]);
+    ct_pair<INV, 3>(x[6], x[7]);
+}
+
+// The radix-8 round at local bits S0..S0+2 of a tile of NB stage bits (the R = 3 case of
+// round_regs, same values). tv: the table's V powers, row G = (hi << (NB - S0 - 3)) |
+// (base >> (S0 + 3)). ZM: GS inputs known to be zero.
+template <bool GS, bool INV, int NB, int S0, bool TOP = false, uint32_t ZM = 0>
+__device__ __forceinline__ void round8_regs(uint64_t *x, uint32_t base, uint64_t hi,
+                                            const uint64_t *__restrict__ tv) {
+    constexpr bool unit = TOP && S0 + 3 == NB;  // G = 0: V = 1
+    uint64_t v[7];
+    if constexpr (!unit) {
+        const uint64_t G = (hi << (NB - S0 - 3)) | (base >> (S0 + 3));
+        const uint64_t *vp = (const uint64_t *)__builtin_assume_aligned(tv + 8 * G, 64);
+#pragma unroll
+        for (int m = 0; m < 7; m++) v[m] = vp[m];
+    }
+    if constexpr (!GS) {
+        if constexpr (!unit) {
+#pragma unroll
+            for (int m = 1; m < 8; m++) x[m] = gl::mul(x[m], v[m - 1]);
+        }
+        ct8_network<INV>(x);
+    } else {
+        gs8_network<INV, ZM>(x);
+        if constexpr (!unit) {
+#pragma unroll
+            for (int m = 1; m < 8; m++) x[m] = gl::mul(x[m], v[m - 1]);
+        }
+    }
+}
+
+template <bool GS, bool INV, int R, int NB, int LOGC, int S0, bool TOP = false, int THREADS = TILE_THREADS>
 __device__ __forceinline__ void stage_round_ct(uint64_t *lds, uint64_t hi, const uint64_t *__restrict__ tw,
-                                               uint32_t tid) {
+                                               const uint64_t *__restrict__ tv, uint32_t tid) {
     constexpr int M = 1 << R;
     constexpr uint32_t groups = 1u << (NB - R + LOGC);
     constexpr uint32_t cmask = (1u << LOGC) - 1;
@@ -168,33 +315,38 @@ __device__ __forceinline__ void stage_round_ct(uint64_t *lds, uint64_t hi, const
         uint64_t x[M];
 #pragma unroll
         for (int j = 0; j < M; j++) x[j] = e0[j * stride];
-        round_regs<GS, R, NB, S0, TOP>(x, base, hi, tw);
+        if constexpr (R == 3)
+            round8_regs<GS, INV, NB, S0, TOP>(x, base, hi, tv);
+        else
+            round_regs<GS, R, NB, S0, TOP>(x, base, hi, tw);
 #pragma unroll
         for (int j = 0; j < M; j++) e0[j * stride] = x[j];
     }
 }
 
-template <bool GS, int NB, int LOGC, int DONE = 0, bool TOP = false, int THREADS = TILE_THREADS>
-__device__ __forceinline__ void tile_stages_ct(uint64_t *lds, uint64_t hi, const uint64_t *__restrict__ tw) {
+template <bool GS, bool INV, int NB, int LOGC, int DONE = 0, bool TOP = false, int THREADS = TILE_THREADS>
+__device__ __forceinline__ void tile_stages_ct(uint64_t *lds, uint64_t hi, const uint64_t *__restrict__ tw,
+                                               const uint64_t *__restrict__ tv) {
     if constexpr (DONE < NB) {
         constexpr int R = NB - DONE >= 3 ? 3 : NB - DONE;
         constexpr int S0 = GS ? DONE : NB - DONE - R;
-        stage_round_ct<GS, R, NB, LOGC, S0, TOP, THREADS>(lds, hi, tw, threadIdx.x);
+        stage_round_ct<GS, INV, R, NB, LOGC, S0, TOP, THREADS>(lds, hi, tw, tv, threadIdx.x);
         __syncthreads();
-        tile_stages_ct<GS, NB, LOGC, DONE + R, TOP, THREADS>(lds, hi, tw);
+        tile_stages_ct<GS, INV, NB, LOGC, DONE + R, TOP, THREADS>(lds, hi, tw, tv);
     }
 }
 
 
 // Stages over bits [lo, lo+nb) of every row of length 2^L. Element e of the tile is
 // (mid = e >> logc, col = e & (C-1)); global index p = hi*2^(lo+nb) + mid*2^lo + lblk*C + col.
-// NB > 0: nb = NB and logc = LOGC, known at compile time. THREADS: the workgroup size.
-template <bool GS, int NB, int LOGC, bool TOP = false, int THREADS = TILE_THREADS>
+// NB > 0: nb = NB and logc = LOGC, known at compile time. THREADS: the workgroup size. INV: tw
+// is an inverse table (w^{-rev(t)}); tv: its V powers (vpow_table).
+template <bool GS, bool INV, int NB, int LOGC, bool TOP = false, int THREADS = TILE_THREADS>
 __global__ void __launch_bounds__(THREADS)
 k_tile_pass(const uint64_t *__restrict__ in, uint64_t *__restrict__ out, uint64_t in_stride,
             uint64_t out_stride, uint32_t L, uint32_t lo, uint32_t nb, uint32_t logc,
-            const uint64_t *__restrict__ tw, const uint64_t *const *in_rows,
-            uint64_t *const *out_rows) {
+            const uint64_t *__restrict__ tw, const uint64_t *__restrict__ tv,
+            const uint64_t *const *in_rows, uint64_t *const *out_rows) {
     extern __shared__ uint64_t lds[];
     const uint32_t C = 1u << logc;
     const uint32_t tiles_per_hi = 1u << (lo - logc);
@@ -220,7 +372,7 @@ k_tile_pass(const uint64_t *__restrict__ in, uint64_t *__restrict__ out, uint64_
     __syncthreads();
 
     if constexpr (NB > 0)
-        tile_stages_ct<GS, NB, LOGC, 0, TOP, THREADS>(lds, hi, tw);
+        tile_stages_ct<GS, INV, NB, LOGC, 0, TOP, THREADS>(lds, hi, tw, tv);
     else
         tile_stages<GS>(lds, nb, logc, hi, tw);
 
@@ -241,6 +393,7 @@ __global__ void __launch_bounds__(MID_THREADS)
 k_encode_mid(const uint64_t *__restrict__ in, uint64_t in_stride, uint64_t *__restrict__ cw,
              uint64_t cw_stride, uint32_t log_k, uint32_t log_r, uint32_t tk,
              const uint64_t *__restrict__ zk_inv, const uint64_t *__restrict__ zn,
+             const uint64_t *__restrict__ vk_inv, const uint64_t *__restrict__ vn,
              const uint64_t *__restrict__ spread, const uint64_t *__restrict__ masks,
              uint64_t P, const uint64_t *__restrict__ pow7, uint64_t shift_pow_k,
              const uint64_t *const *in_rows, uint64_t *const *cw_rows, uint64_t c_off) {
@@ -259,7 +412,7 @@ k_encode_mid(const uint64_t *__restrict__ in, uint64_t in_stride, uint64_t *__re
     // half the waves without a group here; they wait at the barrier while the other
     // workgroup's waves use the SIMDs (radix-4 rounds, which busy every wave, measured slower).
     if constexpr (TK > 0)
-        tile_stages_ct<false, TK, 0, 0, false, MID_THREADS>(lds, c, zk_inv);
+        tile_stages_ct<false, true, TK, 0, 0, false, MID_THREADS>(lds, c, zk_inv, vk_inv);
     else
         tile_stages<false>(lds, tk, 0, c, zk_inv);
 
@@ -270,6 +423,8 @@ k_encode_mid(const uint64_t *__restrict__ in, uint64_t in_stride, uint64_t *__re
             // registers: each group's 8 / s entries, zeros elsewhere, which the butterflies of
             // the round fold away. Every group writes all 8 of its positions.
             constexpr int S = 1 << (TN - TK), PERG = 8 / S;
+            // positions j of a group with j % S != 0 hold zeros
+            constexpr uint32_t ZEROS = S == 2 ? 0xAAu : S == 4 ? 0xEEu : 0xFEu;
             constexpr uint32_t GROUPS = 1u << (TN - 3);
             constexpr uint32_t GPT = (GROUPS + MID_THREADS - 1) / MID_THREADS;
             static_assert(TN - TK >= 1 && TN - TK <= 3 && TN >= 3, "expansion round shape");
@@ -291,12 +446,12 @@ k_encode_mid(const uint64_t *__restrict__ in, uint64_t in_stride, uint64_t *__re
                 uint64_t x[8];
 #pragma unroll
                 for (int j = 0; j < 8; j++) x[j] = (j % S == 0) ? v[i][j / S] : 0;
-                round_regs<true, 3, TN, 0>(x, g << 3, c, zn);
+                round8_regs<true, false, TN, 0, false, ZEROS>(x, g << 3, c, vn);
 #pragma unroll
                 for (int j = 0; j < 8; j++) lds[(g << 3) + j] = x[j];
             }
             __syncthreads();
-            tile_stages_ct<true, TN, 0, 3, false, MID_THREADS>(lds, c, zn);
+            tile_stages_ct<true, false, TN, 0, 3, false, MID_THREADS>(lds, c, zn, vn);
             uint64_t *dst = cw_rows ? cw_rows[row] : cw + row * cw_stride;
             for (uint32_t e = threadIdx.x; e < Tn; e += MID_THREADS) dst[nbase + e] = lds[e];
             return;
@@ -335,7 +490,7 @@ k_encode_mid(const uint64_t *__restrict__ in, uint64_t in_stride, uint64_t *__re
 
     // Gentleman-Sande stages b = 0 .. tk+r-1 of the size-n forward transform.
     if constexpr (TK > 0)
-        tile_stages_ct<true, TN, 0, 0, false, MID_THREADS>(lds, c, zn);
+        tile_stages_ct<true, false, TN, 0, 0, false, MID_THREADS>(lds, c, zn, vn);
     else
         tile_stages<true>(lds, tk + log_r, 0, c, zn);
 
@@ -346,26 +501,27 @@ k_encode_mid(const uint64_t *__restrict__ in, uint64_t in_stride, uint64_t *__re
 // ---------------------------------------------------------------------------------------------
 
 using TilePassFn = void (*)(const uint64_t *, uint64_t *, uint64_t, uint64_t, uint32_t, uint32_t,
-                            uint32_t, uint32_t, const uint64_t *, const uint64_t *const *,
-                            uint64_t *const *);
+                            uint32_t, uint32_t, const uint64_t *, const uint64_t *,
+                            const uint64_t *const *, uint64_t *const *);
 using MidFn = void (*)(const uint64_t *, uint64_t, uint64_t *, uint64_t, uint32_t, uint32_t, uint32_t,
-                       const uint64_t *, const uint64_t *, const uint64_t *, const uint64_t *, uint64_t,
-                       const uint64_t *, uint64_t, const uint64_t *const *, uint64_t *const *, uint64_t);
+                       const uint64_t *, const uint64_t *, const uint64_t *, const uint64_t *,
+                       const uint64_t *, const uint64_t *, uint64_t, const uint64_t *, uint64_t,
+                       const uint64_t *const *, uint64_t *const *, uint64_t);
 
 // The compile-time tile pass for (nb, 2^(12 - nb) columns), else the general one. top: the pass
 // covers the row's top bits (its tiles have hi = 0).
-template <bool GS, bool TOP>
+template <bool GS, bool INV, bool TOP>
 static TilePassFn tile_pass_nb(uint32_t nb) {
     switch (nb) {
-        case 1: return k_tile_pass<GS, 1, PASS_TILE_LOG - 1, TOP>;
-        case 2: return k_tile_pass<GS, 2, PASS_TILE_LOG - 2, TOP>;
-        case 3: return k_tile_pass<GS, 3, PASS_TILE_LOG - 3, TOP>;
-        case 4: return k_tile_pass<GS, 4, PASS_TILE_LOG - 4, TOP>;
-        case 5: return k_tile_pass<GS, 5, PASS_TILE_LOG - 5, TOP>;
-        case 6: return k_tile_pass<GS, 6, PASS_TILE_LOG - 6, TOP>;
-        case 7: return k_tile_pass<GS, 7, PASS_TILE_LOG - 7, TOP>;
-        case 8: return k_tile_pass<GS, 8, PASS_TILE_LOG - 8, TOP>;
-        case 9: return k_tile_pass<GS, 9, PASS_TILE_LOG - 9, TOP>;
+        case 1: return k_tile_pass<GS, INV, 1, PASS_TILE_LOG - 1, TOP>;
+        case 2: return k_tile_pass<GS, INV, 2, PASS_TILE_LOG - 2, TOP>;
+        case 3: return k_tile_pass<GS, INV, 3, PASS_TILE_LOG - 3, TOP>;
+        case 4: return k_tile_pass<GS, INV, 4, PASS_TILE_LOG - 4, TOP>;
+        case 5: return k_tile_pass<GS, INV, 5, PASS_TILE_LOG - 5, TOP>;
+        case 6: return k_tile_pass<GS, INV, 6, PASS_TILE_LOG - 6, TOP>;
+        case 7: return k_tile_pass<GS, INV, 7, PASS_TILE_LOG - 7, TOP>;
+        case 8: return k_tile_pass<GS, INV, 8, PASS_TILE_LOG - 8, TOP>;
+        case 9: return k_tile_pass<GS, INV, 9, PASS_TILE_LOG - 9, TOP>;
         default: return nullptr;
     }
 }
@@ -379,14 +535,15 @@ static bool wide_pass(uint32_t nb, uint32_t logc) {
     return nb == WIDE_PASS_NB && logc == WIDE_PASS_LOGC;
 }
 
-template <bool GS>
+template <bool GS, bool INV>
 static TilePassFn tile_pass_fn(uint32_t nb, uint32_t logc, bool top) {
     if (wide_pass(nb, logc))
-        return top ? k_tile_pass<GS, WIDE_PASS_NB, WIDE_PASS_LOGC, true, WIDE_PASS_THREADS>
-                   : k_tile_pass<GS, WIDE_PASS_NB, WIDE_PASS_LOGC, false, WIDE_PASS_THREADS>;
+        return top ? k_tile_pass<GS, INV, WIDE_PASS_NB, WIDE_PASS_LOGC, true, WIDE_PASS_THREADS>
+                   : k_tile_pass<GS, INV, WIDE_PASS_NB, WIDE_PASS_LOGC, false, WIDE_PASS_THREADS>;
     TilePassFn f = nullptr;
-    if (nb <= 9 && logc == PASS_TILE_LOG - nb) f = top ? tile_pass_nb<GS, true>(nb) : tile_pass_nb<GS, false>(nb);
-    return f ? f : k_tile_pass<GS, 0, 0>;
+    if (nb <= 9 && logc == PASS_TILE_LOG - nb)
+        f = top ? tile_pass_nb<GS, INV, true>(nb) : tile_pass_nb<GS, INV, false>(nb);
+    return f ? f : k_tile_pass<GS, INV, 0, 0>;
 }
 
 static uint32_t tile_pass_threads(uint32_t nb, uint32_t logc) {
@@ -436,6 +593,23 @@ static std::vector<uint64_t> twiddle_table(uint32_t log_n, bool inverse) {
     return out;
 }
 
+// The radix-8 rounds' V powers of a twiddle table Z of 2^(L-1) entries: row G < 2^(L-3) holds
+// Z[4G]^m for m = 1..7 and a zero, eight words (64 bytes) per row. One row for L < 3 (no radix-8
+// round then).
+static std::vector<uint64_t> vpow_table(const std::vector<uint64_t> &Z, uint32_t L) {
+    const uint64_t rows = L >= 3 ? 1ull << (L - 3) : 1;
+    std::vector<uint64_t> out(8 * rows, 0);
+    for (uint64_t G = 0; G < rows; G++) {
+        const uint64_t V = L >= 3 ? Z[4 * G] : 1;
+        uint64_t acc = V;
+        for (int m = 0; m < 7; m++) {
+            out[8 * G + m] = acc;
+            acc = gl::mul(acc, V);
+        }
+    }
+    return out;
+}
+
 static std::vector<std::pair<uint32_t, uint32_t>> split_bits(uint32_t lo, uint32_t hi,
                                                              uint32_t max_nb) {
     std::vector<std::pair<uint32_t, uint32_t>> passes;
@@ -482,6 +656,8 @@ Status Encoder::init(Ctx *ctx, uint32_t log_k, uint32_t log_r, uint64_t P, uint3
     Status st;
     if ((st = zk_inv.upload(ctx, zki))) return st;
     if ((st = zn_fwd.upload(ctx, zn))) return st;
+    if ((st = vk_inv.upload(ctx, vpow_table(zki, log_k)))) return st;
+    if ((st = vn_fwd.upload(ctx, vpow_table(zn, log_n)))) return st;
     if ((st = spread_tab.upload(ctx, spread))) return st;
     if ((st = pow7_tab.upload(ctx, pow7))) return st;
     return Status::ok();
@@ -525,8 +701,10 @@ void Encoder::launch(const uint64_t *src, uint64_t src_stride, const uint64_t *c
         dim3 grid((unsigned)(k >> (nb + logc)), (unsigned)rc);
         size_t shm = sizeof(uint64_t) << (nb + logc);
         KernelTimer timer(ctx, "k_tile_pass<ct>", stream);
-        hipLaunchKernelGGL(tile_pass_fn<false>(nb, logc, lo + nb == log_k), grid, dim3(tile_pass_threads(nb, logc)), shm, stream, src, tmp,
-                           src_stride, k, log_k, lo, nb, logc, zk_inv.ptr, src_rows, tmp_rows);
+        const TilePassFn f = tile_pass_fn<false, true>(nb, logc, lo + nb == log_k);
+        hipLaunchKernelGGL(f, grid,
+                           dim3(tile_pass_threads(nb, logc)), shm, stream, src, tmp, src_stride, k, log_k,
+                           lo, nb, logc, zk_inv.ptr, vk_inv.ptr, src_rows, tmp_rows);
         src = tmp;
         src_stride = k;
         src_rows = (const uint64_t *const *)tmp_rows;
@@ -539,7 +717,7 @@ void Encoder::launch(const uint64_t *src, uint64_t src_stride, const uint64_t *c
             const uint64_t tc = tiles - c0 < 65535 ? tiles - c0 : 65535;
             hipLaunchKernelGGL(mid_fn(tk, log_r), dim3((unsigned)rc, (unsigned)tc), dim3(MID_THREADS), shm,
                                stream, src, src_stride, cw, n, log_k, log_r, tk, zk_inv.ptr, zn_fwd.ptr,
-                               spread_tab.ptr, masks, P, pow7_tab.ptr, shift_pow_k, src_rows, cw_rows,
+                               vk_inv.ptr, vn_fwd.ptr, spread_tab.ptr, masks, P, pow7_tab.ptr, shift_pow_k, src_rows, cw_rows,
                                c0);
         }
     }
@@ -549,9 +727,10 @@ void Encoder::launch(const uint64_t *src, uint64_t src_stride, const uint64_t *c
         dim3 grid((unsigned)(n >> (nb + logc)), (unsigned)rc);
         size_t shm = sizeof(uint64_t) << (nb + logc);
         KernelTimer timer(ctx, "k_tile_pass<gs>", stream);
-        hipLaunchKernelGGL(tile_pass_fn<true>(nb, logc, lo + nb == log_n), grid, dim3(tile_pass_threads(nb, logc)), shm, stream, cw, cw, n, n,
-                           log_n, lo, nb, logc, zn_fwd.ptr, (const uint64_t *const *)cw_rows,
-                           cw_rows);
+        const TilePassFn f = tile_pass_fn<true, false>(nb, logc, lo + nb == log_n);
+        hipLaunchKernelGGL(f, grid,
+                           dim3(tile_pass_threads(nb, logc)), shm, stream, cw, cw, n, n, log_n, lo, nb,
+                           logc, zn_fwd.ptr, vn_fwd.ptr, (const uint64_t *const *)cw_rows, cw_rows);
     }
 }
 
@@ -653,6 +832,7 @@ struct NttPlanDev {
     Ctx *ctx = nullptr;
     uint32_t L = 0;
     DevBuf<uint64_t> fwd, inv;  // Z[t] = w^{rev(t)}, w^{-rev(t)}, t < n/2
+    DevBuf<uint64_t> vfwd, vinv;  // their V powers (vpow_table)
     uint64_t size_inv = 1;
     std::vector<std::pair<uint32_t, uint32_t>> passes;  // stage-bit ranges, ascending
 
@@ -660,8 +840,12 @@ struct NttPlanDev {
         ctx = c;
         L = log_n;
         Status st;
-        if ((st = fwd.upload(ctx, twiddle_table(L, false)))) return st;
-        if ((st = inv.upload(ctx, twiddle_table(L, true)))) return st;
+        {
+            const std::vector<uint64_t> zf = twiddle_table(L, false), zi = twiddle_table(L, true);
+            if ((st = fwd.upload(ctx, zf)) || (st = inv.upload(ctx, zi))) return st;
+            if ((st = vfwd.upload(ctx, vpow_table(zf, L))) || (st = vinv.upload(ctx, vpow_table(zi, L))))
+                return st;
+        }
         size_inv = gl::pow(gl::P / 2 + 1, L);
         // the low bits in one pass of contiguous tiles (2^12 values), the rest in passes of up
         // to 10 strided bits
@@ -679,13 +863,15 @@ struct NttPlanDev {
             const uint32_t lo = ps.first, nb = ps.second, logc = pass_logc(lo, nb);
             const size_t shm = sizeof(uint64_t) << (nb + logc);
             const bool top = lo + nb == L;
-            TilePassFn f = gs ? tile_pass_fn<true>(nb, logc, top) : tile_pass_fn<false>(nb, logc, top);
+            // ntt: CT with the forward table; intt: GS with the inverse table
+            TilePassFn f = gs ? tile_pass_fn<true, true>(nb, logc, top) : tile_pass_fn<false, false>(nb, logc, top);
             for (uint64_t r0 = 0; r0 < rows; r0 += 65535) {
                 const uint64_t rc = rows - r0 < 65535 ? rows - r0 : 65535;
                 KernelTimer timer(ctx, gs ? "k_tile_pass<gs>" : "k_tile_pass<ct>", s);
                 hipLaunchKernelGGL(f, dim3((unsigned)(n >> (nb + logc)), (unsigned)rc),
                                    dim3(tile_pass_threads(nb, logc)), shm, s, v + r0 * n, v + r0 * n, n, n,
-                                   L, lo, nb, logc, gs ? inv.ptr : fwd.ptr, nullptr, nullptr);
+                                   L, lo, nb, logc, gs ? inv.ptr : fwd.ptr, gs ? vinv.ptr : vfwd.ptr,
+                                   nullptr, nullptr);
             }
         }
     }

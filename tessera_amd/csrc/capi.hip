@@ -778,27 +778,32 @@ tsr_status tsr_commitment_answer(const tsr_commitment *c, const uint64_t *querie
     Ctx *ctx = c->ctx;
     TRY_HIP(hipSetDevice(ctx->device));
     memcpy(distinct_out, d.data(), 8 * d.size());
-    DevBuf<uint64_t> idx, cols, sidx;
+    // one upload of both index lists and the readbacks through the context's pinned block, so
+    // every copy is asynchronous and the call waits once
+    const size_t nd = d.size(), np = pos.size();
+    const size_t up_bytes = 8 * (nd + np), col_bytes = 8 * nd * c->count, sib_bytes = 32 * np;
+    uint8_t *pin;
+    TRY(pinned_host(ctx, up_bytes + col_bytes + sib_bytes, &pin));
+    memcpy(pin, d.data(), 8 * nd);
+    if (np) memcpy(pin + 8 * nd, pos.data(), 8 * np);
+    DevBuf<uint64_t> idx, cols;
     DevBuf<uint32_t> sib;
-    TRY(idx.alloc(ctx, d.size()));
-    TRY_HIP(hipMemcpyAsync(idx.ptr, d.data(), 8 * d.size(), hipMemcpyHostToDevice, ctx->stream));
+    TRY(idx.alloc(ctx, nd + np));
+    TRY_HIP(hipMemcpyAsync(idx.ptr, pin, up_bytes, hipMemcpyHostToDevice, ctx->stream));
     if (c->count) {
-        TRY(cols.alloc(ctx, d.size() * c->count));
-        TRY(gather_columns(c->codewords.ptr, c->n, c->count, idx.ptr, d.size(), cols.ptr,
-                           ctx->stream));
-        TRY_HIP(hipMemcpyAsync(columns_out, cols.ptr, 8 * d.size() * c->count,
-                               hipMemcpyDeviceToHost, ctx->stream));
+        TRY(cols.alloc(ctx, nd * c->count));
+        TRY(gather_columns(c->codewords.ptr, c->n, c->count, idx.ptr, nd, cols.ptr, ctx->stream));
+        TRY_HIP(hipMemcpyAsync(pin + up_bytes, cols.ptr, col_bytes, hipMemcpyDeviceToHost, ctx->stream));
     }
-    if (!pos.empty()) {
-        TRY(sidx.alloc(ctx, pos.size()));
-        TRY(sib.alloc(ctx, 8 * pos.size()));
-        TRY_HIP(hipMemcpyAsync(sidx.ptr, pos.data(), 8 * pos.size(), hipMemcpyHostToDevice,
-                               ctx->stream));
-        TRY(gather_digests(c->nodes.ptr, sidx.ptr, pos.size(), sib.ptr, ctx->stream));
-        TRY_HIP(hipMemcpyAsync(siblings_out, sib.ptr, 32 * pos.size(), hipMemcpyDeviceToHost,
+    if (np) {
+        TRY(sib.alloc(ctx, 8 * np));
+        TRY(gather_digests(c->nodes.ptr, idx.ptr + nd, np, sib.ptr, ctx->stream));
+        TRY_HIP(hipMemcpyAsync(pin + up_bytes + col_bytes, sib.ptr, sib_bytes, hipMemcpyDeviceToHost,
                                ctx->stream));
     }
     TRY_HIP(hipStreamSynchronize(ctx->stream));
+    if (col_bytes) memcpy(columns_out, pin + up_bytes, col_bytes);
+    if (sib_bytes) memcpy(siblings_out, pin + up_bytes + col_bytes, sib_bytes);
     return TSR_OK;
 }
 

@@ -82,13 +82,24 @@ class ProvenBatch:
     final_evals: np.ndarray  # n_finals x 2
 
 
+_prove_fn = None
+
+
+def _prove():
+    """tsr_sumcheck_prove with the instance array typed (set once)."""
+    global _prove_fn
+    if _prove_fn is None:
+        f = lib().tsr_sumcheck_prove
+        f.restype = ctypes.c_int
+        f.argtypes = [vp, vp, ctypes.c_char_p, u32, ctypes.POINTER(_CInstance), u32, vp, vp, vp, vp,
+                      u64, ctypes.POINTER(u64)]
+        _prove_fn = f
+    return _prove_fn
+
+
 def prove_batch(device, transcript, prefix: str, degree: int, instances) -> ProvenBatch:
     """prove_batch(transcript, prefix, degree, instances) (prover.rs:1122-1145); each instance is
     a list of Table (an Instance::new product) or a SumOfProducts."""
-    _SIG = lib().tsr_sumcheck_prove
-    _SIG.restype = ctypes.c_int
-    _SIG.argtypes = [vp, vp, ctypes.c_char_p, u32, ctypes.POINTER(_CInstance), u32, vp, vp, vp, vp,
-                     u64, ctypes.POINTER(u64)]
     keep = []
     cinst = (_CInstance * max(1, len(instances)))()
     n_max, n_final = 0, 0
@@ -126,18 +137,20 @@ def prove_batch(device, transcript, prefix: str, degree: int, instances) -> Prov
             cinst[i].terms = tarr
             cinst[i].nterms = len(terms)
         n_max = max(n_max, vars_)
-    gamma = np.zeros(2, dtype=np.uint64)
-    rounds = np.zeros(max(1, n_max * degree) * 2, dtype=np.uint64)
-    chals = np.zeros(max(1, n_max) * 2, dtype=np.uint64)
-    finals = np.zeros(max(1, n_final) * 2, dtype=np.uint64)
+    # one host block for gamma, the round messages, the challenges and the final evaluations
+    n_round = max(1, n_max * degree) * 2
+    n_chal = max(1, n_max) * 2
+    out = np.empty(2 + n_round + n_chal + max(1, n_final) * 2, dtype=np.uint64)
+    base = out.ctypes.data
     nf = u64()
-    check(_SIG(device.handle, transcript.handle, prefix.encode(), degree, cinst, len(instances),
-               gamma.ctypes.data, rounds.ctypes.data, chals.ctypes.data, finals.ctypes.data,
-               max(1, n_final), ctypes.byref(nf)))
-    return ProvenBatch((int(gamma[0]), int(gamma[1])),
-                       rounds[:n_max * degree * 2].reshape(n_max, degree, 2),
-                       chals[:n_max * 2].reshape(n_max, 2),
-                       finals[:nf.value * 2].reshape(nf.value, 2))
+    check(_prove()(device.handle, transcript.handle, prefix.encode(), degree, cinst, len(instances),
+                   base, base + 16, base + 16 + 8 * n_round, base + 16 + 8 * (n_round + n_chal),
+                   max(1, n_final), ctypes.byref(nf)))
+    o_r, o_c, o_f = 2, 2 + n_round, 2 + n_round + n_chal
+    return ProvenBatch((int(out[0]), int(out[1])),
+                       out[o_r:o_r + n_max * degree * 2].reshape(n_max, degree, 2),
+                       out[o_c:o_c + n_max * 2].reshape(n_max, 2),
+                       out[o_f:o_f + nf.value * 2].reshape(nf.value, 2))
 
 
 def eq_table(device, point, out=None):

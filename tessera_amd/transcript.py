@@ -83,9 +83,12 @@ class Transcript:
         return v.value
 
     def challenge_fp2(self, label: str):
-        out = np.zeros(2, dtype=np.uint64)
-        check(lib().tsr_transcript_challenge_fp2(self.handle, label.encode(), out.ctypes.data))
-        return (int(out[0]), int(out[1]))
+        # one reusable output pair: an ndarray's .ctypes costs more than the squeeze itself
+        out = self.__dict__.get("_fp2_out")
+        if out is None:
+            out = self._fp2_out = (ctypes.c_uint64 * 2)()
+        check(lib().tsr_transcript_challenge_fp2(self.handle, label.encode(), out))
+        return (out[0], out[1])
 
     def challenge_indices(self, label: str, log_n: int, count: int):
         out = np.zeros(max(count, 1), dtype=np.uint64)

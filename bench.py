@@ -330,7 +330,8 @@ def run_served(cpu_sample):
     U[16,512] (prompt a quarter): the server (continuous batching, 128 slots, 4 prover streams,
     chunks of 32 positions) vs plain per-request calls on the same arrivals (each starts at the
     later of its arrival and the previous call's end). Latency = completion - arrival. Capacity:
-    the same requests offered all at once to the server."""
+    the same requests offered all at once to the server, and once more to a server that commits
+    each request's witness on its own (group = 1) instead of the waiting requests together."""
     from tessera_amd.serve import Server, report, run_plain as plain_calls, run_served as served_calls, workload
     reqs = workload(SERVE_REQUESTS, seed=0x5E7E, rate_hz=SERVE_RATE_HZ)
     with Server(slots=128, provers=4, chunk=32, model_seed=SEED) as srv:
@@ -343,16 +344,26 @@ def run_served(cpu_sample):
         burst = [(rid + 10_000, p, n, 0.0) for rid, p, n, _ in reqs]
         res_b, wall_b, t0_b = served_calls(srv, burst)
         capacity = report(res_b, burst, wall_b, t0_b)
+        served["mean_group"] = float(np.mean([r.group for r in res_s.values()]))
+        capacity["mean_group"] = float(np.mean([r.group for r in res_b.values()]))
         res_p, lat_p, wall_p = plain_calls(srv, reqs)
+    # the capacity burst again with every request's witness committed on its own (group = 1)
+    with Server(slots=128, provers=4, chunk=32, group=1, model_seed=SEED) as srv:
+        served_calls(srv, warm)
+        res_u, wall_u, t0_u = served_calls(srv, burst)
+        capacity_single = report(res_u, burst, wall_u, t0_u)
     p50, p99 = np.percentile(lat_p, 50), np.percentile(lat_p, 99)
     service = [r.t_done - r.t_admit for r in res_p.values()]
     plain = {"proofs_per_s": len(res_p) / wall_p, "p50_ms": 1e3 * p50, "p99_ms": 1e3 * p99,
              "service_p50_ms": 1e3 * float(np.median(service)), "wall_s": wall_p}
     equal = all(res_s[i].status == 0 and res_s[i].key() == res_p[i].key() for i in res_p)
+    equal = equal and all(res_u[i].status == 0 and res_u[i].key() == res_b[i].key() for i in res_b)
     out = {"rate_hz": SERVE_RATE_HZ, "requests": len(reqs),
            "served": {k: round(v, 3) if isinstance(v, float) else v for k, v in served.items()},
            "plain": {k: round(v, 3) for k, v in plain.items()},
            "capacity": {k: round(v, 3) if isinstance(v, float) else v for k, v in capacity.items()},
+           "capacity_commits_one_by_one": {k: round(v, 3) if isinstance(v, float) else v
+                                           for k, v in capacity_single.items()},
            "speedup_proofs_per_s": round(served["proofs_per_s"] / plain["proofs_per_s"], 3),
            "capacity_vs_plain": round(capacity["proofs_per_s"] / plain["proofs_per_s"], 3),
            "results_equal": bool(equal)}
@@ -754,11 +765,12 @@ def main():
     ap.add_argument("--traffic-command", default="", help="the profiling command, for the summary")
     ap.add_argument("--trace-summary", nargs=3, metavar=("DB", "TITLE", "COMMAND"),
                     help="print a markdown summary of a rocprofv3 kernel-trace database")
-    ap.add_argument("--only", choices=["headline", "copy", "hostfed", "refshapes", "sumcheck", "batched"], default=None,
+    ap.add_argument("--only", choices=["headline", "copy", "hostfed", "refshapes", "sumcheck", "batched", "served"],
+                    default=None,
                     help="time the headline commit only; run the device copy that calibrates "
                          "the HBM byte counters (4 copies of 2 GiB); time the headline commit "
                          "from host rows (the PCIe-inclusive rate); or time the reference's three "
-                         "H100 unit-cost workloads")
+                         "H100 unit-cost workloads; or one of the other sections alone")
     a = ap.parse_args()
     if a.gpus != 1:
         print("bench.py measures one GPU: this build runs on a single MI355X; "
@@ -784,6 +796,9 @@ def main():
         return
     if a.only == "sumcheck":
         print(json.dumps({"workload": "sumcheck/lookup configuration", **run_sumcheck()}), flush=True)
+        return
+    if a.only == "served":
+        print(json.dumps({"workload": "served proving", **run_served(cpu_sample=False)}), flush=True)
         return
     if a.only == "refshapes":
         print(json.dumps({"workload": "the reference's H100 unit-cost workloads", **run_refshapes()}), flush=True)

@@ -289,13 +289,16 @@ void tsr_commit_batch_free(tsr_batch *b);
  *      a degree-3 sumcheck Sum eq(r,x) A(x) B(x) over two witness tables follows on the request's
  *      own transcript. Admitted requests decode together (one launch advances each by up to
  *      `chunk` positions; requests join and leave between launches) and are proved on prover
- *      streams while decoding goes on. ---- */
+ *      streams while decoding goes on: a prover commits the decoded requests waiting for it (up
+ *      to `group`) in one launch set, then proves each on its own transcript. ---- */
 #define TSR_MAX_TOKENS 512
 typedef struct tsr_server tsr_server;
 typedef struct {
     uint32_t slots;       /* requests in flight at once, decoding or proving (1..1024) */
     uint32_t provers;     /* prover threads, one stream each (0 = 4) */
     uint32_t chunk;       /* positions a decoding request advances per decode launch (0 = 32) */
+    uint32_t group;       /* most decoded requests a prover commits in one launch set (0 = 16,
+                           * 1 = each request on its own) */
     uint64_t hbm_budget;  /* admission budget in bytes for the requests in flight (0 = none) */
     uint64_t model_seed;  /* weights: SplitMix64 words mod 255, minus 127 */
 } tsr_server_config;
@@ -308,7 +311,8 @@ typedef struct {
 typedef struct {
     uint64_t id;
     tsr_status status;
-    uint32_t slot;
+    uint32_t slot;                  /* the prover that proved it */
+    uint32_t group;                 /* requests whose witnesses were committed with it (1: alone) */
     uint8_t root[32];               /* the witness commitment */
     uint8_t transcript_state[32];   /* after the sumcheck's final evaluations */
     uint64_t transcript_counter;

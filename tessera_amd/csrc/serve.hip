@@ -37,9 +37,11 @@
 #include <string>
 #include <thread>
 #include <vector>
+#include "commit.h"
 #include "common.h"
 #include "field.h"
 #include "model.h"
+#include "ntt.h"
 #include "sumcheck.h"
 #include "transcript.h"
 
@@ -130,25 +132,116 @@ struct ReqBufs {
     }
 };
 
-// Steps 2-4 of a decoded request on `ctx` (the witness and tokens may belong to another
-// context's pool: they are only read here). Fills `out` except the timestamps.
-static Status prove_decoded(Ctx *ctx, const tsr_request &req, const ReqBufs &b, tsr_result &out) {
-    hipStream_t s = ctx->stream;
-    const uint32_t total = b.total;
-    Status st;
+// The witness commitment's parameters: rows of 2^13, rate 1/4, B = 32, unmasked.
+constexpr uint32_t SERVE_LOG_R = 2, SERVE_BLOCK = 32;
+constexpr uint8_t SERVE_COMMITMENT_ID_BYTE = 0x5e;  // commitment_id = 32 bytes of 0x5e
 
-    // commit the witness rows
+// Step 2 of one decoded request: tsr_commit_unmasked of its witness rows, the root into `root`.
+static Status commit_one(Ctx *ctx, const ReqBufs &b, uint8_t root[32]) {
     tsr_commitment_params params{};
     params.log2_k = SERVE_LOG_K;
-    params.log2_inv_rate = 2;
+    params.log2_inv_rate = SERVE_LOG_R;
     params.mask_coeffs = 0;
-    params.row_block = 32;
-    memset(params.commitment_id, 0x5e, 32);
+    params.row_block = SERVE_BLOCK;
+    memset(params.commitment_id, SERVE_COMMITMENT_ID_BYTE, 32);
     tsr_commitment *c = nullptr;
     tsr_status cs = tsr_commit_unmasked((tsr_ctx *)ctx, &params, b.wit.ptr, b.rows, &c);
     if (cs != TSR_OK) return Status::fail(cs, std::string("serve commit: ") + tsr_last_error());
-    tsr_commitment_root(c, out.root);
+    tsr_commitment_root(c, root);
     tsr_commitment_free(c);
+    return Status::ok();
+}
+
+// Step 2 of several decoded requests together, each root equal to its commit_one root. Every
+// witness row of the group goes through one launch per NTT pass (per-row pointer tables: every
+// witness has rows of 2^13, so one encoder serves all), one launch chains every column of every
+// request, one launch per Merkle level hashes that level of every tree, and the roots come back
+// in one copy. The small per-request commitments thus share their launches instead of paying
+// a launch sequence and a synchronization each.
+static Status commit_group(Ctx *ctx, const std::vector<const ReqBufs *> &reqs, uint8_t (*roots)[32]) {
+    hipStream_t s = ctx->stream;
+    const uint32_t count = (uint32_t)reqs.size();
+    const uint64_t k = 1ull << SERVE_LOG_K, n = k << SERVE_LOG_R;
+    const uint32_t levels = SERVE_LOG_K + SERVE_LOG_R;
+    Status st;
+    Encoder *enc = nullptr;
+    if ((st = get_encoder(ctx, SERVE_LOG_K, SERVE_LOG_R, 0, &enc))) return st;
+    std::vector<DevBuf<uint64_t>> cw(count), tmp(count);
+    std::vector<DevBuf<uint32_t>> nodes(count);
+    std::vector<MatDesc> mats(count);
+    std::vector<uint64_t> src_p, tmp_p, cw_p;
+    std::vector<BlockRef> refs;  // the chains' blocks, then each Merkle level's
+    std::vector<uint32_t> first(levels + 2, 0);
+    for (uint32_t i = 0; i < count; i++) {
+        const ReqBufs &b = *reqs[i];
+        if ((st = cw[i].alloc(ctx, b.rows * n)) || (st = tmp[i].alloc(ctx, b.rows * k)) ||
+            (st = nodes[i].alloc(ctx, 8 * (2 * n - 1))))
+            return st;
+        MatDesc &m = mats[i];
+        m.cw = cw[i].ptr;
+        m.n = n;
+        m.rows = b.rows;
+        m.block = SERVE_BLOCK;
+        memset(m.id, SERVE_COMMITMENT_ID_BYTE, 32);
+        m.nodes = nodes[i].ptr;
+        for (uint64_t r = 0; r < b.rows; r++) {
+            src_p.push_back((uint64_t)(b.wit.ptr + r * k));
+            tmp_p.push_back((uint64_t)(tmp[i].ptr + r * k));
+            cw_p.push_back((uint64_t)(cw[i].ptr + r * n));
+        }
+    }
+    for (uint32_t l = 0; l <= levels; l++) {  // l = 0: chains over n columns; l >= 1: n >> l parents
+        first[l] = (uint32_t)refs.size();
+        for (uint32_t i = 0; i < count; i++)
+            for (uint64_t f = 0; f < (n >> l); f += 256) refs.push_back(BlockRef{i, 0, f});
+    }
+    first[levels + 1] = (uint32_t)refs.size();
+    const uint64_t rows = src_p.size();
+    // one upload: [mats | src | tmp | cw row tables | block refs], then the roots come back
+    size_t at = 0;
+    auto place = [&](size_t bytes) {
+        at = (at + 255) & ~(size_t)255;
+        const size_t o = at;
+        at += bytes;
+        return o;
+    };
+    const size_t o_mats = place(sizeof(MatDesc) * count), o_src = place(8 * rows),
+                 o_tmp = place(8 * rows), o_cw = place(8 * rows), o_refs = place(sizeof(BlockRef) * refs.size());
+    const size_t up_end = at, o_roots = place(32 * count);
+    DevBuf<uint8_t> blob;
+    if ((st = blob.alloc(ctx, at))) return st;
+    uint8_t *host;
+    if ((st = pinned_host(ctx, at, &host))) return st;
+    memcpy(host + o_mats, mats.data(), sizeof(MatDesc) * count);
+    memcpy(host + o_src, src_p.data(), 8 * rows);
+    memcpy(host + o_tmp, tmp_p.data(), 8 * rows);
+    memcpy(host + o_cw, cw_p.data(), 8 * rows);
+    memcpy(host + o_refs, refs.data(), sizeof(BlockRef) * refs.size());
+    TSR_HIP(hipMemcpyAsync(blob.ptr, host, up_end, hipMemcpyHostToDevice, s));
+    const MatDesc *d_mats = (const MatDesc *)(blob.ptr + o_mats);
+    const BlockRef *d_refs = (const BlockRef *)(blob.ptr + o_refs);
+    if ((st = enc->encode_rowmap((const uint64_t *const *)(blob.ptr + o_src), (uint64_t *const *)(blob.ptr + o_tmp),
+                                 (uint64_t *const *)(blob.ptr + o_cw), rows, s)))
+        return st;
+    launch_chain_leaves_multi(ctx, d_mats, d_refs + first[0], first[1] - first[0], s);
+    for (uint32_t l = 1; l <= levels; l++)
+        launch_merkle_level_multi(ctx, d_mats, d_refs + first[l], first[l + 1] - first[l], l, s);
+    launch_gather_roots(d_mats, count, (uint32_t *)(blob.ptr + o_roots), s);
+    if ((st = check_launch("serve group commit"))) return st;
+    // the readback reuses the pinned block: the stream has finished the upload before it
+    TSR_HIP(hipMemcpyAsync(host, blob.ptr + o_roots, 32 * count, hipMemcpyDeviceToHost, s));
+    TSR_HIP(hipStreamSynchronize(s));
+    memcpy(roots, host, 32 * count);
+    return Status::ok();
+}
+
+// Steps 3-4 of a decoded request whose witness root is out.root, on `ctx` (the witness and
+// tokens may belong to another context's pool: they are only read here). Fills `out` except the
+// timestamps.
+static Status prove_committed(Ctx *ctx, const tsr_request &req, const ReqBufs &b, tsr_result &out) {
+    hipStream_t s = ctx->stream;
+    const uint32_t total = b.total;
+    Status st;
 
     // the transcript: the request's own session
     std::vector<uint8_t> tce;
@@ -199,6 +292,12 @@ static Status prove_decoded(Ctx *ctx, const tsr_request &req, const ReqBufs &b, 
     return Status::ok();
 }
 
+// Steps 2-4 of a decoded request on its own.
+static Status prove_decoded(Ctx *ctx, const tsr_request &req, const ReqBufs &b, tsr_result &out) {
+    Status st = commit_one(ctx, b, out.root);
+    return st ? st : prove_committed(ctx, req, b, out);
+}
+
 // Where a request is, for tsr_server_cancel.
 enum JobState { JOB_QUEUED, JOB_DECODING, JOB_DECODED, JOB_PROVING };
 
@@ -217,7 +316,7 @@ struct Job {
 struct tsr_server {
     std::chrono::steady_clock::time_point t0;
     int device = 0;
-    uint32_t slots = 1, provers = 1, chunk = 32;
+    uint32_t slots = 1, provers = 1, chunk = 32, group = 16;
     uint64_t budget = 0, in_flight_bytes = 0;
     tsr_ctx *owner = nullptr;  // holds the weights; runs tsr_server_prove
     tsr_ctx *dec = nullptr;    // the decoder's context: its pool holds the requests' buffers
@@ -345,24 +444,45 @@ static void decoder_loop(tsr_server *srv) {
     srv->cv_prove.notify_all();
 }
 
+// A prover takes every decoded request waiting (up to srv->group, oldest first), commits them
+// together (commit_group) and proves them one after another on its stream. A group whose
+// commitment fails (for memory, say) commits its requests one at a time instead.
 static void prover_loop(tsr_server *srv, uint32_t slot) {
     Ctx *ctx = (Ctx *)srv->prover_ctxs[slot];
     (void)hipSetDevice(srv->device);
     for (;;) {
-        std::unique_ptr<Job> job;
+        std::vector<std::unique_ptr<Job>> jobs;
         {
             std::unique_lock<std::mutex> lk(srv->mu);
             srv->cv_prove.wait(lk, [&] { return !srv->decoded.empty() || srv->decoder_done; });
             if (srv->decoded.empty()) return;
-            job = std::move(srv->decoded.front());
-            srv->decoded.pop_front();
-            srv->state[job->req.id] = JOB_PROVING;
+            while (!srv->decoded.empty() && jobs.size() < srv->group) {
+                jobs.push_back(std::move(srv->decoded.front()));
+                srv->decoded.pop_front();
+                srv->state[jobs.back()->req.id] = JOB_PROVING;
+            }
         }
-        tsr_result res{};
-        res.slot = slot;
-        job->status = prove_decoded(ctx, job->req, job->bufs, res);
-        (void)hipStreamSynchronize(ctx->stream);  // (after an error) nothing still reads the buffers
-        finish(srv, *job, &res);
+        const uint32_t g = (uint32_t)jobs.size();
+        std::vector<tsr_result> res(g);
+        bool grouped = false;
+        if (g > 1) {
+            std::vector<const ReqBufs *> reqs;
+            for (auto &job : jobs) reqs.push_back(&job->bufs);
+            std::vector<uint8_t> roots(32 * (size_t)g);
+            grouped = !commit_group(ctx, reqs, (uint8_t(*)[32])roots.data());
+            (void)hipStreamSynchronize(ctx->stream);  // (after an error) nothing still runs
+            if (grouped)
+                for (uint32_t i = 0; i < g; i++) memcpy(res[i].root, &roots[32 * (size_t)i], 32);
+        }
+        for (uint32_t i = 0; i < g; i++) {
+            Job &job = *jobs[i];
+            res[i].slot = slot;
+            res[i].group = grouped ? g : 1;
+            job.status = grouped ? prove_committed(ctx, job.req, job.bufs, res[i])
+                                 : prove_decoded(ctx, job.req, job.bufs, res[i]);
+            (void)hipStreamSynchronize(ctx->stream);  // (after an error) nothing still reads the buffers
+            finish(srv, job, &res[i]);
+        }
     }
 }
 
@@ -378,6 +498,7 @@ tsr_status tsr_server_create(int device, const tsr_server_config *cfg, tsr_serve
         return Status::fail(TSR_BAD_SHAPE, "slots must be 1..1024").code;
     if (cfg->provers > 64) return Status::fail(TSR_BAD_SHAPE, "provers must be 0..64").code;
     if (cfg->chunk > MAX_TOKENS) return Status::fail(TSR_BAD_SHAPE, "chunk must be 0..512").code;
+    if (cfg->group > 1024) return Status::fail(TSR_BAD_SHAPE, "group must be 0..1024").code;
     struct Cleanup {  // releases what a failed create made (no threads run yet)
         tsr_server *s;
         ~Cleanup() {
@@ -395,6 +516,7 @@ tsr_status tsr_server_create(int device, const tsr_server_config *cfg, tsr_serve
     srv->slots = cfg->slots;
     srv->provers = cfg->provers ? cfg->provers : 4;
     srv->chunk = cfg->chunk ? cfg->chunk : 32;
+    srv->group = cfg->group ? cfg->group : 16;
     srv->budget = cfg->hbm_budget;
     tsr_status s = tsr_ctx_create(device, &srv->owner);
     if (s) return s;

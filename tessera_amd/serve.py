@@ -15,7 +15,8 @@ class Request(ctypes.Structure):
 
 
 class Result(ctypes.Structure):
-    _fields_ = [("id", u64), ("status", ctypes.c_int), ("slot", u32), ("root", ctypes.c_uint8 * 32),
+    _fields_ = [("id", u64), ("status", ctypes.c_int), ("slot", u32), ("group", u32),
+                ("root", ctypes.c_uint8 * 32),
                 ("transcript_state", ctypes.c_uint8 * 32), ("transcript_counter", u64),
                 ("final_a", u64 * 2), ("witness_rows", u64), ("sumcheck_vars", u64),
                 ("n_tokens", u32), ("tokens", u32 * MAX_TOKENS), ("t_submit", ctypes.c_double),
@@ -28,7 +29,7 @@ class Result(ctypes.Structure):
 
 
 class ServerConfig(ctypes.Structure):
-    _fields_ = [("slots", u32), ("provers", u32), ("chunk", u32), ("hbm_budget", u64),
+    _fields_ = [("slots", u32), ("provers", u32), ("chunk", u32), ("group", u32), ("hbm_budget", u64),
                 ("model_seed", u64)]
 
 
@@ -60,9 +61,11 @@ def footprint(tokens):
 
 
 class Server:
-    def __init__(self, device=0, slots=64, hbm_budget=0, model_seed=0x7E55E7A, provers=4, chunk=32):
+    def __init__(self, device=0, slots=64, hbm_budget=0, model_seed=0x7E55E7A, provers=4, chunk=32,
+                 group=16):
+        """group: the most decoded requests a prover commits in one launch set (1: one by one)."""
         L = _declare()
-        cfg = ServerConfig(slots, provers, chunk, hbm_budget, model_seed)
+        cfg = ServerConfig(slots, provers, chunk, group, hbm_budget, model_seed)
         h = vp()
         check(L.tsr_server_create(device, ctypes.byref(cfg), ctypes.byref(h)))
         self.handle = h.value

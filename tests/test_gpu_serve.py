@@ -44,6 +44,28 @@ def test_alone_and_in_the_mix_are_equal():
         assert r.key() == alone[rid], rid
 
 
+def test_grouped_commits_equal_alone():
+    """A burst to one prover: the decoded requests waiting for it are committed in shared
+    launches (result.group > 1), and every result equals its plain call's; a server that commits
+    one request at a time (group = 1) gives the same results."""
+    from tessera_amd.serve import Server, workload
+    reqs = workload(24, seed=11, rate_hz=1e9, min_len=16, max_len=300)
+    with Server(slots=24, provers=1, chunk=64, model_seed=SEED) as srv:
+        alone = {rid: srv.prove(rid, p, n).key() for rid, p, n, _ in reqs}
+        for rid, p, n, _ in reqs:
+            srv.submit(rid, p, n)
+        grouped = [srv.wait() for _ in reqs]
+    with Server(slots=24, provers=1, chunk=64, group=1, model_seed=SEED) as srv:
+        for rid, p, n, _ in reqs:
+            srv.submit(rid, p, n)
+        single = [srv.wait() for _ in reqs]
+    assert max(r.group for r in grouped) > 1
+    assert all(r.group == 1 for r in single)
+    for r in grouped + single:
+        assert r.status == 0
+        assert r.key() == alone[r.id], r.id
+
+
 def test_admission_by_budget():
     from tessera_amd.serve import Server, footprint
     budget = footprint(300) + footprint(100) // 2  # room for one 300-token request at a time

@@ -328,42 +328,48 @@ def run_batched(reps=3):
 def run_served(cpu_sample):
     """SERVE_REQUESTS requests, Poisson arrivals at SERVE_RATE_HZ, prompt+output lengths
     U[16,512] (prompt a quarter): the server (continuous batching, 128 slots, 4 prover streams,
-    chunks of 32 positions) vs plain per-request calls on the same arrivals (each starts at the
-    later of its arrival and the previous call's end). Latency = completion - arrival. Capacity:
-    the same requests offered all at once to the server, and once more to a server that commits
-    each request's witness on its own (group = 1) instead of the waiting requests together."""
+    chunks of 32 positions, each proved request's witness codewords and Merkle tree kept resident
+    on the device until the client releases it) vs plain per-request calls on the same arrivals
+    (each starts at the later of its arrival and the previous call's end). Latency = completion -
+    arrival. Capacity: the same requests offered all at once. Both with each request's witness
+    committed on its own (group 1, the default) and with the decoded requests waiting for a
+    prover committed together (group 16)."""
     from tessera_amd.serve import Server, report, run_plain as plain_calls, run_served as served_calls, workload
     reqs = workload(SERVE_REQUESTS, seed=0x5E7E, rate_hz=SERVE_RATE_HZ)
-    with Server(slots=128, provers=4, chunk=32, model_seed=SEED) as srv:
-        warm = workload(8, seed=1, rate_hz=1e6)
-        served_calls(srv, warm)
-        res_s, wall_s, t0 = served_calls(srv, reqs)
-        served = report(res_s, reqs, wall_s, t0)
-        served["max_in_flight"] = srv.stats()[0]
-        # capacity: the same requests all offered at once
-        burst = [(rid + 10_000, p, n, 0.0) for rid, p, n, _ in reqs]
-        res_b, wall_b, t0_b = served_calls(srv, burst)
-        capacity = report(res_b, burst, wall_b, t0_b)
-        served["mean_group"] = float(np.mean([r.group for r in res_s.values()]))
-        capacity["mean_group"] = float(np.mean([r.group for r in res_b.values()]))
-        res_p, lat_p, wall_p = plain_calls(srv, reqs)
-    # the capacity burst again with every request's witness committed on its own (group = 1)
-    with Server(slots=128, provers=4, chunk=32, group=1, model_seed=SEED) as srv:
-        served_calls(srv, warm)
-        res_u, wall_u, t0_u = served_calls(srv, burst)
-        capacity_single = report(res_u, burst, wall_u, t0_u)
+    burst = [(rid + 10_000, p, n, 0.0) for rid, p, n, _ in reqs]
+    warm = workload(8, seed=1, rate_hz=1e6)
+    runs = {}
+    for group in (1, 16):
+        with Server(slots=128, provers=4, chunk=32, group=group, resident=True, model_seed=SEED) as srv:
+            for phase, wl in (("warm", warm), ("served", reqs), ("capacity", burst)):
+                res, wall, t0 = served_calls(srv, wl)
+                rep = report(res, wl, wall, t0)
+                rep["mean_group"] = float(np.mean([r.group for r in res.values()]))
+                rep["max_in_flight"] = srv.stats()[0]
+                rep["resident_gib"] = round(srv.resident()[1] / 2**30, 2)
+                for rid in res:
+                    srv.release(rid)
+                runs[(group, phase)] = (rep, res)
+            if group == 1:
+                res_p, lat_p, wall_p = plain_calls(srv, reqs)
     p50, p99 = np.percentile(lat_p, 50), np.percentile(lat_p, 99)
     service = [r.t_done - r.t_admit for r in res_p.values()]
     plain = {"proofs_per_s": len(res_p) / wall_p, "p50_ms": 1e3 * p50, "p99_ms": 1e3 * p99,
              "service_p50_ms": 1e3 * float(np.median(service)), "wall_s": wall_p}
-    equal = all(res_s[i].status == 0 and res_s[i].key() == res_p[i].key() for i in res_p)
-    equal = equal and all(res_u[i].status == 0 and res_u[i].key() == res_b[i].key() for i in res_b)
+    equal = True
+    for group in (1, 16):
+        res_s, res_b = runs[(group, "served")][1], runs[(group, "capacity")][1]
+        equal = equal and all(res_s[i].status == 0 and res_s[i].key() == res_p[i].key() for i in res_p)
+        equal = equal and all(res_b[i].status == 0 and res_b[i].key() == runs[(1, "capacity")][1][i].key()
+                              for i in res_b)
+    rnd = lambda d: {k: round(v, 3) if isinstance(v, float) else v for k, v in d.items()}
+    served, capacity = runs[(1, "served")][0], runs[(1, "capacity")][0]
     out = {"rate_hz": SERVE_RATE_HZ, "requests": len(reqs),
-           "served": {k: round(v, 3) if isinstance(v, float) else v for k, v in served.items()},
-           "plain": {k: round(v, 3) for k, v in plain.items()},
-           "capacity": {k: round(v, 3) if isinstance(v, float) else v for k, v in capacity.items()},
-           "capacity_commits_one_by_one": {k: round(v, 3) if isinstance(v, float) else v
-                                           for k, v in capacity_single.items()},
+           "served": rnd(served),
+           "plain": rnd(plain),
+           "capacity": rnd(capacity),
+           "served_commits_grouped": rnd(runs[(16, "served")][0]),
+           "capacity_commits_grouped": rnd(runs[(16, "capacity")][0]),
            "speedup_proofs_per_s": round(served["proofs_per_s"] / plain["proofs_per_s"], 3),
            "capacity_vs_plain": round(capacity["proofs_per_s"] / plain["proofs_per_s"], 3),
            "results_equal": bool(equal)}

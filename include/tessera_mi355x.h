@@ -297,10 +297,13 @@ typedef struct {
     uint32_t slots;       /* requests in flight at once, decoding or proving (1..1024) */
     uint32_t provers;     /* prover threads, one stream each (0 = 4) */
     uint32_t chunk;       /* positions a decoding request advances per decode launch (0 = 32) */
-    uint32_t group;       /* most decoded requests a prover commits in one launch set (0 = 16,
-                           * 1 = each request on its own) */
+    uint32_t group;       /* most decoded requests a prover commits in one launch set (0 = 1:
+                           * each request on its own) */
     uint64_t hbm_budget;  /* admission budget in bytes for the requests in flight (0 = none) */
     uint64_t model_seed;  /* weights: SplitMix64 words mod 255, minus 127 */
+    uint32_t resident;    /* 1: a proved request's witness codewords and Merkle tree stay in device
+                           * memory, holding their bytes of hbm_budget, until tsr_server_release;
+                           * tsr_server_open answers column openings from them */
 } tsr_server_config;
 typedef struct {
     uint64_t id;           /* also the proof's session id */
@@ -339,6 +342,17 @@ tsr_status tsr_server_stats(tsr_server *srv, uint32_t *max_active, uint64_t *in_
 /* The plain per-request call: proves one request synchronously on the calling thread, on the
  * server's own context (concurrent calls take turns). */
 tsr_status tsr_server_prove(tsr_server *srv, const tsr_request *req, tsr_result *out);
+/* Opens columns of a resident request's witness commitment: as tsr_commitment_answer (sorted
+ * distinct columns, each column's symbols of every witness row, the canonical multiproof), on
+ * the server's own context. DECLINED when the request has no resident commitment. */
+tsr_status tsr_server_open(tsr_server *srv, uint64_t id, const uint64_t *queries, uint64_t q,
+                           uint64_t *distinct_out, uint64_t distinct_cap, uint64_t *columns_out,
+                           uint8_t *siblings_out, uint64_t siblings_cap, uint64_t *n_distinct,
+                           uint64_t *n_siblings);
+/* Frees a resident commitment and gives its bytes back to the budget (DECLINED: none). */
+tsr_status tsr_server_release(tsr_server *srv, uint64_t id);
+/* The resident commitments and the device bytes they hold. */
+tsr_status tsr_server_resident(tsr_server *srv, uint32_t *count, uint64_t *bytes);
 /* The admission footprint of a request of `tokens` tokens, in bytes. */
 uint64_t tsr_server_footprint(uint32_t tokens);
 /* Seconds since the server started (the clock of the results' timestamps). */

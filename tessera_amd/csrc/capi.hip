@@ -17,19 +17,6 @@
 
 struct tsr_ctx : tsr::Ctx {};
 
-struct tsr_commitment {
-    tsr::Ctx *ctx = nullptr;
-    tsr_commitment_params params{};
-    uint64_t count = 0, n = 0;
-    tsr::DevBuf<uint64_t> codewords;
-    tsr::DevBuf<uint32_t> nodes;  // (2n-1) x 8 words
-    uint8_t root[32];
-    // A masked (advice) commitment keeps its rows (N x k) and masks (N x P) for its openings;
-    // they and the codewords are secret-class and zeroed when it is freed.
-    tsr::DevBuf<uint64_t> rows, masks;
-    bool secret = false;
-};
-
 struct tsr_chains {
     tsr::Ctx *ctx = nullptr;
     uint64_t columns = 0, block = 0;

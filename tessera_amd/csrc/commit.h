@@ -56,3 +56,19 @@ void launch_merkle_level_multi(Ctx *ctx, const MatDesc *mats, const BlockRef *re
 void launch_gather_roots(const MatDesc *mats, uint32_t count, uint32_t *roots, hipStream_t s);
 
 }  // namespace tsr
+
+// A commitment handle (tsr_commit_unmasked, tsr_commit, tsr_commit_from_encoded): the codewords
+// and the tree stay on the device for its openings. `ctx` is the context whose stream, pool and
+// pinned block its calls use; each buffer goes back to the pool it came from.
+struct tsr_commitment {
+    tsr::Ctx *ctx = nullptr;
+    tsr_commitment_params params{};
+    uint64_t count = 0, n = 0;
+    tsr::DevBuf<uint64_t> codewords;
+    tsr::DevBuf<uint32_t> nodes;  // (2n-1) x 8 words
+    uint8_t root[32];
+    // A masked (advice) commitment keeps its rows (N x k) and masks (N x P) for its openings;
+    // they and the codewords are secret-class and zeroed when it is freed.
+    tsr::DevBuf<uint64_t> rows, masks;
+    bool secret = false;
+};

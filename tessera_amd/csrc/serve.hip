@@ -136,19 +136,29 @@ struct ReqBufs {
 constexpr uint32_t SERVE_LOG_R = 2, SERVE_BLOCK = 32;
 constexpr uint8_t SERVE_COMMITMENT_ID_BYTE = 0x5e;  // commitment_id = 32 bytes of 0x5e
 
-// Step 2 of one decoded request: tsr_commit_unmasked of its witness rows, the root into `root`.
-static Status commit_one(Ctx *ctx, const ReqBufs &b, uint8_t root[32]) {
+static tsr_commitment_params serve_params() {
     tsr_commitment_params params{};
     params.log2_k = SERVE_LOG_K;
     params.log2_inv_rate = SERVE_LOG_R;
     params.mask_coeffs = 0;
     params.row_block = SERVE_BLOCK;
     memset(params.commitment_id, SERVE_COMMITMENT_ID_BYTE, 32);
+    return params;
+}
+
+// A request's witness commitment, freed with tsr_commitment_free.
+struct CommitmentFree {
+    void operator()(tsr_commitment *c) const { tsr_commitment_free(c); }
+};
+using Commitment = std::unique_ptr<tsr_commitment, CommitmentFree>;
+
+// Step 2 of one decoded request: tsr_commit_unmasked of its witness rows.
+static Status commit_one(Ctx *ctx, const ReqBufs &b, Commitment &out) {
+    const tsr_commitment_params params = serve_params();
     tsr_commitment *c = nullptr;
     tsr_status cs = tsr_commit_unmasked((tsr_ctx *)ctx, &params, b.wit.ptr, b.rows, &c);
     if (cs != TSR_OK) return Status::fail(cs, std::string("serve commit: ") + tsr_last_error());
-    tsr_commitment_root(c, root);
-    tsr_commitment_free(c);
+    out.reset(c);
     return Status::ok();
 }
 
@@ -158,7 +168,7 @@ static Status commit_one(Ctx *ctx, const ReqBufs &b, uint8_t root[32]) {
 // request, one launch per Merkle level hashes that level of every tree, and the roots come back
 // in one copy. The small per-request commitments thus share their launches instead of paying
 // a launch sequence and a synchronization each.
-static Status commit_group(Ctx *ctx, const std::vector<const ReqBufs *> &reqs, uint8_t (*roots)[32]) {
+static Status commit_group(Ctx *ctx, const std::vector<const ReqBufs *> &reqs, std::vector<Commitment> &out) {
     hipStream_t s = ctx->stream;
     const uint32_t count = (uint32_t)reqs.size();
     const uint64_t k = 1ull << SERVE_LOG_K, n = k << SERVE_LOG_R;
@@ -231,7 +241,19 @@ static Status commit_group(Ctx *ctx, const std::vector<const ReqBufs *> &reqs, u
     // the readback reuses the pinned block: the stream has finished the upload before it
     TSR_HIP(hipMemcpyAsync(host, blob.ptr + o_roots, 32 * count, hipMemcpyDeviceToHost, s));
     TSR_HIP(hipStreamSynchronize(s));
-    memcpy(roots, host, 32 * count);
+    // each request's codewords and tree become its commitment, as tsr_commit_unmasked's
+    out.clear();
+    for (uint32_t i = 0; i < count; i++) {
+        Commitment c(new tsr_commitment());
+        c->ctx = ctx;
+        c->params = serve_params();
+        c->count = reqs[i]->rows;
+        c->n = n;
+        c->codewords = std::move(cw[i]);
+        c->nodes = std::move(nodes[i]);
+        memcpy(c->root, host + 32 * (size_t)i, 32);
+        out.push_back(std::move(c));
+    }
     return Status::ok();
 }
 
@@ -292,11 +314,21 @@ static Status prove_committed(Ctx *ctx, const tsr_request &req, const ReqBufs &b
     return Status::ok();
 }
 
-// Steps 2-4 of a decoded request on its own.
-static Status prove_decoded(Ctx *ctx, const tsr_request &req, const ReqBufs &b, tsr_result &out) {
-    Status st = commit_one(ctx, b, out.root);
-    return st ? st : prove_committed(ctx, req, b, out);
+// Steps 2-4 of a decoded request on its own; its commitment into `com`.
+static Status prove_decoded(Ctx *ctx, const tsr_request &req, const ReqBufs &b, tsr_result &out,
+                            Commitment &com) {
+    Status st = commit_one(ctx, b, com);
+    if (st) return st;
+    memcpy(out.root, com->root, 32);
+    return prove_committed(ctx, req, b, out);
 }
+
+// A proved request's witness commitment (codewords and tree) kept on the device for openings
+// until it is released; `bytes` is what it holds of the admission budget.
+struct Resident {
+    Commitment c;
+    uint64_t bytes = 0;
+};
 
 // Where a request is, for tsr_server_cancel.
 enum JobState { JOB_QUEUED, JOB_DECODING, JOB_DECODED, JOB_PROVING };
@@ -316,7 +348,7 @@ struct Job {
 struct tsr_server {
     std::chrono::steady_clock::time_point t0;
     int device = 0;
-    uint32_t slots = 1, provers = 1, chunk = 32, group = 16;
+    uint32_t slots = 1, provers = 1, chunk = 32, group = 1;
     uint64_t budget = 0, in_flight_bytes = 0;
     tsr_ctx *owner = nullptr;  // holds the weights; runs tsr_server_prove
     tsr_ctx *dec = nullptr;    // the decoder's context: its pool holds the requests' buffers
@@ -332,14 +364,18 @@ struct tsr_server {
     std::deque<tsr_result> done;
     std::map<uint64_t, int> state;  // JobState of every request in flight, by id
     std::set<uint64_t> cancel;      // decoding requests to end after the current decode launch
+    bool keep = false;              // resident commitments (tsr_server_open, tsr_server_release)
+    std::map<uint64_t, std::shared_ptr<tsr::Resident>> resident;
+    uint64_t resident_bytes = 0;
     bool stop = false, decoder_done = false;
     uint32_t in_flight = 0, max_active = 0;
 };
 
 namespace tsr {
 
-// Ends a job: its result goes to the done queue and its budget and slot are released.
-static void finish(tsr_server *srv, Job &job, const tsr_result *filled) {
+// Ends a job: its result goes to the done queue and its budget and slot are released. A
+// commitment kept resident holds its own bytes of the budget instead, until released.
+static void finish(tsr_server *srv, Job &job, const tsr_result *filled, Commitment keep = nullptr) {
     tsr_result res{};
     if (filled) res = *filled;
     res.id = job.req.id;
@@ -352,6 +388,16 @@ static void finish(tsr_server *srv, Job &job, const tsr_result *filled) {
         std::lock_guard<std::mutex> lk(srv->mu);
         srv->in_flight_bytes -= job.foot;
         srv->in_flight--;
+        if (keep && res.status == TSR_OK) {
+            auto r = std::make_shared<Resident>();
+            r->bytes = 8 * keep->codewords.len + 4 * keep->nodes.len;
+            // openings run on the server's own context (tsr_server_open); the buffers go back to
+            // the prover's pool when released
+            keep->ctx = (Ctx *)srv->owner;
+            r->c = std::move(keep);
+            srv->resident_bytes += r->bytes;
+            srv->resident[res.id] = std::move(r);
+        }
         srv->state.erase(job.req.id);
         srv->cancel.erase(job.req.id);
         srv->done.push_back(res);
@@ -374,7 +420,8 @@ static void decoder_loop(tsr_server *srv) {
             auto admissible = [&] {
                 if (srv->queue.empty() || srv->in_flight >= srv->slots) return false;
                 const uint64_t f = srv->queue.front()->foot;
-                return srv->budget == 0 || srv->in_flight_bytes + f <= srv->budget || srv->in_flight == 0;
+                return srv->budget == 0 || srv->in_flight_bytes + srv->resident_bytes + f <= srv->budget ||
+                       srv->in_flight == 0;
             };
             if (active.empty())
                 srv->cv_work.wait(lk, [&] { return admissible() || (srv->stop && srv->queue.empty()); });
@@ -444,9 +491,13 @@ static void decoder_loop(tsr_server *srv) {
     srv->cv_prove.notify_all();
 }
 
-// A prover takes every decoded request waiting (up to srv->group, oldest first), commits them
+// A prover takes the decoded requests waiting (up to srv->group, oldest first), commits them
 // together (commit_group) and proves them one after another on its stream. A group whose
-// commitment fails (for memory, say) commits its requests one at a time instead.
+// commitment fails (for memory, say) commits its requests one at a time instead. The default
+// group is 1: served witnesses are about 1.4 rows of 2^13 per token, so their commitments keep
+// the GPU busy on their own, and a group's requests would hold their slots until the whole
+// group's commitment is done (measured: capacity 2 240 proofs/s one by one, 2 050 in groups of
+// up to 16, the same latency at 400 requests/s within the runs' spread).
 static void prover_loop(tsr_server *srv, uint32_t slot) {
     Ctx *ctx = (Ctx *)srv->prover_ctxs[slot];
     (void)hipSetDevice(srv->device);
@@ -464,24 +515,29 @@ static void prover_loop(tsr_server *srv, uint32_t slot) {
         }
         const uint32_t g = (uint32_t)jobs.size();
         std::vector<tsr_result> res(g);
+        std::vector<Commitment> coms;
         bool grouped = false;
         if (g > 1) {
             std::vector<const ReqBufs *> reqs;
             for (auto &job : jobs) reqs.push_back(&job->bufs);
-            std::vector<uint8_t> roots(32 * (size_t)g);
-            grouped = !commit_group(ctx, reqs, (uint8_t(*)[32])roots.data());
+            grouped = !commit_group(ctx, reqs, coms);
             (void)hipStreamSynchronize(ctx->stream);  // (after an error) nothing still runs
-            if (grouped)
-                for (uint32_t i = 0; i < g; i++) memcpy(res[i].root, &roots[32 * (size_t)i], 32);
+            if (!grouped) coms.clear();
         }
+        coms.resize(g);
         for (uint32_t i = 0; i < g; i++) {
             Job &job = *jobs[i];
             res[i].slot = slot;
             res[i].group = grouped ? g : 1;
-            job.status = grouped ? prove_committed(ctx, job.req, job.bufs, res[i])
-                                 : prove_decoded(ctx, job.req, job.bufs, res[i]);
+            if (grouped) {
+                memcpy(res[i].root, coms[i]->root, 32);
+                job.status = prove_committed(ctx, job.req, job.bufs, res[i]);
+            } else {
+                job.status = prove_decoded(ctx, job.req, job.bufs, res[i], coms[i]);
+            }
             (void)hipStreamSynchronize(ctx->stream);  // (after an error) nothing still reads the buffers
-            finish(srv, job, &res[i]);
+            finish(srv, job, &res[i], srv->keep ? std::move(coms[i]) : Commitment());
+            coms[i].reset();
         }
     }
 }
@@ -516,7 +572,8 @@ tsr_status tsr_server_create(int device, const tsr_server_config *cfg, tsr_serve
     srv->slots = cfg->slots;
     srv->provers = cfg->provers ? cfg->provers : 4;
     srv->chunk = cfg->chunk ? cfg->chunk : 32;
-    srv->group = cfg->group ? cfg->group : 16;
+    srv->group = cfg->group ? cfg->group : 1;
+    srv->keep = cfg->resident != 0;
     srv->budget = cfg->hbm_budget;
     tsr_status s = tsr_ctx_create(device, &srv->owner);
     if (s) return s;
@@ -551,6 +608,9 @@ tsr_status tsr_server_submit(tsr_server *srv, const tsr_request *req) {
         std::lock_guard<std::mutex> lk(srv->mu);
         if (srv->state.count(req->id))
             return Status::fail(TSR_BAD_ARG, "a request with id " + std::to_string(req->id) + " is in flight").code;
+        if (srv->resident.count(req->id))
+            return Status::fail(TSR_BAD_ARG, "request " + std::to_string(req->id) +
+                                                 "'s commitment is resident: release it first").code;
         job->t_submit = now_s(srv->t0);
         srv->state[req->id] = JOB_QUEUED;
         srv->queue.push_back(std::move(job));
@@ -646,11 +706,58 @@ tsr_status tsr_server_prove(tsr_server *srv, const tsr_request *req, tsr_result 
     if (!st) st = model_decode(ctx, srv->weights, task.ptr, 1, ctx->stream);
     if (!st) {
         b.kv = DevBuf<int32_t>();
-        st = prove_decoded(ctx, *req, b, *out);
+        Commitment com;
+        st = prove_decoded(ctx, *req, b, *out, com);
     }
     out->t_done = now_s(srv->t0);
     out->status = st.code;
     return st.code;
+}
+
+tsr_status tsr_server_open(tsr_server *srv, uint64_t id, const uint64_t *queries, uint64_t q,
+                           uint64_t *distinct_out, uint64_t distinct_cap, uint64_t *columns_out,
+                           uint8_t *siblings_out, uint64_t siblings_cap, uint64_t *n_distinct,
+                           uint64_t *n_siblings) {
+    if (!srv) return Status::fail(TSR_BAD_ARG, "null server").code;
+    std::shared_ptr<Resident> r;
+    {
+        std::lock_guard<std::mutex> lk(srv->mu);
+        auto it = srv->resident.find(id);
+        if (it == srv->resident.end())
+            return Status::fail(TSR_DECLINED, "no resident commitment for request " + std::to_string(id)).code;
+        r = it->second;  // kept alive through the call even if released meanwhile
+    }
+    // on the server's own context, which tsr_server_prove calls share
+    std::lock_guard<std::mutex> lk(srv->prove_mu);
+    hipError_t e = hipSetDevice(srv->device);
+    if (e != hipSuccess) return hip_fail(e, "hipSetDevice").code;
+    return tsr_commitment_answer(r->c.get(), queries, q, distinct_out, distinct_cap, columns_out,
+                                 siblings_out, siblings_cap, n_distinct, n_siblings);
+}
+
+tsr_status tsr_server_release(tsr_server *srv, uint64_t id) {
+    if (!srv) return Status::fail(TSR_BAD_ARG, "null server").code;
+    std::shared_ptr<Resident> r;
+    {
+        std::lock_guard<std::mutex> lk(srv->mu);
+        auto it = srv->resident.find(id);
+        if (it == srv->resident.end())
+            return Status::fail(TSR_DECLINED, "no resident commitment for request " + std::to_string(id)).code;
+        r = std::move(it->second);
+        srv->resident.erase(it);
+        srv->resident_bytes -= r->bytes;
+    }
+    srv->cv_work.notify_all();  // its bytes may let a waiting request in
+    r.reset();                  // the last holder frees the buffers
+    return TSR_OK;
+}
+
+tsr_status tsr_server_resident(tsr_server *srv, uint32_t *count, uint64_t *bytes) {
+    if (!srv) return Status::fail(TSR_BAD_ARG, "null server").code;
+    std::lock_guard<std::mutex> lk(srv->mu);
+    if (count) *count = (uint32_t)srv->resident.size();
+    if (bytes) *bytes = srv->resident_bytes;
+    return TSR_OK;
 }
 
 uint64_t tsr_server_footprint(uint32_t tokens) { return request_footprint(tokens); }
@@ -666,6 +773,7 @@ void tsr_server_destroy(tsr_server *srv) {
     srv->cv_work.notify_all();
     if (srv->decoder.joinable()) srv->decoder.join();
     for (auto &w : srv->workers) w.join();
+    srv->resident.clear();  // back to the provers' pools before the contexts go
     for (auto *c : srv->prover_ctxs) tsr_ctx_destroy(c);
     tsr_ctx_destroy(srv->dec);
     if (srv->weights) tsr_free(srv->owner, srv->weights);

@@ -161,6 +161,9 @@ _SIGS = {
     "tsr_server_wait": (i32, [vp, vp, u32]),
     "tsr_server_stats": (i32, [vp, ctypes.POINTER(u32), ctypes.POINTER(u64)]),
     "tsr_server_prove": (i32, [vp, vp, vp]),
+    "tsr_server_open": (i32, [vp, u64, vp, u64, vp, u64, vp, vp, u64, ctypes.POINTER(u64), ctypes.POINTER(u64)]),
+    "tsr_server_release": (i32, [vp, u64]),
+    "tsr_server_resident": (i32, [vp, ctypes.POINTER(u32), ctypes.POINTER(u64)]),
     "tsr_server_footprint": (u64, [u32]),
     "tsr_server_clock": (ctypes.c_double, [vp]),
     "tsr_server_destroy": (None, [vp]),

@@ -30,7 +30,7 @@ class Result(ctypes.Structure):
 
 class ServerConfig(ctypes.Structure):
     _fields_ = [("slots", u32), ("provers", u32), ("chunk", u32), ("group", u32), ("hbm_budget", u64),
-                ("model_seed", u64)]
+                ("model_seed", u64), ("resident", u32)]
 
 
 def _declare():
@@ -47,6 +47,13 @@ def _declare():
     L.tsr_server_cancel.argtypes = [vp, u64]
     L.tsr_server_stats.restype = ctypes.c_int
     L.tsr_server_stats.argtypes = [vp, ctypes.POINTER(u32), ctypes.POINTER(u64)]
+    L.tsr_server_open.restype = ctypes.c_int
+    L.tsr_server_open.argtypes = [vp, u64, vp, u64, vp, u64, vp, vp, u64, ctypes.POINTER(u64),
+                                  ctypes.POINTER(u64)]
+    L.tsr_server_release.restype = ctypes.c_int
+    L.tsr_server_release.argtypes = [vp, u64]
+    L.tsr_server_resident.restype = ctypes.c_int
+    L.tsr_server_resident.argtypes = [vp, ctypes.POINTER(u32), ctypes.POINTER(u64)]
     L.tsr_server_footprint.restype = u64
     L.tsr_server_footprint.argtypes = [u32]
     L.tsr_server_clock.restype = ctypes.c_double
@@ -62,10 +69,12 @@ def footprint(tokens):
 
 class Server:
     def __init__(self, device=0, slots=64, hbm_budget=0, model_seed=0x7E55E7A, provers=4, chunk=32,
-                 group=16):
-        """group: the most decoded requests a prover commits in one launch set (1: one by one)."""
+                 group=1, resident=False):
+        """group: the most decoded requests a prover commits in one launch set (1: one by one).
+        resident: proved requests keep their witness codewords and Merkle tree on the device
+        (within hbm_budget) for open() until release()."""
         L = _declare()
-        cfg = ServerConfig(slots, provers, chunk, group, hbm_budget, model_seed)
+        cfg = ServerConfig(slots, provers, chunk, group, hbm_budget, model_seed, 1 if resident else 0)
         h = vp()
         check(L.tsr_server_create(device, ctypes.byref(cfg), ctypes.byref(h)))
         self.handle = h.value
@@ -100,6 +109,41 @@ class Server:
         r = Result()
         check(lib().tsr_server_prove(self.handle, ctypes.byref(req), ctypes.byref(r)))
         return r
+
+    def open(self, result, queries):
+        """Columns of a resident request's witness commitment (tsr_server_open), as
+        ReferenceCommitment.answer gives them, for the finished Result `result`."""
+        from .pcs import ColumnAnswer
+        rows = int(result.witness_rows)
+        q = np.ascontiguousarray(np.array(queries, dtype=np.uint64))
+        cap_d = max(1, len(set(int(x) for x in q)))
+        cap_s = 64 * cap_d + 64
+        distinct = np.zeros(cap_d, dtype=np.uint64)
+        cols = np.zeros(cap_d * max(rows, 1), dtype=np.uint64)
+        sib = ctypes.create_string_buffer(32 * cap_s)
+        nd, ns = u64(), u64()
+        check(lib().tsr_server_open(self.handle, int(result.id), q.ctypes.data if q.size else None, q.size,
+                                    distinct.ctypes.data, cap_d, cols.ctypes.data, sib, cap_s,
+                                    ctypes.byref(nd), ctypes.byref(ns)))
+        d = nd.value
+        raw = sib.raw
+        return ColumnAnswer([int(x) for x in distinct[:d]], [cols[i * rows:(i + 1) * rows].copy() for i in range(d)],
+                            [raw[32 * i:32 * i + 32] for i in range(ns.value)])
+
+    def release(self, rid):
+        """Frees request rid's resident commitment: True, or False when it has none."""
+        from .lib import DECLINED
+        rc = lib().tsr_server_release(self.handle, rid)
+        if rc == DECLINED:
+            return False
+        check(rc)
+        return True
+
+    def resident(self):
+        """(resident commitments, device bytes they hold)."""
+        c, b = u32(), u64()
+        check(lib().tsr_server_resident(self.handle, ctypes.byref(c), ctypes.byref(b)))
+        return int(c.value), int(b.value)
 
     def clock(self):
         return float(lib().tsr_server_clock(self.handle))

@@ -46,11 +46,11 @@ def test_alone_and_in_the_mix_are_equal():
 
 def test_grouped_commits_equal_alone():
     """A burst to one prover: the decoded requests waiting for it are committed in shared
-    launches (result.group > 1), and every result equals its plain call's; a server that commits
-    one request at a time (group = 1) gives the same results."""
+    launches (result.group > 1, group 16), and every result equals its plain call's; a server
+    that commits one request at a time (group 1, the default) gives the same results."""
     from tessera_amd.serve import Server, workload
     reqs = workload(24, seed=11, rate_hz=1e9, min_len=16, max_len=300)
-    with Server(slots=24, provers=1, chunk=64, model_seed=SEED) as srv:
+    with Server(slots=24, provers=1, chunk=64, group=16, model_seed=SEED) as srv:
         alone = {rid: srv.prove(rid, p, n).key() for rid, p, n, _ in reqs}
         for rid, p, n, _ in reqs:
             srv.submit(rid, p, n)
@@ -64,6 +64,56 @@ def test_grouped_commits_equal_alone():
     for r in grouped + single:
         assert r.status == 0
         assert r.key() == alone[r.id], r.id
+
+
+def _witness_commitment(weights, prompt, total):
+    """The oracle's codewords and tree of a request's witness (rows of 2^13, rate 1/4, B = 32)."""
+    _, wit, _ = O.model_decode(weights, prompt, total)
+    k = 1 << 13
+    rows = -(-wit.size // k)
+    padded = np.zeros(rows * k, dtype=np.uint64)
+    padded[:wit.size] = wit
+    return O.commit_unmasked(padded, 13, 2, 32, bytes([0x5E] * 32), threads=8)
+
+
+def test_resident_commitments_open_and_release(weights):
+    """With resident commitments each proved request's witness codewords and Merkle tree stay on
+    the device: openings equal the oracle's answer over the same witness (grouped and single
+    commits alike), their bytes are held until released, a released request can no longer be
+    opened, and a resident request's id is not reused until released."""
+    from tessera_amd.lib import Declined, TsrError
+    from tessera_amd.serve import Server
+    rng = np.random.default_rng(21)
+    reqs = [(200 + i, rng.integers(0, 4096, 3).tolist(), n) for i, n in enumerate([40, 130, 260, 20])]
+    nsym = 1 << 15
+    queries = [0, 7, nsym - 1, 12345, 7, 30000]
+    for group in (16, 1):
+        with Server(slots=4, provers=1, group=group, resident=True, model_seed=SEED) as srv:
+            for rid, p, n in reqs:
+                srv.submit(rid, p, n)
+            res = {r.id: r for r in [srv.wait() for _ in reqs]}
+            count, held = srv.resident()
+            assert count == len(reqs) and held > 0
+            for rid, p, n in reqs:
+                r = res[rid]
+                assert r.status == 0
+                cw, nodes = _witness_commitment(weights, p, len(p) + n)
+                assert bytes(r.root) == nodes[-32:]
+                got = srv.open(r, queries)
+                idx, cols, sib = O.answer(cw, nodes, nsym, queries)
+                assert got.indices == idx and got.siblings == sib
+                assert all(np.array_equal(a, b) for a, b in zip(got.columns, cols))
+            with pytest.raises(TsrError):
+                srv.submit(reqs[1][0], [1], 5)
+            assert srv.release(reqs[0][0])
+            assert not srv.release(reqs[0][0])
+            with pytest.raises(Declined):
+                srv.open(res[reqs[0][0]], [1])
+            count2, held2 = srv.resident()
+            assert count2 == len(reqs) - 1 and held2 < held
+            for rid, _, _ in reqs[1:]:
+                assert srv.release(rid)
+            assert srv.resident() == (0, 0)
 
 
 def test_admission_by_budget():

@@ -450,9 +450,9 @@ def traffic_tables(fetch_head, fetch_copy, write_head, write_copy, out_json, out
         fv = sum(fetch[name]) / len(fetch[name]) * f_fac
         wv = sum(write.get(name, [0])) / max(1, len(write.get(name, [0]))) * w_fac
         if name.startswith("k_"):
-            launches = len(fetch[name])  # one headline step (--steps 1 --warmup 0)
-            table[name] = {"rows": ROWS // launches if name.startswith(("k_tile_pass", "k_encode_mid")) else ROWS,
-                           "bytes": round(fv + wv)}
+            # every launch of the headline commit covers all its rows; the run makes one launch
+            # per timed step and per instrumented step, averaged above
+            table[name] = {"rows": ROWS, "bytes": round(fv + wv)}
         lines.append(f"| `{name}` | {len(fetch[name])} | {fv / 2**30:.4f} | {wv / 2**30:.4f} | "
                      f"{(fv + wv) / 2**30:.4f} |")
     with open(out_json, "w") as f:

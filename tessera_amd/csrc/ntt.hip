@@ -498,6 +498,77 @@ k_encode_mid(const uint64_t *__restrict__ in, uint64_t in_stride, uint64_t *__re
     for (uint32_t e = threadIdx.x; e < Tn; e += MID_THREADS) dst[nbase + e] = lds[e];
 }
 
+// The unmasked fused middle over RPW = 2^LOGR rows per workgroup (the values of k_encode_mid
+// with P = 0). One row's 2^TK-value tile has 2^(TK-3) radix-8 groups per CT round, a quarter of
+// the threads at rate 1/4, and the SIMDs idle through those rounds (counted: 75% VALU-busy over
+// the kernel). RPW rows' tiles share the LDS tile in the CT stages, interleaved (word
+// (t << LOGR) | r holds row r's value t), so RPW times the threads have a group there. The
+// spread values of all RPW rows then wait in registers (RPW x 8 / 2^(TN-TK) words per thread)
+// while the GS stages run on one row's codeword tile at a time. Two rows: 64 VGPRs (4 spilled),
+// two workgroups per CU, 5% faster than one row at 256 x 2^20 rate 1/4; four rows spill 22
+// registers and are slower.
+template <int TK, int TN, int LOGR>
+__global__ void __launch_bounds__(MID_THREADS) __attribute__((amdgpu_waves_per_eu(8, 8)))
+k_encode_mid_rows(const uint64_t *__restrict__ in, uint64_t in_stride, uint64_t *__restrict__ cw,
+                  uint64_t cw_stride, const uint64_t *__restrict__ zk_inv, const uint64_t *__restrict__ zn,
+                  const uint64_t *__restrict__ vk_inv, const uint64_t *__restrict__ vn,
+                  const uint64_t *__restrict__ spread, const uint64_t *const *in_rows,
+                  uint64_t *const *cw_rows, uint64_t c_off) {
+    constexpr int RPW = 1 << LOGR, S = 1 << (TN - TK), PERG = 8 / S;
+    constexpr uint32_t Tk = 1u << TK, Tn = 1u << TN;
+    static_assert(S >= 2 && S <= 4, "rate 1/2 or 1/4");
+    static_assert((Tk << LOGR) <= Tn, "the rows' tiles fit the codeword tile");
+    static_assert((1u << (TN - 3)) == MID_THREADS, "one first-round GS group per thread");
+    extern __shared__ uint64_t lds[];
+    const uint64_t row0 = (uint64_t)blockIdx.x * RPW, c = blockIdx.y + c_off;
+    const uint64_t kbase = c << TK, nbase = c << TN;
+    // each wave reads runs of 64 / RPW consecutive values from each of the RPW rows
+    for (uint32_t e = threadIdx.x; e < (Tk << LOGR); e += MID_THREADS) {
+        const uint32_t r = e & (RPW - 1), t = e >> LOGR;
+        const uint64_t *src = in_rows ? in_rows[row0 + r] : in + (row0 + r) * in_stride;
+        lds[e] = src[kbase + t];
+    }
+    __syncthreads();
+    tile_stages_ct<false, true, TK, LOGR, 0, false, MID_THREADS>(lds, c, zk_inv, vk_inv);
+
+    // Thread g's first GS group (positions 8g..8g+7 of the codeword tile) holds the spread values
+    // of k-positions t = g PERG + jj at positions S jj, zeros elsewhere.
+    const uint32_t g = threadIdx.x;
+    uint64_t v[RPW][PERG];
+#pragma unroll
+    for (int jj = 0; jj < PERG; jj++) {
+        const uint32_t t = g * PERG + jj;
+        const uint64_t f = spread[kbase + t];
+#pragma unroll
+        for (int r = 0; r < RPW; r++) v[r][jj] = gl::mul(lds[(t << LOGR) | r], f);
+    }
+    constexpr uint32_t ZEROS = S == 2 ? 0xAAu : 0xEEu;
+    // One row per iteration, not unrolled: the row's values are always v[0] (the held values
+    // move down a row after each iteration).
+#pragma unroll 1
+    for (int r = 0; r < RPW; r++) {
+        __syncthreads();  // the CT tile, or the previous row's codeword tile, has been read
+        // an empty asm statement the compiler must assume changes the table pointer, so the
+        // twiddle loads stay inside the iteration instead of being hoisted (and held) for all rows
+        const uint64_t *vnr = vn;
+        asm volatile("" : "+s"(vnr));
+        uint64_t x[8];
+#pragma unroll
+        for (int j = 0; j < 8; j++) x[j] = (j % S == 0) ? v[0][j / S] : 0;
+#pragma unroll
+        for (int rr = 0; rr + 1 < RPW; rr++)
+#pragma unroll
+            for (int jj = 0; jj < PERG; jj++) v[rr][jj] = v[rr + 1][jj];
+        round8_regs<true, false, TN, 0, false, ZEROS>(x, g << 3, c, vnr);
+#pragma unroll
+        for (int j = 0; j < 8; j++) lds[(g << 3) + j] = x[j];
+        __syncthreads();
+        tile_stages_ct<true, false, TN, 0, 3, false, MID_THREADS>(lds, c, zn, vnr);
+        uint64_t *dst = cw_rows ? cw_rows[row0 + r] : cw + (row0 + r) * cw_stride;
+        for (uint32_t e = threadIdx.x; e < Tn; e += MID_THREADS) dst[nbase + e] = lds[e];
+    }
+}
+
 // ---------------------------------------------------------------------------------------------
 
 using TilePassFn = void (*)(const uint64_t *, uint64_t *, uint64_t, uint64_t, uint32_t, uint32_t,
@@ -565,6 +636,23 @@ static MidFn mid_fn(uint32_t tk, uint32_t log_r) {
     if (tk == 10 && tn == 12) return k_encode_mid<10, 12>;
     if (tk == 11 && tn == 12) return k_encode_mid<11, 12>;
     return k_encode_mid<0, 0>;
+}
+
+using MidRowsFn = void (*)(const uint64_t *, uint64_t, uint64_t *, uint64_t, const uint64_t *,
+                           const uint64_t *, const uint64_t *, const uint64_t *, const uint64_t *,
+                           const uint64_t *const *, uint64_t *const *, uint64_t);
+
+#ifndef TSR_MID_ROWS_LOG
+#define TSR_MID_ROWS_LOG 1
+#endif
+// The several-rows fused middle (2^TSR_MID_ROWS_LOG rows per workgroup), where one exists:
+// unmasked, tile_log 13, rate 1/4 (at rate 1/2 a thread would hold twice the spread values).
+static MidRowsFn mid_rows_fn(uint32_t tk, uint32_t log_r, uint64_t P, uint32_t *rows_log) {
+    *rows_log = TSR_MID_ROWS_LOG;
+#if TSR_MID_THREADS == 1024 && TSR_MID_ROWS_LOG > 0
+    if (P == 0 && tk == 11 && log_r == 2) return k_encode_mid_rows<11, 13, TSR_MID_ROWS_LOG>;
+#endif
+    return nullptr;
 }
 
 static void bit_reversed_powers(uint64_t base, uint32_t bits, uint64_t scale,
@@ -713,12 +801,25 @@ void Encoder::launch(const uint64_t *src, uint64_t src_stride, const uint64_t *c
         const uint64_t tiles = k >> tk;
         size_t shm = sizeof(uint64_t) << (tk + log_r);
         KernelTimer timer(ctx, "k_encode_mid", stream);
+        // whole groups of rows by the several-rows middle, the rest one row per workgroup
+        uint32_t rows_log = 0;
+        const MidRowsFn fr = mid_rows_fn(tk, log_r, P, &rows_log);
+        const uint64_t rpw = 1ull << rows_log, grouped = fr ? rc - rc % rpw : 0;
         for (uint64_t c0 = 0; c0 < tiles; c0 += 65535) {
             const uint64_t tc = tiles - c0 < 65535 ? tiles - c0 : 65535;
-            hipLaunchKernelGGL(mid_fn(tk, log_r), dim3((unsigned)rc, (unsigned)tc), dim3(MID_THREADS), shm,
-                               stream, src, src_stride, cw, n, log_k, log_r, tk, zk_inv.ptr, zn_fwd.ptr,
-                               vk_inv.ptr, vn_fwd.ptr, spread_tab.ptr, masks, P, pow7_tab.ptr, shift_pow_k, src_rows, cw_rows,
-                               c0);
+            if (grouped)
+                hipLaunchKernelGGL(fr, dim3((unsigned)(grouped / rpw), (unsigned)tc), dim3(MID_THREADS), shm,
+                                   stream, src, src_stride, cw, n, zk_inv.ptr, zn_fwd.ptr, vk_inv.ptr,
+                                   vn_fwd.ptr, spread_tab.ptr, src_rows, cw_rows, c0);
+            if (grouped < rc)
+                hipLaunchKernelGGL(mid_fn(tk, log_r), dim3((unsigned)(rc - grouped), (unsigned)tc),
+                                   dim3(MID_THREADS), shm, stream,
+                                   src_rows ? src : src + grouped * src_stride, src_stride,
+                                   cw_rows ? cw : cw + grouped * n, n, log_k, log_r, tk, zk_inv.ptr,
+                                   zn_fwd.ptr, vk_inv.ptr, vn_fwd.ptr, spread_tab.ptr,
+                                   masks ? masks + grouped * P : masks, P, pow7_tab.ptr, shift_pow_k,
+                                   src_rows ? src_rows + grouped : src_rows,
+                                   cw_rows ? cw_rows + grouped : cw_rows, c0);
         }
     }
     for (auto &ps : gs_passes) {

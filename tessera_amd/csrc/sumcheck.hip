@@ -27,6 +27,7 @@
 #include <stdint.h>
 #include <string.h>
 #include <string>
+#include <type_traits>
 #include <vector>
 #include "common.h"
 #include "devtranscript.h"
@@ -242,21 +243,31 @@ k_sc_round(ScTables t, uint32_t F, uint32_t base_mask, bool first, uint64_t out_
     }
 }
 
-template <int REL, bool UNIT, int NA>
+// Node sums: x y added into an Fp2 sum, or into a wide sum reduced later (gl::wide2).
+__device__ __forceinline__ void accumulate(fp2 &acc, const fp2 &x, const fp2 &y) { acc = gl::add(acc, gl::mul(x, y)); }
+__device__ __forceinline__ void accumulate(gl::wide2 &acc, const fp2 &x, const fp2 &y) { gl::mac(acc, x, y); }
+__device__ __forceinline__ void accumulate_base(fp2 &acc, const fp2 &x, uint64_t y) {
+    acc = gl::add(acc, gl::mul_base(x, y));
+}
+__device__ __forceinline__ void accumulate_base(gl::wide2 &acc, const fp2 &x, uint64_t y) { gl::mac_base(acc, x, y); }
+__device__ __forceinline__ fp2 reduced(const fp2 &x) { return x; }
+__device__ __forceinline__ fp2 reduced(const gl::wide2 &x) { return gl::reduce(x); }
+
+template <int REL, bool UNIT, int NA, typename ACC = fp2>
 __device__ __forceinline__ void sc_eq_nodes(const fp2 *a, const fp2 *b, const fp2 &e,
-                                            const fp2 &lambda, fp2 (&acc)[NA]);
+                                            const fp2 &lambda, ACC (&acc)[NA]);
 
 // One item q of a level round in k_sc_round_eq's modes: binds the tables at q and/or adds E F
 // at z = 0, 1, 2 to acc. LO: the pair's weight is eq_lo[q mod 2^eq_a] alone (the caller applies
 // eq_hi[q >> eq_a] to the sums of a run of pairs that share it, or eq_hi is the single entry 1).
 // NA = 6 (unit numerators only): acc[0..2] take E (D0 + D1) and acc[3..5] E D0 D1, so that
 // lambda multiplies the sums instead of every pair.
-template <int REL, bool UNIT, bool BASE, bool LO = false, int NA = 3>
+template <int REL, bool UNIT, bool BASE, bool LO = false, int NA = 3, typename ACC = fp2>
 __device__ __forceinline__ void sc_item_eq(const ScTables &t, uint32_t base_mask, bool first,
                                            bool bind, uint64_t pairs, const fp2 &r,
                                            const fp2 &lambda, const fp2 *__restrict__ eq_lo,
                                            const fp2 *__restrict__ eq_hi, uint32_t eq_a,
-                                           uint64_t q, fp2 (&acc)[NA]) {
+                                           uint64_t q, ACC (&acc)[NA]) {
     constexpr int F = REL == REL_PRODUCT ? 2 : 4;
     constexpr int K0 = (REL == REL_FRACTION && UNIT) ? 2 : 0;
     auto weight = [&](uint64_t x) {
@@ -279,9 +290,9 @@ __device__ __forceinline__ void sc_item_eq(const ScTables &t, uint32_t base_mask
             const fp2 e = weight(q);
             const uint64_t f0 = gl::mul(a0, a1), f1 = gl::mul(b0, b1);
             const uint64_t f2 = gl::mul(gl::sub(gl::add(b0, b0), a0), gl::sub(gl::add(b1, b1), a1));
-            acc[0] = gl::add(acc[0], gl::mul_base(e, f0));
-            acc[1] = gl::add(acc[1], gl::mul_base(e, f1));
-            acc[2] = gl::add(acc[2], gl::mul_base(e, f2));
+            accumulate_base(acc[0], e, f0);
+            accumulate_base(acc[1], e, f1);
+            accumulate_base(acc[2], e, f2);
             return;
         }
         if (pairs == 0) {
@@ -341,67 +352,72 @@ __device__ __forceinline__ void sc_item_eq(const ScTables &t, uint32_t base_mask
             }
         }
         if (pairs == 0) return;
-        sc_eq_nodes<REL, UNIT, NA>(a, b, e, lambda, acc);
+        sc_eq_nodes<REL, UNIT, NA, ACC>(a, b, e, lambda, acc);
         return;
     }
-    sc_eq_nodes<REL, UNIT, NA>(a, b, weight(q), lambda, acc);
+    sc_eq_nodes<REL, UNIT, NA, ACC>(a, b, weight(q), lambda, acc);
 }
 
 // E F(z) at z = 0, 1, 2 of one pair added to acc (see k_sc_round_eq).
-template <int REL, bool UNIT, int NA>
+template <int REL, bool UNIT, int NA, typename ACC>
 __device__ __forceinline__ void sc_eq_nodes(const fp2 *a, const fp2 *b, const fp2 &e,
-                                            const fp2 &lambda, fp2 (&acc)[NA]) {
+                                            const fp2 &lambda, ACC (&acc)[NA]) {
     auto line2 = [](const fp2 &x, const fp2 &y) { return gl::sub(gl::add(y, y), x); };
-    if constexpr (NA == 6) {
-        static_assert(REL == REL_FRACTION && UNIT, "split sums are for unit numerators");
-        // E (D0 + D1) is linear in z; (E D0) D1 at z = 0, 1, 2: 7 products
-        const fp2 s0 = gl::mul(e, gl::add(a[2], a[3])), s1 = gl::mul(e, gl::add(b[2], b[3]));
-        const fp2 x0 = gl::mul(e, a[2]), x1 = gl::mul(e, b[2]);
-        acc[0] = gl::add(acc[0], s0);
-        acc[1] = gl::add(acc[1], s1);
-        acc[2] = gl::add(acc[2], line2(s0, s1));
-        acc[3] = gl::add(acc[3], gl::mul(x0, a[3]));
-        acc[4] = gl::add(acc[4], gl::mul(x1, b[3]));
-        acc[5] = gl::add(acc[5], gl::mul(line2(x0, x1), line2(a[3], b[3])));
-        return;
-    }
-    fp2 v0, v1, v2;
-    if (REL == REL_PRODUCT) {
+    if constexpr (REL == REL_PRODUCT) {
+        // E folded into T0's line; the three node products go straight into the sums
         const fp2 x0 = gl::mul(e, a[0]), x1 = gl::mul(e, b[0]);
-        v0 = gl::mul(x0, a[1]);
-        v1 = gl::mul(x1, b[1]);
-        v2 = gl::mul(line2(x0, x1), line2(a[1], b[1]));
-    } else if (UNIT) {
+        accumulate(acc[0], x0, a[1]);
+        accumulate(acc[1], x1, b[1]);
+        accumulate(acc[2], line2(x0, x1), line2(a[1], b[1]));
+        return;
+    } else if constexpr (NA == 6) {
+        static_assert(REL == REL_FRACTION && UNIT, "split sums are for unit numerators");
+        // E (D0 + D1) is linear in z: its sums at z = 0, 1 give the sum at z = 2 (the caller
+        // forms 2 acc[1] - acc[0]; acc[2] stays unused); (E D0) D1 at z = 0, 1, 2
+        const fp2 s0 = gl::add(a[2], a[3]), s1 = gl::add(b[2], b[3]);
+        const fp2 x0 = gl::mul(e, a[2]), x1 = gl::mul(e, b[2]);
+        accumulate(acc[0], e, s0);
+        accumulate(acc[1], e, s1);
+        accumulate(acc[3], x0, a[3]);
+        accumulate(acc[4], x1, b[3]);
+        accumulate(acc[5], line2(x0, x1), line2(a[3], b[3]));
+        return;
+    } else if constexpr (UNIT) {
+        // E (D0 + D1) + (lambda E D0) D1
         const fp2 u = gl::mul(lambda, e);
-        const fp2 s0 = gl::mul(e, gl::add(a[2], a[3])), s1 = gl::mul(e, gl::add(b[2], b[3]));
+        const fp2 s0 = gl::add(a[2], a[3]), s1 = gl::add(b[2], b[3]);
         const fp2 p0 = gl::mul(u, a[2]), p1 = gl::mul(u, b[2]);
-        v0 = gl::add(s0, gl::mul(p0, a[3]));
-        v1 = gl::add(s1, gl::mul(p1, b[3]));
-        v2 = gl::add(line2(s0, s1), gl::mul(line2(p0, p1), line2(a[3], b[3])));
+        accumulate(acc[0], e, s0);
+        accumulate(acc[0], p0, a[3]);
+        accumulate(acc[1], e, s1);
+        accumulate(acc[1], p1, b[3]);
+        accumulate(acc[2], e, line2(s0, s1));
+        accumulate(acc[2], line2(p0, p1), line2(a[3], b[3]));
     } else {
+        // (E N0) D1 + (E D0)(N1 + lambda D1)
         const fp2 n0 = gl::mul(e, a[0]), n1 = gl::mul(e, b[0]);
         const fp2 d0 = gl::mul(e, a[2]), d1 = gl::mul(e, b[2]);
         const fp2 w0 = gl::add(a[1], gl::mul(lambda, a[3])), w1 = gl::add(b[1], gl::mul(lambda, b[3]));
-        v0 = gl::add(gl::mul(n0, a[3]), gl::mul(d0, w0));
-        v1 = gl::add(gl::mul(n1, b[3]), gl::mul(d1, w1));
-        v2 = gl::add(gl::mul(line2(n0, n1), line2(a[3], b[3])), gl::mul(line2(d0, d1), line2(w0, w1)));
+        accumulate(acc[0], n0, a[3]);
+        accumulate(acc[0], d0, w0);
+        accumulate(acc[1], n1, b[3]);
+        accumulate(acc[1], d1, w1);
+        accumulate(acc[2], line2(n0, n1), line2(a[3], b[3]));
+        accumulate(acc[2], line2(d0, d1), line2(w0, w1));
     }
-    acc[0] = gl::add(acc[0], v0);
-    acc[1] = gl::add(acc[1], v1);
-    acc[2] = gl::add(acc[2], v2);
 }
 
 // The first round's sums of a fraction level with stored numerators, from the level it proves:
 // F(z) at z = 0, 1 is n + lambda d of the parent nodes 2q, 2q + 1 (the level's values are
 // N0 D1 + N1 D0 and D0 D1 of these very tables), and only z = 2 multiplies the tables' line
 // values: 8 Fp2 products per pair instead of 12.
-template <bool LO>
+template <bool LO, typename ACC = fp2>
 __device__ __forceinline__ void sc_item_eq_parents(const ScTables &t, uint32_t base_mask,
                                                    const fp2 &lambda, const fp2 *__restrict__ eq_lo,
                                                    const fp2 *__restrict__ eq_hi, uint32_t eq_a,
                                                    const fp2 *__restrict__ pn,
                                                    const fp2 *__restrict__ pd, uint64_t q,
-                                                   fp2 (&acc)[3]) {
+                                                   ACC (&acc)[3]) {
     auto line2 = [](const fp2 &x, const fp2 &y) { return gl::sub(gl::add(y, y), x); };
     fp2 x2[4];
 #pragma unroll
@@ -415,9 +431,9 @@ __device__ __forceinline__ void sc_item_eq_parents(const ScTables &t, uint32_t b
     const fp2 lo = eq_lo[q & ((1ull << eq_a) - 1)];
     const fp2 e = LO ? lo : gl::mul(lo, eq_hi[q >> eq_a]);
     const fp2 f2 = gl::add(gl::mul(x2[0], x2[3]), gl::mul(x2[2], gl::add(x2[1], gl::mul(lambda, x2[3]))));
-    acc[0] = gl::add(acc[0], gl::mul(e, gl::add(n0, gl::mul(lambda, d0))));
-    acc[1] = gl::add(acc[1], gl::mul(e, gl::add(n1, gl::mul(lambda, d1))));
-    acc[2] = gl::add(acc[2], gl::mul(e, f2));
+    accumulate(acc[0], e, gl::add(n0, gl::mul(lambda, d0)));
+    accumulate(acc[1], e, gl::add(n1, gl::mul(lambda, d1)));
+    accumulate(acc[2], e, f2);
 }
 
 // One own round of a level instance whose relation is eq(r, x) F(x) with eq a factor, not a
@@ -465,24 +481,35 @@ k_sc_round_eq(ScTables t, uint32_t base_mask, bool first, uint64_t out_len, uint
         if (pairs == 0) return;
     } else {
         constexpr int NA = (REL == REL_FRACTION && UNIT) ? 6 : 3;
+        // the run sums are wide: one reduction per run instead of one per product
+        using RunT = gl::wide2;
         const uint64_t per = (pairs + gridDim.x - 1) / gridDim.x;
         const uint64_t chunk = (per + SC_THREADS - 1) / SC_THREADS * SC_THREADS;
         const uint64_t q0 = (uint64_t)blockIdx.x * chunk;
         const uint64_t q1 = q0 + chunk < pairs ? q0 + chunk : pairs;
         const bool hi_one = (pairs >> eq_a) <= 1;
         // a run's sums, folded into acc (times lambda for the E D0 D1 half, times its eq_hi)
-        fp2 run[NA];
+        RunT run[NA];
 #pragma unroll
-        for (int z = 0; z < NA; z++) run[z] = fp2{0, 0};
+        for (int z = 0; z < NA; z++) run[z] = RunT{};
         auto fold = [&](const fp2 *w) {
+            fp2 lin[3];  // NA = 6: the linear part's node 2 from its nodes 0 and 1 (sc_eq_nodes)
+            if constexpr (NA == 6) {
+                lin[0] = reduced(run[0]);
+                lin[1] = reduced(run[1]);
+                lin[2] = gl::sub(gl::add(lin[1], lin[1]), lin[0]);
+            }
 #pragma unroll
             for (int z = 0; z < 3; z++) {
-                fp2 v = run[z];
-                if constexpr (NA == 6) v = gl::add(v, gl::mul(lambda, run[3 + z]));
+                fp2 v;
+                if constexpr (NA == 6)
+                    v = gl::add(lin[z], gl::mul(lambda, reduced(run[3 + z])));
+                else
+                    v = reduced(run[z]);
                 acc[z] = gl::add(acc[z], w ? gl::mul(*w, v) : v);
             }
 #pragma unroll
-            for (int z = 0; z < NA; z++) run[z] = fp2{0, 0};
+            for (int z = 0; z < NA; z++) run[z] = RunT{};
         };
         uint64_t h = q0 >> eq_a;
 #pragma unroll 1
@@ -495,9 +522,9 @@ k_sc_round_eq(ScTables t, uint32_t base_mask, bool first, uint64_t out_len, uint
             const uint64_t q = qb + threadIdx.x;
             if (q < q1) {
                 if constexpr (PAR)
-                    sc_item_eq_parents<true>(t, base_mask, lambda, eq_lo, eq_hi, eq_a, pn, pd, q, run);
+                    sc_item_eq_parents<true, RunT>(t, base_mask, lambda, eq_lo, eq_hi, eq_a, pn, pd, q, run);
                 else
-                    sc_item_eq<REL, UNIT, BASE, true, NA>(t, base_mask, first, bind, pairs, r, lambda, eq_lo, eq_hi, eq_a, q, run);
+                    sc_item_eq<REL, UNIT, BASE, true, NA, RunT>(t, base_mask, first, bind, pairs, r, lambda, eq_lo, eq_hi, eq_a, q, run);
             }
         }
         if (q0 < q1) {

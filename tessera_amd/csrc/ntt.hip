@@ -504,9 +504,11 @@ k_encode_mid(const uint64_t *__restrict__ in, uint64_t in_stride, uint64_t *__re
 // the kernel). RPW rows' tiles share the LDS tile in the CT stages, interleaved (word
 // (t << LOGR) | r holds row r's value t), so RPW times the threads have a group there. The
 // spread values of all RPW rows then wait in registers (RPW x 8 / 2^(TN-TK) words per thread)
-// while the GS stages run on one row's codeword tile at a time. Two rows: 64 VGPRs (4 spilled),
-// two workgroups per CU, 5% faster than one row at 256 x 2^20 rate 1/4; four rows spill 22
-// registers and are slower.
+// while the GS stages run on one row's codeword tile at a time: the current row's in registers,
+// the others parked in LDS past the tile (RPW - 1 rows x 8 / 2^(TN-TK) words per thread). Two
+// rows: 64 VGPRs and 80 KiB of LDS, two workgroups per CU, faster than one row at 256 x 2^20
+// rate 1/4 (holding both rows' values in registers instead spilled 4 VGPRs to scratch memory:
+// 2 GiB more HBM traffic per launch); four rows spill 22 registers and are slower.
 template <int TK, int TN, int LOGR>
 __global__ void __launch_bounds__(MID_THREADS) __attribute__((amdgpu_waves_per_eu(8, 8)))
 k_encode_mid_rows(const uint64_t *__restrict__ in, uint64_t in_stride, uint64_t *__restrict__ cw,
@@ -532,19 +534,24 @@ k_encode_mid_rows(const uint64_t *__restrict__ in, uint64_t in_stride, uint64_t 
     tile_stages_ct<false, true, TK, LOGR, 0, false, MID_THREADS>(lds, c, zk_inv, vk_inv);
 
     // Thread g's first GS group (positions 8g..8g+7 of the codeword tile) holds the spread values
-    // of k-positions t = g PERG + jj at positions S jj, zeros elsewhere.
+    // of k-positions t = g PERG + jj at positions S jj, zeros elsewhere. Row 0's stay in v, row
+    // r > 0's go to park[((r - 1) PERG + jj) MID_THREADS + g] (each thread reads back only what
+    // it wrote: no barrier orders them).
     const uint32_t g = threadIdx.x;
-    uint64_t v[RPW][PERG];
+    uint64_t *park = lds + Tn;
+    uint64_t v[PERG];
 #pragma unroll
     for (int jj = 0; jj < PERG; jj++) {
         const uint32_t t = g * PERG + jj;
         const uint64_t f = spread[kbase + t];
+        v[jj] = gl::mul(lds[t << LOGR], f);
 #pragma unroll
-        for (int r = 0; r < RPW; r++) v[r][jj] = gl::mul(lds[(t << LOGR) | r], f);
+        for (int r = 1; r < RPW; r++)
+            park[((r - 1) * PERG + jj) * MID_THREADS + g] = gl::mul(lds[(t << LOGR) | r], f);
     }
     constexpr uint32_t ZEROS = S == 2 ? 0xAAu : 0xEEu;
-    // One row per iteration, not unrolled: the row's values are always v[0] (the held values
-    // move down a row after each iteration).
+    // One row per iteration, not unrolled: the row's values are always v, reloaded from the park
+    // at the end of the iteration (so nothing of the next row is live through the GS stages).
 #pragma unroll 1
     for (int r = 0; r < RPW; r++) {
         __syncthreads();  // the CT tile, or the previous row's codeword tile, has been read
@@ -554,11 +561,7 @@ k_encode_mid_rows(const uint64_t *__restrict__ in, uint64_t in_stride, uint64_t 
         asm volatile("" : "+s"(vnr));
         uint64_t x[8];
 #pragma unroll
-        for (int j = 0; j < 8; j++) x[j] = (j % S == 0) ? v[0][j / S] : 0;
-#pragma unroll
-        for (int rr = 0; rr + 1 < RPW; rr++)
-#pragma unroll
-            for (int jj = 0; jj < PERG; jj++) v[rr][jj] = v[rr + 1][jj];
+        for (int j = 0; j < 8; j++) x[j] = (j % S == 0) ? v[j / S] : 0;
         round8_regs<true, false, TN, 0, false, ZEROS>(x, g << 3, c, vnr);
 #pragma unroll
         for (int j = 0; j < 8; j++) lds[(g << 3) + j] = x[j];
@@ -566,6 +569,14 @@ k_encode_mid_rows(const uint64_t *__restrict__ in, uint64_t in_stride, uint64_t 
         tile_stages_ct<true, false, TN, 0, 3, false, MID_THREADS>(lds, c, zn, vnr);
         uint64_t *dst = cw_rows ? cw_rows[row0 + r] : cw + (row0 + r) * cw_stride;
         for (uint32_t e = threadIdx.x; e < Tn; e += MID_THREADS) dst[nbase + e] = lds[e];
+        if (r + 1 < RPW) {
+            // the thread's park address recomputed here (an empty asm statement hides that it
+            // equals g), not held through the GS stages
+            uint32_t gg = threadIdx.x;
+            asm volatile("" : "+v"(gg));
+#pragma unroll
+            for (int jj = 0; jj < PERG; jj++) v[jj] = park[(r * PERG + jj) * MID_THREADS + gg];
+        }
     }
 }
 
@@ -647,6 +658,12 @@ using MidRowsFn = void (*)(const uint64_t *, uint64_t, uint64_t *, uint64_t, con
 #endif
 // The several-rows fused middle (2^TSR_MID_ROWS_LOG rows per workgroup), where one exists:
 // unmasked, tile_log 13, rate 1/4 (at rate 1/2 a thread would hold twice the spread values).
+// LDS past the codeword tile that k_encode_mid_rows parks the later rows' spread values in.
+static size_t mid_rows_park_bytes(uint32_t log_r, uint32_t rows_log) {
+    const uint32_t perg = 8u >> log_r;  // spread values per thread and row
+    return sizeof(uint64_t) * (((size_t)1 << rows_log) - 1) * perg * MID_THREADS;
+}
+
 static MidRowsFn mid_rows_fn(uint32_t tk, uint32_t log_r, uint64_t P, uint32_t *rows_log) {
     *rows_log = TSR_MID_ROWS_LOG;
 #if TSR_MID_THREADS == 1024 && TSR_MID_ROWS_LOG > 0
@@ -808,7 +825,8 @@ void Encoder::launch(const uint64_t *src, uint64_t src_stride, const uint64_t *c
         for (uint64_t c0 = 0; c0 < tiles; c0 += 65535) {
             const uint64_t tc = tiles - c0 < 65535 ? tiles - c0 : 65535;
             if (grouped)
-                hipLaunchKernelGGL(fr, dim3((unsigned)(grouped / rpw), (unsigned)tc), dim3(MID_THREADS), shm,
+                hipLaunchKernelGGL(fr, dim3((unsigned)(grouped / rpw), (unsigned)tc), dim3(MID_THREADS),
+                                   shm + mid_rows_park_bytes(log_r, rows_log),
                                    stream, src, src_stride, cw, n, zk_inv.ptr, zn_fwd.ptr, vk_inv.ptr,
                                    vn_fwd.ptr, spread_tab.ptr, src_rows, cw_rows, c0);
             if (grouped < rc)

@@ -773,7 +773,8 @@ def main():
     ap.add_argument("--traffic-command", default="", help="the profiling command, for the summary")
     ap.add_argument("--trace-summary", nargs=3, metavar=("DB", "TITLE", "COMMAND"),
                     help="print a markdown summary of a rocprofv3 kernel-trace database")
-    ap.add_argument("--only", choices=["headline", "copy", "hostfed", "refshapes", "sumcheck", "batched", "served"],
+    ap.add_argument("--only", choices=["headline", "copy", "hostfed", "refshapes", "plain", "sumcheck", "batched",
+                                       "served"],
                     default=None,
                     help="time the headline commit only; run the device copy that calibrates "
                          "the HBM byte counters (4 copies of 2 GiB); time the headline commit "
@@ -801,6 +802,9 @@ def main():
         return
     if a.only == "batched":
         print(json.dumps({"workload": "batched commit", **run_batched()}), flush=True)
+        return
+    if a.only == "plain":
+        print(json.dumps(run_plain()), flush=True)
         return
     if a.only == "sumcheck":
         print(json.dumps({"workload": "sumcheck/lookup configuration", **run_sumcheck()}), flush=True)

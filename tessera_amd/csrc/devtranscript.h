@@ -122,24 +122,45 @@ __device__ inline void wave_hash_chunk(const uint32_t *key, const uint32_t *buf,
 
 // buf = st (8 words) || u64LE(L) || label[0..L) || u64LE(x) || tail[0..tail_len), zero to the
 // next multiple of 64 bytes; returns the length. The absorb input with x = count and the TCE of
-// the values as tail, or the squeeze input with x = counter and no tail.
+// the values as tail, or the squeeze input with x = counter and no tail. Word-level: label (L <= 64,
+// in a 4-byte aligned slot) and tail (4-byte aligned, tail_len a multiple of 4) are read a word at
+// a time, the label's bytes past L masked off, and Y = u64LE(x) || tail, which starts at byte
+// 40 + L, placed by a funnel shift of two adjacent Y words.
 __device__ inline uint32_t wave_put(uint32_t *buf, const uint32_t *st, const uint8_t *label,
                                     uint32_t L, uint64_t x, const uint8_t *tail, uint32_t tail_len) {
     const uint32_t len = 48 + L + tail_len;
     const uint32_t words = 16 * ((len + 63) / 64);
+    const uint32_t *lw = (const uint32_t *)label, *tw = (const uint32_t *)tail;
+    const int ny = 2 + (int)(tail_len / 4);  // words of Y
+    auto yword = [&](int j) -> uint32_t {
+        if (j < 0 || j >= ny) return 0u;
+        if (j == 0) return (uint32_t)x;
+        if (j == 1) return (uint32_t)(x >> 32);
+        return tw[j - 2];
+    };
     for (uint32_t w = threadIdx.x & 63; w < words; w += 64) {
-        uint32_t v = 0;
-#pragma unroll
-        for (uint32_t k = 0; k < 4; k++) {
-            const uint32_t p = 4 * w + k;
-            uint32_t byte = 0;
-            if (p < 32) byte = st[p >> 2] >> (8 * (p & 3));
-            else if (p < 36) byte = L >> (8 * (p - 32));
-            else if (p < 40) byte = 0;
-            else if (p < 40 + L) byte = label[p - 40];
-            else if (p < 48 + L) byte = (uint32_t)(x >> (8 * (p - 40 - L)));
-            else if (p < len) byte = tail[p - 48 - L];
-            v |= (byte & 0xFF) << (8 * k);
+        uint32_t v;
+        if (w < 8) {
+            v = st[w];
+        } else if (w < 10) {
+            v = w == 8 ? L : 0u;  // u64LE(L), L < 2^32
+        } else {
+            const uint32_t o = 4 * (w - 10);  // the word's first byte after the length field
+            uint32_t lab = 0;
+            if (o < L) {
+                lab = lw[o >> 2];
+                const uint32_t keep = L - o;
+                if (keep < 4) lab &= (1u << (8 * keep)) - 1u;
+            }
+            // Y bytes t .. t + 3 (t < 0: the word's first -t bytes are the label's)
+            const int t = (int)o - (int)L;
+            uint32_t y = 0;
+            if (t > -4) {
+                const int j = t >> 2, sh = t & 3;  // floor division
+                const uint64_t pair = ((uint64_t)yword(j + 1) << 32) | yword(j);
+                y = (uint32_t)(pair >> (8 * sh));
+            }
+            v = lab | y;
         }
         buf[w] = v;
     }

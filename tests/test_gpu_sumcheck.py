@@ -85,6 +85,18 @@ def test_degrees_and_padding(dev, degree):
     run_both(dev, f"stage/{degree}", degree, spec)
 
 
+@pytest.mark.parametrize("plen", [1, 2, 3, 4, 22, 37, 48, 55])
+def test_label_lengths(dev, plen):
+    """Stage prefixes whose round labels ("<prefix>/round/<j>", "<prefix>/r/<j>") take every
+    length mod 4 up to the 64-byte label slot, so the device transcript's hash inputs put the
+    message at every byte alignment and span one to three BLAKE3 blocks; 12 variables, so both
+    the transcript launches and the tail's in-block rounds run."""
+    rng = np.random.default_rng(plen)
+    prefix = "".join(chr(ord("a") + (i % 26)) for i in range(plen))
+    spec = [[(rng.integers(0, P, 1 << 12, dtype=np.uint64), False, False) for _ in range(3)]]
+    run_both(dev, prefix, 3, spec)
+
+
 def test_all_instances_without_variables(dev):
     spec = [[(np.array([5], dtype=np.uint64), False, False)], [(np.array([[7, 9]], dtype=np.uint64), True, False)]]
     run_both(dev, "stage/0", 2, spec)

@@ -7,6 +7,7 @@ the reference's PcsError/ColumnError variants would be (tessera_amd.lib.BadShape
 variant's name in the message), before any device work.
 """
 import ctypes
+import struct
 from dataclasses import dataclass, field
 
 import numpy as np
@@ -196,6 +197,21 @@ class CommitmentParamsPy:
         return p
 
 
+_DIGEST_SPLITS = {}
+
+
+def _split_digests(buf, count):
+    """The first count 32-byte digests of a ctypes buffer, as a list of bytes."""
+    if count == 0:
+        return []
+    st = _DIGEST_SPLITS.get(count)
+    if st is None:
+        if len(_DIGEST_SPLITS) >= 64:  # a small cache of the formats last used
+            _DIGEST_SPLITS.clear()
+        st = _DIGEST_SPLITS[count] = struct.Struct("32s" * count)
+    return list(st.unpack_from(buf))
+
+
 @dataclass
 class ColumnAnswer:
     """ColumnAnswer (commitment.rs:578-598): columns of the distinct queries + multiproof."""
@@ -321,7 +337,7 @@ class ReferenceCommitment:
 
     def answer(self, queries) -> ColumnAnswer:
         q = np.ascontiguousarray(np.array(queries, dtype=np.uint64))
-        cap_d = max(1, len(set(int(x) for x in q)))
+        cap_d = max(1, int(np.unique(q).size))
         cap_s = 64 * cap_d + 64
         distinct = np.zeros(cap_d, dtype=np.uint64)
         cols = np.zeros(cap_d * max(self.count, 1), dtype=np.uint64)
@@ -331,10 +347,9 @@ class ReferenceCommitment:
                                           distinct.ctypes.data, cap_d, cols.ctypes.data, sib, cap_s,
                                           ctypes.byref(nd), ctypes.byref(ns)))
         d = nd.value
-        raw = sib.raw
-        columns = [cols[i * self.count:(i + 1) * self.count].copy() for i in range(d)]
-        return ColumnAnswer([int(x) for x in distinct[:d]], columns,
-                            [raw[32 * i:32 * i + 32] for i in range(ns.value)])
+        # rows of one fresh array (views), and the 32-byte digests split in C
+        columns = list(cols[:d * self.count].reshape(d, self.count)) if d else []
+        return ColumnAnswer(distinct[:d].tolist(), columns, _split_digests(sib, ns.value))
 
     def free(self):
         if self.handle:

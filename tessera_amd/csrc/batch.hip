@@ -271,7 +271,7 @@ tsr_status tsr_commit_batch_create(tsr_ctx *ctx_, const tsr_commit_desc *descs, 
         e = hipEventCreateWithFlags(&b->fork, hipEventDisableTiming);
         if (e != hipSuccess) return hip_fail(e, "fork event").code;
     }
-    e = hipStreamSynchronize(s);
+    e = tsr::stream_sync(s);
     if (e != hipSuccess) return hip_fail(e, "sync").code;
     *out = b.release();
     return TSR_OK;
@@ -314,7 +314,7 @@ tsr_status tsr_commit_batch_run(tsr_batch *b) {
         e = hipGraphLaunch(b->exec, s);
         if (e != hipSuccess) return hip_fail(e, "hipGraphLaunch").code;
     }
-    e = hipStreamSynchronize(s);
+    e = tsr::stream_sync(s);
     if (e != hipSuccess) return hip_fail(e, "sync").code;
     return TSR_OK;
 }
@@ -323,7 +323,7 @@ tsr_status tsr_commit_batch_roots(const tsr_batch *b, uint8_t *roots) {
     if (!b || !roots) return Status::fail(TSR_BAD_ARG, "null argument").code;
     hipError_t e = hipSetDevice(b->ctx->device);
     if (e == hipSuccess) e = hipMemcpyAsync(roots, b->roots.ptr, 32 * b->count, hipMemcpyDeviceToHost, b->ctx->stream);
-    if (e == hipSuccess) e = hipStreamSynchronize(b->ctx->stream);
+    if (e == hipSuccess) e = tsr::stream_sync(b->ctx->stream);
     if (e != hipSuccess) return hip_fail(e, "roots").code;
     return TSR_OK;
 }
@@ -346,12 +346,12 @@ tsr_status tsr_commit_batch_tuning(const tsr_batch *b, uint32_t *plans, uint32_t
 void tsr_commit_batch_free(tsr_batch *b) {
     if (!b) return;
     (void)hipSetDevice(b->ctx->device);
-    (void)hipStreamSynchronize(b->ctx->stream);
+    (void)tsr::stream_sync(b->ctx->stream);
     if (b->exec) (void)hipGraphExecDestroy(b->exec);
     if (b->graph) (void)hipGraphDestroy(b->graph);
     for (auto &g : b->groups) {
         if (g->side) {
-            (void)hipStreamSynchronize(g->side);
+            (void)tsr::stream_sync(g->side);
             (void)hipStreamDestroy(g->side);
         }
         if (g->done) (void)hipEventDestroy(g->done);

@@ -81,7 +81,7 @@ Status dev_alloc(Ctx *ctx, uint64_t bytes, void **out) {
     hipError_t e = hipMalloc(out, bytes);
     if (e == hipErrorOutOfMemory && ctx->pooled) {
         (void)hipGetLastError();
-        (void)hipStreamSynchronize(ctx->stream);
+        (void)tsr::stream_sync(ctx->stream);
         pool_trim_locked(ctx);
         e = hipMalloc(out, bytes);
     }
@@ -96,7 +96,7 @@ Status dev_alloc(Ctx *ctx, uint64_t bytes, void **out) {
 Status pinned_host(Ctx *ctx, size_t bytes, uint8_t **out) {
     if (bytes > ctx->pinned_cap) {
         if (ctx->pinned) {
-            (void)hipStreamSynchronize(ctx->stream);
+            (void)tsr::stream_sync(ctx->stream);
             (void)hipHostFree(ctx->pinned);
             ctx->pinned = nullptr;
             ctx->pinned_cap = 0;
@@ -278,7 +278,7 @@ tsr_status tsr_ctx_create(int device, tsr_ctx **out) {
 void tsr_ctx_destroy(tsr_ctx *ctx) {
     if (!ctx) return;
     (void)hipSetDevice(ctx->device);
-    (void)hipStreamSynchronize(ctx->stream);
+    (void)tsr::stream_sync(ctx->stream);
     ctx->encoders.clear();
     for (auto &t : ctx->timed) {
         (void)hipEventDestroy(t.start);
@@ -299,7 +299,7 @@ void tsr_ctx_destroy(tsr_ctx *ctx) {
 tsr_status tsr_ctx_synchronize(tsr_ctx *ctx) {
     CHECK_ARG(ctx, "null context");
     TRY_HIP(hipSetDevice(ctx->device));
-    TRY_HIP(hipStreamSynchronize(ctx->stream));
+    TRY_HIP(tsr::stream_sync(ctx->stream));
     return TSR_OK;
 }
 
@@ -314,7 +314,7 @@ tsr_status tsr_ctx_profile(tsr_ctx *ctx, int enable) {
     CHECK_ARG(ctx, "null context");
     if (enable == 2) {
         TRY_HIP(hipSetDevice(ctx->device));
-        TRY_HIP(hipStreamSynchronize(ctx->stream));
+        TRY_HIP(tsr::stream_sync(ctx->stream));
         for (auto &t : ctx->timed) {
             ctx->spare.push_back(t.start);
             ctx->spare.push_back(t.stop);
@@ -330,7 +330,7 @@ tsr_status tsr_ctx_kernel_time(tsr_ctx *ctx, const char *kernel, double *total_m
                                uint64_t *launches) {
     CHECK_ARG(ctx && kernel, "null argument");
     TRY_HIP(hipSetDevice(ctx->device));
-    TRY_HIP(hipStreamSynchronize(ctx->stream));
+    TRY_HIP(tsr::stream_sync(ctx->stream));
     double total = 0;
     uint64_t count = 0;
     for (auto &t : ctx->timed) {
@@ -378,7 +378,7 @@ tsr_status tsr_free(tsr_ctx *ctx, void *ptr) {
     }
     TRY_HIP(hipSetDevice(ctx->device));
     // work queued on the stream may still read or write the block
-    TRY_HIP(hipStreamSynchronize(ctx->stream));
+    TRY_HIP(tsr::stream_sync(ctx->stream));
     dev_free(ctx, ptr, charged);
     return TSR_OK;
 }
@@ -387,7 +387,7 @@ tsr_status tsr_copy(tsr_ctx *ctx, void *dst, const void *src, uint64_t bytes) {
     CHECK_ARG(ctx && (bytes == 0 || (dst && src)), "null argument");
     TRY_HIP(hipSetDevice(ctx->device));
     TRY_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyDefault, ctx->stream));
-    TRY_HIP(hipStreamSynchronize(ctx->stream));
+    TRY_HIP(tsr::stream_sync(ctx->stream));
     return TSR_OK;
 }
 
@@ -408,7 +408,7 @@ tsr_status tsr_fill_uniform(tsr_ctx *ctx, uint64_t *dst, uint64_t count, uint64_
     TRY(check_launch("find rejects"));
     uint32_t n_rej = 0;
     TRY_HIP(hipMemcpyAsync(&n_rej, cnt.ptr, 4, hipMemcpyDeviceToHost, ctx->stream));
-    TRY_HIP(hipStreamSynchronize(ctx->stream));
+    TRY_HIP(tsr::stream_sync(ctx->stream));
     if (n_rej > cap) return Status::fail(TSR_DECLINED, "too many rejected words").code;
     std::vector<uint64_t> p(n_rej);
     if (n_rej) {
@@ -419,7 +419,7 @@ tsr_status tsr_fill_uniform(tsr_ctx *ctx, uint64_t *dst, uint64_t count, uint64_
     hipLaunchKernelGGL(k_fill_uniform, dim3((unsigned)((count + 255) / 256)), dim3(256), 0,
                        ctx->stream, dst, count, seed, pos.ptr, n_rej);
     TRY(check_launch("fill uniform"));
-    TRY_HIP(hipStreamSynchronize(ctx->stream));
+    TRY_HIP(tsr::stream_sync(ctx->stream));
     return TSR_OK;
 }
 
@@ -472,7 +472,7 @@ tsr_status tsr_encode_rows(tsr_ctx *ctx, uint32_t log_k, uint32_t log_r, uint64_
     if (host_out)
         TRY_HIP(hipMemcpyAsync(codewords, dcw, 8 * count * n, hipMemcpyDeviceToHost,
                                ctx->stream));
-    TRY_HIP(hipStreamSynchronize(ctx->stream));
+    TRY_HIP(tsr::stream_sync(ctx->stream));
     return TSR_OK;
 }
 
@@ -490,7 +490,7 @@ tsr_status tsr_chains_create(tsr_ctx *ctx, const uint8_t id[32], uint64_t column
     c->block = block;
     TRY(c->state.alloc(ctx, 8 * columns));
     TRY(chains_init(ctx, id, columns, c->state.ptr, ctx->stream));
-    TRY_HIP(hipStreamSynchronize(ctx->stream));
+    TRY_HIP(tsr::stream_sync(ctx->stream));
     *out = c.release();
     return TSR_OK;
 }
@@ -514,7 +514,7 @@ tsr_status tsr_chains_absorb(tsr_chains *c, const uint64_t *rows, uint64_t count
         d = buf.ptr;
     }
     TRY(chains_absorb(ctx, d, c->columns, (uint32_t)count, c->state.ptr, ctx->stream));
-    TRY_HIP(hipStreamSynchronize(ctx->stream));
+    TRY_HIP(tsr::stream_sync(ctx->stream));
     c->closed = count < c->block;
     return TSR_OK;
 }
@@ -524,7 +524,7 @@ tsr_status tsr_chains_read(tsr_chains *c, uint8_t *out) {
     TRY_HIP(hipSetDevice(c->ctx->device));
     TRY_HIP(hipMemcpyAsync(out, c->state.ptr, 32 * c->columns, hipMemcpyDefault,
                            c->ctx->stream));
-    TRY_HIP(hipStreamSynchronize(c->ctx->stream));
+    TRY_HIP(tsr::stream_sync(c->ctx->stream));
     return TSR_OK;
 }
 
@@ -547,7 +547,7 @@ tsr_status tsr_merkle_root(tsr_ctx *ctx, const uint8_t *leaves, uint64_t n, uint
     TRY(merkle_build(ctx, nodes.ptr, n, ctx->stream));
     TRY_HIP(hipMemcpyAsync(root, nodes.ptr + 8 * (2 * n - 2), 32, hipMemcpyDeviceToHost,
                            ctx->stream));
-    TRY_HIP(hipStreamSynchronize(ctx->stream));
+    TRY_HIP(tsr::stream_sync(ctx->stream));
     return TSR_OK;
 }
 
@@ -595,7 +595,7 @@ tsr_status tsr_commit_unmasked(tsr_ctx *ctx, const tsr_commitment_params *params
     TRY(merkle_build(ctx, c->nodes.ptr, n, ctx->stream));
     TRY_HIP(hipMemcpyAsync(c->root, c->nodes.ptr + 8 * (2 * n - 2), 32, hipMemcpyDeviceToHost,
                            ctx->stream));
-    TRY_HIP(hipStreamSynchronize(ctx->stream));
+    TRY_HIP(tsr::stream_sync(ctx->stream));
     *out = c.release();
     return TSR_OK;
 }
@@ -630,7 +630,7 @@ static Status finish_commitment(tsr_commitment *c, const uint64_t *segment_rows,
     }
     if ((st = merkle_build(ctx, c->nodes.ptr, n, ctx->stream))) return st;
     TSR_HIP(hipMemcpyAsync(c->root, c->nodes.ptr + 8 * (2 * n - 2), 32, hipMemcpyDeviceToHost, ctx->stream));
-    TSR_HIP(hipStreamSynchronize(ctx->stream));
+    TSR_HIP(tsr::stream_sync(ctx->stream));
     return Status::ok();
 }
 
@@ -726,7 +726,7 @@ tsr_status tsr_commitment_leaves(const tsr_commitment *c, uint8_t *out) {
     CHECK_ARG(c && out, "null argument");
     TRY_HIP(hipSetDevice(c->ctx->device));
     TRY_HIP(hipMemcpyAsync(out, c->nodes.ptr, 32 * c->n, hipMemcpyDefault, c->ctx->stream));
-    TRY_HIP(hipStreamSynchronize(c->ctx->stream));
+    TRY_HIP(tsr::stream_sync(c->ctx->stream));
     return TSR_OK;
 }
 
@@ -788,7 +788,7 @@ tsr_status tsr_commitment_answer(const tsr_commitment *c, const uint64_t *querie
         TRY_HIP(hipMemcpyAsync(pin + up_bytes + col_bytes, sib.ptr, sib_bytes, hipMemcpyDeviceToHost,
                                ctx->stream));
     }
-    TRY_HIP(hipStreamSynchronize(ctx->stream));
+    TRY_HIP(tsr::stream_sync(ctx->stream));
     if (col_bytes) memcpy(columns_out, pin + up_bytes, col_bytes);
     if (sib_bytes) memcpy(siblings_out, pin + up_bytes + col_bytes, sib_bytes);
     return TSR_OK;
@@ -831,7 +831,7 @@ tsr_status tsr_row_coefficients(tsr_ctx *ctx, uint32_t row_vars, const uint64_t 
                          dst, s));
     if (!is_device_ptr(out))
         TRY_HIP(hipMemcpyAsync(out, dst, 16 * (uint64_t)(groups + 1) * N, hipMemcpyDeviceToHost, s));
-    TRY_HIP(hipStreamSynchronize(s));
+    TRY_HIP(tsr::stream_sync(s));
     return TSR_OK;
 }
 
@@ -857,7 +857,7 @@ tsr_status tsr_commitment_open(const tsr_commitment *c, const uint64_t *coeffici
     TRY(combine_rows(ctx, coef, count, c->masks.ptr, N, P, d_m.ptr, s));
     TRY_HIP(hipMemcpyAsync(u_out, d_u.ptr, 16 * (uint64_t)count * k, hipMemcpyDefault, s));
     TRY_HIP(hipMemcpyAsync(mask_out, d_m.ptr, 16 * (uint64_t)count * P, hipMemcpyDefault, s));
-    TRY_HIP(hipStreamSynchronize(s));
+    TRY_HIP(tsr::stream_sync(s));
     return TSR_OK;
 }
 
@@ -869,7 +869,7 @@ void tsr_commitment_free(tsr_commitment *c) {
         zero_words(c->masks.ptr, c->masks.len, c->ctx->stream);
         zero_words(c->codewords.ptr, c->codewords.len, c->ctx->stream);
     }
-    (void)hipStreamSynchronize(c->ctx->stream);
+    (void)tsr::stream_sync(c->ctx->stream);
     delete c;
 }
 

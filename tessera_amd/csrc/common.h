@@ -3,6 +3,7 @@
 #pragma once
 #include <hip/hip_runtime.h>
 #include <stdint.h>
+#include <chrono>
 #include <map>
 #include <mutex>
 #include <memory>
@@ -29,6 +30,21 @@ struct Status {
 
 Status hip_fail(hipError_t e, const char *what);
 Status check_launch(const char *what);
+
+// Waits for a stream's work: polls for up to 200 us, then blocks (hipStreamSynchronize). The
+// waits of small proofs are short, and a blocking wait returns tens of microseconds after the
+// GPU is done (256 x 2^14 commit + 14-variable sumcheck: 0.58 -> 0.55 ms); a long wait does
+// not keep a host core busy.
+inline hipError_t stream_sync(hipStream_t s) {
+    const auto t0 = std::chrono::steady_clock::now();
+    for (;;) {
+        const hipError_t e = hipStreamQuery(s);
+        if (e != hipErrorNotReady) return e;
+        if (std::chrono::steady_clock::now() - t0 > std::chrono::microseconds(200))
+            return hipStreamSynchronize(s);
+        __builtin_ia32_pause();
+    }
+}
 
 #define TSR_HIP(expr)                                         \
     do {                                                      \
@@ -133,7 +149,7 @@ Status DevBuf<T>::upload(Ctx *c, const std::vector<T> &v) {
     if (st) return st;
     TSR_HIP(hipMemcpyAsync(ptr, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice,
                            c->stream));
-    TSR_HIP(hipStreamSynchronize(c->stream));
+    TSR_HIP(stream_sync(c->stream));
     return Status::ok();
 }
 

@@ -154,7 +154,7 @@ tsr_status tsr_level_load(tsr_ctx *ctx_, int shape, const uint64_t lambda[2],
         if (e != hipSuccess) return hip_fail(e, "level upload").code;
     }
     if ((st = L->partials.alloc(ctx, 3 * LVL_MAX_BLOCKS))) return st.code;
-    e = hipStreamSynchronize(ctx->stream);
+    e = tsr::stream_sync(ctx->stream);
     if (e != hipSuccess) return hip_fail(e, "level upload").code;
     *out = L.release();
     return TSR_OK;
@@ -189,7 +189,7 @@ tsr_status tsr_level_sums(tsr_level *L, const uint64_t *low, uint32_t low_bits, 
     hipLaunchKernelGGL(k_level_total, dim3(1), dim3(LVL_THREADS), 0, s, L->partials.ptr, blocks);
     if ((st = check_launch("level sums"))) return st.code;
     if ((e = hipMemcpyAsync(out, L->partials.ptr, 48, hipMemcpyDeviceToHost, s)) != hipSuccess ||
-        (e = hipStreamSynchronize(s)) != hipSuccess)
+        (e = tsr::stream_sync(s)) != hipSuccess)
         return hip_fail(e, "level sums").code;
     return TSR_OK;
 }
@@ -212,7 +212,7 @@ tsr_status tsr_level_bind(tsr_level *L, const uint64_t r[2]) {
     }
     Status st = check_launch("level bind");
     if (st) return st.code;
-    if ((e = hipStreamSynchronize(ctx->stream)) != hipSuccess) return hip_fail(e, "level bind").code;
+    if ((e = tsr::stream_sync(ctx->stream)) != hipSuccess) return hip_fail(e, "level bind").code;
     L->current = next;
     L->len = half;
     return TSR_OK;
@@ -227,7 +227,7 @@ tsr_status tsr_level_firsts(tsr_level *L, uint64_t *out) {
         if ((e = hipMemcpyAsync(out + 2 * i, L->buf[L->current][i].ptr, 16, hipMemcpyDeviceToHost,
                                 ctx->stream)) != hipSuccess)
             return hip_fail(e, "level firsts").code;
-    if ((e = hipStreamSynchronize(ctx->stream)) != hipSuccess) return hip_fail(e, "level firsts").code;
+    if ((e = tsr::stream_sync(ctx->stream)) != hipSuccess) return hip_fail(e, "level firsts").code;
     return TSR_OK;
 }
 

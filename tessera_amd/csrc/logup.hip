@@ -434,13 +434,13 @@ static Status read_root(Ctx *ctx, DevTree &t) {
         } else {
             uint64_t v = 0;
             TSR_HIP(hipMemcpyAsync(&v, t.nbase.ptr, 8, hipMemcpyDeviceToHost, s));
-            TSR_HIP(hipStreamSynchronize(s));
+            TSR_HIP(tsr::stream_sync(s));
             t.num = fp2{v, 0};
         }
     } else {
         TSR_HIP(hipMemcpyAsync(&t.num, t.N[0].ptr, 16, hipMemcpyDeviceToHost, s));
     }
-    TSR_HIP(hipStreamSynchronize(s));
+    TSR_HIP(tsr::stream_sync(s));
     return Status::ok();
 }
 
@@ -476,7 +476,7 @@ static Status eq_dot(Ctx *ctx, const fp2 *point, uint32_t n, const uint64_t *col
     hipLaunchKernelGGL(k_sum_partials, dim3(1), dim3(256), 0, s, part, nb, val);
     if ((st = check_launch("eq dot"))) return st;
     TSR_HIP(hipMemcpyAsync(&out, val, 16, hipMemcpyDeviceToHost, s));
-    TSR_HIP(hipStreamSynchronize(s));
+    TSR_HIP(tsr::stream_sync(s));
     return Status::ok();
 }
 
@@ -704,7 +704,7 @@ Status prove_logup(Ctx *ctx, Transcript &tr, const tsr_logup_instance *inst, uin
         uint64_t key0 = 0, val0 = 0;
         TSR_HIP(hipMemcpyAsync(&key0, keys, 8, hipMemcpyDeviceToHost, s));
         if (vals) TSR_HIP(hipMemcpyAsync(&val0, vals, 8, hipMemcpyDeviceToHost, s));
-        TSR_HIP(hipStreamSynchronize(s));
+        TSR_HIP(tsr::stream_sync(s));
         fp2 t0 = fp2{key0, 0};
         if (vals) t0 = gl::add(gl::mul_base(beta, val0), t0);
         const fp2 pad = gl::sub(alpha, t0);
@@ -741,7 +741,7 @@ Status prove_logup(Ctx *ctx, Transcript &tr, const tsr_logup_instance *inst, uin
             if (n) TSR_HIP(hipMemcpyAsync(d_point.ptr, la.point.data(), 16 * n, hipMemcpyHostToDevice, s));
             uint64_t pad = 0;  // row 0's value fills the padding slots
             TSR_HIP(hipMemcpyAsync(&pad, tvals[k], 8, hipMemcpyDeviceToHost, s));
-            TSR_HIP(hipStreamSynchronize(s));
+            TSR_HIP(tsr::stream_sync(s));
             fp2 v{0, 0};
             if ((st = eq_dot(ctx, d_point.ptr, n, lvals[k], inst[k].lookups, pad, d_part.ptr, d_val.ptr, v)))
                 return st;
@@ -878,7 +878,7 @@ struct ClearOnExit {
     ~ClearOnExit() {
         if (p && n) {
             zero_words(p, n, s);
-            (void)hipStreamSynchronize(s);
+            (void)tsr::stream_sync(s);
         }
     }
 };
@@ -929,7 +929,7 @@ tsr_status tsr_logup_multiplicities(tsr_ctx *ctx_, uint64_t rows, const uint64_t
     e = hipMemcpyAsync(&hbad, bad.ptr, 4, hipMemcpyDeviceToHost, s);
     if (e != hipSuccess) return hip_fail(e, "copy").code;
     // the row check comes back first, so that a bad lookup leaves `out` untouched
-    e = hipStreamSynchronize(s);
+    e = tsr::stream_sync(s);
     if (e != hipSuccess) return hip_fail(e, "sync").code;
     if (hbad) return Status::fail(TSR_BAD_SHAPE, "LookupRow: a lookup names no row of the table").code;
     if (lookups)
@@ -939,7 +939,7 @@ tsr_status tsr_logup_multiplicities(tsr_ctx *ctx_, uint64_t rows, const uint64_t
     // device or host output alike: one copy of the counts
     e = hipMemcpyAsync(out, counts.ptr, 8 * leaves, hipMemcpyDefault, s);
     if (e != hipSuccess) return hip_fail(e, "copy").code;
-    e = hipStreamSynchronize(s);
+    e = tsr::stream_sync(s);
     if (e != hipSuccess) return hip_fail(e, "sync").code;
     return TSR_OK;
 }
@@ -967,7 +967,7 @@ Status eq_table(Ctx *ctx, const gl::fp2 *point, uint32_t n, gl::fp2 *out, hipStr
                        0, s, lo.ptr, a, hi.ptr, len, out);
     if ((st = check_launch("eq table"))) return st;
     // lo and hi go back to the context's pool, which other streams may draw from
-    TSR_HIP(hipStreamSynchronize(s));
+    TSR_HIP(tsr::stream_sync(s));
     return Status::ok();
 }
 }  // namespace tsr
@@ -999,7 +999,7 @@ extern "C" tsr_status tsr_eq_table(tsr_ctx *ctx_, const uint64_t *point, uint32_
         e = hipMemcpyAsync(out, tmp.ptr, 16 * len, hipMemcpyDeviceToHost, s);
         if (e != hipSuccess) return hip_fail(e, "copy").code;
     }
-    e = hipStreamSynchronize(s);
+    e = tsr::stream_sync(s);
     if (e != hipSuccess) return hip_fail(e, "sync").code;
     return TSR_OK;
 }

@@ -179,7 +179,7 @@ static Status mask_draw(tsr_mask_stream *m, uint64_t count, uint64_t *out) {
         uint64_t total = 0, end = 0;
         TSR_HIP(hipMemcpyAsync(&total, offsets.ptr + G, 8, hipMemcpyDeviceToHost, s));
         TSR_HIP(hipMemcpyAsync(&end, endpos.ptr, 8, hipMemcpyDeviceToHost, s));
-        TSR_HIP(hipStreamSynchronize(s));
+        TSR_HIP(tsr::stream_sync(s));
         if (total >= count) {
             m->pos = end;
             return Status::ok();
@@ -223,7 +223,7 @@ tsr_status tsr_mask_stream_draw(tsr_mask_stream *m, uint64_t count, uint64_t *ou
         if (c != hipSuccess) st = hip_fail(c, "copy");
     }
     if (tmp.ptr) zero_words(tmp.ptr, count, ctx->stream);
-    e = hipStreamSynchronize(ctx->stream);
+    e = tsr::stream_sync(ctx->stream);
     if (!st && e != hipSuccess) st = hip_fail(e, "sync");
     return st.code;
 }

@@ -84,7 +84,7 @@ extern "C" tsr_status tsr_merkle_root_noalloc(tsr_ctx *ctx_, const uint8_t *leav
     if ((st = merkle_build(ctx, nodes, n, ctx->stream, false))) return st.code;
     e = hipMemcpyAsync(root, nodes + 8 * (2 * n - 2), 32, hipMemcpyDefault, ctx->stream);
     if (e != hipSuccess) return hip_fail(e, "copy root").code;
-    e = hipStreamSynchronize(ctx->stream);
+    e = tsr::stream_sync(ctx->stream);
     if (e != hipSuccess) return hip_fail(e, "sync").code;
     return TSR_OK;
 }

@@ -1027,7 +1027,7 @@ struct NttPlanDev {
         hipLaunchKernelGGL(k_scale_powers, dim3(blocks), dim3(256), 0, s, v, total, L, dlo.ptr, dhi.ptr, a);
         if ((st = check_launch("scale powers"))) return st;
         // the tables go back to the context's pool, which other streams may draw from
-        TSR_HIP(hipStreamSynchronize(s));
+        TSR_HIP(tsr::stream_sync(s));
         return Status::ok();
     }
 };
@@ -1091,7 +1091,7 @@ tsr_status run_ntt(tsr_ntt_plan *plan, uint64_t *values, uint64_t len, NttKind k
         e = hipMemcpyAsync(values, staged.ptr, 8 * len, hipMemcpyDeviceToHost, s);
         if (e != hipSuccess) return hip_fail(e, "copy").code;
     }
-    e = hipStreamSynchronize(s);
+    e = tsr::stream_sync(s);
     if (e != hipSuccess) return hip_fail(e, "sync").code;
     return TSR_OK;
 }
@@ -1133,7 +1133,7 @@ tsr_status tsr_coset_intt(tsr_ntt_plan *plan, uint64_t *values, uint64_t len, ui
 void tsr_ntt_plan_free(tsr_ntt_plan *plan) {
     if (!plan) return;
     (void)hipSetDevice(plan->p.ctx->device);
-    (void)hipStreamSynchronize(plan->p.ctx->stream);
+    (void)tsr::stream_sync(plan->p.ctx->stream);
     delete plan;
 }
 

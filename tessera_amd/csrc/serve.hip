@@ -240,7 +240,7 @@ static Status commit_group(Ctx *ctx, const std::vector<const ReqBufs *> &reqs, s
     if ((st = check_launch("serve group commit"))) return st;
     // the readback reuses the pinned block: the stream has finished the upload before it
     TSR_HIP(hipMemcpyAsync(host, blob.ptr + o_roots, 32 * count, hipMemcpyDeviceToHost, s));
-    TSR_HIP(hipStreamSynchronize(s));
+    TSR_HIP(tsr::stream_sync(s));
     // each request's codewords and tree become its commitment, as tsr_commit_unmasked's
     out.clear();
     for (uint32_t i = 0; i < count; i++) {
@@ -307,7 +307,7 @@ static Status prove_committed(Ctx *ctx, const tsr_request &req, const ReqBufs &b
     out.n_tokens = total;
     std::vector<uint32_t> toks(total);
     TSR_HIP(hipMemcpyAsync(toks.data(), b.tokens.ptr, 4ull * total, hipMemcpyDeviceToHost, s));
-    TSR_HIP(hipStreamSynchronize(s));
+    TSR_HIP(tsr::stream_sync(s));
     for (uint32_t i = 0; i < total && i < TSR_MAX_TOKENS; i++) out.tokens[i] = toks[i];
     out.final_a[0] = so.finals.size() > 0 ? so.finals[0].c0 : 0;
     out.final_a[1] = so.finals.size() > 0 ? so.finals[0].c1 : 0;
@@ -457,7 +457,7 @@ static void decoder_loop(tsr_server *srv) {
                                       hipMemcpyHostToDevice, ctx->stream);
         if (e != hipSuccess) ls = hip_fail(e, "decode tasks");
         if (!ls) ls = model_decode(ctx, srv->weights, dev_tasks.ptr, (uint32_t)host_tasks.size(), ctx->stream);
-        if (!ls && (e = hipStreamSynchronize(ctx->stream)) != hipSuccess) ls = hip_fail(e, "decode");
+        if (!ls && (e = tsr::stream_sync(ctx->stream)) != hipSuccess) ls = hip_fail(e, "decode");
         std::vector<std::unique_ptr<Job>> still;
         for (size_t i = 0; i < active.size(); i++) {
             std::unique_ptr<Job> &job = active[i];
@@ -521,7 +521,7 @@ static void prover_loop(tsr_server *srv, uint32_t slot) {
             std::vector<const ReqBufs *> reqs;
             for (auto &job : jobs) reqs.push_back(&job->bufs);
             grouped = !commit_group(ctx, reqs, coms);
-            (void)hipStreamSynchronize(ctx->stream);  // (after an error) nothing still runs
+            (void)tsr::stream_sync(ctx->stream);  // (after an error) nothing still runs
             if (!grouped) coms.clear();
         }
         coms.resize(g);
@@ -535,7 +535,7 @@ static void prover_loop(tsr_server *srv, uint32_t slot) {
             } else {
                 job.status = prove_decoded(ctx, job.req, job.bufs, res[i], coms[i]);
             }
-            (void)hipStreamSynchronize(ctx->stream);  // (after an error) nothing still reads the buffers
+            (void)tsr::stream_sync(ctx->stream);  // (after an error) nothing still reads the buffers
             finish(srv, job, &res[i], srv->keep ? std::move(coms[i]) : Commitment());
             coms[i].reset();
         }

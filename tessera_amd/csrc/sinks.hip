@@ -105,7 +105,7 @@ static Status sink_finish(tsr_block_sink *s, uint8_t *digests) {
     hipError_t e = hipSetDevice(s->ctx->device);
     if (e != hipSuccess) return hip_fail(e, "hipSetDevice");
     TSR_HIP(hipMemcpyAsync(digests, s->chains.ptr, 32 * s->n, hipMemcpyDefault, s->ctx->stream));
-    TSR_HIP(hipStreamSynchronize(s->ctx->stream));
+    TSR_HIP(tsr::stream_sync(s->ctx->stream));
     s->finished = true;
     return Status::ok();
 }
@@ -116,7 +116,7 @@ static void sink_free(tsr_block_sink *s) {
     zero(s->rows, s->ctx->stream);
     zero(s->masks, s->ctx->stream);
     zero(s->codewords, s->ctx->stream);
-    (void)hipStreamSynchronize(s->ctx->stream);
+    (void)tsr::stream_sync(s->ctx->stream);
     delete s;
 }
 
@@ -138,7 +138,7 @@ tsr_status tsr_block_sink_absorb(tsr_block_sink *s, const uint64_t *rows, uint64
     if (!s) return Status::fail(TSR_BAD_ARG, "null sink").code;
     Status st = sink_absorb(s, rows, masks, count);
     if (!st) {
-        hipError_t e = hipStreamSynchronize(s->ctx->stream);
+        hipError_t e = tsr::stream_sync(s->ctx->stream);
         if (e != hipSuccess) st = hip_fail(e, "block sink");
     }
     return st.code;
@@ -202,7 +202,7 @@ tsr_status tsr_matrix_sink_absorb(tsr_block_sink *s, const uint64_t *rows, uint6
         hipError_t e = hipMemcpyAsync(symbols_out, s->out.ptr, 8 * total, hipMemcpyDefault, stream);
         if (e != hipSuccess) return hip_fail(e, "symbols").code;
     }
-    hipError_t e = hipStreamSynchronize(stream);
+    hipError_t e = tsr::stream_sync(stream);
     if (e != hipSuccess) return hip_fail(e, "matrix sink").code;
     return TSR_OK;
 }
@@ -289,7 +289,7 @@ static Status column_sink_run(tsr_column_sink *s, const uint64_t *rows, const ui
                        s->codewords.ptr, s->n, count, s->columns.ptr, s->m, s->out.ptr);
     if ((st = check_launch("column sink"))) return st;
     TSR_HIP(hipMemcpyAsync(symbols, s->out.ptr, 8 * total, hipMemcpyDefault, st_));
-    TSR_HIP(hipStreamSynchronize(st_));
+    TSR_HIP(tsr::stream_sync(st_));
     return Status::ok();
 }
 
@@ -333,7 +333,7 @@ tsr_status tsr_column_sink_create(tsr_ctx *ctx_, uint32_t log_k, uint32_t log2_i
         return st.code;
     e = hipMemcpyAsync(s->columns.ptr, columns, 8 * m, hipMemcpyDefault, ctx->stream);
     if (e != hipSuccess) return hip_fail(e, "columns").code;
-    e = hipStreamSynchronize(ctx->stream);
+    e = tsr::stream_sync(ctx->stream);
     if (e != hipSuccess) return hip_fail(e, "columns").code;
     *out = s.release();
     return TSR_OK;
@@ -365,10 +365,10 @@ tsr_status tsr_column_sink_combined(tsr_column_sink *s, uint64_t *out, uint32_t 
     const uint64_t words = 2ull * groups * s->k;
     std::vector<uint64_t> h(words);
     e = hipMemcpyAsync(h.data(), s->sums.ptr, 8 * words, hipMemcpyDeviceToHost, s->ctx->stream);
-    if (e == hipSuccess) e = hipStreamSynchronize(s->ctx->stream);
+    if (e == hipSuccess) e = tsr::stream_sync(s->ctx->stream);
     // the sums are secret-class: cleared on the device whatever happened, then on the host
     zero_words((uint64_t *)s->sums.ptr, words, s->ctx->stream);
-    (void)hipStreamSynchronize(s->ctx->stream);
+    (void)tsr::stream_sync(s->ctx->stream);
     s->groups = 0;
     if (e != hipSuccess) {
         memset(h.data(), 0, 8 * words);
@@ -389,7 +389,7 @@ void tsr_column_sink_free(tsr_column_sink *s) {
     if (!s) return;
     (void)hipSetDevice(s->ctx->device);
     column_sink_clear(s);
-    (void)hipStreamSynchronize(s->ctx->stream);
+    (void)tsr::stream_sync(s->ctx->stream);
     delete s;
 }
 

@@ -1376,7 +1376,7 @@ Status run_batch(Ctx *ctx, Transcript &tr, const std::string &p, uint32_t D,
                 v = fp2{0, 0};
                 TSR_HIP(hipMemcpyAsync(&v, (const uint8_t *)tab.src + (tab.base ? 8 : 16) * tab.off,
                                        tab.base ? 8 : 16, hipMemcpyDeviceToHost, s));
-                TSR_HIP(hipStreamSynchronize(s));
+                TSR_HIP(tsr::stream_sync(s));
             }
             single.push_back(v);
         }
@@ -1669,7 +1669,7 @@ Status run_batch(Ctx *ctx, Transcript &tr, const std::string &p, uint32_t D,
     // the readback reuses the pinned block: the stream has finished the upload before it
     uint8_t *const down = up;
     TSR_HIP(hipMemcpyAsync(down, dev + o_ctl, at - o_ctl, hipMemcpyDeviceToHost, s));
-    TSR_HIP(hipStreamSynchronize(s));
+    TSR_HIP(tsr::stream_sync(s));
     memcpy(&hctl, &down[0], sizeof(ScCtl));
     out.rounds.resize((size_t)n_max * D);
     out.challenges.resize(n_max);
@@ -1690,7 +1690,7 @@ static Status eq_point_of(Ctx *ctx, const tsr_table &tt, uint32_t vars, std::vec
     pt.assign(vars, fp2{0, 0});
     if (vars == 0) return Status::ok();
     TSR_HIP(hipMemcpyAsync(pt.data(), tt.data, 16ull * vars, hipMemcpyDefault, ctx->stream));
-    TSR_HIP(hipStreamSynchronize(ctx->stream));
+    TSR_HIP(tsr::stream_sync(ctx->stream));
     for (auto &c : pt)
         if (c.c0 >= gl::P || c.c1 >= gl::P)
             return Status::fail(TSR_BAD_ARG, "an eq point coordinate is not canonical");

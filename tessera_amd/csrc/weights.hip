@@ -225,7 +225,7 @@ Status weight_commit(Ctx *ctx, const tsr_weight_layout_params &params, uint32_t 
                                shapes[t].out, shapes[t].inputs, log2_pad(shapes[t].inputs), ms[t].a,
                                m / ms[t].layers, ms[t].log2_k, total, rows.ptr);
             if ((st = check_launch("weight rows"))) return st;
-            TSR_HIP(hipStreamSynchronize(s));  // w8 is freed at the end of this scope
+            TSR_HIP(tsr::stream_sync(s));  // w8 is freed at the end of this scope
         } else if (m) {
             std::vector<uint64_t> host(m * k, 0);
             for (auto &x : chs) {
@@ -239,7 +239,7 @@ Status weight_commit(Ctx *ctx, const tsr_weight_layout_params &params, uint32_t 
                 }
             }
             TSR_HIP(hipMemcpyAsync(rows.ptr, host.data(), 8 * m * k, hipMemcpyHostToDevice, s));
-            TSR_HIP(hipStreamSynchronize(s));
+            TSR_HIP(tsr::stream_sync(s));
         }
         tsr_commitment *c = nullptr;
         tsr_status rc = tsr_commit_unmasked((tsr_ctx *)ctx, &cp, m ? rows.ptr : nullptr, m, &c);

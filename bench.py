@@ -782,8 +782,8 @@ def main():
                          "H100 unit-cost workloads; or one of the other sections alone")
     a = ap.parse_args()
     if a.gpus != 1:
-        print("bench.py measures one GPU: this build runs on a single MI355X; "
-              "multi-GPU sharding is described in DESIGN.md, not run", file=sys.stderr)
+        print("bench.py measures one GPU (a single MI355X); running the path sharded over "
+              "several GPUs is not implemented", file=sys.stderr)
         sys.exit(2)
 
     if a.trace_summary:

@@ -792,6 +792,14 @@ struct ScTailStep {
 
 static_assert(sizeof(ScTailStep) % 16 == 0, "tail steps are staged in LDS as 16-byte words");
 
+// The final evaluations: out[e] = entry off[e] of the buffer at ptrs[e] (base or Fp2), e < n.
+struct ScFinals {
+    const void *const *ptrs;
+    const uint32_t *base, *off;
+    fp2 *out;
+    uint32_t n;
+};
+
 // A level round's pairs in the tail, eight lanes per pair: lane 2k + h of the pair binds (or
 // loads) half h of table K0 + k and leaves it in wbuf; lanes z = 0, 1, 2 then take the line
 // values at z, E and the relation, so one pair's products are spread over four lanes and no
@@ -942,7 +950,7 @@ k_sc_tail(uint32_t j0, uint32_t n_max, const ScRoundInst *__restrict__ rtab,
           const fp2 *__restrict__ partials, const ScTailStep *__restrict__ steps_g,
           const ScTerms *__restrict__ terms, fp2 lambda, ScCtl *__restrict__ ctl, DevKeys keys,
           const uint8_t *__restrict__ labels, const uint32_t *__restrict__ label_lens,
-          fp2 *__restrict__ rounds_out, fp2 *__restrict__ chals_out) {
+          fp2 *__restrict__ rounds_out, fp2 *__restrict__ chals_out, ScFinals fin) {
     extern __shared__ uint4 dyn[];  // the steps, then count x (D+1) sums
     constexpr int WAVES = SC_TAIL_THREADS / 64;
     __shared__ ScRoundInst sri[SC_TAIL_MAX];
@@ -1087,6 +1095,10 @@ k_sc_tail(uint32_t j0, uint32_t n_max, const ScRoundInst *__restrict__ rtab,
             ctl->r = r;
         }
     }
+    // the final evaluations, from the last binds above (as k_sc_gather would, one launch less)
+    __syncthreads();
+    for (uint32_t e = threadIdx.x; e < fin.n; e += SC_TAIL_THREADS)
+        fin.out[e] = ld_entry(fin.ptrs[e], fin.base[e] != 0, fin.off[e]);
 }
 
 // Gathers each final evaluation: entry off[e] of the buffer at ptrs[e] (base or Fp2).
@@ -1225,7 +1237,8 @@ static void launch_tail_d(uint32_t d, hipStream_t s, uint32_t j0, uint32_t n_max
                           const ScRoundInst *rtab, ScInstState *ist, uint32_t count,
                           uint32_t nblk_max0, const fp2 *partials, const ScTailStep *steps,
                           const ScTerms *terms, fp2 lambda, ScCtl *ctl, const DevKeys &keys,
-                          const uint8_t *labels, const uint32_t *lens, fp2 *rounds, fp2 *chals) {
+                          const uint8_t *labels, const uint32_t *lens, fp2 *rounds, fp2 *chals,
+                          const ScFinals &fin) {
     size_t shm = sizeof(ScTailStep) * (size_t)(n_max - j0) * count + sizeof(fp2) * count * (d + 1);
     switch (d) {
 #define CASE(D)                                                                               \
@@ -1233,7 +1246,7 @@ static void launch_tail_d(uint32_t d, hipStream_t s, uint32_t j0, uint32_t n_max
         hipLaunchKernelGGL(k_sc_tail<D>, dim3(1), dim3(SC_TAIL_THREADS), shm, s, j0, n_max,    \
                            rtab, ist, count, nblk_max0, partials, steps, terms, lambda, ctl,   \
                            keys,                                                               \
-                           labels, lens, rounds, chals);                                       \
+                           labels, lens, rounds, chals, fin);                                  \
         break;
         CASE(1) CASE(2) CASE(3) CASE(4) CASE(5) CASE(6) CASE(7) CASE(8) CASE(9) CASE(10)
 #undef CASE
@@ -1652,15 +1665,17 @@ Status run_batch(Ctx *ctx, Transcript &tr, const std::string &p, uint32_t D,
                            &d_ctl->r, lambda, partials.ptr + pstride * i, sterms[i], eq);
         }
     }
+    const ScFinals fin{(const void *const *)(dev + o_fptr), (const uint32_t *)(dev + o_fbase),
+                       (const uint32_t *)(dev + o_foff), (fp2 *)(dev + o_fin), (uint32_t)nf};
     if (tail) {
         KernelTimer timer(ctx, "k_sc_tail", s);
         launch_tail_d(D, s, j0, n_max, d_rtab, d_state, count, nblk_max[j0], partials.ptr,
                       (const ScTailStep *)(dev + o_steps), (const ScTerms *)(dev + o_terms), lambda,
                       d_ctl, keys, dev + o_labels, (const uint32_t *)(dev + o_lens),
-                      (fp2 *)(dev + o_rounds), (fp2 *)(dev + o_chals));
+                      (fp2 *)(dev + o_rounds), (fp2 *)(dev + o_chals), fin);
     }
     if ((st = check_launch("sumcheck rounds"))) return st;
-    if (nf) {
+    if (nf && !tail) {
         hipLaunchKernelGGL(k_sc_gather, dim3((unsigned)((nf + 63) / 64)), dim3(64), 0, s,
                            (const void *const *)(dev + o_fptr), (const uint32_t *)(dev + o_fbase),
                            (const uint32_t *)(dev + o_foff), (uint32_t)nf, (fp2 *)(dev + o_fin));

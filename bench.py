@@ -489,14 +489,19 @@ def trace_summary(db, title, command):
     extra = {r[0]: r[1:] for r in c.execute(
         "select name, min(duration), max(duration), max(vgpr_count), max(sgpr_count), max(lds_size), "
         "max(grid_x*grid_y*grid_z), max(workgroup_x) from kernels group by name")}
+    durs = {}
+    for name, d in c.execute("select name, duration from kernels"):
+        durs.setdefault(name, []).append(d)
     out = [f"# {title}\n", "Command (from the repository root on the GPU box):\n",
            "```\n" + command + "\n```\n",
-           "| kernel | calls | total (us) | avg (us) | min (us) | max (us) | % | VGPR | SGPR | LDS (B) | grid | WG |",
-           "|---|---|---|---|---|---|---|---|---|---|---|---|"]
+           "The median leaves out the slower first launches of a run (cold caches and clocks).\n",
+           "| kernel | calls | total (us) | avg (us) | median (us) | min (us) | max (us) | % | VGPR | SGPR | LDS (B) | grid | WG |",
+           "|---|---|---|---|---|---|---|---|---|---|---|---|---|"]
     for name, calls, tot, avg, pct in rows:
         short = re.sub(r"\(.*", "", name)
         mn, mx, vg, sg, lds, grid, wg = extra.get(name, (0,) * 7)
-        out.append(f"| `{short}` | {calls} | {tot:.1f} | {avg:.1f} | {mn / 1000:.1f} | {mx / 1000:.1f} | "
+        med = float(np.median(durs.get(name, [0]))) / 1000
+        out.append(f"| `{short}` | {calls} | {tot:.1f} | {avg:.1f} | {med:.1f} | {mn / 1000:.1f} | {mx / 1000:.1f} | "
                    f"{pct:.1f} | {vg} | {sg} | {lds} | {grid} | {wg} |")
     return "\n".join(out) + "\n"
 

@@ -1704,8 +1704,12 @@ Status run_batch(Ctx *ctx, Transcript &tr, const std::string &p, uint32_t D,
 static Status eq_point_of(Ctx *ctx, const tsr_table &tt, uint32_t vars, std::vector<fp2> &pt) {
     pt.assign(vars, fp2{0, 0});
     if (vars == 0) return Status::ok();
-    TSR_HIP(hipMemcpyAsync(pt.data(), tt.data, 16ull * vars, hipMemcpyDefault, ctx->stream));
-    TSR_HIP(tsr::stream_sync(ctx->stream));
+    if (!is_device_ptr(tt.data)) {
+        memcpy(pt.data(), tt.data, 16ull * vars);  // host memory: no trip through the stream
+    } else {
+        TSR_HIP(hipMemcpyAsync(pt.data(), tt.data, 16ull * vars, hipMemcpyDeviceToHost, ctx->stream));
+        TSR_HIP(tsr::stream_sync(ctx->stream));
+    }
     for (auto &c : pt)
         if (c.c0 >= gl::P || c.c1 >= gl::P)
             return Status::fail(TSR_BAD_ARG, "an eq point coordinate is not canonical");

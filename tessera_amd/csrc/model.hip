@@ -248,6 +248,10 @@ k_decode(const int8_t *__restrict__ W, const DecodeTask *__restrict__ tasks, uin
                 const int i = tid & (D_MODEL - 1), q = tid >> 8, hd = i / D_HEAD;
                 const int32_t *vc = vpl + i;
                 int32_t num = 0, nsum = 0;
+                // four cache words in flight per thread: a request of 512 positions decodes in
+                // 36 ms instead of 41 (unrolled 8 or 16, or with the score loop unrolled too: no
+                // faster)
+#pragma unroll 4
                 for (uint32_t wd = q; wd < n4 / 4; wd += 4) {
                     const int32_t word = vc[(uint64_t)wd * D_MODEL];
 #pragma unroll

@@ -575,6 +575,8 @@ struct ScRoundInst {
 };
 
 // The per-instance state the rounds update.
+static_assert(sizeof(ScRoundInst) % 16 == 0, "round tables are staged in LDS as 16-byte words");
+
 struct ScInstState {
     fp2 wsum;  // gamma^i S_i
     fp2 eq_k;  // eqf: gamma^i times the product over the bound own rounds t of eq(point_t, c_t)
@@ -963,11 +965,21 @@ k_sc_tail(uint32_t j0, uint32_t n_max, const ScRoundInst *__restrict__ rtab,
     __shared__ WaveKeys wk;
     __shared__ fp2 red[WAVES][D + 1];
     __shared__ fp2 wbuf[SC_TAIL_THREADS];  // eight lanes per pair: the pair's bound values
-    const uint32_t n_steps = (n_max - j0) * count;
+    const uint32_t n_steps = (n_max - j0) * count, nr_t = n_max - j0;
     ScTailStep *steps = (ScTailStep *)dyn;
     fp2 *sums = (fp2 *)(steps + n_steps);
+    // the rounds' instance tables, labels and label lengths, staged with the steps, so a round
+    // starts without a load from global memory
+    ScRoundInst *rst = (ScRoundInst *)(sums + count * (D + 1));
+    uint32_t *lst = (uint32_t *)(rst + (size_t)nr_t * count);
+    uint32_t *nst = lst + (size_t)nr_t * (2 * LABEL_SLOT / 4);
     for (uint32_t w = threadIdx.x; w < n_steps * (sizeof(ScTailStep) / 16); w += SC_TAIL_THREADS)
         dyn[w] = ((const uint4 *)steps_g)[w];
+    for (uint32_t w = threadIdx.x; w < nr_t * count * (sizeof(ScRoundInst) / 16); w += SC_TAIL_THREADS)
+        ((uint4 *)rst)[w] = ((const uint4 *)(rtab + (size_t)j0 * count))[w];
+    for (uint32_t w = threadIdx.x; w < nr_t * (2 * LABEL_SLOT / 4); w += SC_TAIL_THREADS)
+        lst[w] = ((const uint32_t *)(labels + (size_t)2 * j0 * LABEL_SLOT))[w];
+    for (uint32_t w = threadIdx.x; w < 2 * nr_t; w += SC_TAIL_THREADS) nst[w] = label_lens[2 * j0 + w];
     const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const uint32_t t_word = ((const uint32_t *)&ctl->t)[lane % 10];
     if (wave == 0 && lane < 8) st[lane] = t_word;
@@ -981,14 +993,13 @@ k_sc_tail(uint32_t j0, uint32_t n_max, const ScRoundInst *__restrict__ rtab,
     const uint32_t G = SC_TAIL_THREADS / groups, gi = threadIdx.x / G, gt = threadIdx.x % G;
     const uint32_t gw = G / 64;  // waves per group
     fp2 r{0, 0};
+    __syncthreads();  // the staged steps and round tables
     for (uint32_t j = j0; j < n_max; j++) {
-        const ScRoundInst *ri = rtab + (size_t)j * count;
+        const ScRoundInst *ri = rst + (size_t)(j - j0) * count;
         if (threadIdx.x < count) sri[threadIdx.x] = ri[threadIdx.x];
-        if (threadIdx.x < 2 * LABEL_SLOT / 4)
-            lab[threadIdx.x] = ((const uint32_t *)(labels + (size_t)2 * j * LABEL_SLOT))[threadIdx.x];
-        const uint32_t len_word = label_lens[2 * j + (lane & 1)];
+        if (threadIdx.x < 2 * LABEL_SLOT / 4) lab[threadIdx.x] = lst[(j - j0) * (2 * LABEL_SLOT / 4) + threadIdx.x];
+        const uint32_t len_word = nst[2 * (j - j0) + (lane & 1)];
         if (j == j0) {
-            __syncthreads();  // the staged steps
             // the grid launches' partials, as k_sc_transcript reduces them
             for (uint32_t x = wave; x < count * (D + 1); x += WAVES) {
                 const uint32_t i = x / (D + 1), z = x % (D + 1);
@@ -1239,7 +1250,8 @@ static void launch_tail_d(uint32_t d, hipStream_t s, uint32_t j0, uint32_t n_max
                           const ScTerms *terms, fp2 lambda, ScCtl *ctl, const DevKeys &keys,
                           const uint8_t *labels, const uint32_t *lens, fp2 *rounds, fp2 *chals,
                           const ScFinals &fin) {
-    size_t shm = sizeof(ScTailStep) * (size_t)(n_max - j0) * count + sizeof(fp2) * count * (d + 1);
+    size_t shm = sizeof(ScTailStep) * (size_t)(n_max - j0) * count + sizeof(fp2) * count * (d + 1) +
+                 (size_t)(n_max - j0) * (sizeof(ScRoundInst) * count + 2 * LABEL_SLOT + 8);
     switch (d) {
 #define CASE(D)                                                                               \
     case D:                                                                                   \

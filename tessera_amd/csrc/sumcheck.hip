@@ -765,8 +765,9 @@ k_sc_transcript(uint32_t j, const ScRoundInst *__restrict__ ri, ScInstState *__r
 // The tail of a batch: once every remaining round has at most SC_TAIL_PAIRS pairs per instance,
 // one workgroup runs all of them (the message of the first from the partials of the grid
 // launches, then per round: the instances' bind + sums in the block, the shares, the hashes on
-// wave 0, the eq folds) and the final binds. Nothing leaves the workgroup between rounds: no
-// kernel boundary and no trip through another XCD's cache per round.
+// wave 0, the eq folds), the final binds and the gather of the final evaluations. Nothing leaves
+// the workgroup between rounds: no kernel boundary and no trip through another XCD's cache per
+// round.
 
 constexpr uint32_t SC_TAIL_LOG = 8;
 constexpr uint64_t SC_TAIL_PAIRS = 1ull << SC_TAIL_LOG;
@@ -944,7 +945,8 @@ __device__ __forceinline__ void tail_items(const ScTailStep &sp, const ScTerms &
 
 // One workgroup of SC_TAIL_THREADS: instance i's items run on the thread group
 // [i G, (i + 1) G) (G = SC_TAIL_THREADS / the power of two >= count), so the instances' chains of
-// products overlap; the steps are staged in LDS at the start.
+// products overlap; the steps, the rounds' instance tables and their labels are staged in LDS at
+// the start.
 template <int D>
 __global__ void __launch_bounds__(SC_TAIL_THREADS)
 k_sc_tail(uint32_t j0, uint32_t n_max, const ScRoundInst *__restrict__ rtab,

@@ -270,17 +270,74 @@ k_merkle_top(uint32_t *__restrict__ nodes, uint64_t start, uint64_t width, Key8 
     }
 }
 
+// Nine Merkle levels in one launch: workgroup b hashes the 512 nodes [512 b, 512 b + 512) of the
+// level of `width` nodes at node index `start` into their 256 parents, those into 128, ..., down
+// to their one common ancestor, each level in LDS and written out where k_merkle_level would
+// write it. width is a multiple of 512.
+constexpr uint32_t MERKLE_SUB_LEVELS = 9, MERKLE_SUB_IN = 1u << MERKLE_SUB_LEVELS;
+__global__ void __launch_bounds__(MERKLE_SUB_IN / 2)
+k_merkle_subtree(uint32_t *__restrict__ nodes, uint64_t start, uint64_t width, Key8 key) {
+    __shared__ uint32_t d[MERKLE_SUB_IN / 2][8];
+    const uint32_t t = threadIdx.x;
+    const uint64_t b = blockIdx.x;
+    uint32_t out[8];
+    {
+        const uint4 *l = (const uint4 *)(nodes + 8 * (start + b * MERKLE_SUB_IN + 2 * t));
+        const uint4 a = l[0], c = l[1], e = l[2], f = l[3];
+        const uint32_t L[8] = {a.x, a.y, a.z, a.w, c.x, c.y, c.z, c.w};
+        const uint32_t R[8] = {e.x, e.y, e.z, e.w, f.x, f.y, f.z, f.w};
+        b3::hash64(key.w, L, R, out);
+    }
+    uint32_t cnt = MERKLE_SUB_IN / 2;  // this workgroup's nodes on the level just hashed
+    for (uint32_t lv = 0;; lv++) {
+        start += width;
+        width >>= 1;
+        if (t < cnt) {
+#pragma unroll
+            for (int i = 0; i < 8; i++) d[t][i] = out[i];
+            uint4 *o = (uint4 *)(nodes + 8 * (start + b * cnt + t));
+            o[0] = make_uint4(out[0], out[1], out[2], out[3]);
+            o[1] = make_uint4(out[4], out[5], out[6], out[7]);
+        }
+        if (lv + 1 == MERKLE_SUB_LEVELS) break;
+        __syncthreads();
+        cnt >>= 1;
+        if (t < cnt) {
+            uint32_t L[8], R[8];
+#pragma unroll
+            for (int i = 0; i < 8; i++) {
+                L[i] = d[2 * t][i];
+                R[i] = d[2 * t + 1][i];
+            }
+            b3::hash64(key.w, L, R, out);
+        }
+        __syncthreads();  // the level below is read before it is overwritten
+    }
+}
+
 Status merkle_build(Ctx *ctx, uint32_t *nodes, uint64_t n, hipStream_t stream, bool timed) {
     Key8 key = key_of(ctx, DOM_MERKLE_NODE);
     uint64_t start = 0, width = n;
     KernelTimer timer(timed ? ctx : nullptr, "k_merkle_level", stream);
-    // one launch per level while a level has more than 2 MERKLE_TOP nodes, then the top at once
+    // Levels of more than 2^20 nodes one launch each (their hashes keep the GPU busy), smaller ones
+    // nine levels per launch while a level has more than 2 MERKLE_TOP nodes (widths are powers of
+    // two), then the top at once. (Nine levels per launch from 2^22 leaves up: 0.19 ms against
+    // 0.15 ms for the single levels; the 2^16 leaves of a 256 x 2^14 commit: 28 us less.)
     while (width > 2 * MERKLE_TOP) {
-        uint64_t count = width / 2;
-        hipLaunchKernelGGL(k_merkle_level, dim3(blocks_for(count, 256)), dim3(256), 0, stream,
-                           nodes + 8 * start, nodes + 8 * (start + width), count, key);
-        start += width;
-        width /= 2;
+        if (width > (1ull << 20)) {
+            const uint64_t count = width / 2;
+            hipLaunchKernelGGL(k_merkle_level, dim3(blocks_for(count, 256)), dim3(256), 0, stream,
+                               nodes + 8 * start, nodes + 8 * (start + width), count, key);
+            start += width;
+            width /= 2;
+            continue;
+        }
+        hipLaunchKernelGGL(k_merkle_subtree, dim3((unsigned)(width / MERKLE_SUB_IN)), dim3(MERKLE_SUB_IN / 2),
+                           0, stream, nodes, start, width, key);
+        for (uint32_t lv = 0; lv < MERKLE_SUB_LEVELS; lv++) {
+            start += width;
+            width /= 2;
+        }
     }
     if (width > 1)
         hipLaunchKernelGGL(k_merkle_top, dim3(1), dim3(MERKLE_TOP), 0, stream, nodes, start, width, key);

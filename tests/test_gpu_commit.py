@@ -109,6 +109,17 @@ def test_merkle_golden(dev):
         assert T.merkle_root(dev, leaves).hex() == case["root"]
 
 
+@pytest.mark.parametrize("log_n", [11, 12, 13, 17, 20, 21])
+def test_merkle_root_sizes(dev, log_n):
+    """Trees around the build's launch boundaries (one launch per level above 2^20 nodes, nine
+    levels per launch down to 2048 nodes, the rest in one workgroup) against the oracle."""
+    import tessera_amd as T
+    n = 1 << log_n
+    raw = splitmix_uniform(4 * n, 1000 + log_n).view(np.uint8).tobytes()
+    leaves = [raw[32 * i:32 * i + 32] for i in range(n)]
+    assert T.merkle_root(dev, leaves) == O.merkle_tree(leaves, threads=8)[-32:]
+
+
 def commit_and_check_small(dev, log_k, log_r, block, rows, seed):
     import tessera_amd as T
     k, n = 1 << log_k, 1 << (log_k + log_r)

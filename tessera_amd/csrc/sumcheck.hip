@@ -818,6 +818,9 @@ __device__ __forceinline__ void tail_eq_wide(const ScTailStep &sp, const fp2 &r,
     const uint32_t role = gt & 7, slots = G / 8;
     fp2 *my = wbuf + (threadIdx.x & ~7u);
     for (uint64_t q = gt >> 3; q < sp.pairs; q += slots) {
+        // the pair's eq weight, loaded with the tables' entries (the wave barrier below would
+        // hold a load placed after it back by a whole memory latency)
+        const fp2 e = role < 3 ? sp.eq_lo[q] : fp2{0, 0};  // eq_hi is [1] in the tail's rounds
         if (role < NV) {
             const uint32_t k = K0 + (role >> 1), h = role & 1;
             const bool base = first && ((sp.base_mask >> k) & 1);
@@ -842,7 +845,6 @@ __device__ __forceinline__ void tail_eq_wide(const ScTailStep &sp, const fp2 &r,
                 const fp2 a = my[2 * (k - K0)], b = my[2 * (k - K0) + 1];
                 x[k] = role == 0 ? a : role == 1 ? b : gl::sub(gl::add(b, b), a);
             }
-            const fp2 e = sp.eq_lo[q];  // eq_hi is [1] in the tail's rounds
             fp2 f;
             if (REL == REL_PRODUCT)
                 f = gl::mul(gl::mul(e, x[0]), x[1]);
